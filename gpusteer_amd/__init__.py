@@ -1,0 +1,32 @@
+"""gpusteer_amd — MI355X (gfx950) implementation of GPUSteer's per-step
+simulation path behind a C-ABI (include/gpusteer.h).
+
+Importing the package loads lib/libgpusteer.so; there is no CPU fallback.
+"""
+from .build import LIB  # noqa: F401
+from . import _capi  # noqa: F401  (raises ImportError when the library is not built)
+from .api import (  # noqa: F401
+    MODE_GPUREF,
+    MODE_OPENSTEER,
+    AgentGroup,
+    Context,
+    KNNBinData,
+    KNNData,
+    ObstacleGroup,
+    WallGroup,
+    clearAppliedKernels,
+    findKNearestNeighbors,
+    findKNearestNeighborsBruteForce,
+    steerForAlignment,
+    steerForCohesion,
+    steerForFlocking,
+    steerForSeek,
+    steerForSeparation,
+    steerToAvoidObstacles,
+    steerToAvoidWalls,
+    stepBoids,
+    updateGroup,
+    updateKNNDatabase,
+    wrapWorld,
+)
+from .state import AgentState, WallState  # noqa: F401

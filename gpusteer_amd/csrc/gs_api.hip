@@ -1,0 +1,704 @@
+// C-ABI entry points of libgpusteer (declared in include/gpusteer.h).
+//
+// Each entry point validates its handles and sizes, converts internal errors
+// to status codes (never aborting, unlike the reference's CUDA_SAFE_CALL /
+// cutilCheckMsg, include/OpenSteer/CUDA/CUDAGlobals.cuh:11) and queues device
+// work on the context's stream.
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "gs_internal.hpp"
+
+namespace gs {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+void fail(int status, const std::string& msg) { throw Error(status, msg); }
+void hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory) fail(GS_ERR_OUT_OF_MEMORY, std::string(what) + ": out of device memory");
+    fail(GS_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Allocation-failure injection for tests: after `g_fail_after` more
+// successful allocations, every allocation fails (negative = never).
+static std::atomic<int64_t> g_fail_after{-1};
+void* dev_alloc(size_t bytes) {
+    int64_t left = g_fail_after.load();
+    if (left >= 0) {
+        if (left == 0) fail(GS_ERR_OUT_OF_MEMORY, "device allocation failed (injected)");
+        g_fail_after.store(left - 1);
+    }
+    void* p = nullptr;
+    if (bytes == 0) bytes = 4;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(GS_ERR_OUT_OF_MEMORY, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+    }
+    return p;
+}
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+// Live-handle registry: a destroyed or foreign pointer is rejected instead of
+// being dereferenced.
+static std::mutex g_reg_mu;
+static std::unordered_set<const void*>& registry() {
+    static std::unordered_set<const void*> r;
+    return r;
+}
+static void reg_add(const void* h) {
+    std::lock_guard<std::mutex> l(g_reg_mu);
+    registry().insert(h);
+}
+static void reg_del(const void* h) {
+    std::lock_guard<std::mutex> l(g_reg_mu);
+    registry().erase(h);
+}
+static bool reg_has(const void* h) {
+    if (!h) return false;
+    std::lock_guard<std::mutex> l(g_reg_mu);
+    return registry().count(h) != 0;
+}
+static std::atomic<uint64_t> g_epoch{1};
+static uint64_t next_epoch() { return g_epoch.fetch_add(1); }
+
+template <class H> static H* check(H* h, const char* what) {
+    if (!reg_has(h)) fail(GS_ERR_INVALID_HANDLE, std::string("invalid ") + what + " handle");
+    return h;
+}
+static gs_group_s* check_group(gs_group g, const char* what, int kind = -1) {
+    check(g, what);
+    if (kind >= 0 && g->kind != kind) fail(GS_ERR_INVALID_HANDLE, std::string(what) + ": wrong group kind");
+    return g;
+}
+static void use_device(gs_context ctx) { GS_HIP(hipSetDevice(ctx->device)); }
+
+template <class F> static int guarded(F&& f) {
+    try {
+        f();
+        return GS_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return GS_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return GS_ERR_DEVICE;
+    }
+}
+
+static GridDev make_grid(const float* o, const float* s, const uint32_t* c) {
+    for (int a = 0; a < 3; ++a) {
+        if (!(s[a] > 0.f) || !std::isfinite(s[a]) || !std::isfinite(o[a]))
+            fail(GS_ERR_INVALID_ARGUMENT, "bin lattice size must be finite and positive");
+        if (c[a] == 0) fail(GS_ERR_INVALID_ARGUMENT, "bin lattice needs at least one cell per axis");
+    }
+    const uint64_t C = (uint64_t)c[0] * c[1] * c[2];
+    if (C > (1ull << 24)) fail(GS_ERR_UNSUPPORTED, "bin lattice larger than 2^24 cells");
+    GridDev g;
+    g.ox = o[0];
+    g.oy = o[1];
+    g.oz = o[2];
+    g.sx = s[0];
+    g.sy = s[1];
+    g.sz = s[2];
+    g.dx = (int)c[0];
+    g.dy = (int)c[1];
+    g.dz = (int)c[2];
+    g.hx = g.sx / (float)g.dx;
+    g.hy = g.sy / (float)g.dy;
+    g.hz = g.sz / (float)g.dz;
+    g.slack = 1e-5f * (std::fabs(g.ox) + std::fabs(g.oy) + std::fabs(g.oz) + g.sx + g.sy + g.sz);
+    return g;
+}
+
+static bool same_grid(const GridDev& a, const GridDev& b) { return memcmp(&a, &b, sizeof(GridDev)) == 0; }
+
+static void require_knn_shape(const gs_knn_s* knn, const gs_group_s* A) {
+    if (knn->ctx != A->ctx) fail(GS_ERR_INVALID_ARGUMENT, "KNN container belongs to another context");
+    if (knn->n != A->n)
+        fail(GS_ERR_SIZE_MISMATCH, "KNN container holds " + std::to_string(knn->n) + " rows, group has " +
+                                       std::to_string(A->n));
+}
+
+static void h2d(void* d, const void* h, size_t bytes, hipStream_t st) {
+    if (bytes) GS_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
+}
+static void d2h(void* h, const void* d, size_t bytes, hipStream_t st) {
+    if (bytes) GS_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
+}
+
+static void mark_seed(gs_knn_s* out, const gs_group_s* A, const gs_group_s* B) {
+    out->seed_valid = true;
+    out->seed_a = A;
+    out->seed_b = B;
+    out->seed_epoch_a = A->membership_epoch;
+    out->seed_epoch_b = B->membership_epoch;
+}
+static bool seed_usable(const gs_knn_s* out, const gs_group_s* A, const gs_group_s* B, uint32_t flags) {
+    return !(flags & GS_KNN_NO_SEED) && out->seed_valid && out->seed_a == A && out->seed_b == B &&
+           out->seed_epoch_a == A->membership_epoch && out->seed_epoch_b == B->membership_epoch;
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+gs_agents gs_group_s::dev_agents() const {
+    gs_agents a;
+    float** fp = (float**)&a;
+    for (int j = 0; j < 23; ++j) fp[j] = f[j].p;
+    a.bits = bits.p;
+    return a;
+}
+
+extern "C" {
+
+const char* gs_status_string(int status) {
+    switch (status) {
+        case GS_OK: return "ok";
+        case GS_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case GS_ERR_INVALID_HANDLE: return "invalid handle";
+        case GS_ERR_SIZE_MISMATCH: return "size mismatch";
+        case GS_ERR_UNSUPPORTED: return "unsupported";
+        case GS_ERR_NOT_BINNED: return "group has no current bin database";
+        case GS_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case GS_ERR_DEVICE: return "device error";
+        case GS_ERR_NONFINITE: return "non-finite state";
+        default: return "unknown status";
+    }
+}
+
+const char* gs_last_error(void) { return g_last_error.c_str(); }
+int gs_version(void) { return GS_VERSION; }
+
+/* Test hook (not part of the public header's stable surface): make device
+ * allocations fail after n more successes; n < 0 disables. */
+int gs_debug_fail_alloc_after(int64_t n) {
+    g_fail_after.store(n);
+    return GS_OK;
+}
+
+// ---- context -----------------------------------------------------------------
+int gs_context_create(int device, void* hip_stream, gs_context* out) {
+    return guarded([&] {
+        if (!out) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
+        *out = nullptr;
+        int count = 0;
+        hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess || count == 0) {
+            (void)hipGetLastError();
+            fail(GS_ERR_DEVICE, "no HIP device available");
+        }
+        if (device < 0 || device >= count) fail(GS_ERR_INVALID_ARGUMENT, "device index out of range");
+        GS_HIP(hipSetDevice(device));
+        auto* c = new gs_context_s;
+        c->device = device;
+        if (hip_stream) {
+            c->stream = (hipStream_t)hip_stream;
+        } else {
+            e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+            if (e != hipSuccess) {
+                delete c;
+                hip_check(e, "hipStreamCreateWithFlags");
+            }
+            c->own_stream = true;
+        }
+        reg_add(c);
+        *out = c;
+    });
+}
+
+int gs_context_destroy(gs_context ctx) {
+    return guarded([&] {
+        check(ctx, "context");
+        use_device(ctx);
+        (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+        reg_del(ctx);
+        delete ctx;
+    });
+}
+
+int gs_context_synchronize(gs_context ctx) {
+    return guarded([&] {
+        check(ctx, "context");
+        use_device(ctx);
+        GS_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int gs_context_stream(gs_context ctx, void** s) {
+    return guarded([&] {
+        check(ctx, "context");
+        if (!s) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
+        *s = (void*)ctx->stream;
+    });
+}
+
+// ---- bins --------------------------------------------------------------------
+int gs_bins_create(gs_context ctx, const float origin[3], const float size[3], const uint32_t cells[3], gs_bins* out) {
+    return guarded([&] {
+        check(ctx, "context");
+        if (!origin || !size || !cells || !out) fail(GS_ERR_INVALID_ARGUMENT, "NULL argument");
+        auto* b = new gs_bins_s;
+        b->ctx = ctx;
+        try {
+            b->g = make_grid(origin, size, cells);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        b->cells = cells[0] * cells[1] * cells[2];
+        reg_add(b);
+        *out = b;
+    });
+}
+
+int gs_bins_create_centered(gs_context ctx, const uint32_t world_cells[3], const float world_size[3], gs_bins* out) {
+    if (!world_size) return GS_ERR_INVALID_ARGUMENT;
+    const float origin[3] = {-0.5f * world_size[0], -0.5f * world_size[1], -0.5f * world_size[2]};
+    return gs_bins_create(ctx, origin, world_size, world_cells, out);
+}
+
+int gs_bins_destroy(gs_bins b) {
+    return guarded([&] {
+        check(b, "bins");
+        reg_del(b);
+        delete b;
+    });
+}
+
+int gs_bins_geometry(gs_bins b, float origin[3], float size[3], uint32_t cells[3]) {
+    return guarded([&] {
+        check(b, "bins");
+        if (origin) {
+            origin[0] = b->g.ox;
+            origin[1] = b->g.oy;
+            origin[2] = b->g.oz;
+        }
+        if (size) {
+            size[0] = b->g.sx;
+            size[1] = b->g.sy;
+            size[2] = b->g.sz;
+        }
+        if (cells) {
+            cells[0] = (uint32_t)b->g.dx;
+            cells[1] = (uint32_t)b->g.dy;
+            cells[2] = (uint32_t)b->g.dz;
+        }
+    });
+}
+
+// ---- groups ------------------------------------------------------------------
+int gs_group_create(gs_context ctx, int kind, gs_group* out) {
+    return guarded([&] {
+        check(ctx, "context");
+        if (!out) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
+        if (kind != GS_KIND_AGENTS && kind != GS_KIND_OBSTACLES && kind != GS_KIND_WALLS)
+            fail(GS_ERR_INVALID_ARGUMENT, "unknown group kind");
+        auto* g = new gs_group_s;
+        g->ctx = ctx;
+        g->kind = kind;
+        g->membership_epoch = next_epoch();
+        reg_add(g);
+        *out = g;
+    });
+}
+
+int gs_group_destroy(gs_group g) {
+    return guarded([&] {
+        check_group(g, "group");
+        use_device(g->ctx);
+        (void)hipStreamSynchronize(g->ctx->stream);
+        for (auto& b : g->f) b.release();
+        for (auto& b : g->w) b.release();
+        g->bits.release();
+        g->key.release();
+        g->rank.release();
+        g->perm_tmp.release();
+        g->perm.release();
+        g->start.release();
+        g->count.release();
+        g->spx.release();
+        g->spy.release();
+        g->spz.release();
+        g->sfx.release();
+        g->sfy.release();
+        g->sfz.release();
+        g->inv.release();
+        g->flag.release();
+        reg_del(g);
+        delete g;
+    });
+}
+
+int gs_group_size(gs_group g, uint32_t* n) {
+    return guarded([&] {
+        check_group(g, "group");
+        if (!n) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
+        *n = g->n;
+    });
+}
+
+static void set_members(gs_group_s* g, uint32_t n) {
+    g->n = n;
+    g->binned = false;
+    g->membership_epoch = next_epoch();
+}
+
+int gs_group_upload_agents(gs_group g, uint32_t n, const gs_agents* host) {
+    return guarded([&] {
+        check_group(g, "group", GS_KIND_AGENTS);
+        if (n > 0 && !host) fail(GS_ERR_INVALID_ARGUMENT, "host arrays are NULL");
+        const float* const* hf = host ? (const float* const*)host : nullptr;
+        if (n > 0) {
+            for (int j = 0; j < 23; ++j)
+                if (!hf[j]) fail(GS_ERR_INVALID_ARGUMENT, "agent array " + std::to_string(j) + " is NULL");
+            if (!host->bits) fail(GS_ERR_INVALID_ARGUMENT, "bits array is NULL");
+        }
+        use_device(g->ctx);
+        hipStream_t st = g->ctx->stream;
+        for (int j = 0; j < 23; ++j) g->f[j].reserve(n);
+        g->bits.reserve(n);
+        for (int j = 0; j < 23 && n; ++j) h2d(g->f[j].p, hf[j], (size_t)n * 4, st);
+        if (n) h2d(g->bits.p, host->bits, (size_t)n * 4, st);
+        GS_HIP(hipStreamSynchronize(st));
+        set_members(g, n);
+    });
+}
+
+int gs_group_download_agents(gs_group g, gs_agents* host) {
+    return guarded([&] {
+        check_group(g, "group", GS_KIND_AGENTS);
+        if (!host) fail(GS_ERR_INVALID_ARGUMENT, "host arrays are NULL");
+        use_device(g->ctx);
+        hipStream_t st = g->ctx->stream;
+        float* const* hf = (float* const*)host;
+        for (int j = 0; j < 23 && g->n; ++j)
+            if (hf[j]) d2h(hf[j], g->f[j].p, (size_t)g->n * 4, st);
+        if (host->bits && g->n) d2h(host->bits, g->bits.p, (size_t)g->n * 4, st);
+        GS_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int gs_group_device_agents(gs_group g, gs_agents* dev) {
+    return guarded([&] {
+        check_group(g, "group", GS_KIND_AGENTS);
+        if (!dev) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
+        *dev = g->dev_agents();
+    });
+}
+
+int gs_group_upload_obstacles(gs_group g, uint32_t n, const float* cx, const float* cy, const float* cz,
+                              const float* radius) {
+    return guarded([&] {
+        check_group(g, "group", GS_KIND_OBSTACLES);
+        if (n > 0 && (!cx || !cy || !cz || !radius)) fail(GS_ERR_INVALID_ARGUMENT, "obstacle arrays are NULL");
+        use_device(g->ctx);
+        hipStream_t st = g->ctx->stream;
+        const float* src[4] = {cx, cy, cz, radius};
+        const int dst[4] = {0, 1, 2, 19};
+        for (int j = 0; j < 4; ++j) {
+            g->f[dst[j]].reserve(n);
+            h2d(g->f[dst[j]].p, src[j], (size_t)n * 4, st);
+        }
+        GS_HIP(hipStreamSynchronize(st));
+        set_members(g, n);
+    });
+}
+
+int gs_group_upload_walls(gs_group g, uint32_t n, const gs_walls* host) {
+    return guarded([&] {
+        check_group(g, "group", GS_KIND_WALLS);
+        if (n > 0 && !host) fail(GS_ERR_INVALID_ARGUMENT, "wall arrays are NULL");
+        const float* const* hw = host ? (const float* const*)host : nullptr;
+        for (int j = 0; j < 9 && n; ++j)
+            if (!hw[j]) fail(GS_ERR_INVALID_ARGUMENT, "wall array is NULL");
+        // The KNN position of a wall is its midpoint, mid = 0.5f*(start+end)
+        // (CUDA/WallGroupData.cu:87-89, WallGroup.h:31).
+        std::vector<float> mid[3];
+        for (int a = 0; a < 3; ++a) {
+            mid[a].resize(n);
+            for (uint32_t i = 0; i < n; ++i) mid[a][i] = 0.5f * (hw[a][i] + hw[3 + a][i]);
+        }
+        use_device(g->ctx);
+        hipStream_t st = g->ctx->stream;
+        for (int j = 0; j < 9; ++j) {
+            g->w[j].reserve(n);
+            if (n) h2d(g->w[j].p, hw[j], (size_t)n * 4, st);
+        }
+        for (int a = 0; a < 3; ++a) {
+            g->f[a].reserve(n);
+            if (n) h2d(g->f[a].p, mid[a].data(), (size_t)n * 4, st);
+        }
+        GS_HIP(hipStreamSynchronize(st));
+        set_members(g, n);
+    });
+}
+
+int gs_group_download_bins(gs_group g, uint32_t* key, uint32_t* start, uint32_t* perm, float* spx, float* spy,
+                           float* spz) {
+    return guarded([&] {
+        check_group(g, "group");
+        if (!g->binned) fail(GS_ERR_NOT_BINNED, "group has no bin database");
+        use_device(g->ctx);
+        hipStream_t st = g->ctx->stream;
+        const size_t n = g->n;
+        if (key) d2h(key, g->key.p, n * 4, st);
+        if (start) d2h(start, g->start.p, ((size_t)g->bin_cells + 1) * 4, st);
+        if (perm) d2h(perm, g->perm.p, n * 4, st);
+        if (spx) d2h(spx, g->spx.p, n * 4, st);
+        if (spy) d2h(spy, g->spy.p, n * 4, st);
+        if (spz) d2h(spz, g->spz.p, n * 4, st);
+        GS_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int gs_group_check_finite(gs_group g, uint32_t* first_bad) {
+    uint32_t bad = UINT32_MAX;
+    int s = guarded([&] {
+        check_group(g, "group", GS_KIND_AGENTS);
+        use_device(g->ctx);
+        launch_check_finite(g, &bad);
+    });
+    if (first_bad) *first_bad = bad;
+    if (s != GS_OK) return s;
+    if (bad != UINT32_MAX) {
+        set_last_error("non-finite state at agent " + std::to_string(bad));
+        return GS_ERR_NONFINITE;
+    }
+    return GS_OK;
+}
+
+// ---- KNN container --------------------------------------------------------------
+int gs_knn_create(gs_context ctx, uint32_t n, uint32_t k, gs_knn* out) {
+    return guarded([&] {
+        check(ctx, "context");
+        if (!out) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
+        if (k == 0) fail(GS_ERR_INVALID_ARGUMENT, "k must be at least 1");
+        if (k > GS_MAX_K) fail(GS_ERR_UNSUPPORTED, "k larger than GS_MAX_K");
+        use_device(ctx);
+        auto* c = new gs_knn_s;
+        c->ctx = ctx;
+        c->n = n;
+        c->k = k;
+        try {
+            c->idx.reserve((size_t)n * k);
+            c->dist.reserve((size_t)n * k);
+        } catch (...) {
+            c->idx.release();
+            c->dist.release();
+            delete c;
+            throw;
+        }
+        reg_add(c);
+        *out = c;
+    });
+}
+
+int gs_knn_destroy(gs_knn knn) {
+    return guarded([&] {
+        check(knn, "knn");
+        use_device(knn->ctx);
+        (void)hipStreamSynchronize(knn->ctx->stream);
+        knn->idx.release();
+        knn->dist.release();
+        reg_del(knn);
+        delete knn;
+    });
+}
+
+int gs_knn_shape(gs_knn knn, uint32_t* n, uint32_t* k) {
+    return guarded([&] {
+        check(knn, "knn");
+        if (n) *n = knn->n;
+        if (k) *k = knn->k;
+    });
+}
+
+int gs_knn_download(gs_knn knn, uint32_t* idx, float* dist) {
+    return guarded([&] {
+        check(knn, "knn");
+        use_device(knn->ctx);
+        hipStream_t st = knn->ctx->stream;
+        const size_t m = (size_t)knn->n * knn->k;
+        if (idx) d2h(idx, knn->idx.p, m * 4, st);
+        if (dist) d2h(dist, knn->dist.p, m * 4, st);
+        GS_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int gs_knn_device(gs_knn knn, uint32_t** idx, float** dist) {
+    return guarded([&] {
+        check(knn, "knn");
+        // The caller may now write the buffers: their contents can no longer
+        // be trusted as an exact previous result.
+        knn->seed_valid = false;
+        if (idx) *idx = knn->idx.p;
+        if (dist) *dist = knn->dist.p;
+    });
+}
+
+// ---- operator API ------------------------------------------------------------------
+int gs_update_knn_database(gs_group g, gs_bins bins) {
+    return guarded([&] {
+        check_group(g, "group");
+        check(bins, "bins");
+        if (bins->ctx != g->ctx) fail(GS_ERR_INVALID_ARGUMENT, "bins belong to another context");
+        use_device(g->ctx);
+        launch_bins_build(g, bins, false, -1, nullptr, nullptr);
+    });
+}
+
+int gs_find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t search_radius, uint32_t flags) {
+    return guarded([&] {
+        check_group(A, "group A");
+        check_group(B, "group B");
+        check(out, "knn");
+        check(bins, "bins");
+        if (A->ctx != B->ctx || bins->ctx != A->ctx) fail(GS_ERR_INVALID_ARGUMENT, "handles of different contexts");
+        require_knn_shape(out, A);
+        if (!A->binned || !B->binned) fail(GS_ERR_NOT_BINNED, "call gs_update_knn_database on both groups first");
+        if (!same_grid(B->bin_grid, bins->g) || !same_grid(A->bin_grid, bins->g))
+            fail(GS_ERR_NOT_BINNED, "groups were binned with another lattice");
+        use_device(A->ctx);
+        launch_knn_binned(A, out, B, search_radius, seed_usable(out, A, B, flags));
+        mark_seed(out, A, B);
+    });
+}
+
+int gs_find_knn_brute(gs_group A, gs_knn out, gs_group B) {
+    return guarded([&] {
+        check_group(A, "group A");
+        check_group(B, "group B");
+        check(out, "knn");
+        if (A->ctx != B->ctx) fail(GS_ERR_INVALID_ARGUMENT, "handles of different contexts");
+        require_knn_shape(out, A);
+        use_device(A->ctx);
+        launch_knn_brute(A, out, B);
+        mark_seed(out, A, B);
+    });
+}
+
+static int flock_call(gs_group A, gs_knn knn, gs_group B, const gs_flock_term* s, const gs_flock_term* a,
+                      const gs_flock_term* c, int mode) {
+    return guarded([&] {
+        check_group(A, "group A", GS_KIND_AGENTS);
+        check_group(B, "group B", GS_KIND_AGENTS);
+        check(knn, "knn");
+        require_knn_shape(knn, A);
+        if (mode != GS_MODE_OPENSTEER && mode != GS_MODE_GPUREF) fail(GS_ERR_INVALID_ARGUMENT, "unknown mode");
+        use_device(A->ctx);
+        launch_flock(A, knn, B, s, a, c, mode);
+    });
+}
+
+int gs_steer_separation(gs_group A, gs_knn knn, gs_group B, float min_d, float max_d, float cos_max, float w,
+                        uint32_t mask, int mode) {
+    const gs_flock_term t{min_d, max_d, cos_max, w, mask};
+    return flock_call(A, knn, B, &t, nullptr, nullptr, mode);
+}
+int gs_steer_alignment(gs_group A, gs_knn knn, gs_group B, float min_d, float max_d, float cos_max, float w,
+                       uint32_t mask, int mode) {
+    const gs_flock_term t{min_d, max_d, cos_max, w, mask};
+    return flock_call(A, knn, B, nullptr, &t, nullptr, mode);
+}
+int gs_steer_cohesion(gs_group A, gs_knn knn, gs_group B, float min_d, float max_d, float cos_max, float w,
+                      uint32_t mask, int mode) {
+    const gs_flock_term t{min_d, max_d, cos_max, w, mask};
+    return flock_call(A, knn, B, nullptr, nullptr, &t, mode);
+}
+int gs_steer_flocking(gs_group A, gs_knn knn, gs_group B, const gs_flock_term* s, const gs_flock_term* a,
+                      const gs_flock_term* c, int mode) {
+    return flock_call(A, knn, B, s, a, c, mode);
+}
+
+int gs_steer_seek(gs_group A, const float target[3], float w, uint32_t mask, int mode) {
+    return guarded([&] {
+        check_group(A, "group A", GS_KIND_AGENTS);
+        if (!target) fail(GS_ERR_INVALID_ARGUMENT, "target is NULL");
+        if (mode != GS_MODE_OPENSTEER && mode != GS_MODE_GPUREF) fail(GS_ERR_INVALID_ARGUMENT, "unknown mode");
+        use_device(A->ctx);
+        launch_seek(A, target, w, mask, mode);
+    });
+}
+
+int gs_steer_avoid_obstacles(gs_group A, gs_group O, gs_knn knn, float min_time, float w, uint32_t mask) {
+    return guarded([&] {
+        check_group(A, "group A", GS_KIND_AGENTS);
+        check_group(O, "obstacles", GS_KIND_OBSTACLES);
+        check(knn, "knn");
+        require_knn_shape(knn, A);
+        use_device(A->ctx);
+        launch_avoid_obstacles(A, O, knn, min_time, w, mask);
+    });
+}
+
+int gs_steer_avoid_walls(gs_group A, gs_knn knn, gs_group W, float min_time, float w, uint32_t mask) {
+    return guarded([&] {
+        check_group(A, "group A", GS_KIND_AGENTS);
+        check_group(W, "walls", GS_KIND_WALLS);
+        check(knn, "knn");
+        require_knn_shape(knn, A);
+        use_device(A->ctx);
+        launch_avoid_walls(A, knn, W, min_time, w, mask);
+    });
+}
+
+int gs_update_group(gs_group A, float dt, int mode) {
+    return guarded([&] {
+        check_group(A, "group A", GS_KIND_AGENTS);
+        if (mode != GS_MODE_OPENSTEER && mode != GS_MODE_GPUREF) fail(GS_ERR_INVALID_ARGUMENT, "unknown mode");
+        use_device(A->ctx);
+        launch_update(A, dt, mode);
+    });
+}
+
+int gs_wrap_world(gs_group A, int shape, const float center[3], const float size_or_radius[3]) {
+    return guarded([&] {
+        check_group(A, "group A", GS_KIND_AGENTS);
+        if (!center || !size_or_radius) fail(GS_ERR_INVALID_ARGUMENT, "NULL argument");
+        if (shape != GS_WRAP_BOX && shape != GS_WRAP_SPHERE) fail(GS_ERR_INVALID_ARGUMENT, "unknown wrap shape");
+        use_device(A->ctx);
+        launch_wrap(A, shape, center, size_or_radius);
+    });
+}
+
+int gs_clear_bits(gs_group A) {
+    return guarded([&] {
+        check_group(A, "group A", GS_KIND_AGENTS);
+        use_device(A->ctx);
+        launch_clear_bits(A);
+    });
+}
+
+int gs_step_boids(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* p, uint32_t steps) {
+    return guarded([&] {
+        check_group(A, "group A", GS_KIND_AGENTS);
+        check(knn, "knn");
+        check(bins, "bins");
+        if (!p) fail(GS_ERR_INVALID_ARGUMENT, "params is NULL");
+        if (bins->ctx != A->ctx) fail(GS_ERR_INVALID_ARGUMENT, "handles of different contexts");
+        require_knn_shape(knn, A);
+        if (p->mode != GS_MODE_OPENSTEER && p->mode != GS_MODE_GPUREF) fail(GS_ERR_INVALID_ARGUMENT, "unknown mode");
+        if (p->wrap_shape != -1 && p->wrap_shape != GS_WRAP_BOX && p->wrap_shape != GS_WRAP_SPHERE)
+            fail(GS_ERR_INVALID_ARGUMENT, "unknown wrap shape");
+        use_device(A->ctx);
+        for (uint32_t s = 0; s < steps; ++s) {
+            launch_bins_build(A, bins, true, p->wrap_shape, p->wrap_center, p->wrap_size);
+            launch_boids_fused(A, knn, p, seed_usable(knn, A, A, p->knn_flags));
+            mark_seed(knn, A, A);
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,261 @@
+// Bin database build: counting sort of a group's points by lattice cell.
+//
+// Replaces KNNBinningV2UpdateDBCUDA::run (include/OpenSteer/CUDA/KNNBinningV2.cu:87-162):
+// BuildDB (cell key per agent, KNNBinningV2Kernel.cu:269-296), thrust::sort_by_key
+// (:127-129), the cellStart memset (:132) and ReorderDB (KNNBinningV2Kernel.cu:314-379).
+//
+// Passes (all coalesced over agents or cells):
+//   1. key      cell per point + atomic arrival rank within the cell (+ optional wrap)
+//   2. scan     exclusive scan of the per-cell counts -> start[C+1]; clears counts
+//   3. scatter  perm_tmp[start[cell] + rank] = i
+//   4. canon    makes the order inside each cell ascending by point index (a
+//               stable sort, as thrust's), gathers sorted positions (and
+//               forward vectors for the fused step) and records each point's
+//               sorted slot (inverse permutation)
+// The counts array is all-zero between builds (pass 2 clears it), so no
+// memset is recorded in the stream.
+#include "gs_internal.hpp"
+
+namespace gs {
+
+// Cells with more members than this keep the atomic arrival order in pass 4
+// (KNN results never depend on the order inside a cell).
+constexpr uint32_t kCanonCap = 4096;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;  // 2048 cells per scan block
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_key(uint32_t n, float* __restrict__ px, float* __restrict__ py,
+                                                    float* __restrict__ pz, GridDev g, uint32_t* __restrict__ key,
+                                                    uint32_t* __restrict__ rank, uint32_t* __restrict__ count,
+                                                    int wrap_shape, float wcx, float wcy, float wcz, float wsx,
+                                                    float wsy, float wsz) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = px[i], y = py[i], z = pz[i];
+    if (wrap_shape == GS_WRAP_BOX) {
+        // WrapWorldKernel (CUDA/WrapWorldCUDAKernel.cu:25-35), all three axes.
+        const float hx = wsx * 0.5f, hy = wsy * 0.5f, hz = wsz * 0.5f;
+        const float x0 = x, y0 = y, z0 = z;
+        if (x < wcx - hx) x = wcx + hx;
+        else if (x > wcx + hx) x = wcx - hx;
+        if (y < wcy - hy) y = wcy + hy;
+        else if (y > wcy + hy) y = wcy - hy;
+        if (z < wcz - hz) z = wcz + hz;
+        else if (z > wcz + hz) z = wcz - hz;
+        if (x != x0) px[i] = x;
+        if (y != y0) py[i] = y;
+        if (z != z0) pz[i] = z;
+    } else if (wrap_shape == GS_WRAP_SPHERE) {
+        // sphericalWrapAround (plugins/Boids.cpp:200-206).
+        const f3 off = mk3(x - wcx, y - wcy, z - wcz);
+        const float r = len3(off);
+        if (r > wsx) {
+            const f3 p = mk3(x, y, z) + ((off / r) * wsx) * -2.f;
+            x = p.x;
+            y = p.y;
+            z = p.z;
+            px[i] = x;
+            py[i] = y;
+            pz[i] = z;
+        }
+    }
+    const uint32_t c = cell_index(g, axis_cell(x, g.ox, g.sx, g.dx), axis_cell(y, g.oy, g.sy, g.dy),
+                                  axis_cell(z, g.oz, g.sz, g.dz));
+    key[i] = c;
+    rank[i] = atomicAdd(&count[c], 1u);
+}
+
+// Wavefront-then-block exclusive scan of one value per thread.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds_wave, uint32_t& total) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
+    }
+    if (lane == 63) lds_wave[wave] = inc;
+    __syncthreads();
+    uint32_t wave_off = 0, sum = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const uint32_t s = lds_wave[w];
+        if (w < wave) wave_off += s;
+        sum += s;
+    }
+    total = sum;
+    return wave_off + inc - v;
+}
+
+// Pass 2a: each block scans kScanTile counts locally, writes local offsets
+// into start and its tile sum into partial[block].
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(uint32_t C, const uint32_t* __restrict__ count,
+                                                       uint32_t* __restrict__ start, uint32_t* __restrict__ partial) {
+    __shared__ uint32_t lds[kBlock / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = (base + j < C) ? count[base + j] : 0u;
+        sum += v[j];
+    }
+    uint32_t total;
+    uint32_t off = block_exclusive_scan(sum, lds, total);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j < C) start[base + j] = off;
+        off += v[j];
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// Pass 2b: one block scans the tile sums in place (exclusive) and writes the
+// grand total to start[C].
+__global__ __launch_bounds__(kBlock) void k_scan_partials(uint32_t nparts, uint32_t* __restrict__ partial,
+                                                          uint32_t* __restrict__ start, uint32_t C) {
+    __shared__ uint32_t lds[kBlock / 64];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nparts; base += kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nparts ? partial[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, lds, total);
+        if (i < nparts) partial[i] = carry + ex;
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[C] = carry;
+}
+
+// Pass 2c: add the tile offsets and clear the counts for the next build.
+__global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t C, const uint32_t* __restrict__ partial,
+                                                     uint32_t* __restrict__ start, uint32_t* __restrict__ count) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    const uint32_t add = partial[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint64_t c = base + (uint64_t)j * kBlock + threadIdx.x;
+        if (c < C) {
+            start[c] += add;
+            count[c] = 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(uint32_t n, const uint32_t* __restrict__ key,
+                                                    const uint32_t* __restrict__ rank,
+                                                    const uint32_t* __restrict__ start,
+                                                    uint32_t* __restrict__ perm_tmp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // start[key]+rank < n whenever the counts were zero before the key pass;
+    // the guard keeps a corrupted count array from writing out of bounds.
+    const uint32_t d = start[key[i]] + rank[i];
+    if (d < n) perm_tmp[d] = i;
+}
+
+// Pass 4: slot s holds point i = perm_tmp[s]; its final slot is the cell start
+// plus the number of members of its cell with a smaller index.
+__global__ __launch_bounds__(kBlock) void k_canon(uint32_t n, const uint32_t* __restrict__ perm_tmp,
+                                                  const uint32_t* __restrict__ key,
+                                                  const uint32_t* __restrict__ start, const float* __restrict__ px,
+                                                  const float* __restrict__ py, const float* __restrict__ pz,
+                                                  const float* __restrict__ fx, const float* __restrict__ fy,
+                                                  const float* __restrict__ fz, uint32_t* __restrict__ perm,
+                                                  float* __restrict__ spx, float* __restrict__ spy,
+                                                  float* __restrict__ spz, float* __restrict__ sfx,
+                                                  float* __restrict__ sfy, float* __restrict__ sfz,
+                                                  uint32_t* __restrict__ inv) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t i = perm_tmp[s];
+    if (i >= n) return;  // only after a corrupted build (see k_scatter)
+    const uint32_t c = key[i];
+    const uint32_t b = start[c], e = min(start[c + 1], n);
+    uint32_t d = s;
+    if (e - b <= kCanonCap) {
+        uint32_t r = 0;
+        for (uint32_t j = b; j < e; ++j) r += (perm_tmp[j] < i) ? 1u : 0u;
+        d = b + r;
+    }
+    perm[d] = i;
+    spx[d] = px[i];
+    spy[d] = py[i];
+    spz[d] = pz[i];
+    inv[i] = d;
+    if (sfx) {
+        sfx[d] = fx[i];
+        sfy[d] = fy[i];
+        sfz[d] = fz[i];
+    }
+}
+
+void launch_bins_build(gs_group_s* g, const gs_bins_s* b, bool gather_forward, int wrap_shape, const float* wc,
+                       const float* ws) {
+    const uint32_t n = g->n;
+    const uint32_t C = b->cells;
+    hipStream_t st = g->ctx->stream;
+    const bool new_cells = !g->binned || g->bin_cells != C || g->count.cap < C;
+    g->key.reserve(n);
+    g->rank.reserve(n);
+    g->perm_tmp.reserve(n);
+    g->perm.reserve(n);
+    g->spx.reserve(n);
+    g->spy.reserve(n);
+    g->spz.reserve(n);
+    g->inv.reserve(n);
+    if (gather_forward) {
+        g->sfx.reserve(n);
+        g->sfy.reserve(n);
+        g->sfz.reserve(n);
+    }
+    const uint32_t ntiles = (C + kScanTile - 1) / kScanTile;
+    g->start.reserve((size_t)C + 1 + ntiles);  // partial tile sums live after start[C]
+    if (new_cells) {
+        g->count.reserve(C);
+        // A fill kernel rather than hipMemsetAsync, so the build can be
+        // captured into a graph.
+        k_fill_u32<<<blocks_for(C), kBlock, 0, st>>>(g->count.p, C, 0u);
+        GS_LAUNCHED("k_fill_u32");
+    }
+    g->bin_cells = C;
+    g->bin_grid = b->g;
+    uint32_t* partial = g->start.p + C + 1;
+    float wcv[3] = {0, 0, 0}, wsv[3] = {0, 0, 0};
+    if (wc) memcpy(wcv, wc, sizeof wcv);
+    if (ws) memcpy(wsv, ws, sizeof wsv);
+    if (n > 0) {
+        k_bin_key<<<blocks_for(n), kBlock, 0, st>>>(n, g->f[0].p, g->f[1].p, g->f[2].p, b->g, g->key.p, g->rank.p,
+                                                    g->count.p, wrap_shape, wcv[0], wcv[1], wcv[2], wsv[0], wsv[1],
+                                                    wsv[2]);
+        GS_LAUNCHED("k_bin_key");
+    }
+    k_scan_tiles<<<ntiles, kBlock, 0, st>>>(C, g->count.p, g->start.p, partial);
+    GS_LAUNCHED("k_scan_tiles");
+    k_scan_partials<<<1, kBlock, 0, st>>>(ntiles, partial, g->start.p, C);
+    GS_LAUNCHED("k_scan_partials");
+    k_scan_add<<<ntiles, kBlock, 0, st>>>(C, partial, g->start.p, g->count.p);
+    GS_LAUNCHED("k_scan_add");
+    if (n > 0) {
+        k_scatter<<<blocks_for(n), kBlock, 0, st>>>(n, g->key.p, g->rank.p, g->start.p, g->perm_tmp.p);
+        GS_LAUNCHED("k_scatter");
+        const bool fwd = gather_forward && g->kind == GS_KIND_AGENTS;
+        k_canon<<<blocks_for(n), kBlock, 0, st>>>(n, g->perm_tmp.p, g->key.p, g->start.p, g->f[0].p, g->f[1].p,
+                                                  g->f[2].p, fwd ? g->f[3].p : nullptr, fwd ? g->f[4].p : nullptr,
+                                                  fwd ? g->f[5].p : nullptr, g->perm.p, g->spx.p, g->spy.p,
+                                                  g->spz.p, fwd ? g->sfx.p : nullptr, fwd ? g->sfy.p : nullptr,
+                                                  fwd ? g->sfz.p : nullptr, g->inv.p);
+        GS_LAUNCHED("k_canon");
+    }
+    g->binned = true;
+    g->bin_epoch++;
+}
+
+}  // namespace gs

@@ -1,0 +1,198 @@
+// Internal declarations shared by the HIP translation units of libgpusteer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/gpusteer.h"
+
+namespace gs {
+
+// ---------------------------------------------------------------------------
+// Errors: internal code throws gs::Error; every C entry point converts it to a
+// status and records the message for gs_last_error().
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+void set_last_error(const std::string& m);
+[[noreturn]] void fail(int status, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+#define GS_HIP(call) ::gs::hip_check((call), #call)
+// Checks the launch of the preceding kernel.
+#define GS_LAUNCHED(name) ::gs::hip_check(hipGetLastError(), name)
+
+constexpr int kBlock = 256;
+inline unsigned blocks_for(uint64_t n, unsigned per = kBlock) { return (unsigned)((n + per - 1) / per); }
+
+// ---------------------------------------------------------------------------
+// Device memory: a typed buffer that grows on demand.  Owned by handles.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+template <class T> struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;  // elements
+    void reserve(size_t n) {
+        if (n <= cap) return;
+        T* q = (T*)dev_alloc(n * sizeof(T));
+        if (p) dev_free(p);
+        p = q;
+        cap = n;
+    }
+    void release() {
+        if (p) dev_free(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Bin-lattice geometry as kernels see it.  Cell mapping follows src/lq.c:185-208
+// (ix = (int)(((x - ox) / sx) * dx)), index ix*dy*dz + iy*dz + iz (:175-176);
+// points outside the lattice are clamped into the boundary cells.
+struct GridDev {
+    float ox, oy, oz;
+    float sx, sy, sz;
+    int dx, dy, dz;
+    float hx, hy, hz;  // cell size (s/d)
+    float slack;       // absolute rounding allowance of the mapping
+};
+
+__device__ __forceinline__ int axis_cell(float p, float o, float s, int d) {
+    if (!(p >= o)) return 0;
+    if (p >= o + s) return d - 1;
+    int i = (int)(((p - o) / s) * (float)d);
+    return i < d ? i : d - 1;
+}
+__device__ __forceinline__ uint32_t cell_index(const GridDev& g, int ix, int iy, int iz) {
+    return (uint32_t)((ix * g.dy * g.dz) + (iy * g.dz) + iz);
+}
+
+// ---------------------------------------------------------------------------
+// fp32 vector algebra in the reference's evaluation order
+// (include/OpenSteer/VectorUtils.cuh).  The library is compiled with
+// -ffp-contract=off and correctly rounded division and square root.
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator-(f3 a) { return mk3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return ((a.x * b.x) + (a.y * b.y)) + (a.z * b.z); }
+__device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ f3 normalize3(f3 a) {
+    const float l = len3(a);
+    return (l > 0.f) ? a / l : a;
+}
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+    return mk3((a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x));
+}
+__device__ __forceinline__ bool zero3(f3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
+__device__ __forceinline__ f3 perp3(f3 v, f3 unit) { return v - unit * dot3(v, unit); }
+__device__ __forceinline__ f3 truncate3(f3 v, float m) {
+    const float m2 = m * m;
+    const float l2 = dot3(v, v);
+    if (l2 <= m2) return v;
+    return v * (m / sqrtf(l2));
+}
+__device__ __forceinline__ float sqdist(float ax, float ay, float az, float bx, float by, float bz) {
+    const float dx = ax - bx, dy = ay - by, dz = az - bz;
+    return ((dx * dx) + (dy * dy)) + (dz * dz);
+}
+
+// Agent arrays as kernels take them (device pointers).
+typedef gs_agents Agents;
+__device__ __forceinline__ f3 ld3(const float* x, const float* y, const float* z, uint32_t i) {
+    return mk3(x[i], y[i], z[i]);
+}
+__device__ __forceinline__ void st3(float* x, float* y, float* z, uint32_t i, f3 v) {
+    x[i] = v.x;
+    y[i] = v.y;
+    z[i] = v.z;
+}
+
+}  // namespace gs
+
+// ---------------------------------------------------------------------------
+// Handles.
+struct gs_context_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    uint32_t magic = 0x67736378u;
+};
+
+struct gs_bins_s {
+    gs_context ctx = nullptr;
+    gs::GridDev g{};
+    uint32_t cells = 0;
+    uint32_t magic = 0x67736269u;
+};
+
+struct gs_group_s {
+    gs_context ctx = nullptr;
+    int kind = GS_KIND_AGENTS;
+    uint32_t n = 0;
+    uint64_t membership_epoch = 0;  // bumped whenever the members change
+    uint32_t magic = 0x67736772u;
+
+    // Agents: 23 float arrays + bits, one allocation each (SoA).
+    gs::DevBuf<float> f[23];
+    gs::DevBuf<uint32_t> bits;
+    // Obstacles: centre in f[0..2] (px,py,pz), radius in f[19].
+    // Walls: midpoint in f[0..2]; start/end/normal in w[0..8].
+    gs::DevBuf<float> w[9];
+
+    // Bin database (KNNDatabase, include/OpenSteer/CUDA/KNNDatabase.cuh:30-38).
+    bool binned = false;
+    gs::GridDev bin_grid{};
+    uint32_t bin_cells = 0;
+    uint64_t bin_epoch = 0;
+    gs::DevBuf<uint32_t> key, rank, perm_tmp, perm, start, count;
+    gs::DevBuf<float> spx, spy, spz;  // sorted positions
+    gs::DevBuf<float> sfx, sfy, sfz;  // sorted forward (agents, fused step)
+    gs::DevBuf<uint32_t> inv;         // sorted slot of each point
+    gs::DevBuf<uint32_t> flag;        // small device scratch (finite check)
+
+    gs_agents dev_agents() const;
+};
+
+struct gs_knn_s {
+    gs_context ctx = nullptr;
+    uint32_t n = 0, k = 0;
+    gs::DevBuf<uint32_t> idx;
+    gs::DevBuf<float> dist;
+    // Seed validity: the contents are the exact result for (seed_a, seed_b)
+    // with these membership epochs.
+    const gs_group_s* seed_a = nullptr;
+    const gs_group_s* seed_b = nullptr;
+    uint64_t seed_epoch_a = 0, seed_epoch_b = 0;
+    bool seed_valid = false;
+    uint32_t magic = 0x67736b6eu;
+};
+
+namespace gs {
+// Host-side launchers (one per operation) implemented in the .hip files.
+void launch_bins_build(gs_group_s* g, const gs_bins_s* b, bool gather_forward, int wrap_shape, const float* wc,
+                       const float* ws);
+void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, bool use_seed);
+void launch_knn_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B);
+void launch_flock(gs_group_s* A, gs_knn_s* knn, gs_group_s* B, const gs_flock_term* sep, const gs_flock_term* ali,
+                  const gs_flock_term* coh, int mode);
+void launch_seek(gs_group_s* A, const float* target, float w, uint32_t mask, int mode);
+void launch_avoid_obstacles(gs_group_s* A, gs_group_s* O, gs_knn_s* knn, float min_time, float w, uint32_t mask);
+void launch_avoid_walls(gs_group_s* A, gs_knn_s* knn, gs_group_s* W, float min_time, float w, uint32_t mask);
+void launch_update(gs_group_s* A, float dt, int mode);
+void launch_wrap(gs_group_s* A, int shape, const float* c, const float* s);
+void launch_clear_bits(gs_group_s* A);
+void launch_check_finite(gs_group_s* A, uint32_t* first_bad_host);
+void launch_boids_fused(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, bool use_seed);
+}  // namespace gs

@@ -1,0 +1,322 @@
+// Steering behaviours, integration and world wrap: one lane per agent, SoA
+// fp32 state read and written coalesced; neighbour data gathered through the
+// KNN indices (spatially close agents, served from L2 / Infinity Cache).
+//
+// Every behaviour follows the reference kernels' accumulate convention
+// (e.g. SteerForSeparationCUDAKernel.cu:74-75,124-134): skip the agent when
+// bits & mask, add f*w into the steering accumulator as (f*w) + steering, and
+// set the behaviour's bit when f*w is non-zero.
+#include "gs_ops.hpp"
+
+namespace gs {
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_flock(uint32_t nA, Agents A, uint32_t nB, const float* __restrict__ bpx,
+                                                  const float* __restrict__ bpy, const float* __restrict__ bpz,
+                                                  const float* __restrict__ bfx, const float* __restrict__ bfy,
+                                                  const float* __restrict__ bfz, const uint32_t* __restrict__ knn,
+                                                  uint32_t k, FlockTerm sep, FlockTerm ali, FlockTerm coh) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nA) return;
+    uint32_t bits = A.bits[i];
+    f3 steering = ld3(A.gx, A.gy, A.gz, i);
+    const uint32_t bits0 = bits;
+    flock_core<MODE, 0>(ld3(A.px, A.py, A.pz, i), ld3(A.fx, A.fy, A.fz, i), A.radius[i], k, sep, ali, coh,
+                        steering, bits, [&](int j, bool want_fwd, f3& bp, f3& bf) {
+                            const uint32_t b = knn[(size_t)j * nA + i];
+                            if (b >= nB) return (int)kNbrStop;
+                            bp = ld3(bpx, bpy, bpz, b);
+                            if (want_fwd) bf = ld3(bfx, bfy, bfz, b);
+                            return (int)kNbrValid;
+                        });
+    if (!(sep.on && !(bits0 & sep.mask)) && !(ali.on && !(bits0 & ali.mask)) && !(coh.on && !(bits0 & coh.mask)))
+        return;
+    st3(A.gx, A.gy, A.gz, i, steering);
+    A.bits[i] = bits;
+}
+
+// Seek: OPENSTEER = src/SteerLibrary.cpp:36-42 (target - position - velocity);
+// GPUREF = SteerForSeekCUDAKernel.cu:52-61 (normalize(target - position - forward)).
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_seek(uint32_t n, Agents A, float tx, float ty, float tz, float w,
+                                                 uint32_t mask) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t bits = A.bits[i];
+    if (bits & mask) return;
+    const f3 desired = mk3(tx, ty, tz) - ld3(A.px, A.py, A.pz, i);
+    const f3 fwd = ld3(A.fx, A.fy, A.fz, i);
+    const f3 f = (MODE == GS_MODE_GPUREF) ? normalize3(desired - fwd) : desired - fwd * A.speed[i];
+    f3 steering = ld3(A.gx, A.gy, A.gz, i);
+    apply(steering, bits, f, w, GS_BIT_SEEK);
+    st3(A.gx, A.gy, A.gz, i, steering);
+    A.bits[i] = bits;
+}
+
+// Obstacle avoidance over the k nearest obstacles: SteerToAvoidObstaclesKernel
+// with findNextIntersectionWithSphere (CUDA/AvoidObstaclesCUDAKernel.cu:47-99,136-253).
+__global__ __launch_bounds__(kBlock) void k_avoid_obstacles(uint32_t n, Agents A, uint32_t nO,
+                                                            const float* __restrict__ ox, const float* __restrict__ oy,
+                                                            const float* __restrict__ oz, const float* __restrict__ orad,
+                                                            const uint32_t* __restrict__ knn, uint32_t k,
+                                                            float min_time, float w, uint32_t mask) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t bits = A.bits[i];
+    if (bits & mask) return;
+    const f3 pos = ld3(A.px, A.py, A.pz, i), fwd = ld3(A.fx, A.fy, A.fz, i);
+    const f3 side = ld3(A.sx, A.sy, A.sz, i), up = ld3(A.ux, A.uy, A.uz, i);
+    const float radius = A.radius[i];
+    const float min_dist = min_time * A.speed[i];
+    bool n_hit = false;
+    float n_dist = 0.f;
+    f3 n_pos = mk3(0.f, 0.f, 0.f);
+    for (uint32_t j = 0; j < k; ++j) {
+        const uint32_t o = knn[(size_t)j * n + i];
+        if (o >= nO) break;
+        const f3 op = ld3(ox, oy, oz, o);
+        const float orr = orad[o];
+        bool x_hit = false;
+        float x_dist = 0.f;
+        const f3 go = op - pos;
+        const f3 lc = mk3(dot3(go, side), dot3(go, up), dot3(go, fwd));
+        const float b = -2.f * lc.z;
+        const float c = ((lc.x * lc.x + lc.y * lc.y) + lc.z * lc.z) - (orr + radius) * (orr + radius);
+        const float d = (b * b) - (4.f * c);
+        if (!(d < 0.f)) {
+            const float s = sqrtf(d);
+            const float p = (-b + s) / 2.f;
+            const float q = (-b - s) / 2.f;
+            if (!((p < 0.f) && (q < 0.f))) {
+                x_hit = true;
+                x_dist = ((p > 0.f) && (q > 0.f)) ? ((p < q) ? p : q) : ((p > 0.f) ? p : q);
+            }
+        }
+        if ((n_hit == false) || ((x_hit != false) && (x_dist < n_dist))) {
+            n_hit = x_hit;
+            n_dist = x_dist;
+            n_pos = op;
+        }
+    }
+    f3 f = mk3(0.f, 0.f, 0.f);
+    if (n_hit && (n_dist < min_dist)) f = perp3(pos - n_pos, fwd);
+    f3 steering = ld3(A.gx, A.gy, A.gz, i);
+    apply(steering, bits, f, w, GS_BIT_AVOID_OBSTACLES);
+    st3(A.gx, A.gy, A.gz, i, steering);
+    A.bits[i] = bits;
+}
+
+// Wall avoidance over the k nearest wall segments: SteerToAvoidWallsCUDAKernel
+// (CUDA/AvoidWallsCUDAKernel.cu:67-245) — three feelers, touching test, the
+// anti-penetration move of touching agents (:215-223) and its bit rules.
+__global__ __launch_bounds__(kBlock) void k_avoid_walls(uint32_t n, Agents A, uint32_t nW, gs_walls W,
+                                                        const uint32_t* __restrict__ knn, uint32_t k, float min_time,
+                                                        float w, uint32_t mask) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t bits = A.bits[i];
+    if (bits & mask) return;
+    f3 pos = ld3(A.px, A.py, A.pz, i), dir = ld3(A.fx, A.fy, A.fz, i);
+    const f3 side = ld3(A.sx, A.sy, A.sz, i);
+    const float radius = A.radius[i];
+    const float fl = 4.f * radius;  // FEELER_LENGTH (:41) * radius
+    const f3 vel = dir * A.speed[i];
+    f3 feelers[3];
+    feelers[0] = pos + vel * min_time;
+    feelers[1] = pos + (side * fl + dir * fl);
+    feelers[2] = pos + ((-side) * fl + dir * fl);
+    int feeler = 0;
+    float nearest_dist = FLT_MAX;
+    uint32_t nearest_line = UINT32_MAX;
+    f3 nearest_ip = mk3(0.f, 0.f, 0.f), nearest_normal = nearest_ip;
+    bool touching = false;
+    for (uint32_t j = 0; j < k; ++j) {
+        const uint32_t l = knn[(size_t)j * n + i];
+        if (l >= nW) break;
+        const f3 ln = ld3(W.nx, W.ny, W.nz, l);
+        const f3 ls = ld3(W.sx, W.sy, W.sz, l);
+        const f3 le = ld3(W.ex, W.ey, W.ez, l);
+        f3 ip;
+        if (lines_intersect(pos, pos + (-ln) * radius, ls, le, ip)) {
+            touching = true;
+            nearest_normal = ln;
+            nearest_dist = len3(pos - ip);
+        }
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            if (lines_intersect(pos, feelers[f], ls, le, ip)) {
+                const float dist = len3(pos - ip);
+                if (dist < nearest_dist) {
+                    nearest_dist = dist;
+                    nearest_line = l;
+                    nearest_ip = ip;
+                    feeler = f;
+                    nearest_normal = ln;
+                }
+            }
+        }
+    }
+    f3 steering = mk3(0.f, 0.f, 0.f);
+    if (touching) {
+        const float pen = radius - nearest_dist;
+        pos = pos + nearest_normal * pen;
+        dir = normalize3(perp3(dir, nearest_normal));
+        bits |= GS_BIT_ANTI_PENETRATION_WALL;
+    } else if (nearest_line != UINT32_MAX) {
+        const f3 fe = feeler == 0 ? feelers[0] : (feeler == 1 ? feelers[1] : feelers[2]);
+        steering = nearest_normal * len3(fe - nearest_ip);
+        bits |= GS_BIT_AVOID_WALLS;
+    }
+    steering = steering * w;
+    st3(A.gx, A.gy, A.gz, i, steering + ld3(A.gx, A.gy, A.gz, i));
+    st3(A.px, A.py, A.pz, i, pos);
+    st3(A.fx, A.fy, A.fz, i, dir);
+    A.bits[i] = bits;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_update(uint32_t n, Agents A, float dt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 pos = ld3(A.px, A.py, A.pz, i), fwd = ld3(A.fx, A.fy, A.fz, i);
+    f3 side = ld3(A.sx, A.sy, A.sz, i), up = ld3(A.ux, A.uy, A.uz, i);
+    f3 sacc = ld3(A.ax, A.ay, A.az, i);
+    float speed = A.speed[i];
+    integrate<MODE>(ld3(A.gx, A.gy, A.gz, i), pos, fwd, speed, sacc, side, up, A.max_force[i], A.max_speed[i],
+                    A.mass[i], dt);
+    st3(A.px, A.py, A.pz, i, pos);
+    st3(A.fx, A.fy, A.fz, i, fwd);
+    st3(A.sx, A.sy, A.sz, i, side);
+    st3(A.ux, A.uy, A.uz, i, up);
+    if (MODE == GS_MODE_OPENSTEER) st3(A.ax, A.ay, A.az, i, sacc);
+    A.speed[i] = speed;
+    st3(A.gx, A.gy, A.gz, i, mk3(0.f, 0.f, 0.f));
+    A.bits[i] = 0u;
+}
+
+// World wrap: GS_WRAP_BOX = WrapWorldKernel face teleport
+// (CUDA/WrapWorldCUDAKernel.cu:25-35, all three axes); GS_WRAP_SPHERE =
+// sphericalWrapAround of plugins/Boids.cpp:200-206.
+__global__ __launch_bounds__(kBlock) void k_wrap(uint32_t n, float* __restrict__ px, float* __restrict__ py,
+                                                 float* __restrict__ pz, int shape, float cx, float cy, float cz,
+                                                 float s0, float s1, float s2) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 p = ld3(px, py, pz, i);
+    if (shape == GS_WRAP_BOX) {
+        const float hx = s0 * 0.5f, hy = s1 * 0.5f, hz = s2 * 0.5f;
+        if (p.x < cx - hx) p.x = cx + hx;
+        else if (p.x > cx + hx) p.x = cx - hx;
+        if (p.y < cy - hy) p.y = cy + hy;
+        else if (p.y > cy + hy) p.y = cy - hy;
+        if (p.z < cz - hz) p.z = cz + hz;
+        else if (p.z > cz + hz) p.z = cz - hz;
+    } else {
+        const f3 off = p - mk3(cx, cy, cz);
+        const float r = len3(off);
+        if (r > s0) p = p + ((off / r) * s0) * -2.f;
+    }
+    st3(px, py, pz, i, p);
+}
+
+__global__ __launch_bounds__(kBlock) void k_clear_bits(uint32_t n, uint32_t* __restrict__ bits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) bits[i] = 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_finite(uint32_t n, Agents A, uint32_t* __restrict__ first_bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* const* f = (const float* const*)&A;
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 23; ++j) ok = ok && isfinite(f[j][i]);
+    if (!ok) atomicMin(first_bad, i);
+}
+
+static FlockTerm term(const gs_flock_term* t) {
+    FlockTerm r{0.f, 0.f, 0.f, 0.f, 0u, 0};
+    if (t) r = FlockTerm{t->min_d, t->max_d, t->cos_max, t->weight, t->mask, 1};
+    return r;
+}
+
+void launch_flock(gs_group_s* A, gs_knn_s* knn, gs_group_s* B, const gs_flock_term* sep, const gs_flock_term* ali,
+                  const gs_flock_term* coh, int mode) {
+    if (A->n == 0) return;
+    hipStream_t st = A->ctx->stream;
+    const Agents a = A->dev_agents();
+    if (mode == GS_MODE_GPUREF)
+        k_flock<GS_MODE_GPUREF><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, a, B->n, B->f[0].p, B->f[1].p, B->f[2].p,
+                                                                     B->f[3].p, B->f[4].p, B->f[5].p, knn->idx.p,
+                                                                     knn->k, term(sep), term(ali), term(coh));
+    else
+        k_flock<GS_MODE_OPENSTEER><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, a, B->n, B->f[0].p, B->f[1].p,
+                                                                        B->f[2].p, B->f[3].p, B->f[4].p, B->f[5].p,
+                                                                        knn->idx.p, knn->k, term(sep), term(ali),
+                                                                        term(coh));
+    GS_LAUNCHED("k_flock");
+}
+
+void launch_seek(gs_group_s* A, const float* t, float w, uint32_t mask, int mode) {
+    if (A->n == 0) return;
+    hipStream_t st = A->ctx->stream;
+    if (mode == GS_MODE_GPUREF)
+        k_seek<GS_MODE_GPUREF><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, A->dev_agents(), t[0], t[1], t[2], w, mask);
+    else
+        k_seek<GS_MODE_OPENSTEER><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, A->dev_agents(), t[0], t[1], t[2], w,
+                                                                       mask);
+    GS_LAUNCHED("k_seek");
+}
+
+void launch_avoid_obstacles(gs_group_s* A, gs_group_s* O, gs_knn_s* knn, float min_time, float w, uint32_t mask) {
+    if (A->n == 0) return;
+    k_avoid_obstacles<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->dev_agents(), O->n, O->f[0].p,
+                                                                        O->f[1].p, O->f[2].p, O->f[19].p, knn->idx.p,
+                                                                        knn->k, min_time, w, mask);
+    GS_LAUNCHED("k_avoid_obstacles");
+}
+
+void launch_avoid_walls(gs_group_s* A, gs_knn_s* knn, gs_group_s* W, float min_time, float w, uint32_t mask) {
+    if (A->n == 0) return;
+    gs_walls wd{W->w[0].p, W->w[1].p, W->w[2].p, W->w[3].p, W->w[4].p, W->w[5].p, W->w[6].p, W->w[7].p, W->w[8].p};
+    k_avoid_walls<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->dev_agents(), W->n, wd, knn->idx.p,
+                                                                    knn->k, min_time, w, mask);
+    GS_LAUNCHED("k_avoid_walls");
+}
+
+void launch_update(gs_group_s* A, float dt, int mode) {
+    if (A->n == 0) return;
+    hipStream_t st = A->ctx->stream;
+    if (mode == GS_MODE_GPUREF)
+        k_update<GS_MODE_GPUREF><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, A->dev_agents(), dt);
+    else
+        k_update<GS_MODE_OPENSTEER><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, A->dev_agents(), dt);
+    GS_LAUNCHED("k_update");
+}
+
+void launch_wrap(gs_group_s* A, int shape, const float* c, const float* s) {
+    if (A->n == 0) return;
+    k_wrap<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->f[0].p, A->f[1].p, A->f[2].p, shape, c[0], c[1],
+                                                            c[2], s[0], s[1], s[2]);
+    GS_LAUNCHED("k_wrap");
+}
+
+void launch_clear_bits(gs_group_s* A) {
+    if (A->n == 0) return;
+    k_clear_bits<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->bits.p);
+    GS_LAUNCHED("k_clear_bits");
+}
+
+void launch_check_finite(gs_group_s* A, uint32_t* first_bad_host) {
+    *first_bad_host = UINT32_MAX;
+    if (A->n == 0) return;
+    hipStream_t st = A->ctx->stream;
+    A->flag.reserve(1);
+    GS_HIP(hipMemcpyAsync(A->flag.p, first_bad_host, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    k_check_finite<<<blocks_for(A->n), kBlock, 0, st>>>(A->n, A->dev_agents(), A->flag.p);
+    GS_LAUNCHED("k_check_finite");
+    GS_HIP(hipMemcpyAsync(first_bad_host, A->flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    GS_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace gs

@@ -1,0 +1,222 @@
+"""Scenario generators: seeded synthetic states for the benchmark workloads.
+
+Every generator is deterministic (numpy Philox, seed 12345 + workload index,
+the seed of data/ChokePoint.params:3) and returns float32 struct-of-arrays
+state, so the HIP path and the CPU oracle run on identical inputs.
+
+Workloads
+---------
+boids_sphere   4,096-agent Boids flock in a sphere of radius 50 with antipode
+               wrap (plugins/Boids.cpp:89-209 parameters; k = 7, 16^3 lattice
+               over the 1.1 x R margin of plugins/Boids.cpp:375-380).
+choke          two 5,000-agent groups in 40 x 40 regions either side of a
+               wall with one gap (full width 15), 50 spherical obstacles,
+               seek goal in the opposite region (data/ChokePoint.params style).
+boids_box      2^20-agent flock in a 1000^3 periodic box (k = 7 or 16).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Tuple
+
+import numpy as np
+
+from .state import AgentState, WallState, basis_from_forward
+
+SEED = 12345
+
+
+def rng(workload: int, extra: int = 0) -> np.random.Generator:
+    return np.random.Generator(np.random.Philox(key=SEED + workload + 1000 * extra))
+
+
+def unit_vectors(g: np.random.Generator, n: int) -> np.ndarray:
+    v = g.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return v
+
+
+def make_agents(pos: np.ndarray, fwd: np.ndarray, speed: np.ndarray, radius: float, max_speed: float,
+                max_force: float, mass: float = 1.0) -> AgentState:
+    n = pos.shape[0]
+    s = AgentState.empty(n)
+    a = s.arrays
+    a["px"][:], a["py"][:], a["pz"][:] = pos[:, 0], pos[:, 1], pos[:, 2]
+    f = fwd.astype(np.float32)
+    a["fx"][:], a["fy"][:], a["fz"][:] = f[:, 0], f[:, 1], f[:, 2]
+    side, up = basis_from_forward(a["fx"], a["fy"], a["fz"])
+    a["sx"][:], a["sy"][:], a["sz"][:] = side[:, 0], side[:, 1], side[:, 2]
+    a["ux"][:], a["uy"][:], a["uz"][:] = up[:, 0], up[:, 1], up[:, 2]
+    a["speed"][:] = speed
+    a["radius"][:] = radius
+    a["max_speed"][:] = max_speed
+    a["max_force"][:] = max_force
+    a["mass"][:] = mass
+    return s
+
+
+@dataclass
+class FlockTerm:
+    min_d: float
+    max_d: float
+    cos_max: float
+    weight: float
+    mask: int = 0
+
+
+@dataclass
+class BoidsParams:
+    """Flocking composition of plugins/Boids.cpp:136-178: separation r=5,
+    cos -0.707, w 12; alignment r=7.5, cos 0.7, w 8; cohesion r=9, cos -0.15,
+    w 8.  min_d < 0 selects SteerLibrary's 3 x radius minimum distance."""
+
+    separation: FlockTerm = None
+    alignment: FlockTerm = None
+    cohesion: FlockTerm = None
+    dt: float = 1.0 / 60.0
+    k: int = 7
+    wrap_shape: int = 1  # GS_WRAP_SPHERE
+    wrap_center: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    wrap_size: Tuple[float, float, float] = (50.0, 0.0, 0.0)
+    grid_origin: Tuple[float, float, float] = (-55.0, -55.0, -55.0)
+    grid_size: Tuple[float, float, float] = (110.0, 110.0, 110.0)
+    grid_cells: Tuple[int, int, int] = (16, 16, 16)
+    mode: int = 0  # GS_MODE_OPENSTEER
+    search_radius: int = 1
+
+    def __post_init__(self):
+        if self.separation is None:
+            self.separation = FlockTerm(-1.0, 5.0, -0.707, 12.0)
+        if self.alignment is None:
+            self.alignment = FlockTerm(-1.0, 7.5, 0.7, 8.0)
+        if self.cohesion is None:
+            self.cohesion = FlockTerm(-1.0, 9.0, -0.15, 8.0)
+
+
+def boids_sphere(n: int = 4096, radius: float = 50.0, workload: int = 0) -> Tuple[AgentState, BoidsParams]:
+    """Positions uniform in the ball (rejection sampling), forward uniform on
+    the sphere, speed U(0, 9); radius 0.5, maxSpeed 9, maxForce 27, mass 1
+    (plugins/Boids.cpp:95-101, include/OpenSteer/SimpleVehicle.h:107-110)."""
+    g = rng(workload)
+    pts = []
+    count = 0
+    while count < n:
+        c = g.uniform(-1.0, 1.0, size=(2 * n, 3))
+        c = c[np.einsum("ij,ij->i", c, c) < 1.0]
+        pts.append(c)
+        count += c.shape[0]
+    pos = (np.concatenate(pts)[:n] * radius).astype(np.float32)
+    fwd = unit_vectors(g, n)
+    speed = g.uniform(0.0, 9.0, size=n).astype(np.float32)
+    st = make_agents(pos, fwd, speed, 0.5, 9.0, 27.0)
+    margin = radius * 1.1
+    p = BoidsParams(wrap_shape=1, wrap_center=(0.0, 0.0, 0.0), wrap_size=(radius, 0.0, 0.0),
+                    grid_origin=(-margin,) * 3, grid_size=(2 * margin,) * 3, grid_cells=(16, 16, 16))
+    return st, p
+
+
+def boids_box(n: int = 1 << 20, size: float = 1000.0, k: int = 7, cells: int = 64,
+              workload: int = 2) -> Tuple[AgentState, BoidsParams]:
+    """Positions uniform in [0, size)^3 with face wrap, velocities as in
+    boids_sphere."""
+    g = rng(workload)
+    pos = g.uniform(0.0, size, size=(n, 3)).astype(np.float32)
+    fwd = unit_vectors(g, n)
+    speed = g.uniform(0.0, 9.0, size=n).astype(np.float32)
+    st = make_agents(pos, fwd, speed, 0.5, 9.0, 27.0)
+    h = size / 2
+    p = BoidsParams(k=k, wrap_shape=0, wrap_center=(h, h, h), wrap_size=(size, size, size),
+                    grid_origin=(0.0, 0.0, 0.0), grid_size=(size, size, size), grid_cells=(cells, cells, cells))
+    return st, p
+
+
+# ---------------------------------------------------------------------------
+# Choke point.
+
+
+@dataclass
+class ChokeScenario:
+    groups: Tuple[AgentState, AgentState]
+    goals: Tuple[Tuple[float, float, float], Tuple[float, float, float]]
+    walls: WallState
+    obstacles: Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]  # cx, cy, cz, radius
+    grid_origin: Tuple[float, float, float]
+    grid_size: Tuple[float, float, float]
+    grid_cells: Tuple[int, int, int]
+    k: int = 7
+    kno: int = 2
+    knw: int = 3
+    dt: float = 0.05
+    min_time_walls: float = 10.0
+    min_time_obstacles: float = 5.0
+    sep: FlockTerm = None
+    w_seek: float = 1.5
+    mask_seek: int = 4480  # ANTI_PENETRATION_WALL | AVOID_WALLS | AVOID_OBSTACLES (ChokePoint.params)
+    w_walls: float = 1.0
+    w_obstacles: float = 1.0
+    mode: int = 0
+
+
+def choke_walls(gap: float, half_len: float = 100.0, thickness: float = 0.5) -> WallState:
+    """A wall along x at z = 0 of the given thickness with one gap of full
+    width `gap` centred at x = 0, as one-sided segments in the .map layout of
+    data/ChokePoint15.map (start, end, normal): the +z face (normal +z), the
+    -z face (normal -z) and the end cap at the gap (normal toward the gap),
+    for each side.  (The reference's ChokePointN.map gap spans x in [-N, N];
+    here the number is the full width.)"""
+    h = gap / 2.0
+    t = thickness / 2.0
+    segs = [
+        # left piece x in [-half_len, -h]
+        ((-half_len, 0, t), (-h, 0, t), (0, 0, 1)),
+        ((-half_len, 0, -t), (-h, 0, -t), (0, 0, -1)),
+        ((-h, 0, -t), (-h, 0, t), (1, 0, 0)),
+        # right piece x in [h, half_len]
+        ((h, 0, t), (half_len, 0, t), (0, 0, 1)),
+        ((h, 0, -t), (half_len, 0, -t), (0, 0, -1)),
+        ((h, 0, t), (h, 0, -t), (-1, 0, 0)),
+    ]
+    arr = np.array([[*s, *e, *nrm] for s, e, nrm in segs], dtype=np.float32)
+    return WallState({f: np.ascontiguousarray(arr[:, j]) for j, f in enumerate(
+        ("sx", "sy", "sz", "ex", "ey", "ez", "nx", "ny", "nz"))})
+
+
+def choke(n_total: int = 10000, gap: float = 15.0, n_obstacles: int = 50, workload: int = 1,
+          region: float = 40.0, offset: float = 10.0) -> ChokeScenario:
+    """2-D crowd on the XZ plane (y = 0).  Group 0 spawns uniform in
+    x in [-20, 20], z in [offset, offset + 40] and seeks the centre of group
+    1's region; group 1 mirrors it at negative z.  Agents: radius 0.5,
+    maxSpeed 3, maxForce 8, speed 0, random heading on the plane.  Lattice:
+    2-D, cell size 5."""
+    g = rng(workload)
+    n0 = n_total // 2
+    groups = []
+    goals = []
+    for side in (+1, -1):
+        x = g.uniform(-region / 2, region / 2, size=n0)
+        z = side * g.uniform(offset, offset + region, size=n0)
+        pos = np.stack([x, np.zeros(n0), z], axis=1).astype(np.float32)
+        ang = g.uniform(0.0, 2 * np.pi, size=n0)
+        fwd = np.stack([np.cos(ang), np.zeros(n0), np.sin(ang)], axis=1)
+        groups.append(make_agents(pos, fwd, np.zeros(n0, np.float32), 0.5, 3.0, 8.0))
+        goals.append((0.0, 0.0, float(-side * (offset + region / 2))))
+    walls = choke_walls(gap)
+    # Obstacles uniform in the play area, outside the gap corridor and the
+    # wall band.
+    ox, oz, orad = [], [], []
+    while len(ox) < n_obstacles:
+        r = float(g.uniform(0.5, 2.0))
+        x = float(g.uniform(-50.0, 50.0))
+        z = float(g.uniform(-offset - region, offset + region))
+        if abs(x) < gap / 2 + r + 1.0 or abs(z) < r + 1.0:
+            continue
+        ox.append(x)
+        oz.append(z)
+        orad.append(r)
+    obstacles = (np.array(ox, np.float32), np.zeros(n_obstacles, np.float32), np.array(oz, np.float32),
+                 np.array(orad, np.float32))
+    extent = 200.0
+    return ChokeScenario(groups=(groups[0], groups[1]), goals=(goals[0], goals[1]), walls=walls,
+                         obstacles=obstacles, grid_origin=(-extent / 2, -5.0, -extent / 2),
+                         grid_size=(extent, 10.0, extent), grid_cells=(int(extent / 5), 1, int(extent / 5)),
+                         sep=FlockTerm(-1.0, 3.0, -0.707, 10.0, 4096))

@@ -1,0 +1,237 @@
+/*
+ * gpusteer.h — C-ABI of the MI355X (gfx950) implementation of GPUSteer's
+ * per-step simulation path: spatial-bin construction, exact binned and
+ * brute-force k-nearest-neighbour search, the SteerLibrary forces and the
+ * SimpleVehicle integrate step.
+ *
+ * Plain C types only.  Every entry point returns an int status (GS_OK == 0),
+ * never aborts, and queues its device work on the stream of the context the
+ * handle belongs to.  There is no global state: handles of different contexts
+ * may be used from different threads.  A single handle must not be used by
+ * two threads at once.
+ *
+ * Reference interface each entry point replaces (paths relative to the
+ * reference repository; "GSL" = include/OpenSteer/CUDA/GroupSteerLibrary.cuh):
+ *   gs_update_knn_database   GSL:129  updateKNNDatabase(BaseGroup*, KNNBinData*)
+ *   gs_find_knn              GSL:138  findKNearestNeighbors(A, KNNData*, KNNBinData*, B, searchRadius)
+ *   gs_find_knn_brute        KNNBruteForceCUDAV2::run (CUDA/KNNBruteForceCUDA.cu:163-187)
+ *   gs_steer_separation      GSL:161  steerForSeparation(A, KNNData*, B, minD, maxD, cosA, w, mask)
+ *   gs_steer_alignment       GSL:170  steerForAlignment(...)
+ *   gs_steer_cohesion        GSL:179  steerForCohesion(...)
+ *   gs_steer_avoid_obstacles GSL:102  steerToAvoidObstacles(A, ObstacleGroup*, KNNData*, minTime, w, mask)
+ *   gs_steer_avoid_walls     GSL:120  steerToAvoidWalls(A, KNNData*, WallGroup*, minTime, w, mask)
+ *   gs_steer_seek            GSL:52   steerForSeek(A, float3 target, w, mask)
+ *   gs_update_group          GSL:189  updateGroup(A, elapsedTime)
+ *   gs_wrap_world            GSL:43   wrapWorld(A, float3 worldSize)
+ *   gs_clear_bits            plugins/ChokePointNew.cpp:595 (cudaMemset of pdAppliedKernels)
+ *   gs_group_*               include/OpenSteer/AgentGroup.h:32-54, ObstacleGroup.h:23-37,
+ *                            WallGroup.h:18-33 (AddAgent/SyncDevice/GetDataForAgent/pd*)
+ *   gs_knn_*                 include/OpenSteer/CUDA/KNNData.cuh:31-54
+ *   gs_bins_*                include/OpenSteer/CUDA/KNNBinDataV2.cuh:43-61
+ *   gs_step_boids            the per-frame sequence of plugins/BoidsGroup.cpp:606-637
+ *                            (wrap, DB, KNN, separation/alignment/cohesion, update)
+ *                            fused into one call; no single reference counterpart.
+ */
+#ifndef GPUSTEER_H
+#define GPUSTEER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_VERSION 100
+
+/* ---- status codes ------------------------------------------------------- */
+enum {
+    GS_OK = 0,
+    GS_ERR_INVALID_ARGUMENT = 1, /* bad value (size 0 where not allowed, NULL pointer, bad enum) */
+    GS_ERR_INVALID_HANDLE = 2,   /* NULL / destroyed / wrong-kind handle */
+    GS_ERR_SIZE_MISMATCH = 3,    /* e.g. KNN container sized for another group */
+    GS_ERR_UNSUPPORTED = 4,      /* e.g. k > GS_MAX_K, too many cells */
+    GS_ERR_NOT_BINNED = 5,       /* KNN on a group without a current bin database */
+    GS_ERR_OUT_OF_MEMORY = 6,    /* device allocation failed */
+    GS_ERR_DEVICE = 7,           /* HIP runtime error (message in gs_last_error) */
+    GS_ERR_NONFINITE = 8         /* gs_group_check_finite found NaN/Inf */
+};
+
+#define GS_MAX_K 32u
+
+/* Force / integrate semantics.
+ * GS_MODE_OPENSTEER: the CPU OpenSteer formulas (src/SteerLibrary.cpp,
+ *   src/SimpleVehicle.cpp): squared-distance neighbourhood tests with a
+ *   minimum distance of 3 x radius when min_d < 0, unnormalized seek,
+ *   applySteeringForce with steering adjustment and acceleration smoothing.
+ * GS_MODE_GPUREF: the reference CUDA kernels (FlockingCommon.cuh,
+ *   SteerForSeekCUDAKernel.cu, UpdateCUDAKernel.cu). */
+enum { GS_MODE_OPENSTEER = 0, GS_MODE_GPUREF = 1 };
+
+enum { GS_KIND_AGENTS = 0, GS_KIND_OBSTACLES = 1, GS_KIND_WALLS = 2 };
+
+enum { GS_WRAP_BOX = 0, GS_WRAP_SPHERE = 1 };
+
+/* gs_find_knn flags */
+enum {
+    GS_KNN_DEFAULT = 0,
+    /* Do not use the container's previous (exact) result as a search bound. */
+    GS_KNN_NO_SEED = 1u << 0
+};
+
+/* Behaviour bits, include/OpenSteer/CUDA/CUDAKernelGlobals.cuh:12-31. */
+#define GS_BIT_PURSUE (1u << 0)
+#define GS_BIT_EVADE (1u << 1)
+#define GS_BIT_SEEK (1u << 2)
+#define GS_BIT_FLEE (1u << 3)
+#define GS_BIT_SEPARATION (1u << 4)
+#define GS_BIT_ALIGNMENT (1u << 5)
+#define GS_BIT_COHESION (1u << 6)
+#define GS_BIT_AVOID_OBSTACLES (1u << 7)
+#define GS_BIT_AVOID_WALLS (1u << 8)
+#define GS_BIT_AVOID_NEIGHBORS (1u << 9)
+#define GS_BIT_AVOID_CLOSE_NEIGHBORS (1u << 10)
+#define GS_BIT_FOLLOW_PATH (1u << 11)
+#define GS_BIT_ANTI_PENETRATION_WALL (1u << 12)
+#define GS_BIT_ANTI_PENETRATION_AGENT (1u << 13)
+
+/* ---- handles -------------------------------------------------------------- */
+typedef struct gs_context_s* gs_context;
+typedef struct gs_bins_s* gs_bins;
+typedef struct gs_group_s* gs_group;
+typedef struct gs_knn_s* gs_knn;
+
+/* Agent state, struct of arrays (one float / uint32 per agent in each array).
+ * Used for host upload/download and to expose the device arrays.  Field
+ * meaning: include/OpenSteer/AgentData.h:13-42 plus SimpleVehicle's smoothed
+ * acceleration (include/OpenSteer/SimpleVehicle.h:335-345). */
+typedef struct {
+    float *px, *py, *pz;    /* position */
+    float *fx, *fy, *fz;    /* unit forward (direction) */
+    float *speed;
+    float *ax, *ay, *az;    /* smoothed acceleration */
+    float *sx, *sy, *sz;    /* side */
+    float *ux, *uy, *uz;    /* up */
+    float *gx, *gy, *gz;    /* steering accumulator */
+    float *radius, *max_speed, *max_force, *mass;
+    uint32_t *bits;         /* applied-behaviour bits */
+} gs_agents;
+
+/* Wall segments (include/OpenSteer/CUDA/WallGroupData.cuh), SoA. */
+typedef struct {
+    float *sx, *sy, *sz; /* start */
+    float *ex, *ey, *ez; /* end */
+    float *nx, *ny, *nz; /* normal */
+} gs_walls;
+
+/* One flocking term, the arguments of steerFor{Separation,Alignment,Cohesion}. */
+typedef struct {
+    float min_d, max_d, cos_max, weight;
+    uint32_t mask;
+} gs_flock_term;
+
+/* ---- errors ---------------------------------------------------------------- */
+const char* gs_status_string(int status);
+/* Detail of the last failure on the calling thread ("" if none). */
+const char* gs_last_error(void);
+int gs_version(void);
+
+/* ---- context ---------------------------------------------------------------- */
+/* hip_stream: a hipStream_t to queue work on, or NULL for a context-owned
+ * non-blocking stream at default priority. */
+int gs_context_create(int device, void* hip_stream, gs_context* out);
+int gs_context_destroy(gs_context ctx);
+int gs_context_synchronize(gs_context ctx);
+int gs_context_stream(gs_context ctx, void** hip_stream);
+
+/* ---- bin lattice (KNNBinData) ------------------------------------------------ */
+/* origin = minimum corner, size = extent, cells = divisions per axis. */
+int gs_bins_create(gs_context ctx, const float origin[3], const float size[3], const uint32_t cells[3], gs_bins* out);
+/* The reference's centred form, KNNBinDataV2(worldCells, worldSize, searchRadius)
+ * (CUDA/KNNBinDataV2.cu:30-42): origin = -size/2. */
+int gs_bins_create_centered(gs_context ctx, const uint32_t world_cells[3], const float world_size[3], gs_bins* out);
+int gs_bins_destroy(gs_bins bins);
+int gs_bins_geometry(gs_bins bins, float origin[3], float size[3], uint32_t cells[3]);
+
+/* ---- groups (AgentGroup / ObstacleGroup / WallGroup) ------------------------- */
+int gs_group_create(gs_context ctx, int kind, gs_group* out);
+int gs_group_destroy(gs_group g);
+int gs_group_size(gs_group g, uint32_t* n);
+/* Agents: every pointer of *host must be non-NULL (n > 0).  Synchronous. */
+int gs_group_upload_agents(gs_group g, uint32_t n, const gs_agents* host);
+int gs_group_download_agents(gs_group g, gs_agents* host);
+/* Device pointers of the agent arrays (valid until the next upload). */
+int gs_group_device_agents(gs_group g, gs_agents* dev);
+/* Obstacles: centre and radius per obstacle (ObstacleGroup::AddObstacle). */
+int gs_group_upload_obstacles(gs_group g, uint32_t n, const float* cx, const float* cy, const float* cz,
+                              const float* radius);
+/* Walls: segments; the KNN position of a wall is its midpoint (WallGroup.h:31). */
+int gs_group_upload_walls(gs_group g, uint32_t n, const gs_walls* host);
+/* Copy the bin database of a group (after gs_update_knn_database): any output
+ * may be NULL.  key[n] cell per agent; start[C+1] first sorted slot per cell;
+ * perm[n] agent index per sorted slot (ascending within a cell);
+ * sorted positions spx/spy/spz[n]. */
+int gs_group_download_bins(gs_group g, uint32_t* key, uint32_t* start, uint32_t* perm, float* spx, float* spy,
+                           float* spz);
+/* Scan the agent state for NaN/Inf: GS_NONFINITE if any, *first_bad = agent index. */
+int gs_group_check_finite(gs_group g, uint32_t* first_bad);
+
+/* ---- KNN container (KNNData) ----------------------------------------------------- */
+int gs_knn_create(gs_context ctx, uint32_t n, uint32_t k, gs_knn* out);
+int gs_knn_destroy(gs_knn knn);
+int gs_knn_shape(gs_knn knn, uint32_t* n, uint32_t* k);
+/* Slot-major [k][n]: idx[j*n + i] is the j-th nearest of agent i;
+ * pads are UINT32_MAX / FLT_MAX.  Either output may be NULL. */
+int gs_knn_download(gs_knn knn, uint32_t* idx, float* dist);
+int gs_knn_device(gs_knn knn, uint32_t** idx, float** dist);
+
+/* ---- operator API (GroupSteerLibrary.cuh) ---------------------------------------- */
+int gs_update_knn_database(gs_group g, gs_bins bins);
+/* Exact k nearest of B for each agent of A, key (d^2, index).  A and B must
+ * both have a current bin database built with `bins`.  search_radius is the
+ * cell radius scanned before the first exactness test (the result does not
+ * depend on it).  Unless GS_KNN_NO_SEED, a previous exact result held by `out`
+ * for the same A, B and memberships bounds the search (result unchanged). */
+int gs_find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t search_radius, uint32_t flags);
+int gs_find_knn_brute(gs_group A, gs_knn out, gs_group B);
+int gs_steer_separation(gs_group A, gs_knn knn, gs_group B, float min_d, float max_d, float cos_max, float weight,
+                        uint32_t mask, int mode);
+int gs_steer_alignment(gs_group A, gs_knn knn, gs_group B, float min_d, float max_d, float cos_max, float weight,
+                       uint32_t mask, int mode);
+int gs_steer_cohesion(gs_group A, gs_knn knn, gs_group B, float min_d, float max_d, float cos_max, float weight,
+                      uint32_t mask, int mode);
+/* The three flocking terms in one pass, applied in the order separation,
+ * alignment, cohesion (bit-identical to the three calls in that order). */
+int gs_steer_flocking(gs_group A, gs_knn knn, gs_group B, const gs_flock_term* separation,
+                      const gs_flock_term* alignment, const gs_flock_term* cohesion, int mode);
+int gs_steer_seek(gs_group A, const float target[3], float weight, uint32_t mask, int mode);
+int gs_steer_avoid_obstacles(gs_group A, gs_group obstacles, gs_knn knn, float min_time_to_collision, float weight,
+                             uint32_t mask);
+int gs_steer_avoid_walls(gs_group A, gs_knn knn, gs_group walls, float min_time_to_collision, float weight,
+                         uint32_t mask);
+int gs_update_group(gs_group A, float elapsed_time, int mode);
+/* GS_WRAP_BOX: size_or_radius = box size (centred at center);
+ * GS_WRAP_SPHERE: size_or_radius[0] = radius. */
+int gs_wrap_world(gs_group A, int shape, const float center[3], const float size_or_radius[3]);
+int gs_clear_bits(gs_group A);
+
+/* ---- fused Boids step ------------------------------------------------------------ */
+typedef struct {
+    int wrap_shape;               /* GS_WRAP_BOX / GS_WRAP_SPHERE, or -1 for none */
+    float wrap_center[3];
+    float wrap_size[3];
+    uint32_t search_radius;
+    uint32_t knn_flags;
+    gs_flock_term separation, alignment, cohesion;
+    float dt;
+    int mode;
+} gs_boids_params;
+/* `steps` times: wrap -> bin database -> exact KNN (A with itself) ->
+ * separation, alignment, cohesion -> integrate.  Same results as the separate
+ * calls in that order. */
+int gs_step_boids(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* params, uint32_t steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUSTEER_H */

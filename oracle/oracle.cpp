@@ -1,0 +1,740 @@
+// CPU oracle for the GPUSteer per-step simulation path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is a plain C++ restatement of the
+// reference's CPU path (OpenSteer lq.c binning, brute-force proximity,
+// SteerLibrary forces, SimpleVehicle integration) and of the GPU-only kernel
+// semantics that have no CPU counterpart (wall and obstacle avoidance, the
+// GPU-reference update, the face wrap).  It exists so that tests/, the smoke
+// check and bench.py's cpu_baseline leg can check and time the HIP path.  The
+// product (libgpusteer.so) never links, loads or calls it.
+//
+// Every routine names the reference file:line it restates.  Arithmetic is
+// written operation-for-operation in the reference's evaluation order and the
+// file is compiled with -ffp-contract=off, so that every fp32 result is the
+// IEEE result of the same sequence of correctly rounded operations the GPU
+// kernels perform.
+//
+// Pins (see tests/test_oracle_*.py): closed forms for each force, brute force
+// versus binned KNN on constructed ties, the real lq.c (compiled from the
+// reference sources into oracle/_ref) for cell membership and radius queries,
+// std::pow for the x^20 restatement, exact antipode / face mappings.
+//
+// Templated on the scalar type so that the same restatement runs in float
+// (parity and CPU baseline) and double (drift bound and gradient checks).
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Vector algebra: include/OpenSteer/VectorUtils.cuh:35-189.
+template <class T> struct V3 { T x, y, z; };
+template <class T> static inline V3<T> mk(T x, T y, T z) { return V3<T>{x, y, z}; }
+template <class T> static inline V3<T> add(V3<T> a, V3<T> b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class T> static inline V3<T> sub(V3<T> a, V3<T> b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class T> static inline V3<T> neg(V3<T> a) { return mk(-a.x, -a.y, -a.z); }
+template <class T> static inline V3<T> mul(V3<T> a, T s) { return mk(a.x * s, a.y * s, a.z * s); }
+template <class T> static inline V3<T> dvd(V3<T> a, T s) { return mk(a.x / s, a.y / s, a.z / s); }
+// VectorUtils.cuh:86-89: (x*x' + y*y') + z*z', left to right.
+template <class T> static inline T dot(V3<T> a, V3<T> b) { return ((a.x * b.x) + (a.y * b.y)) + (a.z * b.z); }
+template <class T> static inline T len(V3<T> a) { return std::sqrt(dot(a, a)); }
+// VectorUtils.cuh:115-120: normalize skips the divide when the length is 0.
+template <class T> static inline V3<T> normalize(V3<T> a) {
+    const T l = len(a);
+    return (l > T(0)) ? dvd(a, l) : a;
+}
+// VectorUtils.cuh:131-136.
+template <class T> static inline V3<T> cross(V3<T> a, V3<T> b) {
+    return mk((a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x));
+}
+template <class T> static inline bool is_zero(V3<T> a) { return a.x == T(0) && a.y == T(0) && a.z == T(0); }
+template <class T> static inline T distance(V3<T> a, V3<T> b) { return len(sub(a, b)); }
+// VectorUtils.cuh:163-175.
+template <class T> static inline V3<T> parallel_component(V3<T> v, V3<T> unit) { return mul(unit, dot(v, unit)); }
+template <class T> static inline V3<T> perpendicular_component(V3<T> v, V3<T> unit) {
+    return sub(v, parallel_component(v, unit));
+}
+// VectorUtils.cuh:181-189.
+template <class T> static inline V3<T> truncate_length(V3<T> v, T max_len) {
+    const T max2 = max_len * max_len;
+    const T l2 = dot(v, v);
+    if (l2 <= max2) return v;
+    return mul(v, max_len / std::sqrt(l2));
+}
+// VectorUtils.cuh:305-344 (insideOrOutside == true: limitMaxDeviationAngle).
+template <class T> static inline V3<T> limit_max_deviation_angle(V3<T> source, T cos_cone, V3<T> basis) {
+    const T source_len = len(source);
+    if (source_len == T(0)) return source;
+    const V3<T> direction = dvd(source, source_len);
+    const T cos_source = dot(direction, basis);
+    if (cos_source >= cos_cone) return source;
+    const V3<T> perp = perpendicular_component(source, basis);
+    const V3<T> unit_perp = normalize(perp);
+    const T perp_dist = std::sqrt(T(1) - (cos_cone * cos_cone));
+    const V3<T> c0 = mul(basis, cos_cone);
+    const V3<T> c1 = mul(unit_perp, perp_dist);
+    return mul(add(c0, c1), source_len);
+}
+// Utilities.h:97 clip.
+template <class T> static inline T clip(T x, T lo, T hi) {
+    if (x < lo) return lo;
+    if (x > hi) return hi;
+    return x;
+}
+// Utilities.h:203-210 blendIntoAccumulator with VectorUtils.cuh:275-278 interpolate.
+template <class T> static inline V3<T> blend(T rate, V3<T> new_value, V3<T> acc) {
+    const T a = clip(rate, T(0), T(1));
+    return add(acc, mul(sub(new_value, acc), a));
+}
+// pow(range, 20) of SimpleVehicle.cpp:129 (a float argument promoted to double
+// by C++ overloading, the result narrowed back to float by interpolate's alpha
+// parameter).  Restated as double-precision repeated squaring so that CPU and
+// GPU produce the same bits: r^20 = ((r^2)^2 * r)^2)^2.  Pinned against
+// std::pow in tests/test_oracle_math.py.
+template <class T> static inline T pow20(T range) {
+    const double r = double(range);
+    const double r2 = r * r;
+    const double r4 = r2 * r2;
+    const double r5 = r4 * r;
+    const double r10 = r5 * r5;
+    return T(r10 * r10);
+}
+
+// LinesIntersect, include/OpenSteer/VectorUtils.cuh:390-428 (XZ plane).
+template <class T> static inline bool lines_intersect(V3<T> sa, V3<T> ea, V3<T> sb, V3<T> eb, V3<T>& ip) {
+    const T eps = T(0.001f);  // VectorUtils.cuh:8 EPSILON
+    const T denom = ((eb.z - sb.z) * (ea.x - sa.x)) - ((eb.x - sb.x) * (ea.z - sa.z));
+    const T numera = ((eb.x - sb.x) * (sa.z - sb.z)) - ((eb.z - sb.z) * (sa.x - sb.x));
+    const T numerb = ((ea.x - sa.x) * (sa.z - sb.z)) - ((ea.z - sa.z) * (sa.x - sb.x));
+    if (std::fabs(denom) < eps) {
+        if (std::fabs(numera) < eps && std::fabs(numerb) < eps) {
+            ip = sa;  // coincident
+            return true;
+        }
+        return false;  // parallel
+    }
+    const T ua = numera / denom;
+    const T ub = numerb / denom;
+    if (ua >= T(0) && ua <= T(1) && ub >= T(0) && ub <= T(1)) {
+        ip.x = sa.x + ua * (ea.x - sa.x);
+        ip.y = T(0);
+        ip.z = sa.z + ua * (ea.z - sa.z);
+        return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// Bin lattice: src/lq.c:185-208 (lqBinForLocation), index order :175-176
+// (ix*divy*divz + iy*divz + iz).  lq.c puts points outside the super-brick in
+// one "other" bin; here they are clamped into the boundary cell of each axis,
+// which keeps every KNN query exact because the search bound below treats a
+// boundary cell as extending to infinity outward.
+struct Grid {
+    float ox, oy, oz;  // super-brick minimum corner
+    float sx, sy, sz;  // super-brick size
+    int dx, dy, dz;    // divisions
+};
+static inline int axis_cell(float p, float o, float s, int d) {
+    if (!(p >= o)) return 0;          // below (or NaN): lower boundary cell
+    if (p >= o + s) return d - 1;     // at/after the far face: upper boundary cell
+    int i = (int)(((p - o) / s) * (float)d);  // lq.c:198-200
+    return i < d ? i : d - 1;
+}
+static inline uint32_t cell_of(const Grid& g, float x, float y, float z) {
+    const int ix = axis_cell(x, g.ox, g.sx, g.dx);
+    const int iy = axis_cell(y, g.oy, g.sy, g.dy);
+    const int iz = axis_cell(z, g.oz, g.sz, g.dz);
+    return (uint32_t)((ix * g.dy * g.dz) + (iy * g.dz) + iz);
+}
+
+// Stable counting sort of agents by cell: the reference builds the same
+// structure with BuildDB + thrust::sort_by_key + ReorderDB
+// (include/OpenSteer/CUDA/KNNBinningV2.cu:87-162).  start has C+1 entries;
+// cell c holds sorted slots [start[c], start[c+1]) in ascending agent index.
+static void bins_build(const Grid& g, int n, const float* px, const float* py, const float* pz,
+                       uint32_t* key, uint32_t* start, uint32_t* perm, float* spx, float* spy, float* spz) {
+    const int C = g.dx * g.dy * g.dz;
+    std::vector<uint32_t> count((size_t)C + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        key[i] = cell_of(g, px[i], py[i], pz[i]);
+        count[key[i]]++;
+    }
+    uint32_t run = 0;
+    for (int c = 0; c < C; ++c) {
+        start[c] = run;
+        run += count[c];
+    }
+    start[C] = run;
+    std::vector<uint32_t> cursor(start, start + C);
+    for (int i = 0; i < n; ++i) {
+        const uint32_t s = cursor[key[i]]++;
+        perm[s] = (uint32_t)i;
+        if (spx) {
+            spx[s] = px[i];
+            spy[s] = py[i];
+            spz[s] = pz[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// KNN.  Key (d2, index) ascending; d2 = (dx*dx + dy*dy) + dz*dz in fp32.
+// Brute force follows KNNBruteForceCUDAKernelV2
+// (include/OpenSteer/CUDA/KNNBruteForceCudaKernel.cu:207-283): ascending scan
+// of B with strict '<' against the k-th entry, so the lower index wins ties.
+// Output is slot-major [k][nA] (KNNBinningV2Kernel.cu:520-526), padded with
+// UINT32_MAX / FLT_MAX (:451-457); dist = sqrtf(d2).
+struct Best {
+    float d2;
+    uint32_t idx;
+};
+static inline bool key_less(float d2a, uint32_t ia, float d2b, uint32_t ib) {
+    return d2a < d2b || (d2a == d2b && ia < ib);
+}
+static inline void insert_best(Best* h, int k, float d2, uint32_t idx) {
+    if (!key_less(d2, idx, h[k - 1].d2, h[k - 1].idx)) return;
+    int j = k - 1;
+    while (j > 0 && key_less(d2, idx, h[j - 1].d2, h[j - 1].idx)) {
+        h[j] = h[j - 1];
+        --j;
+    }
+    h[j].d2 = d2;
+    h[j].idx = idx;
+}
+static inline float d2_of(float ax, float ay, float az, float bx, float by, float bz) {
+    const float dx = ax - bx, dy = ay - by, dz = az - bz;
+    return ((dx * dx) + (dy * dy)) + (dz * dz);
+}
+static inline void write_knn(const Best* h, int k, int nA, int q, uint32_t* idx, float* dist) {
+    for (int j = 0; j < k; ++j) {
+        if (h[j].idx == UINT32_MAX) {
+            idx[(size_t)j * nA + q] = UINT32_MAX;
+            dist[(size_t)j * nA + q] = FLT_MAX;
+        } else {
+            idx[(size_t)j * nA + q] = h[j].idx;
+            dist[(size_t)j * nA + q] = std::sqrt(h[j].d2);
+        }
+    }
+}
+
+static void knn_brute(int nA, const float* ax, const float* ay, const float* az, int nB, const float* bx,
+                      const float* by, const float* bz, int self, int k, uint32_t* idx, float* dist,
+                      const uint32_t* queries, int nq) {
+    // queries (optional): compute only these query indices (results still
+    // written to column q of the [k][nA] arrays).
+    const int total = queries ? nq : nA;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int t = 0; t < total; ++t) {
+        const int q = queries ? (int)queries[t] : t;
+        std::vector<Best> h(k, Best{FLT_MAX, UINT32_MAX});
+        for (int b = 0; b < nB; ++b) {
+            if (self && b == q) continue;
+            insert_best(h.data(), k, d2_of(ax[q], ay[q], az[q], bx[b], by[b], bz[b]), (uint32_t)b);
+        }
+        write_knn(h.data(), k, nA, q, idx, dist);
+    }
+}
+
+// Exact KNN over the bin lattice: scan shells of cells of growing Chebyshev
+// radius around the query's cell until the k-th best d2 is below the squared
+// distance from the query to every unscanned cell.  Replaces the reference's
+// fixed-radius scan (KNNBinningV2Kernel.cu:470-515), which is not exact; the
+// result equals knn_brute by construction and is tested to.
+static void knn_binned(const Grid& g, int nA, const float* ax, const float* ay, const float* az, int nB,
+                       const float* sbx, const float* sby, const float* sbz, const uint32_t* bperm,
+                       const uint32_t* bstart, int self, int k, uint32_t* idx, float* dist) {
+    const float hx = g.sx / (float)g.dx, hy = g.sy / (float)g.dy, hz = g.sz / (float)g.dz;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int q = 0; q < nA; ++q) {
+        std::vector<Best> h(k, Best{FLT_MAX, UINT32_MAX});
+        const float qx = ax[q], qy = ay[q], qz = az[q];
+        const int cx = axis_cell(qx, g.ox, g.sx, g.dx);
+        const int cy = axis_cell(qy, g.oy, g.sy, g.dy);
+        const int cz = axis_cell(qz, g.oz, g.sz, g.dz);
+        for (int R = 0;; ++R) {
+            for (int ix = cx - R; ix <= cx + R; ++ix) {
+                if (ix < 0 || ix >= g.dx) continue;
+                for (int iy = cy - R; iy <= cy + R; ++iy) {
+                    if (iy < 0 || iy >= g.dy) continue;
+                    const bool edge = (ix == cx - R || ix == cx + R || iy == cy - R || iy == cy + R);
+                    for (int iz = cz - R; iz <= cz + R; iz += edge ? 1 : (R > 0 ? 2 * R : 1)) {
+                        if (iz < 0 || iz >= g.dz) continue;
+                        const uint32_t c = (uint32_t)((ix * g.dy * g.dz) + (iy * g.dz) + iz);
+                        for (uint32_t s = bstart[c]; s < bstart[c + 1]; ++s) {
+                            const uint32_t b = bperm[s];
+                            if (self && (int)b == q) continue;
+                            insert_best(h.data(), k, d2_of(qx, qy, qz, sbx[s], sby[s], sbz[s]), b);
+                        }
+                    }
+                }
+            }
+            // Distance from the query to the nearest unscanned cell (faces of
+            // the scanned box that are not grid boundaries).
+            float dmin = FLT_MAX;
+            bool all = true;
+            if (cx - R > 0) { dmin = std::min(dmin, qx - (g.ox + (float)(cx - R) * hx)); all = false; }
+            if (cx + R < g.dx - 1) { dmin = std::min(dmin, (g.ox + (float)(cx + R + 1) * hx) - qx); all = false; }
+            if (cy - R > 0) { dmin = std::min(dmin, qy - (g.oy + (float)(cy - R) * hy)); all = false; }
+            if (cy + R < g.dy - 1) { dmin = std::min(dmin, (g.oy + (float)(cy + R + 1) * hy) - qy); all = false; }
+            if (cz - R > 0) { dmin = std::min(dmin, qz - (g.oz + (float)(cz - R) * hz)); all = false; }
+            if (cz + R < g.dz - 1) { dmin = std::min(dmin, (g.oz + (float)(cz + R + 1) * hz) - qz); all = false; }
+            if (all) break;
+            // Conservative slack absorbs the rounding of the cell mapping.
+            dmin -= 1e-5f * (std::fabs(g.ox) + std::fabs(g.oy) + std::fabs(g.oz) + g.sx + g.sy + g.sz);
+            if (dmin > 0.f && h[k - 1].d2 < dmin * dmin * (1.f - 1e-5f)) break;
+        }
+        write_knn(h.data(), k, nA, q, idx, dist);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Agent state, struct-of-arrays.  Field meaning follows AgentData
+// (include/OpenSteer/AgentData.h:13-42) plus SimpleVehicle's smoothed
+// acceleration (include/OpenSteer/SimpleVehicle.h:335-345).
+template <class T> struct Agents {
+    T *px, *py, *pz;    // position
+    T *fx, *fy, *fz;    // unit forward (direction)
+    T *speed;
+    T *ax, *ay, *az;    // smoothed acceleration
+    T *sx, *sy, *sz;    // side
+    T *ux, *uy, *uz;    // up
+    T *gx, *gy, *gz;    // steering accumulator
+    T *radius, *max_speed, *max_force, *mass;
+    uint32_t* bits;     // applied-behaviour bits (CUDAKernelGlobals.cuh:11-31)
+};
+template <class T> static inline V3<T> P(const Agents<T>& a, int i) { return mk(a.px[i], a.py[i], a.pz[i]); }
+template <class T> static inline V3<T> F(const Agents<T>& a, int i) { return mk(a.fx[i], a.fy[i], a.fz[i]); }
+template <class T> static inline V3<T> S(const Agents<T>& a, int i) { return mk(a.sx[i], a.sy[i], a.sz[i]); }
+template <class T> static inline V3<T> U(const Agents<T>& a, int i) { return mk(a.ux[i], a.uy[i], a.uz[i]); }
+template <class T> static inline V3<T> G(const Agents<T>& a, int i) { return mk(a.gx[i], a.gy[i], a.gz[i]); }
+template <class T> static inline void setG(Agents<T>& a, int i, V3<T> v) { a.gx[i] = v.x; a.gy[i] = v.y; a.gz[i] = v.z; }
+
+enum { MODE_OPENSTEER = 0, MODE_GPUREF = 1 };
+enum {
+    BIT_SEEK = 1u << 2,
+    BIT_SEPARATION = 1u << 4,
+    BIT_ALIGNMENT = 1u << 5,
+    BIT_COHESION = 1u << 6,
+    BIT_AVOID_OBSTACLES = 1u << 7,
+    BIT_AVOID_WALLS = 1u << 8,
+    BIT_ANTI_PENETRATION_WALL = 1u << 12,
+};
+
+// Applies a weighted behaviour result with the reference kernels' mask / bit /
+// accumulate convention (e.g. SteerForSeparationCUDAKernel.cu:74-75,124-134):
+// steering = (f*w) + steering, bit set when f*w is non-zero.
+template <class T> static inline void accumulate(Agents<T>& a, int i, V3<T> f, T w, uint32_t bit) {
+    const V3<T> fw = mul(f, w);
+    if (!is_zero(fw)) a.bits[i] |= bit;
+    setG(a, i, add(fw, G(a, i)));
+}
+
+// Boid neighbourhood predicate.
+//  OPENSTEER: SteerLibrary::inBoidNeighborhood (src/SteerLibrary.cpp:506-544),
+//             squared-distance tests; min distance = 3*radius when min_d < 0
+//             (the value SteerLibrary passes, :566,607,646).
+//  GPUREF:    FlockingCommon.cuh:4-30, sqrt distance tests, explicit min_d.
+template <class T>
+static inline bool in_neighborhood(int mode, V3<T> pos, V3<T> fwd, T radius, V3<T> other, T min_d, T max_d, T cos_max) {
+    const V3<T> offset = sub(other, pos);
+    if (mode == MODE_GPUREF) {
+        const T d = len(offset);
+        if (d < min_d) return true;
+        if (d > max_d) return false;
+        return dot(fwd, dvd(offset, d)) > cos_max;
+    }
+    const T md = (min_d < T(0)) ? radius * T(3) : min_d;
+    const T d2 = dot(offset, offset);
+    if (d2 < (md * md)) return true;
+    if (d2 > (max_d * max_d)) return false;
+    const V3<T> unit = dvd(offset, std::sqrt(d2));
+    return dot(fwd, unit) > cos_max;
+}
+
+enum { FLOCK_SEPARATION = 0, FLOCK_ALIGNMENT = 1, FLOCK_COHESION = 2 };
+// Separation (src/SteerLibrary.cpp:552-585; SteerForSeparationCUDAKernel.cu:98-121),
+// alignment (:593-623; SteerForAlignmentCUDAKernel.cu), cohesion (:632-662;
+// SteerForCohesionCUDAKernel.cu).  Neighbours are the KNN slots in slot order,
+// stopping at the first pad (SteerForSeparationCUDAKernel.cu:104).
+// Reads the step-t snapshot of B (Jacobi), writes A's steering and bits.
+template <class T>
+static void steer_flock_one(int which, int mode, Agents<T>& A, int nA, const uint32_t* knn, int k, const Agents<T>& B,
+                            int nB, T min_d, T max_d, T cos_max, T w, uint32_t mask) {
+    const uint32_t bit = which == FLOCK_SEPARATION ? BIT_SEPARATION : which == FLOCK_ALIGNMENT ? BIT_ALIGNMENT : BIT_COHESION;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nA; ++i) {
+        if (A.bits[i] & mask) continue;
+        const V3<T> pos = P(A, i), fwd = F(A, i);
+        V3<T> steering = mk(T(0), T(0), T(0));
+        int count = 0;
+        for (int j = 0; j < k; ++j) {
+            const uint32_t b = knn[(size_t)j * nA + i];
+            if (b >= (uint32_t)nB) break;
+            const V3<T> bp = P(B, (int)b);
+            if (!in_neighborhood(mode, pos, fwd, A.radius[i], bp, min_d, max_d, cos_max)) continue;
+            if (which == FLOCK_SEPARATION) {
+                const V3<T> offset = sub(bp, pos);
+                const T d2 = dot(offset, offset);
+                steering = add(steering, dvd(offset, -d2));
+            } else if (which == FLOCK_ALIGNMENT) {
+                steering = add(steering, F(B, (int)b));
+            } else {
+                steering = add(steering, bp);
+            }
+            ++count;
+        }
+        if (count > 0) {
+            const V3<T> mean = dvd(steering, (T)count);
+            if (which == FLOCK_SEPARATION) steering = normalize(mean);
+            else if (which == FLOCK_ALIGNMENT) steering = normalize(sub(mean, fwd));
+            else steering = normalize(sub(mean, pos));
+        }
+        accumulate(A, i, steering, w, bit);
+    }
+}
+
+// Seek.  OPENSTEER: src/SteerLibrary.cpp:36-42 (target - position - velocity).
+// GPUREF: SteerForSeekCUDAKernel.cu:52-61 (normalize(target - position - forward)).
+template <class T>
+static void steer_seek(int mode, Agents<T>& A, int nA, const T* tx, const T* ty, const T* tz, int per_agent, T w,
+                       uint32_t mask) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nA; ++i) {
+        if (A.bits[i] & mask) continue;
+        const int t = per_agent ? i : 0;
+        const V3<T> desired = sub(mk(tx[t], ty[t], tz[t]), P(A, i));
+        V3<T> f;
+        if (mode == MODE_GPUREF) f = normalize(sub(desired, F(A, i)));
+        else f = sub(desired, mul(F(A, i), A.speed[i]));
+        accumulate(A, i, f, w, BIT_SEEK);
+    }
+}
+
+// Obstacle avoidance: SteerToAvoidObstaclesKernel and its
+// findNextIntersectionWithSphere (include/OpenSteer/CUDA/AvoidObstaclesCUDAKernel.cu:47-99,136-253).
+// (The CPU group version, src/SteerLibrary.cpp:222-228, returns zero.)
+template <class T>
+static void avoid_obstacles(Agents<T>& A, int nA, const uint32_t* knn, int k, int nO, const T* ox, const T* oy,
+                            const T* oz, const T* orad, T min_time, T w, uint32_t mask) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nA; ++i) {
+        if (A.bits[i] & mask) continue;
+        const V3<T> pos = P(A, i), side = S(A, i), up = U(A, i), fwd = F(A, i);
+        const T radius = A.radius[i];
+        const T min_dist = min_time * A.speed[i];
+        bool n_hit = false, x_hit = false;
+        T n_dist = T(0), x_dist = T(0);
+        V3<T> n_pos = mk(T(0), T(0), T(0)), x_pos = n_pos;
+        for (int j = 0; j < k; ++j) {
+            const uint32_t o = knn[(size_t)j * nA + i];
+            if (o >= (uint32_t)nO) break;
+            const V3<T> op = mk(ox[o], oy[o], oz[o]);
+            const T orr = orad[o];
+            // findNextIntersectionWithSphere (:47-99)
+            x_hit = false;
+            x_pos = op;
+            const V3<T> go = sub(op, pos);
+            const V3<T> lc = mk(dot(go, side), dot(go, up), dot(go, fwd));
+            const T b = T(-2) * lc.z;
+            const T c = ((lc.x * lc.x + lc.y * lc.y) + lc.z * lc.z) - (orr + radius) * (orr + radius);
+            const T d = (b * b) - (T(4) * c);
+            if (!(d < T(0))) {
+                const T s = std::sqrt(d);
+                const T p = (-b + s) / T(2);
+                const T q = (-b - s) / T(2);
+                if (!((p < T(0)) && (q < T(0)))) {
+                    x_hit = true;
+                    x_dist = ((p > T(0)) && (q > T(0))) ? ((p < q) ? p : q) : ((p > T(0)) ? p : q);
+                }
+            }
+            // :220-221 nearest selection
+            if ((n_hit == false) || ((x_hit != false) && (x_dist < n_dist))) {
+                n_hit = x_hit;
+                n_dist = x_dist;
+                n_pos = x_pos;
+            }
+        }
+        V3<T> steering = mk(T(0), T(0), T(0));
+        if (n_hit && (n_dist < min_dist)) steering = perpendicular_component(sub(pos, n_pos), fwd);
+        accumulate(A, i, steering, w, BIT_AVOID_OBSTACLES);
+    }
+}
+
+// Wall avoidance: SteerToAvoidWallsCUDAKernel
+// (include/OpenSteer/CUDA/AvoidWallsCUDAKernel.cu:67-245), including its side
+// effect of moving agents that touch a wall (:215-223) and its bit rules.
+template <class T>
+static void avoid_walls(Agents<T>& A, int nA, const uint32_t* knn, int k, int nW, const T* wsx, const T* wsy,
+                        const T* wsz, const T* wex, const T* wey, const T* wez, const T* wnx, const T* wny,
+                        const T* wnz, T min_time, T w, uint32_t mask) {
+    const T feeler_len = T(4);  // :41 FEELER_LENGTH
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nA; ++i) {
+        if (A.bits[i] & mask) continue;
+        V3<T> pos = P(A, i), dir = F(A, i);
+        const V3<T> side = S(A, i);
+        const T radius = A.radius[i];
+        const V3<T> vel = mul(dir, A.speed[i]);
+        V3<T> feelers[3];
+        feelers[0] = add(pos, mul(vel, min_time));
+        feelers[1] = add(pos, add(mul(side, feeler_len * radius), mul(dir, feeler_len * radius)));
+        feelers[2] = add(pos, add(mul(neg(side), feeler_len * radius), mul(dir, feeler_len * radius)));
+        int feeler_index = 0;
+        T nearest_dist = T(FLT_MAX);
+        uint32_t nearest_line = UINT32_MAX;
+        V3<T> nearest_ip = mk(T(0), T(0), T(0)), nearest_normal = nearest_ip;
+        bool touching = false;
+        for (int j = 0; j < k; ++j) {
+            const uint32_t l = knn[(size_t)j * nA + i];
+            if (l >= (uint32_t)nW) break;
+            const V3<T> ln = mk(wnx[l], wny[l], wnz[l]);
+            const V3<T> ls = mk(wsx[l], wsy[l], wsz[l]);
+            const V3<T> le = mk(wex[l], wey[l], wez[l]);
+            V3<T> ip;
+            const V3<T> closest = add(pos, mul(neg(ln), radius));
+            if (lines_intersect(pos, closest, ls, le, ip)) {
+                touching = true;
+                nearest_normal = ln;
+                nearest_dist = distance(pos, ip);
+            }
+            for (int f = 0; f < 3; ++f) {
+                if (lines_intersect(pos, feelers[f], ls, le, ip)) {
+                    const T dist = distance(pos, ip);
+                    if (dist < nearest_dist) {
+                        nearest_dist = dist;
+                        nearest_line = l;
+                        nearest_ip = ip;
+                        feeler_index = f;
+                        nearest_normal = ln;
+                    }
+                }
+            }
+        }
+        V3<T> steering = mk(T(0), T(0), T(0));
+        if (touching) {
+            const T pen = radius - nearest_dist;
+            pos = add(pos, mul(nearest_normal, pen));
+            dir = normalize(perpendicular_component(dir, nearest_normal));
+            A.bits[i] |= BIT_ANTI_PENETRATION_WALL;
+        } else if (nearest_line != UINT32_MAX) {
+            const T pen = len(sub(feelers[feeler_index], nearest_ip));
+            steering = mul(nearest_normal, pen);
+            A.bits[i] |= BIT_AVOID_WALLS;
+        }
+        steering = mul(steering, w);
+        setG(A, i, add(steering, G(A, i)));
+        A.px[i] = pos.x; A.py[i] = pos.y; A.pz[i] = pos.z;
+        A.fx[i] = dir.x; A.fy[i] = dir.y; A.fz[i] = dir.z;
+    }
+}
+
+// Integrate one step.
+//  OPENSTEER: SimpleVehicle::applySteeringForce (src/SimpleVehicle.cpp:167-214)
+//    with adjustRawSteeringForce (:111-132), regenerateLocalSpace (:224-230)
+//    and regenerateOrthonormalBasisUF (include/OpenSteer/SimpleVehicle.h:190-217;
+//    right-handed, side = forward x up un-normalized, :193-194).  Curvature and
+//    smoothed position are annotation-only and dropped.
+//  GPUREF: UpdateCUDAKernel (include/OpenSteer/CUDA/UpdateCUDAKernel.cu:78-123):
+//    no adjustment, no smoothing, normalized side.  When the new speed is 0
+//    the side is left unchanged (the reference writes an unset value).
+// Both clear the steering accumulator and the bits (UpdateCUDAKernel.cu:111,123).
+template <class T> static void update(int mode, Agents<T>& A, int nA, T dt) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nA; ++i) {
+        V3<T> force = G(A, i);
+        V3<T> fwd = F(A, i);
+        V3<T> pos = P(A, i);
+        V3<T> up = U(A, i);
+        V3<T> side = S(A, i);
+        T speed = A.speed[i];
+        const T max_force = A.max_force[i], max_speed = A.max_speed[i], mass = A.mass[i];
+        V3<T> new_vel;
+        if (mode == MODE_GPUREF) {
+            force = truncate_length(force, max_force);
+            const V3<T> acc = dvd(force, mass);
+            new_vel = add(mul(fwd, speed), mul(acc, dt));
+            new_vel = truncate_length(new_vel, max_speed);
+            speed = len(new_vel);
+            if (speed > T(0)) {
+                fwd = dvd(new_vel, speed);
+                side = normalize(cross(fwd, up));
+                up = cross(side, fwd);
+            }
+            pos = add(pos, mul(new_vel, dt));
+        } else {
+            // adjustRawSteeringForce (:111-132)
+            V3<T> adjusted = force;
+            const T max_adj = T(0.2f) * max_speed;
+            if (!((speed > max_adj) || is_zero(force))) {
+                const T range = speed / max_adj;
+                const T cosine = T(1) + (T(-1) - T(1)) * pow20(range);  // interpolate(pow(range,20), 1, -1)
+                adjusted = limit_max_deviation_angle(force, cosine, fwd);
+            }
+            const V3<T> clipped = truncate_length(adjusted, max_force);
+            const V3<T> new_acc = dvd(clipped, mass);
+            V3<T> sacc = mk(A.ax[i], A.ay[i], A.az[i]);
+            if (dt > T(0)) {
+                const T rate = clip(T(9) * dt, T(0.15f), T(0.4f));
+                sacc = blend(rate, new_acc, sacc);
+            }
+            new_vel = add(mul(fwd, speed), mul(sacc, dt));
+            new_vel = truncate_length(new_vel, max_speed);
+            speed = len(new_vel);
+            pos = add(pos, mul(new_vel, dt));
+            if (speed > T(0)) {
+                fwd = dvd(new_vel, speed);
+                side = cross(fwd, up);
+                up = cross(side, fwd);
+            }
+            A.ax[i] = sacc.x; A.ay[i] = sacc.y; A.az[i] = sacc.z;
+        }
+        A.px[i] = pos.x; A.py[i] = pos.y; A.pz[i] = pos.z;
+        A.fx[i] = fwd.x; A.fy[i] = fwd.y; A.fz[i] = fwd.z;
+        A.sx[i] = side.x; A.sy[i] = side.y; A.sz[i] = side.z;
+        A.ux[i] = up.x; A.uy[i] = up.y; A.uz[i] = up.z;
+        A.speed[i] = speed;
+        A.gx[i] = T(0); A.gy[i] = T(0); A.gz[i] = T(0);
+        A.bits[i] = 0u;
+    }
+}
+
+// World boundary.
+//  WRAP_BOX:    WrapWorldKernel (include/OpenSteer/CUDA/WrapWorldCUDAKernel.cu:25-35)
+//               face teleport, applied on all three axes of a box given by
+//               centre and size (the choke-point-only z rule at :42-43 is not
+//               restated).
+//  WRAP_SPHERE: Boid::handleBoundary mode 1 (plugins/Boids.cpp:185-209):
+//               OpenSteer's sphericalWrapAround, p + (offset/r)*R*-2 when r > R.
+enum { WRAP_BOX = 0, WRAP_SPHERE = 1 };
+template <class T> static void wrap_world(int shape, Agents<T>& A, int nA, const T* c, const T* size_or_radius) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nA; ++i) {
+        V3<T> p = P(A, i);
+        if (shape == WRAP_BOX) {
+            const T hx = size_or_radius[0] * T(0.5f), hy = size_or_radius[1] * T(0.5f), hz = size_or_radius[2] * T(0.5f);
+            if (p.x < c[0] - hx) p.x = c[0] + hx;
+            else if (p.x > c[0] + hx) p.x = c[0] - hx;
+            if (p.y < c[1] - hy) p.y = c[1] + hy;
+            else if (p.y > c[1] + hy) p.y = c[1] - hy;
+            if (p.z < c[2] - hz) p.z = c[2] + hz;
+            else if (p.z > c[2] + hz) p.z = c[2] - hz;
+        } else {
+            const T R = size_or_radius[0];
+            const V3<T> offset = sub(p, mk(c[0], c[1], c[2]));
+            const T r = len(offset);
+            if (r > R) p = add(p, mul(mul(dvd(offset, r), R), T(-2)));
+        }
+        A.px[i] = p.x; A.py[i] = p.y; A.pz[i] = p.z;
+    }
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------
+// C exports for ctypes (tests, smoke check, bench cpu_baseline).
+using namespace orc;
+
+extern "C" {
+
+typedef struct {
+    float ox, oy, oz, sx, sy, sz;
+    int dx, dy, dz;
+} or_grid;
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+void or_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+uint32_t or_cell_of(const or_grid* g, float x, float y, float z) {
+    Grid G{g->ox, g->oy, g->oz, g->sx, g->sy, g->sz, g->dx, g->dy, g->dz};
+    return cell_of(G, x, y, z);
+}
+
+void or_bins_build(const or_grid* g, int n, const float* px, const float* py, const float* pz, uint32_t* key,
+                   uint32_t* start, uint32_t* perm, float* spx, float* spy, float* spz) {
+    Grid G{g->ox, g->oy, g->oz, g->sx, g->sy, g->sz, g->dx, g->dy, g->dz};
+    bins_build(G, n, px, py, pz, key, start, perm, spx, spy, spz);
+}
+
+void or_knn_brute(int nA, const float* ax, const float* ay, const float* az, int nB, const float* bx, const float* by,
+                  const float* bz, int self, int k, uint32_t* idx, float* dist, const uint32_t* queries, int nq) {
+    knn_brute(nA, ax, ay, az, nB, bx, by, bz, self, k, idx, dist, queries, nq);
+}
+
+void or_knn_binned(const or_grid* g, int nA, const float* ax, const float* ay, const float* az, int nB,
+                   const float* bx, const float* by, const float* bz, int self, int k, uint32_t* idx, float* dist) {
+    Grid G{g->ox, g->oy, g->oz, g->sx, g->sy, g->sz, g->dx, g->dy, g->dz};
+    const int C = g->dx * g->dy * g->dz;
+    std::vector<uint32_t> key(nB), start((size_t)C + 1), perm(nB);
+    std::vector<float> sx(nB), sy(nB), sz(nB);
+    bins_build(G, nB, bx, by, bz, key.data(), start.data(), perm.data(), sx.data(), sy.data(), sz.data());
+    knn_binned(G, nA, ax, ay, az, nB, sx.data(), sy.data(), sz.data(), perm.data(), start.data(), self, k, idx, dist);
+}
+
+#define OR_AGENT_API(T, SUF)                                                                                        \
+    typedef Agents<T> or_agents_##SUF;                                                                              \
+    void or_steer_flock_##SUF(int which, int mode, or_agents_##SUF* A, int nA, const uint32_t* knn, int k,          \
+                              const or_agents_##SUF* B, int nB, T min_d, T max_d, T cos_max, T w, uint32_t mask) { \
+        steer_flock_one<T>(which, mode, *A, nA, knn, k, *B, nB, min_d, max_d, cos_max, w, mask);                    \
+    }                                                                                                               \
+    void or_steer_seek_##SUF(int mode, or_agents_##SUF* A, int nA, const T* tx, const T* ty, const T* tz,           \
+                             int per_agent, T w, uint32_t mask) {                                                   \
+        steer_seek<T>(mode, *A, nA, tx, ty, tz, per_agent, w, mask);                                                \
+    }                                                                                                               \
+    void or_avoid_obstacles_##SUF(or_agents_##SUF* A, int nA, const uint32_t* knn, int k, int nO, const T* ox,      \
+                                  const T* oy, const T* oz, const T* orad, T min_time, T w, uint32_t mask) {        \
+        avoid_obstacles<T>(*A, nA, knn, k, nO, ox, oy, oz, orad, min_time, w, mask);                                \
+    }                                                                                                               \
+    void or_avoid_walls_##SUF(or_agents_##SUF* A, int nA, const uint32_t* knn, int k, int nW, const T* sx,          \
+                              const T* sy, const T* sz, const T* ex, const T* ey, const T* ez, const T* nx,         \
+                              const T* ny, const T* nz, T min_time, T w, uint32_t mask) {                           \
+        avoid_walls<T>(*A, nA, knn, k, nW, sx, sy, sz, ex, ey, ez, nx, ny, nz, min_time, w, mask);                  \
+    }                                                                                                               \
+    void or_update_##SUF(int mode, or_agents_##SUF* A, int nA, T dt) { update<T>(mode, *A, nA, dt); }               \
+    void or_wrap_world_##SUF(int shape, or_agents_##SUF* A, int nA, const T* c, const T* s) {                       \
+        wrap_world<T>(shape, *A, nA, c, s);                                                                         \
+    }
+
+OR_AGENT_API(float, f32)
+OR_AGENT_API(double, f64)
+
+float or_pow20(float r) { return pow20(r); }
+void or_limit_max_deviation_angle(const float* src, float cosc, const float* basis, float* out) {
+    V3<float> r = limit_max_deviation_angle(mk(src[0], src[1], src[2]), cosc, mk(basis[0], basis[1], basis[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void or_truncate_length(const float* v, float m, float* out) {
+    V3<float> r = truncate_length(mk(v[0], v[1], v[2]), m);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+int or_lines_intersect(const float* sa, const float* ea, const float* sb, const float* eb, float* ip) {
+    V3<float> p{0, 0, 0};
+    const int r = lines_intersect(mk(sa[0], sa[1], sa[2]), mk(ea[0], ea[1], ea[2]), mk(sb[0], sb[1], sb[2]),
+                                  mk(eb[0], eb[1], eb[2]), p);
+    ip[0] = p.x; ip[1] = p.y; ip[2] = p.z;
+    return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,284 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+
+The composite steps below issue the oracle's per-behaviour passes in the same
+order as the corresponding sequence of C-ABI calls, so a state advanced here
+and on the GPU must agree (bit for bit for KNN and forces).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_LQ = HERE / "_ref" / "liblq.so"
+
+AGENT_FIELDS = (
+    "px", "py", "pz", "fx", "fy", "fz", "speed", "ax", "ay", "az",
+    "sx", "sy", "sz", "ux", "uy", "uz", "gx", "gy", "gz",
+    "radius", "max_speed", "max_force", "mass",
+)
+
+MODE_OPENSTEER, MODE_GPUREF = 0, 1
+WRAP_BOX, WRAP_SPHERE = 0, 1
+FLOCK_SEPARATION, FLOCK_ALIGNMENT, FLOCK_COHESION = 0, 1, 2
+
+
+def build() -> Path:
+    """Compile liboracle.so (and the reference's lq.c into _ref/ when the
+    reference sources are present)."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    if Path("/root/reference/src/lq.c").exists():
+        subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
+    return LIB_PATH
+
+
+class OrGrid(C.Structure):
+    _fields_ = [("ox", C.c_float), ("oy", C.c_float), ("oz", C.c_float), ("sx", C.c_float),
+                ("sy", C.c_float), ("sz", C.c_float), ("dx", C.c_int), ("dy", C.c_int), ("dz", C.c_int)]
+
+
+def _agents_type(ctype):
+    class OrAgents(C.Structure):
+        _fields_ = [(f, C.POINTER(ctype)) for f in AGENT_FIELDS] + [("bits", C.POINTER(C.c_uint32))]
+    return OrAgents
+
+
+OrAgents32 = _agents_type(C.c_float)
+OrAgents64 = _agents_type(C.c_double)
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        f, u, i = C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.c_int
+        L.or_cell_of.restype = C.c_uint32
+        L.or_cell_of.argtypes = [C.POINTER(OrGrid), C.c_float, C.c_float, C.c_float]
+        L.or_bins_build.argtypes = [C.POINTER(OrGrid), i, f, f, f, u, u, u, f, f, f]
+        L.or_knn_brute.argtypes = [i, f, f, f, i, f, f, f, i, i, u, f, u, i]
+        L.or_knn_binned.argtypes = [C.POINTER(OrGrid), i, f, f, f, i, f, f, f, i, i, u, f]
+        L.or_pow20.restype = C.c_float
+        L.or_pow20.argtypes = [C.c_float]
+        L.or_num_threads.restype = C.c_int
+        L.or_set_num_threads.argtypes = [C.c_int]
+        for suf, T, A in (("f32", C.c_float, OrAgents32), ("f64", C.c_double, OrAgents64)):
+            t = C.POINTER(T)
+            getattr(L, f"or_steer_flock_{suf}").argtypes = [i, i, C.POINTER(A), i, u, i, C.POINTER(A), i, T, T, T,
+                                                             T, C.c_uint32]
+            getattr(L, f"or_steer_seek_{suf}").argtypes = [i, C.POINTER(A), i, t, t, t, i, T, C.c_uint32]
+            getattr(L, f"or_avoid_obstacles_{suf}").argtypes = [C.POINTER(A), i, u, i, i, t, t, t, t, T, T,
+                                                                 C.c_uint32]
+            getattr(L, f"or_avoid_walls_{suf}").argtypes = [C.POINTER(A), i, u, i, i] + [t] * 9 + [T, T, C.c_uint32]
+            getattr(L, f"or_update_{suf}").argtypes = [i, C.POINTER(A), i, T]
+            getattr(L, f"or_wrap_world_{suf}").argtypes = [i, C.POINTER(A), i, t, t]
+        L.or_limit_max_deviation_angle.argtypes = [f, C.c_float, f, f]
+        L.or_truncate_length.argtypes = [f, C.c_float, f]
+        L.or_lines_intersect.restype = C.c_int
+        L.or_lines_intersect.argtypes = [f, f, f, f, f]
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int) -> None:
+    lib().or_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return lib().or_num_threads()
+
+
+def _f(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _u(a):
+    assert a.dtype == np.uint32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def grid(origin, size, cells) -> OrGrid:
+    return OrGrid(*[float(x) for x in origin], *[float(x) for x in size], *[int(c) for c in cells])
+
+
+def _c32(x):
+    return np.ascontiguousarray(x, dtype=np.float32)
+
+
+# ---------------------------------------------------------------------------
+# Bins and KNN.
+
+
+def cell_of(g: OrGrid, x, y, z) -> int:
+    return int(lib().or_cell_of(C.byref(g), float(x), float(y), float(z)))
+
+
+def bins_build(g: OrGrid, pos: np.ndarray):
+    n = pos.shape[0]
+    C_ = g.dx * g.dy * g.dz
+    px, py, pz = (_c32(pos[:, j]) for j in range(3))
+    key = np.zeros(n, np.uint32)
+    start = np.zeros(C_ + 1, np.uint32)
+    perm = np.zeros(n, np.uint32)
+    sp = [np.zeros(n, np.float32) for _ in range(3)]
+    lib().or_bins_build(C.byref(g), n, _f(px), _f(py), _f(pz), _u(key), _u(start), _u(perm), _f(sp[0]),
+                        _f(sp[1]), _f(sp[2]))
+    return key, start, perm, np.stack(sp, axis=1)
+
+
+def knn_brute(a_pos: np.ndarray, b_pos: np.ndarray, k: int, self_group: bool, queries=None):
+    """Exact KNN (key (d2, index)); returns idx, dist as [k][nA]."""
+    nA, nB = a_pos.shape[0], b_pos.shape[0]
+    A = [_c32(a_pos[:, j]) for j in range(3)]
+    B = A if self_group else [_c32(b_pos[:, j]) for j in range(3)]
+    idx = np.full((k, nA), 0xFFFFFFFF, np.uint32)
+    dist = np.full((k, nA), np.finfo(np.float32).max, np.float32)
+    q = None if queries is None else np.ascontiguousarray(queries, dtype=np.uint32)
+    lib().or_knn_brute(nA, _f(A[0]), _f(A[1]), _f(A[2]), nB, _f(B[0]), _f(B[1]), _f(B[2]), int(self_group), k,
+                       _u(idx), _f(dist), None if q is None else _u(q), 0 if q is None else q.shape[0])
+    return idx, dist
+
+
+def knn_binned(g: OrGrid, a_pos: np.ndarray, b_pos: np.ndarray, k: int, self_group: bool):
+    nA, nB = a_pos.shape[0], b_pos.shape[0]
+    A = [_c32(a_pos[:, j]) for j in range(3)]
+    B = A if self_group else [_c32(b_pos[:, j]) for j in range(3)]
+    idx = np.zeros((k, nA), np.uint32)
+    dist = np.zeros((k, nA), np.float32)
+    lib().or_knn_binned(C.byref(g), nA, _f(A[0]), _f(A[1]), _f(A[2]), nB, _f(B[0]), _f(B[1]), _f(B[2]),
+                        int(self_group), k, _u(idx), _f(dist))
+    return idx, dist
+
+
+# ---------------------------------------------------------------------------
+# Agent passes (operate in place on an AgentState of float32 or float64).
+
+
+class _Bound:
+    """Keeps the numpy arrays of an AgentState alive while a struct points at them."""
+
+    def __init__(self, state):
+        dt = state.arrays["px"].dtype
+        self.f64 = dt == np.float64
+        for k in AGENT_FIELDS:
+            a = state.arrays[k]
+            assert a.dtype == dt and a.flags.c_contiguous
+        conv = _d if self.f64 else _f
+        S = OrAgents64 if self.f64 else OrAgents32
+        self.s = S(*[conv(state.arrays[k]) for k in AGENT_FIELDS], _u(state.bits))
+        self.suf = "f64" if self.f64 else "f32"
+        self.T = np.float64 if self.f64 else np.float32
+        self.conv = conv
+        self.state = state
+
+
+def steer_flock(which, state, knn_idx, other, min_d, max_d, cos_max, w, mask=0, mode=MODE_OPENSTEER):
+    a = _Bound(state)
+    b = a if other is state else _Bound(other)
+    k = knn_idx.shape[0]
+    knn = np.ascontiguousarray(knn_idx, dtype=np.uint32)
+    getattr(lib(), f"or_steer_flock_{a.suf}")(which, mode, C.byref(a.s), state.n, _u(knn), k, C.byref(b.s),
+                                               other.n, min_d, max_d, cos_max, w, mask)
+
+
+def steer_seek(state, target, w, mask=0, mode=MODE_OPENSTEER):
+    a = _Bound(state)
+    t = [np.array([target[j]], dtype=a.T) for j in range(3)]
+    getattr(lib(), f"or_steer_seek_{a.suf}")(mode, C.byref(a.s), state.n, a.conv(t[0]), a.conv(t[1]),
+                                              a.conv(t[2]), 0, w, mask)
+
+
+def avoid_obstacles(state, knn_idx, obstacles, min_time, w, mask=0):
+    a = _Bound(state)
+    o = [np.ascontiguousarray(x, dtype=a.T) for x in obstacles]
+    knn = np.ascontiguousarray(knn_idx, dtype=np.uint32)
+    getattr(lib(), f"or_avoid_obstacles_{a.suf}")(C.byref(a.s), state.n, _u(knn), knn.shape[0], o[0].shape[0],
+                                                   *[a.conv(x) for x in o], min_time, w, mask)
+
+
+def avoid_walls(state, knn_idx, walls, min_time, w, mask=0):
+    a = _Bound(state)
+    ws = [np.ascontiguousarray(walls.arrays[f], dtype=a.T) for f in ("sx", "sy", "sz", "ex", "ey", "ez", "nx",
+                                                                      "ny", "nz")]
+    knn = np.ascontiguousarray(knn_idx, dtype=np.uint32)
+    getattr(lib(), f"or_avoid_walls_{a.suf}")(C.byref(a.s), state.n, _u(knn), knn.shape[0], walls.n,
+                                               *[a.conv(x) for x in ws], min_time, w, mask)
+
+
+def update(state, dt, mode=MODE_OPENSTEER):
+    a = _Bound(state)
+    getattr(lib(), f"or_update_{a.suf}")(mode, C.byref(a.s), state.n, dt)
+
+
+def wrap_world(state, shape, center, size_or_radius):
+    a = _Bound(state)
+    c = np.array(center, dtype=a.T)
+    s = np.array(size_or_radius, dtype=a.T)
+    getattr(lib(), f"or_wrap_world_{a.suf}")(shape, C.byref(a.s), state.n, a.conv(c), a.conv(s))
+
+
+# ---------------------------------------------------------------------------
+# Composite steps.
+
+
+def boids_step(state, p, knn_fn=None):
+    """One Boids step in the order of plugins/BoidsGroup.cpp:606-637 with the
+    force composition of plugins/Boids.cpp:157-178: wrap -> KNN (self) ->
+    separation, alignment, cohesion -> integrate.  Returns the KNN (idx, dist)
+    computed from the post-wrap positions."""
+    if p.wrap_shape in (WRAP_BOX, WRAP_SPHERE):
+        wrap_world(state, p.wrap_shape, p.wrap_center, p.wrap_size)
+    pos = np.stack([state.arrays["px"], state.arrays["py"], state.arrays["pz"]], axis=1).astype(np.float32)
+    g = grid(p.grid_origin, p.grid_size, p.grid_cells)
+    idx, dist = (knn_fn or (lambda pp: knn_binned(g, pp, pp, p.k, True)))(pos)
+    for which, t in ((FLOCK_SEPARATION, p.separation), (FLOCK_ALIGNMENT, p.alignment),
+                     (FLOCK_COHESION, p.cohesion)):
+        steer_flock(which, state, idx, state, t.min_d, t.max_d, t.cos_max, t.weight, t.mask, p.mode)
+    update(state, p.dt, p.mode)
+    return idx, dist
+
+
+def choke_step(groups, sc):
+    """One choke-point step for both groups (Jacobi across groups): per group,
+    clear bits -> KNN to itself, to the other group, to walls and obstacles ->
+    avoid walls -> avoid obstacles -> separation (self, other) -> seek; then
+    integrate both.  Returns the KNN results per group."""
+    g = grid(sc.grid_origin, sc.grid_size, sc.grid_cells)
+    wmid = sc.walls.mid()
+    opos = np.stack(sc.obstacles[:3], axis=1).astype(np.float32)
+    pos = [np.stack([s.arrays["px"], s.arrays["py"], s.arrays["pz"]], axis=1).astype(np.float32) for s in groups]
+    out = []
+    for gi, st in enumerate(groups):
+        other = groups[1 - gi]
+        st.bits[:] = 0
+        ks = knn_binned(g, pos[gi], pos[gi], sc.k, True)[0]
+        ko = knn_binned(g, pos[gi], pos[1 - gi], sc.k, False)[0]
+        kw = knn_binned(g, pos[gi], wmid, sc.knw, False)[0]
+        kob = knn_binned(g, pos[gi], opos, sc.kno, False)[0]
+        avoid_walls(st, kw, sc.walls, sc.min_time_walls, sc.w_walls, 0)
+        avoid_obstacles(st, kob, sc.obstacles, sc.min_time_obstacles, sc.w_obstacles, 0)
+        steer_flock(FLOCK_SEPARATION, st, ks, st, sc.sep.min_d, sc.sep.max_d, sc.sep.cos_max, sc.sep.weight,
+                    sc.sep.mask, sc.mode)
+        steer_flock(FLOCK_SEPARATION, st, ko, other, sc.sep.min_d, sc.sep.max_d, sc.sep.cos_max, sc.sep.weight,
+                    sc.sep.mask, sc.mode)
+        steer_seek(st, sc.goals[gi], sc.w_seek, sc.mask_seek, sc.mode)
+        out.append((ks, ko, kw, kob))
+    for st in groups:
+        update(st, sc.dt, sc.mode)
+    return out

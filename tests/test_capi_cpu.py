@@ -1,0 +1,75 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/gpusteer.h declares, and rejects bad handles / arguments with status
+codes instead of aborting."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    text = (ROOT / "include" / "gpusteer.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(gs_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from gpusteer_amd import _capi
+
+    names = declared_functions()
+    assert len(names) >= 35
+    for n in names:
+        assert hasattr(_capi.lib, n), f"{n} declared in gpusteer.h but not exported"
+    # The ctypes table covers the whole header too.
+    assert set(names) <= set(_capi.EXPORTED)
+
+
+def test_status_strings():
+    from gpusteer_amd import _capi
+
+    for s in range(9):
+        assert _capi.lib.gs_status_string(s)
+    assert _capi.lib.gs_version() == 100
+
+
+def test_null_handles_rejected():
+    from gpusteer_amd import _capi as c
+
+    L = c.lib
+    assert L.gs_group_destroy(None) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_update_knn_database(None, None) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_find_knn(None, None, None, None, 1, 0) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_steer_separation(None, None, None, 1, 2, 0, 1, 0, 0) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_update_group(None, 0.1, 0) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_knn_download(None, None, None) == c.GS_ERR_INVALID_HANDLE
+    assert b"invalid" in L.gs_last_error()
+    # A pointer that was never a handle is rejected, not dereferenced.
+    bogus = C.c_void_p(0x1234)
+    assert L.gs_context_synchronize(bogus) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_step_boids(bogus, bogus, bogus, None, 1) == c.GS_ERR_INVALID_HANDLE
+
+
+def test_context_without_gpu_reports_device_error():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from gpusteer_amd import _capi as c
+
+    h = C.c_void_p()
+    st = c.lib.gs_context_create(0, None, C.byref(h))
+    assert st == c.GS_ERR_DEVICE
+    assert not h.value
+
+
+def test_package_import_requires_built_library(tmp_path):
+    # The package has no CPU fallback: its loader raises when the .so is absent.
+    import importlib.util
+
+    src = (ROOT / "gpusteer_amd" / "_capi.py").read_text()
+    assert "raise ImportError" in src
+    spec = importlib.util.find_spec("gpusteer_amd")
+    assert spec is not None

@@ -1,0 +1,168 @@
+"""GPU parity: bin database and exact KNN (binned and brute force) against the
+CPU oracle, bit-exact (indices and fp32 distances)."""
+import numpy as np
+import pytest
+
+from gpusteer_amd import scenarios
+from helpers import lattice_points
+
+pytestmark = pytest.mark.gpu
+
+
+def _group(gs, ctx, pos):
+    st = scenarios.make_agents(pos, np.tile([[0.0, 0.0, 1.0]], (pos.shape[0], 1)), np.zeros(pos.shape[0]), 0.5,
+                               1.0, 1.0)
+    return gs.AgentGroup(ctx, st)
+
+
+def _run_binned(gs, ctx, a_pos, b_pos, k, origin, size, cells, self_group, flags=0, repeat=1):
+    bins = gs.KNNBinData(ctx, cells, size, origin=origin)
+    A = _group(gs, ctx, a_pos)
+    B = A if self_group else _group(gs, ctx, b_pos)
+    gs.updateKNNDatabase(A, bins)
+    if not self_group:
+        gs.updateKNNDatabase(B, bins)
+    knn = gs.KNNData(ctx, a_pos.shape[0], k)
+    for _ in range(repeat):
+        gs.findKNearestNeighbors(A, knn, bins, B, 1, flags)
+    out = knn.download()
+    knn.close()
+    A.close()
+    if not self_group:
+        B.close()
+    bins.close()
+    return out
+
+
+CASES = [
+    # name, n, k, cells
+    ("c0-boids-k7", 4096, 7, (16, 16, 16)),
+    ("c0-boids-k1", 4096, 1, (16, 16, 16)),
+    ("c0-boids-k16", 4096, 16, (16, 16, 16)),
+    ("c0-boids-k32", 4096, 32, (16, 16, 16)),
+    ("coarse-1cell", 500, 7, (1, 1, 1)),
+    ("fine-64", 3000, 9, (64, 64, 64)),
+]
+
+
+@pytest.mark.parametrize("name,n,k,cells", CASES)
+def test_binned_equals_oracle_brute(gs, ctx, orc, name, n, k, cells):
+    print(f"case {name}", flush=True)
+    st, p = scenarios.boids_sphere(n=n)
+    pos = st.pos()
+    r_idx, r_dist = orc.knn_brute(pos, pos, k, True)
+    idx, dist = _run_binned(gs, ctx, pos, pos, k, p.grid_origin, p.grid_size, cells, True)
+    np.testing.assert_array_equal(idx, r_idx)
+    np.testing.assert_array_equal(dist, r_dist)
+
+
+def test_brute_equals_oracle(gs, ctx, orc):
+    st, _ = scenarios.boids_sphere(n=3000)
+    pos = st.pos()
+    r_idx, r_dist = orc.knn_brute(pos, pos, 7, True)
+    A = _group(gs, ctx, pos)
+    knn = gs.KNNData(ctx, pos.shape[0], 7)
+    gs.findKNearestNeighborsBruteForce(A, knn, A)
+    idx, dist = knn.download()
+    np.testing.assert_array_equal(idx, r_idx)
+    np.testing.assert_array_equal(dist, r_dist)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_lattice_ties_lower_index_wins(gs, ctx, orc, dims):
+    # Integer lattice, shuffled so index order differs from spatial order.
+    pts = lattice_points(12 if dims == 3 else 40, 1.0, dims=dims)
+    perm = np.random.default_rng(3).permutation(pts.shape[0])
+    pts = np.ascontiguousarray(pts[perm])
+    k = 12
+    r_idx, r_dist = orc.knn_brute(pts, pts, k, True)
+    cells = (6, 6, 6) if dims == 3 else (10, 1, 10)
+    size = (12.0, 12.0, 12.0) if dims == 3 else (40.0, 2.0, 40.0)
+    origin = (-0.5, -0.5, -0.5) if dims == 3 else (-0.5, -1.0, -0.5)
+    idx, dist = _run_binned(gs, ctx, pts, pts, k, origin, size, cells, True)
+    np.testing.assert_array_equal(idx, r_idx)
+    np.testing.assert_array_equal(dist, r_dist)
+    # Ties exist and the lower index comes first among equal distances.
+    d = r_dist
+    ties = (d[1:] == d[:-1])
+    assert ties.any()
+    assert np.all(r_idx[1:][ties] > r_idx[:-1][ties])
+
+
+def test_two_groups_and_padding(gs, ctx, orc):
+    g = np.random.default_rng(5)
+    a = g.uniform(-50, 50, size=(2000, 3)).astype(np.float32)
+    b = g.uniform(-50, 50, size=(5, 3)).astype(np.float32)  # fewer than k
+    k = 8
+    r_idx, r_dist = orc.knn_brute(a, b, k, False)
+    idx, dist = _run_binned(gs, ctx, a, b, k, (-55, -55, -55), (110, 110, 110), (8, 8, 8), False)
+    np.testing.assert_array_equal(idx, r_idx)
+    np.testing.assert_array_equal(dist, r_dist)
+    assert np.all(idx[5:] == 0xFFFFFFFF)
+    assert np.all(dist[5:] == np.finfo(np.float32).max)
+
+
+def test_out_of_grid_points(gs, ctx, orc):
+    # Lattice covers [-10, 10]^3, points spread over [-40, 40]^3: clamped
+    # boundary cells must still give exact answers.
+    g = np.random.default_rng(7)
+    pos = g.uniform(-40, 40, size=(3000, 3)).astype(np.float32)
+    r_idx, r_dist = orc.knn_brute(pos, pos, 7, True)
+    idx, dist = _run_binned(gs, ctx, pos, pos, 7, (-10, -10, -10), (20, 20, 20), (4, 4, 4), True)
+    np.testing.assert_array_equal(idx, r_idx)
+    np.testing.assert_array_equal(dist, r_dist)
+
+
+def test_seeded_call_equals_unseeded(gs, ctx, orc):
+    st, p = scenarios.boids_sphere(n=4096)
+    pos = st.pos()
+    r_idx, r_dist = orc.knn_brute(pos, pos, 7, True)
+    # Second call on the same container uses the first result as its bound.
+    idx, dist = _run_binned(gs, ctx, pos, pos, 7, p.grid_origin, p.grid_size, p.grid_cells, True, repeat=3)
+    np.testing.assert_array_equal(idx, r_idx)
+    np.testing.assert_array_equal(dist, r_dist)
+
+
+def test_bin_database_matches_oracle(gs, ctx, orc):
+    for name, pos, origin, size, cells in [
+        ("c0", scenarios.boids_sphere(n=4096)[0].pos(), (-55,) * 3, (110,) * 3, (16, 16, 16)),
+        ("dense-2d", scenarios.choke(n_total=4000).groups[0].pos(), (-100, -5, -100), (200, 10, 200), (40, 1, 40)),
+        ("out-of-grid", np.random.default_rng(1).uniform(-30, 30, size=(2000, 3)).astype(np.float32),
+         (-10,) * 3, (20,) * 3, (5, 5, 5)),
+        ("one-point", np.zeros((1, 3), np.float32), (-1,) * 3, (2,) * 3, (2, 2, 2)),
+    ]:
+        print(f"case {name}", flush=True)
+        g = orc.grid(origin, size, cells)
+        r_key, r_start, r_perm, r_sp = orc.bins_build(g, pos)
+        bins = gs.KNNBinData(ctx, cells, size, origin=origin)
+        A = _group(gs, ctx, pos)
+        gs.updateKNNDatabase(A, bins)
+        key, start, perm, sp = A.download_bins(bins.num_cells)
+        np.testing.assert_array_equal(key, r_key, err_msg=name)
+        np.testing.assert_array_equal(start, r_start, err_msg=name)
+        np.testing.assert_array_equal(perm, r_perm, err_msg=name)
+        np.testing.assert_array_equal(sp, r_sp, err_msg=name)
+        # Rebuild on the same group (counts must have been cleared).
+        gs.updateKNNDatabase(A, bins)
+        key2, start2, perm2, _ = A.download_bins(bins.num_cells)
+        np.testing.assert_array_equal(start2, r_start, err_msg=name)
+        np.testing.assert_array_equal(perm2, r_perm, err_msg=name)
+        A.close()
+        bins.close()
+
+
+def test_large_flock_subset_against_brute(gs, ctx, orc):
+    """2^20 agents in a 1000^3 box, 64^3 lattice: binned KNN for every agent,
+    checked against the oracle's brute force on 1,024 random queries (each
+    against all 2^20 agents)."""
+    st, p = scenarios.boids_box(n=1 << 20)
+    pos = st.pos()
+    idx, dist = _run_binned(gs, ctx, pos, pos, 7, p.grid_origin, p.grid_size, p.grid_cells, True, repeat=2)
+    q = np.random.default_rng(11).choice(pos.shape[0], size=1024, replace=False).astype(np.uint32)
+    r_idx, r_dist = orc.knn_brute(pos, pos, 7, True, queries=q)
+    np.testing.assert_array_equal(idx[:, q], r_idx[:, q])
+    np.testing.assert_array_equal(dist[:, q], r_dist[:, q])
+    # Size-independent properties for all agents: no self, ascending keys.
+    n = pos.shape[0]
+    assert not np.any(idx == np.arange(n, dtype=np.uint32)[None, :])
+    assert np.all(dist[1:] >= dist[:-1])
