@@ -25,6 +25,7 @@ from .api import (  # noqa: F401
     steerToAvoidObstacles,
     steerToAvoidWalls,
     stepBoids,
+    stepBoidsTimed,
     updateGroup,
     updateKNNDatabase,
     wrapWorld,

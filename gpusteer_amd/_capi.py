@@ -86,6 +86,10 @@ class GsBoidsParams(C.Structure):
     ]
 
 
+class GsBoidsTiming(C.Structure):
+    _fields_ = [("bin_build_ms", C.c_float), ("fused_ms", C.c_float), ("step_ms", C.c_float)]
+
+
 class GpuSteerError(RuntimeError):
     def __init__(self, status: int, where: str, detail: str):
         self.status = status
@@ -124,6 +128,7 @@ _SIGS = {
     "gs_group_upload_agents": (C.c_int, [_h, C.c_uint32, C.POINTER(GsAgents)]),
     "gs_group_download_agents": (C.c_int, [_h, C.POINTER(GsAgents)]),
     "gs_group_device_agents": (C.c_int, [_h, C.POINTER(GsAgents)]),
+    "gs_group_device_order": (C.c_int, [_h, C.POINTER(_up), C.POINTER(_up)]),
     "gs_group_upload_obstacles": (C.c_int, [_h, C.c_uint32, _fp, _fp, _fp, _fp]),
     "gs_group_upload_walls": (C.c_int, [_h, C.c_uint32, C.POINTER(GsWalls)]),
     "gs_group_download_bins": (C.c_int, [_h, _up, _up, _up, _fp, _fp, _fp]),
@@ -148,6 +153,7 @@ _SIGS = {
     "gs_wrap_world": (C.c_int, [_h, C.c_int, _fp, _fp]),
     "gs_clear_bits": (C.c_int, [_h]),
     "gs_step_boids": (C.c_int, [_h, _h, _h, C.POINTER(GsBoidsParams), C.c_uint32]),
+    "gs_step_boids_timed": (C.c_int, [_h, _h, _h, C.POINTER(GsBoidsParams), C.c_uint32, C.POINTER(GsBoidsTiming)]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)

@@ -136,16 +136,81 @@ static void d2h(void* h, const void* d, size_t bytes, hipStream_t st) {
     if (bytes) GS_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st));
 }
 
-static void mark_seed(gs_knn_s* out, const gs_group_s* A, const gs_group_s* B) {
-    out->seed_valid = true;
-    out->seed_a = A;
-    out->seed_b = B;
-    out->seed_epoch_a = A->membership_epoch;
-    out->seed_epoch_b = B->membership_epoch;
+// Records that out's current buffer holds the exact result for (A, B) now.
+static void mark_result(gs_knn_s* out, const gs_group_s* A, const gs_group_s* B) {
+    out->valid = true;
+    out->a = A;
+    out->b = B;
+    out->a_members = A->membership_epoch;
+    out->b_members = B->membership_epoch;
+    out->a_order = A->order_epoch;
+    out->b_order = B->order_epoch;
 }
-static bool seed_usable(const gs_knn_s* out, const gs_group_s* A, const gs_group_s* B, uint32_t flags) {
-    return !(flags & GS_KNN_NO_SEED) && out->seed_valid && out->seed_a == A && out->seed_b == B &&
-           out->seed_epoch_a == A->membership_epoch && out->seed_epoch_b == B->membership_epoch;
+
+static bool holds_result_for(const gs_knn_s* out, const gs_group_s* A, const gs_group_s* B) {
+    return out->valid && out->a == A && out->b == B && out->a_members == A->membership_epoch &&
+           out->b_members == B->membership_epoch;
+}
+
+// Seed of a KNN call: the container's previous exact result for (A, B), if
+// it is at most one reorder behind on either side.  row_map sends a current
+// slot of A to its previous slot, val_map a previous slot of B to its
+// current one.
+struct Seed {
+    const uint32_t* seed = nullptr;
+    const uint32_t* row_map = nullptr;
+    const uint32_t* val_map = nullptr;
+};
+static Seed seed_for(const gs_knn_s* out, const gs_group_s* A, const gs_group_s* B, uint32_t flags) {
+    Seed s;
+    if ((flags & GS_KNN_NO_SEED) || !holds_result_for(out, A, B)) return s;
+    if (out->a_order != A->order_epoch) {
+        if (A->prev_order_epoch == 0 || out->a_order != A->prev_order_epoch) return s;
+        s.row_map = A->new_to_old.p;
+    }
+    if (out->b_order != B->order_epoch) {
+        if (B->prev_order_epoch == 0 || out->b_order != B->prev_order_epoch) return s;
+        s.val_map = B->old_to_new.p;
+    }
+    s.seed = out->idx();
+    return s;
+}
+
+// Brings a container that holds (A, B)'s result to the current storage order
+// of both groups (one reorder behind at most), or fails.
+static void ensure_current(gs_knn_s* knn, const gs_group_s* A, const gs_group_s* B) {
+    if (!holds_result_for(knn, A, B))
+        fail(GS_ERR_INVALID_ARGUMENT, "KNN container does not hold a result for these groups (call gs_find_knn)");
+    const uint32_t* row = nullptr;
+    const uint32_t* val = nullptr;
+    if (knn->a_order != A->order_epoch) {
+        if (A->prev_order_epoch == 0 || knn->a_order != A->prev_order_epoch)
+            fail(GS_ERR_NOT_BINNED, "KNN result is older than the last two bin builds of its group; recompute it");
+        row = A->old_to_new.p;
+    }
+    if (knn->b_order != B->order_epoch) {
+        if (B->prev_order_epoch == 0 || knn->b_order != B->prev_order_epoch)
+            fail(GS_ERR_NOT_BINNED, "KNN result is older than the last two bin builds of its group; recompute it");
+        val = B->old_to_new.p;
+    }
+    if (row || val) {
+        launch_knn_remap(knn, A->n, row, val);
+        knn->cur ^= 1;
+        knn->a_order = A->order_epoch;
+        knn->b_order = B->order_epoch;
+    }
+}
+
+static void after_reorder(gs_group_s* g) {
+    if (g->n == 0) return;
+    g->prev_order_epoch = g->order_epoch;
+    g->order_epoch = next_epoch();
+}
+
+template <class T> static std::vector<T> d2h_vec(const T* d, size_t n, hipStream_t st) {
+    std::vector<T> v(n);
+    if (n) GS_HIP(hipMemcpyAsync(v.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, st));
+    return v;
 }
 
 }  // namespace gs
@@ -155,8 +220,8 @@ using namespace gs;
 gs_agents gs_group_s::dev_agents() const {
     gs_agents a;
     float** fp = (float**)&a;
-    for (int j = 0; j < 23; ++j) fp[j] = f[j].p;
-    a.bits = bits.p;
+    for (int j = 0; j < kFields; ++j) fp[j] = F(j);
+    a.bits = bits();
     return a;
 }
 
@@ -308,10 +373,25 @@ int gs_group_create(gs_context ctx, int kind, gs_group* out) {
         auto* g = new gs_group_s;
         g->ctx = ctx;
         g->kind = kind;
+        g->live = kind == GS_KIND_AGENTS ? kFieldsAgents : kind == GS_KIND_OBSTACLES ? kFieldsObstacles : kFieldsWalls;
         g->membership_epoch = next_epoch();
+        g->order_epoch = next_epoch();
         reg_add(g);
         *out = g;
     });
+}
+
+static void release_group(gs_group_s* g) {
+    for (auto& f : g->fb)
+        for (auto& b : f) b.release();
+    for (auto& b : g->bitsb) b.release();
+    for (auto& b : g->idb) b.release();
+    gs::DevBuf<uint32_t>* u[] = {&g->slot_of_id, &g->new_to_old, &g->old_to_new, &g->key, &g->rank,
+                                 &g->perm_tmp,   &g->start,      &g->count,      &g->flag};
+    for (auto* b : u) b->release();
+    g->spx.release();
+    g->spy.release();
+    g->spz.release();
 }
 
 int gs_group_destroy(gs_group g) {
@@ -319,23 +399,7 @@ int gs_group_destroy(gs_group g) {
         check_group(g, "group");
         use_device(g->ctx);
         (void)hipStreamSynchronize(g->ctx->stream);
-        for (auto& b : g->f) b.release();
-        for (auto& b : g->w) b.release();
-        g->bits.release();
-        g->key.release();
-        g->rank.release();
-        g->perm_tmp.release();
-        g->perm.release();
-        g->start.release();
-        g->count.release();
-        g->spx.release();
-        g->spy.release();
-        g->spz.release();
-        g->sfx.release();
-        g->sfy.release();
-        g->sfz.release();
-        g->inv.release();
-        g->flag.release();
+        release_group(g);
         reg_del(g);
         delete g;
     });
@@ -349,10 +413,29 @@ int gs_group_size(gs_group g, uint32_t* n) {
     });
 }
 
-static void set_members(gs_group_s* g, uint32_t n) {
+// New members in index order: allocate both buffers of every live field,
+// identity id <-> slot maps, fresh epochs.
+static void begin_members(gs_group_s* g, uint32_t n) {
+    for (int j = 0; j < kFields; ++j)
+        if (g->live & (1u << j)) {
+            g->fb[j][0].reserve(n);
+            g->fb[j][1].reserve(n);
+        }
+    if (g->kind == GS_KIND_AGENTS) {
+        g->bitsb[0].reserve(n);
+        g->bitsb[1].reserve(n);
+    }
+    g->idb[0].reserve(n);
+    g->idb[1].reserve(n);
+    g->slot_of_id.reserve(n);
+    launch_iota(g->ids(), n, g->ctx->stream);
+    launch_iota(g->slot_of_id.p, n, g->ctx->stream);
     g->n = n;
     g->binned = false;
+    g->snap_valid = false;
     g->membership_epoch = next_epoch();
+    g->order_epoch = next_epoch();
+    g->prev_order_epoch = 0;
 }
 
 int gs_group_upload_agents(gs_group g, uint32_t n, const gs_agents* host) {
@@ -361,18 +444,16 @@ int gs_group_upload_agents(gs_group g, uint32_t n, const gs_agents* host) {
         if (n > 0 && !host) fail(GS_ERR_INVALID_ARGUMENT, "host arrays are NULL");
         const float* const* hf = host ? (const float* const*)host : nullptr;
         if (n > 0) {
-            for (int j = 0; j < 23; ++j)
+            for (int j = 0; j < kFields; ++j)
                 if (!hf[j]) fail(GS_ERR_INVALID_ARGUMENT, "agent array " + std::to_string(j) + " is NULL");
             if (!host->bits) fail(GS_ERR_INVALID_ARGUMENT, "bits array is NULL");
         }
         use_device(g->ctx);
         hipStream_t st = g->ctx->stream;
-        for (int j = 0; j < 23; ++j) g->f[j].reserve(n);
-        g->bits.reserve(n);
-        for (int j = 0; j < 23 && n; ++j) h2d(g->f[j].p, hf[j], (size_t)n * 4, st);
-        if (n) h2d(g->bits.p, host->bits, (size_t)n * 4, st);
+        begin_members(g, n);
+        for (int j = 0; j < kFields && n; ++j) h2d(g->F(j), hf[j], (size_t)n * 4, st);
+        if (n) h2d(g->bits(), host->bits, (size_t)n * 4, st);
         GS_HIP(hipStreamSynchronize(st));
-        set_members(g, n);
     });
 }
 
@@ -382,11 +463,24 @@ int gs_group_download_agents(gs_group g, gs_agents* host) {
         if (!host) fail(GS_ERR_INVALID_ARGUMENT, "host arrays are NULL");
         use_device(g->ctx);
         hipStream_t st = g->ctx->stream;
-        float* const* hf = (float* const*)host;
-        for (int j = 0; j < 23 && g->n; ++j)
-            if (hf[j]) d2h(hf[j], g->f[j].p, (size_t)g->n * 4, st);
-        if (host->bits && g->n) d2h(host->bits, g->bits.p, (size_t)g->n * 4, st);
+        const size_t n = g->n;
+        if (!n) return;
+        // Storage order -> agent index order on the host.
+        auto ids = d2h_vec(g->ids(), n, st);
+        std::vector<float> tmp(n);
         GS_HIP(hipStreamSynchronize(st));
+        float* const* hf = (float* const*)host;
+        for (int j = 0; j < kFields; ++j) {
+            if (!hf[j]) continue;
+            GS_HIP(hipMemcpyAsync(tmp.data(), g->F(j), n * 4, hipMemcpyDeviceToHost, st));
+            GS_HIP(hipStreamSynchronize(st));
+            for (size_t s = 0; s < n; ++s) hf[j][ids[s]] = tmp[s];
+        }
+        if (host->bits) {
+            auto b = d2h_vec(g->bits(), n, st);
+            GS_HIP(hipStreamSynchronize(st));
+            for (size_t s = 0; s < n; ++s) host->bits[ids[s]] = b[s];
+        }
     });
 }
 
@@ -398,6 +492,14 @@ int gs_group_device_agents(gs_group g, gs_agents* dev) {
     });
 }
 
+int gs_group_device_order(gs_group g, uint32_t** id_of_slot, uint32_t** slot_of_id) {
+    return guarded([&] {
+        check_group(g, "group");
+        if (id_of_slot) *id_of_slot = g->ids();
+        if (slot_of_id) *slot_of_id = g->slot_of_id.p;
+    });
+}
+
 int gs_group_upload_obstacles(gs_group g, uint32_t n, const float* cx, const float* cy, const float* cz,
                               const float* radius) {
     return guarded([&] {
@@ -405,14 +507,11 @@ int gs_group_upload_obstacles(gs_group g, uint32_t n, const float* cx, const flo
         if (n > 0 && (!cx || !cy || !cz || !radius)) fail(GS_ERR_INVALID_ARGUMENT, "obstacle arrays are NULL");
         use_device(g->ctx);
         hipStream_t st = g->ctx->stream;
+        begin_members(g, n);
         const float* src[4] = {cx, cy, cz, radius};
         const int dst[4] = {0, 1, 2, 19};
-        for (int j = 0; j < 4; ++j) {
-            g->f[dst[j]].reserve(n);
-            h2d(g->f[dst[j]].p, src[j], (size_t)n * 4, st);
-        }
+        for (int j = 0; j < 4; ++j) h2d(g->F(dst[j]), src[j], (size_t)n * 4, st);
         GS_HIP(hipStreamSynchronize(st));
-        set_members(g, n);
     });
 }
 
@@ -432,16 +531,10 @@ int gs_group_upload_walls(gs_group g, uint32_t n, const gs_walls* host) {
         }
         use_device(g->ctx);
         hipStream_t st = g->ctx->stream;
-        for (int j = 0; j < 9; ++j) {
-            g->w[j].reserve(n);
-            if (n) h2d(g->w[j].p, hw[j], (size_t)n * 4, st);
-        }
-        for (int a = 0; a < 3; ++a) {
-            g->f[a].reserve(n);
-            if (n) h2d(g->f[a].p, mid[a].data(), (size_t)n * 4, st);
-        }
+        begin_members(g, n);
+        for (int j = 0; j < 9 && n; ++j) h2d(g->F(kWallField + j), hw[j], (size_t)n * 4, st);
+        for (int a = 0; a < 3 && n; ++a) h2d(g->F(a), mid[a].data(), (size_t)n * 4, st);
         GS_HIP(hipStreamSynchronize(st));
-        set_members(g, n);
     });
 }
 
@@ -449,17 +542,22 @@ int gs_group_download_bins(gs_group g, uint32_t* key, uint32_t* start, uint32_t*
                            float* spz) {
     return guarded([&] {
         check_group(g, "group");
-        if (!g->binned) fail(GS_ERR_NOT_BINNED, "group has no bin database");
+        if (!g->binned || !g->snap_valid) fail(GS_ERR_NOT_BINNED, "group has no bin database");
         use_device(g->ctx);
         hipStream_t st = g->ctx->stream;
         const size_t n = g->n;
-        if (key) d2h(key, g->key.p, n * 4, st);
-        if (start) d2h(start, g->start.p, ((size_t)g->bin_cells + 1) * 4, st);
-        if (perm) d2h(perm, g->perm.p, n * 4, st);
+        const size_t C = g->bin_cells;
+        auto st_h = d2h_vec(g->start.p, C + 1, st);
+        auto ids = d2h_vec(g->ids(), n, st);
         if (spx) d2h(spx, g->spx.p, n * 4, st);
         if (spy) d2h(spy, g->spy.p, n * 4, st);
         if (spz) d2h(spz, g->spz.p, n * 4, st);
         GS_HIP(hipStreamSynchronize(st));
+        if (start) memcpy(start, st_h.data(), (C + 1) * 4);
+        if (perm) memcpy(perm, ids.data(), n * 4);  // agent index of each sorted slot
+        if (key)
+            for (size_t c = 0; c < C; ++c)
+                for (uint32_t s = st_h[c]; s < st_h[c + 1] && s < n; ++s) key[ids[s]] = (uint32_t)c;
     });
 }
 
@@ -468,7 +566,11 @@ int gs_group_check_finite(gs_group g, uint32_t* first_bad) {
     int s = guarded([&] {
         check_group(g, "group", GS_KIND_AGENTS);
         use_device(g->ctx);
-        launch_check_finite(g, &bad);
+        uint32_t slot = UINT32_MAX;
+        launch_check_finite(g, &slot);
+        if (slot != UINT32_MAX) {
+            GS_HIP(hipMemcpy(&bad, g->ids() + slot, 4, hipMemcpyDeviceToHost));
+        }
     });
     if (first_bad) *first_bad = bad;
     if (s != GS_OK) return s;
@@ -492,11 +594,15 @@ int gs_knn_create(gs_context ctx, uint32_t n, uint32_t k, gs_knn* out) {
         c->n = n;
         c->k = k;
         try {
-            c->idx.reserve((size_t)n * k);
-            c->dist.reserve((size_t)n * k);
+            for (int b = 0; b < 2; ++b) {
+                c->idxb[b].reserve((size_t)n * k);
+                c->distb[b].reserve((size_t)n * k);
+            }
         } catch (...) {
-            c->idx.release();
-            c->dist.release();
+            for (int b = 0; b < 2; ++b) {
+                c->idxb[b].release();
+                c->distb[b].release();
+            }
             delete c;
             throw;
         }
@@ -510,8 +616,10 @@ int gs_knn_destroy(gs_knn knn) {
         check(knn, "knn");
         use_device(knn->ctx);
         (void)hipStreamSynchronize(knn->ctx->stream);
-        knn->idx.release();
-        knn->dist.release();
+        for (int b = 0; b < 2; ++b) {
+            knn->idxb[b].release();
+            knn->distb[b].release();
+        }
         reg_del(knn);
         delete knn;
     });
@@ -528,12 +636,27 @@ int gs_knn_shape(gs_knn knn, uint32_t* n, uint32_t* k) {
 int gs_knn_download(gs_knn knn, uint32_t* idx, float* dist) {
     return guarded([&] {
         check(knn, "knn");
+        if (!knn->valid) fail(GS_ERR_INVALID_ARGUMENT, "KNN container holds no result");
+        if (!reg_has(knn->a) || !reg_has(knn->b)) fail(GS_ERR_INVALID_HANDLE, "the KNN result's groups are gone");
+        gs_group_s* A = (gs_group_s*)knn->a;
+        gs_group_s* B = (gs_group_s*)knn->b;
         use_device(knn->ctx);
+        ensure_current(knn, A, B);
         hipStream_t st = knn->ctx->stream;
-        const size_t m = (size_t)knn->n * knn->k;
-        if (idx) d2h(idx, knn->idx.p, m * 4, st);
-        if (dist) d2h(dist, knn->dist.p, m * 4, st);
+        const size_t n = knn->n, k = knn->k;
+        auto si = d2h_vec(knn->idx(), n * k, st);
+        auto sd = d2h_vec(knn->dist(), n * k, st);
+        auto ida = d2h_vec(A->ids(), A->n, st);
+        auto idb = d2h_vec(B->ids(), B->n, st);
         GS_HIP(hipStreamSynchronize(st));
+        // Slot space -> agent index space: row = A's agent index, value = B's.
+        for (size_t j = 0; j < k; ++j)
+            for (size_t s = 0; s < n; ++s) {
+                const uint32_t v = si[j * n + s];
+                const size_t o = j * n + ida[s];
+                if (idx) idx[o] = v == UINT32_MAX ? UINT32_MAX : idb[v];
+                if (dist) dist[o] = sd[j * n + s];
+            }
     });
 }
 
@@ -542,9 +665,9 @@ int gs_knn_device(gs_knn knn, uint32_t** idx, float** dist) {
         check(knn, "knn");
         // The caller may now write the buffers: their contents can no longer
         // be trusted as an exact previous result.
-        knn->seed_valid = false;
-        if (idx) *idx = knn->idx.p;
-        if (dist) *dist = knn->dist.p;
+        knn->valid = false;
+        if (idx) *idx = knn->idx();
+        if (dist) *dist = knn->dist();
     });
 }
 
@@ -555,7 +678,9 @@ int gs_update_knn_database(gs_group g, gs_bins bins) {
         check(bins, "bins");
         if (bins->ctx != g->ctx) fail(GS_ERR_INVALID_ARGUMENT, "bins belong to another context");
         use_device(g->ctx);
-        launch_bins_build(g, bins, false, -1, nullptr, nullptr);
+        launch_bins_build(g, bins, -1, nullptr, nullptr, true);
+        g->snap_valid = true;
+        after_reorder(g);
     });
 }
 
@@ -567,12 +692,15 @@ int gs_find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t searc
         check(bins, "bins");
         if (A->ctx != B->ctx || bins->ctx != A->ctx) fail(GS_ERR_INVALID_ARGUMENT, "handles of different contexts");
         require_knn_shape(out, A);
-        if (!A->binned || !B->binned) fail(GS_ERR_NOT_BINNED, "call gs_update_knn_database on both groups first");
+        if (!A->binned || !B->binned || !A->snap_valid || !B->snap_valid)
+            fail(GS_ERR_NOT_BINNED, "call gs_update_knn_database on both groups first");
         if (!same_grid(B->bin_grid, bins->g) || !same_grid(A->bin_grid, bins->g))
             fail(GS_ERR_NOT_BINNED, "groups were binned with another lattice");
         use_device(A->ctx);
-        launch_knn_binned(A, out, B, search_radius, seed_usable(out, A, B, flags));
-        mark_seed(out, A, B);
+        const Seed sd = seed_for(out, A, B, flags);
+        launch_knn_binned(A, out, B, search_radius, sd.seed, sd.row_map, sd.val_map);
+        out->cur ^= 1;
+        mark_result(out, A, B);
     });
 }
 
@@ -585,7 +713,8 @@ int gs_find_knn_brute(gs_group A, gs_knn out, gs_group B) {
         require_knn_shape(out, A);
         use_device(A->ctx);
         launch_knn_brute(A, out, B);
-        mark_seed(out, A, B);
+        out->cur ^= 1;
+        mark_result(out, A, B);
     });
 }
 
@@ -598,6 +727,7 @@ static int flock_call(gs_group A, gs_knn knn, gs_group B, const gs_flock_term* s
         require_knn_shape(knn, A);
         if (mode != GS_MODE_OPENSTEER && mode != GS_MODE_GPUREF) fail(GS_ERR_INVALID_ARGUMENT, "unknown mode");
         use_device(A->ctx);
+        ensure_current(knn, A, B);
         launch_flock(A, knn, B, s, a, c, mode);
     });
 }
@@ -639,6 +769,7 @@ int gs_steer_avoid_obstacles(gs_group A, gs_group O, gs_knn knn, float min_time,
         check(knn, "knn");
         require_knn_shape(knn, A);
         use_device(A->ctx);
+        ensure_current(knn, A, O);
         launch_avoid_obstacles(A, O, knn, min_time, w, mask);
     });
 }
@@ -650,6 +781,7 @@ int gs_steer_avoid_walls(gs_group A, gs_knn knn, gs_group W, float min_time, flo
         check(knn, "knn");
         require_knn_shape(knn, A);
         use_device(A->ctx);
+        ensure_current(knn, A, W);
         launch_avoid_walls(A, knn, W, min_time, w, mask);
     });
 }
@@ -681,24 +813,80 @@ int gs_clear_bits(gs_group A) {
     });
 }
 
+static void check_boids_args(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* p) {
+    check_group(A, "group A", GS_KIND_AGENTS);
+    check(knn, "knn");
+    check(bins, "bins");
+    if (!p) fail(GS_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (bins->ctx != A->ctx) fail(GS_ERR_INVALID_ARGUMENT, "handles of different contexts");
+    require_knn_shape(knn, A);
+    if (p->mode != GS_MODE_OPENSTEER && p->mode != GS_MODE_GPUREF) fail(GS_ERR_INVALID_ARGUMENT, "unknown mode");
+    if (p->wrap_shape != -1 && p->wrap_shape != GS_WRAP_BOX && p->wrap_shape != GS_WRAP_SPHERE)
+        fail(GS_ERR_INVALID_ARGUMENT, "unknown wrap shape");
+}
+
+// One fused step: build + reorder (no snapshot needed: the fused kernel reads
+// the current buffers and writes positions / forwards to the alternates).
+static void boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const gs_boids_params* p) {
+    launch_bins_build(A, bins, p->wrap_shape, p->wrap_center, p->wrap_size, false);
+    A->snap_valid = false;
+    after_reorder(A);
+    const Seed sd = seed_for(knn, A, A, p->knn_flags);
+    launch_boids_fused(A, knn, p, sd.seed, sd.row_map, sd.val_map);
+    if (A->n) {
+        knn->cur ^= 1;
+        for (int j = 0; j < 6; ++j) A->flip(j);
+    }
+    mark_result(knn, A, A);
+}
+
 int gs_step_boids(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* p, uint32_t steps) {
     return guarded([&] {
-        check_group(A, "group A", GS_KIND_AGENTS);
-        check(knn, "knn");
-        check(bins, "bins");
-        if (!p) fail(GS_ERR_INVALID_ARGUMENT, "params is NULL");
-        if (bins->ctx != A->ctx) fail(GS_ERR_INVALID_ARGUMENT, "handles of different contexts");
-        require_knn_shape(knn, A);
-        if (p->mode != GS_MODE_OPENSTEER && p->mode != GS_MODE_GPUREF) fail(GS_ERR_INVALID_ARGUMENT, "unknown mode");
-        if (p->wrap_shape != -1 && p->wrap_shape != GS_WRAP_BOX && p->wrap_shape != GS_WRAP_SPHERE)
-            fail(GS_ERR_INVALID_ARGUMENT, "unknown wrap shape");
+        check_boids_args(A, knn, bins, p);
         use_device(A->ctx);
-        for (uint32_t s = 0; s < steps; ++s) {
-            launch_bins_build(A, bins, true, p->wrap_shape, p->wrap_center, p->wrap_size);
-            launch_boids_fused(A, knn, p, seed_usable(knn, A, A, p->knn_flags));
-            mark_seed(knn, A, A);
-        }
+        for (uint32_t s = 0; s < steps; ++s) boids_step_once(A, knn, bins, p);
     });
+}
+
+int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* p, uint32_t steps,
+                        gs_boids_timing* timing) {
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    int status = guarded([&] {
+        check_boids_args(A, knn, bins, p);
+        if (!timing) fail(GS_ERR_INVALID_ARGUMENT, "timing is NULL");
+        if (steps == 0) fail(GS_ERR_INVALID_ARGUMENT, "steps must be at least 1");
+        use_device(A->ctx);
+        hipStream_t st = A->ctx->stream;
+        for (auto& e : ev) GS_HIP(hipEventCreate(&e));
+        double bin_ms = 0, fused_ms = 0;
+        for (uint32_t s = 0; s < steps; ++s) {
+            GS_HIP(hipEventRecord(ev[0], st));
+            launch_bins_build(A, bins, p->wrap_shape, p->wrap_center, p->wrap_size, false);
+            A->snap_valid = false;
+            after_reorder(A);
+            GS_HIP(hipEventRecord(ev[1], st));
+            const Seed sd = seed_for(knn, A, A, p->knn_flags);
+            launch_boids_fused(A, knn, p, sd.seed, sd.row_map, sd.val_map);
+            if (A->n) {
+                knn->cur ^= 1;
+                for (int j = 0; j < 6; ++j) A->flip(j);
+            }
+            mark_result(knn, A, A);
+            GS_HIP(hipEventRecord(ev[2], st));
+            GS_HIP(hipEventSynchronize(ev[2]));
+            float a = 0, b = 0;
+            GS_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+            GS_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+            bin_ms += a;
+            fused_ms += b;
+        }
+        timing->bin_build_ms = (float)(bin_ms / steps);
+        timing->fused_ms = (float)(fused_ms / steps);
+        timing->step_ms = (float)((bin_ms + fused_ms) / steps);
+    });
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return status;
 }
 
 }  // extern "C"

@@ -1,19 +1,22 @@
-// Bin database build: counting sort of a group's points by lattice cell.
+// Bin database build: counting sort of a group's points by lattice cell, and
+// the physical reorder of the group's state into that order.
 //
 // Replaces KNNBinningV2UpdateDBCUDA::run (include/OpenSteer/CUDA/KNNBinningV2.cu:87-162):
 // BuildDB (cell key per agent, KNNBinningV2Kernel.cu:269-296), thrust::sort_by_key
 // (:127-129), the cellStart memset (:132) and ReorderDB (KNNBinningV2Kernel.cu:314-379).
 //
-// Passes (all coalesced over agents or cells):
+// Passes (all over storage slots, coalesced):
 //   1. key      cell per point + atomic arrival rank within the cell (+ optional wrap)
 //   2. scan     exclusive scan of the per-cell counts -> start[C+1]; clears counts
-//   3. scatter  perm_tmp[start[cell] + rank] = i
-//   4. canon    makes the order inside each cell ascending by point index (a
-//               stable sort, as thrust's), gathers sorted positions (and
-//               forward vectors for the fused step) and records each point's
-//               sorted slot (inverse permutation)
-// The counts array is all-zero between builds (pass 2 clears it), so no
-// memset is recorded in the stream.
+//   3. scatter  perm_tmp[start[cell] + rank] = old slot
+//   4. reorder  fixes the order inside each cell to ascending agent index (a
+//               stable sort by cell, as thrust's), then gathers every live
+//               field, the bits and the agent index into the alternate
+//               buffers in the new order and records old <-> new slot maps.
+// Because the previous build left the state in cell order and agents move
+// little between builds, the reorder's gathers read nearly sequential
+// addresses.  The counts array is all-zero between builds (pass 2 clears it),
+// so no memset is recorded in the stream.
 #include "gs_internal.hpp"
 
 namespace gs {
@@ -27,6 +30,23 @@ constexpr int kScanTile = kBlock * kScanItems;  // 2048 cells per scan block
 __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota(uint32_t* __restrict__ p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st) {
+    if (n == 0) return;
+    k_fill_u32<<<blocks_for(n), kBlock, 0, st>>>(p, n, v);
+    GS_LAUNCHED("k_fill_u32");
+}
+
+void launch_iota(uint32_t* p, uint32_t n, hipStream_t st) {
+    if (n == 0) return;
+    k_iota<<<blocks_for(n), kBlock, 0, st>>>(p, n);
+    GS_LAUNCHED("k_iota");
 }
 
 __global__ __launch_bounds__(kBlock) void k_bin_key(uint32_t n, float* __restrict__ px, float* __restrict__ py,
@@ -161,44 +181,57 @@ __global__ __launch_bounds__(kBlock) void k_scatter(uint32_t n, const uint32_t* 
     if (d < n) perm_tmp[d] = i;
 }
 
-// Pass 4: slot s holds point i = perm_tmp[s]; its final slot is the cell start
-// plus the number of members of its cell with a smaller index.
-__global__ __launch_bounds__(kBlock) void k_canon(uint32_t n, const uint32_t* __restrict__ perm_tmp,
-                                                  const uint32_t* __restrict__ key,
-                                                  const uint32_t* __restrict__ start, const float* __restrict__ px,
-                                                  const float* __restrict__ py, const float* __restrict__ pz,
-                                                  const float* __restrict__ fx, const float* __restrict__ fy,
-                                                  const float* __restrict__ fz, uint32_t* __restrict__ perm,
-                                                  float* __restrict__ spx, float* __restrict__ spy,
-                                                  float* __restrict__ spz, float* __restrict__ sfx,
-                                                  float* __restrict__ sfy, float* __restrict__ sfz,
-                                                  uint32_t* __restrict__ inv) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint32_t i = perm_tmp[s];
-    if (i >= n) return;  // only after a corrupted build (see k_scatter)
-    const uint32_t c = key[i];
+// Field pointers of one reorder (current -> alternate buffers).
+struct FieldSet {
+    const float* src[kFields];
+    float* dst[kFields];
+    int nf;
+};
+
+// Pass 4: new position t initially holds old slot s = perm_tmp[t] (arrival
+// order).  Its final position is the cell start plus the number of members of
+// its cell with a smaller agent index.
+__global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* __restrict__ perm_tmp,
+                                                    const uint32_t* __restrict__ key,
+                                                    const uint32_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ ids, uint32_t* __restrict__ ids_new,
+                                                    uint32_t* __restrict__ slot_of_id,
+                                                    uint32_t* __restrict__ new_to_old,
+                                                    uint32_t* __restrict__ old_to_new,
+                                                    const uint32_t* __restrict__ bits, uint32_t* __restrict__ bits_new,
+                                                    FieldSet fs, float* __restrict__ spx, float* __restrict__ spy,
+                                                    float* __restrict__ spz) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t s = perm_tmp[t];
+    if (s >= n) return;  // only after a corrupted build (see k_scatter)
+    const uint32_t c = key[s];
     const uint32_t b = start[c], e = min(start[c + 1], n);
-    uint32_t d = s;
+    const uint32_t id = ids[s];
+    uint32_t d = t;
     if (e - b <= kCanonCap) {
         uint32_t r = 0;
-        for (uint32_t j = b; j < e; ++j) r += (perm_tmp[j] < i) ? 1u : 0u;
+        for (uint32_t j = b; j < e; ++j) r += (ids[perm_tmp[j]] < id) ? 1u : 0u;
         d = b + r;
     }
-    perm[d] = i;
-    spx[d] = px[i];
-    spy[d] = py[i];
-    spz[d] = pz[i];
-    inv[i] = d;
-    if (sfx) {
-        sfx[d] = fx[i];
-        sfy[d] = fy[i];
-        sfz[d] = fz[i];
+    new_to_old[d] = s;
+    old_to_new[s] = d;
+    ids_new[d] = id;
+    slot_of_id[id] = d;
+    if (bits) bits_new[d] = bits[s];
+#pragma unroll
+    for (int f = 0; f < kFields; ++f)
+        if (f < fs.nf) fs.dst[f][d] = fs.src[f][s];
+    if (spx) {
+        // fields 0..2 (positions) are always first in the set
+        spx[d] = fs.src[0][s];
+        spy[d] = fs.src[1][s];
+        spz[d] = fs.src[2][s];
     }
 }
 
-void launch_bins_build(gs_group_s* g, const gs_bins_s* b, bool gather_forward, int wrap_shape, const float* wc,
-                       const float* ws) {
+void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws,
+                       bool snapshot) {
     const uint32_t n = g->n;
     const uint32_t C = b->cells;
     hipStream_t st = g->ctx->stream;
@@ -206,15 +239,12 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, bool gather_forward, i
     g->key.reserve(n);
     g->rank.reserve(n);
     g->perm_tmp.reserve(n);
-    g->perm.reserve(n);
-    g->spx.reserve(n);
-    g->spy.reserve(n);
-    g->spz.reserve(n);
-    g->inv.reserve(n);
-    if (gather_forward) {
-        g->sfx.reserve(n);
-        g->sfy.reserve(n);
-        g->sfz.reserve(n);
+    g->new_to_old.reserve(n);
+    g->old_to_new.reserve(n);
+    if (snapshot) {
+        g->spx.reserve(n);
+        g->spy.reserve(n);
+        g->spz.reserve(n);
     }
     const uint32_t ntiles = (C + kScanTile - 1) / kScanTile;
     g->start.reserve((size_t)C + 1 + ntiles);  // partial tile sums live after start[C]
@@ -222,8 +252,7 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, bool gather_forward, i
         g->count.reserve(C);
         // A fill kernel rather than hipMemsetAsync, so the build can be
         // captured into a graph.
-        k_fill_u32<<<blocks_for(C), kBlock, 0, st>>>(g->count.p, C, 0u);
-        GS_LAUNCHED("k_fill_u32");
+        launch_fill_u32(g->count.p, C, 0u, st);
     }
     g->bin_cells = C;
     g->bin_grid = b->g;
@@ -232,7 +261,7 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, bool gather_forward, i
     if (wc) memcpy(wcv, wc, sizeof wcv);
     if (ws) memcpy(wsv, ws, sizeof wsv);
     if (n > 0) {
-        k_bin_key<<<blocks_for(n), kBlock, 0, st>>>(n, g->f[0].p, g->f[1].p, g->f[2].p, b->g, g->key.p, g->rank.p,
+        k_bin_key<<<blocks_for(n), kBlock, 0, st>>>(n, g->F(0), g->F(1), g->F(2), b->g, g->key.p, g->rank.p,
                                                     g->count.p, wrap_shape, wcv[0], wcv[1], wcv[2], wsv[0], wsv[1],
                                                     wsv[2]);
         GS_LAUNCHED("k_bin_key");
@@ -246,16 +275,29 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, bool gather_forward, i
     if (n > 0) {
         k_scatter<<<blocks_for(n), kBlock, 0, st>>>(n, g->key.p, g->rank.p, g->start.p, g->perm_tmp.p);
         GS_LAUNCHED("k_scatter");
-        const bool fwd = gather_forward && g->kind == GS_KIND_AGENTS;
-        k_canon<<<blocks_for(n), kBlock, 0, st>>>(n, g->perm_tmp.p, g->key.p, g->start.p, g->f[0].p, g->f[1].p,
-                                                  g->f[2].p, fwd ? g->f[3].p : nullptr, fwd ? g->f[4].p : nullptr,
-                                                  fwd ? g->f[5].p : nullptr, g->perm.p, g->spx.p, g->spy.p,
-                                                  g->spz.p, fwd ? g->sfx.p : nullptr, fwd ? g->sfy.p : nullptr,
-                                                  fwd ? g->sfz.p : nullptr, g->inv.p);
-        GS_LAUNCHED("k_canon");
+        FieldSet fs{};
+        int nf = 0;
+        for (int f = 0; f < kFields; ++f)
+            if (g->live & (1u << f)) {
+                fs.src[nf] = g->F(f);
+                fs.dst[nf] = g->Falt(f);
+                ++nf;
+            }
+        fs.nf = nf;
+        const bool has_bits = g->kind == GS_KIND_AGENTS;
+        k_reorder<<<blocks_for(n), kBlock, 0, st>>>(n, g->perm_tmp.p, g->key.p, g->start.p, g->ids(), g->ids_alt(),
+                                                    g->slot_of_id.p, g->new_to_old.p, g->old_to_new.p,
+                                                    has_bits ? g->bits() : nullptr,
+                                                    has_bits ? g->bitsb[g->bcur ^ 1].p : nullptr, fs,
+                                                    snapshot ? g->spx.p : nullptr, snapshot ? g->spy.p : nullptr,
+                                                    snapshot ? g->spz.p : nullptr);
+        GS_LAUNCHED("k_reorder");
+        for (int f = 0; f < kFields; ++f)
+            if (g->live & (1u << f)) g->flip(f);
+        if (has_bits) g->bcur ^= 1;
+        g->icur ^= 1;
     }
     g->binned = true;
-    g->bin_epoch++;
 }
 
 }  // namespace gs

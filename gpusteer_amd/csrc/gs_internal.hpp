@@ -137,54 +137,99 @@ struct gs_bins_s {
     uint32_t magic = 0x67736269u;
 };
 
+// Groups keep their state in storage order: every bin build physically
+// reorders all live fields into cell order (ascending agent index inside a
+// cell), so neighbours are neighbours in memory and every kernel reads and
+// writes coalesced.  Agent indices of the API stay stable through an
+// id <-> slot map; downloads translate back to index order.
+constexpr int kFields = 23;  // gs_agents float fields
+// Live fields per kind: agents all 23; obstacles centre (0..2) + radius (19);
+// walls midpoint (0..2) + start/end/normal (3..11).
+constexpr uint32_t kFieldsAgents = (1u << kFields) - 1u;
+constexpr uint32_t kFieldsObstacles = 0x7u | (1u << 19);
+constexpr uint32_t kFieldsWalls = 0xFFFu;
+constexpr int kWallField = 3;  // walls: start = F(3..5), end = F(6..8), normal = F(9..11)
+
 struct gs_group_s {
     gs_context ctx = nullptr;
     int kind = GS_KIND_AGENTS;
     uint32_t n = 0;
-    uint64_t membership_epoch = 0;  // bumped whenever the members change
+    uint32_t live = 0;               // live field mask
+    uint64_t membership_epoch = 0;   // changes whenever the members change
+    uint64_t order_epoch = 0;        // changes whenever the storage order changes
+    uint64_t prev_order_epoch = 0;   // order before the last reorder (0: none)
     uint32_t magic = 0x67736772u;
 
-    // Agents: 23 float arrays + bits, one allocation each (SoA).
-    gs::DevBuf<float> f[23];
-    gs::DevBuf<uint32_t> bits;
-    // Obstacles: centre in f[0..2] (px,py,pz), radius in f[19].
-    // Walls: midpoint in f[0..2]; start/end/normal in w[0..8].
-    gs::DevBuf<float> w[9];
+    // Double-buffered SoA fields; cur[j] selects the current buffer.
+    gs::DevBuf<float> fb[kFields][2];
+    uint8_t fcur[kFields] = {};
+    gs::DevBuf<uint32_t> bitsb[2];
+    uint8_t bcur = 0;
+    gs::DevBuf<uint32_t> idb[2];     // agent index of each storage slot
+    uint8_t icur = 0;
+    gs::DevBuf<uint32_t> slot_of_id; // storage slot of each agent index
+    // The last reorder: new slot -> old slot and old slot -> new slot.
+    gs::DevBuf<uint32_t> new_to_old, old_to_new;
 
-    // Bin database (KNNDatabase, include/OpenSteer/CUDA/KNNDatabase.cuh:30-38).
+    // Bin database (KNNDatabase, include/OpenSteer/CUDA/KNNDatabase.cuh:30-38):
+    // after a build, cell c holds storage slots [start[c], start[c+1]).
     bool binned = false;
+    bool snap_valid = false;  // spx/spy/spz hold the positions of the last build
     gs::GridDev bin_grid{};
     uint32_t bin_cells = 0;
-    uint64_t bin_epoch = 0;
-    gs::DevBuf<uint32_t> key, rank, perm_tmp, perm, start, count;
-    gs::DevBuf<float> spx, spy, spz;  // sorted positions
-    gs::DevBuf<float> sfx, sfy, sfz;  // sorted forward (agents, fused step)
-    gs::DevBuf<uint32_t> inv;         // sorted slot of each point
-    gs::DevBuf<uint32_t> flag;        // small device scratch (finite check)
+    gs::DevBuf<uint32_t> key, rank, perm_tmp, start, count;
+    // Positions at the last build, in storage order (what stand-alone KNN
+    // calls search, like the reference's positionSorted).
+    gs::DevBuf<float> spx, spy, spz;
+    gs::DevBuf<uint32_t> flag;  // small device scratch (finite check)
 
+    float* F(int j) const { return fb[j][fcur[j]].p; }
+    float* Falt(int j) const { return fb[j][fcur[j] ^ 1].p; }
+    void flip(int j) { fcur[j] ^= 1; }
+    uint32_t* bits() const { return bitsb[bcur].p; }
+    uint32_t* ids() const { return idb[icur].p; }
+    uint32_t* ids_alt() const { return idb[icur ^ 1].p; }
     gs_agents dev_agents() const;
 };
 
 struct gs_knn_s {
     gs_context ctx = nullptr;
     uint32_t n = 0, k = 0;
-    gs::DevBuf<uint32_t> idx;
-    gs::DevBuf<float> dist;
-    // Seed validity: the contents are the exact result for (seed_a, seed_b)
-    // with these membership epochs.
-    const gs_group_s* seed_a = nullptr;
-    const gs_group_s* seed_b = nullptr;
-    uint64_t seed_epoch_a = 0, seed_epoch_b = 0;
-    bool seed_valid = false;
+    // Slot-major [k][n] in storage slots: row = slot of the query in A,
+    // value = slot of the neighbour in B.  Double-buffered so a call can read
+    // its previous result while writing the new one.
+    gs::DevBuf<uint32_t> idxb[2];
+    gs::DevBuf<float> distb[2];
+    uint8_t cur = 0;
+    // The contents are the exact result for (a, b) at these epochs.
+    bool valid = false;
+    const gs_group_s* a = nullptr;
+    const gs_group_s* b = nullptr;
+    uint64_t a_members = 0, b_members = 0, a_order = 0, b_order = 0;
     uint32_t magic = 0x67736b6eu;
+
+    uint32_t* idx() const { return idxb[cur].p; }
+    float* dist() const { return distb[cur].p; }
+    uint32_t* idx_alt() const { return idxb[cur ^ 1].p; }
+    float* dist_alt() const { return distb[cur ^ 1].p; }
 };
 
 namespace gs {
 // Host-side launchers (one per operation) implemented in the .hip files.
-void launch_bins_build(gs_group_s* g, const gs_bins_s* b, bool gather_forward, int wrap_shape, const float* wc,
-                       const float* ws);
-void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, bool use_seed);
+void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
+void launch_iota(uint32_t* p, uint32_t n, hipStream_t st);
+// Bin build + reorder of g into cell order (optionally wrapping positions
+// first); snapshot = also write the position snapshot spx/spy/spz.
+void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws,
+                       bool snapshot);
+// KNN into out's alternate buffer (caller flips).  row_map / val_map translate
+// a seed held at the previous storage order of A / B (NULL: same order).
+void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, const uint32_t* seed,
+                       const uint32_t* row_map, const uint32_t* val_map);
 void launch_knn_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B);
+// Rewrites out's contents from the previous storage order of A / B to the
+// current one (into the alternate buffer; caller flips).
+void launch_knn_remap(gs_knn_s* out, uint32_t nA, const uint32_t* row_old_to_new, const uint32_t* val_old_to_new);
 void launch_flock(gs_group_s* A, gs_knn_s* knn, gs_group_s* B, const gs_flock_term* sep, const gs_flock_term* ali,
                   const gs_flock_term* coh, int mode);
 void launch_seek(gs_group_s* A, const float* target, float w, uint32_t mask, int mode);
@@ -193,6 +238,10 @@ void launch_avoid_walls(gs_group_s* A, gs_knn_s* knn, gs_group_s* W, float min_t
 void launch_update(gs_group_s* A, float dt, int mode);
 void launch_wrap(gs_group_s* A, int shape, const float* c, const float* s);
 void launch_clear_bits(gs_group_s* A);
-void launch_check_finite(gs_group_s* A, uint32_t* first_bad_host);
-void launch_boids_fused(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, bool use_seed);
+void launch_check_finite(gs_group_s* A, uint32_t* first_bad_slot_host);
+// Fused KNN + flocking + integrate after a build; writes the KNN into out's
+// alternate buffer (caller flips) and the new positions / forwards into the
+// alternate field buffers (caller flips fields 0..5).
+void launch_boids_fused(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, const uint32_t* seed,
+                        const uint32_t* row_map, const uint32_t* val_map);
 }  // namespace gs

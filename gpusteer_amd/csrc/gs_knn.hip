@@ -29,35 +29,37 @@
 
 namespace gs {
 
+// One lane per query slot s of A (storage = cell order, so a wavefront's lanes
+// share cells and their candidate runs overlap in cache).
 template <int KMAX>
 __global__ __launch_bounds__(kBlock) void k_knn_binned(
-    uint32_t nA, const float* __restrict__ aspx, const float* __restrict__ aspy, const float* __restrict__ aspz,
-    const uint32_t* __restrict__ aperm, uint32_t nB, const float* __restrict__ bspx, const float* __restrict__ bspy,
-    const float* __restrict__ bspz, const uint32_t* __restrict__ bperm, const uint32_t* __restrict__ bstart,
-    const uint32_t* __restrict__ binv, GridDev g, int self,
-    uint32_t k, int R0, const uint32_t* __restrict__ seed, uint32_t* __restrict__ out_idx,
-    float* __restrict__ out_dist) {
+    uint32_t nA, const float* __restrict__ ax, const float* __restrict__ ay, const float* __restrict__ az,
+    uint32_t nB, const float* __restrict__ bx, const float* __restrict__ by, const float* __restrict__ bz,
+    const uint32_t* __restrict__ bids, const uint32_t* __restrict__ bstart, GridDev g, int self, uint32_t k, int R0,
+    const uint32_t* __restrict__ seed, const uint32_t* __restrict__ row_map, const uint32_t* __restrict__ val_map,
+    uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nA) return;
-    const float qx = aspx[s], qy = aspy[s], qz = aspz[s];
-    const uint32_t q = aperm[s];
+    const float qx = ax[s], qy = ay[s], qz = az[s];
     TopK<KMAX> h;
     h.init(k);
-    const float thr = seed ? seed_bound(seed, k, nA, q, nB, qx, qy, qz, binv, bspx, bspy, bspz) : INFINITY;
-    binned_search<KMAX>(h, thr, qx, qy, qz, self ? s : UINT32_MAX, g, R0, bspx, bspy, bspz, bperm, bstart);
-    h.store(k, nA, q, out_idx, out_dist);
+    const float thr = seed ? seed_bound(seed, k, nA, row_map ? row_map[s] : s, nB, val_map, qx, qy, qz, bx, by, bz)
+                           : INFINITY;
+    binned_search<KMAX>(h, thr, qx, qy, qz, self ? s : UINT32_MAX, g, R0, bx, by, bz, bids, bstart);
+    h.store(k, nA, s, out_idx, out_dist);
 }
 
-// Brute force: one lane per query in index order, B staged through LDS in
-// tiles of one block.
+// Brute force: one lane per query slot, B staged through LDS in tiles of one
+// block (positions and agent indices).
 template <int KMAX>
 __global__ __launch_bounds__(kBlock) void k_knn_brute(uint32_t nA, const float* __restrict__ ax,
                                                       const float* __restrict__ ay, const float* __restrict__ az,
                                                       uint32_t nB, const float* __restrict__ bx,
                                                       const float* __restrict__ by, const float* __restrict__ bz,
-                                                      int self, uint32_t k, uint32_t* __restrict__ out_idx,
-                                                      float* __restrict__ out_dist) {
+                                                      const uint32_t* __restrict__ bids, int self, uint32_t k,
+                                                      uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
     __shared__ float tx[kBlock], ty[kBlock], tz[kBlock];
+    __shared__ uint32_t tid[kBlock];
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = q < nA;
     const float qx = active ? ax[q] : 0.f, qy = active ? ay[q] : 0.f, qz = active ? az[q] : 0.f;
@@ -69,6 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_knn_brute(uint32_t nA, const float* 
             tx[threadIdx.x] = bx[t];
             ty[threadIdx.x] = by[t];
             tz[threadIdx.x] = bz[t];
+            tid[threadIdx.x] = bids[t];
         }
         __syncthreads();
         const uint32_t m = min((uint32_t)kBlock, nB - base);
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_knn_brute(uint32_t nA, const float* 
                 const uint32_t b = base + j;
                 if (self && b == q) continue;
                 const float d2 = sqdist(qx, qy, qz, tx[j], ty[j], tz[j]);
-                if (h.accepts(d2, b)) h.push(d2, b);
+                if (h.accepts(d2, tid[j])) h.push(d2, tid[j], b);
             }
         }
         __syncthreads();
@@ -85,24 +88,40 @@ __global__ __launch_bounds__(kBlock) void k_knn_brute(uint32_t nA, const float* 
     if (active) h.store(k, nA, q, out_idx, out_dist);
 }
 
+// Moves a [k][nA] result from the previous storage order of A and/or B to the
+// current one: row s_old -> row_map[s_old], value v_old -> val_map[v_old].
+__global__ __launch_bounds__(kBlock) void k_knn_remap(uint32_t nA, uint32_t k, const uint32_t* __restrict__ in_idx,
+                                                      const float* __restrict__ in_dist,
+                                                      const uint32_t* __restrict__ row_map,
+                                                      const uint32_t* __restrict__ val_map,
+                                                      uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nA) return;
+    const uint32_t r = row_map ? row_map[s] : s;
+    for (uint32_t j = 0; j < k; ++j) {
+        uint32_t v = in_idx[(size_t)j * nA + s];
+        if (v != UINT32_MAX && val_map) v = val_map[v];
+        out_idx[(size_t)j * nA + r] = v;
+        out_dist[(size_t)j * nA + r] = in_dist[(size_t)j * nA + s];
+    }
+}
+
 template <int KMAX>
-static void run_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, const uint32_t* seed) {
+static void run_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, const uint32_t* seed,
+                       const uint32_t* row_map, const uint32_t* val_map) {
     const uint32_t nA = A->n;
     k_knn_binned<KMAX><<<blocks_for(nA), kBlock, 0, A->ctx->stream>>>(
-        nA, A->spx.p, A->spy.p, A->spz.p, A->perm.p, B->n, B->spx.p, B->spy.p, B->spz.p, B->perm.p, B->start.p,
-        B->inv.p, B->bin_grid, A == B ? 1 : 0, out->k, (int)R0, seed, out->idx.p,
-        out->dist.p);
+        nA, A->spx.p, A->spy.p, A->spz.p, B->n, B->spx.p, B->spy.p, B->spz.p, B->ids(), B->start.p, B->bin_grid,
+        A == B ? 1 : 0, out->k, (int)R0, seed, row_map, val_map, out->idx_alt(), out->dist_alt());
     GS_LAUNCHED("k_knn_binned");
 }
 
-void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, bool use_seed) {
+void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, const uint32_t* seed,
+                       const uint32_t* row_map, const uint32_t* val_map) {
     if (A->n == 0) return;
-    const uint32_t* seed = use_seed ? out->idx.p : nullptr;
-    // The seed is read and the result written in the same buffer: every lane
-    // reads only its own column (query q) before writing it.
-    if (out->k <= 8) run_binned<8>(A, out, B, search_radius, seed);
-    else if (out->k <= 16) run_binned<16>(A, out, B, search_radius, seed);
-    else run_binned<32>(A, out, B, search_radius, seed);
+    if (out->k <= 8) run_binned<8>(A, out, B, search_radius, seed, row_map, val_map);
+    else if (out->k <= 16) run_binned<16>(A, out, B, search_radius, seed, row_map, val_map);
+    else run_binned<32>(A, out, B, search_radius, seed, row_map, val_map);
 }
 
 void launch_knn_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B) {
@@ -111,13 +130,21 @@ void launch_knn_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B) {
     hipStream_t st = A->ctx->stream;
     const int self = A == B ? 1 : 0;
 #define GS_BRUTE(KM)                                                                                              \
-    k_knn_brute<KM><<<blocks_for(nA), kBlock, 0, st>>>(nA, A->f[0].p, A->f[1].p, A->f[2].p, B->n, B->f[0].p,     \
-                                                       B->f[1].p, B->f[2].p, self, out->k, out->idx.p, out->dist.p)
+    k_knn_brute<KM><<<blocks_for(nA), kBlock, 0, st>>>(nA, A->F(0), A->F(1), A->F(2), B->n, B->F(0), B->F(1),    \
+                                                       B->F(2), B->ids(), self, out->k, out->idx_alt(),          \
+                                                       out->dist_alt())
     if (out->k <= 8) GS_BRUTE(8);
     else if (out->k <= 16) GS_BRUTE(16);
     else GS_BRUTE(32);
 #undef GS_BRUTE
     GS_LAUNCHED("k_knn_brute");
+}
+
+void launch_knn_remap(gs_knn_s* out, uint32_t nA, const uint32_t* row_map, const uint32_t* val_map) {
+    if (nA == 0) return;
+    k_knn_remap<<<blocks_for(nA), kBlock, 0, out->ctx->stream>>>(nA, out->k, out->idx(), out->dist(), row_map,
+                                                                 val_map, out->idx_alt(), out->dist_alt());
+    GS_LAUNCHED("k_knn_remap");
 }
 
 }  // namespace gs

@@ -1,30 +1,38 @@
-// Exact KNN building blocks shared by the stand-alone KNN kernel and the
+// Exact KNN building blocks shared by the stand-alone KNN kernels and the
 // fused step kernel: register top-k list, lattice distance bounds, seeded
-// shell search.
+// shell search.  Points are addressed by storage slot; ties are broken by
+// agent index (the API's index), which each group keeps per slot.
 #pragma once
 #include "gs_internal.hpp"
 
 namespace gs {
 
+// Sorted list of KMAX (d2, agent index, slot) entries, ascending by
+// (d2, agent index), in registers.  For k < KMAX the first KMAX-k entries hold
+// a sentinel below every real key, so the last entry is always the k-th best
+// and no register is ever indexed dynamically.
 template <int KMAX> struct TopK {
     float d[KMAX];
     uint32_t i[KMAX];
+    uint32_t t[KMAX];
     __device__ __forceinline__ void init(uint32_t k) {
 #pragma unroll
         for (int j = 0; j < KMAX; ++j) {
             const bool sentinel = j < KMAX - (int)k;
             d[j] = sentinel ? -1.f : INFINITY;
             i[j] = sentinel ? 0u : UINT32_MAX;
+            t[j] = UINT32_MAX;
         }
     }
     __device__ __forceinline__ float worst() const { return d[KMAX - 1]; }
     __device__ __forceinline__ bool accepts(float dd, uint32_t ii) const {
         return dd < d[KMAX - 1] || (dd == d[KMAX - 1] && ii < i[KMAX - 1]);
     }
-    // Caller guarantees accepts(dd, ii).
-    __device__ __forceinline__ void push(float dd, uint32_t ii) {
+    // Caller guarantees accepts(dd, ii).  Branch-free compare-exchange ripple.
+    __device__ __forceinline__ void push(float dd, uint32_t ii, uint32_t tt) {
         d[KMAX - 1] = dd;
         i[KMAX - 1] = ii;
+        t[KMAX - 1] = tt;
 #pragma unroll
         for (int j = KMAX - 2; j >= 0; --j) {
             const bool sw = d[j + 1] < d[j] || (d[j + 1] == d[j] && i[j + 1] < i[j]);
@@ -32,20 +40,26 @@ template <int KMAX> struct TopK {
             const float hi_d = sw ? d[j] : d[j + 1];
             const uint32_t lo_i = sw ? i[j + 1] : i[j];
             const uint32_t hi_i = sw ? i[j] : i[j + 1];
+            const uint32_t lo_t = sw ? t[j + 1] : t[j];
+            const uint32_t hi_t = sw ? t[j] : t[j + 1];
             d[j] = lo_d;
             d[j + 1] = hi_d;
             i[j] = lo_i;
             i[j + 1] = hi_i;
+            t[j] = lo_t;
+            t[j + 1] = hi_t;
         }
     }
-    __device__ __forceinline__ void store(uint32_t k, uint32_t nA, uint32_t q, uint32_t* __restrict__ out_idx,
+    // Writes column `row` of the slot-major [k][nA] output: neighbour slots
+    // and distances (pads UINT32_MAX / FLT_MAX).
+    __device__ __forceinline__ void store(uint32_t k, uint32_t nA, uint32_t row, uint32_t* __restrict__ out_idx,
                                           float* __restrict__ out_dist) const {
 #pragma unroll
         for (int j = 0; j < KMAX; ++j) {
             if (j >= KMAX - (int)k) {
-                const size_t o = (size_t)(j - (KMAX - (int)k)) * nA + q;
-                const bool pad = i[j] == UINT32_MAX;
-                out_idx[o] = i[j];
+                const size_t o = (size_t)(j - (KMAX - (int)k)) * nA + row;
+                const bool pad = t[j] == UINT32_MAX;
+                out_idx[o] = t[j];
                 out_dist[o] = pad ? FLT_MAX : sqrtf(d[j]);
             }
         }
@@ -62,31 +76,33 @@ __device__ __forceinline__ float axis_gap(float q, int i0, int i1, float o, floa
     return fmaxf(gap - slack, 0.f);
 }
 
-// Largest d2 from the query to the k previous neighbours, measured on the
-// same sorted snapshot the search scans (binv = sorted slot of each point),
-// or +inf when the seed is incomplete.
-__device__ __forceinline__ float seed_bound(const uint32_t* __restrict__ seed, uint32_t k, uint32_t nA, uint32_t q,
-                                            uint32_t nB, float qx, float qy, float qz,
-                                            const uint32_t* __restrict__ binv, const float* __restrict__ bspx,
-                                            const float* __restrict__ bspy, const float* __restrict__ bspz) {
+// Largest d2 from the query to the k neighbours of its previous exact result
+// (a valid upper bound on the current k-th distance: they are k distinct
+// candidates), or +inf when that result is incomplete.  The previous result
+// may be held at the previous storage order of A (row = its old slot) and of
+// B (val_map translates old B slots to current ones).
+__device__ __forceinline__ float seed_bound(const uint32_t* __restrict__ seed, uint32_t k, uint32_t nA, uint32_t row,
+                                            uint32_t nB, const uint32_t* __restrict__ val_map, float qx, float qy,
+                                            float qz, const float* __restrict__ bx, const float* __restrict__ by,
+                                            const float* __restrict__ bz) {
     float m = 0.f;
     for (uint32_t j = 0; j < k; ++j) {
-        const uint32_t b = seed[(size_t)j * nA + q];
-        if (b >= nB) return INFINITY;
-        const uint32_t t = binv[b];
-        m = fmaxf(m, sqdist(qx, qy, qz, bspx[t], bspy[t], bspz[t]));
+        uint32_t v = seed[(size_t)j * nA + row];
+        if (v >= nB) return INFINITY;
+        if (val_map) v = val_map[v];
+        m = fmaxf(m, sqdist(qx, qy, qz, bx[v], by[v], bz[v]));
     }
     return m;
 }
 
-// Scans the lattice around (qx,qy,qz) until the top-k is exact.  self_slot is
-// the query's own sorted slot when A == B (excluded), else UINT32_MAX.
-// visit(t) is called for every candidate sorted slot t with d2 <= bound.
+// Scans the lattice around (qx,qy,qz) until the top-k is exact.  Points of B
+// are in storage (cell) order: cell c holds slots [bstart[c], bstart[c+1]);
+// bids gives each slot's agent index.  self_slot (A == B) is excluded.
 template <int KMAX>
 __device__ __forceinline__ void binned_search(TopK<KMAX>& h, float thr, float qx, float qy, float qz,
                                               uint32_t self_slot, const GridDev& g, int R0,
-                                              const float* __restrict__ bspx, const float* __restrict__ bspy,
-                                              const float* __restrict__ bspz, const uint32_t* __restrict__ bperm,
+                                              const float* __restrict__ bx, const float* __restrict__ by,
+                                              const float* __restrict__ bz, const uint32_t* __restrict__ bids,
                                               const uint32_t* __restrict__ bstart) {
     const int cx = axis_cell(qx, g.ox, g.sx, g.dx);
     const int cy = axis_cell(qy, g.oy, g.sy, g.dy);
@@ -121,11 +137,11 @@ __device__ __forceinline__ void binned_search(TopK<KMAX>& h, float thr, float qx
                     const uint32_t t1 = bstart[c0 + (uint32_t)(z1 - z0) + 1u];
                     for (uint32_t t = t0; t < t1; ++t) {
                         if (t == self_slot) continue;
-                        const float d2 = sqdist(qx, qy, qz, bspx[t], bspy[t], bspz[t]);
+                        const float d2 = sqdist(qx, qy, qz, bx[t], by[t], bz[t]);
                         if (d2 <= lim) {
-                            const uint32_t b = bperm[t];
-                            if (h.accepts(d2, b)) {
-                                h.push(d2, b);
+                            const uint32_t id = bids[t];
+                            if (h.accepts(d2, id)) {
+                                h.push(d2, id, t);
                                 lim = fminf(thr, h.worst());
                             }
                         }

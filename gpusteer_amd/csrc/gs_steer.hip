@@ -246,14 +246,13 @@ void launch_flock(gs_group_s* A, gs_knn_s* knn, gs_group_s* B, const gs_flock_te
     hipStream_t st = A->ctx->stream;
     const Agents a = A->dev_agents();
     if (mode == GS_MODE_GPUREF)
-        k_flock<GS_MODE_GPUREF><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, a, B->n, B->f[0].p, B->f[1].p, B->f[2].p,
-                                                                     B->f[3].p, B->f[4].p, B->f[5].p, knn->idx.p,
-                                                                     knn->k, term(sep), term(ali), term(coh));
+        k_flock<GS_MODE_GPUREF><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, a, B->n, B->F(0), B->F(1), B->F(2),
+                                                                     B->F(3), B->F(4), B->F(5), knn->idx(), knn->k,
+                                                                     term(sep), term(ali), term(coh));
     else
-        k_flock<GS_MODE_OPENSTEER><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, a, B->n, B->f[0].p, B->f[1].p,
-                                                                        B->f[2].p, B->f[3].p, B->f[4].p, B->f[5].p,
-                                                                        knn->idx.p, knn->k, term(sep), term(ali),
-                                                                        term(coh));
+        k_flock<GS_MODE_OPENSTEER><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, a, B->n, B->F(0), B->F(1), B->F(2),
+                                                                        B->F(3), B->F(4), B->F(5), knn->idx(),
+                                                                        knn->k, term(sep), term(ali), term(coh));
     GS_LAUNCHED("k_flock");
 }
 
@@ -270,16 +269,18 @@ void launch_seek(gs_group_s* A, const float* t, float w, uint32_t mask, int mode
 
 void launch_avoid_obstacles(gs_group_s* A, gs_group_s* O, gs_knn_s* knn, float min_time, float w, uint32_t mask) {
     if (A->n == 0) return;
-    k_avoid_obstacles<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->dev_agents(), O->n, O->f[0].p,
-                                                                        O->f[1].p, O->f[2].p, O->f[19].p, knn->idx.p,
+    k_avoid_obstacles<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->dev_agents(), O->n, O->F(0),
+                                                                        O->F(1), O->F(2), O->F(19), knn->idx(),
                                                                         knn->k, min_time, w, mask);
     GS_LAUNCHED("k_avoid_obstacles");
 }
 
 void launch_avoid_walls(gs_group_s* A, gs_knn_s* knn, gs_group_s* W, float min_time, float w, uint32_t mask) {
     if (A->n == 0) return;
-    gs_walls wd{W->w[0].p, W->w[1].p, W->w[2].p, W->w[3].p, W->w[4].p, W->w[5].p, W->w[6].p, W->w[7].p, W->w[8].p};
-    k_avoid_walls<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->dev_agents(), W->n, wd, knn->idx.p,
+    const int f0 = kWallField;
+    gs_walls wd{W->F(f0), W->F(f0 + 1), W->F(f0 + 2), W->F(f0 + 3), W->F(f0 + 4),
+                W->F(f0 + 5), W->F(f0 + 6), W->F(f0 + 7), W->F(f0 + 8)};
+    k_avoid_walls<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->dev_agents(), W->n, wd, knn->idx(),
                                                                     knn->k, min_time, w, mask);
     GS_LAUNCHED("k_avoid_walls");
 }
@@ -296,14 +297,14 @@ void launch_update(gs_group_s* A, float dt, int mode) {
 
 void launch_wrap(gs_group_s* A, int shape, const float* c, const float* s) {
     if (A->n == 0) return;
-    k_wrap<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->f[0].p, A->f[1].p, A->f[2].p, shape, c[0], c[1],
-                                                            c[2], s[0], s[1], s[2]);
+    k_wrap<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->F(0), A->F(1), A->F(2), shape, c[0], c[1], c[2],
+                                                            s[0], s[1], s[2]);
     GS_LAUNCHED("k_wrap");
 }
 
 void launch_clear_bits(gs_group_s* A) {
     if (A->n == 0) return;
-    k_clear_bits<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->bits.p);
+    k_clear_bits<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->bits());
     GS_LAUNCHED("k_clear_bits");
 }
 

@@ -160,8 +160,14 @@ int gs_group_size(gs_group g, uint32_t* n);
 /* Agents: every pointer of *host must be non-NULL (n > 0).  Synchronous. */
 int gs_group_upload_agents(gs_group g, uint32_t n, const gs_agents* host);
 int gs_group_download_agents(gs_group g, gs_agents* host);
-/* Device pointers of the agent arrays (valid until the next upload). */
+/* Device pointers of the agent arrays, in storage order: each bin build
+ * (gs_update_knn_database, gs_step_boids) reorders a group's state into
+ * cell order, so storage slot s holds agent id_of_slot[s] (see
+ * gs_group_device_order).  Valid until the next build or upload. */
 int gs_group_device_agents(gs_group g, gs_agents* dev);
+/* The group's storage order: agent index of each slot and slot of each agent
+ * index (device arrays of n entries; valid until the next build or upload). */
+int gs_group_device_order(gs_group g, uint32_t** id_of_slot, uint32_t** slot_of_id);
 /* Obstacles: centre and radius per obstacle (ObstacleGroup::AddObstacle). */
 int gs_group_upload_obstacles(gs_group g, uint32_t n, const float* cx, const float* cy, const float* cz,
                               const float* radius);
@@ -180,9 +186,13 @@ int gs_group_check_finite(gs_group g, uint32_t* first_bad);
 int gs_knn_create(gs_context ctx, uint32_t n, uint32_t k, gs_knn* out);
 int gs_knn_destroy(gs_knn knn);
 int gs_knn_shape(gs_knn knn, uint32_t* n, uint32_t* k);
-/* Slot-major [k][n]: idx[j*n + i] is the j-th nearest of agent i;
- * pads are UINT32_MAX / FLT_MAX.  Either output may be NULL. */
+/* Slot-major [k][n] in agent indices: idx[j*n + i] is the agent index (in
+ * B) of the j-th nearest neighbour of agent i (of A); pads are UINT32_MAX /
+ * FLT_MAX.  Either output may be NULL. */
 int gs_knn_download(gs_knn knn, uint32_t* idx, float* dist);
+/* Device buffers of the current result in storage slots ([k][n]: row = slot
+ * of the query in A, value = slot of the neighbour in B).  Marks the
+ * container's contents as no longer trusted by later calls. */
 int gs_knn_device(gs_knn knn, uint32_t** idx, float** dist);
 
 /* ---- operator API (GroupSteerLibrary.cuh) ---------------------------------------- */
@@ -230,6 +240,17 @@ typedef struct {
  * separation, alignment, cohesion -> integrate.  Same results as the separate
  * calls in that order. */
 int gs_step_boids(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* params, uint32_t steps);
+
+/* gs_step_boids with HIP-event timing recorded on the context's stream:
+ * average milliseconds per step of the bin build (key, scan, scatter,
+ * canonical order) and of the fused KNN + flocking + integrate kernel. */
+typedef struct {
+    float bin_build_ms;
+    float fused_ms;
+    float step_ms;
+} gs_boids_timing;
+int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* params, uint32_t steps,
+                        gs_boids_timing* timing);
 
 #ifdef __cplusplus
 }
