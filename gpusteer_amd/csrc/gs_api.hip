@@ -389,9 +389,7 @@ static void release_group(gs_group_s* g) {
     gs::DevBuf<uint32_t>* u[] = {&g->slot_of_id, &g->new_to_old, &g->old_to_new, &g->key, &g->rank,
                                  &g->perm_tmp,   &g->start,      &g->count,      &g->flag};
     for (auto* b : u) b->release();
-    g->spx.release();
-    g->spy.release();
-    g->spz.release();
+    g->spos.release();
 }
 
 int gs_group_destroy(gs_group g) {
@@ -549,10 +547,13 @@ int gs_group_download_bins(gs_group g, uint32_t* key, uint32_t* start, uint32_t*
         const size_t C = g->bin_cells;
         auto st_h = d2h_vec(g->start.p, C + 1, st);
         auto ids = d2h_vec(g->ids(), n, st);
-        if (spx) d2h(spx, g->spx.p, n * 4, st);
-        if (spy) d2h(spy, g->spy.p, n * 4, st);
-        if (spz) d2h(spz, g->spz.p, n * 4, st);
+        auto sp = d2h_vec(g->spos.p, n, st);
         GS_HIP(hipStreamSynchronize(st));
+        for (size_t t = 0; t < n; ++t) {
+            if (spx) spx[t] = sp[t].x;
+            if (spy) spy[t] = sp[t].y;
+            if (spz) spz[t] = sp[t].z;
+        }
         if (start) memcpy(start, st_h.data(), (C + 1) * 4);
         if (perm) memcpy(perm, ids.data(), n * 4);  // agent index of each sorted slot
         if (key)
@@ -678,8 +679,7 @@ int gs_update_knn_database(gs_group g, gs_bins bins) {
         check(bins, "bins");
         if (bins->ctx != g->ctx) fail(GS_ERR_INVALID_ARGUMENT, "bins belong to another context");
         use_device(g->ctx);
-        launch_bins_build(g, bins, -1, nullptr, nullptr, true);
-        g->snap_valid = true;
+        launch_bins_build(g, bins, -1, nullptr, nullptr);
         after_reorder(g);
     });
 }
@@ -825,11 +825,11 @@ static void check_boids_args(gs_group A, gs_knn knn, gs_bins bins, const gs_boid
         fail(GS_ERR_INVALID_ARGUMENT, "unknown wrap shape");
 }
 
-// One fused step: build + reorder (no snapshot needed: the fused kernel reads
-// the current buffers and writes positions / forwards to the alternates).
+// One fused step: build + reorder, then the fused kernel (it reads the current
+// buffers and the snapshot, and writes positions / forwards to the
+// alternates, so the snapshot no longer matches the live positions).
 static void boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const gs_boids_params* p) {
-    launch_bins_build(A, bins, p->wrap_shape, p->wrap_center, p->wrap_size, false);
-    A->snap_valid = false;
+    launch_bins_build(A, bins, p->wrap_shape, p->wrap_center, p->wrap_size);
     after_reorder(A);
     const Seed sd = seed_for(knn, A, A, p->knn_flags);
     launch_boids_fused(A, knn, p, sd.seed, sd.row_map, sd.val_map);
@@ -861,8 +861,7 @@ int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_par
         double bin_ms = 0, fused_ms = 0;
         for (uint32_t s = 0; s < steps; ++s) {
             GS_HIP(hipEventRecord(ev[0], st));
-            launch_bins_build(A, bins, p->wrap_shape, p->wrap_center, p->wrap_size, false);
-            A->snap_valid = false;
+            launch_bins_build(A, bins, p->wrap_shape, p->wrap_center, p->wrap_size);
             after_reorder(A);
             GS_HIP(hipEventRecord(ev[1], st));
             const Seed sd = seed_for(knn, A, A, p->knn_flags);

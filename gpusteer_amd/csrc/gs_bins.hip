@@ -199,8 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* 
                                                     uint32_t* __restrict__ new_to_old,
                                                     uint32_t* __restrict__ old_to_new,
                                                     const uint32_t* __restrict__ bits, uint32_t* __restrict__ bits_new,
-                                                    FieldSet fs, float* __restrict__ spx, float* __restrict__ spy,
-                                                    float* __restrict__ spz) {
+                                                    FieldSet fs, float4* __restrict__ spos) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t s = perm_tmp[t];
@@ -222,16 +221,11 @@ __global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* 
 #pragma unroll
     for (int f = 0; f < kFields; ++f)
         if (f < fs.nf) fs.dst[f][d] = fs.src[f][s];
-    if (spx) {
-        // fields 0..2 (positions) are always first in the set
-        spx[d] = fs.src[0][s];
-        spy[d] = fs.src[1][s];
-        spz[d] = fs.src[2][s];
-    }
+    // fields 0..2 (positions) are always first in the set
+    spos[d] = make_float4(fs.src[0][s], fs.src[1][s], fs.src[2][s], __uint_as_float(id));
 }
 
-void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws,
-                       bool snapshot) {
+void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws) {
     const uint32_t n = g->n;
     const uint32_t C = b->cells;
     hipStream_t st = g->ctx->stream;
@@ -241,11 +235,7 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
     g->perm_tmp.reserve(n);
     g->new_to_old.reserve(n);
     g->old_to_new.reserve(n);
-    if (snapshot) {
-        g->spx.reserve(n);
-        g->spy.reserve(n);
-        g->spz.reserve(n);
-    }
+    g->spos.reserve(n);
     const uint32_t ntiles = (C + kScanTile - 1) / kScanTile;
     g->start.reserve((size_t)C + 1 + ntiles);  // partial tile sums live after start[C]
     if (new_cells) {
@@ -288,9 +278,7 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
         k_reorder<<<blocks_for(n), kBlock, 0, st>>>(n, g->perm_tmp.p, g->key.p, g->start.p, g->ids(), g->ids_alt(),
                                                     g->slot_of_id.p, g->new_to_old.p, g->old_to_new.p,
                                                     has_bits ? g->bits() : nullptr,
-                                                    has_bits ? g->bitsb[g->bcur ^ 1].p : nullptr, fs,
-                                                    snapshot ? g->spx.p : nullptr, snapshot ? g->spy.p : nullptr,
-                                                    snapshot ? g->spz.p : nullptr);
+                                                    has_bits ? g->bitsb[g->bcur ^ 1].p : nullptr, fs, g->spos.p);
         GS_LAUNCHED("k_reorder");
         for (int f = 0; f < kFields; ++f)
             if (g->live & (1u << f)) g->flip(f);
@@ -298,6 +286,7 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
         g->icur ^= 1;
     }
     g->binned = true;
+    g->snap_valid = true;
 }
 
 }  // namespace gs

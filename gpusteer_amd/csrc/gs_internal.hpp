@@ -174,13 +174,13 @@ struct gs_group_s {
     // Bin database (KNNDatabase, include/OpenSteer/CUDA/KNNDatabase.cuh:30-38):
     // after a build, cell c holds storage slots [start[c], start[c+1]).
     bool binned = false;
-    bool snap_valid = false;  // spx/spy/spz hold the positions of the last build
+    bool snap_valid = false;  // spos holds the positions of the last build
     gs::GridDev bin_grid{};
     uint32_t bin_cells = 0;
     gs::DevBuf<uint32_t> key, rank, perm_tmp, start, count;
-    // Positions at the last build, in storage order (what stand-alone KNN
-    // calls search, like the reference's positionSorted).
-    gs::DevBuf<float> spx, spy, spz;
+    // Positions at the last build in storage order, packed (x, y, z, agent
+    // index bits): what KNN searches, like the reference's positionSorted.
+    gs::DevBuf<float4> spos;
     gs::DevBuf<uint32_t> flag;  // small device scratch (finite check)
 
     float* F(int j) const { return fb[j][fcur[j]].p; }
@@ -219,9 +219,8 @@ namespace gs {
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t st);
 // Bin build + reorder of g into cell order (optionally wrapping positions
-// first); snapshot = also write the position snapshot spx/spy/spz.
-void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws,
-                       bool snapshot);
+// first); also writes the position snapshot spos.
+void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws);
 // KNN into out's alternate buffer (caller flips).  row_map / val_map translate
 // a seed held at the previous storage order of A / B (NULL: same order).
 void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, const uint32_t* seed,

@@ -32,20 +32,22 @@ namespace gs {
 // One lane per query slot s of A (storage = cell order, so a wavefront's lanes
 // share cells and their candidate runs overlap in cache).
 template <int KMAX>
-__global__ __launch_bounds__(kBlock) void k_knn_binned(
-    uint32_t nA, const float* __restrict__ ax, const float* __restrict__ ay, const float* __restrict__ az,
-    uint32_t nB, const float* __restrict__ bx, const float* __restrict__ by, const float* __restrict__ bz,
-    const uint32_t* __restrict__ bids, const uint32_t* __restrict__ bstart, GridDev g, int self, uint32_t k, int R0,
-    const uint32_t* __restrict__ seed, const uint32_t* __restrict__ row_map, const uint32_t* __restrict__ val_map,
-    uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+__global__ __launch_bounds__(kBlock) void k_knn_binned(uint32_t nA, const float4* __restrict__ apos, uint32_t nB,
+                                                       const float4* __restrict__ bpos,
+                                                       const uint32_t* __restrict__ bstart, GridDev g, int self,
+                                                       uint32_t k, int R0, const uint32_t* __restrict__ seed,
+                                                       const uint32_t* __restrict__ row_map,
+                                                       const uint32_t* __restrict__ val_map,
+                                                       uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+    __shared__ uint2 lbuf[kKnnBuf * kBlock];
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nA) return;
-    const float qx = ax[s], qy = ay[s], qz = az[s];
+    const float4 q = apos[s];
     TopK<KMAX> h;
     h.init(k);
-    const float thr = seed ? seed_bound(seed, k, nA, row_map ? row_map[s] : s, nB, val_map, qx, qy, qz, bx, by, bz)
-                           : INFINITY;
-    binned_search<KMAX>(h, thr, qx, qy, qz, self ? s : UINT32_MAX, g, R0, bx, by, bz, bids, bstart);
+    const float thr =
+        seed ? seed_bound(seed, k, nA, row_map ? row_map[s] : s, nB, val_map, q.x, q.y, q.z, bpos) : INFINITY;
+    knn_search<KMAX>(h, thr, q.x, q.y, q.z, self ? s : UINT32_MAX, g, R0, bpos, bstart, lbuf + threadIdx.x);
     h.store(k, nA, s, out_idx, out_dist);
 }
 
@@ -111,8 +113,8 @@ static void run_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0,
                        const uint32_t* row_map, const uint32_t* val_map) {
     const uint32_t nA = A->n;
     k_knn_binned<KMAX><<<blocks_for(nA), kBlock, 0, A->ctx->stream>>>(
-        nA, A->spx.p, A->spy.p, A->spz.p, B->n, B->spx.p, B->spy.p, B->spz.p, B->ids(), B->start.p, B->bin_grid,
-        A == B ? 1 : 0, out->k, (int)R0, seed, row_map, val_map, out->idx_alt(), out->dist_alt());
+        nA, A->spos.p, B->n, B->spos.p, B->start.p, B->bin_grid, A == B ? 1 : 0, out->k, (int)R0, seed, row_map,
+        val_map, out->idx_alt(), out->dist_alt());
     GS_LAUNCHED("k_knn_binned");
 }
 

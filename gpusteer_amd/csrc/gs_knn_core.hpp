@@ -1,11 +1,16 @@
 // Exact KNN building blocks shared by the stand-alone KNN kernels and the
 // fused step kernel: register top-k list, lattice distance bounds, seeded
-// shell search.  Points are addressed by storage slot; ties are broken by
-// agent index (the API's index), which each group keeps per slot.
+// search with deferred insertion.  Points are addressed by storage slot; ties
+// are broken by agent index (the API's index), which the position snapshot
+// carries in its fourth component.
 #pragma once
 #include "gs_internal.hpp"
 
 namespace gs {
+
+// Candidates a lane may defer (per-lane LDS buffer) before the wave merges
+// them into the top-k lists.
+constexpr int kKnnBuf = 16;
 
 // Sorted list of KMAX (d2, agent index, slot) entries, ascending by
 // (d2, agent index), in registers.  For k < KMAX the first KMAX-k entries hold
@@ -66,6 +71,11 @@ template <int KMAX> struct TopK {
     }
 };
 
+__device__ __forceinline__ uint32_t id_of(const float4& p) { return __float_as_uint(p.w); }
+__device__ __forceinline__ float sqdist4(float qx, float qy, float qz, const float4& p) {
+    return sqdist(qx, qy, qz, p.x, p.y, p.z);
+}
+
 // Distance from coordinate q to the slab of cells [i0, i1] along one axis;
 // boundary cells extend to infinity outward (points outside the lattice are
 // clamped into them).  The rounding slack makes the gap an under-estimate.
@@ -83,32 +93,62 @@ __device__ __forceinline__ float axis_gap(float q, int i0, int i1, float o, floa
 // B (val_map translates old B slots to current ones).
 __device__ __forceinline__ float seed_bound(const uint32_t* __restrict__ seed, uint32_t k, uint32_t nA, uint32_t row,
                                             uint32_t nB, const uint32_t* __restrict__ val_map, float qx, float qy,
-                                            float qz, const float* __restrict__ bx, const float* __restrict__ by,
-                                            const float* __restrict__ bz) {
+                                            float qz, const float4* __restrict__ bpos) {
     float m = 0.f;
     for (uint32_t j = 0; j < k; ++j) {
         uint32_t v = seed[(size_t)j * nA + row];
         if (v >= nB) return INFINITY;
         if (val_map) v = val_map[v];
-        m = fmaxf(m, sqdist(qx, qy, qz, bx[v], by[v], bz[v]));
+        m = fmaxf(m, sqdist4(qx, qy, qz, bpos[v]));
     }
     return m;
 }
 
-// Scans the lattice around (qx,qy,qz) until the top-k is exact.  Points of B
-// are in storage (cell) order: cell c holds slots [bstart[c], bstart[c+1]);
-// bids gives each slot's agent index.  self_slot (A == B) is excluded.
+// Distance from the query to the nearest cell outside the box of Chebyshev
+// radius R around its cell; `all` when that box covers the whole lattice.
+__device__ __forceinline__ float outside_gap(const GridDev& g, int cx, int cy, int cz, int R, float qx, float qy,
+                                             float qz, bool& all) {
+    float dmin = INFINITY;
+    all = true;
+    if (cx - R > 0) { dmin = fminf(dmin, qx - (g.ox + (float)(cx - R) * g.hx)); all = false; }
+    if (cx + R < g.dx - 1) { dmin = fminf(dmin, (g.ox + (float)(cx + R + 1) * g.hx) - qx); all = false; }
+    if (cy - R > 0) { dmin = fminf(dmin, qy - (g.oy + (float)(cy - R) * g.hy)); all = false; }
+    if (cy + R < g.dy - 1) { dmin = fminf(dmin, (g.oy + (float)(cy + R + 1) * g.hy) - qy); all = false; }
+    if (cz - R > 0) { dmin = fminf(dmin, qz - (g.oz + (float)(cz - R) * g.hz)); all = false; }
+    if (cz + R < g.dz - 1) { dmin = fminf(dmin, (g.oz + (float)(cz + R + 1) * g.hz) - qz); all = false; }
+    return dmin - g.slack;
+}
+__device__ __forceinline__ bool exact_after(float lim, float dmin) {
+    return dmin > 0.f && lim < dmin * dmin * (1.0f - 2e-5f);
+}
+
+// Merges every lane's deferred candidates into its list.  Called with the
+// whole wavefront converged; iterates as long as any lane has entries left.
 template <int KMAX>
-__device__ __forceinline__ void binned_search(TopK<KMAX>& h, float thr, float qx, float qy, float qz,
-                                              uint32_t self_slot, const GridDev& g, int R0,
-                                              const float* __restrict__ bx, const float* __restrict__ by,
-                                              const float* __restrict__ bz, const uint32_t* __restrict__ bids,
-                                              const uint32_t* __restrict__ bstart) {
-    const int cx = axis_cell(qx, g.ox, g.sx, g.dx);
-    const int cy = axis_cell(qy, g.oy, g.sy, g.dy);
-    const int cz = axis_cell(qz, g.oz, g.sz, g.dz);
+__device__ __forceinline__ void flush_deferred(TopK<KMAX>& h, int& cnt, const uint2* lbuf,
+                                               const float4* __restrict__ bpos) {
+    for (int e = 0; e < kKnnBuf; ++e) {
+        if (!__any(e < cnt)) break;
+        if (e < cnt) {
+            const uint2 v = lbuf[e * kBlock];
+            const float d2 = __uint_as_float(v.x);
+            const uint32_t id = id_of(bpos[v.y]);
+            if (h.accepts(d2, id)) h.push(d2, id, v.y);
+        }
+    }
+    cnt = 0;
+}
+
+// Generic shells R >= Rstart (rare path): scan shells of cells of growing
+// Chebyshev radius, each shell as contiguous z-runs of storage slots, skip
+// runs farther than the bound, stop once the k-th best is below the distance
+// to every unscanned cell.
+template <int KMAX>
+__device__ void shell_search(TopK<KMAX>& h, float thr, float qx, float qy, float qz, uint32_t self_slot,
+                             const GridDev& g, int cx, int cy, int cz, int Rstart, int R0,
+                             const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart) {
     const float grow = 1.0f + 2e-5f;
-    for (int R = 0;; ++R) {
+    for (int R = Rstart;; ++R) {
         const int x0 = max(cx - R, 0), x1 = min(cx + R, g.dx - 1);
         const int y0 = max(cy - R, 0), y1 = min(cy + R, g.dy - 1);
         for (int ix = x0; ix <= x1; ++ix) {
@@ -118,8 +158,6 @@ __device__ __forceinline__ void binned_search(TopK<KMAX>& h, float thr, float qx
                 const bool edge = xedge || (iy == cy - R) || (iy == cy + R);
                 const float gy = axis_gap(qy, iy, iy, g.oy, g.hy, g.dy, g.slack);
                 const float gxy = gx * gx + gy * gy;
-                // z-runs of this column inside shell R: the whole [cz-R, cz+R]
-                // on an edge column, else its two end cells.
                 for (int run = 0; run < (edge ? 1 : 2); ++run) {
                     int z0, z1;
                     if (edge) {
@@ -136,34 +174,87 @@ __device__ __forceinline__ void binned_search(TopK<KMAX>& h, float thr, float qx
                     const uint32_t t0 = bstart[c0];
                     const uint32_t t1 = bstart[c0 + (uint32_t)(z1 - z0) + 1u];
                     for (uint32_t t = t0; t < t1; ++t) {
-                        if (t == self_slot) continue;
-                        const float d2 = sqdist(qx, qy, qz, bx[t], by[t], bz[t]);
-                        if (d2 <= lim) {
-                            const uint32_t id = bids[t];
-                            if (h.accepts(d2, id)) {
-                                h.push(d2, id, t);
-                                lim = fminf(thr, h.worst());
-                            }
+                        const float4 p = bpos[t];
+                        const float d2 = sqdist4(qx, qy, qz, p);
+                        if (d2 <= lim && t != self_slot && h.accepts(d2, id_of(p))) {
+                            h.push(d2, id_of(p), t);
+                            lim = fminf(thr, h.worst());
                         }
                     }
                 }
             }
         }
         if (R < R0) continue;
-        // Stop when every unscanned cell is farther than the k-th best.
-        float dmin = INFINITY;
-        bool all = true;
-        if (cx - R > 0) { dmin = fminf(dmin, qx - (g.ox + (float)(cx - R) * g.hx)); all = false; }
-        if (cx + R < g.dx - 1) { dmin = fminf(dmin, (g.ox + (float)(cx + R + 1) * g.hx) - qx); all = false; }
-        if (cy - R > 0) { dmin = fminf(dmin, qy - (g.oy + (float)(cy - R) * g.hy)); all = false; }
-        if (cy + R < g.dy - 1) { dmin = fminf(dmin, (g.oy + (float)(cy + R + 1) * g.hy) - qy); all = false; }
-        if (cz - R > 0) { dmin = fminf(dmin, qz - (g.oz + (float)(cz - R) * g.hz)); all = false; }
-        if (cz + R < g.dz - 1) { dmin = fminf(dmin, (g.oz + (float)(cz + R + 1) * g.hz) - qz); all = false; }
-        if (all) break;
-        dmin -= g.slack;
-        const float lim = fminf(thr, h.worst());
-        if (dmin > 0.f && lim < dmin * dmin * (1.0f - 2e-5f)) break;
+        bool all;
+        const float dmin = outside_gap(g, cx, cy, cz, R, qx, qy, qz, all);
+        if (all || exact_after(fminf(thr, h.worst()), dmin)) break;
     }
+}
+
+// Exact top-k of the query among the points of B.  B is in storage (cell)
+// order: cell c holds slots [bstart[c], bstart[c+1]); bpos[t] =
+// (x, y, z, agent index bits).  self_slot (A == B) is excluded.
+//
+// Fast path (wave-uniform outer loop): the 3 x 3 columns around the query's
+// cell, each a z-run of up to 3 cells whose end cells are culled against the
+// bound.  Candidates within the bound are appended to the lane's LDS buffer
+// (lbuf, stride kBlock) and merged into the list by the whole wavefront at
+// column boundaries, so the insertion ripple runs about once per accepted
+// candidate instead of once per candidate of any lane.  If the 3 x 3 x 3 box
+// does not prove the list exact, the rare path continues with shells R >= 2.
+template <int KMAX>
+__device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, float qy, float qz,
+                                           uint32_t self_slot, const GridDev& g, int R0,
+                                           const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart,
+                                           uint2* lbuf) {
+    const int cx = axis_cell(qx, g.ox, g.sx, g.dx);
+    const int cy = axis_cell(qy, g.oy, g.sy, g.dy);
+    const int cz = axis_cell(qz, g.oz, g.sz, g.dz);
+    const float grow = 1.0f + 2e-5f;
+    int cnt = 0;
+    float lim = fminf(thr, h.worst());
+    const float gz_mid = axis_gap(qz, cz, cz, g.oz, g.hz, g.dz, g.slack);
+    const float gz_lo = axis_gap(qz, cz - 1, cz - 1, g.oz, g.hz, g.dz, g.slack);
+    const float gz_hi = axis_gap(qz, cz + 1, cz + 1, g.oz, g.hz, g.dz, g.slack);
+#pragma unroll 1
+    for (int col = 0; col < 9; ++col) {
+        const int ix = cx + col / 3 - 1, iy = cy + col % 3 - 1;
+        if (ix >= 0 && ix < g.dx && iy >= 0 && iy < g.dy) {
+            const float gx = axis_gap(qx, ix, ix, g.ox, g.hx, g.dx, g.slack);
+            const float gy = axis_gap(qy, iy, iy, g.oy, g.hy, g.dy, g.slack);
+            const float gxy = gx * gx + gy * gy;
+            if (gxy + gz_mid * gz_mid <= lim * grow) {
+                int z0 = cz, z1 = cz;
+                if (cz > 0 && gxy + gz_lo * gz_lo <= lim * grow) z0 = cz - 1;
+                if (cz + 1 < g.dz && gxy + gz_hi * gz_hi <= lim * grow) z1 = cz + 1;
+                const uint32_t c0 = cell_index(g, ix, iy, z0);
+                const uint32_t t0 = bstart[c0];
+                const uint32_t t1 = bstart[c0 + (uint32_t)(z1 - z0) + 1u];
+                for (uint32_t t = t0; t < t1; ++t) {
+                    const float4 p = bpos[t];
+                    const float d2 = sqdist4(qx, qy, qz, p);
+                    if (d2 <= lim && t != self_slot) {
+                        if (cnt < kKnnBuf) {
+                            lbuf[cnt * kBlock] = make_uint2(__float_as_uint(d2), t);
+                            ++cnt;
+                        } else if (h.accepts(d2, id_of(p))) {
+                            h.push(d2, id_of(p), t);
+                            lim = fminf(thr, h.worst());
+                        }
+                    }
+                }
+            }
+        }
+        if (__any(cnt >= kKnnBuf / 2)) {
+            flush_deferred<KMAX>(h, cnt, lbuf, bpos);
+            lim = fminf(thr, h.worst());
+        }
+    }
+    flush_deferred<KMAX>(h, cnt, lbuf, bpos);
+    bool all;
+    const float dmin = outside_gap(g, cx, cy, cz, 1, qx, qy, qz, all);
+    if (R0 <= 1 && (all || exact_after(fminf(thr, h.worst()), dmin))) return;
+    shell_search<KMAX>(h, thr, qx, qy, qz, self_slot, g, cx, cy, cz, 2, R0, bpos, bstart);
 }
 
 }  // namespace gs

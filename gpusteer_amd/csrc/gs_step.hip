@@ -23,27 +23,28 @@ template <int KMAX, int MODE>
 __global__ __launch_bounds__(kBlock) void k_boids(uint32_t n, Agents A, float* __restrict__ npx,
                                                   float* __restrict__ npy, float* __restrict__ npz,
                                                   float* __restrict__ nfx, float* __restrict__ nfy,
-                                                  float* __restrict__ nfz, const uint32_t* __restrict__ ids,
+                                                  float* __restrict__ nfz, const float4* __restrict__ spos,
                                                   const uint32_t* __restrict__ start, GridDev g, uint32_t k, int R0,
                                                   const uint32_t* __restrict__ seed,
                                                   const uint32_t* __restrict__ row_map,
                                                   const uint32_t* __restrict__ val_map,
                                                   uint32_t* __restrict__ out_idx, float* __restrict__ out_dist,
                                                   FlockTerm sep, FlockTerm ali, FlockTerm coh, float dt) {
+    __shared__ uint2 lbuf[kKnnBuf * kBlock];
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     // A's current buffers are the step-t snapshot (just built in cell order);
     // new positions and forward vectors go to the alternate buffers.
-    const f3 pos = ld3(A.px, A.py, A.pz, s);
+    const float4 q = spos[s];
+    const f3 pos = mk3(q.x, q.y, q.z);
     const f3 fwd = ld3(A.fx, A.fy, A.fz, s);
 
     // Exact KNN within the group (self excluded by slot).
     TopK<KMAX> h;
     h.init(k);
-    const float thr = seed ? seed_bound(seed, k, n, row_map ? row_map[s] : s, n, val_map, pos.x, pos.y, pos.z, A.px,
-                                        A.py, A.pz)
-                           : INFINITY;
-    binned_search<KMAX>(h, thr, pos.x, pos.y, pos.z, s, g, R0, A.px, A.py, A.pz, ids, start);
+    const float thr =
+        seed ? seed_bound(seed, k, n, row_map ? row_map[s] : s, n, val_map, pos.x, pos.y, pos.z, spos) : INFINITY;
+    knn_search<KMAX>(h, thr, pos.x, pos.y, pos.z, s, g, R0, spos, start, lbuf + threadIdx.x);
     h.store(k, n, s, out_idx, out_dist);
 
     // Flocking from the snapshot in slot order.  The list's entries
@@ -56,7 +57,8 @@ __global__ __launch_bounds__(kBlock) void k_boids(uint32_t n, Agents A, float* _
                                if (j < KMAX - (int)k) return (int)kNbrSkip;
                                const uint32_t t = h.t[j];
                                if (t == UINT32_MAX) return (int)kNbrStop;
-                               bp = ld3(A.px, A.py, A.pz, t);
+                               const float4 b = spos[t];
+                               bp = mk3(b.x, b.y, b.z);
                                if (want_fwd) bf = ld3(A.fx, A.fy, A.fz, t);
                                return (int)kNbrValid;
                            });
@@ -88,7 +90,7 @@ static void run_boids(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, co
     const Agents a = A->dev_agents();
 #define GS_BOIDS(MODE)                                                                                               \
     k_boids<KMAX, MODE><<<blocks_for(n), kBlock, 0, st>>>(                                                           \
-        n, a, A->Falt(0), A->Falt(1), A->Falt(2), A->Falt(3), A->Falt(4), A->Falt(5), A->ids(), A->start.p,         \
+        n, a, A->Falt(0), A->Falt(1), A->Falt(2), A->Falt(3), A->Falt(4), A->Falt(5), A->spos.p, A->start.p,        \
         A->bin_grid, knn->k, (int)p->search_radius, seed, row_map, val_map, knn->idx_alt(), knn->dist_alt(),        \
         fterm(p->separation), fterm(p->alignment), fterm(p->cohesion), p->dt)
     if (p->mode == GS_MODE_GPUREF) GS_BOIDS(GS_MODE_GPUREF);
