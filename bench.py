@@ -9,7 +9,7 @@ path over the group: wrap -> bin build (counting sort) -> exact KNN ->
 flocking forces -> integrate, inputs resident in HBM.
 
 Prints one JSON line (the driver's contract).  `roofline` is computed for the
-dominant kernel (the fused KNN + flocking + integrate kernel) from its
+dominant kernel (the exact KNN kernel) from its
 HIP-event-timed average duration on the library's stream; `cpu_baseline`
 times the CPU oracle restatement of the same step on this host's cores on a
 bounded sample (a few full steps of the same 2^20-agent state).
@@ -36,10 +36,10 @@ def bytes_per_agent_step(k: int, cells: int, n: int) -> float:
     return 224.0 + 8.0 * k + 12.0 * cells / n
 
 
-def bytes_fused_kernel(k: int) -> float:
-    """Algorithmic bytes per agent of the fused KNN + steering + integrate
-    kernel: KNN (12 + 4k) + steering (40 + 4k) + integrate 92."""
-    return 144.0 + 8.0 * k
+def bytes_knn_kernel(k: int) -> float:
+    """Algorithmic bytes per agent of the KNN kernel: read the sorted
+    position (12) and write k neighbour indices (4k)."""
+    return 12.0 + 4.0 * k
 
 
 def cpu_baseline(st, p, seconds: float):
@@ -105,7 +105,7 @@ def main() -> int:
         el = time.perf_counter() - t0
         # Per-stage HIP-event timing on the library's stream.
         tsteps = min(args.steps, 50)
-        bin_ms, fused_ms, step_ms = gs.stepBoidsTimed(A, knn, bins, p, tsteps)
+        bin_ms, knn_ms, si_ms, step_ms = gs.stepBoidsTimed(A, knn, bins, p, tsteps)
         status, bad = A.check_finite()
         A.close()
         bins.close()
@@ -115,10 +115,10 @@ def main() -> int:
         return 1
     ms = el * 1e3 / args.steps
     value = n * args.steps / el
-    b_fused = bytes_fused_kernel(args.k)
-    achieved = n * b_fused / (fused_ms * 1e-3) / 1e9
-    print(f"bench: {ms:.3f} ms/step, bin build {bin_ms:.3f} ms, fused kernel {fused_ms:.3f} ms "
-          f"(event-timed step {step_ms:.3f} ms); whole-step effective "
+    b_knn = bytes_knn_kernel(args.k)
+    achieved = n * b_knn / (knn_ms * 1e-3) / 1e9
+    print(f"bench: {ms:.3f} ms/step, bin build {bin_ms:.3f} ms, knn {knn_ms:.3f} ms, steer+integrate "
+          f"{si_ms:.3f} ms (event-timed step {step_ms:.3f} ms); whole-step effective "
           f"{n * bytes_per_agent_step(args.k, C, n) / (ms * 1e-3) / 1e9:.1f} GB/s", file=sys.stderr, flush=True)
     out = {
         "metric": "agent-steps/s at 1M agents k=7",
@@ -143,15 +143,16 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_boids (fused exact KNN + flocking + integrate)",
+            "kernel": "k_knn_binned (exact seeded KNN)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "bytes_per_agent": b_fused,
-            "kernel_ms": fused_ms,
+            "bytes_per_agent": b_knn,
+            "kernel_ms": knn_ms,
         },
+        "stage_ms": {"bin_build": bin_ms, "knn": knn_ms, "steer_integrate": si_ms},
         "whole_step_effective_gbs": n * bytes_per_agent_step(args.k, C, n) / (ms * 1e-3) / 1e9,
     }
     if not args.no_cpu:

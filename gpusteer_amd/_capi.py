@@ -87,7 +87,8 @@ class GsBoidsParams(C.Structure):
 
 
 class GsBoidsTiming(C.Structure):
-    _fields_ = [("bin_build_ms", C.c_float), ("fused_ms", C.c_float), ("step_ms", C.c_float)]
+    _fields_ = [("bin_build_ms", C.c_float), ("knn_ms", C.c_float), ("steer_integrate_ms", C.c_float),
+                ("step_ms", C.c_float)]
 
 
 class GpuSteerError(RuntimeError):

@@ -311,9 +311,9 @@ def stepBoids(A: AgentGroup, knn: KNNData, bins: KNNBinData, params, steps: int 
 
 def stepBoidsTimed(A: AgentGroup, knn: KNNData, bins: KNNBinData, params, steps: int = 1, knn_flags: int = 0):
     """stepBoids with HIP-event timing on the library's stream; returns
-    (bin_build_ms, fused_ms, step_ms) averaged per step."""
+    (bin_build_ms, knn_ms, steer_integrate_ms, step_ms) averaged per step."""
     s = boids_params_struct(params)
     s.knn_flags = knn_flags
     t = capi.GsBoidsTiming()
     check(capi.lib.gs_step_boids_timed(A.h, knn.h, bins.h, C.byref(s), steps, C.byref(t)), "gs_step_boids_timed")
-    return t.bin_build_ms, t.fused_ms, t.step_ms
+    return t.bin_build_ms, t.knn_ms, t.steer_integrate_ms, t.step_ms

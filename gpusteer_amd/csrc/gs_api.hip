@@ -387,7 +387,7 @@ static void release_group(gs_group_s* g) {
     for (auto& b : g->bitsb) b.release();
     for (auto& b : g->idb) b.release();
     gs::DevBuf<uint32_t>* u[] = {&g->slot_of_id, &g->new_to_old, &g->old_to_new, &g->key, &g->rank,
-                                 &g->perm_tmp,   &g->start,      &g->count,      &g->flag};
+                                 &g->perm_tmp,   &g->id_tmp, &g->start,      &g->count,      &g->flag};
     for (auto* b : u) b->release();
     g->spos.release();
 }
@@ -427,7 +427,6 @@ static void begin_members(gs_group_s* g, uint32_t n) {
     g->idb[1].reserve(n);
     g->slot_of_id.reserve(n);
     launch_iota(g->ids(), n, g->ctx->stream);
-    launch_iota(g->slot_of_id.p, n, g->ctx->stream);
     g->n = n;
     g->binned = false;
     g->snap_valid = false;
@@ -494,7 +493,12 @@ int gs_group_device_order(gs_group g, uint32_t** id_of_slot, uint32_t** slot_of_
     return guarded([&] {
         check_group(g, "group");
         if (id_of_slot) *id_of_slot = g->ids();
-        if (slot_of_id) *slot_of_id = g->slot_of_id.p;
+        if (slot_of_id) {
+            // Not maintained by the hot path: inverted on request.
+            use_device(g->ctx);
+            launch_invert_ids(g);
+            *slot_of_id = g->slot_of_id.p;
+        }
     });
 }
 
@@ -825,63 +829,58 @@ static void check_boids_args(gs_group A, gs_knn knn, gs_bins bins, const gs_boid
         fail(GS_ERR_INVALID_ARGUMENT, "unknown wrap shape");
 }
 
-// One fused step: build + reorder, then the fused kernel (it reads the current
-// buffers and the snapshot, and writes positions / forwards to the
-// alternates, so the snapshot no longer matches the live positions).
-static void boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const gs_boids_params* p) {
+// One Boids step: build + reorder (wrap fused), seeded exact KNN of A with
+// itself, then flocking + integrate (new positions / forwards into the
+// alternate buffers, flipped afterwards).
+static void boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const gs_boids_params* p,
+                            hipEvent_t* ev) {
+    hipStream_t st = A->ctx->stream;
+    if (ev) GS_HIP(hipEventRecord(ev[0], st));
     launch_bins_build(A, bins, p->wrap_shape, p->wrap_center, p->wrap_size);
     after_reorder(A);
+    if (ev) GS_HIP(hipEventRecord(ev[1], st));
     const Seed sd = seed_for(knn, A, A, p->knn_flags);
-    launch_boids_fused(A, knn, p, sd.seed, sd.row_map, sd.val_map);
-    if (A->n) {
-        knn->cur ^= 1;
-        for (int j = 0; j < 6; ++j) A->flip(j);
-    }
+    launch_knn_binned(A, knn, A, p->search_radius, sd.seed, sd.row_map, sd.val_map);
+    if (A->n) knn->cur ^= 1;
     mark_result(knn, A, A);
+    if (ev) GS_HIP(hipEventRecord(ev[2], st));
+    launch_flock_update(A, knn, p);
+    if (A->n)
+        for (int j = 0; j < 6; ++j) A->flip(j);
+    if (ev) GS_HIP(hipEventRecord(ev[3], st));
 }
 
 int gs_step_boids(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* p, uint32_t steps) {
     return guarded([&] {
         check_boids_args(A, knn, bins, p);
         use_device(A->ctx);
-        for (uint32_t s = 0; s < steps; ++s) boids_step_once(A, knn, bins, p);
+        for (uint32_t s = 0; s < steps; ++s) boids_step_once(A, knn, bins, p, nullptr);
     });
 }
 
 int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* p, uint32_t steps,
                         gs_boids_timing* timing) {
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int status = guarded([&] {
         check_boids_args(A, knn, bins, p);
         if (!timing) fail(GS_ERR_INVALID_ARGUMENT, "timing is NULL");
         if (steps == 0) fail(GS_ERR_INVALID_ARGUMENT, "steps must be at least 1");
         use_device(A->ctx);
-        hipStream_t st = A->ctx->stream;
         for (auto& e : ev) GS_HIP(hipEventCreate(&e));
-        double bin_ms = 0, fused_ms = 0;
+        double t[3] = {0, 0, 0};
         for (uint32_t s = 0; s < steps; ++s) {
-            GS_HIP(hipEventRecord(ev[0], st));
-            launch_bins_build(A, bins, p->wrap_shape, p->wrap_center, p->wrap_size);
-            after_reorder(A);
-            GS_HIP(hipEventRecord(ev[1], st));
-            const Seed sd = seed_for(knn, A, A, p->knn_flags);
-            launch_boids_fused(A, knn, p, sd.seed, sd.row_map, sd.val_map);
-            if (A->n) {
-                knn->cur ^= 1;
-                for (int j = 0; j < 6; ++j) A->flip(j);
+            boids_step_once(A, knn, bins, p, ev);
+            GS_HIP(hipEventSynchronize(ev[3]));
+            for (int j = 0; j < 3; ++j) {
+                float ms = 0;
+                GS_HIP(hipEventElapsedTime(&ms, ev[j], ev[j + 1]));
+                t[j] += ms;
             }
-            mark_result(knn, A, A);
-            GS_HIP(hipEventRecord(ev[2], st));
-            GS_HIP(hipEventSynchronize(ev[2]));
-            float a = 0, b = 0;
-            GS_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
-            GS_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
-            bin_ms += a;
-            fused_ms += b;
         }
-        timing->bin_build_ms = (float)(bin_ms / steps);
-        timing->fused_ms = (float)(fused_ms / steps);
-        timing->step_ms = (float)((bin_ms + fused_ms) / steps);
+        timing->bin_build_ms = (float)(t[0] / steps);
+        timing->knn_ms = (float)(t[1] / steps);
+        timing->steer_integrate_ms = (float)(t[2] / steps);
+        timing->step_ms = (float)((t[0] + t[1] + t[2]) / steps);
     });
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);

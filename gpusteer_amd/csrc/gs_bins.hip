@@ -37,6 +37,18 @@ __global__ __launch_bounds__(kBlock) void k_iota(uint32_t* __restrict__ p, uint3
     if (i < n) p[i] = i;
 }
 
+__global__ __launch_bounds__(kBlock) void k_invert(uint32_t n, const uint32_t* __restrict__ ids,
+                                                   uint32_t* __restrict__ slot_of_id) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) slot_of_id[ids[s]] = s;
+}
+
+void launch_invert_ids(gs_group_s* g) {
+    if (g->n == 0) return;
+    k_invert<<<blocks_for(g->n), kBlock, 0, g->ctx->stream>>>(g->n, g->ids(), g->slot_of_id.p);
+    GS_LAUNCHED("k_invert");
+}
+
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st) {
     if (n == 0) return;
     k_fill_u32<<<blocks_for(n), kBlock, 0, st>>>(p, n, v);
@@ -55,39 +67,56 @@ __global__ __launch_bounds__(kBlock) void k_bin_key(uint32_t n, float* __restric
                                                     int wrap_shape, float wcx, float wcy, float wcz, float wsx,
                                                     float wsy, float wsz) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float x = px[i], y = py[i], z = pz[i];
-    if (wrap_shape == GS_WRAP_BOX) {
-        // WrapWorldKernel (CUDA/WrapWorldCUDAKernel.cu:25-35), all three axes.
-        const float hx = wsx * 0.5f, hy = wsy * 0.5f, hz = wsz * 0.5f;
-        const float x0 = x, y0 = y, z0 = z;
-        if (x < wcx - hx) x = wcx + hx;
-        else if (x > wcx + hx) x = wcx - hx;
-        if (y < wcy - hy) y = wcy + hy;
-        else if (y > wcy + hy) y = wcy - hy;
-        if (z < wcz - hz) z = wcz + hz;
-        else if (z > wcz + hz) z = wcz - hz;
-        if (x != x0) px[i] = x;
-        if (y != y0) py[i] = y;
-        if (z != z0) pz[i] = z;
-    } else if (wrap_shape == GS_WRAP_SPHERE) {
-        // sphericalWrapAround (plugins/Boids.cpp:200-206).
-        const f3 off = mk3(x - wcx, y - wcy, z - wcz);
-        const float r = len3(off);
-        if (r > wsx) {
-            const f3 p = mk3(x, y, z) + ((off / r) * wsx) * -2.f;
-            x = p.x;
-            y = p.y;
-            z = p.z;
-            px[i] = x;
-            py[i] = y;
-            pz[i] = z;
+    const bool active = i < n;
+    const int lane = threadIdx.x & 63;
+    // Inactive lanes get a key no cell has, so they form segments of their own.
+    uint32_t c = 0xFFFFFFC0u | (uint32_t)lane;
+    if (active) {
+        float x = px[i], y = py[i], z = pz[i];
+        if (wrap_shape == GS_WRAP_BOX) {
+            // WrapWorldKernel (CUDA/WrapWorldCUDAKernel.cu:25-35), all three axes.
+            const float hx = wsx * 0.5f, hy = wsy * 0.5f, hz = wsz * 0.5f;
+            const float x0 = x, y0 = y, z0 = z;
+            if (x < wcx - hx) x = wcx + hx;
+            else if (x > wcx + hx) x = wcx - hx;
+            if (y < wcy - hy) y = wcy + hy;
+            else if (y > wcy + hy) y = wcy - hy;
+            if (z < wcz - hz) z = wcz + hz;
+            else if (z > wcz + hz) z = wcz - hz;
+            if (x != x0) px[i] = x;
+            if (y != y0) py[i] = y;
+            if (z != z0) pz[i] = z;
+        } else if (wrap_shape == GS_WRAP_SPHERE) {
+            // sphericalWrapAround (plugins/Boids.cpp:200-206).
+            const f3 off = mk3(x - wcx, y - wcy, z - wcz);
+            const float r = len3(off);
+            if (r > wsx) {
+                const f3 p = mk3(x, y, z) + ((off / r) * wsx) * -2.f;
+                x = p.x;
+                y = p.y;
+                z = p.z;
+                px[i] = x;
+                py[i] = y;
+                pz[i] = z;
+            }
         }
+        c = cell_index(g, axis_cell(x, g.ox, g.sx, g.dx), axis_cell(y, g.oy, g.sy, g.dy),
+                       axis_cell(z, g.oz, g.sz, g.dz));
+        key[i] = c;
     }
-    const uint32_t c = cell_index(g, axis_cell(x, g.ox, g.sx, g.dx), axis_cell(y, g.oy, g.sy, g.dy),
-                                  axis_cell(z, g.oz, g.sz, g.dz));
-    key[i] = c;
-    rank[i] = atomicAdd(&count[c], 1u);
+    // Storage is in cell order from the previous build, so equal cells sit in
+    // runs of lanes: one atomic per run, ranks inside the run in lane order.
+    const uint32_t prev = __shfl_up(c, 1, 64);
+    const bool head = lane == 0 || prev != c;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const int head_lane = 63 - __clzll((unsigned long long)(heads & upto));
+    const uint64_t after = heads & ~upto;
+    const int run_end = after ? (__ffsll((unsigned long long)after) - 1) : 64;
+    uint32_t base = 0;
+    if (head && active) base = atomicAdd(&count[c], (uint32_t)(run_end - lane));
+    base = __shfl(base, head_lane, 64);
+    if (active) rank[i] = base + (uint32_t)(lane - head_lane);
 }
 
 // Wavefront-then-block exclusive scan of one value per thread.
@@ -172,13 +201,17 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t C, const uint32_t*
 __global__ __launch_bounds__(kBlock) void k_scatter(uint32_t n, const uint32_t* __restrict__ key,
                                                     const uint32_t* __restrict__ rank,
                                                     const uint32_t* __restrict__ start,
-                                                    uint32_t* __restrict__ perm_tmp) {
+                                                    const uint32_t* __restrict__ ids,
+                                                    uint32_t* __restrict__ perm_tmp, uint32_t* __restrict__ id_tmp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     // start[key]+rank < n whenever the counts were zero before the key pass;
     // the guard keeps a corrupted count array from writing out of bounds.
     const uint32_t d = start[key[i]] + rank[i];
-    if (d < n) perm_tmp[d] = i;
+    if (d < n) {
+        perm_tmp[d] = i;
+        id_tmp[d] = ids[i];
+    }
 }
 
 // Field pointers of one reorder (current -> alternate buffers).
@@ -192,10 +225,10 @@ struct FieldSet {
 // order).  Its final position is the cell start plus the number of members of
 // its cell with a smaller agent index.
 __global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* __restrict__ perm_tmp,
+                                                    const uint32_t* __restrict__ id_tmp,
                                                     const uint32_t* __restrict__ key,
                                                     const uint32_t* __restrict__ start,
                                                     const uint32_t* __restrict__ ids, uint32_t* __restrict__ ids_new,
-                                                    uint32_t* __restrict__ slot_of_id,
                                                     uint32_t* __restrict__ new_to_old,
                                                     uint32_t* __restrict__ old_to_new,
                                                     const uint32_t* __restrict__ bits, uint32_t* __restrict__ bits_new,
@@ -206,17 +239,16 @@ __global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* 
     if (s >= n) return;  // only after a corrupted build (see k_scatter)
     const uint32_t c = key[s];
     const uint32_t b = start[c], e = min(start[c + 1], n);
-    const uint32_t id = ids[s];
+    const uint32_t id = id_tmp[t];
     uint32_t d = t;
     if (e - b <= kCanonCap) {
         uint32_t r = 0;
-        for (uint32_t j = b; j < e; ++j) r += (ids[perm_tmp[j]] < id) ? 1u : 0u;
+        for (uint32_t j = b; j < e; ++j) r += (id_tmp[j] < id) ? 1u : 0u;
         d = b + r;
     }
     new_to_old[d] = s;
     old_to_new[s] = d;
     ids_new[d] = id;
-    slot_of_id[id] = d;
     if (bits) bits_new[d] = bits[s];
 #pragma unroll
     for (int f = 0; f < kFields; ++f)
@@ -233,6 +265,7 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
     g->key.reserve(n);
     g->rank.reserve(n);
     g->perm_tmp.reserve(n);
+    g->id_tmp.reserve(n);
     g->new_to_old.reserve(n);
     g->old_to_new.reserve(n);
     g->spos.reserve(n);
@@ -263,7 +296,8 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
     k_scan_add<<<ntiles, kBlock, 0, st>>>(C, partial, g->start.p, g->count.p);
     GS_LAUNCHED("k_scan_add");
     if (n > 0) {
-        k_scatter<<<blocks_for(n), kBlock, 0, st>>>(n, g->key.p, g->rank.p, g->start.p, g->perm_tmp.p);
+        k_scatter<<<blocks_for(n), kBlock, 0, st>>>(n, g->key.p, g->rank.p, g->start.p, g->ids(), g->perm_tmp.p,
+                                                    g->id_tmp.p);
         GS_LAUNCHED("k_scatter");
         FieldSet fs{};
         int nf = 0;
@@ -275,8 +309,8 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
             }
         fs.nf = nf;
         const bool has_bits = g->kind == GS_KIND_AGENTS;
-        k_reorder<<<blocks_for(n), kBlock, 0, st>>>(n, g->perm_tmp.p, g->key.p, g->start.p, g->ids(), g->ids_alt(),
-                                                    g->slot_of_id.p, g->new_to_old.p, g->old_to_new.p,
+        k_reorder<<<blocks_for(n), kBlock, 0, st>>>(n, g->perm_tmp.p, g->id_tmp.p, g->key.p, g->start.p, g->ids(), g->ids_alt(),
+                                                    g->new_to_old.p, g->old_to_new.p,
                                                     has_bits ? g->bits() : nullptr,
                                                     has_bits ? g->bitsb[g->bcur ^ 1].p : nullptr, fs, g->spos.p);
         GS_LAUNCHED("k_reorder");

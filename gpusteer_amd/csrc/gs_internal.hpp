@@ -177,7 +177,7 @@ struct gs_group_s {
     bool snap_valid = false;  // spos holds the positions of the last build
     gs::GridDev bin_grid{};
     uint32_t bin_cells = 0;
-    gs::DevBuf<uint32_t> key, rank, perm_tmp, start, count;
+    gs::DevBuf<uint32_t> key, rank, perm_tmp, id_tmp, start, count;
     // Positions at the last build in storage order, packed (x, y, z, agent
     // index bits): what KNN searches, like the reference's positionSorted.
     gs::DevBuf<float4> spos;
@@ -238,9 +238,10 @@ void launch_update(gs_group_s* A, float dt, int mode);
 void launch_wrap(gs_group_s* A, int shape, const float* c, const float* s);
 void launch_clear_bits(gs_group_s* A);
 void launch_check_finite(gs_group_s* A, uint32_t* first_bad_slot_host);
-// Fused KNN + flocking + integrate after a build; writes the KNN into out's
-// alternate buffer (caller flips) and the new positions / forwards into the
-// alternate field buffers (caller flips fields 0..5).
-void launch_boids_fused(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, const uint32_t* seed,
-                        const uint32_t* row_map, const uint32_t* val_map);
+// Flocking + integrate of the Boids step from a current KNN result of A with
+// itself; new positions / forwards go to the alternate field buffers (caller
+// flips fields 0..5).
+void launch_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p);
+// Inverts the id map into slot_of_id.
+void launch_invert_ids(gs_group_s* g);
 }  // namespace gs

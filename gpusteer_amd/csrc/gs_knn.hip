@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kBlock) void k_knn_binned(uint32_t nA, const float4
     TopK<KMAX> h;
     h.init(k);
     const float thr =
-        seed ? seed_bound(seed, k, nA, row_map ? row_map[s] : s, nB, val_map, q.x, q.y, q.z, bpos) : INFINITY;
+        seed ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB, val_map, q.x, q.y, q.z, bpos) : INFINITY;
     knn_search<KMAX>(h, thr, q.x, q.y, q.z, self ? s : UINT32_MAX, g, R0, bpos, bstart, lbuf + threadIdx.x);
     h.store(k, nA, s, out_idx, out_dist);
 }
@@ -81,8 +81,8 @@ __global__ __launch_bounds__(kBlock) void k_knn_brute(uint32_t nA, const float* 
             for (uint32_t j = 0; j < m; ++j) {
                 const uint32_t b = base + j;
                 if (self && b == q) continue;
-                const float d2 = sqdist(qx, qy, qz, tx[j], ty[j], tz[j]);
-                if (h.accepts(d2, tid[j])) h.push(d2, tid[j], b);
+                const uint64_t kk = make_key(sqdist(qx, qy, qz, tx[j], ty[j], tz[j]), tid[j]);
+                if (h.accepts(kk)) h.push(kk, b);
             }
         }
         __syncthreads();

@@ -12,45 +12,42 @@ namespace gs {
 // them into the top-k lists.
 constexpr int kKnnBuf = 16;
 
-// Sorted list of KMAX (d2, agent index, slot) entries, ascending by
-// (d2, agent index), in registers.  For k < KMAX the first KMAX-k entries hold
-// a sentinel below every real key, so the last entry is always the k-th best
-// and no register is ever indexed dynamically.
+// Sorted list of KMAX entries in registers: a 64-bit key (fp32 bits of d2 in
+// the high word -- monotonic for d2 >= 0 -- and the agent index in the low
+// word, so one unsigned compare orders by (d2, agent index)) plus the
+// neighbour's storage slot.  For k < KMAX the first KMAX-k entries hold the
+// sentinel key 0, below every real key, so the last entry is always the k-th
+// best and no register is ever indexed dynamically.  Empty entries hold
+// (+inf, UINT32_MAX).
+constexpr uint64_t kKeyEmpty = (0x7F800000ull << 32) | 0xFFFFFFFFull;
+__device__ __forceinline__ uint64_t make_key(float d2, uint32_t id) {
+    return ((uint64_t)__float_as_uint(d2) << 32) | id;
+}
 template <int KMAX> struct TopK {
-    float d[KMAX];
-    uint32_t i[KMAX];
+    uint64_t key[KMAX];
     uint32_t t[KMAX];
     __device__ __forceinline__ void init(uint32_t k) {
 #pragma unroll
         for (int j = 0; j < KMAX; ++j) {
-            const bool sentinel = j < KMAX - (int)k;
-            d[j] = sentinel ? -1.f : INFINITY;
-            i[j] = sentinel ? 0u : UINT32_MAX;
+            key[j] = (j < KMAX - (int)k) ? 0ull : kKeyEmpty;
             t[j] = UINT32_MAX;
         }
     }
-    __device__ __forceinline__ float worst() const { return d[KMAX - 1]; }
-    __device__ __forceinline__ bool accepts(float dd, uint32_t ii) const {
-        return dd < d[KMAX - 1] || (dd == d[KMAX - 1] && ii < i[KMAX - 1]);
-    }
-    // Caller guarantees accepts(dd, ii).  Branch-free compare-exchange ripple.
-    __device__ __forceinline__ void push(float dd, uint32_t ii, uint32_t tt) {
-        d[KMAX - 1] = dd;
-        i[KMAX - 1] = ii;
+    __device__ __forceinline__ float worst() const { return __uint_as_float((uint32_t)(key[KMAX - 1] >> 32)); }
+    __device__ __forceinline__ bool accepts(uint64_t kk) const { return kk < key[KMAX - 1]; }
+    // Caller guarantees accepts(kk).  Branch-free compare-exchange ripple.
+    __device__ __forceinline__ void push(uint64_t kk, uint32_t tt) {
+        key[KMAX - 1] = kk;
         t[KMAX - 1] = tt;
 #pragma unroll
         for (int j = KMAX - 2; j >= 0; --j) {
-            const bool sw = d[j + 1] < d[j] || (d[j + 1] == d[j] && i[j + 1] < i[j]);
-            const float lo_d = sw ? d[j + 1] : d[j];
-            const float hi_d = sw ? d[j] : d[j + 1];
-            const uint32_t lo_i = sw ? i[j + 1] : i[j];
-            const uint32_t hi_i = sw ? i[j] : i[j + 1];
+            const bool sw = key[j + 1] < key[j];
+            const uint64_t lo_k = sw ? key[j + 1] : key[j];
+            const uint64_t hi_k = sw ? key[j] : key[j + 1];
             const uint32_t lo_t = sw ? t[j + 1] : t[j];
             const uint32_t hi_t = sw ? t[j] : t[j + 1];
-            d[j] = lo_d;
-            d[j + 1] = hi_d;
-            i[j] = lo_i;
-            i[j + 1] = hi_i;
+            key[j] = lo_k;
+            key[j + 1] = hi_k;
             t[j] = lo_t;
             t[j + 1] = hi_t;
         }
@@ -65,7 +62,7 @@ template <int KMAX> struct TopK {
                 const size_t o = (size_t)(j - (KMAX - (int)k)) * nA + row;
                 const bool pad = t[j] == UINT32_MAX;
                 out_idx[o] = t[j];
-                out_dist[o] = pad ? FLT_MAX : sqrtf(d[j]);
+                out_dist[o] = pad ? FLT_MAX : sqrtf(__uint_as_float((uint32_t)(key[j] >> 32)));
             }
         }
     }
@@ -90,17 +87,30 @@ __device__ __forceinline__ float axis_gap(float q, int i0, int i1, float o, floa
 // (a valid upper bound on the current k-th distance: they are k distinct
 // candidates), or +inf when that result is incomplete.  The previous result
 // may be held at the previous storage order of A (row = its old slot) and of
-// B (val_map translates old B slots to current ones).
+// B (val_map translates old B slots to current ones).  Unrolled over KMAX so
+// all loads of a level are in flight together.
+template <int KMAX>
 __device__ __forceinline__ float seed_bound(const uint32_t* __restrict__ seed, uint32_t k, uint32_t nA, uint32_t row,
                                             uint32_t nB, const uint32_t* __restrict__ val_map, float qx, float qy,
                                             float qz, const float4* __restrict__ bpos) {
-    float m = 0.f;
-    for (uint32_t j = 0; j < k; ++j) {
-        uint32_t v = seed[(size_t)j * nA + row];
-        if (v >= nB) return INFINITY;
-        if (val_map) v = val_map[v];
-        m = fmaxf(m, sqdist4(qx, qy, qz, bpos[v]));
+    // Entries j >= k repeat entry k-1, so every load is unconditional.
+    uint32_t v[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) v[j] = seed[(size_t)min(j, (int)k - 1) * nA + row];
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) ok = ok && v[j] < nB;
+    if (!ok) return INFINITY;
+    if (val_map) {
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) v[j] = val_map[v[j]];
     }
+    float4 p[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) p[j] = bpos[v[j]];
+    float m = 0.f;
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) m = fmaxf(m, sqdist4(qx, qy, qz, p[j]));
     return m;
 }
 
@@ -131,9 +141,8 @@ __device__ __forceinline__ void flush_deferred(TopK<KMAX>& h, int& cnt, const ui
         if (!__any(e < cnt)) break;
         if (e < cnt) {
             const uint2 v = lbuf[e * kBlock];
-            const float d2 = __uint_as_float(v.x);
-            const uint32_t id = id_of(bpos[v.y]);
-            if (h.accepts(d2, id)) h.push(d2, id, v.y);
+            const uint64_t kk = ((uint64_t)v.x << 32) | id_of(bpos[v.y]);
+            if (h.accepts(kk)) h.push(kk, v.y);
         }
     }
     cnt = 0;
@@ -176,9 +185,12 @@ __device__ void shell_search(TopK<KMAX>& h, float thr, float qx, float qy, float
                     for (uint32_t t = t0; t < t1; ++t) {
                         const float4 p = bpos[t];
                         const float d2 = sqdist4(qx, qy, qz, p);
-                        if (d2 <= lim && t != self_slot && h.accepts(d2, id_of(p))) {
-                            h.push(d2, id_of(p), t);
-                            lim = fminf(thr, h.worst());
+                        if (d2 <= lim && t != self_slot) {
+                            const uint64_t kk = make_key(d2, id_of(p));
+                            if (h.accepts(kk)) {
+                                h.push(kk, t);
+                                lim = fminf(thr, h.worst());
+                            }
                         }
                     }
                 }
@@ -195,13 +207,15 @@ __device__ void shell_search(TopK<KMAX>& h, float thr, float qx, float qy, float
 // order: cell c holds slots [bstart[c], bstart[c+1]); bpos[t] =
 // (x, y, z, agent index bits).  self_slot (A == B) is excluded.
 //
-// Fast path (wave-uniform outer loop): the 3 x 3 columns around the query's
-// cell, each a z-run of up to 3 cells whose end cells are culled against the
-// bound.  Candidates within the bound are appended to the lane's LDS buffer
-// (lbuf, stride kBlock) and merged into the list by the whole wavefront at
-// column boundaries, so the insertion ripple runs about once per accepted
-// candidate instead of once per candidate of any lane.  If the 3 x 3 x 3 box
-// does not prove the list exact, the rare path continues with shells R >= 2.
+// Fast path: the 3 x 3 columns around the query's cell, each a z-run of up
+// to 3 cells whose end cells are culled against the bound.  All nine runs
+// are located first (their start loads in flight together); the column walk
+// is unrolled and wave-uniform, and candidates are loaded four at a time.
+// Candidates within the bound are appended to the lane's LDS buffer (lbuf,
+// stride kBlock) and merged into the list by the whole wavefront at column
+// boundaries, so the insertion ripple runs about once per accepted candidate
+// instead of once per candidate of any lane.  If the 3 x 3 x 3 box does not
+// prove the list exact, the rare path continues with shells R >= 2.
 template <int KMAX>
 __device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, float qy, float qz,
                                            uint32_t self_slot, const GridDev& g, int R0,
@@ -211,41 +225,62 @@ __device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, f
     const int cy = axis_cell(qy, g.oy, g.sy, g.dy);
     const int cz = axis_cell(qz, g.oz, g.sz, g.dz);
     const float grow = 1.0f + 2e-5f;
-    int cnt = 0;
     float lim = fminf(thr, h.worst());
+    const float bound = lim * grow;
     const float gz_mid = axis_gap(qz, cz, cz, g.oz, g.hz, g.dz, g.slack);
     const float gz_lo = axis_gap(qz, cz - 1, cz - 1, g.oz, g.hz, g.dz, g.slack);
     const float gz_hi = axis_gap(qz, cz + 1, cz + 1, g.oz, g.hz, g.dz, g.slack);
-#pragma unroll 1
+    // Locate the nine runs against the initial bound (tighter bounds found
+    // later only make the scan do more than needed, never less).
+    uint32_t r0[9], r1[9];
+#pragma unroll
     for (int col = 0; col < 9; ++col) {
         const int ix = cx + col / 3 - 1, iy = cy + col % 3 - 1;
+        r0[col] = r1[col] = 0u;
         if (ix >= 0 && ix < g.dx && iy >= 0 && iy < g.dy) {
             const float gx = axis_gap(qx, ix, ix, g.ox, g.hx, g.dx, g.slack);
             const float gy = axis_gap(qy, iy, iy, g.oy, g.hy, g.dy, g.slack);
             const float gxy = gx * gx + gy * gy;
-            if (gxy + gz_mid * gz_mid <= lim * grow) {
+            if (gxy + gz_mid * gz_mid <= bound) {
                 int z0 = cz, z1 = cz;
-                if (cz > 0 && gxy + gz_lo * gz_lo <= lim * grow) z0 = cz - 1;
-                if (cz + 1 < g.dz && gxy + gz_hi * gz_hi <= lim * grow) z1 = cz + 1;
+                if (cz > 0 && gxy + gz_lo * gz_lo <= bound) z0 = cz - 1;
+                if (cz + 1 < g.dz && gxy + gz_hi * gz_hi <= bound) z1 = cz + 1;
                 const uint32_t c0 = cell_index(g, ix, iy, z0);
-                const uint32_t t0 = bstart[c0];
-                const uint32_t t1 = bstart[c0 + (uint32_t)(z1 - z0) + 1u];
-                for (uint32_t t = t0; t < t1; ++t) {
-                    const float4 p = bpos[t];
-                    const float d2 = sqdist4(qx, qy, qz, p);
-                    if (d2 <= lim && t != self_slot) {
+                r0[col] = bstart[c0];
+                r1[col] = bstart[c0 + (uint32_t)(z1 - z0) + 1u];
+            }
+        }
+    }
+    int cnt = 0;
+#pragma unroll
+    for (int col = 0; col < 9; ++col) {
+        const uint32_t t1 = r1[col];
+        for (uint32_t t = r0[col]; t < t1; t += 4) {
+            // Unconditional loads at clamped (in-range) slots keep all four in
+            // flight together; only their use is predicated.
+            float4 p[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) p[u] = bpos[min(t + u, t1 - 1u)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (t + u < t1) {
+                    const float d2 = sqdist4(qx, qy, qz, p[u]);
+                    if (d2 <= lim && t + u != self_slot) {
                         if (cnt < kKnnBuf) {
-                            lbuf[cnt * kBlock] = make_uint2(__float_as_uint(d2), t);
+                            lbuf[cnt * kBlock] = make_uint2(__float_as_uint(d2), t + u);
                             ++cnt;
-                        } else if (h.accepts(d2, id_of(p))) {
-                            h.push(d2, id_of(p), t);
-                            lim = fminf(thr, h.worst());
+                        } else {
+                            const uint64_t kk = make_key(d2, id_of(p[u]));
+                            if (h.accepts(kk)) {
+                                h.push(kk, t + u);
+                                lim = fminf(thr, h.worst());
+                            }
                         }
                     }
                 }
             }
         }
-        if (__any(cnt >= kKnnBuf / 2)) {
+        if (col < 8 && __any(cnt >= kKnnBuf / 2)) {
             flush_deferred<KMAX>(h, cnt, lbuf, bpos);
             lim = fminf(thr, h.worst());
         }

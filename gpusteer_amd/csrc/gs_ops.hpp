@@ -40,15 +40,12 @@ __device__ __forceinline__ void apply(f3& steering, uint32_t& bits, f3 f, float 
 // of each other, so they are gathered in one pass over the neighbours and then
 // applied in the order separation, alignment, cohesion with the masks tested
 // against the bits as each previous term left them, exactly as three
-// consecutive launches would.  nbr(j, want_fwd, bp, bf) fetches neighbour
-// slot j and returns kNbrValid, kNbrStop at the first pad, or kNbrSkip for a
-// slot to pass over.  NSLOT == 0 walks j = 0..k-1; NSLOT > 0 walks
-// j = 0..NSLOT-1 fully unrolled (for register-resident neighbour lists).
-enum { kNbrValid = 0, kNbrStop = 1, kNbrSkip = 2 };
-template <int MODE, int NSLOT, class Nbr>
-__device__ __forceinline__ void flock_core(f3 pos, f3 fwd, float radius, uint32_t k, const FlockTerm& sep,
-                                           const FlockTerm& ali, const FlockTerm& coh, f3& steering,
-                                           uint32_t& bits, Nbr nbr) {
+// consecutive launches would.  bp / bf hold the first m neighbours' positions
+// and forward vectors (register arrays of KMAX, walked fully unrolled).
+template <int MODE, int KMAX>
+__device__ __forceinline__ void flock_core(f3 pos, f3 fwd, float radius, int m, const f3* bp, const f3* bf,
+                                           const FlockTerm& sep, const FlockTerm& ali, const FlockTerm& coh,
+                                           f3& steering, uint32_t& bits) {
     // Bits only accumulate, so a term masked now stays masked.
     const bool do_s = sep.on && !(bits & sep.mask);
     const bool do_a = ali.on && !(bits & ali.mask);
@@ -56,37 +53,22 @@ __device__ __forceinline__ void flock_core(f3 pos, f3 fwd, float radius, uint32_
     if (!do_s && !do_a && !do_c) return;
     f3 ss = mk3(0.f, 0.f, 0.f), sa = ss, sc = ss;
     int ns = 0, na = 0, nc = 0;
-    auto visit = [&](f3 bp, f3 bf) {
-        const f3 off = bp - pos;
-        const float d2 = dot3(off, off);
-        if (do_s && in_hood<MODE>(off, d2, fwd, radius, sep)) {
-            ss = ss + off / -d2;
-            ++ns;
-        }
-        if (do_a && in_hood<MODE>(off, d2, fwd, radius, ali)) {
-            sa = sa + bf;
-            ++na;
-        }
-        if (do_c && in_hood<MODE>(off, d2, fwd, radius, coh)) {
-            sc = sc + bp;
-            ++nc;
-        }
-    };
-    if (NSLOT == 0) {
-        for (uint32_t j = 0; j < k; ++j) {
-            f3 bp, bf = mk3(0.f, 0.f, 0.f);
-            if (nbr((int)j, do_a, bp, bf) != kNbrValid) break;
-            visit(bp, bf);
-        }
-    } else {
-        bool stopped = false;
 #pragma unroll
-        for (int j = 0; j < (NSLOT > 0 ? NSLOT : 1); ++j) {
-            f3 bp = mk3(0.f, 0.f, 0.f), bf = bp;
-            if (!stopped) {
-                const int r = nbr(j, do_a, bp, bf);
-                if (r == kNbrStop) stopped = true;
-                else if (r == kNbrValid) visit(bp, bf);
+    for (int j = 0; j < KMAX; ++j) {
+        if (j < m) {
+            const f3 off = bp[j] - pos;
+            const float d2 = dot3(off, off);
+            if (do_s && in_hood<MODE>(off, d2, fwd, radius, sep)) {
+                ss = ss + off / -d2;
+                ++ns;
+            }
+            if (do_a && in_hood<MODE>(off, d2, fwd, radius, ali)) {
+                sa = sa + bf[j];
+                ++na;
+            }
+            if (do_c && in_hood<MODE>(off, d2, fwd, radius, coh)) {
+                sc = sc + bp[j];
+                ++nc;
             }
         }
     }
@@ -102,6 +84,48 @@ __device__ __forceinline__ void flock_core(f3 pos, f3 fwd, float radius, uint32_
         if (nc > 0) sc = normalize3((sc / (float)nc) - pos);
         apply(steering, bits, sc, coh.w, GS_BIT_COHESION);
     }
+}
+
+// Loads the KNN slots of row `row` ([k][nA] slot-major) and the neighbours'
+// positions (and forward vectors when want_fwd) into register arrays, every
+// level of loads in flight together.  Returns m, the number of neighbours
+// before the first pad (>= nB).  Positions come from the packed snapshot when
+// bpos is given, else from the SoA arrays bx/by/bz.
+template <int KMAX>
+__device__ __forceinline__ int gather_neighbours(const uint32_t* __restrict__ knn, uint32_t k, uint32_t nA,
+                                                 uint32_t row, uint32_t nB, const float4* __restrict__ bpos,
+                                                 const float* __restrict__ bx, const float* __restrict__ by,
+                                                 const float* __restrict__ bz, const float* __restrict__ bfx,
+                                                 const float* __restrict__ bfy, const float* __restrict__ bfz,
+                                                 bool want_fwd, f3* bp, f3* bf) {
+    // Loads are unconditional at in-range indices (slots past the first pad
+    // reuse slot 0's neighbour) so each level is in flight together; only
+    // the first m results are used.
+    uint32_t v[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) v[j] = knn[(size_t)min(j, (int)k - 1) * nA + row];
+    int m = (int)k;
+#pragma unroll
+    for (int j = KMAX - 1; j >= 0; --j)
+        if (j < (int)k && v[j] >= nB) m = j;
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) bp[j] = bf[j] = mk3(0.f, 0.f, 0.f);
+    if (m == 0) return 0;
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+        const uint32_t b = (j < m) ? v[j] : v[0];
+        if (bpos) {
+            const float4 q = bpos[b];
+            bp[j] = mk3(q.x, q.y, q.z);
+        } else {
+            bp[j] = ld3(bx, by, bz, b);
+        }
+    }
+    if (want_fwd) {
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) bf[j] = ld3(bfx, bfy, bfz, (j < m) ? v[j] : v[0]);
+    }
+    return m;
 }
 
 // LinesIntersect in the XZ plane, include/OpenSteer/VectorUtils.cuh:390-428.

@@ -10,7 +10,7 @@
 
 namespace gs {
 
-template <int MODE>
+template <int KMAX, int MODE>
 __global__ __launch_bounds__(kBlock) void k_flock(uint32_t nA, Agents A, uint32_t nB, const float* __restrict__ bpx,
                                                   const float* __restrict__ bpy, const float* __restrict__ bpz,
                                                   const float* __restrict__ bfx, const float* __restrict__ bfy,
@@ -19,18 +19,15 @@ __global__ __launch_bounds__(kBlock) void k_flock(uint32_t nA, Agents A, uint32_
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nA) return;
     uint32_t bits = A.bits[i];
+    const bool any = (sep.on && !(bits & sep.mask)) || (ali.on && !(bits & ali.mask)) ||
+                     (coh.on && !(bits & coh.mask));
+    if (!any) return;
+    f3 bp[KMAX], bf[KMAX];
+    const int m = gather_neighbours<KMAX>(knn, k, nA, i, nB, nullptr, bpx, bpy, bpz, bfx, bfy, bfz,
+                                          ali.on && !(bits & ali.mask), bp, bf);
     f3 steering = ld3(A.gx, A.gy, A.gz, i);
-    const uint32_t bits0 = bits;
-    flock_core<MODE, 0>(ld3(A.px, A.py, A.pz, i), ld3(A.fx, A.fy, A.fz, i), A.radius[i], k, sep, ali, coh,
-                        steering, bits, [&](int j, bool want_fwd, f3& bp, f3& bf) {
-                            const uint32_t b = knn[(size_t)j * nA + i];
-                            if (b >= nB) return (int)kNbrStop;
-                            bp = ld3(bpx, bpy, bpz, b);
-                            if (want_fwd) bf = ld3(bfx, bfy, bfz, b);
-                            return (int)kNbrValid;
-                        });
-    if (!(sep.on && !(bits0 & sep.mask)) && !(ali.on && !(bits0 & ali.mask)) && !(coh.on && !(bits0 & coh.mask)))
-        return;
+    flock_core<MODE, KMAX>(ld3(A.px, A.py, A.pz, i), ld3(A.fx, A.fy, A.fz, i), A.radius[i], m, bp, bf, sep, ali,
+                           coh, steering, bits);
     st3(A.gx, A.gy, A.gz, i, steering);
     A.bits[i] = bits;
 }
@@ -240,20 +237,27 @@ static FlockTerm term(const gs_flock_term* t) {
     return r;
 }
 
+template <int KMAX>
+static void run_flock(gs_group_s* A, gs_knn_s* knn, gs_group_s* B, const gs_flock_term* sep,
+                      const gs_flock_term* ali, const gs_flock_term* coh, int mode) {
+    hipStream_t st = A->ctx->stream;
+    const Agents a = A->dev_agents();
+#define GS_FLOCK(MODE)                                                                                              \
+    k_flock<KMAX, MODE><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, a, B->n, B->F(0), B->F(1), B->F(2), B->F(3),      \
+                                                             B->F(4), B->F(5), knn->idx(), knn->k, term(sep),        \
+                                                             term(ali), term(coh))
+    if (mode == GS_MODE_GPUREF) GS_FLOCK(GS_MODE_GPUREF);
+    else GS_FLOCK(GS_MODE_OPENSTEER);
+#undef GS_FLOCK
+    GS_LAUNCHED("k_flock");
+}
+
 void launch_flock(gs_group_s* A, gs_knn_s* knn, gs_group_s* B, const gs_flock_term* sep, const gs_flock_term* ali,
                   const gs_flock_term* coh, int mode) {
     if (A->n == 0) return;
-    hipStream_t st = A->ctx->stream;
-    const Agents a = A->dev_agents();
-    if (mode == GS_MODE_GPUREF)
-        k_flock<GS_MODE_GPUREF><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, a, B->n, B->F(0), B->F(1), B->F(2),
-                                                                     B->F(3), B->F(4), B->F(5), knn->idx(), knn->k,
-                                                                     term(sep), term(ali), term(coh));
-    else
-        k_flock<GS_MODE_OPENSTEER><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, a, B->n, B->F(0), B->F(1), B->F(2),
-                                                                        B->F(3), B->F(4), B->F(5), knn->idx(),
-                                                                        knn->k, term(sep), term(ali), term(coh));
-    GS_LAUNCHED("k_flock");
+    if (knn->k <= 8) run_flock<8>(A, knn, B, sep, ali, coh, mode);
+    else if (knn->k <= 16) run_flock<16>(A, knn, B, sep, ali, coh, mode);
+    else run_flock<32>(A, knn, B, sep, ali, coh, mode);
 }
 
 void launch_seek(gs_group_s* A, const float* t, float w, uint32_t mask, int mode) {

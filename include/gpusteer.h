@@ -243,10 +243,11 @@ int gs_step_boids(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* p
 
 /* gs_step_boids with HIP-event timing recorded on the context's stream:
  * average milliseconds per step of the bin build (key, scan, scatter,
- * canonical order) and of the fused KNN + flocking + integrate kernel. */
+ * reorder), of the exact KNN, and of the flocking + integrate kernel. */
 typedef struct {
     float bin_build_ms;
-    float fused_ms;
+    float knn_ms;
+    float steer_integrate_ms;
     float step_ms;
 } gs_boids_timing;
 int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* params, uint32_t steps,
