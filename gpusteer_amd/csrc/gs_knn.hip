@@ -31,7 +31,7 @@ namespace gs {
 
 // One lane per query slot s of A (storage = cell order, so a wavefront's lanes
 // share cells and their candidate runs overlap in cache).
-template <int KMAX>
+template <int KMAX, int RF>
 __global__ __launch_bounds__(kBlock) void k_knn_binned(uint32_t nA, const float4* __restrict__ apos, uint32_t nB,
                                                        const float4* __restrict__ bpos,
                                                        const uint32_t* __restrict__ bstart, GridDev g, int self,
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kBlock) void k_knn_binned(uint32_t nA, const float4
                                                        const uint32_t* __restrict__ row_map,
                                                        const uint32_t* __restrict__ val_map,
                                                        uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
-    __shared__ uint2 lbuf[kKnnBuf * kBlock];
+    GS_DEFER_LDS
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nA) return;
     const float4 q = apos[s];
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kBlock) void k_knn_binned(uint32_t nA, const float4
     h.init(k);
     const float thr =
         seed ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB, val_map, q.x, q.y, q.z, bpos) : INFINITY;
-    knn_search<KMAX>(h, thr, q.x, q.y, q.z, self ? s : UINT32_MAX, g, R0, bpos, bstart, lbuf + threadIdx.x);
+    knn_search<KMAX, RF>(h, thr, q.x, q.y, q.z, self ? s : UINT32_MAX, g, R0, bpos, bstart, lbuf);
     h.store(k, nA, s, out_idx, out_dist);
 }
 
@@ -108,22 +108,32 @@ __global__ __launch_bounds__(kBlock) void k_knn_remap(uint32_t nA, uint32_t k, c
     }
 }
 
-template <int KMAX>
+template <int KMAX, int RF>
 static void run_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, const uint32_t* seed,
                        const uint32_t* row_map, const uint32_t* val_map) {
     const uint32_t nA = A->n;
-    k_knn_binned<KMAX><<<blocks_for(nA), kBlock, 0, A->ctx->stream>>>(
+    k_knn_binned<KMAX, RF><<<blocks_for(nA), kBlock, 0, A->ctx->stream>>>(
         nA, A->spos.p, B->n, B->spos.p, B->start.p, B->bin_grid, A == B ? 1 : 0, out->k, (int)R0, seed, row_map,
         val_map, out->idx_alt(), out->dist_alt());
     GS_LAUNCHED("k_knn_binned");
 }
 
+template <int KMAX>
+static void run_binned_rf(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, const uint32_t* seed,
+                          const uint32_t* row_map, const uint32_t* val_map) {
+    // search_radius selects the fast path's column radius: 3 x 3 columns for
+    // lattices whose cell is about the k-th neighbour distance, 5 x 5 for
+    // finer lattices.
+    if (R0 >= 2) run_binned<KMAX, 2>(A, out, B, R0, seed, row_map, val_map);
+    else run_binned<KMAX, 1>(A, out, B, R0, seed, row_map, val_map);
+}
+
 void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, const uint32_t* seed,
                        const uint32_t* row_map, const uint32_t* val_map) {
     if (A->n == 0) return;
-    if (out->k <= 8) run_binned<8>(A, out, B, search_radius, seed, row_map, val_map);
-    else if (out->k <= 16) run_binned<16>(A, out, B, search_radius, seed, row_map, val_map);
-    else run_binned<32>(A, out, B, search_radius, seed, row_map, val_map);
+    if (out->k <= 8) run_binned_rf<8>(A, out, B, search_radius, seed, row_map, val_map);
+    else if (out->k <= 16) run_binned_rf<16>(A, out, B, search_radius, seed, row_map, val_map);
+    else run_binned_rf<32>(A, out, B, search_radius, seed, row_map, val_map);
 }
 
 void launch_knn_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B) {

@@ -9,8 +9,17 @@
 namespace gs {
 
 // Candidates a lane may defer (per-lane LDS buffer) before the wave merges
-// them into the top-k lists.
-constexpr int kKnnBuf = 16;
+// them into the top-k lists.  Each entry is its 64-bit key (d2 bits, agent
+// index) and its slot, in two planes: 12 bytes per entry.
+constexpr int kKnnBuf = 12;
+struct DeferBuf {
+    uint2* key;     // [kKnnBuf][kBlock]
+    uint32_t* slot; // [kKnnBuf][kBlock]
+};
+#define GS_DEFER_LDS                                          \
+    __shared__ uint2 lds_defer_key[kKnnBuf * kBlock];         \
+    __shared__ uint32_t lds_defer_slot[kKnnBuf * kBlock];     \
+    const DeferBuf lbuf{lds_defer_key + threadIdx.x, lds_defer_slot + threadIdx.x};
 
 // Sorted list of KMAX entries in registers: a 64-bit key (fp32 bits of d2 in
 // the high word -- monotonic for d2 >= 0 -- and the agent index in the low
@@ -135,14 +144,13 @@ __device__ __forceinline__ bool exact_after(float lim, float dmin) {
 // Merges every lane's deferred candidates into its list.  Called with the
 // whole wavefront converged; iterates as long as any lane has entries left.
 template <int KMAX>
-__device__ __forceinline__ void flush_deferred(TopK<KMAX>& h, int& cnt, const uint2* lbuf,
-                                               const float4* __restrict__ bpos) {
+__device__ __forceinline__ void flush_deferred(TopK<KMAX>& h, int& cnt, const DeferBuf& lbuf) {
     for (int e = 0; e < kKnnBuf; ++e) {
         if (!__any(e < cnt)) break;
         if (e < cnt) {
-            const uint2 v = lbuf[e * kBlock];
-            const uint64_t kk = ((uint64_t)v.x << 32) | id_of(bpos[v.y]);
-            if (h.accepts(kk)) h.push(kk, v.y);
+            const uint2 v = lbuf.key[e * kBlock];
+            const uint64_t kk = ((uint64_t)v.x << 32) | v.y;
+            if (h.accepts(kk)) h.push(kk, lbuf.slot[e * kBlock]);
         }
     }
     cnt = 0;
@@ -207,53 +215,62 @@ __device__ void shell_search(TopK<KMAX>& h, float thr, float qx, float qy, float
 // order: cell c holds slots [bstart[c], bstart[c+1]); bpos[t] =
 // (x, y, z, agent index bits).  self_slot (A == B) is excluded.
 //
-// Fast path: the 3 x 3 columns around the query's cell, each a z-run of up
-// to 3 cells whose end cells are culled against the bound.  All nine runs
-// are located first (their start loads in flight together); the column walk
-// is unrolled and wave-uniform, and candidates are loaded four at a time.
+// Fast path over the (2RF+1)^2 columns around the query's cell (RF = 1 or 2,
+// a compile-time choice matched to the lattice).  For each column the z-run
+// covers exactly the cells that intersect the bound's sphere (its half-chord
+// sqrt(bound - gap_xy^2) around the query), capped to cz +- RF.  All runs are
+// located first (their start loads in flight together); the column walk is
+// unrolled and wave-uniform, and candidates are loaded four at a time.
 // Candidates within the bound are appended to the lane's LDS buffer (lbuf,
-// stride kBlock) and merged into the list by the whole wavefront at column
+// entries strided by kBlock) and merged into the list by the whole wavefront at column
 // boundaries, so the insertion ripple runs about once per accepted candidate
-// instead of once per candidate of any lane.  If the 3 x 3 x 3 box does not
-// prove the list exact, the rare path continues with shells R >= 2.
-template <int KMAX>
+// instead of once per candidate of any lane.  If the box of radius RF does
+// not prove the list exact, the rare path continues with shells R > RF.
+template <int KMAX, int RF>
 __device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, float qy, float qz,
                                            uint32_t self_slot, const GridDev& g, int R0,
                                            const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart,
-                                           uint2* lbuf) {
+                                           const DeferBuf& lbuf) {
+    constexpr int W = 2 * RF + 1;
+    constexpr int NC = W * W;
     const int cx = axis_cell(qx, g.ox, g.sx, g.dx);
     const int cy = axis_cell(qy, g.oy, g.sy, g.dy);
     const int cz = axis_cell(qz, g.oz, g.sz, g.dz);
     const float grow = 1.0f + 2e-5f;
     float lim = fminf(thr, h.worst());
     const float bound = lim * grow;
-    const float gz_mid = axis_gap(qz, cz, cz, g.oz, g.hz, g.dz, g.slack);
-    const float gz_lo = axis_gap(qz, cz - 1, cz - 1, g.oz, g.hz, g.dz, g.slack);
-    const float gz_hi = axis_gap(qz, cz + 1, cz + 1, g.oz, g.hz, g.dz, g.slack);
-    // Locate the nine runs against the initial bound (tighter bounds found
-    // later only make the scan do more than needed, never less).
-    uint32_t r0[9], r1[9];
+    const int zlo = max(cz - RF, 0), zhi = min(cz + RF, g.dz - 1);
+    // Locate the runs against the initial bound (tighter bounds found later
+    // only make the scan do more than needed, never less).
+    uint32_t r0[NC], r1[NC];
 #pragma unroll
-    for (int col = 0; col < 9; ++col) {
-        const int ix = cx + col / 3 - 1, iy = cy + col % 3 - 1;
+    for (int col = 0; col < NC; ++col) {
+        const int ix = cx + col / W - RF, iy = cy + col % W - RF;
         r0[col] = r1[col] = 0u;
         if (ix >= 0 && ix < g.dx && iy >= 0 && iy < g.dy) {
             const float gx = axis_gap(qx, ix, ix, g.ox, g.hx, g.dx, g.slack);
             const float gy = axis_gap(qy, iy, iy, g.oy, g.hy, g.dy, g.slack);
             const float gxy = gx * gx + gy * gy;
-            if (gxy + gz_mid * gz_mid <= bound) {
-                int z0 = cz, z1 = cz;
-                if (cz > 0 && gxy + gz_lo * gz_lo <= bound) z0 = cz - 1;
-                if (cz + 1 < g.dz && gxy + gz_hi * gz_hi <= bound) z1 = cz + 1;
-                const uint32_t c0 = cell_index(g, ix, iy, z0);
-                r0[col] = bstart[c0];
-                r1[col] = bstart[c0 + (uint32_t)(z1 - z0) + 1u];
+            if (gxy <= bound) {
+                int z0 = zlo, z1 = zhi;
+                if (bound < INFINITY) {
+                    // Cells of this column that can hold a point within the
+                    // bound: |z - qz| <= sqrt(bound - gxy) (+ rounding slack).
+                    const float half = sqrtf(bound - gxy) * grow + g.slack;
+                    z0 = max(z0, axis_cell(qz - half, g.oz, g.sz, g.dz));
+                    z1 = min(z1, axis_cell(qz + half, g.oz, g.sz, g.dz));
+                }
+                if (z0 <= z1) {
+                    const uint32_t c0 = cell_index(g, ix, iy, z0);
+                    r0[col] = bstart[c0];
+                    r1[col] = bstart[c0 + (uint32_t)(z1 - z0) + 1u];
+                }
             }
         }
     }
     int cnt = 0;
 #pragma unroll
-    for (int col = 0; col < 9; ++col) {
+    for (int col = 0; col < NC; ++col) {
         const uint32_t t1 = r1[col];
         for (uint32_t t = r0[col]; t < t1; t += 4) {
             // Unconditional loads at clamped (in-range) slots keep all four in
@@ -267,7 +284,8 @@ __device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, f
                     const float d2 = sqdist4(qx, qy, qz, p[u]);
                     if (d2 <= lim && t + u != self_slot) {
                         if (cnt < kKnnBuf) {
-                            lbuf[cnt * kBlock] = make_uint2(__float_as_uint(d2), t + u);
+                            lbuf.key[cnt * kBlock] = make_uint2(__float_as_uint(d2), id_of(p[u]));
+                            lbuf.slot[cnt * kBlock] = t + u;
                             ++cnt;
                         } else {
                             const uint64_t kk = make_key(d2, id_of(p[u]));
@@ -280,16 +298,16 @@ __device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, f
                 }
             }
         }
-        if (col < 8 && __any(cnt >= kKnnBuf / 2)) {
-            flush_deferred<KMAX>(h, cnt, lbuf, bpos);
+        if (col < NC - 1 && __any(cnt >= kKnnBuf / 2)) {
+            flush_deferred<KMAX>(h, cnt, lbuf);
             lim = fminf(thr, h.worst());
         }
     }
-    flush_deferred<KMAX>(h, cnt, lbuf, bpos);
+    flush_deferred<KMAX>(h, cnt, lbuf);
     bool all;
-    const float dmin = outside_gap(g, cx, cy, cz, 1, qx, qy, qz, all);
-    if (R0 <= 1 && (all || exact_after(fminf(thr, h.worst()), dmin))) return;
-    shell_search<KMAX>(h, thr, qx, qy, qz, self_slot, g, cx, cy, cz, 2, R0, bpos, bstart);
+    const float dmin = outside_gap(g, cx, cy, cz, RF, qx, qy, qz, all);
+    if (R0 <= RF && (all || exact_after(fminf(thr, h.worst()), dmin))) return;
+    shell_search<KMAX>(h, thr, qx, qy, qz, self_slot, g, cx, cy, cz, RF + 1, R0, bpos, bstart);
 }
 
 }  // namespace gs

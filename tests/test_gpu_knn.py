@@ -15,7 +15,7 @@ def _group(gs, ctx, pos):
     return gs.AgentGroup(ctx, st)
 
 
-def _run_binned(gs, ctx, a_pos, b_pos, k, origin, size, cells, self_group, flags=0, repeat=1):
+def _run_binned(gs, ctx, a_pos, b_pos, k, origin, size, cells, self_group, flags=0, repeat=1, sr=1):
     bins = gs.KNNBinData(ctx, cells, size, origin=origin)
     A = _group(gs, ctx, a_pos)
     B = A if self_group else _group(gs, ctx, b_pos)
@@ -24,7 +24,7 @@ def _run_binned(gs, ctx, a_pos, b_pos, k, origin, size, cells, self_group, flags
         gs.updateKNNDatabase(B, bins)
     knn = gs.KNNData(ctx, a_pos.shape[0], k)
     for _ in range(repeat):
-        gs.findKNearestNeighbors(A, knn, bins, B, 1, flags)
+        gs.findKNearestNeighbors(A, knn, bins, B, sr, flags)
     out = knn.download()
     knn.close()
     A.close()
@@ -35,23 +35,26 @@ def _run_binned(gs, ctx, a_pos, b_pos, k, origin, size, cells, self_group, flags
 
 
 CASES = [
-    # name, n, k, cells
-    ("c0-boids-k7", 4096, 7, (16, 16, 16)),
-    ("c0-boids-k1", 4096, 1, (16, 16, 16)),
-    ("c0-boids-k16", 4096, 16, (16, 16, 16)),
-    ("c0-boids-k32", 4096, 32, (16, 16, 16)),
-    ("coarse-1cell", 500, 7, (1, 1, 1)),
-    ("fine-64", 3000, 9, (64, 64, 64)),
+    # name, n, k, cells, search radius (fast-path column radius)
+    ("c0-boids-k7", 4096, 7, (16, 16, 16), 1),
+    ("c0-boids-k1", 4096, 1, (16, 16, 16), 1),
+    ("c0-boids-k16", 4096, 16, (16, 16, 16), 1),
+    ("c0-boids-k32", 4096, 32, (16, 16, 16), 1),
+    ("coarse-1cell", 500, 7, (1, 1, 1), 1),
+    ("fine-64", 3000, 9, (64, 64, 64), 1),
+    ("c0-boids-k7-r2", 4096, 7, (16, 16, 16), 2),
+    ("fine-32-k16-r2", 4096, 16, (32, 32, 32), 2),
+    ("fine-64-r3", 3000, 7, (64, 64, 64), 3),
 ]
 
 
-@pytest.mark.parametrize("name,n,k,cells", CASES)
-def test_binned_equals_oracle_brute(gs, ctx, orc, name, n, k, cells):
+@pytest.mark.parametrize("name,n,k,cells,sr", CASES)
+def test_binned_equals_oracle_brute(gs, ctx, orc, name, n, k, cells, sr):
     print(f"case {name}", flush=True)
     st, p = scenarios.boids_sphere(n=n)
     pos = st.pos()
     r_idx, r_dist = orc.knn_brute(pos, pos, k, True)
-    idx, dist = _run_binned(gs, ctx, pos, pos, k, p.grid_origin, p.grid_size, cells, True)
+    idx, dist = _run_binned(gs, ctx, pos, pos, k, p.grid_origin, p.grid_size, cells, True, sr=sr)
     np.testing.assert_array_equal(idx, r_idx)
     np.testing.assert_array_equal(dist, r_dist)
 
@@ -113,12 +116,14 @@ def test_out_of_grid_points(gs, ctx, orc):
     np.testing.assert_array_equal(dist, r_dist)
 
 
-def test_seeded_call_equals_unseeded(gs, ctx, orc):
+@pytest.mark.parametrize("sr", [1, 2])
+def test_seeded_call_equals_unseeded(gs, ctx, orc, sr):
     st, p = scenarios.boids_sphere(n=4096)
     pos = st.pos()
     r_idx, r_dist = orc.knn_brute(pos, pos, 7, True)
     # Second call on the same container uses the first result as its bound.
-    idx, dist = _run_binned(gs, ctx, pos, pos, 7, p.grid_origin, p.grid_size, p.grid_cells, True, repeat=3)
+    idx, dist = _run_binned(gs, ctx, pos, pos, 7, p.grid_origin, p.grid_size, p.grid_cells, True, repeat=3,
+                            sr=sr)
     np.testing.assert_array_equal(idx, r_idx)
     np.testing.assert_array_equal(dist, r_dist)
 

@@ -68,9 +68,11 @@ def test_fused_equals_separate_calls(gs, ctx):
         h.close()
 
 
-def test_boids_box_large_steps(gs, ctx, orc):
-    """2^20-agent box flock: 3 fused steps vs 3 oracle steps, bit-exact."""
-    st, p = scenarios.boids_box(n=1 << 20)
+@pytest.mark.parametrize("cells,sr", [(64, 1), (128, 2)])
+def test_boids_box_large_steps(gs, ctx, orc, cells, sr):
+    """2^20-agent box flock: 3 steps vs 3 oracle steps, bit-exact."""
+    st, p = scenarios.boids_box(n=1 << 20, cells=cells)
+    p.search_radius = sr
     ref = st.copy()
     A = gs.AgentGroup(ctx, st)
     bins = gs.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
