@@ -42,6 +42,93 @@ def bytes_knn_kernel(k: int) -> float:
     return 12.0 + 4.0 * k
 
 
+TRAFFIC_SUMMARY = ROOT / "profiles" / "k_knn_binned_boids_box_1M_k7_hbm_traffic.md"
+
+
+def knn_traffic_from_profile():
+    """HBM bytes per launch of the KNN kernel as measured by the committed
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes (each scaled by the same
+    counter's reading for a copy of known size), or None when absent."""
+    import re
+
+    if not TRAFFIC_SUMMARY.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", TRAFFIC_SUMMARY.read_text(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def _hip_runtime():
+    """The HIP runtime libgpusteer is linked against (already loaded by the
+    package import), driven through ctypes for the copy measurements so that
+    no second HIP runtime enters the process."""
+    import ctypes as C
+
+    import gpusteer_amd  # noqa: F401  (loads libgpusteer and its HIP runtime)
+
+    L = C.CDLL("libamdhip64.so.7")
+    vp, sz = C.c_void_p, C.c_size_t
+    L.hipMalloc.argtypes = [C.POINTER(vp), sz]
+    L.hipFree.argtypes = [vp]
+    L.hipMemset.argtypes = [vp, C.c_int, sz]
+    L.hipMemcpyAsync.argtypes = [vp, vp, sz, C.c_int, vp]
+    L.hipEventCreate.argtypes = [C.POINTER(vp)]
+    L.hipEventRecord.argtypes = [vp, vp]
+    L.hipEventSynchronize.argtypes = [vp]
+    L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), vp, vp]
+    L.hipEventDestroy.argtypes = [vp]
+    return L
+
+
+def device_copy_gbs(mib: int, reps: int) -> float:
+    """Times `reps` device-to-device copies of `mib` MiB (hipMemcpyAsync, the
+    runtime's copy kernel) with HIP events; returns (read + write) GB/s."""
+    import ctypes as C
+
+    L = _hip_runtime()
+    nbytes = mib * (1 << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    ok(L.hipMalloc(C.byref(src), nbytes), "hipMalloc")
+    ok(L.hipMalloc(C.byref(dst), nbytes), "hipMalloc")
+    try:
+        ok(L.hipMemset(src, 1, nbytes), "hipMemset")
+        ok(L.hipEventCreate(C.byref(ev0)), "hipEventCreate")
+        ok(L.hipEventCreate(C.byref(ev1)), "hipEventCreate")
+        ok(L.hipMemcpyAsync(dst, src, nbytes, 3, None), "hipMemcpyAsync")  # 3 = device to device
+        ok(L.hipEventRecord(ev0, None), "hipEventRecord")
+        for _ in range(reps):
+            ok(L.hipMemcpyAsync(dst, src, nbytes, 3, None), "hipMemcpyAsync")
+        ok(L.hipEventRecord(ev1, None), "hipEventRecord")
+        ok(L.hipEventSynchronize(ev1), "hipEventSynchronize")
+        ms = C.c_float()
+        ok(L.hipEventElapsedTime(C.byref(ms), ev0, ev1), "hipEventElapsedTime")
+        L.hipEventDestroy(ev0)
+        L.hipEventDestroy(ev1)
+    finally:
+        L.hipFree(src)
+        L.hipFree(dst)
+    return 2 * nbytes / (ms.value / reps * 1e-3) / 1e9
+
+
+def copy_calibration(mib: int = 512, reps: int = 5) -> int:
+    """A plain device copy of known size (mib MiB read + mib MiB written), run
+    under rocprofv3 --pmc to calibrate FETCH_SIZE / WRITE_SIZE in bytes."""
+    gbs = device_copy_gbs(mib, reps)
+    print(json.dumps({"copy_calibration_mib": mib, "reps": reps, "copy_gbs": gbs}), flush=True)
+    return 0
+
+
+def achievable_copy_gbs() -> float:
+    """Device-to-device copy bandwidth of 1 GiB buffers: the practical HBM
+    ceiling next to the 8 TB/s nominal peak."""
+    return device_copy_gbs(1024, 10)
+
+
 def cpu_baseline(st, p, seconds: float):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as orc
@@ -77,11 +164,15 @@ def main() -> int:
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 512 MiB device copy (the counter calibration for the traffic figure)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: the per-step path runs on a single MI355X and has no "
+              "multi-GPU launch; run with --gpus 1.", file=sys.stderr)
         return 2
 
     import gpusteer_amd as gs
@@ -113,6 +204,7 @@ def main() -> int:
     if status != 0:
         print(f"bench: non-finite state at agent {bad}", file=sys.stderr)
         return 1
+    copy_gbs = achievable_copy_gbs()
     ms = el * 1e3 / args.steps
     value = n * args.steps / el
     b_knn = bytes_knn_kernel(args.k)
@@ -148,12 +240,21 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": knn_traffic_from_profile(),
+            "bytes_per_launch": n * b_knn,
             "bytes_per_agent": b_knn,
+            "achievable_copy_gbs": copy_gbs,
             "kernel_ms": knn_ms,
         },
         "stage_ms": {"bin_build": bin_ms, "knn": knn_ms, "steer_integrate": si_ms},
-        "whole_step_effective_gbs": n * bytes_per_agent_step(args.k, C, n) / (ms * 1e-3) / 1e9,
+        "roofline_step": {
+            "what": "whole step, algorithmic bytes 224 + 8k + 12 C/N per agent-step over the timed wall time",
+            "bytes_per_agent_step": bytes_per_agent_step(args.k, C, n),
+            "achieved": n * bytes_per_agent_step(args.k, C, n) / (ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": n * bytes_per_agent_step(args.k, C, n) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        },
     }
     if not args.no_cpu:
         print("bench: cpu baseline (oracle) ...", file=sys.stderr, flush=True)

@@ -866,7 +866,9 @@ int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_par
         if (!timing) fail(GS_ERR_INVALID_ARGUMENT, "timing is NULL");
         if (steps == 0) fail(GS_ERR_INVALID_ARGUMENT, "steps must be at least 1");
         use_device(A->ctx);
-        for (auto& e : ev) GS_HIP(hipEventCreate(&e));
+        // Timing-only events: no system-scope fence, so recording them does not
+        // write back and invalidate L2 between the stages being timed.
+        for (auto& e : ev) GS_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         double t[3] = {0, 0, 0};
         for (uint32_t s = 0; s < steps; ++s) {
             boids_step_once(A, knn, bins, p, ev);
@@ -877,6 +879,7 @@ int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_par
                 t[j] += ms;
             }
         }
+        GS_HIP(hipStreamSynchronize(A->ctx->stream));
         timing->bin_build_ms = (float)(t[0] / steps);
         timing->knn_ms = (float)(t[1] / steps);
         timing->steer_integrate_ms = (float)(t[2] / steps);

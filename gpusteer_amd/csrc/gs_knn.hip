@@ -39,15 +39,23 @@ __global__ __launch_bounds__(kBlock) void k_knn_binned(uint32_t nA, const float4
                                                        const uint32_t* __restrict__ row_map,
                                                        const uint32_t* __restrict__ val_map,
                                                        uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
-    GS_DEFER_LDS
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nA) return;
     const float4 q = apos[s];
     TopK<KMAX> h;
     h.init(k);
-    const float thr =
-        seed ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB, val_map, q.x, q.y, q.z, bpos) : INFINITY;
-    knn_search<KMAX, RF>(h, thr, q.x, q.y, q.z, self ? s : UINT32_MAX, g, R0, bpos, bstart, lbuf);
+    if (nB > 0) {
+        const float thr = seed ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB, val_map, q.x, q.y,
+                                                  q.z, bpos)
+                               : INFINITY;
+        if constexpr (RF == 1) {
+            GS_FLAT_LDS
+            knn_search_flat<KMAX>(h, thr, q.x, q.y, q.z, self ? s : UINT32_MAX, g, R0, nB, bpos, bstart, flds);
+        } else {
+            GS_DEFER_LDS
+            knn_search<KMAX, RF>(h, thr, q.x, q.y, q.z, self ? s : UINT32_MAX, g, R0, bpos, bstart, lbuf);
+        }
+    }
     h.store(k, nA, s, out_idx, out_dist);
 }
 

@@ -310,4 +310,166 @@ __device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, f
     shell_search<KMAX>(h, thr, qx, qy, qz, self_slot, g, cx, cy, cz, RF + 1, R0, bpos, bstart);
 }
 
+// ---------------------------------------------------------------------------
+// Flat-stream fast path over the 3 x 3 columns (lattices whose cell is about
+// the k-th neighbour distance).
+//
+// The column walk above runs the columns in lockstep, so a wavefront pays,
+// per column, for its lane with the longest z-run.  Here each lane instead
+// walks its own run list -- the non-empty sphere-exact z-runs of its nine
+// columns, compacted into LDS -- as one stream of candidate slots, four per
+// iteration, so the wavefront pays for its longest whole stream.  The next
+// chunk's loads are issued before the current chunk is evaluated (two chunks
+// in flight per lane), and the run after the current one is read from LDS a
+// run ahead.  Candidates within the bound are written unconditionally to the
+// lane's LDS buffer (the count advances only for passing ones); the buffer is
+// merged into the list by the whole wavefront once any lane holds more than
+// kFlatBuf - 4 entries, so an iteration can never overflow it.
+constexpr int kFlatBuf = 8;
+constexpr int kFlatRuns = 9;
+struct FlatLds {
+    uint2* key;       // [kFlatBuf][kBlock]: (d2 bits, agent index)
+    uint32_t* slot;   // [kFlatBuf][kBlock]
+    uint32_t* run_t;  // [kFlatRuns][kBlock]: first slot of a non-empty run
+    uint16_t* run_n;  // [kFlatRuns][kBlock]: its length
+};
+#define GS_FLAT_LDS                                                                            \
+    __shared__ uint2 lds_flat_key[kFlatBuf * kBlock];                                          \
+    __shared__ uint32_t lds_flat_slot[kFlatBuf * kBlock];                                      \
+    __shared__ uint32_t lds_flat_run_t[kFlatRuns * kBlock];                                    \
+    __shared__ uint16_t lds_flat_run_n[kFlatRuns * kBlock];                                    \
+    const FlatLds flds{lds_flat_key + threadIdx.x, lds_flat_slot + threadIdx.x,                \
+                      lds_flat_run_t + threadIdx.x, lds_flat_run_n + threadIdx.x};
+
+template <int KMAX>
+__device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLds& L) {
+    for (int e = 0; e < kFlatBuf; ++e) {
+        if (!__any(e < cnt)) break;
+        if (e < cnt) {
+            const uint2 v = L.key[e * kBlock];
+            const uint64_t kk = ((uint64_t)v.x << 32) | v.y;
+            if (h.accepts(kk)) h.push(kk, L.slot[e * kBlock]);
+        }
+    }
+    cnt = 0;
+}
+
+// Requires nB >= 1 (every load is clamped to slot nB - 1).
+template <int KMAX>
+__device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, float thr, float qx, float qy, float qz,
+                                                uint32_t self_slot, const GridDev& g, int R0, uint32_t nB,
+                                                const float4* __restrict__ bpos,
+                                                const uint32_t* __restrict__ bstart, const FlatLds& L) {
+    const int cx = axis_cell(qx, g.ox, g.sx, g.dx);
+    const int cy = axis_cell(qy, g.oy, g.sy, g.dy);
+    const int cz = axis_cell(qz, g.oz, g.sz, g.dz);
+    const float grow = 1.0f + 2e-5f;
+    float lim = fminf(thr, h.worst());
+    const float bound = lim * grow;
+    const int zlo = max(cz - 1, 0), zhi = min(cz + 1, g.dz - 1);
+    uint32_t r0[kFlatRuns], r1[kFlatRuns];
+#pragma unroll
+    for (int col = 0; col < kFlatRuns; ++col) {
+        const int ix = cx + col / 3 - 1, iy = cy + col % 3 - 1;
+        r0[col] = r1[col] = 0u;
+        if (ix >= 0 && ix < g.dx && iy >= 0 && iy < g.dy) {
+            const float gx = axis_gap(qx, ix, ix, g.ox, g.hx, g.dx, g.slack);
+            const float gy = axis_gap(qy, iy, iy, g.oy, g.hy, g.dy, g.slack);
+            const float gxy = gx * gx + gy * gy;
+            if (gxy <= bound) {
+                int z0 = zlo, z1 = zhi;
+                if (bound < INFINITY) {
+                    const float half = sqrtf(bound - gxy) * grow + g.slack;
+                    z0 = max(z0, axis_cell(qz - half, g.oz, g.sz, g.dz));
+                    z1 = min(z1, axis_cell(qz + half, g.oz, g.sz, g.dz));
+                }
+                if (z0 <= z1) {
+                    const uint32_t c0 = cell_index(g, ix, iy, z0);
+                    r0[col] = bstart[c0];
+                    r1[col] = bstart[c0 + (uint32_t)(z1 - z0) + 1u];
+                }
+            }
+        }
+    }
+    // Compact the non-empty runs into the lane's LDS list.  A run longer than
+    // the 16-bit length field sends the lane to the shell search instead.
+    int M = 0;
+    bool long_run = false;
+#pragma unroll
+    for (int col = 0; col < kFlatRuns; ++col) {
+        const uint32_t len = r1[col] - r0[col];
+        long_run = long_run || len > 0xFFFFu;
+        if (len != 0u) {
+            L.run_t[M * kBlock] = r0[col];
+            L.run_n[M * kBlock] = (uint16_t)len;
+            ++M;
+        }
+    }
+    if (long_run) M = 0;
+    uint32_t t = 1u, e = 1u;  // empty stream (slot 0 is a valid load address)
+    uint32_t nt = 1u, nn = 0u;  // the run after the current one
+    int m = 0;
+    if (M > 0) {
+        t = L.run_t[0];
+        e = t + L.run_n[0];
+        m = 1;
+    }
+    if (m < M) {
+        nt = L.run_t[m * kBlock];
+        nn = L.run_n[m * kBlock];
+    }
+    const uint32_t last = nB - 1u;
+    int cnt = 0;
+    // One chunk: choose the next chunk's position (rest of this run, else the
+    // next run), issue its loads, evaluate the current chunk `cur`.
+    auto step = [&](const float4 (&cur)[4], float4 (&nxt)[4]) {
+        uint32_t tn = t + 4u, en = e;
+        if (tn >= e && m < M) {
+            tn = nt;
+            en = nt + nn;
+            ++m;
+            if (m < M) {
+                nt = L.run_t[m * kBlock];
+                nn = L.run_n[m * kBlock];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) nxt[u] = bpos[min(tn + (uint32_t)u, last)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t slot = t + (uint32_t)u;
+            const float d2 = sqdist4(qx, qy, qz, cur[u]);
+            const bool pass = (slot < e) && (d2 <= lim) && (slot != self_slot);
+            L.key[cnt * kBlock] = make_uint2(__float_as_uint(d2), id_of(cur[u]));
+            L.slot[cnt * kBlock] = slot;
+            cnt += pass ? 1 : 0;
+        }
+        t = tn;
+        e = en;
+    };
+    float4 pa[4], pb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pa[u] = bpos[min(t + (uint32_t)u, last)];
+    // The list is only touched between scan segments, so the scan loop keeps
+    // the list's registers untouched; two chunks per trip alternate buffers.
+    for (;;) {
+        while (__any(t < e) && !__any(cnt > kFlatBuf - 4)) {
+            step(pa, pb);
+            if (!__any(t < e) || __any(cnt > kFlatBuf - 4)) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) pa[u] = pb[u];
+                break;
+            }
+            step(pb, pa);
+        }
+        flat_flush<KMAX>(h, cnt, L);
+        lim = fminf(thr, h.worst());
+        if (!__any(t < e)) break;
+    }
+    bool all;
+    const float dmin = outside_gap(g, cx, cy, cz, 1, qx, qy, qz, all);
+    if (!long_run && R0 <= 1 && (all || exact_after(fminf(thr, h.worst()), dmin))) return;
+    shell_search<KMAX>(h, thr, qx, qy, qz, self_slot, g, cx, cy, cz, long_run ? 0 : 2, R0, bpos, bstart);
+}
+
 }  // namespace gs
