@@ -69,9 +69,28 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    build_cxx_examples()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+ROOT = PKG.parent
+CXX_EXAMPLES = {"boids_group_plugin": ROOT / "tests" / "cxx" / "boids_group_plugin.cpp"}
+
+
+def build_cxx_examples() -> None:
+    """Host-only C++ programs written against include/gpusteer_opensteer.hpp
+    (the reference's group API over the C-ABI), linked to libgpusteer.so."""
+    for name, src in CXX_EXAMPLES.items():
+        if not src.exists():
+            continue
+        out = LIB_DIR / name
+        cmd = [hipcc(), "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(src), f"-L{LIB_DIR}", "-lgpusteer",
+               "-Wl,-rpath,$ORIGIN", "-o", str(out)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr[-6000:]}")
 
 
 if __name__ == "__main__":

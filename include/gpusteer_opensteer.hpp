@@ -1,0 +1,465 @@
+/*
+ * gpusteer_opensteer.hpp — the reference's OpenSteer group API on top of the
+ * gpusteer C-ABI, so a plugin written against the reference compiles with a
+ * change of includes only.
+ *
+ * Re-created names (reference file:line):
+ *   AgentData                     include/OpenSteer/AgentData.h:13-42
+ *   ObstacleData                  include/OpenSteer/CUDA/ObstacleGroupData.cuh:10-14
+ *   AgentGroup                    include/OpenSteer/AgentGroup.h:12-56, src/AgentGroup.cpp:8-140
+ *   ObstacleGroup                 include/OpenSteer/ObstacleGroup.h:10-43
+ *   WallGroup                     include/OpenSteer/WallGroup.h:12-37 (segments added directly)
+ *   KNNData                       include/OpenSteer/CUDA/KNNData.cuh:10-55
+ *   KNNBinData                    include/OpenSteer/CUDA/KNNBinDataV2.cuh:43-61
+ *   wrapWorld .. updateGroup      include/OpenSteer/CUDA/GroupSteerLibrary.cuh:43-189
+ *
+ * Differences a plugin can observe:
+ *   - failures throw gpusteer::Error (the reference aborts through
+ *     CUDA_SAFE_CALL); nothing is left half-applied on the device;
+ *   - every group / container / lattice belongs to a gpusteer::Context; the
+ *     constructors without one use a process-wide context on device 0;
+ *   - the force and integrate formulas are those of the reference CUDA
+ *     kernels (GS_MODE_GPUREF) unless gpusteer::Context::SetMode selects the
+ *     CPU OpenSteer formulas (GS_MODE_OPENSTEER);
+ *   - the binned KNN is exact (the reference's V2 kernel scans a fixed cell
+ *     radius and can miss neighbours).
+ */
+#ifndef GPUSTEER_OPENSTEER_HPP
+#define GPUSTEER_OPENSTEER_HPP
+
+#include <hip/hip_vector_types.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gpusteer.h"
+
+namespace gpusteer {
+
+class Error : public std::runtime_error {
+public:
+    Error(int status, const std::string& what) : std::runtime_error(what), status_(status) {}
+    int status() const { return status_; }
+
+private:
+    int status_;
+};
+
+inline void check(int status, const char* call) {
+    if (status != GS_OK)
+        throw Error(status, std::string(call) + ": " + gs_status_string(status) + ": " + gs_last_error());
+}
+
+class Context {
+public:
+    explicit Context(int device = 0, void* hip_stream = nullptr) {
+        check(gs_context_create(device, hip_stream, &h_), "gs_context_create");
+    }
+    ~Context() { gs_context_destroy(h_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    gs_context handle() const { return h_; }
+    void Synchronize() { check(gs_context_synchronize(h_), "gs_context_synchronize"); }
+    int Mode() const { return mode_; }
+    void SetMode(int mode) { mode_ = mode; }
+    // The context used by constructors that are not given one.
+    static Context& Default() {
+        static Context ctx(0, nullptr);
+        return ctx;
+    }
+
+private:
+    gs_context h_ = nullptr;
+    int mode_ = GS_MODE_GPUREF;
+};
+
+}  // namespace gpusteer
+
+namespace OpenSteer {
+
+typedef unsigned int uint;
+typedef unsigned int id_type;
+
+struct AgentData {
+    id_type id = 0;
+    float3 side = make_float3(0.f, 0.f, 0.f);
+    float3 up = make_float3(0.f, 0.f, 0.f);
+    float4 direction = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 position = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 steering = make_float4(0.f, 0.f, 0.f, 0.f);
+    float speed = 0.f;
+    float maxSpeed = 0.f;
+    float maxForce = 0.f;
+    float radius = 0.f;
+    float mass = 0.f;
+    uint appliedKernels = 0u;
+};
+
+struct ObstacleData {
+    float4 position = make_float4(0.f, 0.f, 0.f, 0.f);
+    float radius = 0.f;
+};
+
+// Common base of the three group kinds (include/OpenSteer/BaseGroup.h:8-33).
+class BaseGroup {
+public:
+    BaseGroup(int kind, uint3 const& worldCells, uint k, gpusteer::Context& ctx)
+        : ctx_(&ctx), worldCells_(worldCells), k_(k) {
+        gpusteer::check(gs_group_create(ctx.handle(), kind, &h_), "gs_group_create");
+    }
+    virtual ~BaseGroup() { gs_group_destroy(h_); }
+    BaseGroup(const BaseGroup&) = delete;
+    BaseGroup& operator=(const BaseGroup&) = delete;
+
+    virtual uint const& Size() const { return n_; }
+    uint3 const& WorldCells() const { return worldCells_; }
+    uint K() const { return k_; }
+    gpusteer::Context& context() const { return *ctx_; }
+    // Uploads pending host edits; called by every operation before it runs.
+    virtual void SyncDevice() {}
+    gs_group device_handle() {
+        SyncDevice();
+        return h_;
+    }
+
+protected:
+    gpusteer::Context* ctx_;
+    gs_group h_ = nullptr;
+    uint3 worldCells_;
+    uint k_;
+    uint n_ = 0;
+};
+
+class AgentGroup : public BaseGroup {
+public:
+    AgentGroup(uint3 const& worldCells, uint const knn, gpusteer::Context& ctx = gpusteer::Context::Default())
+        : BaseGroup(GS_KIND_AGENTS, worldCells, knn, ctx) {}
+
+    // Adds an agent unless its id is already present (src/AgentGroup.cpp:34-55).
+    bool AddAgent(AgentData const& ad) {
+        SyncHost();
+        if (index_.count(ad.id)) return false;
+        index_[ad.id] = agents_.size();
+        agents_.push_back(ad);
+        n_ = (uint)agents_.size();
+        dirty_ = true;
+        return true;
+    }
+    // Removes an agent, keeping the order of the others (src/AgentGroup.cpp:58-80).
+    void RemoveAgent(id_type const id) {
+        SyncHost();
+        auto it = index_.find(id);
+        if (it == index_.end()) return;
+        agents_.erase(agents_.begin() + (long)it->second);
+        index_.clear();
+        for (size_t i = 0; i < agents_.size(); ++i) index_[agents_[i].id] = i;
+        n_ = (uint)agents_.size();
+        dirty_ = true;
+    }
+    void Clear() {
+        agents_.clear();
+        index_.clear();
+        n_ = 0;
+        dirty_ = true;
+        host_valid_ = true;
+    }
+    // State of one agent as the device last left it (src/AgentGroup.cpp:96-112).
+    bool GetDataForAgent(id_type const id, AgentData& ad) {
+        SyncHost();
+        auto it = index_.find(id);
+        if (it == index_.end()) return false;
+        ad = agents_[it->second];
+        return true;
+    }
+    // Marks the host copy stale; the next host access downloads.
+    void SetSyncHost() { host_valid_ = false; }
+
+    void SyncDevice() override {
+        if (!dirty_) return;
+        const size_t n = agents_.size();
+        std::vector<float> f(23 * (n ? n : 1));
+        std::vector<uint32_t> bits(n ? n : 1);
+        gs_agents a = soa(f.data(), bits.data(), n);
+        for (size_t i = 0; i < n; ++i) {
+            const AgentData& d = agents_[i];
+            a.px[i] = d.position.x; a.py[i] = d.position.y; a.pz[i] = d.position.z;
+            a.fx[i] = d.direction.x; a.fy[i] = d.direction.y; a.fz[i] = d.direction.z;
+            a.speed[i] = d.speed;
+            a.ax[i] = a.ay[i] = a.az[i] = 0.f;
+            a.sx[i] = d.side.x; a.sy[i] = d.side.y; a.sz[i] = d.side.z;
+            a.ux[i] = d.up.x; a.uy[i] = d.up.y; a.uz[i] = d.up.z;
+            a.gx[i] = d.steering.x; a.gy[i] = d.steering.y; a.gz[i] = d.steering.z;
+            a.radius[i] = d.radius; a.max_speed[i] = d.maxSpeed; a.max_force[i] = d.maxForce; a.mass[i] = d.mass;
+            a.bits[i] = d.appliedKernels;
+        }
+        gpusteer::check(gs_group_upload_agents(h_, (uint32_t)n, &a), "gs_group_upload_agents");
+        dirty_ = false;
+        host_valid_ = true;
+    }
+    void SyncHost() {
+        if (host_valid_ || dirty_) return;
+        const size_t n = agents_.size();
+        std::vector<float> f(23 * (n ? n : 1));
+        std::vector<uint32_t> bits(n ? n : 1);
+        gs_agents a = soa(f.data(), bits.data(), n);
+        gpusteer::check(gs_group_download_agents(h_, &a), "gs_group_download_agents");
+        for (size_t i = 0; i < n; ++i) {
+            AgentData& d = agents_[i];
+            d.position = make_float4(a.px[i], a.py[i], a.pz[i], 0.f);
+            d.direction = make_float4(a.fx[i], a.fy[i], a.fz[i], 0.f);
+            d.speed = a.speed[i];
+            d.side = make_float3(a.sx[i], a.sy[i], a.sz[i]);
+            d.up = make_float3(a.ux[i], a.uy[i], a.uz[i]);
+            d.steering = make_float4(a.gx[i], a.gy[i], a.gz[i], 0.f);
+            d.radius = a.radius[i]; d.maxSpeed = a.max_speed[i]; d.maxForce = a.max_force[i]; d.mass = a.mass[i];
+            d.appliedKernels = a.bits[i];
+        }
+        host_valid_ = true;
+    }
+
+private:
+    static gs_agents soa(float* f, uint32_t* bits, size_t n) {
+        gs_agents a;
+        float** p[23] = {&a.px, &a.py, &a.pz, &a.fx, &a.fy, &a.fz, &a.speed, &a.ax, &a.ay, &a.az, &a.sx, &a.sy,
+                         &a.sz, &a.ux, &a.uy, &a.uz, &a.gx, &a.gy, &a.gz, &a.radius, &a.max_speed, &a.max_force,
+                         &a.mass};
+        const size_t stride = n ? n : 1;
+        for (int j = 0; j < 23; ++j) *p[j] = f + j * stride;
+        a.bits = bits;
+        return a;
+    }
+
+    std::vector<AgentData> agents_;
+    std::map<id_type, size_t> index_;
+    bool dirty_ = false;
+    bool host_valid_ = true;
+};
+
+class ObstacleGroup : public BaseGroup {
+public:
+    ObstacleGroup(uint3 const& worldCells, uint const kno, gpusteer::Context& ctx = gpusteer::Context::Default())
+        : BaseGroup(GS_KIND_OBSTACLES, worldCells, kno, ctx) {}
+    void Clear() {
+        obstacles_.clear();
+        n_ = 0;
+        dirty_ = true;
+    }
+    void AddObstacle(ObstacleData const& od) {
+        obstacles_.push_back(od);
+        n_ = (uint)obstacles_.size();
+        dirty_ = true;
+    }
+    void RemoveObstacle(uint const index) {
+        if (index >= obstacles_.size()) return;
+        obstacles_.erase(obstacles_.begin() + index);
+        n_ = (uint)obstacles_.size();
+        dirty_ = true;
+    }
+    bool GetDataForObstacle(uint const index, ObstacleData& od) const {
+        if (index >= obstacles_.size()) return false;
+        od = obstacles_[index];
+        return true;
+    }
+    void SyncDevice() override {
+        if (!dirty_) return;
+        const size_t n = obstacles_.size();
+        std::vector<float> x(n + 1), y(n + 1), z(n + 1), r(n + 1);
+        for (size_t i = 0; i < n; ++i) {
+            x[i] = obstacles_[i].position.x;
+            y[i] = obstacles_[i].position.y;
+            z[i] = obstacles_[i].position.z;
+            r[i] = obstacles_[i].radius;
+        }
+        gpusteer::check(gs_group_upload_obstacles(h_, (uint32_t)n, x.data(), y.data(), z.data(), r.data()),
+                        "gs_group_upload_obstacles");
+        dirty_ = false;
+    }
+
+private:
+    std::vector<ObstacleData> obstacles_;
+    bool dirty_ = false;
+};
+
+// Wall segments (line start, end and normal; include/OpenSteer/CUDA/WallGroupData.cuh).
+class WallGroup : public BaseGroup {
+public:
+    WallGroup(uint3 const& worldCells, uint const knw, gpusteer::Context& ctx = gpusteer::Context::Default())
+        : BaseGroup(GS_KIND_WALLS, worldCells, knw, ctx) {}
+    void Clear() {
+        s_.clear();
+        e_.clear();
+        nrm_.clear();
+        n_ = 0;
+        dirty_ = true;
+    }
+    void AddWall(float3 const& start, float3 const& end, float3 const& normal) {
+        s_.push_back(start);
+        e_.push_back(end);
+        nrm_.push_back(normal);
+        n_ = (uint)s_.size();
+        dirty_ = true;
+    }
+    void SyncDevice() override {
+        if (!dirty_) return;
+        const size_t n = s_.size();
+        std::vector<float> v(9 * (n + 1));
+        gs_walls w;
+        float** p[9] = {&w.sx, &w.sy, &w.sz, &w.ex, &w.ey, &w.ez, &w.nx, &w.ny, &w.nz};
+        for (int j = 0; j < 9; ++j) *p[j] = v.data() + j * (n + 1);
+        for (size_t i = 0; i < n; ++i) {
+            w.sx[i] = s_[i].x; w.sy[i] = s_[i].y; w.sz[i] = s_[i].z;
+            w.ex[i] = e_[i].x; w.ey[i] = e_[i].y; w.ez[i] = e_[i].z;
+            w.nx[i] = nrm_[i].x; w.ny[i] = nrm_[i].y; w.nz[i] = nrm_[i].z;
+        }
+        gpusteer::check(gs_group_upload_walls(h_, (uint32_t)n, &w), "gs_group_upload_walls");
+        dirty_ = false;
+    }
+
+private:
+    std::vector<float3> s_, e_, nrm_;
+    bool dirty_ = false;
+};
+
+class KNNData {
+public:
+    KNNData(uint const nSize, uint const nK, gpusteer::Context& ctx = gpusteer::Context::Default()) : n_(nSize), k_(nK) {
+        gpusteer::check(gs_knn_create(ctx.handle(), nSize, nK, &h_), "gs_knn_create");
+    }
+    ~KNNData() { gs_knn_destroy(h_); }
+    KNNData(const KNNData&) = delete;
+    KNNData& operator=(const KNNData&) = delete;
+    uint k() const { return k_; }
+    uint size() const { return n_; }
+    void setSyncHost() { host_valid_ = false; }
+    std::vector<uint> const& hvKNNIndices() { syncHost(); return idx_; }
+    std::vector<float> const& hvKNNDistances() { syncHost(); return dist_; }
+    // Neighbours of agent `index` (slots 0..k-1; CUDA/KNNData.cu:65-69).
+    void getAgentData(uint const index, uint* pKNNIndices, float* pKNNDistances) {
+        syncHost();
+        for (uint j = 0; j < k_; ++j) {
+            if (pKNNIndices) pKNNIndices[j] = idx_[(size_t)j * n_ + index];
+            if (pKNNDistances) pKNNDistances[j] = dist_[(size_t)j * n_ + index];
+        }
+    }
+    gs_knn handle() const { return h_; }
+
+private:
+    void syncHost() {
+        if (host_valid_) return;
+        idx_.resize((size_t)n_ * k_);
+        dist_.resize((size_t)n_ * k_);
+        gpusteer::check(gs_knn_download(h_, idx_.data(), dist_.data()), "gs_knn_download");
+        host_valid_ = true;
+    }
+    gs_knn h_ = nullptr;
+    uint n_, k_;
+    std::vector<uint> idx_;
+    std::vector<float> dist_;
+    bool host_valid_ = false;
+};
+
+// Uniform lattice centred on the origin (CUDA/KNNBinDataV2.cu:30-42).
+class KNNBinData {
+public:
+    KNNBinData(uint3 const& worldCells, float3 const& worldSize, uint const searchRadius,
+               gpusteer::Context& ctx = gpusteer::Context::Default())
+        : cells_(worldCells), size_(worldSize), radius_(searchRadius) {
+        const uint32_t c[3] = {worldCells.x, worldCells.y, worldCells.z};
+        const float s[3] = {worldSize.x, worldSize.y, worldSize.z};
+        gpusteer::check(gs_bins_create_centered(ctx.handle(), c, s, &h_), "gs_bins_create_centered");
+    }
+    ~KNNBinData() { gs_bins_destroy(h_); }
+    KNNBinData(const KNNBinData&) = delete;
+    KNNBinData& operator=(const KNNBinData&) = delete;
+    uint radius() const { return radius_; }
+    uint3 const& WorldCells() const { return cells_; }
+    float3 const& WorldSize() const { return size_; }
+    uint getNumCells() const { return cells_.x * cells_.y * cells_.z; }
+    bool is3D() const { return cells_.y > 1; }
+    gs_bins handle() const { return h_; }
+
+private:
+    gs_bins h_ = nullptr;
+    uint3 cells_;
+    float3 size_;
+    uint radius_;
+};
+
+// ---- GroupSteerLibrary.cuh free functions -----------------------------------
+
+inline void wrapWorld(AgentGroup* pAgentGroup, float3 const& worldSize) {
+    const float c[3] = {0.f, 0.f, 0.f}, s[3] = {worldSize.x, worldSize.y, worldSize.z};
+    gpusteer::check(gs_wrap_world(pAgentGroup->device_handle(), GS_WRAP_BOX, c, s), "gs_wrap_world");
+    pAgentGroup->SetSyncHost();
+}
+inline void steerForSeek(AgentGroup* pAgentGroup, float3 const& target, float const fWeight, uint const doNotApplyWith) {
+    const float t[3] = {target.x, target.y, target.z};
+    gpusteer::check(gs_steer_seek(pAgentGroup->device_handle(), t, fWeight, doNotApplyWith,
+                                  pAgentGroup->context().Mode()),
+                    "gs_steer_seek");
+    pAgentGroup->SetSyncHost();
+}
+inline void steerToAvoidObstacles(AgentGroup* pAgentGroup, ObstacleGroup* pObstacleGroup, KNNData* pKNNData,
+                                  float const fMinTimeToCollision, float const fWeight, uint const doNotApplyWith) {
+    gpusteer::check(gs_steer_avoid_obstacles(pAgentGroup->device_handle(), pObstacleGroup->device_handle(),
+                                             pKNNData->handle(), fMinTimeToCollision, fWeight, doNotApplyWith),
+                    "gs_steer_avoid_obstacles");
+    pAgentGroup->SetSyncHost();
+}
+inline void steerToAvoidWalls(AgentGroup* pAgentGroup, KNNData* pKNNData, WallGroup* pWallGroup,
+                              float const fMinTimeToCollision, float const fWeight, uint const doNotApplyWith) {
+    gpusteer::check(gs_steer_avoid_walls(pAgentGroup->device_handle(), pKNNData->handle(),
+                                         pWallGroup->device_handle(), fMinTimeToCollision, fWeight, doNotApplyWith),
+                    "gs_steer_avoid_walls");
+    pAgentGroup->SetSyncHost();
+}
+inline void updateKNNDatabase(BaseGroup* pGroup, KNNBinData* pKNNBinData) {
+    gpusteer::check(gs_update_knn_database(pGroup->device_handle(), pKNNBinData->handle()), "gs_update_knn_database");
+}
+inline void findKNearestNeighbors(AgentGroup* pAgentGroup, KNNData* pKNNData, KNNBinData* pKNNBinData,
+                                  BaseGroup* pOtherGroup, uint const searchRadius) {
+    gpusteer::check(gs_find_knn(pAgentGroup->device_handle(), pKNNData->handle(), pKNNBinData->handle(),
+                                pOtherGroup->device_handle(), searchRadius, GS_KNN_DEFAULT),
+                    "gs_find_knn");
+    pKNNData->setSyncHost();
+}
+inline void steerForSeparation(AgentGroup* pAgentGroup, KNNData* pKNNData, AgentGroup* pOtherGroup,
+                               float const minDistance, float const maxDistance, float const cosMaxAngle,
+                               float const fWeight, uint const doNotApplyWith) {
+    gpusteer::check(gs_steer_separation(pAgentGroup->device_handle(), pKNNData->handle(),
+                                        pOtherGroup->device_handle(), minDistance, maxDistance, cosMaxAngle, fWeight,
+                                        doNotApplyWith, pAgentGroup->context().Mode()),
+                    "gs_steer_separation");
+    pAgentGroup->SetSyncHost();
+}
+inline void steerForAlignment(AgentGroup* pAgentGroup, KNNData* pKNNData, AgentGroup* pOtherGroup,
+                              float const minDistance, float const maxDistance, float const cosMaxAngle,
+                              float const fWeight, uint const doNotApplyWith) {
+    gpusteer::check(gs_steer_alignment(pAgentGroup->device_handle(), pKNNData->handle(),
+                                       pOtherGroup->device_handle(), minDistance, maxDistance, cosMaxAngle, fWeight,
+                                       doNotApplyWith, pAgentGroup->context().Mode()),
+                    "gs_steer_alignment");
+    pAgentGroup->SetSyncHost();
+}
+inline void steerForCohesion(AgentGroup* pAgentGroup, KNNData* pKNNData, AgentGroup* pOtherGroup,
+                             float const minDistance, float const maxDistance, float const cosMaxAngle,
+                             float const fWeight, uint const doNotApplyWith) {
+    gpusteer::check(gs_steer_cohesion(pAgentGroup->device_handle(), pKNNData->handle(), pOtherGroup->device_handle(),
+                                      minDistance, maxDistance, cosMaxAngle, fWeight, doNotApplyWith,
+                                      pAgentGroup->context().Mode()),
+                    "gs_steer_cohesion");
+    pAgentGroup->SetSyncHost();
+}
+inline void updateGroup(AgentGroup* pAgentGroup, const float elapsedTime) {
+    gpusteer::check(gs_update_group(pAgentGroup->device_handle(), elapsedTime, pAgentGroup->context().Mode()),
+                    "gs_update_group");
+    pAgentGroup->SetSyncHost();
+}
+
+}  // namespace OpenSteer
+
+#endif  // GPUSTEER_OPENSTEER_HPP
