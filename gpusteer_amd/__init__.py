@@ -30,4 +30,5 @@ from .api import (  # noqa: F401
     updateKNNDatabase,
     wrapWorld,
 )
+from .ensemble import Ensemble, Sequential  # noqa: F401
 from .state import AgentState, WallState  # noqa: F401

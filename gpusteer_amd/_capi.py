@@ -91,6 +91,38 @@ class GsBoidsTiming(C.Structure):
                 ("step_ms", C.c_float)]
 
 
+GS_SCENARIO_BOIDS, GS_SCENARIO_CHOKE = 0, 1
+GS_ENSEMBLE_GRAPH, GS_ENSEMBLE_EAGER = 0, 1
+
+
+class GsScenario(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("n", C.c_uint32 * 2),
+        ("origin", C.c_float * 3),
+        ("size", C.c_float * 3),
+        ("cells", C.c_uint32 * 3),
+        ("dt", C.c_float),
+        ("mode", C.c_int),
+        ("wrap_shape", C.c_int),
+        ("wrap_center", C.c_float * 3),
+        ("wrap_size", C.c_float * 3),
+        ("separation", GsFlockTerm),
+        ("alignment", GsFlockTerm),
+        ("cohesion", GsFlockTerm),
+        ("choke_separation", GsFlockTerm),
+        ("goal", (C.c_float * 3) * 2),
+        ("seek_weight", C.c_float),
+        ("seek_mask", C.c_uint32),
+        ("walls_min_time", C.c_float),
+        ("walls_weight", C.c_float),
+        ("obstacles_min_time", C.c_float),
+        ("obstacles_weight", C.c_float),
+        ("k_walls", C.c_uint32),
+        ("k_obstacles", C.c_uint32),
+    ]
+
+
 class GpuSteerError(RuntimeError):
     def __init__(self, status: int, where: str, detail: str):
         self.status = status
@@ -155,6 +187,15 @@ _SIGS = {
     "gs_clear_bits": (C.c_int, [_h]),
     "gs_step_boids": (C.c_int, [_h, _h, _h, C.POINTER(GsBoidsParams), C.c_uint32]),
     "gs_step_boids_timed": (C.c_int, [_h, _h, _h, C.POINTER(GsBoidsParams), C.c_uint32, C.POINTER(GsBoidsTiming)]),
+    "gs_ensemble_create": (C.c_int, [_h, C.c_uint32, C.POINTER(GsScenario), C.c_uint32, C.POINTER(_h)]),
+    "gs_ensemble_destroy": (C.c_int, [_h]),
+    "gs_ensemble_slots": (C.c_int, [_h, _up]),
+    "gs_ensemble_upload_agents": (C.c_int, [_h, C.c_uint32, C.c_uint32, C.POINTER(GsAgents)]),
+    "gs_ensemble_download_agents": (C.c_int, [_h, C.c_uint32, C.c_uint32, C.POINTER(GsAgents)]),
+    "gs_ensemble_upload_walls": (C.c_int, [_h, C.c_uint32, C.c_uint32, C.POINTER(GsWalls)]),
+    "gs_ensemble_upload_obstacles": (C.c_int, [_h, C.c_uint32, C.c_uint32, _fp, _fp, _fp, _fp]),
+    "gs_ensemble_download_knn": (C.c_int, [_h, C.c_uint32, C.c_uint32, _up]),
+    "gs_ensemble_step": (C.c_int, [_h, C.c_uint32, C.c_uint32]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)

@@ -10,6 +10,7 @@
 #include <unordered_set>
 #include <vector>
 
+#include "gs_ensemble.hpp"
 #include "gs_internal.hpp"
 
 namespace gs {
@@ -888,6 +889,250 @@ int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_par
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
     return status;
+}
+
+
+// ---- batched ensemble -----------------------------------------------------------
+
+static gs_ensemble_s* check_ens(gs_ensemble e) { return check(e, "ensemble"); }
+
+static void release_ensemble(gs_ensemble_s* e) {
+    for (int p = 0; p < 2; ++p) {
+        if (e->exec[p]) (void)hipGraphExecDestroy(e->exec[p]);
+        if (e->graph[p]) (void)hipGraphDestroy(e->graph[p]);
+        e->exec[p] = nullptr;
+        e->graph[p] = nullptr;
+    }
+}
+
+static void drop_graphs(gs_ensemble_s* e) { release_ensemble(e); }
+
+int gs_ensemble_create(gs_context ctx, uint32_t ns, const gs_scenario* sc, uint32_t k, gs_ensemble* out) {
+    return guarded([&] {
+        check(ctx, "context");
+        if (!out) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
+        *out = nullptr;
+        if (ns > 0 && !sc) fail(GS_ERR_INVALID_ARGUMENT, "scenarios is NULL");
+        if (k < 1 || k > 8) fail(GS_ERR_UNSUPPORTED, "ensemble k must be 1..8");
+        uint64_t slots = 0, cells = 0;
+        for (uint32_t i = 0; i < ns; ++i) {
+            const gs_scenario& d = sc[i];
+            if (d.kind != GS_SCENARIO_BOIDS && d.kind != GS_SCENARIO_CHOKE)
+                fail(GS_ERR_INVALID_ARGUMENT, "scenario " + std::to_string(i) + ": unknown kind");
+            if (d.mode != GS_MODE_OPENSTEER && d.mode != GS_MODE_GPUREF)
+                fail(GS_ERR_INVALID_ARGUMENT, "scenario " + std::to_string(i) + ": unknown mode");
+            if (d.kind == GS_SCENARIO_BOIDS && d.wrap_shape != GS_WRAP_BOX && d.wrap_shape != GS_WRAP_SPHERE &&
+                d.wrap_shape != -1)
+                fail(GS_ERR_INVALID_ARGUMENT, "scenario " + std::to_string(i) + ": unknown wrap shape");
+            if (d.kind == GS_SCENARIO_CHOKE &&
+                (d.k_walls < 1 || d.k_walls > 8 || d.k_obstacles < 1 || d.k_obstacles > 8))
+                fail(GS_ERR_UNSUPPORTED, "scenario " + std::to_string(i) + ": k_walls / k_obstacles must be 1..8");
+            (void)make_grid(d.origin, d.size, d.cells);
+            const int ng = d.kind == GS_SCENARIO_CHOKE ? 2 : 1;
+            for (int g = 0; g < ng; ++g) {
+                slots += (d.n[g] + kBlock - 1) / kBlock * (uint64_t)kBlock;
+                cells += (uint64_t)d.cells[0] * d.cells[1] * d.cells[2] + 1;
+            }
+        }
+        if (slots >= (1ull << 31) || cells >= (1ull << 31)) fail(GS_ERR_UNSUPPORTED, "ensemble too large");
+        use_device(ctx);
+        auto* e = new gs_ensemble_s();
+        e->ctx = ctx;
+        try {
+            ens_init(e, ns, sc, k, make_grid);
+            e->walls_h.assign(ns, {});
+            e->obs_h.assign(ns, {});
+            ens_alloc(e);
+            ens_upload_geometry(e);
+            GS_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            delete e;
+            throw;
+        }
+        reg_add(e);
+        *out = e;
+    });
+}
+
+int gs_ensemble_destroy(gs_ensemble e) {
+    return guarded([&] {
+        check_ens(e);
+        reg_del(e);
+        use_device(e->ctx);
+        (void)hipStreamSynchronize(e->ctx->stream);
+        release_ensemble(e);
+        delete e;
+    });
+}
+
+int gs_ensemble_slots(gs_ensemble e, uint32_t* slots) {
+    return guarded([&] {
+        check_ens(e);
+        if (!slots) fail(GS_ERR_INVALID_ARGUMENT, "slots is NULL");
+        *slots = e->N;
+    });
+}
+
+static const EnsPop& ens_pop(gs_ensemble_s* e, uint32_t scenario, uint32_t group) {
+    if (scenario >= e->scen.size()) fail(GS_ERR_INVALID_ARGUMENT, "scenario index out of range");
+    const int ng = e->scen[scenario].kind == GS_SCENARIO_CHOKE ? 2 : 1;
+    if ((int)group >= ng) fail(GS_ERR_INVALID_ARGUMENT, "group index out of range");
+    for (const auto& P : e->pops)
+        if (P.scen == scenario && P.group == (int)group) return P;
+    fail(GS_ERR_INVALID_ARGUMENT, "no such population");
+}
+
+int gs_ensemble_upload_agents(gs_ensemble e, uint32_t scenario, uint32_t group, const gs_agents* host) {
+    return guarded([&] {
+        check_ens(e);
+        const EnsPop& P = ens_pop(e, scenario, group);
+        if (P.n && !host) fail(GS_ERR_INVALID_ARGUMENT, "host arrays are NULL");
+        const float* const* hf = (const float* const*)host;
+        for (int j = 0; j < kFields && P.n; ++j)
+            if (!hf[j]) fail(GS_ERR_INVALID_ARGUMENT, "agent array is NULL");
+        use_device(e->ctx);
+        hipStream_t st = e->ctx->stream;
+        GS_HIP(hipStreamSynchronize(st));
+        std::vector<uint32_t> ids(P.n);
+        for (uint32_t i = 0; i < P.n; ++i) ids[i] = i;
+        for (int j = 0; j < kFields && P.n; ++j) h2d(e->F(j) + P.a0, hf[j], (size_t)P.n * 4, st);
+        std::vector<uint32_t> bits(P.n, 0u);
+        if (P.n && host->bits) memcpy(bits.data(), host->bits, (size_t)P.n * 4);
+        if (P.n) {
+            h2d(e->bits() + P.a0, bits.data(), (size_t)P.n * 4, st);
+            h2d(e->ids() + P.a0, ids.data(), (size_t)P.n * 4, st);
+        }
+        GS_HIP(hipStreamSynchronize(st));
+        e->seeded = false;
+    });
+}
+
+int gs_ensemble_download_agents(gs_ensemble e, uint32_t scenario, uint32_t group, gs_agents* host) {
+    return guarded([&] {
+        check_ens(e);
+        const EnsPop& P = ens_pop(e, scenario, group);
+        if (!host) fail(GS_ERR_INVALID_ARGUMENT, "host arrays are NULL");
+        if (!P.n) return;
+        use_device(e->ctx);
+        hipStream_t st = e->ctx->stream;
+        auto ids = d2h_vec(e->ids() + P.a0, P.n, st);
+        GS_HIP(hipStreamSynchronize(st));
+        for (uint32_t s = 0; s < P.n; ++s)
+            if (ids[s] >= P.n) fail(GS_ERR_DEVICE, "ensemble id map corrupted");
+        float* const* hf = (float* const*)host;
+        for (int j = 0; j < kFields; ++j) {
+            if (!hf[j]) continue;
+            auto v = d2h_vec(e->F(j) + P.a0, P.n, st);
+            GS_HIP(hipStreamSynchronize(st));
+            for (uint32_t s = 0; s < P.n; ++s) hf[j][ids[s]] = v[s];
+        }
+        if (host->bits) {
+            auto b = d2h_vec(e->bits() + P.a0, P.n, st);
+            GS_HIP(hipStreamSynchronize(st));
+            for (uint32_t s = 0; s < P.n; ++s) host->bits[ids[s]] = b[s];
+        }
+    });
+}
+
+int gs_ensemble_upload_walls(gs_ensemble e, uint32_t scenario, uint32_t n, const gs_walls* host) {
+    return guarded([&] {
+        check_ens(e);
+        if (scenario >= e->scen.size()) fail(GS_ERR_INVALID_ARGUMENT, "scenario index out of range");
+        if (n && !host) fail(GS_ERR_INVALID_ARGUMENT, "wall arrays are NULL");
+        const float* const* hw = (const float* const*)host;
+        for (int j = 0; j < 9 && n; ++j)
+            if (!hw[j]) fail(GS_ERR_INVALID_ARGUMENT, "wall array is NULL");
+        auto& v = e->walls_h[scenario];
+        v.assign((size_t)n * 9, 0.f);
+        for (uint32_t i = 0; i < n; ++i)
+            for (int j = 0; j < 9; ++j) v[(size_t)i * 9 + j] = hw[j][i];
+        use_device(e->ctx);
+        drop_graphs(e);
+        ens_upload_geometry(e);
+        GS_HIP(hipStreamSynchronize(e->ctx->stream));
+    });
+}
+
+int gs_ensemble_upload_obstacles(gs_ensemble e, uint32_t scenario, uint32_t n, const float* cx, const float* cy,
+                                 const float* cz, const float* radius) {
+    return guarded([&] {
+        check_ens(e);
+        if (scenario >= e->scen.size()) fail(GS_ERR_INVALID_ARGUMENT, "scenario index out of range");
+        if (n && (!cx || !cy || !cz || !radius)) fail(GS_ERR_INVALID_ARGUMENT, "obstacle arrays are NULL");
+        auto& v = e->obs_h[scenario];
+        v.assign((size_t)n * 4, 0.f);
+        for (uint32_t i = 0; i < n; ++i) {
+            v[(size_t)i * 4 + 0] = cx[i];
+            v[(size_t)i * 4 + 1] = cy[i];
+            v[(size_t)i * 4 + 2] = cz[i];
+            v[(size_t)i * 4 + 3] = radius[i];
+        }
+        use_device(e->ctx);
+        drop_graphs(e);
+        ens_upload_geometry(e);
+        GS_HIP(hipStreamSynchronize(e->ctx->stream));
+    });
+}
+
+int gs_ensemble_download_knn(gs_ensemble e, uint32_t scenario, uint32_t group, uint32_t* idx) {
+    return guarded([&] {
+        check_ens(e);
+        const EnsPop& P = ens_pop(e, scenario, group);
+        if (!idx) fail(GS_ERR_INVALID_ARGUMENT, "idx is NULL");
+        if (!e->seeded) fail(GS_ERR_NOT_BINNED, "the ensemble has no KNN result yet");
+        if (!P.n) return;
+        use_device(e->ctx);
+        hipStream_t st = e->ctx->stream;
+        auto ids = d2h_vec(e->ids(), e->N, st);
+        std::vector<uint32_t> col((size_t)e->k * P.n);
+        for (uint32_t j = 0; j < e->k; ++j)
+            d2h(col.data() + (size_t)j * P.n, e->knn_self[e->kcur].p + (size_t)j * e->N + P.a0, (size_t)P.n * 4, st);
+        GS_HIP(hipStreamSynchronize(st));
+        for (uint32_t j = 0; j < e->k; ++j)
+            for (uint32_t s = 0; s < P.n; ++s) {
+                const uint32_t v = col[(size_t)j * P.n + s];
+                idx[(size_t)j * P.n + ids[P.a0 + s]] = v == UINT32_MAX ? UINT32_MAX : ids[v];
+            }
+    });
+}
+
+int gs_ensemble_step(gs_ensemble e, uint32_t steps, uint32_t flags) {
+    return guarded([&] {
+        check_ens(e);
+        if (flags & ~(uint32_t)GS_ENSEMBLE_EAGER) fail(GS_ERR_INVALID_ARGUMENT, "unknown flags");
+        use_device(e->ctx);
+        hipStream_t st = e->ctx->stream;
+        uint32_t left = steps;
+        if (left && !e->seeded) {
+            ens_step(e);  // the first step has no previous KNN to bound its search
+            --left;
+        }
+        if (flags & GS_ENSEMBLE_EAGER) {
+            for (; left; --left) ens_step(e);
+        } else {
+            while (left >= 2) {
+                const int par = e->bcur;
+                if (!e->exec[par]) {
+                    GS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                    try {
+                        ens_step(e);
+                        ens_step(e);
+                    } catch (...) {
+                        hipGraph_t g = nullptr;
+                        (void)hipStreamEndCapture(st, &g);
+                        if (g) (void)hipGraphDestroy(g);
+                        throw;
+                    }
+                    GS_HIP(hipStreamEndCapture(st, &e->graph[par]));
+                    GS_HIP(hipGraphInstantiate(&e->exec[par], e->graph[par], nullptr, nullptr, 0));
+                }
+                GS_HIP(hipGraphLaunch(e->exec[par], st));
+                left -= 2;
+            }
+            if (left) ens_step(e);
+        }
+        GS_HIP(hipGetLastError());
+    });
 }
 
 }  // extern "C"

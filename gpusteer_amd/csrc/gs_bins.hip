@@ -104,19 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_key(uint32_t n, float* __restric
                        axis_cell(z, g.oz, g.sz, g.dz));
         key[i] = c;
     }
-    // Storage is in cell order from the previous build, so equal cells sit in
-    // runs of lanes: one atomic per run, ranks inside the run in lane order.
-    const uint32_t prev = __shfl_up(c, 1, 64);
-    const bool head = lane == 0 || prev != c;
-    const uint64_t heads = __ballot(head);
-    const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-    const int head_lane = 63 - __clzll((unsigned long long)(heads & upto));
-    const uint64_t after = heads & ~upto;
-    const int run_end = after ? (__ffsll((unsigned long long)after) - 1) : 64;
-    uint32_t base = 0;
-    if (head && active) base = atomicAdd(&count[c], (uint32_t)(run_end - lane));
-    base = __shfl(base, head_lane, 64);
-    if (active) rank[i] = base + (uint32_t)(lane - head_lane);
+    count_and_rank(active, c, i, count, rank);
 }
 
 // Wavefront-then-block exclusive scan of one value per thread.
@@ -214,13 +202,6 @@ __global__ __launch_bounds__(kBlock) void k_scatter(uint32_t n, const uint32_t* 
     }
 }
 
-// Field pointers of one reorder (current -> alternate buffers).
-struct FieldSet {
-    const float* src[kFields];
-    float* dst[kFields];
-    int nf;
-};
-
 // Pass 4: new position t initially holds old slot s = perm_tmp[t] (arrival
 // order).  Its final position is the cell start plus the number of members of
 // its cell with a smaller agent index.
@@ -257,6 +238,29 @@ __global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* 
     spos[d] = make_float4(fs.src[0][s], fs.src[1][s], fs.src[2][s], __uint_as_float(id));
 }
 
+void launch_scan_counts(uint32_t* count, uint32_t* start, uint32_t C, hipStream_t st) {
+    const uint32_t ntiles = (C + kScanTile - 1) / kScanTile;
+    uint32_t* partial = start + C + 1;
+    k_scan_tiles<<<ntiles, kBlock, 0, st>>>(C, count, start, partial);
+    GS_LAUNCHED("k_scan_tiles");
+    k_scan_partials<<<1, kBlock, 0, st>>>(ntiles, partial, start, C);
+    GS_LAUNCHED("k_scan_partials");
+    k_scan_add<<<ntiles, kBlock, 0, st>>>(C, partial, start, count);
+    GS_LAUNCHED("k_scan_add");
+}
+
+void launch_scatter_reorder(uint32_t n, const uint32_t* key, const uint32_t* rank, const uint32_t* start,
+                            const uint32_t* ids, uint32_t* ids_new, uint32_t* perm_tmp, uint32_t* id_tmp,
+                            uint32_t* new_to_old, uint32_t* old_to_new, const uint32_t* bits, uint32_t* bits_new,
+                            const FieldSet& fs, float4* spos, hipStream_t st) {
+    if (n == 0) return;
+    k_scatter<<<blocks_for(n), kBlock, 0, st>>>(n, key, rank, start, ids, perm_tmp, id_tmp);
+    GS_LAUNCHED("k_scatter");
+    k_reorder<<<blocks_for(n), kBlock, 0, st>>>(n, perm_tmp, id_tmp, key, start, ids, ids_new, new_to_old,
+                                                old_to_new, bits, bits_new, fs, spos);
+    GS_LAUNCHED("k_reorder");
+}
+
 void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws) {
     const uint32_t n = g->n;
     const uint32_t C = b->cells;
@@ -279,7 +283,6 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
     }
     g->bin_cells = C;
     g->bin_grid = b->g;
-    uint32_t* partial = g->start.p + C + 1;
     float wcv[3] = {0, 0, 0}, wsv[3] = {0, 0, 0};
     if (wc) memcpy(wcv, wc, sizeof wcv);
     if (ws) memcpy(wsv, ws, sizeof wsv);
@@ -289,16 +292,8 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
                                                     wsv[2]);
         GS_LAUNCHED("k_bin_key");
     }
-    k_scan_tiles<<<ntiles, kBlock, 0, st>>>(C, g->count.p, g->start.p, partial);
-    GS_LAUNCHED("k_scan_tiles");
-    k_scan_partials<<<1, kBlock, 0, st>>>(ntiles, partial, g->start.p, C);
-    GS_LAUNCHED("k_scan_partials");
-    k_scan_add<<<ntiles, kBlock, 0, st>>>(C, partial, g->start.p, g->count.p);
-    GS_LAUNCHED("k_scan_add");
+    launch_scan_counts(g->count.p, g->start.p, C, st);
     if (n > 0) {
-        k_scatter<<<blocks_for(n), kBlock, 0, st>>>(n, g->key.p, g->rank.p, g->start.p, g->ids(), g->perm_tmp.p,
-                                                    g->id_tmp.p);
-        GS_LAUNCHED("k_scatter");
         FieldSet fs{};
         int nf = 0;
         for (int f = 0; f < kFields; ++f)
@@ -309,11 +304,9 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
             }
         fs.nf = nf;
         const bool has_bits = g->kind == GS_KIND_AGENTS;
-        k_reorder<<<blocks_for(n), kBlock, 0, st>>>(n, g->perm_tmp.p, g->id_tmp.p, g->key.p, g->start.p, g->ids(), g->ids_alt(),
-                                                    g->new_to_old.p, g->old_to_new.p,
-                                                    has_bits ? g->bits() : nullptr,
-                                                    has_bits ? g->bitsb[g->bcur ^ 1].p : nullptr, fs, g->spos.p);
-        GS_LAUNCHED("k_reorder");
+        launch_scatter_reorder(n, g->key.p, g->rank.p, g->start.p, g->ids(), g->ids_alt(), g->perm_tmp.p,
+                               g->id_tmp.p, g->new_to_old.p, g->old_to_new.p, has_bits ? g->bits() : nullptr,
+                               has_bits ? g->bitsb[g->bcur ^ 1].p : nullptr, fs, g->spos.p, st);
         for (int f = 0; f < kFields; ++f)
             if (g->live & (1u << f)) g->flip(f);
         if (has_bits) g->bcur ^= 1;

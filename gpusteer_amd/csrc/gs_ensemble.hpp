@@ -1,0 +1,80 @@
+// Batched-ensemble state shared by gs_ensemble.hip (kernels, step) and
+// gs_api.hip (entry points).
+#pragma once
+#include <vector>
+
+#include "gs_ops.hpp"
+
+namespace gs {
+
+struct EnsPop {
+    uint32_t a0, n, c0, cells;  // slots [a0, a0 + n) are agents; cells c0 .. c0 + cells - 1
+    GridDev g;
+    int kind, group, other;     // other: the crowd's other population (-1 for Boids)
+    uint32_t scen;
+};
+
+struct EnsScen {
+    int kind, mode;
+    float dt;
+    int wrap_shape;
+    float wc[3], ws[3];
+    FlockTerm sep, ali, coh;    // Boids
+    FlockTerm csep;             // crowd separation
+    float goal[2][3];
+    float w_seek;
+    uint32_t mask_seek;
+    float mt_walls, w_walls, mt_obs, w_obs;
+    uint32_t k_walls, k_obs;
+    uint32_t w0, nw, o0, no;    // ranges in the wall / obstacle arrays
+};
+
+}  // namespace gs
+
+struct gs_ensemble_s {
+    gs_context ctx = nullptr;
+    uint32_t k = 0;
+    uint32_t N = 0, C = 0;  // padded slots, cells (with one extra per population)
+    std::vector<gs_scenario> scen;
+    std::vector<gs::EnsPop> pops;
+    std::vector<gs::EnsScen> scens_host;
+    std::vector<uint32_t> block_pop_h, blocks_boids, blocks_g0, blocks_g1, blocks_choke;
+    gs::DevBuf<gs::EnsPop> d_pops;
+    gs::DevBuf<gs::EnsScen> d_scens;
+    gs::DevBuf<uint32_t> d_block_pop, d_blocks_boids, d_blocks_g0, d_blocks_g1, d_blocks_choke;
+
+    gs::DevBuf<float> fb[kFields][2];
+    uint8_t fcur[kFields] = {};
+    gs::DevBuf<uint32_t> bitsb[2], idb[2];
+    uint8_t bcur = 0, icur = 0;
+    gs::DevBuf<uint32_t> knn_self[2], knn_other[2];
+    uint8_t kcur = 0;
+    bool seeded = false;
+    uint32_t kw_max = 1, ko_max = 1;
+    gs::DevBuf<uint32_t> knn_walls, knn_obs;
+    gs::DevBuf<uint32_t> key, rank, perm_tmp, id_tmp, new_to_old, old_to_new, count, start;
+    gs::DevBuf<float4> spos;
+
+    // Walls: start, end, normal, midpoint (12 arrays); obstacles: centre, radius.
+    gs::DevBuf<float> walls[12], obs[4];
+    std::vector<std::vector<float>> walls_h, obs_h;  // per scenario, 9 / 4 floats per entry
+    bool walls_mid_ready = false;
+
+    // Captured pairs of steps, one per buffer parity.
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    uint32_t magic = 0x67736573u;
+
+    float* F(int f) const { return fb[f][fcur[f]].p; }
+    float* Falt(int f) const { return fb[f][fcur[f] ^ 1].p; }
+    uint32_t* bits() const { return bitsb[bcur].p; }
+    uint32_t* ids() const { return idb[icur].p; }
+};
+
+namespace gs {
+void ens_init(gs_ensemble_s* e, uint32_t ns, const gs_scenario* sc, uint32_t k,
+              GridDev (*make_grid)(const float*, const float*, const uint32_t*));
+void ens_alloc(gs_ensemble_s* e);
+void ens_upload_geometry(gs_ensemble_s* e);
+void ens_step(gs_ensemble_s* e);
+}  // namespace gs

@@ -215,6 +215,44 @@ struct gs_knn_s {
 };
 
 namespace gs {
+// Field pointers of one reorder (current -> alternate buffers).
+struct FieldSet {
+    const float* src[kFields];
+    float* dst[kFields];
+    int nf;
+};
+
+// Wave-aggregated histogram + rank: lanes holding the same cell in a run of
+// consecutive lanes (storage is in cell order from the previous build) take
+// one atomic per run; rank[i] = the lane's arrival position in its cell.
+// Inactive lanes must hold a key no cell has, distinct per lane.
+__device__ __forceinline__ void count_and_rank(bool active, uint32_t c, uint32_t i, uint32_t* __restrict__ count,
+                                               uint32_t* __restrict__ rank) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(c, 1, 64);
+    const bool head = lane == 0 || prev != c;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const int head_lane = 63 - __clzll((unsigned long long)(heads & upto));
+    const uint64_t after = heads & ~upto;
+    const int run_end = after ? (__ffsll((unsigned long long)after) - 1) : 64;
+    uint32_t base = 0;
+    if (head && active) base = atomicAdd(&count[c], (uint32_t)(run_end - lane));
+    base = __shfl(base, head_lane, 64);
+    if (active) rank[i] = base + (uint32_t)(lane - head_lane);
+}
+
+// Exclusive scan of C counts into start[0..C] (start needs C + 1 + ceil(C /
+// 2048) entries: the tile sums live after start[C]); clears the counts.
+void launch_scan_counts(uint32_t* count, uint32_t* start, uint32_t C, hipStream_t st);
+// Stable scatter of n keyed slots into cell order and the gather of every
+// field in fs, bits and ids into the new order; writes the order maps and
+// the packed position snapshot.
+void launch_scatter_reorder(uint32_t n, const uint32_t* key, const uint32_t* rank, const uint32_t* start,
+                            const uint32_t* ids, uint32_t* ids_new, uint32_t* perm_tmp, uint32_t* id_tmp,
+                            uint32_t* new_to_old, uint32_t* old_to_new, const uint32_t* bits, uint32_t* bits_new,
+                            const FieldSet& fs, float4* spos, hipStream_t st);
+
 // Host-side launchers (one per operation) implemented in the .hip files.
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t st);

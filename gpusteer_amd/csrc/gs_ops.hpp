@@ -150,6 +150,113 @@ __device__ __forceinline__ bool lines_intersect(f3 sa, f3 ea, f3 sb, f3 eb, f3& 
     return false;
 }
 
+// Obstacle avoidance of one agent against the first k entries of its KNN
+// column (knn[j * stride], stopping at the first index >= nO):
+// SteerToAvoidObstaclesKernel + findNextIntersectionWithSphere
+// (CUDA/AvoidObstaclesCUDAKernel.cu:47-99,136-253).  Applies the result to
+// steering / bits with the accumulate convention.
+__device__ __forceinline__ void avoid_obstacles_one(f3 pos, f3 fwd, f3 side, f3 up, float radius, float speed,
+                                                    const uint32_t* __restrict__ knn, uint32_t stride, uint32_t k,
+                                                    uint32_t nO, const float* __restrict__ ox,
+                                                    const float* __restrict__ oy, const float* __restrict__ oz,
+                                                    const float* __restrict__ orad, float min_time, float w,
+                                                    f3& steering, uint32_t& bits) {
+    const float min_dist = min_time * speed;
+    bool n_hit = false;
+    float n_dist = 0.f;
+    f3 n_pos = mk3(0.f, 0.f, 0.f);
+    for (uint32_t j = 0; j < k; ++j) {
+        const uint32_t o = knn[(size_t)j * stride];
+        if (o >= nO) break;
+        const f3 op = ld3(ox, oy, oz, o);
+        const float orr = orad[o];
+        bool x_hit = false;
+        float x_dist = 0.f;
+        const f3 go = op - pos;
+        const f3 lc = mk3(dot3(go, side), dot3(go, up), dot3(go, fwd));
+        const float b = -2.f * lc.z;
+        const float c = ((lc.x * lc.x + lc.y * lc.y) + lc.z * lc.z) - (orr + radius) * (orr + radius);
+        const float d = (b * b) - (4.f * c);
+        if (!(d < 0.f)) {
+            const float s = sqrtf(d);
+            const float p = (-b + s) / 2.f;
+            const float q = (-b - s) / 2.f;
+            if (!((p < 0.f) && (q < 0.f))) {
+                x_hit = true;
+                x_dist = ((p > 0.f) && (q > 0.f)) ? ((p < q) ? p : q) : ((p > 0.f) ? p : q);
+            }
+        }
+        if ((n_hit == false) || ((x_hit != false) && (x_dist < n_dist))) {
+            n_hit = x_hit;
+            n_dist = x_dist;
+            n_pos = op;
+        }
+    }
+    f3 f = mk3(0.f, 0.f, 0.f);
+    if (n_hit && (n_dist < min_dist)) f = perp3(pos - n_pos, fwd);
+    apply(steering, bits, f, w, GS_BIT_AVOID_OBSTACLES);
+}
+
+// Wall avoidance of one agent against the first k entries of its KNN column:
+// SteerToAvoidWallsCUDAKernel (CUDA/AvoidWallsCUDAKernel.cu:67-245) — three
+// feelers, the touching test, the anti-penetration move of a touching agent
+// (:215-223, updates pos / dir) and its bit rules.  steering += w * force.
+__device__ __forceinline__ void avoid_walls_one(f3& pos, f3& dir, f3 side, float radius, float speed,
+                                                const uint32_t* __restrict__ knn, uint32_t stride, uint32_t k,
+                                                uint32_t nW, const gs_walls& W, float min_time, float w,
+                                                f3& steering_acc, uint32_t& bits) {
+    const float fl = 4.f * radius;  // FEELER_LENGTH (:41) * radius
+    const f3 vel = dir * speed;
+    f3 feelers[3];
+    feelers[0] = pos + vel * min_time;
+    feelers[1] = pos + (side * fl + dir * fl);
+    feelers[2] = pos + ((-side) * fl + dir * fl);
+    int feeler = 0;
+    float nearest_dist = FLT_MAX;
+    uint32_t nearest_line = UINT32_MAX;
+    f3 nearest_ip = mk3(0.f, 0.f, 0.f), nearest_normal = nearest_ip;
+    bool touching = false;
+    for (uint32_t j = 0; j < k; ++j) {
+        const uint32_t l = knn[(size_t)j * stride];
+        if (l >= nW) break;
+        const f3 ln = ld3(W.nx, W.ny, W.nz, l);
+        const f3 ls = ld3(W.sx, W.sy, W.sz, l);
+        const f3 le = ld3(W.ex, W.ey, W.ez, l);
+        f3 ip;
+        if (lines_intersect(pos, pos + (-ln) * radius, ls, le, ip)) {
+            touching = true;
+            nearest_normal = ln;
+            nearest_dist = len3(pos - ip);
+        }
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            if (lines_intersect(pos, feelers[f], ls, le, ip)) {
+                const float dist = len3(pos - ip);
+                if (dist < nearest_dist) {
+                    nearest_dist = dist;
+                    nearest_line = l;
+                    nearest_ip = ip;
+                    feeler = f;
+                    nearest_normal = ln;
+                }
+            }
+        }
+    }
+    f3 steering = mk3(0.f, 0.f, 0.f);
+    if (touching) {
+        const float pen = radius - nearest_dist;
+        pos = pos + nearest_normal * pen;
+        dir = normalize3(perp3(dir, nearest_normal));
+        bits |= GS_BIT_ANTI_PENETRATION_WALL;
+    } else if (nearest_line != UINT32_MAX) {
+        const f3 fe = feeler == 0 ? feelers[0] : (feeler == 1 ? feelers[1] : feelers[2]);
+        steering = nearest_normal * len3(fe - nearest_ip);
+        bits |= GS_BIT_AVOID_WALLS;
+    }
+    steering = steering * w;
+    steering_acc = steering + steering_acc;
+}
+
 // limitMaxDeviationAngle, include/OpenSteer/VectorUtils.cuh:305-364.
 __device__ __forceinline__ f3 limit_max_deviation(f3 source, float cos_cone, f3 basis) {
     const float source_len = len3(source);

@@ -61,44 +61,10 @@ __global__ __launch_bounds__(kBlock) void k_avoid_obstacles(uint32_t n, Agents A
     if (i >= n) return;
     uint32_t bits = A.bits[i];
     if (bits & mask) return;
-    const f3 pos = ld3(A.px, A.py, A.pz, i), fwd = ld3(A.fx, A.fy, A.fz, i);
-    const f3 side = ld3(A.sx, A.sy, A.sz, i), up = ld3(A.ux, A.uy, A.uz, i);
-    const float radius = A.radius[i];
-    const float min_dist = min_time * A.speed[i];
-    bool n_hit = false;
-    float n_dist = 0.f;
-    f3 n_pos = mk3(0.f, 0.f, 0.f);
-    for (uint32_t j = 0; j < k; ++j) {
-        const uint32_t o = knn[(size_t)j * n + i];
-        if (o >= nO) break;
-        const f3 op = ld3(ox, oy, oz, o);
-        const float orr = orad[o];
-        bool x_hit = false;
-        float x_dist = 0.f;
-        const f3 go = op - pos;
-        const f3 lc = mk3(dot3(go, side), dot3(go, up), dot3(go, fwd));
-        const float b = -2.f * lc.z;
-        const float c = ((lc.x * lc.x + lc.y * lc.y) + lc.z * lc.z) - (orr + radius) * (orr + radius);
-        const float d = (b * b) - (4.f * c);
-        if (!(d < 0.f)) {
-            const float s = sqrtf(d);
-            const float p = (-b + s) / 2.f;
-            const float q = (-b - s) / 2.f;
-            if (!((p < 0.f) && (q < 0.f))) {
-                x_hit = true;
-                x_dist = ((p > 0.f) && (q > 0.f)) ? ((p < q) ? p : q) : ((p > 0.f) ? p : q);
-            }
-        }
-        if ((n_hit == false) || ((x_hit != false) && (x_dist < n_dist))) {
-            n_hit = x_hit;
-            n_dist = x_dist;
-            n_pos = op;
-        }
-    }
-    f3 f = mk3(0.f, 0.f, 0.f);
-    if (n_hit && (n_dist < min_dist)) f = perp3(pos - n_pos, fwd);
     f3 steering = ld3(A.gx, A.gy, A.gz, i);
-    apply(steering, bits, f, w, GS_BIT_AVOID_OBSTACLES);
+    avoid_obstacles_one(ld3(A.px, A.py, A.pz, i), ld3(A.fx, A.fy, A.fz, i), ld3(A.sx, A.sy, A.sz, i),
+                        ld3(A.ux, A.uy, A.uz, i), A.radius[i], A.speed[i], knn + i, n, k, nO, ox, oy, oz, orad,
+                        min_time, w, steering, bits);
     st3(A.gx, A.gy, A.gz, i, steering);
     A.bits[i] = bits;
 }
@@ -114,58 +80,10 @@ __global__ __launch_bounds__(kBlock) void k_avoid_walls(uint32_t n, Agents A, ui
     uint32_t bits = A.bits[i];
     if (bits & mask) return;
     f3 pos = ld3(A.px, A.py, A.pz, i), dir = ld3(A.fx, A.fy, A.fz, i);
-    const f3 side = ld3(A.sx, A.sy, A.sz, i);
-    const float radius = A.radius[i];
-    const float fl = 4.f * radius;  // FEELER_LENGTH (:41) * radius
-    const f3 vel = dir * A.speed[i];
-    f3 feelers[3];
-    feelers[0] = pos + vel * min_time;
-    feelers[1] = pos + (side * fl + dir * fl);
-    feelers[2] = pos + ((-side) * fl + dir * fl);
-    int feeler = 0;
-    float nearest_dist = FLT_MAX;
-    uint32_t nearest_line = UINT32_MAX;
-    f3 nearest_ip = mk3(0.f, 0.f, 0.f), nearest_normal = nearest_ip;
-    bool touching = false;
-    for (uint32_t j = 0; j < k; ++j) {
-        const uint32_t l = knn[(size_t)j * n + i];
-        if (l >= nW) break;
-        const f3 ln = ld3(W.nx, W.ny, W.nz, l);
-        const f3 ls = ld3(W.sx, W.sy, W.sz, l);
-        const f3 le = ld3(W.ex, W.ey, W.ez, l);
-        f3 ip;
-        if (lines_intersect(pos, pos + (-ln) * radius, ls, le, ip)) {
-            touching = true;
-            nearest_normal = ln;
-            nearest_dist = len3(pos - ip);
-        }
-#pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            if (lines_intersect(pos, feelers[f], ls, le, ip)) {
-                const float dist = len3(pos - ip);
-                if (dist < nearest_dist) {
-                    nearest_dist = dist;
-                    nearest_line = l;
-                    nearest_ip = ip;
-                    feeler = f;
-                    nearest_normal = ln;
-                }
-            }
-        }
-    }
-    f3 steering = mk3(0.f, 0.f, 0.f);
-    if (touching) {
-        const float pen = radius - nearest_dist;
-        pos = pos + nearest_normal * pen;
-        dir = normalize3(perp3(dir, nearest_normal));
-        bits |= GS_BIT_ANTI_PENETRATION_WALL;
-    } else if (nearest_line != UINT32_MAX) {
-        const f3 fe = feeler == 0 ? feelers[0] : (feeler == 1 ? feelers[1] : feelers[2]);
-        steering = nearest_normal * len3(fe - nearest_ip);
-        bits |= GS_BIT_AVOID_WALLS;
-    }
-    steering = steering * w;
-    st3(A.gx, A.gy, A.gz, i, steering + ld3(A.gx, A.gy, A.gz, i));
+    f3 steering = ld3(A.gx, A.gy, A.gz, i);
+    avoid_walls_one(pos, dir, ld3(A.sx, A.sy, A.sz, i), A.radius[i], A.speed[i], knn + i, n, k, nW, W, min_time, w,
+                    steering, bits);
+    st3(A.gx, A.gy, A.gz, i, steering);
     st3(A.px, A.py, A.pz, i, pos);
     st3(A.fx, A.fy, A.fz, i, dir);
     A.bits[i] = bits;

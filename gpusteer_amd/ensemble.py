@@ -1,0 +1,202 @@
+"""Batched ensemble (gs_ensemble_*): many independent Boids flocks and
+choke-point crowds stepped together, one launch per phase over all scenarios
+and steps replayed from a captured hipGraph.  Also the sequential runner that
+steps the same scenarios one at a time through the single-instance calls,
+which is what the ensemble must equal and what it is measured against."""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List
+
+import numpy as np
+
+from . import _capi as capi
+from . import api
+from ._capi import check, fptr
+from .state import AGENT_FIELDS, WALL_FIELDS, AgentState
+
+
+def _term(t) -> capi.GsFlockTerm:
+    return capi.GsFlockTerm(t.min_d, t.max_d, t.cos_max, t.weight, t.mask)
+
+
+def scenario_struct(spec) -> capi.GsScenario:
+    s = capi.GsScenario()
+    if spec[0] == "boids":
+        _, st, p = spec
+        s.kind = capi.GS_SCENARIO_BOIDS
+        s.n[0], s.n[1] = st.n, 0
+        s.origin, s.size = (C.c_float * 3)(*p.grid_origin), (C.c_float * 3)(*p.grid_size)
+        s.cells = (C.c_uint32 * 3)(*p.grid_cells)
+        s.dt, s.mode = p.dt, p.mode
+        s.wrap_shape = p.wrap_shape
+        s.wrap_center, s.wrap_size = (C.c_float * 3)(*p.wrap_center), (C.c_float * 3)(*p.wrap_size)
+        s.separation, s.alignment, s.cohesion = _term(p.separation), _term(p.alignment), _term(p.cohesion)
+    else:
+        _, sc = spec
+        s.kind = capi.GS_SCENARIO_CHOKE
+        s.n[0], s.n[1] = sc.groups[0].n, sc.groups[1].n
+        s.origin, s.size = (C.c_float * 3)(*sc.grid_origin), (C.c_float * 3)(*sc.grid_size)
+        s.cells = (C.c_uint32 * 3)(*sc.grid_cells)
+        s.dt, s.mode = sc.dt, sc.mode
+        s.wrap_shape = -1
+        s.choke_separation = _term(sc.sep)
+        for gi in range(2):
+            s.goal[gi] = (C.c_float * 3)(*sc.goals[gi])
+        s.seek_weight, s.seek_mask = sc.w_seek, sc.mask_seek
+        s.walls_min_time, s.walls_weight = sc.min_time_walls, sc.w_walls
+        s.obstacles_min_time, s.obstacles_weight = sc.min_time_obstacles, sc.w_obstacles
+        s.k_walls, s.k_obstacles = sc.knw, sc.kno
+    return s
+
+
+def _f32_state(st: AgentState) -> AgentState:
+    out = st.copy(np.float32)
+    for f in AGENT_FIELDS:
+        out.arrays[f] = np.ascontiguousarray(out.arrays[f], dtype=np.float32)
+    out.bits = np.ascontiguousarray(out.bits, dtype=np.uint32)
+    return out
+
+
+class Ensemble:
+    """All scenarios of `specs` in one ensemble (k neighbours per agent)."""
+
+    def __init__(self, ctx: api.Context, specs, k: int = 7):
+        self.specs = specs
+        self.k = k
+        arr = (capi.GsScenario * max(1, len(specs)))(*[scenario_struct(s) for s in specs])
+        h = C.c_void_p()
+        check(capi.lib.gs_ensemble_create(ctx.h, len(specs), arr, k, C.byref(h)), "gs_ensemble_create")
+        self.h = h
+        for i, spec in enumerate(specs):
+            groups = [spec[1]] if spec[0] == "boids" else list(spec[1].groups)
+            for gi, st in enumerate(groups):
+                self.upload(i, gi, st)
+            if spec[0] == "choke":
+                sc = spec[1]
+                w = {f: np.ascontiguousarray(sc.walls.arrays[f], dtype=np.float32) for f in WALL_FIELDS}
+                ws = capi.GsWalls(*[fptr(w[f]) for f in WALL_FIELDS])
+                check(capi.lib.gs_ensemble_upload_walls(self.h, i, sc.walls.n, C.byref(ws)),
+                      "gs_ensemble_upload_walls")
+                o = [np.ascontiguousarray(a, dtype=np.float32) for a in sc.obstacles]
+                check(capi.lib.gs_ensemble_upload_obstacles(self.h, i, o[0].shape[0], *[fptr(a) for a in o]),
+                      "gs_ensemble_upload_obstacles")
+
+    @property
+    def slots(self) -> int:
+        v = C.c_uint32()
+        check(capi.lib.gs_ensemble_slots(self.h, C.byref(v)), "gs_ensemble_slots")
+        return v.value
+
+    def upload(self, scenario: int, group: int, st: AgentState) -> None:
+        s = _f32_state(st)
+        check(capi.lib.gs_ensemble_upload_agents(self.h, scenario, group, C.byref(api._agents_struct(s))),
+              "gs_ensemble_upload_agents")
+
+    def download(self, scenario: int, group: int = 0) -> AgentState:
+        spec = self.specs[scenario]
+        n = spec[1].n if spec[0] == "boids" else spec[1].groups[group].n
+        out = AgentState.empty(n)
+        check(capi.lib.gs_ensemble_download_agents(self.h, scenario, group, C.byref(api._agents_struct(out))),
+              "gs_ensemble_download_agents")
+        return out
+
+    def download_knn(self, scenario: int, group: int = 0) -> np.ndarray:
+        spec = self.specs[scenario]
+        n = spec[1].n if spec[0] == "boids" else spec[1].groups[group].n
+        idx = np.zeros((self.k, n), np.uint32)
+        check(capi.lib.gs_ensemble_download_knn(self.h, scenario, group, capi.uptr(idx)), "gs_ensemble_download_knn")
+        return idx
+
+    def step(self, steps: int = 1, eager: bool = False) -> None:
+        check(capi.lib.gs_ensemble_step(self.h, steps, capi.GS_ENSEMBLE_EAGER if eager else capi.GS_ENSEMBLE_GRAPH),
+              "gs_ensemble_step")
+
+    def agents(self) -> int:
+        return sum(s[1].n if s[0] == "boids" else s[1].groups[0].n + s[1].groups[1].n for s in self.specs)
+
+    def close(self) -> None:
+        if self.h:
+            check(capi.lib.gs_ensemble_destroy(self.h), "gs_ensemble_destroy")
+            self.h = None
+
+
+def choke_step_calls(G, knns, W, O, bins, sc) -> None:
+    """One choke-point step through the single-instance calls, in the order of
+    plugins/ChokePointGroup.cpp:683-728: bin both groups; per group clear
+    bits, KNN to itself / the other group / the walls / the obstacles, avoid
+    walls, avoid obstacles, separation from both groups, seek; integrate both."""
+    for grp in G:
+        api.updateKNNDatabase(grp, bins)
+    for gi, grp in enumerate(G):
+        other = G[1 - gi]
+        ks, ko, kw, kob = knns[gi]
+        api.clearAppliedKernels(grp)
+        api.findKNearestNeighbors(grp, ks, bins, grp, 1)
+        api.findKNearestNeighbors(grp, ko, bins, other, 1)
+        api.findKNearestNeighbors(grp, kw, bins, W, 1)
+        api.findKNearestNeighbors(grp, kob, bins, O, 1)
+        api.steerToAvoidWalls(grp, kw, W, sc.min_time_walls, sc.w_walls, 0)
+        api.steerToAvoidObstacles(grp, O, kob, sc.min_time_obstacles, sc.w_obstacles, 0)
+        t = sc.sep
+        api.steerForSeparation(grp, ks, grp, t.min_d, t.max_d, t.cos_max, t.weight, t.mask, sc.mode)
+        api.steerForSeparation(grp, ko, other, t.min_d, t.max_d, t.cos_max, t.weight, t.mask, sc.mode)
+        api.steerForSeek(grp, sc.goals[gi], sc.w_seek, sc.mask_seek, sc.mode)
+    for grp in G:
+        api.updateGroup(grp, sc.dt, sc.mode)
+
+
+class Sequential:
+    """The same scenarios, each stepped alone through the single-instance
+    calls (gs_step_boids for a flock, the choke call sequence for a crowd),
+    one scenario after another."""
+
+    def __init__(self, ctx: api.Context, specs, k: int = 7):
+        self.specs = specs
+        self.items = []
+        for spec in specs:
+            if spec[0] == "boids":
+                _, st, p = spec
+                A = api.AgentGroup(ctx, st)
+                bins = api.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
+                knn = api.KNNData(ctx, st.n, k)
+                self.items.append(("boids", A, bins, knn, p))
+            else:
+                sc = spec[1]
+                bins = api.KNNBinData(ctx, sc.grid_cells, sc.grid_size, origin=sc.grid_origin)
+                G = [api.AgentGroup(ctx, g) for g in sc.groups]
+                W = api.WallGroup(ctx, sc.walls)
+                O = api.ObstacleGroup(ctx, *sc.obstacles)
+                api.updateKNNDatabase(W, bins)
+                api.updateKNNDatabase(O, bins)
+                knns = [[api.KNNData(ctx, g.n, kk) for kk in (k, k, sc.knw, sc.kno)] for g in sc.groups]
+                self.items.append(("choke", G, knns, W, O, bins, sc))
+
+    def step(self, steps: int = 1) -> None:
+        for it in self.items:
+            if it[0] == "boids":
+                _, A, bins, knn, p = it
+                api.stepBoids(A, knn, bins, p, steps)
+            else:
+                _, G, knns, W, O, bins, sc = it
+                for _ in range(steps):
+                    choke_step_calls(G, knns, W, O, bins, sc)
+
+    def download(self, scenario: int, group: int = 0) -> AgentState:
+        it = self.items[scenario]
+        return it[1].download() if it[0] == "boids" else it[1][group].download()
+
+    def download_knn(self, scenario: int, group: int = 0) -> np.ndarray:
+        it = self.items[scenario]
+        return (it[3] if it[0] == "boids" else it[2][group][0]).download()[0]
+
+    def close(self) -> None:
+        for it in self.items:
+            if it[0] == "boids":
+                for h in it[1:4]:
+                    h.close()
+            else:
+                _, G, knns, W, O, bins, sc = it
+                for h in G + [W, O, bins] + [x for ks in knns for x in ks]:
+                    h.close()
+        self.items = []

@@ -220,3 +220,57 @@ def choke(n_total: int = 10000, gap: float = 15.0, n_obstacles: int = 50, worklo
                          obstacles=obstacles, grid_origin=(-extent / 2, -5.0, -extent / 2),
                          grid_size=(extent, 10.0, extent), grid_cells=(int(extent / 5), 1, int(extent / 5)),
                          sep=FlockTerm(-1.0, 3.0, -0.707, 10.0, 4096))
+
+
+# ---------------------------------------------------------------------------
+# Ensembles of scenarios.
+
+
+def boids_cells_for(n: int, radius: float = 50.0, cell_factor: float = 1.0, k: int = 7) -> int:
+    """Cells per axis of a Boids lattice over the ball's 1.1 R margin cube for
+    n agents: cell edge = cell_factor x the expected k-th neighbour distance
+    (k / (n / V) = 4/3 pi r^3)."""
+    r_k = (k * (4.0 / 3.0) * radius ** 3 / n) ** (1.0 / 3.0)
+    return int(np.clip(np.floor(2.2 * radius / (cell_factor * r_k)), 1, 128))
+
+
+def choke_cells_for(n_total: int, region: float = 40.0, extent: float = 200.0, cell_factor: float = 1.0,
+                    k: int = 7) -> int:
+    """Cells per axis (x and z) of a crowd lattice: cell edge = cell_factor x
+    the expected k-th neighbour distance at the spawn density."""
+    rho = (n_total / 2) / (region * region)
+    r_k = np.sqrt(k / (np.pi * rho))
+    return int(np.clip(np.floor(extent / (cell_factor * r_k)), 1, 2048))
+
+
+def ensemble_mix(n_scenarios: int = 256, n_min: int = 1024, n_max: int = 65536, workload: int = 3,
+                 cell_factor: float = 1.0, mode: int = 0):
+    """Mixed ensemble: agent counts log-uniform over [n_min, n_max]; even
+    scenarios are Boids flocks (config 0 layout) with the three weights
+    jittered by U(0.7, 1.3); odd ones are choke-point crowds (config 1 layout)
+    with gap widths cycling through 7.5, 10, ..., 32.5.  Lattices are sized to
+    each scenario's density.  Returns a list of ("boids", state, params) and
+    ("choke", ChokeScenario) specs."""
+    g = rng(workload)
+    gaps = [7.5 + 2.5 * i for i in range(13)]
+    specs = []
+    for i in range(n_scenarios):
+        n = int(round(2.0 ** g.uniform(np.log2(n_min), np.log2(n_max))))
+        if i % 2 == 0:
+            st, p = boids_sphere(n=n, workload=100 + i)
+            jit = g.uniform(0.7, 1.3, size=3)
+            p.separation.weight *= float(jit[0])
+            p.alignment.weight *= float(jit[1])
+            p.cohesion.weight *= float(jit[2])
+            c = boids_cells_for(n, cell_factor=cell_factor)
+            p.grid_cells = (c, c, c)
+            p.mode = mode
+            specs.append(("boids", st, p))
+        else:
+            n = max(2, n - (n % 2))
+            sc = choke(n_total=n, gap=gaps[(i // 2) % len(gaps)], workload=100 + i)
+            c = choke_cells_for(n, cell_factor=cell_factor)
+            sc.grid_cells = (c, 1, c)
+            sc.mode = mode
+            specs.append(("choke", sc))
+    return specs

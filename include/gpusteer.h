@@ -253,6 +253,61 @@ typedef struct {
 int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* params, uint32_t steps,
                         gs_boids_timing* timing);
 
+/* ---- batched ensemble -------------------------------------------------------------
+ * Many independent scenarios of uneven size stepped together: every phase of a
+ * step is one launch over all scenarios and the steps are replayed from a
+ * captured hipGraph.  Each scenario's results are identical, bit for bit, to
+ * running it alone through the calls above:
+ *   GS_SCENARIO_BOIDS  the gs_step_boids sequence (wrap, bins, KNN, flocking,
+ *                      integrate) on one group;
+ *   GS_SCENARIO_CHOKE  two groups; per group in order 0, 1: gs_clear_bits,
+ *                      KNN to itself / to the other group / to the walls /
+ *                      to the obstacles, gs_steer_avoid_walls,
+ *                      gs_steer_avoid_obstacles, gs_steer_separation (own
+ *                      group, other group), gs_steer_seek (the group's goal);
+ *                      then gs_update_group on both
+ *                      (plugins/ChokePointGroup.cpp:683-728).
+ * No counterpart in the reference, which runs one scenario per process. */
+typedef struct gs_ensemble_s* gs_ensemble;
+enum { GS_SCENARIO_BOIDS = 0, GS_SCENARIO_CHOKE = 1 };
+typedef struct {
+    int kind;                          /* GS_SCENARIO_BOIDS / GS_SCENARIO_CHOKE */
+    uint32_t n[2];                     /* agents of group 0 (and group 1 for a crowd) */
+    float origin[3], size[3];          /* this scenario's bin lattice */
+    uint32_t cells[3];
+    float dt;
+    int mode;                          /* GS_MODE_* */
+    /* Boids */
+    int wrap_shape;                    /* GS_WRAP_BOX / GS_WRAP_SPHERE / -1 */
+    float wrap_center[3], wrap_size[3];
+    gs_flock_term separation, alignment, cohesion;
+    /* choke-point crowd */
+    gs_flock_term choke_separation;
+    float goal[2][3];
+    float seek_weight;
+    uint32_t seek_mask;
+    float walls_min_time, walls_weight, obstacles_min_time, obstacles_weight;
+    uint32_t k_walls, k_obstacles;     /* 1..8 */
+} gs_scenario;
+/* k: neighbours per agent (1..8), the same for every scenario. */
+int gs_ensemble_create(gs_context ctx, uint32_t n_scenarios, const gs_scenario* scenarios, uint32_t k,
+                       gs_ensemble* out);
+int gs_ensemble_destroy(gs_ensemble e);
+/* Total slots (agents plus per-group padding to 256) of the ensemble. */
+int gs_ensemble_slots(gs_ensemble e, uint32_t* slots);
+int gs_ensemble_upload_agents(gs_ensemble e, uint32_t scenario, uint32_t group, const gs_agents* host);
+int gs_ensemble_download_agents(gs_ensemble e, uint32_t scenario, uint32_t group, gs_agents* host);
+int gs_ensemble_upload_walls(gs_ensemble e, uint32_t scenario, uint32_t n, const gs_walls* host);
+int gs_ensemble_upload_obstacles(gs_ensemble e, uint32_t scenario, uint32_t n, const float* cx, const float* cy,
+                                 const float* cz, const float* radius);
+/* The last KNN of a group with itself, [k][n] agent indices (pads UINT32_MAX). */
+int gs_ensemble_download_knn(gs_ensemble e, uint32_t scenario, uint32_t group, uint32_t* idx);
+enum { GS_ENSEMBLE_GRAPH = 0, GS_ENSEMBLE_EAGER = 1 };
+/* Advance every scenario `steps` steps.  GS_ENSEMBLE_GRAPH replays captured
+ * pairs of steps (the first step after an upload runs eagerly); the results
+ * do not depend on the flag. */
+int gs_ensemble_step(gs_ensemble e, uint32_t steps, uint32_t flags);
+
 #ifdef __cplusplus
 }
 #endif

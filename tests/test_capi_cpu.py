@@ -73,3 +73,24 @@ def test_package_import_requires_built_library(tmp_path):
     assert "raise ImportError" in src
     spec = importlib.util.find_spec("gpusteer_amd")
     assert spec is not None
+
+
+def test_ensemble_specs_and_struct():
+    """Scenario generator and the gs_scenario struct (no GPU needed)."""
+    from gpusteer_amd import _capi as capi
+    from gpusteer_amd import scenarios
+    from gpusteer_amd.ensemble import scenario_struct
+
+    specs = scenarios.ensemble_mix(n_scenarios=8, n_min=1024, n_max=65536, workload=3)
+    assert [s[0] for s in specs] == ["boids", "choke"] * 4
+    for s in specs:
+        st = scenario_struct(s)
+        if s[0] == "boids":
+            assert st.kind == capi.GS_SCENARIO_BOIDS and st.n[0] == s[1].n and st.wrap_shape == capi.GS_WRAP_SPHERE
+            assert 1024 <= s[1].n <= 65536
+        else:
+            assert st.kind == capi.GS_SCENARIO_CHOKE and st.n[0] + st.n[1] == 2 * s[1].groups[0].n
+            assert st.cells[1] == 1 and st.k_walls == 3 and st.k_obstacles == 2
+    # Lattices follow density: more agents, more cells.
+    assert scenarios.boids_cells_for(65536) > scenarios.boids_cells_for(1024)
+    assert scenarios.choke_cells_for(65536) > scenarios.choke_cells_for(1024)
