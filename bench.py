@@ -129,6 +129,70 @@ def achievable_copy_gbs() -> float:
     return device_copy_gbs(1024, 10)
 
 
+def ensemble_bench(n_scenarios: int, steps: int, warmup: int) -> dict:
+    """Batched-ensemble speedup: total agent-steps/s of the mixed ensemble
+    (Boids flocks and choke-point crowds, agent counts log-uniform over
+    1,024-65,536) stepped together -- one launch per phase over all
+    scenarios, step pairs replayed from a hipGraph -- over the same scenarios
+    stepped one after another through the single-instance calls.  Both runs
+    do the same warmup and timed steps; two scenarios are then compared bit
+    for bit."""
+    import numpy as np
+
+    import gpusteer_amd as gs
+    from gpusteer_amd import scenarios
+
+    t0 = time.perf_counter()
+    specs = scenarios.ensemble_mix(n_scenarios=n_scenarios)
+    print(f"bench: ensemble of {n_scenarios} scenarios generated in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    with gs.Context(0) as ctx:
+        ens = gs.Ensemble(ctx, specs)
+        agents = ens.agents()
+        ens.step(warmup)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ens.step(steps)
+        ctx.synchronize()
+        t_batched = time.perf_counter() - t0
+        print(f"bench: ensemble batched {t_batched * 1e3 / steps:.2f} ms/step ({agents} agents)", file=sys.stderr,
+              flush=True)
+        seq = gs.Sequential(ctx, specs)
+        seq.step(warmup)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        seq.step(steps)
+        ctx.synchronize()
+        t_seq = time.perf_counter() - t0
+        print(f"bench: ensemble sequential {t_seq * 1e3 / steps:.2f} ms/step", file=sys.stderr, flush=True)
+        same = True
+        for i in (0, 1):
+            a, b = ens.download(i, 0), seq.download(i, 0)
+            same = same and all(np.array_equal(a.arrays[f], b.arrays[f]) for f in ("px", "py", "pz", "fx", "fy",
+                                                                                      "fz", "speed"))
+        ens.close()
+        seq.close()
+    batched = agents * steps / t_batched
+    sequential = agents * steps / t_seq
+    return {
+        "metric": "batched ensemble speedup",
+        "value": batched / sequential,
+        "unit": "x (batched / sequential agent-steps/s)",
+        "batched_agent_steps_per_s": batched,
+        "sequential_agent_steps_per_s": sequential,
+        "ms_per_step_batched": t_batched * 1e3 / steps,
+        "ms_per_step_sequential": t_seq * 1e3 / steps,
+        "scenarios": n_scenarios,
+        "agents": agents,
+        "steps": steps,
+        "warmup": warmup,
+        "bit_identical_spot_check": bool(same),
+        "workload": "half Boids flocks (ball R=50, config 0 forces, weights x U(0.7,1.3)), half choke-point "
+                    "crowds (config 1 layout, gaps 7.5..32.5); agents log-uniform 1,024-65,536 per scenario; "
+                    "k=7; lattices sized to density",
+    }
+
+
 def cpu_baseline(st, p, seconds: float):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as orc
@@ -164,6 +228,9 @@ def main() -> int:
     ap.add_argument("--cells", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble-scenarios", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 512 MiB device copy (the counter calibration for the traffic figure)")
     args = ap.parse_args()
@@ -256,6 +323,8 @@ def main() -> int:
             "frac": n * bytes_per_agent_step(args.k, C, n) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
     }
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(args.ensemble_scenarios, args.ensemble_steps, 3)
     if not args.no_cpu:
         print("bench: cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(st, p, args.cpu_seconds)

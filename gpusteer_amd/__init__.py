@@ -17,6 +17,7 @@ from .api import (  # noqa: F401
     clearAppliedKernels,
     findKNearestNeighbors,
     findKNearestNeighborsBruteForce,
+    findKNearestNeighborsWithin,
     steerForAlignment,
     steerForCohesion,
     steerForFlocking,

@@ -120,6 +120,7 @@ class GsScenario(C.Structure):
         ("obstacles_weight", C.c_float),
         ("k_walls", C.c_uint32),
         ("k_obstacles", C.c_uint32),
+        ("other_max_distance", C.c_float),
     ]
 
 
@@ -173,6 +174,7 @@ _SIGS = {
     "gs_knn_device": (C.c_int, [_h, C.POINTER(_up), C.POINTER(_fp)]),
     "gs_update_knn_database": (C.c_int, [_h, _h]),
     "gs_find_knn": (C.c_int, [_h, _h, _h, _h, C.c_uint32, C.c_uint32]),
+    "gs_find_knn_within": (C.c_int, [_h, _h, _h, _h, C.c_uint32, C.c_uint32, C.c_float]),
     "gs_find_knn_brute": (C.c_int, [_h, _h, _h]),
     "gs_steer_separation": (C.c_int, [_h, _h, _h, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_int]),
     "gs_steer_alignment": (C.c_int, [_h, _h, _h, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_int]),

@@ -227,6 +227,13 @@ def findKNearestNeighbors(A: _Group, knn: KNNData, bins: KNNBinData, B: _Group, 
     check(capi.lib.gs_find_knn(A.h, knn.h, bins.h, B.h, searchRadius, flags), "gs_find_knn")
 
 
+def findKNearestNeighborsWithin(A: _Group, knn: KNNData, bins: KNNBinData, B: _Group, maxDistance: float,
+                                searchRadius: int = 1, flags: int = capi.GS_KNN_DEFAULT) -> None:
+    """findKNearestNeighbors keeping only points closer than maxDistance."""
+    check(capi.lib.gs_find_knn_within(A.h, knn.h, bins.h, B.h, searchRadius, flags, maxDistance),
+          "gs_find_knn_within")
+
+
 def findKNearestNeighborsBruteForce(A: _Group, knn: KNNData, B: _Group) -> None:
     check(capi.lib.gs_find_knn_brute(A.h, knn.h, B.h), "gs_find_knn_brute")
 

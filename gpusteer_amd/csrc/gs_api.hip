@@ -689,24 +689,47 @@ int gs_update_knn_database(gs_group g, gs_bins bins) {
     });
 }
 
-int gs_find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t search_radius, uint32_t flags) {
-    return guarded([&] {
-        check_group(A, "group A");
-        check_group(B, "group B");
-        check(out, "knn");
-        check(bins, "bins");
-        if (A->ctx != B->ctx || bins->ctx != A->ctx) fail(GS_ERR_INVALID_ARGUMENT, "handles of different contexts");
-        require_knn_shape(out, A);
-        if (!A->binned || !B->binned || !A->snap_valid || !B->snap_valid)
-            fail(GS_ERR_NOT_BINNED, "call gs_update_knn_database on both groups first");
-        if (!same_grid(B->bin_grid, bins->g) || !same_grid(A->bin_grid, bins->g))
-            fail(GS_ERR_NOT_BINNED, "groups were binned with another lattice");
-        use_device(A->ctx);
+// Strict radius cap r (lq.c:327: distanceSquared < radiusSquared, both fp32)
+// as an inclusive bound on d2: the largest float below r*r.
+static float cap2_of(float max_distance) {
+    if (std::isnan(max_distance) || max_distance <= 0.f)
+        fail(GS_ERR_INVALID_ARGUMENT, "max_distance must be positive (INFINITY for no cap)");
+    if (std::isinf(max_distance)) return INFINITY;
+    const float r2 = max_distance * max_distance;
+    return std::nextafter(r2, 0.f);
+}
+
+static void find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t search_radius, uint32_t flags,
+                     float max_distance) {
+    check_group(A, "group A");
+    check_group(B, "group B");
+    check(out, "knn");
+    check(bins, "bins");
+    if (A->ctx != B->ctx || bins->ctx != A->ctx) fail(GS_ERR_INVALID_ARGUMENT, "handles of different contexts");
+    require_knn_shape(out, A);
+    if (!A->binned || !B->binned || !A->snap_valid || !B->snap_valid)
+        fail(GS_ERR_NOT_BINNED, "call gs_update_knn_database on both groups first");
+    if (!same_grid(B->bin_grid, bins->g) || !same_grid(A->bin_grid, bins->g))
+        fail(GS_ERR_NOT_BINNED, "groups were binned with another lattice");
+    const float cap2 = cap2_of(max_distance);
+    use_device(A->ctx);
+    if (B->n <= kSmallTargetSet) {
+        launch_knn_snap_brute(A, out, B, cap2);
+    } else {
         const Seed sd = seed_for(out, A, B, flags);
-        launch_knn_binned(A, out, B, search_radius, sd.seed, sd.row_map, sd.val_map);
-        out->cur ^= 1;
-        mark_result(out, A, B);
-    });
+        launch_knn_binned(A, out, B, search_radius, cap2, sd.seed, sd.row_map, sd.val_map);
+    }
+    out->cur ^= 1;
+    mark_result(out, A, B);
+}
+
+int gs_find_knn_within(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t search_radius, uint32_t flags,
+                       float max_distance) {
+    return guarded([&] { find_knn(A, out, bins, B, search_radius, flags, max_distance); });
+}
+
+int gs_find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t search_radius, uint32_t flags) {
+    return guarded([&] { find_knn(A, out, bins, B, search_radius, flags, INFINITY); });
 }
 
 int gs_find_knn_brute(gs_group A, gs_knn out, gs_group B) {
@@ -717,7 +740,7 @@ int gs_find_knn_brute(gs_group A, gs_knn out, gs_group B) {
         if (A->ctx != B->ctx) fail(GS_ERR_INVALID_ARGUMENT, "handles of different contexts");
         require_knn_shape(out, A);
         use_device(A->ctx);
-        launch_knn_brute(A, out, B);
+        launch_knn_brute(A, out, B, INFINITY);
         out->cur ^= 1;
         mark_result(out, A, B);
     });
@@ -841,7 +864,7 @@ static void boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const
     after_reorder(A);
     if (ev) GS_HIP(hipEventRecord(ev[1], st));
     const Seed sd = seed_for(knn, A, A, p->knn_flags);
-    launch_knn_binned(A, knn, A, p->search_radius, sd.seed, sd.row_map, sd.val_map);
+    launch_knn_binned(A, knn, A, p->search_radius, INFINITY, sd.seed, sd.row_map, sd.val_map);
     if (A->n) knn->cur ^= 1;
     mark_result(knn, A, A);
     if (ev) GS_HIP(hipEventRecord(ev[2], st));

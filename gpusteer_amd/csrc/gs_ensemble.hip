@@ -27,6 +27,7 @@
 // fields 0..5 flip after the step, so every buffer returns to its place every
 // two steps: a captured pair of steps replays indefinitely.
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "gs_ensemble.hpp"
@@ -139,9 +140,10 @@ __global__ __launch_bounds__(kBlock) void e_knn(EnsDev E, uint32_t N, uint32_t k
     {
         TopK<8> h;
         h.init(k);
-        const float thr = out.seed_other ? seed_bound<8>(out.seed_other, k, N, new_to_old[s], N, old_to_new, q.x,
-                                                         q.y, q.z, spos)
-                                         : INFINITY;
+        const float thr = fminf(E.scens[P.scen].other_cap2,
+                                out.seed_other ? seed_bound<8>(out.seed_other, k, N, new_to_old[s], N, old_to_new,
+                                                               q.x, q.y, q.z, spos)
+                                               : INFINITY);
         if (O.n > 0) knn_search_flat<8>(h, thr, q.x, q.y, q.z, UINT32_MAX, O.g, 1, N, spos, start + O.c0, flds);
         store_idx<8>(h, k, N, s, out.other);
     }
@@ -349,6 +351,9 @@ void ens_init(gs_ensemble_s* e, uint32_t ns, const gs_scenario* sc, uint32_t k,
         S.w_obs = d.obstacles_weight;
         S.k_walls = d.k_walls;
         S.k_obs = d.k_obstacles;
+        S.other_cap2 = d.other_max_distance > 0.f && std::isfinite(d.other_max_distance)
+                           ? std::nextafter(d.other_max_distance * d.other_max_distance, 0.f)
+                           : INFINITY;
         const GridDev g = make_grid(d.origin, d.size, d.cells);
         const uint32_t cells = d.cells[0] * d.cells[1] * d.cells[2];
         const int ng = d.kind == GS_SCENARIO_CHOKE ? 2 : 1;

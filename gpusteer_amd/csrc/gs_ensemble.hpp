@@ -26,6 +26,7 @@ struct EnsScen {
     uint32_t mask_seek;
     float mt_walls, w_walls, mt_obs, w_obs;
     uint32_t k_walls, k_obs;
+    float other_cap2;           // inclusive d2 bound of the other-group KNN
     uint32_t w0, nw, o0, no;    // ranges in the wall / obstacle arrays
 };
 

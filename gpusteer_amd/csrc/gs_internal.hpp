@@ -261,9 +261,14 @@ void launch_iota(uint32_t* p, uint32_t n, hipStream_t st);
 void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws);
 // KNN into out's alternate buffer (caller flips).  row_map / val_map translate
 // a seed held at the previous storage order of A / B (NULL: same order).
-void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, const uint32_t* seed,
-                       const uint32_t* row_map, const uint32_t* val_map);
-void launch_knn_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B);
+// cap2: only points with d2 <= cap2 qualify (+inf: no cap).
+void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, float cap2,
+                       const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map);
+void launch_knn_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B, float cap2);
+// The binned search's semantics (position snapshots of both builds) by an
+// exhaustive scan, for small B.
+void launch_knn_snap_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B, float cap2);
+constexpr uint32_t kSmallTargetSet = 256;
 // Rewrites out's contents from the previous storage order of A / B to the
 // current one (into the alternate buffer; caller flips).
 void launch_knn_remap(gs_knn_s* out, uint32_t nA, const uint32_t* row_old_to_new, const uint32_t* val_old_to_new);

@@ -35,7 +35,7 @@ template <int KMAX, int RF>
 __global__ __launch_bounds__(kBlock) void k_knn_binned(uint32_t nA, const float4* __restrict__ apos, uint32_t nB,
                                                        const float4* __restrict__ bpos,
                                                        const uint32_t* __restrict__ bstart, GridDev g, int self,
-                                                       uint32_t k, int R0, const uint32_t* __restrict__ seed,
+                                                       uint32_t k, int R0, float cap2, const uint32_t* __restrict__ seed,
                                                        const uint32_t* __restrict__ row_map,
                                                        const uint32_t* __restrict__ val_map,
                                                        uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
@@ -45,9 +45,9 @@ __global__ __launch_bounds__(kBlock) void k_knn_binned(uint32_t nA, const float4
     TopK<KMAX> h;
     h.init(k);
     if (nB > 0) {
-        const float thr = seed ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB, val_map, q.x, q.y,
-                                                  q.z, bpos)
-                               : INFINITY;
+        const float thr = fminf(cap2, seed ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB, val_map,
+                                                              q.x, q.y, q.z, bpos)
+                                           : INFINITY);
         if constexpr (RF == 1) {
             GS_FLAT_LDS
             knn_search_flat<KMAX>(h, thr, q.x, q.y, q.z, self ? s : UINT32_MAX, g, R0, nB, bpos, bstart, flds);
@@ -67,7 +67,8 @@ __global__ __launch_bounds__(kBlock) void k_knn_brute(uint32_t nA, const float* 
                                                       uint32_t nB, const float* __restrict__ bx,
                                                       const float* __restrict__ by, const float* __restrict__ bz,
                                                       const uint32_t* __restrict__ bids, int self, uint32_t k,
-                                                      uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+                                                      float cap2, uint32_t* __restrict__ out_idx,
+                                                      float* __restrict__ out_dist) {
     __shared__ float tx[kBlock], ty[kBlock], tz[kBlock];
     __shared__ uint32_t tid[kBlock];
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -89,13 +90,65 @@ __global__ __launch_bounds__(kBlock) void k_knn_brute(uint32_t nA, const float* 
             for (uint32_t j = 0; j < m; ++j) {
                 const uint32_t b = base + j;
                 if (self && b == q) continue;
-                const uint64_t kk = make_key(sqdist(qx, qy, qz, tx[j], ty[j], tz[j]), tid[j]);
-                if (h.accepts(kk)) h.push(kk, b);
+                const float d2 = sqdist(qx, qy, qz, tx[j], ty[j], tz[j]);
+                const uint64_t kk = make_key(d2, tid[j]);
+                if (d2 <= cap2 && h.accepts(kk)) h.push(kk, b);
             }
         }
         __syncthreads();
     }
     if (active) h.store(k, nA, q, out_idx, out_dist);
+}
+
+// Binned-database semantics over a small B (a few walls or obstacles): the
+// same queries and candidates as k_knn_binned (the position snapshots of the
+// last builds, agent indices in .w) scanned exhaustively from LDS, so a small
+// target set costs nB distance tests instead of a shell expansion over empty
+// cells.  Identical results (the search is exact either way).
+template <int KMAX>
+__global__ __launch_bounds__(kBlock) void k_knn_snap_brute(uint32_t nA, const float4* __restrict__ apos, uint32_t nB,
+                                                           const float4* __restrict__ bpos, int self, uint32_t k,
+                                                           float cap2, uint32_t* __restrict__ out_idx,
+                                                           float* __restrict__ out_dist) {
+    __shared__ float4 tile[kBlock];
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = s < nA;
+    const float4 q = active ? apos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    TopK<KMAX> h;
+    h.init(k);
+    for (uint32_t base = 0; base < nB; base += kBlock) {
+        const uint32_t t = base + threadIdx.x;
+        if (t < nB) tile[threadIdx.x] = bpos[t];
+        __syncthreads();
+        const uint32_t m = min((uint32_t)kBlock, nB - base);
+        if (active) {
+            for (uint32_t j = 0; j < m; ++j) {
+                const uint32_t b = base + j;
+                if (self && b == s) continue;
+                const float4 p = tile[j];
+                const float d2 = sqdist4(q.x, q.y, q.z, p);
+                const uint64_t kk = make_key(d2, id_of(p));
+                if (d2 <= cap2 && h.accepts(kk)) h.push(kk, b);
+            }
+        }
+        __syncthreads();
+    }
+    if (active) h.store(k, nA, s, out_idx, out_dist);
+}
+
+void launch_knn_snap_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B, float cap2) {
+    const uint32_t nA = A->n;
+    if (nA == 0) return;
+    hipStream_t st = A->ctx->stream;
+    const int self = A == B ? 1 : 0;
+#define GS_SB(KM)                                                                                             \
+    k_knn_snap_brute<KM><<<blocks_for(nA), kBlock, 0, st>>>(nA, A->spos.p, B->n, B->spos.p, self, out->k, cap2, \
+                                                            out->idx_alt(), out->dist_alt())
+    if (out->k <= 8) GS_SB(8);
+    else if (out->k <= 16) GS_SB(16);
+    else GS_SB(32);
+#undef GS_SB
+    GS_LAUNCHED("k_knn_snap_brute");
 }
 
 // Moves a [k][nA] result from the previous storage order of A and/or B to the
@@ -117,41 +170,41 @@ __global__ __launch_bounds__(kBlock) void k_knn_remap(uint32_t nA, uint32_t k, c
 }
 
 template <int KMAX, int RF>
-static void run_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, const uint32_t* seed,
+static void run_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, float cap2, const uint32_t* seed,
                        const uint32_t* row_map, const uint32_t* val_map) {
     const uint32_t nA = A->n;
     k_knn_binned<KMAX, RF><<<blocks_for(nA), kBlock, 0, A->ctx->stream>>>(
-        nA, A->spos.p, B->n, B->spos.p, B->start.p, B->bin_grid, A == B ? 1 : 0, out->k, (int)R0, seed, row_map,
-        val_map, out->idx_alt(), out->dist_alt());
+        nA, A->spos.p, B->n, B->spos.p, B->start.p, B->bin_grid, A == B ? 1 : 0, out->k, (int)R0, cap2, seed,
+        row_map, val_map, out->idx_alt(), out->dist_alt());
     GS_LAUNCHED("k_knn_binned");
 }
 
 template <int KMAX>
-static void run_binned_rf(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, const uint32_t* seed,
-                          const uint32_t* row_map, const uint32_t* val_map) {
+static void run_binned_rf(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, float cap2,
+                          const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map) {
     // search_radius selects the fast path's column radius: 3 x 3 columns for
     // lattices whose cell is about the k-th neighbour distance, 5 x 5 for
     // finer lattices.
-    if (R0 >= 2) run_binned<KMAX, 2>(A, out, B, R0, seed, row_map, val_map);
-    else run_binned<KMAX, 1>(A, out, B, R0, seed, row_map, val_map);
+    if (R0 >= 2) run_binned<KMAX, 2>(A, out, B, R0, cap2, seed, row_map, val_map);
+    else run_binned<KMAX, 1>(A, out, B, R0, cap2, seed, row_map, val_map);
 }
 
-void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, const uint32_t* seed,
-                       const uint32_t* row_map, const uint32_t* val_map) {
+void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, float cap2,
+                       const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map) {
     if (A->n == 0) return;
-    if (out->k <= 8) run_binned_rf<8>(A, out, B, search_radius, seed, row_map, val_map);
-    else if (out->k <= 16) run_binned_rf<16>(A, out, B, search_radius, seed, row_map, val_map);
-    else run_binned_rf<32>(A, out, B, search_radius, seed, row_map, val_map);
+    if (out->k <= 8) run_binned_rf<8>(A, out, B, search_radius, cap2, seed, row_map, val_map);
+    else if (out->k <= 16) run_binned_rf<16>(A, out, B, search_radius, cap2, seed, row_map, val_map);
+    else run_binned_rf<32>(A, out, B, search_radius, cap2, seed, row_map, val_map);
 }
 
-void launch_knn_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B) {
+void launch_knn_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B, float cap2) {
     const uint32_t nA = A->n;
     if (nA == 0) return;
     hipStream_t st = A->ctx->stream;
     const int self = A == B ? 1 : 0;
 #define GS_BRUTE(KM)                                                                                              \
     k_knn_brute<KM><<<blocks_for(nA), kBlock, 0, st>>>(nA, A->F(0), A->F(1), A->F(2), B->n, B->F(0), B->F(1),    \
-                                                       B->F(2), B->ids(), self, out->k, out->idx_alt(),          \
+                                                       B->F(2), B->ids(), self, out->k, cap2, out->idx_alt(),    \
                                                        out->dist_alt())
     if (out->k <= 8) GS_BRUTE(8);
     else if (out->k <= 16) GS_BRUTE(16);

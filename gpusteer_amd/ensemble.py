@@ -47,6 +47,7 @@ def scenario_struct(spec) -> capi.GsScenario:
         s.walls_min_time, s.walls_weight = sc.min_time_walls, sc.w_walls
         s.obstacles_min_time, s.obstacles_weight = sc.min_time_obstacles, sc.w_obstacles
         s.k_walls, s.k_obstacles = sc.knw, sc.kno
+        s.other_max_distance = sc.other_radius
     return s
 
 
@@ -124,7 +125,8 @@ class Ensemble:
 def choke_step_calls(G, knns, W, O, bins, sc) -> None:
     """One choke-point step through the single-instance calls, in the order of
     plugins/ChokePointGroup.cpp:683-728: bin both groups; per group clear
-    bits, KNN to itself / the other group / the walls / the obstacles, avoid
+    bits, KNN to itself / the other group (within sc.other_radius) / the
+    walls / the obstacles, avoid
     walls, avoid obstacles, separation from both groups, seek; integrate both."""
     for grp in G:
         api.updateKNNDatabase(grp, bins)
@@ -133,7 +135,7 @@ def choke_step_calls(G, knns, W, O, bins, sc) -> None:
         ks, ko, kw, kob = knns[gi]
         api.clearAppliedKernels(grp)
         api.findKNearestNeighbors(grp, ks, bins, grp, 1)
-        api.findKNearestNeighbors(grp, ko, bins, other, 1)
+        api.findKNearestNeighborsWithin(grp, ko, bins, other, sc.other_radius, 1)
         api.findKNearestNeighbors(grp, kw, bins, W, 1)
         api.findKNearestNeighbors(grp, kob, bins, O, 1)
         api.steerToAvoidWalls(grp, kw, W, sc.min_time_walls, sc.w_walls, 0)

@@ -155,6 +155,10 @@ class ChokeScenario:
     w_walls: float = 1.0
     w_obstacles: float = 1.0
     mode: int = 0
+    # KNN to the other group keeps only agents closer than this: beyond the
+    # separation term's max distance (3), so the forces equal the uncapped
+    # search's.
+    other_radius: float = 3.5
 
 
 def choke_walls(gap: float, half_len: float = 100.0, thickness: float = 0.5) -> WallState:

@@ -203,6 +203,14 @@ int gs_update_knn_database(gs_group g, gs_bins bins);
  * depend on it).  Unless GS_KNN_NO_SEED, a previous exact result held by `out`
  * for the same A, B and memberships bounds the search (result unchanged). */
 int gs_find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t search_radius, uint32_t flags);
+/* gs_find_knn restricted to points closer than max_distance (strict, the
+ * locality test of src/lq.c:327: d^2 < r^2 in fp32); rows with fewer than k
+ * such points are padded.  Gives the same forces as the uncapped search for
+ * any behaviour whose neighbourhood lies inside the cap (e.g. separation
+ * from another group, SteerLibrary.cpp:524-541), at a cost independent of
+ * how far away the rest of that group is.  INFINITY = no cap. */
+int gs_find_knn_within(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t search_radius, uint32_t flags,
+                       float max_distance);
 int gs_find_knn_brute(gs_group A, gs_knn out, gs_group B);
 int gs_steer_separation(gs_group A, gs_knn knn, gs_group B, float min_d, float max_d, float cos_max, float weight,
                         uint32_t mask, int mode);
@@ -261,8 +269,9 @@ int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_par
  *   GS_SCENARIO_BOIDS  the gs_step_boids sequence (wrap, bins, KNN, flocking,
  *                      integrate) on one group;
  *   GS_SCENARIO_CHOKE  two groups; per group in order 0, 1: gs_clear_bits,
- *                      KNN to itself / to the other group / to the walls /
- *                      to the obstacles, gs_steer_avoid_walls,
+ *                      KNN to itself / to the other group (capped at
+ *                      other_max_distance) / to the walls / to the
+ *                      obstacles, gs_steer_avoid_walls,
  *                      gs_steer_avoid_obstacles, gs_steer_separation (own
  *                      group, other group), gs_steer_seek (the group's goal);
  *                      then gs_update_group on both
@@ -288,6 +297,7 @@ typedef struct {
     uint32_t seek_mask;
     float walls_min_time, walls_weight, obstacles_min_time, obstacles_weight;
     uint32_t k_walls, k_obstacles;     /* 1..8 */
+    float other_max_distance;          /* KNN to the other group: gs_find_knn_within cap (<= 0: none) */
 } gs_scenario;
 /* k: neighbours per agent (1..8), the same for every scenario. */
 int gs_ensemble_create(gs_context ctx, uint32_t n_scenarios, const gs_scenario* scenarios, uint32_t k,

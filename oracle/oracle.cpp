@@ -228,19 +228,23 @@ static inline void write_knn(const Best* h, int k, int nA, int q, uint32_t* idx,
     }
 }
 
+// Optional radius cap: only points with d2 < cap*cap (fp32, strict) qualify,
+// the locality test of src/lq.c:327; cap = +inf for none.
 static void knn_brute(int nA, const float* ax, const float* ay, const float* az, int nB, const float* bx,
                       const float* by, const float* bz, int self, int k, uint32_t* idx, float* dist,
-                      const uint32_t* queries, int nq) {
+                      const uint32_t* queries, int nq, float cap) {
     // queries (optional): compute only these query indices (results still
     // written to column q of the [k][nA] arrays).
     const int total = queries ? nq : nA;
+    const float r2 = cap * cap;
 #pragma omp parallel for schedule(dynamic, 64)
     for (int t = 0; t < total; ++t) {
         const int q = queries ? (int)queries[t] : t;
         std::vector<Best> h(k, Best{FLT_MAX, UINT32_MAX});
         for (int b = 0; b < nB; ++b) {
             if (self && b == q) continue;
-            insert_best(h.data(), k, d2_of(ax[q], ay[q], az[q], bx[b], by[b], bz[b]), (uint32_t)b);
+            const float d2 = d2_of(ax[q], ay[q], az[q], bx[b], by[b], bz[b]);
+            if (d2 < r2) insert_best(h.data(), k, d2, (uint32_t)b);
         }
         write_knn(h.data(), k, nA, q, idx, dist);
     }
@@ -253,8 +257,9 @@ static void knn_brute(int nA, const float* ax, const float* ay, const float* az,
 // result equals knn_brute by construction and is tested to.
 static void knn_binned(const Grid& g, int nA, const float* ax, const float* ay, const float* az, int nB,
                        const float* sbx, const float* sby, const float* sbz, const uint32_t* bperm,
-                       const uint32_t* bstart, int self, int k, uint32_t* idx, float* dist) {
+                       const uint32_t* bstart, int self, int k, uint32_t* idx, float* dist, float cap) {
     const float hx = g.sx / (float)g.dx, hy = g.sy / (float)g.dy, hz = g.sz / (float)g.dz;
+    const float r2 = cap * cap;
 #pragma omp parallel for schedule(dynamic, 256)
     for (int q = 0; q < nA; ++q) {
         std::vector<Best> h(k, Best{FLT_MAX, UINT32_MAX});
@@ -274,7 +279,8 @@ static void knn_binned(const Grid& g, int nA, const float* ax, const float* ay, 
                         for (uint32_t s = bstart[c]; s < bstart[c + 1]; ++s) {
                             const uint32_t b = bperm[s];
                             if (self && (int)b == q) continue;
-                            insert_best(h.data(), k, d2_of(qx, qy, qz, sbx[s], sby[s], sbz[s]), b);
+                            const float d2 = d2_of(qx, qy, qz, sbx[s], sby[s], sbz[s]);
+                            if (d2 < r2) insert_best(h.data(), k, d2, b);
                         }
                     }
                 }
@@ -292,7 +298,7 @@ static void knn_binned(const Grid& g, int nA, const float* ax, const float* ay, 
             if (all) break;
             // Conservative slack absorbs the rounding of the cell mapping.
             dmin -= 1e-5f * (std::fabs(g.ox) + std::fabs(g.oy) + std::fabs(g.oz) + g.sx + g.sy + g.sz);
-            if (dmin > 0.f && h[k - 1].d2 < dmin * dmin * (1.f - 1e-5f)) break;
+            if (dmin > 0.f && std::min(h[k - 1].d2, r2) < dmin * dmin * (1.f - 1e-5f)) break;
         }
         write_knn(h.data(), k, nA, q, idx, dist);
     }
@@ -679,18 +685,21 @@ void or_bins_build(const or_grid* g, int n, const float* px, const float* py, co
 }
 
 void or_knn_brute(int nA, const float* ax, const float* ay, const float* az, int nB, const float* bx, const float* by,
-                  const float* bz, int self, int k, uint32_t* idx, float* dist, const uint32_t* queries, int nq) {
-    knn_brute(nA, ax, ay, az, nB, bx, by, bz, self, k, idx, dist, queries, nq);
+                  const float* bz, int self, int k, uint32_t* idx, float* dist, const uint32_t* queries, int nq,
+                  float cap) {
+    knn_brute(nA, ax, ay, az, nB, bx, by, bz, self, k, idx, dist, queries, nq, cap);
 }
 
 void or_knn_binned(const or_grid* g, int nA, const float* ax, const float* ay, const float* az, int nB,
-                   const float* bx, const float* by, const float* bz, int self, int k, uint32_t* idx, float* dist) {
+                   const float* bx, const float* by, const float* bz, int self, int k, uint32_t* idx, float* dist,
+                   float cap) {
     Grid G{g->ox, g->oy, g->oz, g->sx, g->sy, g->sz, g->dx, g->dy, g->dz};
     const int C = g->dx * g->dy * g->dz;
     std::vector<uint32_t> key(nB), start((size_t)C + 1), perm(nB);
     std::vector<float> sx(nB), sy(nB), sz(nB);
     bins_build(G, nB, bx, by, bz, key.data(), start.data(), perm.data(), sx.data(), sy.data(), sz.data());
-    knn_binned(G, nA, ax, ay, az, nB, sx.data(), sy.data(), sz.data(), perm.data(), start.data(), self, k, idx, dist);
+    knn_binned(G, nA, ax, ay, az, nB, sx.data(), sy.data(), sz.data(), perm.data(), start.data(), self, k, idx, dist,
+               cap);
 }
 
 #define OR_AGENT_API(T, SUF)                                                                                        \

@@ -66,8 +66,8 @@ def lib() -> C.CDLL:
         L.or_cell_of.restype = C.c_uint32
         L.or_cell_of.argtypes = [C.POINTER(OrGrid), C.c_float, C.c_float, C.c_float]
         L.or_bins_build.argtypes = [C.POINTER(OrGrid), i, f, f, f, u, u, u, f, f, f]
-        L.or_knn_brute.argtypes = [i, f, f, f, i, f, f, f, i, i, u, f, u, i]
-        L.or_knn_binned.argtypes = [C.POINTER(OrGrid), i, f, f, f, i, f, f, f, i, i, u, f]
+        L.or_knn_brute.argtypes = [i, f, f, f, i, f, f, f, i, i, u, f, u, i, C.c_float]
+        L.or_knn_binned.argtypes = [C.POINTER(OrGrid), i, f, f, f, i, f, f, f, i, i, u, f, C.c_float]
         L.or_pow20.restype = C.c_float
         L.or_pow20.argtypes = [C.c_float]
         L.or_num_threads.restype = C.c_int
@@ -142,8 +142,9 @@ def bins_build(g: OrGrid, pos: np.ndarray):
     return key, start, perm, np.stack(sp, axis=1)
 
 
-def knn_brute(a_pos: np.ndarray, b_pos: np.ndarray, k: int, self_group: bool, queries=None):
-    """Exact KNN (key (d2, index)); returns idx, dist as [k][nA]."""
+def knn_brute(a_pos: np.ndarray, b_pos: np.ndarray, k: int, self_group: bool, queries=None, cap=float("inf")):
+    """Exact KNN (key (d2, index)) among points with d2 < cap^2; returns idx,
+    dist as [k][nA]."""
     nA, nB = a_pos.shape[0], b_pos.shape[0]
     A = [_c32(a_pos[:, j]) for j in range(3)]
     B = A if self_group else [_c32(b_pos[:, j]) for j in range(3)]
@@ -151,18 +152,18 @@ def knn_brute(a_pos: np.ndarray, b_pos: np.ndarray, k: int, self_group: bool, qu
     dist = np.full((k, nA), np.finfo(np.float32).max, np.float32)
     q = None if queries is None else np.ascontiguousarray(queries, dtype=np.uint32)
     lib().or_knn_brute(nA, _f(A[0]), _f(A[1]), _f(A[2]), nB, _f(B[0]), _f(B[1]), _f(B[2]), int(self_group), k,
-                       _u(idx), _f(dist), None if q is None else _u(q), 0 if q is None else q.shape[0])
+                       _u(idx), _f(dist), None if q is None else _u(q), 0 if q is None else q.shape[0], cap)
     return idx, dist
 
 
-def knn_binned(g: OrGrid, a_pos: np.ndarray, b_pos: np.ndarray, k: int, self_group: bool):
+def knn_binned(g: OrGrid, a_pos: np.ndarray, b_pos: np.ndarray, k: int, self_group: bool, cap=float("inf")):
     nA, nB = a_pos.shape[0], b_pos.shape[0]
     A = [_c32(a_pos[:, j]) for j in range(3)]
     B = A if self_group else [_c32(b_pos[:, j]) for j in range(3)]
     idx = np.zeros((k, nA), np.uint32)
     dist = np.zeros((k, nA), np.float32)
     lib().or_knn_binned(C.byref(g), nA, _f(A[0]), _f(A[1]), _f(A[2]), nB, _f(B[0]), _f(B[1]), _f(B[2]),
-                        int(self_group), k, _u(idx), _f(dist))
+                        int(self_group), k, _u(idx), _f(dist), cap)
     return idx, dist
 
 
@@ -256,7 +257,8 @@ def boids_step(state, p, knn_fn=None):
 
 def choke_step(groups, sc):
     """One choke-point step for both groups (Jacobi across groups): per group,
-    clear bits -> KNN to itself, to the other group, to walls and obstacles ->
+    clear bits -> KNN to itself, to the other group (within sc.other_radius),
+    to walls and obstacles ->
     avoid walls -> avoid obstacles -> separation (self, other) -> seek; then
     integrate both.  Returns the KNN results per group."""
     g = grid(sc.grid_origin, sc.grid_size, sc.grid_cells)
@@ -268,7 +270,7 @@ def choke_step(groups, sc):
         other = groups[1 - gi]
         st.bits[:] = 0
         ks = knn_binned(g, pos[gi], pos[gi], sc.k, True)[0]
-        ko = knn_binned(g, pos[gi], pos[1 - gi], sc.k, False)[0]
+        ko = knn_binned(g, pos[gi], pos[1 - gi], sc.k, False, cap=sc.other_radius)[0]
         kw = knn_binned(g, pos[gi], wmid, sc.knw, False)[0]
         kob = knn_binned(g, pos[gi], opos, sc.kno, False)[0]
         avoid_walls(st, kw, sc.walls, sc.min_time_walls, sc.w_walls, 0)

@@ -171,3 +171,44 @@ def test_large_flock_subset_against_brute(gs, ctx, orc):
     n = pos.shape[0]
     assert not np.any(idx == np.arange(n, dtype=np.uint32)[None, :])
     assert np.all(dist[1:] >= dist[:-1])
+
+
+@pytest.mark.parametrize("cap", [2.0, 7.5])
+@pytest.mark.parametrize("self_group", [True, False])
+def test_capped_knn_equals_oracle(gs, ctx, orc, cap, self_group):
+    """gs_find_knn_within: the k nearest among points closer than the cap
+    (strict, lq.c's locality test), padded when fewer qualify."""
+    g = np.random.default_rng(21)
+    a = g.uniform(-50, 50, size=(3000, 3)).astype(np.float32)
+    b = a if self_group else g.uniform(-50, 50, size=(2500, 3)).astype(np.float32)
+    k = 7
+    r_idx, r_dist = orc.knn_brute(a, b, k, self_group, cap=cap)
+    bins = gs.KNNBinData(ctx, (10, 10, 10), (110.0,) * 3, origin=(-55.0,) * 3)
+    A = _group(gs, ctx, a)
+    B = A if self_group else _group(gs, ctx, b)
+    gs.updateKNNDatabase(A, bins)
+    if not self_group:
+        gs.updateKNNDatabase(B, bins)
+    knn = gs.KNNData(ctx, a.shape[0], k)
+    for _ in range(2):  # the second call is seeded by the first
+        gs.findKNearestNeighborsWithin(A, knn, bins, B, cap)
+        idx, dist = knn.download()
+        np.testing.assert_array_equal(idx, r_idx)
+        np.testing.assert_array_equal(dist, r_dist)
+    assert np.any(idx == 0xFFFFFFFF) and np.all(dist[idx != 0xFFFFFFFF] < cap)
+    for h in (knn, A, bins) + (() if self_group else (B,)):
+        h.close()
+
+
+@pytest.mark.parametrize("nb", [1, 6, 50, 256])
+def test_small_target_set(gs, ctx, orc, nb):
+    """Target sets of at most 256 points are scanned exhaustively (walls,
+    obstacles); the result is the same exact KNN."""
+    g = np.random.default_rng(nb)
+    a = g.uniform(-50, 50, size=(2000, 3)).astype(np.float32)
+    b = g.uniform(-50, 50, size=(nb, 3)).astype(np.float32)
+    k = 3
+    r_idx, r_dist = orc.knn_brute(a, b, k, False)
+    idx, dist = _run_binned(gs, ctx, a, b, k, (-55,) * 3, (110,) * 3, (40, 40, 40), False)
+    np.testing.assert_array_equal(idx, r_idx)
+    np.testing.assert_array_equal(dist, r_dist)

@@ -87,24 +87,9 @@ def test_boids_box_large_steps(gs, ctx, orc, cells, sr):
 
 
 def _choke_gpu_step(gs, ctx, G, knns, W, O, bins, sc):
-    for grp in G:
-        gs.updateKNNDatabase(grp, bins)
-    for gi, grp in enumerate(G):
-        other = G[1 - gi]
-        ks, ko, kw, kob = knns[gi]
-        gs.clearAppliedKernels(grp)
-        gs.findKNearestNeighbors(grp, ks, bins, grp, 1)
-        gs.findKNearestNeighbors(grp, ko, bins, other, 1)
-        gs.findKNearestNeighbors(grp, kw, bins, W, 1)
-        gs.findKNearestNeighbors(grp, kob, bins, O, 1)
-        gs.steerToAvoidWalls(grp, kw, W, sc.min_time_walls, sc.w_walls, 0)
-        gs.steerToAvoidObstacles(grp, O, kob, sc.min_time_obstacles, sc.w_obstacles, 0)
-        t = sc.sep
-        gs.steerForSeparation(grp, ks, grp, t.min_d, t.max_d, t.cos_max, t.weight, t.mask, sc.mode)
-        gs.steerForSeparation(grp, ko, other, t.min_d, t.max_d, t.cos_max, t.weight, t.mask, sc.mode)
-        gs.steerForSeek(grp, sc.goals[gi], sc.w_seek, sc.mask_seek, sc.mode)
-    for grp in G:
-        gs.updateGroup(grp, sc.dt, sc.mode)
+    from gpusteer_amd.ensemble import choke_step_calls
+
+    choke_step_calls(G, knns, W, O, bins, sc)
 
 
 def test_choke_pipeline_bit_exact(gs, ctx, orc):
