@@ -31,5 +31,5 @@ from .api import (  # noqa: F401
     updateKNNDatabase,
     wrapWorld,
 )
-from .ensemble import Ensemble, Sequential  # noqa: F401
+from .ensemble import Calibration, Ensemble, Sequential  # noqa: F401
 from .state import AgentState, WallState  # noqa: F401

@@ -124,6 +124,28 @@ class GsScenario(C.Structure):
     ]
 
 
+class GsCalibParams(C.Structure):
+    _fields_ = [(f, C.c_float * 2) for f in ("w_sep", "w_ali", "w_coh", "w_obs", "w_wall", "radius", "max_speed")]
+
+
+class GsCalibConfig(C.Structure):
+    _fields_ = [
+        ("separation", GsFlockTerm),
+        ("alignment", GsFlockTerm),
+        ("cohesion", GsFlockTerm),
+        ("seek_weight", C.c_float),
+        ("seek_mask", C.c_uint32),
+        ("walls_min_time", C.c_float),
+        ("obstacles_min_time", C.c_float),
+        ("max_force", C.c_float),
+        ("mass", C.c_float),
+        ("dt", C.c_float),
+        ("penetration_weight", C.c_float),
+        ("steps", C.c_uint32),
+        ("checkpoint_every", C.c_uint32),
+    ]
+
+
 class GpuSteerError(RuntimeError):
     def __init__(self, status: int, where: str, detail: str):
         self.status = status
@@ -198,6 +220,12 @@ _SIGS = {
     "gs_ensemble_upload_obstacles": (C.c_int, [_h, C.c_uint32, C.c_uint32, _fp, _fp, _fp, _fp]),
     "gs_ensemble_download_knn": (C.c_int, [_h, C.c_uint32, C.c_uint32, _up]),
     "gs_ensemble_step": (C.c_int, [_h, C.c_uint32, C.c_uint32]),
+    "gs_calib_create": (C.c_int, [_h, C.POINTER(GsCalibConfig), C.POINTER(_h)]),
+    "gs_calib_destroy": (C.c_int, [_h]),
+    "gs_calib_forward": (C.c_int, [_h, C.POINTER(GsCalibParams), C.POINTER(C.c_double)]),
+    "gs_calib_backward": (C.c_int, [_h, C.POINTER(GsCalibParams)]),
+    "gs_calib_download_state": (C.c_int, [_h, C.c_uint32, C.c_uint32, C.POINTER(GsAgents)]),
+    "gs_calib_timing": (C.c_int, [_h, _fp, _fp]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _fn = getattr(lib, _name)

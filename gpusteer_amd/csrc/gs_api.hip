@@ -1158,4 +1158,88 @@ int gs_ensemble_step(gs_ensemble e, uint32_t steps, uint32_t flags) {
     });
 }
 
+// ---- differentiable calibration ------------------------------------------------
+
+int gs_calib_create(gs_ensemble e, const gs_calib_config* cfg, gs_calib* out) {
+    return guarded([&] {
+        check_ens(e);
+        if (!cfg || !out) fail(GS_ERR_INVALID_ARGUMENT, "config / out is NULL");
+        *out = nullptr;
+        if (cfg->steps == 0 || cfg->checkpoint_every == 0)
+            fail(GS_ERR_INVALID_ARGUMENT, "steps and checkpoint_every must be positive");
+        if (!(cfg->max_force > 0.f) || !(cfg->mass > 0.f) || !(cfg->dt > 0.f))
+            fail(GS_ERR_INVALID_ARGUMENT, "max_force, mass and dt must be positive");
+        for (const auto& d : e->scen)
+            if (d.kind != GS_SCENARIO_CHOKE) fail(GS_ERR_UNSUPPORTED, "calibration needs crowd scenarios only");
+        use_device(e->ctx);
+        auto* c = new gs_calib_s();
+        try {
+            calib_init(c, e, *cfg);
+        } catch (...) {
+            calib_release(c);
+            delete c;
+            throw;
+        }
+        reg_add(c);
+        *out = c;
+    });
+}
+
+int gs_calib_destroy(gs_calib c) {
+    return guarded([&] {
+        check(c, "calibration");
+        reg_del(c);
+        use_device(c->ens->ctx);
+        (void)hipStreamSynchronize(c->ens->ctx->stream);
+        calib_release(c);
+        delete c;
+    });
+}
+
+int gs_calib_forward(gs_calib c, const gs_calib_params* params, double* loss) {
+    return guarded([&] {
+        check(c, "calibration");
+        check_ens(c->ens);
+        if (!params || !loss) fail(GS_ERR_INVALID_ARGUMENT, "params / loss is NULL");
+        const float* p = (const float*)params;
+        for (int j = 0; j < 14; ++j)
+            if (!std::isfinite(p[j])) fail(GS_ERR_INVALID_ARGUMENT, "parameters must be finite");
+        for (int g = 0; g < 2; ++g)
+            if (!(params->radius[g] > 0.f) || !(params->max_speed[g] > 0.f))
+                fail(GS_ERR_INVALID_ARGUMENT, "radius and max_speed must be positive");
+        use_device(c->ens->ctx);
+        *loss = calib_forward(c, *params);
+    });
+}
+
+int gs_calib_backward(gs_calib c, gs_calib_params* grad) {
+    return guarded([&] {
+        check(c, "calibration");
+        check_ens(c->ens);
+        if (!grad) fail(GS_ERR_INVALID_ARGUMENT, "grad is NULL");
+        if (!c->forward_done) fail(GS_ERR_INVALID_ARGUMENT, "call gs_calib_forward first");
+        use_device(c->ens->ctx);
+        calib_backward(c, *grad);
+    });
+}
+
+int gs_calib_download_state(gs_calib c, uint32_t scenario, uint32_t group, gs_agents* host) {
+    return guarded([&] {
+        check(c, "calibration");
+        if (!host) fail(GS_ERR_INVALID_ARGUMENT, "host is NULL");
+        if (!c->forward_done) fail(GS_ERR_INVALID_ARGUMENT, "call gs_calib_forward first");
+        const EnsPop& P = ens_pop(c->ens, scenario, group);
+        use_device(c->ens->ctx);
+        calib_download(c, P, host);
+    });
+}
+
+int gs_calib_timing(gs_calib c, float* forward_ms, float* backward_ms) {
+    return guarded([&] {
+        check(c, "calibration");
+        if (forward_ms) *forward_ms = c->fwd_ms;
+        if (backward_ms) *backward_ms = c->bwd_ms;
+    });
+}
+
 }  // extern "C"

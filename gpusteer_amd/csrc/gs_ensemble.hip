@@ -99,10 +99,23 @@ __device__ __forceinline__ void brute_small(TopK<KMAX>& h, float qx, float qy, f
 
 template <int KMAX>
 __device__ __forceinline__ void store_idx(const TopK<KMAX>& h, uint32_t k, uint32_t N, uint32_t s,
-                                          uint32_t* __restrict__ out) {
+                                          uint32_t* __restrict__ out, const uint32_t* __restrict__ map = nullptr) {
+    const uint32_t row = map ? map[s] : s;
 #pragma unroll
     for (int j = 0; j < KMAX; ++j)
-        if (j >= KMAX - (int)k) out[(size_t)(j - (KMAX - (int)k)) * N + s] = h.t[j];
+        if (j >= KMAX - (int)k) {
+            const uint32_t v = h.t[j];
+            out[(size_t)(j - (KMAX - (int)k)) * N + row] = (map && v != UINT32_MAX) ? map[v] : v;
+        }
+}
+// Wall / obstacle lists hold indices within the scenario: only the row moves.
+template <int KMAX>
+__device__ __forceinline__ void store_local(const TopK<KMAX>& h, uint32_t k, uint32_t N, uint32_t s,
+                                            uint32_t* __restrict__ out, const uint32_t* __restrict__ map) {
+    const uint32_t row = map ? map[s] : s;
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j)
+        if (j >= KMAX - (int)k) out[(size_t)(j - (KMAX - (int)k)) * N + row] = h.t[j];
 }
 
 struct EnsKnnOut {
@@ -112,6 +125,10 @@ struct EnsKnnOut {
     uint32_t* obs;     // [ko][N] obstacle index within the scenario
     const uint32_t* seed_self;   // previous results at the previous order, or NULL
     const uint32_t* seed_other;
+    // NULL: rows and neighbour values are current (sorted) slots.  Otherwise
+    // both are translated through it (sorted slot -> fixed index), for state
+    // kept in a fixed order with a sorted snapshot for the search.
+    const uint32_t* out_map;
 };
 
 __global__ __launch_bounds__(kBlock) void e_knn(EnsDev E, uint32_t N, uint32_t k, const float4* __restrict__ spos,
@@ -133,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void e_knn(EnsDev E, uint32_t N, uint32_t k
                                                         q.z, spos)
                                         : INFINITY;
         knn_search_flat<8>(h, thr, q.x, q.y, q.z, s, P.g, 1, N, spos, start + P.c0, flds);
-        store_idx<8>(h, k, N, s, out.self);
+        store_idx<8>(h, k, N, s, out.self, out.out_map);
     }
     if (P.kind != GS_SCENARIO_CHOKE) return;
     const EnsPop& O = E.pops[P.other];
@@ -145,20 +162,20 @@ __global__ __launch_bounds__(kBlock) void e_knn(EnsDev E, uint32_t N, uint32_t k
                                                                q.x, q.y, q.z, spos)
                                                : INFINITY);
         if (O.n > 0) knn_search_flat<8>(h, thr, q.x, q.y, q.z, UINT32_MAX, O.g, 1, N, spos, start + O.c0, flds);
-        store_idx<8>(h, k, N, s, out.other);
+        store_idx<8>(h, k, N, s, out.other, out.out_map);
     }
     const EnsScen& S = E.scens[P.scen];
     {
         TopK<8> h;
         h.init(S.k_walls);
         brute_small<8>(h, q.x, q.y, q.z, S.nw, wmx + S.w0, wmy + S.w0, wmz + S.w0);
-        store_idx<8>(h, S.k_walls, N, s, out.walls);
+        store_local<8>(h, S.k_walls, N, s, out.walls, out.out_map);
     }
     {
         TopK<8> h;
         h.init(S.k_obs);
         brute_small<8>(h, q.x, q.y, q.z, S.no, ox + S.o0, oy + S.o0, oz + S.o0);
-        store_idx<8>(h, S.k_obs, N, s, out.obs);
+        store_local<8>(h, S.k_obs, N, s, out.obs, out.out_map);
     }
 }
 
@@ -486,6 +503,45 @@ static EnsDev ens_dev(gs_ensemble_s* e) {
     return EnsDev{e->d_pops.p, e->d_scens.p, e->d_block_pop.p};
 }
 
+// Exact KNN lists of every agent (own group, other group capped, walls,
+// obstacles) for state held in a fixed order: the positions px/py/pz are
+// binned into a sorted snapshot (the ensemble's bin buffers as scratch; its
+// own state and KNN containers are not touched) and the lists are written in
+// the fixed order (rows and neighbour values are fixed indices).  Seeds are
+// previous lists in the same fixed order (or NULL).
+void ens_search_fixed(gs_ensemble_s* e, float* px, float* py, float* pz, const uint32_t* ids_fixed,
+                      uint32_t* ids_scratch, float* const scratch[3], const uint32_t* seed_self,
+                      const uint32_t* seed_other, uint32_t* out_self, uint32_t* out_other, uint32_t* out_walls,
+                      uint32_t* out_obs) {
+    hipStream_t st = e->ctx->stream;
+    const uint32_t N = e->N, nblocks = N / kBlock;
+    if (N == 0) return;
+    const EnsDev E = ens_dev(e);
+    e_bin_key<<<nblocks, kBlock, 0, st>>>(E, px, py, pz, e->key.p, e->rank.p, e->count.p);
+    GS_LAUNCHED("e_bin_key");
+    launch_scan_counts(e->count.p, e->start.p, e->C, st);
+    FieldSet fs{};
+    fs.src[0] = px;
+    fs.src[1] = py;
+    fs.src[2] = pz;
+    for (int a = 0; a < 3; ++a) fs.dst[a] = scratch[a];
+    fs.nf = 3;
+    launch_scatter_reorder(N, e->key.p, e->rank.p, e->start.p, ids_fixed, ids_scratch, e->perm_tmp.p, e->id_tmp.p,
+                           e->new_to_old.p, e->old_to_new.p, nullptr, nullptr, fs, e->spos.p, st);
+    EnsKnnOut ko;
+    ko.self = out_self;
+    ko.other = out_other;
+    ko.walls = out_walls;
+    ko.obs = out_obs;
+    ko.seed_self = seed_self;
+    ko.seed_other = seed_other;
+    ko.out_map = e->new_to_old.p;
+    e_knn<<<nblocks, kBlock, 0, st>>>(E, N, e->k, e->spos.p, e->start.p, e->new_to_old.p, e->old_to_new.p, ko,
+                                      e->walls[9].p, e->walls[10].p, e->walls[11].p, e->obs[0].p, e->obs[1].p,
+                                      e->obs[2].p);
+    GS_LAUNCHED("e_knn");
+}
+
 // One step, queued on the ensemble's stream (capturable: kernels only).
 void ens_step(gs_ensemble_s* e) {
     hipStream_t st = e->ctx->stream;
@@ -514,6 +570,7 @@ void ens_step(gs_ensemble_s* e) {
     ko.obs = e->knn_obs.p;
     ko.seed_self = e->seeded ? e->knn_self[e->kcur].p : nullptr;
     ko.seed_other = e->seeded ? e->knn_other[e->kcur].p : nullptr;
+    ko.out_map = nullptr;
     e_knn<<<nblocks, kBlock, 0, st>>>(E, N, e->k, e->spos.p, e->start.p, e->new_to_old.p, e->old_to_new.p, ko,
                                       e->walls[9].p, e->walls[10].p, e->walls[11].p, e->obs[0].p, e->obs[1].p,
                                       e->obs[2].p);

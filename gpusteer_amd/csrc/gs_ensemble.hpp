@@ -72,10 +72,39 @@ struct gs_ensemble_s {
     uint32_t* ids() const { return idb[icur].p; }
 };
 
+struct gs_calib_s {
+    gs_ensemble_s* ens = nullptr;
+    gs_calib_config cfg{};
+    gs_calib_params theta{};
+    bool forward_done = false;
+    uint32_t nseg = 0;
+    // fixed-order state: initial, checkpoints, the recomputed segment
+    std::vector<gs::DevBuf<float>> x0, ckpt, seg;  // 13 arrays per state
+    gs::DevBuf<uint32_t> ids_fixed, ids_scratch;
+    gs::DevBuf<float> pos_scratch[3];
+    // KNN lists of the segment's steps: per step own k, other k, walls kw, obstacles ko
+    std::vector<gs::DevBuf<uint32_t>> lists;
+    gs::DevBuf<float> adj[2][10];
+    gs::DevBuf<double> loss, grad;
+    int final_buf = 0;  // which of seg[0] / seg[1] holds the forward's final state
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    float fwd_ms = 0.f, bwd_ms = 0.f;
+    uint32_t magic = 0x67736361u;
+};
+
 namespace gs {
+void calib_init(gs_calib_s* c, gs_ensemble_s* e, const gs_calib_config& cfg);
+void calib_release(gs_calib_s* c);
+double calib_forward(gs_calib_s* c, const gs_calib_params& th);
+void calib_backward(gs_calib_s* c, gs_calib_params& grad);
+void calib_download(gs_calib_s* c, const EnsPop& P, gs_agents* host);
 void ens_init(gs_ensemble_s* e, uint32_t ns, const gs_scenario* sc, uint32_t k,
               GridDev (*make_grid)(const float*, const float*, const uint32_t*));
 void ens_alloc(gs_ensemble_s* e);
 void ens_upload_geometry(gs_ensemble_s* e);
 void ens_step(gs_ensemble_s* e);
+void ens_search_fixed(gs_ensemble_s* e, float* px, float* py, float* pz, const uint32_t* ids_fixed,
+                      uint32_t* ids_scratch, float* const scratch[3], const uint32_t* seed_self,
+                      const uint32_t* seed_other, uint32_t* out_self, uint32_t* out_other, uint32_t* out_walls,
+                      uint32_t* out_obs);
 }  // namespace gs

@@ -202,3 +202,61 @@ class Sequential:
                 for h in G + [W, O, bins] + [x for ks in knns for x in ks]:
                     h.close()
         self.items = []
+
+
+class Calibration:
+    """gs_calib_*: loss and its gradient with respect to the 14 parameters
+    (scenarios.PARAM_NAMES x 2 groups) over the crowd scenarios of an
+    ensemble, from the ensemble's current state."""
+
+    def __init__(self, ens: Ensemble, config):
+        c = capi.GsCalibConfig()
+        c.separation, c.alignment, c.cohesion = _term(config.separation), _term(config.alignment), _term(
+            config.cohesion)
+        c.seek_weight, c.seek_mask = config.seek_weight, config.seek_mask
+        c.walls_min_time, c.obstacles_min_time = config.walls_min_time, config.obstacles_min_time
+        c.max_force, c.mass, c.dt = config.max_force, config.mass, config.dt
+        c.penetration_weight = config.penetration_weight
+        c.steps, c.checkpoint_every = config.steps, config.checkpoint_every
+        h = C.c_void_p()
+        check(capi.lib.gs_calib_create(ens.h, C.byref(c), C.byref(h)), "gs_calib_create")
+        self.h = h
+        self.ens = ens
+        self.config = config
+
+    @staticmethod
+    def _params(theta) -> capi.GsCalibParams:
+        p = capi.GsCalibParams()
+        v = np.asarray(theta, dtype=np.float32)
+        assert v.shape == (14,)
+        C.memmove(C.byref(p), v.ctypes.data, 14 * 4)
+        return p
+
+    def forward(self, theta) -> float:
+        loss = C.c_double()
+        check(capi.lib.gs_calib_forward(self.h, C.byref(self._params(theta)), C.byref(loss)), "gs_calib_forward")
+        return loss.value
+
+    def backward(self) -> np.ndarray:
+        g = capi.GsCalibParams()
+        check(capi.lib.gs_calib_backward(self.h, C.byref(g)), "gs_calib_backward")
+        out = np.zeros(14, np.float32)
+        C.memmove(out.ctypes.data, C.byref(g), 14 * 4)
+        return out.astype(np.float64)
+
+    def timing(self):
+        f, b = C.c_float(), C.c_float()
+        check(capi.lib.gs_calib_timing(self.h, C.byref(f), C.byref(b)), "gs_calib_timing")
+        return f.value, b.value
+
+    def final_state(self, scenario: int, group: int) -> AgentState:
+        sc = self.ens.specs[scenario][1]
+        out = AgentState.empty(sc.groups[group].n)
+        check(capi.lib.gs_calib_download_state(self.h, scenario, group, C.byref(api._agents_struct(out))),
+              "gs_calib_download_state")
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            check(capi.lib.gs_calib_destroy(self.h), "gs_calib_destroy")
+            self.h = None

@@ -278,3 +278,62 @@ def ensemble_mix(n_scenarios: int = 256, n_min: int = 1024, n_max: int = 65536, 
             sc.mode = mode
             specs.append(("choke", sc))
     return specs
+
+
+# ---------------------------------------------------------------------------
+# Differentiable calibration (crowd scenarios, the adjoint path).
+
+PARAM_NAMES = ("w_sep", "w_ali", "w_coh", "w_obs", "w_wall", "radius", "max_speed")
+
+
+@dataclass
+class CalibConfig:
+    """Behaviour constants of the calibration model (weights, radius and max
+    speed are the parameters).  GPU-kernel neighbourhoods (explicit minimum
+    distance): separation 1.5-3 cone -0.707 masked by wall anti-penetration;
+    alignment 1.5-4 cone 0.7; cohesion 1.5-5 cone -0.15; seek weight 1.5
+    masked by walls / obstacles (ChokePoint.params mask 4480); walls 10 s and
+    obstacles 5 s look-ahead; maxForce 8, mass 1, dt 0.05; penetration weight
+    10; 100 steps, checkpoints every 10."""
+
+    separation: FlockTerm = None
+    alignment: FlockTerm = None
+    cohesion: FlockTerm = None
+    seek_weight: float = 1.5
+    seek_mask: int = 4480
+    walls_min_time: float = 10.0
+    obstacles_min_time: float = 5.0
+    max_force: float = 8.0
+    mass: float = 1.0
+    dt: float = 0.05
+    penetration_weight: float = 10.0
+    steps: int = 100
+    checkpoint_every: int = 10
+
+    def __post_init__(self):
+        if self.separation is None:
+            self.separation = FlockTerm(1.5, 3.0, -0.707, 0.0, 4096)
+        if self.alignment is None:
+            self.alignment = FlockTerm(1.5, 4.0, 0.7, 0.0, 0)
+        if self.cohesion is None:
+            self.cohesion = FlockTerm(1.5, 5.0, -0.15, 0.0, 0)
+
+
+def calib_params(w_sep=(10.0, 10.0), w_ali=(2.0, 2.0), w_coh=(1.0, 1.0), w_obs=(1.0, 1.0), w_wall=(1.0, 1.0),
+                 radius=(0.5, 0.5), max_speed=(3.0, 3.0)) -> np.ndarray:
+    """The 14 parameters as a float64 vector in gs_calib_params order
+    (per name: group 0, group 1)."""
+    return np.array([*w_sep, *w_ali, *w_coh, *w_obs, *w_wall, *radius, *max_speed], dtype=np.float64)
+
+
+def calib_scenarios(n_scenarios: int = 64, n_agents: int = 16384, gap: float = 15.0, workload: int = 4):
+    """Choke-point crowds of the config 1 layout (gap 15) for calibration,
+    lattices sized to density."""
+    specs = []
+    for i in range(n_scenarios):
+        sc = choke(n_total=n_agents, gap=gap, workload=workload * 1000 + i)
+        c = choke_cells_for(n_agents)
+        sc.grid_cells = (c, 1, c)
+        sc.mode = 1
+        specs.append(("choke", sc))
+    return specs

@@ -318,6 +318,50 @@ enum { GS_ENSEMBLE_GRAPH = 0, GS_ENSEMBLE_EAGER = 1 };
  * do not depend on the flag. */
 int gs_ensemble_step(gs_ensemble e, uint32_t steps, uint32_t flags);
 
+/* ---- differentiable calibration (adjoint) -----------------------------------------
+ * Gradients of a trajectory loss over crowd (GS_SCENARIO_CHOKE) scenarios with
+ * respect to per-group steering weights, agent radius and max speed.  One
+ * step of the model, per agent of group g, with every neighbour read from the
+ * step's start (Jacobi) and the reference CUDA kernels' formulas: wall
+ * avoidance with the anti-penetration move (weight w_wall[g]), obstacle
+ * avoidance (w_obs[g]), separation from the own and from the other group
+ * (w_sep[g]), alignment and cohesion with the own group (w_ali[g], w_coh[g]),
+ * seek of the group's goal, then the UpdateCUDAKernel integrate with max speed
+ * max_speed[g]; agent radius radius[g].  KNN lists (own group k, other group
+ * within the scenario's other_max_distance, walls, obstacles) come from the
+ * step's start positions and are held fixed under differentiation, as are all
+ * other branches.
+ *   loss = mean over scenarios of [ mean_i |p_T,i - goal_i|^2
+ *          + penetration_weight * sum_t sum_i sum_{j in the two lists of i}
+ *            max(0, r_i + r_j - |p_j - p_i|)^2 ]
+ * gs_calib_backward runs the hand-written adjoint backwards through the steps,
+ * recomputing each checkpoint_every-step segment from its checkpoint.
+ * No counterpart in the reference. */
+typedef struct gs_calib_s* gs_calib;
+typedef struct {
+    float w_sep[2], w_ali[2], w_coh[2], w_obs[2], w_wall[2], radius[2], max_speed[2];
+} gs_calib_params;
+typedef struct {
+    gs_flock_term separation, alignment, cohesion; /* distances, cone, mask (weights: gs_calib_params) */
+    float seek_weight;
+    uint32_t seek_mask;
+    float walls_min_time, obstacles_min_time;
+    float max_force, mass, dt;
+    float penetration_weight;
+    uint32_t steps, checkpoint_every;
+} gs_calib_config;
+/* The initial state is the ensemble's agent state at this call; every
+ * scenario of e must be GS_SCENARIO_CHOKE.  e must outlive c. */
+int gs_calib_create(gs_ensemble e, const gs_calib_config* config, gs_calib* out);
+int gs_calib_destroy(gs_calib c);
+int gs_calib_forward(gs_calib c, const gs_calib_params* params, double* loss);
+/* After gs_calib_forward: d(loss)/d(params) at the forward's params. */
+int gs_calib_backward(gs_calib c, gs_calib_params* grad);
+/* After gs_calib_forward: the final state (position, forward, speed, side, up). */
+int gs_calib_download_state(gs_calib c, uint32_t scenario, uint32_t group, gs_agents* host);
+/* HIP-event milliseconds of the last forward and backward passes. */
+int gs_calib_timing(gs_calib c, float* forward_ms, float* backward_ms);
+
 #ifdef __cplusplus
 }
 #endif

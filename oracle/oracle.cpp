@@ -645,6 +645,384 @@ template <class T> static void wrap_world(int shape, Agents<T>& A, int nA, const
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Calibration model (the differentiable crowd step) for gradient checks.
+//
+// One step per agent of group g, every neighbour read from the step's start
+// (Jacobi), the reference CUDA kernels' formulas: wall avoidance with the
+// anti-penetration move (CUDA/AvoidWallsCUDAKernel.cu:67-245), obstacle
+// avoidance (CUDA/AvoidObstaclesCUDAKernel.cu:136-253), separation from the
+// own and the other group, alignment and cohesion with the own group
+// (CUDA/FlockingCommon.cuh:4-30 neighbourhoods, SteerForSeparation /
+// Alignment / CohesionCUDAKernel.cu), seek (SteerForSeekCUDAKernel.cu:52-61),
+// then UpdateCUDAKernel (CUDA/UpdateCUDAKernel.cu:78-108).  Loss: mean over
+// agents of |p_T - goal|^2 + pen_w * sum over steps and over each agent's two
+// KNN lists of max(0, r_i + r_j - d)^2.
+//
+// Every branch goes through a decision tape: recorded on a base run, replayed
+// under perturbed parameters, so central differences measure the derivative
+// of the branch the base run took (KNN lists are taped the same way).
+struct C4Params {
+    double w_sep[2], w_ali[2], w_coh[2], w_obs[2], w_wall[2], radius[2], vmax[2];
+};
+struct C4Config {
+    double sep_min, sep_max, sep_cos, ali_min, ali_max, ali_cos, coh_min, coh_max, coh_cos;
+    uint32_t sep_mask, ali_mask, coh_mask, seek_mask;
+    double w_seek, mt_walls, mt_obs, max_force, mass, dt, pen_w;
+    int k, kw, ko, steps;
+    double other_cap;
+};
+struct C4Scenario {
+    int n[2];
+    std::vector<double> st[2][13];  // p f speed side up per group
+    double goal[2][3];
+    std::vector<double> w[9];       // walls: start end normal
+    std::vector<double> o[4];       // obstacles: centre radius
+    Grid grid;
+    // tape: per step, per agent (both groups, group 0 first) decisions; KNN lists
+    std::vector<std::vector<std::vector<uint8_t>>> tape;
+    std::vector<std::vector<uint32_t>> knn;  // per step: own[k][n], other[k][n], walls[kw][n], obs[ko][n] per group
+};
+
+struct Dec {
+    int mode;  // 1 record, 2 replay
+    std::vector<uint8_t>* v;
+    size_t at;
+    bool operator()(bool c) {
+        if (mode == 1) {
+            v->push_back(c ? 1 : 0);
+            return c;
+        }
+        return (*v)[at++] != 0;
+    }
+};
+
+template <class T> static inline V3<T> normalize_d(V3<T> a, Dec& D) {
+    const T l = len(a);
+    return D(l > T(0)) ? dvd(a, l) : a;
+}
+template <class T> static inline V3<T> truncate_d(V3<T> v, T m, Dec& D) {
+    const T l2 = dot(v, v);
+    if (D(l2 <= m * m)) return v;
+    return mul(v, m / std::sqrt(l2));
+}
+template <class T> static bool li_d(V3<T> sa, V3<T> ea, V3<T> sb, V3<T> eb, V3<T>& ip, Dec& D) {
+    const T eps = T(0.001f);
+    const T denom = ((eb.z - sb.z) * (ea.x - sa.x)) - ((eb.x - sb.x) * (ea.z - sa.z));
+    const T numera = ((eb.x - sb.x) * (sa.z - sb.z)) - ((eb.z - sb.z) * (sa.x - sb.x));
+    const T numerb = ((ea.x - sa.x) * (sa.z - sb.z)) - ((ea.z - sa.z) * (sa.x - sb.x));
+    if (D(std::fabs(denom) < eps)) {
+        if (D(std::fabs(numera) < eps && std::fabs(numerb) < eps)) {
+            ip = sa;
+            return true;
+        }
+        return false;
+    }
+    const T ua = numera / denom;
+    const T ub = numerb / denom;
+    if (D(ua >= T(0) && ua <= T(1) && ub >= T(0) && ub <= T(1))) {
+        ip = mk(sa.x + ua * (ea.x - sa.x), T(0), sa.z + ua * (ea.z - sa.z));
+        return true;
+    }
+    return false;
+}
+template <class T> static bool hood_d(V3<T> off, T d2, V3<T> fwd, T mn, T mx, T cs, Dec& D) {
+    const T d = std::sqrt(d2);
+    if (D(d < mn)) return true;
+    if (D(d > mx)) return false;
+    return D(dot(fwd, dvd(off, d)) > cs);
+}
+template <class T> static void apply_d(V3<T>& G, uint32_t& bits, V3<T> f, T w, uint32_t bit, Dec& D) {
+    const V3<T> fw = mul(f, w);
+    if (D(!is_zero(fw))) bits |= bit;
+    G = add(fw, G);
+}
+
+// One step of scenario sc from state X (both groups) into Y; returns the
+// penetration loss of the step.  lists: this step's KNN lists (see C4Scenario).
+template <class T>
+static T c4_step(const C4Scenario& sc, const C4Params& th, const C4Config& cf, const std::vector<T>* X,
+                 std::vector<T>* Y, const std::vector<uint32_t>* lists, std::vector<std::vector<uint8_t>>& tape,
+                 int mode) {
+    const int k = cf.k;
+    T pen_total = T(0);
+    for (int g = 0; g < 2; ++g) {
+        const int n = sc.n[g], no = sc.n[1 - g];
+        const std::vector<T>* Xg = X + 13 * g;
+        const std::vector<T>* Xo = X + 13 * (1 - g);
+        std::vector<T>* Yg = Y + 13 * g;
+        const uint32_t* own = lists[4 * g + 0].data();
+        const uint32_t* oth = lists[4 * g + 1].data();
+        const uint32_t* kw = lists[4 * g + 2].data();
+        const uint32_t* ko = lists[4 * g + 3].data();
+        const T r = T(th.radius[g]), vmax = T(th.vmax[g]);
+        T pen = T(0);
+#pragma omp parallel for schedule(static) reduction(+ : pen)
+        for (int i = 0; i < n; ++i) {
+            Dec D{mode, &tape[(size_t)g * sc.n[0] + i], 0};
+            if (mode == 1) tape[(size_t)g * sc.n[0] + i].clear();
+            auto P3 = [&](const std::vector<T>* A, int base, int j) { return mk(A[base][j], A[base + 1][j], A[base + 2][j]); };
+            V3<T> p = P3(Xg, 0, i), f = P3(Xg, 3, i), sd = P3(Xg, 7, i), up = P3(Xg, 10, i);
+            const T s = Xg[6][i];
+            uint32_t bits = 0;
+            V3<T> G = mk(T(0), T(0), T(0));
+            // walls
+            {
+                const T fl = T(4) * r;
+                const V3<T> vel = mul(f, s);
+                V3<T> fe[3] = {add(p, mul(vel, T(cf.mt_walls))), add(p, add(mul(sd, fl), mul(f, fl))),
+                               add(p, add(mul(neg(sd), fl), mul(f, fl)))};
+                int feeler = 0;
+                T nearest_dist = T(FLT_MAX);
+                bool have_line = false, touching = false;
+                V3<T> nearest_ip = mk(T(0), T(0), T(0)), nearest_normal = nearest_ip;
+                for (int j = 0; j < cf.kw; ++j) {
+                    const uint32_t l = kw[(size_t)j * n + i];
+                    if (D(l >= (uint32_t)sc.w[0].size())) break;
+                    const V3<T> ln = mk(T(sc.w[6][l]), T(sc.w[7][l]), T(sc.w[8][l]));
+                    const V3<T> ls = mk(T(sc.w[0][l]), T(sc.w[1][l]), T(sc.w[2][l]));
+                    const V3<T> le = mk(T(sc.w[3][l]), T(sc.w[4][l]), T(sc.w[5][l]));
+                    V3<T> ip;
+                    if (li_d(p, add(p, mul(neg(ln), r)), ls, le, ip, D)) {
+                        touching = true;
+                        nearest_normal = ln;
+                        nearest_dist = distance(p, ip);
+                    }
+                    for (int q = 0; q < 3; ++q) {
+                        if (li_d(p, fe[q], ls, le, ip, D)) {
+                            const T dist = distance(p, ip);
+                            if (D(dist < nearest_dist)) {
+                                nearest_dist = dist;
+                                have_line = true;
+                                nearest_ip = ip;
+                                feeler = q;
+                                nearest_normal = ln;
+                            }
+                        }
+                    }
+                }
+                V3<T> steer = mk(T(0), T(0), T(0));
+                if (D(touching)) {
+                    const T pn_ = r - nearest_dist;
+                    p = add(p, mul(nearest_normal, pn_));
+                    f = normalize_d(perpendicular_component(f, nearest_normal), D);
+                    bits |= BIT_ANTI_PENETRATION_WALL;
+                } else if (D(have_line)) {
+                    steer = mul(nearest_normal, len(sub(fe[feeler], nearest_ip)));
+                    bits |= BIT_AVOID_WALLS;
+                }
+                G = add(mul(steer, T(th.w_wall[g])), G);
+            }
+            // obstacles
+            {
+                const T min_dist = T(cf.mt_obs) * s;
+                bool n_hit = false;
+                T n_dist = T(0);
+                V3<T> n_pos = mk(T(0), T(0), T(0));
+                for (int j = 0; j < cf.ko; ++j) {
+                    const uint32_t o = ko[(size_t)j * n + i];
+                    if (D(o >= (uint32_t)sc.o[0].size())) break;
+                    const V3<T> op = mk(T(sc.o[0][o]), T(sc.o[1][o]), T(sc.o[2][o]));
+                    const T orr = T(sc.o[3][o]);
+                    bool x_hit = false;
+                    T x_dist = T(0);
+                    const V3<T> go = sub(op, p);
+                    const V3<T> lc = mk(dot(go, sd), dot(go, up), dot(go, f));
+                    const T b = T(-2) * lc.z;
+                    const T c = ((lc.x * lc.x + lc.y * lc.y) + lc.z * lc.z) - (orr + r) * (orr + r);
+                    const T d = (b * b) - (T(4) * c);
+                    if (D(!(d < T(0)))) {
+                        const T sq = std::sqrt(d);
+                        const T pp = (-b + sq) / T(2), qq = (-b - sq) / T(2);
+                        if (D(!((pp < T(0)) && (qq < T(0))))) {
+                            x_hit = true;
+                            x_dist = D((pp > T(0)) && (qq > T(0))) ? (D(pp < qq) ? pp : qq) : (D(pp > T(0)) ? pp : qq);
+                        }
+                    }
+                    if (D((n_hit == false) || ((x_hit != false) && (x_dist < n_dist)))) {
+                        n_hit = x_hit;
+                        n_dist = x_dist;
+                        n_pos = op;
+                    }
+                }
+                V3<T> fo = mk(T(0), T(0), T(0));
+                if (D(n_hit && (n_dist < min_dist))) fo = perpendicular_component(sub(p, n_pos), f);
+                apply_d(G, bits, fo, T(th.w_obs[g]), BIT_AVOID_OBSTACLES, D);
+            }
+            // separation: own group, other group
+            for (int w = 0; w < 2; ++w) {
+                if (D(!(bits & cf.sep_mask))) {
+                    const uint32_t* lst = w == 0 ? own : oth;
+                    const std::vector<T>* B = w == 0 ? Xg : Xo;
+                    const int nb = w == 0 ? n : no;
+                    V3<T> acc = mk(T(0), T(0), T(0));
+                    int cnt = 0;
+                    for (int j = 0; j < k; ++j) {
+                        const uint32_t b = lst[(size_t)j * n + i];
+                        if (D(b >= (uint32_t)nb)) break;
+                        const V3<T> off = sub(P3(B, 0, b), p);
+                        const T d2 = dot(off, off);
+                        if (hood_d(off, d2, f, T(cf.sep_min), T(cf.sep_max), T(cf.sep_cos), D)) {
+                            acc = add(acc, dvd(off, -d2));
+                            ++cnt;
+                        }
+                    }
+                    if (D(cnt > 0)) acc = normalize_d(dvd(acc, (T)cnt), D);
+                    apply_d(G, bits, acc, T(th.w_sep[g]), BIT_SEPARATION, D);
+                }
+            }
+            // alignment
+            if (D(!(bits & cf.ali_mask))) {
+                V3<T> acc = mk(T(0), T(0), T(0));
+                int cnt = 0;
+                for (int j = 0; j < k; ++j) {
+                    const uint32_t b = own[(size_t)j * n + i];
+                    if (D(b >= (uint32_t)n)) break;
+                    const V3<T> off = sub(P3(Xg, 0, b), p);
+                    if (hood_d(off, dot(off, off), f, T(cf.ali_min), T(cf.ali_max), T(cf.ali_cos), D)) {
+                        acc = add(acc, P3(Xg, 3, b));
+                        ++cnt;
+                    }
+                }
+                if (D(cnt > 0)) acc = normalize_d(sub(dvd(acc, (T)cnt), f), D);
+                apply_d(G, bits, acc, T(th.w_ali[g]), BIT_ALIGNMENT, D);
+            }
+            // cohesion
+            if (D(!(bits & cf.coh_mask))) {
+                V3<T> acc = mk(T(0), T(0), T(0));
+                int cnt = 0;
+                for (int j = 0; j < k; ++j) {
+                    const uint32_t b = own[(size_t)j * n + i];
+                    if (D(b >= (uint32_t)n)) break;
+                    const V3<T> pb = P3(Xg, 0, b);
+                    const V3<T> off = sub(pb, p);
+                    if (hood_d(off, dot(off, off), f, T(cf.coh_min), T(cf.coh_max), T(cf.coh_cos), D)) {
+                        acc = add(acc, pb);
+                        ++cnt;
+                    }
+                }
+                if (D(cnt > 0)) acc = normalize_d(sub(dvd(acc, (T)cnt), p), D);
+                apply_d(G, bits, acc, T(th.w_coh[g]), BIT_COHESION, D);
+            }
+            // seek
+            if (D(!(bits & cf.seek_mask))) {
+                const V3<T> goal = mk(T(sc.goal[g][0]), T(sc.goal[g][1]), T(sc.goal[g][2]));
+                apply_d(G, bits, normalize_d(sub(sub(goal, p), f), D), T(cf.w_seek), BIT_SEEK, D);
+            }
+            // integrate (UpdateCUDAKernel)
+            const V3<T> Ft = truncate_d(G, T(cf.max_force), D);
+            const V3<T> acc = dvd(Ft, T(cf.mass));
+            const V3<T> v = truncate_d(add(mul(f, s), mul(acc, T(cf.dt))), vmax, D);
+            const T sn = len(v);
+            V3<T> fn = f, sdn = sd, upn = up;
+            if (D(sn > T(0))) {
+                fn = dvd(v, sn);
+                sdn = normalize_d(cross(fn, up), D);
+                upn = cross(sdn, fn);
+            }
+            const V3<T> pn = add(p, mul(v, T(cf.dt)));
+            const V3<T> out[4] = {pn, fn, sdn, upn};
+            for (int q = 0; q < 3; ++q) {
+                Yg[q][i] = (&out[0].x)[q];
+                Yg[3 + q][i] = (&out[1].x)[q];
+                Yg[7 + q][i] = (&out[2].x)[q];
+                Yg[10 + q][i] = (&out[3].x)[q];
+            }
+            Yg[6][i] = sn;
+            // penetration of this step (step-start positions)
+            const V3<T> p0 = P3(Xg, 0, i);
+            for (int w = 0; w < 2; ++w) {
+                const uint32_t* lst = w == 0 ? own : oth;
+                const std::vector<T>* B = w == 0 ? Xg : Xo;
+                const int nb = w == 0 ? n : no;
+                const T R = r + T(th.radius[w == 0 ? g : 1 - g]);
+                for (int j = 0; j < k; ++j) {
+                    const uint32_t b = lst[(size_t)j * n + i];
+                    if (D(b >= (uint32_t)nb)) break;
+                    const T e = R - distance(P3(B, 0, b), p0);
+                    if (D(e > T(0))) pen += e * e;
+                }
+            }
+        }
+        pen_total += pen;
+    }
+    return pen_total * T(cf.pen_w);
+}
+
+// KNN lists of one step from the state (positions rounded to fp32, as the
+// GPU's), for both groups: own (exact), other (capped), walls / obstacles.
+template <class T>
+static void c4_lists(const C4Scenario& sc, const C4Config& cf, const std::vector<T>* X, std::vector<uint32_t>* lists) {
+    std::vector<float> pos[2][3];
+    for (int g = 0; g < 2; ++g)
+        for (int a = 0; a < 3; ++a) {
+            pos[g][a].resize(sc.n[g]);
+            for (int i = 0; i < sc.n[g]; ++i) pos[g][a][i] = (float)X[13 * g + a][i];
+        }
+    const int nw = (int)sc.w[0].size(), nob = (int)sc.o[0].size();
+    std::vector<float> wm[3], oc[3];
+    for (int a = 0; a < 3; ++a) {
+        wm[a].resize(nw);
+        for (int l = 0; l < nw; ++l) wm[a][l] = 0.5f * ((float)sc.w[a][l] + (float)sc.w[3 + a][l]);
+        oc[a].resize(nob);
+        for (int o = 0; o < nob; ++o) oc[a][o] = (float)sc.o[a][o];
+    }
+    for (int g = 0; g < 2; ++g) {
+        const int n = sc.n[g], no = sc.n[1 - g];
+        std::vector<float> dist;
+        auto run = [&](int which, const float* bx, const float* by, const float* bz, int nb, int k, bool self,
+                       float cap) {
+            std::vector<uint32_t>& L = lists[4 * g + which];
+            L.assign((size_t)k * n, UINT32_MAX);
+            dist.assign((size_t)k * n, 0.f);
+            knn_brute(n, pos[g][0].data(), pos[g][1].data(), pos[g][2].data(), nb, bx, by, bz, self ? 1 : 0, k,
+                      L.data(), dist.data(), nullptr, 0, cap);
+        };
+        run(0, pos[g][0].data(), pos[g][1].data(), pos[g][2].data(), n, cf.k, true, INFINITY);
+        run(1, pos[1 - g][0].data(), pos[1 - g][1].data(), pos[1 - g][2].data(), no, cf.k, false, (float)cf.other_cap);
+        run(2, wm[0].data(), wm[1].data(), wm[2].data(), nw, cf.kw, false, INFINITY);
+        run(3, oc[0].data(), oc[1].data(), oc[2].data(), nob, cf.ko, false, INFINITY);
+    }
+}
+
+// Whole trajectory: returns the loss.  mode 1 records tape + lists, 2 replays.
+template <class T>
+static double c4_run(C4Scenario& sc, const C4Params& th, const C4Config& cf, int mode, double* final_state) {
+    std::vector<T> X[26], Y[26];
+    for (int g = 0; g < 2; ++g)
+        for (int c = 0; c < 13; ++c) {
+            X[13 * g + c].assign(sc.st[g][c].begin(), sc.st[g][c].end());
+            Y[13 * g + c].assign(sc.n[g], T(0));
+        }
+    if (mode == 1) {
+        sc.tape.assign(cf.steps, std::vector<std::vector<uint8_t>>((size_t)sc.n[0] + sc.n[1]));
+        sc.knn.assign((size_t)cf.steps * 8, {});
+    }
+    T loss = T(0);
+    for (int t = 0; t < cf.steps; ++t) {
+        std::vector<uint32_t>* lists = &sc.knn[(size_t)t * 8];
+        if (mode == 1) c4_lists(sc, cf, X, lists);
+        loss += c4_step(sc, th, cf, X, Y, lists, sc.tape[t], mode);
+        for (int c = 0; c < 26; ++c) std::swap(X[c], Y[c]);
+    }
+    T term = T(0);
+    for (int g = 0; g < 2; ++g)
+        for (int i = 0; i < sc.n[g]; ++i) {
+            const V3<T> d = sub(mk(X[13 * g][i], X[13 * g + 1][i], X[13 * g + 2][i]),
+                                mk(T(sc.goal[g][0]), T(sc.goal[g][1]), T(sc.goal[g][2])));
+            term += dot(d, d);
+        }
+    loss += term / T(sc.n[0] + sc.n[1]);
+    if (final_state) {
+        size_t at = 0;
+        for (int g = 0; g < 2; ++g)
+            for (int c = 0; c < 13; ++c)
+                for (int i = 0; i < sc.n[g]; ++i) final_state[at++] = (double)X[13 * g + c][i];
+    }
+    return (double)loss;
+}
+
 }  // namespace orc
 
 // ---------------------------------------------------------------------------
@@ -728,6 +1106,43 @@ void or_knn_binned(const or_grid* g, int nA, const float* ax, const float* ay, c
 
 OR_AGENT_API(float, f32)
 OR_AGENT_API(double, f64)
+
+
+// Calibration model (gradient checks).  or_c4_create copies the scenario:
+// state[2][13][n_g] (p f speed side up), goals[2][3], walls[9][nw], obstacles[4][no].
+void* or_c4_create(int n0, int n1, const double* state0, const double* state1, const double* goals, int nw,
+                   const double* walls, int no, const double* obstacles) {
+    auto* sc = new C4Scenario();
+    sc->n[0] = n0;
+    sc->n[1] = n1;
+    const double* st[2] = {state0, state1};
+    for (int g = 0; g < 2; ++g)
+        for (int c = 0; c < 13; ++c) sc->st[g][c].assign(st[g] + (size_t)c * sc->n[g], st[g] + (size_t)(c + 1) * sc->n[g]);
+    for (int g = 0; g < 2; ++g)
+        for (int a = 0; a < 3; ++a) sc->goal[g][a] = goals[g * 3 + a];
+    for (int j = 0; j < 9; ++j) sc->w[j].assign(walls + (size_t)j * nw, walls + (size_t)(j + 1) * nw);
+    for (int j = 0; j < 4; ++j) sc->o[j].assign(obstacles + (size_t)j * no, obstacles + (size_t)(j + 1) * no);
+    return sc;
+}
+void or_c4_destroy(void* h) { delete (C4Scenario*)h; }
+// params: 14 doubles (gs_calib_params order); config: see C4Config (doubles then ints).
+// precision 32: the model in fp32 (the GPU's arithmetic, for recording the
+// branch decisions the GPU takes); 64: in FP64.
+double or_c4_run(void* h, const double* params, const double* cfgd, const int* cfgi, int mode, double* final_state,
+                 int precision) {
+    C4Params th;
+    memcpy(&th, params, sizeof th);
+    C4Config cf;
+    cf.sep_min = cfgd[0]; cf.sep_max = cfgd[1]; cf.sep_cos = cfgd[2];
+    cf.ali_min = cfgd[3]; cf.ali_max = cfgd[4]; cf.ali_cos = cfgd[5];
+    cf.coh_min = cfgd[6]; cf.coh_max = cfgd[7]; cf.coh_cos = cfgd[8];
+    cf.w_seek = cfgd[9]; cf.mt_walls = cfgd[10]; cf.mt_obs = cfgd[11]; cf.max_force = cfgd[12];
+    cf.mass = cfgd[13]; cf.dt = cfgd[14]; cf.pen_w = cfgd[15]; cf.other_cap = cfgd[16];
+    cf.sep_mask = (uint32_t)cfgi[0]; cf.ali_mask = (uint32_t)cfgi[1]; cf.coh_mask = (uint32_t)cfgi[2];
+    cf.seek_mask = (uint32_t)cfgi[3]; cf.k = cfgi[4]; cf.kw = cfgi[5]; cf.ko = cfgi[6]; cf.steps = cfgi[7];
+    if (precision == 32) return c4_run<float>(*(C4Scenario*)h, th, cf, mode, final_state);
+    return c4_run<double>(*(C4Scenario*)h, th, cf, mode, final_state);
+}
 
 float or_pow20(float r) { return pow20(r); }
 void or_limit_max_deviation_angle(const float* src, float cosc, const float* basis, float* out) {

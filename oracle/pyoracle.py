@@ -284,3 +284,68 @@ def choke_step(groups, sc):
     for st in groups:
         update(st, sc.dt, sc.mode)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Calibration model (FP64), for gradient checks of the adjoint.
+
+C4_STATE = ("px", "py", "pz", "fx", "fy", "fz", "speed", "sx", "sy", "sz", "ux", "uy", "uz")
+
+
+class C4:
+    """One crowd scenario in the oracle's FP64 calibration model.  run(theta,
+    mode=1) records the decision tape and KNN lists; mode=2 replays them, so
+    central differences around the recorded run are the derivative of its
+    branch."""
+
+    def __init__(self, sc, cfg, k=7):
+        L = lib()
+        d = C.POINTER(C.c_double)
+        L.or_c4_create.restype = C.c_void_p
+        L.or_c4_create.argtypes = [C.c_int, C.c_int, d, d, d, C.c_int, d, C.c_int, d]
+        L.or_c4_destroy.argtypes = [C.c_void_p]
+        L.or_c4_run.restype = C.c_double
+        L.or_c4_run.argtypes = [C.c_void_p, d, d, C.POINTER(C.c_int), C.c_int, d, C.c_int]
+        self.n = [g.n for g in sc.groups]
+        st = [np.ascontiguousarray(np.stack([g.arrays[f].astype(np.float64) for f in C4_STATE])) for g in sc.groups]
+        goals = np.ascontiguousarray(np.array(sc.goals, np.float64))
+        w = np.ascontiguousarray(np.stack([sc.walls.arrays[f].astype(np.float64) for f in
+                                           ("sx", "sy", "sz", "ex", "ey", "ez", "nx", "ny", "nz")]))
+        o = np.ascontiguousarray(np.stack([np.asarray(a, np.float64) for a in sc.obstacles]))
+        self._keep = (st, goals, w, o)
+        self.h = L.or_c4_create(self.n[0], self.n[1], _d(st[0]), _d(st[1]), _d(goals), w.shape[1], _d(w),
+                                o.shape[1], _d(o))
+        t = cfg
+        self.cfgd = np.array([t.separation.min_d, t.separation.max_d, t.separation.cos_max, t.alignment.min_d,
+                              t.alignment.max_d, t.alignment.cos_max, t.cohesion.min_d, t.cohesion.max_d,
+                              t.cohesion.cos_max, t.seek_weight, t.walls_min_time, t.obstacles_min_time,
+                              t.max_force, t.mass, t.dt, t.penetration_weight, sc.other_radius], np.float64)
+        self.cfgi = (C.c_int * 8)(t.separation.mask, t.alignment.mask, t.cohesion.mask, t.seek_mask, k, sc.knw,
+                                  sc.kno, t.steps)
+
+    def run(self, theta, mode: int, want_state: bool = False, precision: int = 64):
+        """mode 1 records the tape, 2 replays it; precision 32 runs the model
+        in fp32 (the GPU's arithmetic), 64 in FP64."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        fs = np.zeros(13 * (self.n[0] + self.n[1]), np.float64) if want_state else None
+        loss = lib().or_c4_run(self.h, _d(th), _d(self.cfgd), self.cfgi, mode, None if fs is None else _d(fs),
+                               precision)
+        return (loss, fs) if want_state else loss
+
+    def fd_grad(self, theta, h: float = 1e-3, record_precision: int = 64) -> np.ndarray:
+        """Central differences of the FP64 loss at theta with the branch
+        decisions and KNN lists frozen at a run recorded in the given
+        precision (32: the branches the GPU's fp32 run takes)."""
+        self.run(theta, 1, precision=record_precision)
+        base = self.run(theta, 2)
+        g = np.zeros(14)
+        for j in range(14):
+            e = np.zeros(14)
+            e[j] = h
+            g[j] = (self.run(theta + e, 2) - self.run(theta - e, 2)) / (2 * h)
+        return base, g
+
+    def close(self):
+        if self.h:
+            lib().or_c4_destroy(self.h)
+            self.h = None
