@@ -193,6 +193,49 @@ def ensemble_bench(n_scenarios: int, steps: int, warmup: int) -> dict:
     }
 
 
+def calib_bench(n_scenarios: int, n_agents: int, steps: int, every: int) -> dict:
+    """Adjoint / forward cost of the differentiable calibration: crowd
+    scenarios of the config 1 layout (gap 15), the 14-parameter loss over
+    `steps` steps with checkpoints every `every`; HIP-event times of one
+    forward pass and of one backward pass (segment recompute + adjoint)."""
+    import gpusteer_amd as gs
+    from gpusteer_amd import scenarios
+
+    specs = scenarios.calib_scenarios(n_scenarios=n_scenarios, n_agents=n_agents)
+    cfg = scenarios.CalibConfig(steps=steps, checkpoint_every=every)
+    th = scenarios.calib_params()
+    with gs.Context(0) as ctx:
+        ens = gs.Ensemble(ctx, specs)
+        cal = gs.Calibration(ens, cfg)
+        cal.forward(th)
+        cal.backward()
+        loss = cal.forward(th)
+        grad = cal.backward()
+        fwd_ms, bwd_ms = cal.timing()
+        agents = ens.agents()
+        cal.close()
+        ens.close()
+    print(f"bench: calibration forward {fwd_ms:.1f} ms, backward {bwd_ms:.1f} ms ({steps} steps, {agents} agents)",
+          file=sys.stderr, flush=True)
+    return {
+        "metric": "adjoint/forward cost",
+        "value": bwd_ms / fwd_ms,
+        "unit": "x (backward pass time / forward pass time, per step)",
+        "higher_is_better": False,
+        "forward_ms_per_step": fwd_ms / steps,
+        "adjoint_ms_per_step": bwd_ms / steps,
+        "scenarios": n_scenarios,
+        "agents": agents,
+        "steps": steps,
+        "checkpoint_every": every,
+        "loss": loss,
+        "grad_norm": float((grad ** 2).sum() ** 0.5),
+        "workload": "crowds of the config 1 layout (gap 15); loss = mean |p_T - goal|^2 + 10 x penetration; "
+                    "gradients of 14 parameters (per group: separation, alignment, cohesion, obstacle and wall "
+                    "weights, radius, max speed)",
+    }
+
+
 def cpu_baseline(st, p, seconds: float):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as orc
@@ -231,6 +274,10 @@ def main() -> int:
     ap.add_argument("--ensemble-scenarios", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--calib-scenarios", type=int, default=64)
+    ap.add_argument("--calib-agents", type=int, default=16384)
+    ap.add_argument("--calib-steps", type=int, default=100)
+    ap.add_argument("--no-calib", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 512 MiB device copy (the counter calibration for the traffic figure)")
     args = ap.parse_args()
@@ -325,6 +372,8 @@ def main() -> int:
     }
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(args.ensemble_scenarios, args.ensemble_steps, 3)
+    if not args.no_calib:
+        out["adjoint"] = calib_bench(args.calib_scenarios, args.calib_agents, args.calib_steps, 10)
     if not args.no_cpu:
         print("bench: cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(st, p, args.cpu_seconds)

@@ -470,23 +470,19 @@ __global__ __launch_bounds__(kBlock) void c4_terminal(CalibDev D, CState X, CAdj
     if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, (double)l);
 }
 
-__device__ __forceinline__ void aadd3(const CAdj& A, int base, uint32_t i, f3 v) {
-    unsafeAtomicAdd(A.a[base] + i, v.x);
-    unsafeAtomicAdd(A.a[base + 1] + i, v.y);
-    unsafeAtomicAdd(A.a[base + 2] + i, v.z);
-}
-
 // Parameter-gradient slots (gs_calib_params order).
 enum { TW_SEP = 0, TW_ALI = 2, TW_COH = 4, TW_OBS = 6, TW_WALL = 8, TR = 10, TV = 12, kTheta = 14 };
 
 // Adjoint of one step.  In: the step-t state X, the lists of step t, the
-// adjoint of the step-(t+1) state Ab.  Out (accumulated): the adjoint of the
-// step-t state At and the parameter gradient.
+// adjoint of the step-(t+1) state as own + acc.  Out: the adjoint of the
+// step-t state as own_out (each agent's own terms, plain stores) + acc_out
+// (neighbour terms, one atomic add per neighbour slot and component), and
+// the parameter gradient.
 __global__ __launch_bounds__(kBlock) void c4_backward(CalibDev D, CState X, const uint32_t* __restrict__ kself,
                                                       const uint32_t* __restrict__ kother,
                                                       const uint32_t* __restrict__ kw,
-                                                      const uint32_t* __restrict__ ko, CAdj Ab, CAdj At,
-                                                      double* __restrict__ theta_grad) {
+                                                      const uint32_t* __restrict__ ko, CAdj own_in, CAdj acc_in,
+                                                      CAdj own_out, CAdj acc_out, double* __restrict__ theta_grad) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const EnsPop& P = D.pops[D.block_pop[blockIdx.x]];
     float tg[kTheta];
@@ -500,23 +496,21 @@ __global__ __launch_bounds__(kBlock) void c4_backward(CalibDev D, CState X, cons
         f3 p1, f1, G, pn, fn, sdn, upn;
         float sn;
         c4_fwd_agent(D, P, S, i, X, kself, kother, kw, ko, in, p1, f1, G, pn, fn, sn, sdn, upn, rec);
-        // adjoint of the outputs
-        f3 pnb = mk3(Ab.a[0][i], Ab.a[1][i], Ab.a[2][i]);
-        f3 fnb = mk3(Ab.a[3][i], Ab.a[4][i], Ab.a[5][i]);
-        float snb = Ab.a[6][i];
-        f3 sdnb = mk3(Ab.a[7][i], Ab.a[8][i], Ab.a[9][i]);
-        // own-state adjoints being built
+        auto lam = [&](int c) { return own_in.a[c][i] + acc_in.a[c][i]; };
+        f3 pnb = mk3(lam(0), lam(1), lam(2));
+        f3 fnb = mk3(lam(3), lam(4), lam(5));
+        const float snb = lam(6);
+        const f3 sdnb = mk3(lam(7), lam(8), lam(9));
         f3 pb = mk3(0.f, 0.f, 0.f), fb = pb, sdb = pb;
         float sb = 0.f;
         f3 p1b = mk3(0.f, 0.f, 0.f), f1b = p1b;
         // ---- integrate
         const f3 Ft = truncate3(G, D.max_force);
-        const f3 acc = Ft / D.mass;
-        const f3 v1 = f1 * in.s + acc * D.dt;
+        const f3 accv = Ft / D.mass;
+        const f3 v1 = f1 * in.s + accv * D.dt;
         f3 vb = mk3(0.f, 0.f, 0.f);
         if (rec.moved) {
-            // sdn = normalize(cross(fn, up)), up constant
-            const f3 cb = normalize_adj(cross3(fn, in.up), sdnb);
+            const f3 cb = normalize_adj(cross3(fn, in.up), sdnb);  // sdn = normalize(fn x up), up constant
             fnb = fnb + cross_adj_a(in.up, cb);
             vb = vb + (fnb - fn * dot3(fn, fnb)) / sn;  // fn = v / |v|
             vb = vb + fn * snb;                         // sn = |v|
@@ -531,9 +525,8 @@ __global__ __launch_bounds__(kBlock) void c4_backward(CalibDev D, CState X, cons
         tg[TV + g] += vmaxb;
         f1b = f1b + v1b * in.s;
         sb += dot3(f1, v1b);
-        const f3 accb = v1b * D.dt;
         float fmb = 0.f;
-        const f3 Gb = truncate_adj(G, D.max_force, rec.tf != 0, accb / D.mass, fmb);
+        const f3 Gb = truncate_adj(G, D.max_force, rec.tf != 0, (v1b * D.dt) / D.mass, fmb);
         // ---- seek
         if (rec.on_seek) {
             const f3 goal = mk3(S.goal[g][0], S.goal[g][1], S.goal[g][2]);
@@ -541,70 +534,114 @@ __global__ __launch_bounds__(kBlock) void c4_backward(CalibDev D, CState X, cons
             p1b = p1b - qb;
             f1b = f1b - qb;
         }
-        // ---- cohesion
-        if (rec.on_co && rec.m_co) {
-            const float cnt = (float)__popc(rec.m_co);
-            f3 sum = mk3(0.f, 0.f, 0.f);
-            for (uint32_t j = 0; j < D.k; ++j)
-                if (rec.m_co & (1u << j)) sum = sum + ldS(X, 0, kself[(size_t)j * D.N + i]);
-            const f3 q = (sum / cnt) - p1;
-            const f3 F = normalize3(q);
-            tg[TW_COH + g] += dot3(F, Gb);
-            const f3 qb = normalize_adj(q, Gb * D.th.w_coh[g]);
-            p1b = p1b - qb;
-            const f3 nb = qb / cnt;
-            for (uint32_t j = 0; j < D.k; ++j)
-                if (rec.m_co & (1u << j)) aadd3(At, 0, kself[(size_t)j * D.N + i], nb);
-        }
-        // ---- alignment
-        if (rec.on_al && rec.m_al) {
-            const float cnt = (float)__popc(rec.m_al);
-            f3 sum = mk3(0.f, 0.f, 0.f);
-            for (uint32_t j = 0; j < D.k; ++j)
-                if (rec.m_al & (1u << j)) sum = sum + ldS(X, 3, kself[(size_t)j * D.N + i]);
-            const f3 q = (sum / cnt) - f1;
-            const f3 F = normalize3(q);
-            tg[TW_ALI + g] += dot3(F, Gb);
-            const f3 qb = normalize_adj(q, Gb * D.th.w_ali[g]);
-            f1b = f1b - qb;
-            const f3 nb = qb / cnt;
-            for (uint32_t j = 0; j < D.k; ++j)
-                if (rec.m_al & (1u << j)) aadd3(At, 3, kself[(size_t)j * D.N + i], nb);
-        }
-        // ---- separation (other group, then own group)
-        for (int w = 1; w >= 0; --w) {
-            const uint32_t m = w == 0 ? rec.m_ss : rec.m_so;
-            const int on = w == 0 ? rec.on_ss : rec.on_so;
-            if (!on || !m) continue;
-            const uint32_t* lst = w == 0 ? kself : kother;
-            const float cnt = (float)__popc(m);
-            f3 sum = mk3(0.f, 0.f, 0.f);
-            for (uint32_t j = 0; j < D.k; ++j)
-                if (m & (1u << j)) {
-                    const f3 off = ldS(X, 0, lst[(size_t)j * D.N + i]) - p1;
-                    sum = sum + off / -dot3(off, off);
+        // ---- upstream adjoints of the neighbour terms (first pass over the lists)
+        f3 nb_coh = mk3(0.f, 0.f, 0.f), nb_ali = nb_coh, ab_ss = nb_coh, ab_so = nb_coh;
+        {
+            f3 sc = mk3(0.f, 0.f, 0.f), sa = sc, s_ss = sc, s_so = sc;
+            for (uint32_t j = 0; j < D.k; ++j) {
+                const uint32_t b = kself[(size_t)j * D.N + i];
+                if (b >= D.N) break;
+                const uint32_t bit = 1u << j;
+                if (!((rec.m_co | rec.m_al | rec.m_ss) & bit)) continue;
+                const f3 pj = ldS(X, 0, b);
+                if (rec.m_co & bit) sc = sc + pj;
+                if (rec.m_al & bit) sa = sa + ldS(X, 3, b);
+                if (rec.m_ss & bit) {
+                    const f3 off = pj - p1;
+                    s_ss = s_ss + off / -dot3(off, off);
                 }
-            const f3 mean = sum / cnt;
-            const f3 F = normalize3(mean);
-            tg[TW_SEP + g] += dot3(F, Gb);
-            const f3 accb_ = normalize_adj(mean, Gb * D.th.w_sep[g]) / cnt;
-            for (uint32_t j = 0; j < D.k; ++j)
-                if (m & (1u << j)) {
-                    const uint32_t b = lst[(size_t)j * D.N + i];
+            }
+            for (uint32_t j = 0; j < D.k; ++j) {
+                const uint32_t b = kother[(size_t)j * D.N + i];
+                if (b >= D.N) break;
+                if (rec.m_so & (1u << j)) {
                     const f3 off = ldS(X, 0, b) - p1;
+                    s_so = s_so + off / -dot3(off, off);
+                }
+            }
+            if (rec.on_co && rec.m_co) {
+                const float cnt = (float)__popc(rec.m_co);
+                const f3 q = (sc / cnt) - p1;
+                tg[TW_COH + g] += dot3(normalize3(q), Gb);
+                const f3 qb = normalize_adj(q, Gb * D.th.w_coh[g]);
+                p1b = p1b - qb;
+                nb_coh = qb / cnt;
+            }
+            if (rec.on_al && rec.m_al) {
+                const float cnt = (float)__popc(rec.m_al);
+                const f3 q = (sa / cnt) - f1;
+                tg[TW_ALI + g] += dot3(normalize3(q), Gb);
+                const f3 qb = normalize_adj(q, Gb * D.th.w_ali[g]);
+                f1b = f1b - qb;
+                nb_ali = qb / cnt;
+            }
+            if (rec.on_ss && rec.m_ss) {
+                const float cnt = (float)__popc(rec.m_ss);
+                const f3 mean = s_ss / cnt;
+                tg[TW_SEP + g] += dot3(normalize3(mean), Gb);
+                ab_ss = normalize_adj(mean, Gb * D.th.w_sep[g]) / cnt;
+            }
+            if (rec.on_so && rec.m_so) {
+                const float cnt = (float)__popc(rec.m_so);
+                const f3 mean = s_so / cnt;
+                tg[TW_SEP + g] += dot3(normalize3(mean), Gb);
+                ab_so = normalize_adj(mean, Gb * D.th.w_sep[g]) / cnt;
+            }
+        }
+        // ---- per-neighbour pass: separation, cohesion, alignment and the
+        // step's penetration term, one accumulation per neighbour slot
+        for (int w = 0; w < 2; ++w) {
+            const uint32_t* lst = w == 0 ? kself : kother;
+            const uint32_t msep = w == 0 ? rec.m_ss : rec.m_so;
+            const f3 ab = w == 0 ? ab_ss : ab_so;
+            const int go = w == 0 ? g : 1 - g;
+            const float R = D.th.radius[g] + D.th.radius[go];
+            for (uint32_t j = 0; j < D.k; ++j) {
+                const uint32_t b = lst[(size_t)j * D.N + i];
+                if (b >= D.N) break;
+                const uint32_t bit = 1u << j;
+                const f3 pj = ldS(X, 0, b);
+                f3 cp = mk3(0.f, 0.f, 0.f), cf = cp;
+                if (msep & bit) {
+                    const f3 off = pj - p1;
                     const float d2 = dot3(off, off);
                     // term = -off / d2
-                    const f3 offb = accb_ / -d2 + off * (2.f * dot3(off, accb_) / (d2 * d2));
-                    aadd3(At, 0, b, offb);
+                    const f3 offb = ab / -d2 + off * (2.f * dot3(off, ab) / (d2 * d2));
+                    cp = cp + offb;
                     p1b = p1b - offb;
                 }
+                if (w == 0) {
+                    if (rec.m_co & bit) cp = cp + nb_coh;
+                    if (rec.m_al & bit) cf = cf + nb_ali;
+                }
+                const f3 dv = pj - in.p;
+                const float d = len3(dv);
+                const float e = R - d;
+                if (e > 0.f && d > 0.f) {
+                    const float eb = 2.f * D.pen_w * e * D.inv_S;
+                    tg[TR + g] += eb;
+                    tg[TR + go] += eb;
+                    const f3 gpj = dv * (-eb / d);
+                    cp = cp + gpj;
+                    pb = pb - gpj;
+                }
+                if (!zero3(cp)) {
+                    unsafeAtomicAdd(acc_out.a[0] + b, cp.x);
+                    unsafeAtomicAdd(acc_out.a[1] + b, cp.y);
+                    unsafeAtomicAdd(acc_out.a[2] + b, cp.z);
+                }
+                if (!zero3(cf)) {
+                    unsafeAtomicAdd(acc_out.a[3] + b, cf.x);
+                    unsafeAtomicAdd(acc_out.a[4] + b, cf.y);
+                    unsafeAtomicAdd(acc_out.a[5] + b, cf.z);
+                }
+            }
         }
         // ---- obstacles
         if (rec.o_apply) {
             const f3 c = mk3(D.ox[rec.o_idx], D.oy[rec.o_idx], D.oz[rec.o_idx]);
             const f3 vv = p1 - c;
-            const f3 F = perp3(vv, f1);
-            tg[TW_OBS + g] += dot3(F, Gb);
+            tg[TW_OBS + g] += dot3(perp3(vv, f1), Gb);
             const f3 Fb = Gb * D.th.w_obs[g];
             p1b = p1b + (Fb - f1 * dot3(f1, Fb));
             f1b = f1b - (Fb * dot3(vv, f1) + vv * dot3(f1, Fb));
@@ -620,8 +657,7 @@ __global__ __launch_bounds__(kBlock) void c4_backward(CalibDev D, CState X, cons
             fe[2] = p + ((-sd) * fl + f * fl);
             f3 feb[3] = {mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
             float rb = 0.f;
-            // the own position / forward after walls flow back to p, f
-            pb = pb + p1b;
+            pb = pb + p1b;  // p1 = p (+ n * pen when touching)
             if (rec.w_touch) {
                 const uint32_t gw = S.w0 + rec.w_wall;
                 const f3 n = mk3(D.wnx[gw], D.wny[gw], D.wnz[gw]);
@@ -631,8 +667,7 @@ __global__ __launch_bounds__(kBlock) void c4_backward(CalibDev D, CState X, cons
                 const float penb = dot3(n, p1b);
                 rb += penb;
                 const float distb = -penb;
-                const f3 q = perp3(f, n);
-                const f3 qb = normalize_adj(q, f1b);
+                const f3 qb = normalize_adj(perp3(f, n), f1b);
                 fb = fb + (qb - n * dot3(n, qb));
                 const f3 ea = rec.w_event == 1 ? p + (-n) * r : fe[rec.w_feeler];
                 f3 ip;
@@ -686,34 +721,16 @@ __global__ __launch_bounds__(kBlock) void c4_backward(CalibDev D, CState X, cons
             rb += 4.f * (dot3(feb[1], sd + f) + dot3(feb[2], f - sd));
             tg[TR + g] += rb;
         }
-        // ---- penetration term of step t (step-t positions)
-        {
-            const f3 p = in.p;
-            for (int w = 0; w < 2; ++w) {
-                const uint32_t* lst = w == 0 ? kself : kother;
-                const int go = w == 0 ? g : 1 - g;
-                const float R = D.th.radius[g] + D.th.radius[go];
-                for (uint32_t j = 0; j < D.k; ++j) {
-                    const uint32_t b = lst[(size_t)j * D.N + i];
-                    if (b >= D.N) break;
-                    const f3 dv = ldS(X, 0, b) - p;
-                    const float d = len3(dv);
-                    const float e = R - d;
-                    if (e > 0.f && d > 0.f) {
-                        const float eb = 2.f * D.pen_w * e * D.inv_S;
-                        tg[TR + g] += eb;
-                        tg[TR + go] += eb;
-                        const f3 gpj = dv * (-eb / d);
-                        aadd3(At, 0, b, gpj);
-                        pb = pb - gpj;
-                    }
-                }
-            }
-        }
-        aadd3(At, 0, i, pb);
-        aadd3(At, 3, i, fb);
-        unsafeAtomicAdd(At.a[6] + i, sb);
-        aadd3(At, 7, i, sdb);
+        own_out.a[0][i] = pb.x;
+        own_out.a[1][i] = pb.y;
+        own_out.a[2][i] = pb.z;
+        own_out.a[3][i] = fb.x;
+        own_out.a[4][i] = fb.y;
+        own_out.a[5][i] = fb.z;
+        own_out.a[6][i] = sb;
+        own_out.a[7][i] = sdb.x;
+        own_out.a[8][i] = sdb.y;
+        own_out.a[9][i] = sdb.z;
     }
     // parameter gradient: wave sums, one atomic per slot per wave
 #pragma unroll
@@ -739,9 +756,9 @@ static CState cstate(std::vector<DevBuf<float>>& v, int idx) {
     for (int j = 0; j < kState; ++j) s.a[j] = v[(size_t)idx * kState + j].p;
     return s;
 }
-static CAdj cadj(gs_calib_s* c, int b) {
+static CAdj cadj(DevBuf<float>& buf, size_t N) {
     CAdj a;
-    for (int j = 0; j < kAdj; ++j) a.a[j] = c->adj[b][j].p;
+    for (int j = 0; j < kAdj; ++j) a.a[j] = buf.p + (size_t)j * N;
     return a;
 }
 static FlockTerm cterm(const gs_flock_term& t) { return FlockTerm{t.min_d, t.max_d, t.cos_max, 0.f, t.mask, 1}; }
@@ -822,8 +839,10 @@ void calib_init(gs_calib_s* c, gs_ensemble_s* e, const gs_calib_config& cfg) {
         const int w = (int)(j % 4);
         c->lists[j].reserve(N * (w < 2 ? e->k : (w == 2 ? e->kw_max : e->ko_max)));
     }
-    for (int b = 0; b < 2; ++b)
-        for (int j = 0; j < kAdj; ++j) c->adj[b][j].reserve(N);
+    for (int b = 0; b < 2; ++b) {
+        c->adj_own[b].reserve(N * kAdj);
+        c->adj_acc[b].reserve(N * kAdj);
+    }
     c->ids_fixed.reserve(N);
     c->ids_scratch.reserve(N);
     for (int a = 0; a < 3; ++a) c->pos_scratch[a].reserve(N);
@@ -834,6 +853,45 @@ void calib_init(gs_calib_s* c, gs_ensemble_s* e, const gs_calib_config& cfg) {
         GS_HIP(hipMemcpyAsync(c->x0[j].p, e->F(kStateField[j]), bytes, hipMemcpyDeviceToDevice, st));
     GS_HIP(hipMemcpyAsync(c->ids_fixed.p, e->ids(), bytes, hipMemcpyDeviceToDevice, st));
     for (auto& ev : c->ev) GS_HIP(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
+    // The fixed order is each population's agents sorted once by their
+    // initial cell, so the neighbour gathers and adjoint accumulations of the
+    // whole run touch nearby memory (agents stay near their start).
+    std::vector<std::vector<float>> h(kState, std::vector<float>(N));
+    std::vector<uint32_t> ids(N);
+    for (int j = 0; j < kState; ++j)
+        GS_HIP(hipMemcpyAsync(h[j].data(), c->x0[j].p, bytes, hipMemcpyDeviceToHost, st));
+    GS_HIP(hipMemcpyAsync(ids.data(), c->ids_fixed.p, bytes, hipMemcpyDeviceToHost, st));
+    GS_HIP(hipStreamSynchronize(st));
+    auto axis = [](float p, float o, float s, int d) {
+        if (!(p >= o)) return 0;
+        if (p >= o + s) return d - 1;
+        const int q = (int)(((p - o) / s) * (float)d);
+        return q < d ? q : d - 1;
+    };
+    std::vector<uint32_t> order, key;
+    std::vector<float> tmp;
+    std::vector<uint32_t> tmpi;
+    for (const EnsPop& P : e->pops) {
+        const GridDev& gd = P.g;
+        order.resize(P.n);
+        key.resize(P.n);
+        for (uint32_t q = 0; q < P.n; ++q) {
+            const uint32_t s = P.a0 + q;
+            order[q] = q;
+            key[q] = (uint32_t)(axis(h[0][s], gd.ox, gd.sx, gd.dx) * gd.dy * gd.dz +
+                                axis(h[1][s], gd.oy, gd.sy, gd.dy) * gd.dz + axis(h[2][s], gd.oz, gd.sz, gd.dz));
+        }
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+        for (int j = 0; j < kState; ++j) {
+            tmp.assign(h[j].begin() + P.a0, h[j].begin() + P.a0 + P.n);
+            for (uint32_t q = 0; q < P.n; ++q) h[j][P.a0 + q] = tmp[order[q]];
+        }
+        tmpi.assign(ids.begin() + P.a0, ids.begin() + P.a0 + P.n);
+        for (uint32_t q = 0; q < P.n; ++q) ids[P.a0 + q] = tmpi[order[q]];
+    }
+    for (int j = 0; j < kState; ++j)
+        GS_HIP(hipMemcpyAsync(c->x0[j].p, h[j].data(), bytes, hipMemcpyHostToDevice, st));
+    GS_HIP(hipMemcpyAsync(c->ids_fixed.p, ids.data(), bytes, hipMemcpyHostToDevice, st));
     GS_HIP(hipStreamSynchronize(st));
 }
 
@@ -843,8 +901,10 @@ void calib_release(gs_calib_s* c) {
     for (auto* v : {&c->x0, &c->ckpt, &c->seg})
         for (auto& b : *v) b.release();
     for (auto& b : c->lists) b.release();
-    for (int b = 0; b < 2; ++b)
-        for (int j = 0; j < kAdj; ++j) c->adj[b][j].release();
+    for (int b = 0; b < 2; ++b) {
+        c->adj_own[b].release();
+        c->adj_acc[b].release();
+    }
     c->ids_fixed.release();
     c->ids_scratch.release();
     for (int a = 0; a < 3; ++a) c->pos_scratch[a].release();
@@ -874,8 +934,9 @@ double calib_forward(gs_calib_s* c, const gs_calib_params& th) {
         GS_LAUNCHED("c4_forward");
         cur ^= 1;
     }
-    if (nb) c4_terminal<<<nb, kBlock, 0, st>>>(D, cstate(c->seg, cur), cadj(c, 0), c->loss.p);
+    if (nb) c4_terminal<<<nb, kBlock, 0, st>>>(D, cstate(c->seg, cur), cadj(c->adj_own[0], e->N), c->loss.p);
     GS_LAUNCHED("c4_terminal");
+    zero_words(c->adj_acc[0].p, (size_t)e->N * kAdj, st);
     copy_state(c, c->x0, 1, c->seg, cur);
     GS_HIP(hipEventRecord(c->ev[1], st));
     double loss = 0.0;
@@ -908,10 +969,12 @@ void calib_backward(gs_calib_s* c, gs_calib_params& grad) {
             GS_LAUNCHED("c4_forward");
         }
         for (int j = (int)L - 1; j >= 0; --j) {
-            for (int q = 0; q < kAdj; ++q) zero_words(c->adj[a ^ 1][q].p, N, st);
+            zero_words(c->adj_acc[a ^ 1].p, N * kAdj, st);
             if (nb)
                 c4_backward<<<nb, kBlock, 0, st>>>(D, cstate(c->seg, j), list(c, j, 0), list(c, j, 1), list(c, j, 2),
-                                                   list(c, j, 3), cadj(c, a), cadj(c, a ^ 1), c->grad.p);
+                                                   list(c, j, 3), cadj(c->adj_own[a], N), cadj(c->adj_acc[a], N),
+                                                   cadj(c->adj_own[a ^ 1], N), cadj(c->adj_acc[a ^ 1], N),
+                                                   c->grad.p);
             GS_LAUNCHED("c4_backward");
             a ^= 1;
         }

@@ -84,7 +84,9 @@ struct gs_calib_s {
     gs::DevBuf<float> pos_scratch[3];
     // KNN lists of the segment's steps: per step own k, other k, walls kw, obstacles ko
     std::vector<gs::DevBuf<uint32_t>> lists;
-    gs::DevBuf<float> adj[2][10];
+    // adjoint of a state, as each agent's own terms + accumulated neighbour
+    // terms (10 components x N each), double-buffered
+    gs::DevBuf<float> adj_own[2], adj_acc[2];
     gs::DevBuf<double> loss, grad;
     int final_buf = 0;  // which of seg[0] / seg[1] holds the forward's final state
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
