@@ -140,13 +140,17 @@ def ensemble_bench(n_scenarios: int, steps: int, warmup: int) -> dict:
     import numpy as np
 
     import gpusteer_amd as gs
-    from gpusteer_amd import scenarios
+    from gpusteer_amd import autotune, scenarios
 
-    t0 = time.perf_counter()
-    specs = scenarios.ensemble_mix(n_scenarios=n_scenarios)
-    print(f"bench: ensemble of {n_scenarios} scenarios generated in {time.perf_counter() - t0:.1f} s",
-          file=sys.stderr, flush=True)
     with gs.Context(0) as ctx:
+        # Cell-size factor of the per-scenario lattices, tuned on a 32-scenario sample.
+        factor, ftable = autotune.tune_ensemble_cells(
+            ctx, lambda f: scenarios.ensemble_mix(n_scenarios=min(32, n_scenarios), cell_factor=f))
+        print(f"bench: ensemble autotune {ftable} -> cell factor {factor}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        specs = scenarios.ensemble_mix(n_scenarios=n_scenarios, cell_factor=factor)
+        print(f"bench: ensemble of {n_scenarios} scenarios generated in {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
         ens = gs.Ensemble(ctx, specs)
         agents = ens.agents()
         ens.step(warmup)
@@ -187,6 +191,7 @@ def ensemble_bench(n_scenarios: int, steps: int, warmup: int) -> dict:
         "steps": steps,
         "warmup": warmup,
         "bit_identical_spot_check": bool(same),
+        "autotune": {"cell_factor": factor, "table_ms_per_step": [{"cell_factor": f, "ms": m} for f, m in ftable]},
         "workload": "half Boids flocks (ball R=50, config 0 forces, weights x U(0.7,1.3)), half choke-point "
                     "crowds (config 1 layout, gaps 7.5..32.5); agents log-uniform 1,024-65,536 per scenario; "
                     "k=7; lattices sized to density",
@@ -268,7 +273,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--k", type=int, default=7)
-    ap.add_argument("--cells", type=int, default=64)
+    ap.add_argument("--cells", type=int, default=0, help="cells per axis; 0 = chosen by the autotuner")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble-scenarios", type=int, default=256)
@@ -292,12 +297,23 @@ def main() -> int:
     import gpusteer_amd as gs
     from gpusteer_amd import scenarios
 
-    st, p = scenarios.boids_box(n=args.n, k=args.k, cells=args.cells)
+    from gpusteer_amd import autotune
+
+    st, p = scenarios.boids_box(n=args.n, k=args.k, cells=args.cells if args.cells > 0 else 64)
     n = st.n
-    C = args.cells ** 3
-    print(f"bench: boids_box n={n} k={args.k} cells={args.cells}^3 warmup={args.warmup} steps={args.steps}",
-          file=sys.stderr, flush=True)
     with gs.Context(0) as ctx:
+        tuned = None
+        if args.cells <= 0:
+            (cells, rf), table = autotune.tune_flock(ctx, st, p, k=args.k)
+            p.grid_cells = (cells, cells, cells)
+            p.search_radius = rf
+            tuned = {"chosen": {"cells": cells, "column_radius": rf},
+                     "table_ms_per_step": [{"cells": c, "column_radius": r, "ms": m} for c, r, m in table]}
+            print(f"bench: autotune {table} -> {cells}^3, radius {rf}", file=sys.stderr, flush=True)
+        cells = p.grid_cells[0]
+        C = cells ** 3
+        print(f"bench: boids_box n={n} k={args.k} cells={cells}^3 radius={p.search_radius} warmup={args.warmup} "
+              f"steps={args.steps}", file=sys.stderr, flush=True)
         A = gs.AgentGroup(ctx, st)
         bins = gs.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
         knn = gs.KNNData(ctx, n, args.k)
@@ -341,10 +357,12 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"boids_box: {n} agents, 1000^3 box with face wrap, exact KNN k={args.k}, "
-                        f"{args.cells}^3 bins, SteerLibrary flocking + SimpleVehicle integrate, dt=1/60",
+                        f"{cells}^3 bins, SteerLibrary flocking + SimpleVehicle integrate, dt=1/60",
             "n_agents": n,
             "k": args.k,
-            "cells": [args.cells] * 3,
+            "cells": [cells] * 3,
+            "column_radius": p.search_radius,
+            "autotune": tuned,
             "parallelism": "single GPU",
         },
         "roofline": {

@@ -1,0 +1,58 @@
+"""Autotuner: picks the bin lattice (cells per axis) and the KNN fast path's
+column radius (3 x 3 or 5 x 5 columns) for a flock, and the cell-size factor
+of an ensemble's per-scenario lattices, by timing short runs of the real
+step on the device.  Every candidate computes the same (exact) result; only
+the time differs."""
+from __future__ import annotations
+
+import copy
+import time
+from typing import List, Sequence, Tuple
+
+from . import api
+
+BOIDS_CANDIDATES: Tuple[Tuple[int, int], ...] = ((48, 1), (64, 1), (80, 1), (96, 2), (128, 2))
+
+
+def tune_flock(ctx: api.Context, state, params, k: int = 7,
+               candidates: Sequence[Tuple[int, int]] = BOIDS_CANDIDATES, steps: int = 10, warmup: int = 3):
+    """Times `steps` fused Boids steps (after `warmup`) of the given flock for
+    each (cells per axis, column radius) candidate; returns the fastest
+    candidate and the table of (cells, radius, ms per step)."""
+    table: List[Tuple[int, int, float]] = []
+    for cells, rf in candidates:
+        p = copy.deepcopy(params)
+        p.grid_cells = (cells, cells, cells)
+        p.search_radius = rf
+        A = api.AgentGroup(ctx, state)
+        bins = api.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
+        knn = api.KNNData(ctx, state.n, k)
+        api.stepBoids(A, knn, bins, p, warmup)
+        _, _, _, step_ms = api.stepBoidsTimed(A, knn, bins, p, steps)
+        table.append((cells, rf, step_ms))
+        knn.close()
+        bins.close()
+        A.close()
+    best = min(table, key=lambda r: r[2])
+    return (best[0], best[1]), table
+
+
+def tune_ensemble_cells(ctx: api.Context, make_specs, factors: Sequence[float] = (0.8, 1.0, 1.25, 1.6),
+                        steps: int = 6):
+    """Times an ensemble built by make_specs(cell_factor) for each factor
+    (the first step after the upload and the graph capture excluded);
+    returns the fastest factor and the table of (factor, ms per step)."""
+    from .ensemble import Ensemble
+
+    table: List[Tuple[float, float]] = []
+    for f in factors:
+        ens = Ensemble(ctx, make_specs(f))
+        ens.step(3)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ens.step(steps)
+        ctx.synchronize()
+        table.append((f, (time.perf_counter() - t0) * 1e3 / steps))
+        ens.close()
+    best = min(table, key=lambda r: r[1])
+    return best[0], table

@@ -751,9 +751,10 @@ namespace gs {
 // Ensemble field of each state component (gs_agents order).
 static const int kStateField[kState] = {0, 1, 2, 3, 4, 5, 6, 10, 11, 12, 13, 14, 15};
 
-static CState cstate(std::vector<DevBuf<float>>& v, int idx) {
+// A state is one buffer of kState x N floats (component-major).
+static CState cstate(gs_calib_s* c, std::vector<DevBuf<float>>& v, int idx) {
     CState s;
-    for (int j = 0; j < kState; ++j) s.a[j] = v[(size_t)idx * kState + j].p;
+    for (int j = 0; j < kState; ++j) s.a[j] = v[(size_t)idx].p + (size_t)j * c->ens->N;
     return s;
 }
 static CAdj cadj(DevBuf<float>& buf, size_t N) {
@@ -803,11 +804,8 @@ static CalibDev calib_dev(gs_calib_s* c) {
 
 static void copy_state(gs_calib_s* c, std::vector<DevBuf<float>>& dst, int di, std::vector<DevBuf<float>>& src,
                        int si) {
-    hipStream_t st = c->ens->ctx->stream;
-    const size_t bytes = (size_t)c->ens->N * 4;
-    for (int j = 0; j < kState; ++j)
-        GS_HIP(hipMemcpyAsync(dst[(size_t)di * kState + j].p, src[(size_t)si * kState + j].p, bytes,
-                              hipMemcpyDeviceToDevice, st));
+    GS_HIP(hipMemcpyAsync(dst[(size_t)di].p, src[(size_t)si].p, (size_t)c->ens->N * kState * 4,
+                          hipMemcpyDeviceToDevice, c->ens->ctx->stream));
 }
 
 // Lists of step slot j: own, other, walls, obstacles.
@@ -828,13 +826,19 @@ void calib_init(gs_calib_s* c, gs_ensemble_s* e, const gs_calib_config& cfg) {
     const size_t N = e->N ? e->N : 1;
     hipStream_t st = e->ctx->stream;
     auto alloc_states = [&](std::vector<DevBuf<float>>& v, size_t count) {
-        v.resize(count * kState);
-        for (auto& b : v) b.reserve(N);
+        v.resize(count);
+        for (auto& b : v) b.reserve(N * kState);
     };
     alloc_states(c->x0, 2);  // initial state and the forward's final state
     alloc_states(c->ckpt, std::max<uint32_t>(1, c->nseg));
     alloc_states(c->seg, (size_t)std::max<uint32_t>(K, 1) + 1);
-    c->lists.resize((size_t)std::max<uint32_t>(K, 2) * 4);
+    // The forward keeps every step's KNN lists (integers, held fixed under
+    // differentiation) when they fit in a modest share of HBM, so the
+    // backward recomputes only the state steps; otherwise it recomputes the
+    // lists of each segment too.
+    const size_t list_bytes = N * 4 * (2 * (size_t)e->k + e->kw_max + e->ko_max);
+    c->record_lists = (size_t)T * list_bytes <= (size_t)48 << 30;
+    c->lists.resize((size_t)(c->record_lists ? std::max<uint32_t>(T, 2) : std::max<uint32_t>(K, 2)) * 4);
     for (size_t j = 0; j < c->lists.size(); ++j) {
         const int w = (int)(j % 4);
         c->lists[j].reserve(N * (w < 2 ? e->k : (w == 2 ? e->kw_max : e->ko_max)));
@@ -850,7 +854,7 @@ void calib_init(gs_calib_s* c, gs_ensemble_s* e, const gs_calib_config& cfg) {
     c->grad.reserve(kTheta);
     const size_t bytes = N * 4;
     for (int j = 0; j < kState; ++j)
-        GS_HIP(hipMemcpyAsync(c->x0[j].p, e->F(kStateField[j]), bytes, hipMemcpyDeviceToDevice, st));
+        GS_HIP(hipMemcpyAsync(c->x0[0].p + (size_t)j * N, e->F(kStateField[j]), bytes, hipMemcpyDeviceToDevice, st));
     GS_HIP(hipMemcpyAsync(c->ids_fixed.p, e->ids(), bytes, hipMemcpyDeviceToDevice, st));
     for (auto& ev : c->ev) GS_HIP(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
     // The fixed order is each population's agents sorted once by their
@@ -859,7 +863,7 @@ void calib_init(gs_calib_s* c, gs_ensemble_s* e, const gs_calib_config& cfg) {
     std::vector<std::vector<float>> h(kState, std::vector<float>(N));
     std::vector<uint32_t> ids(N);
     for (int j = 0; j < kState; ++j)
-        GS_HIP(hipMemcpyAsync(h[j].data(), c->x0[j].p, bytes, hipMemcpyDeviceToHost, st));
+        GS_HIP(hipMemcpyAsync(h[j].data(), c->x0[0].p + (size_t)j * N, bytes, hipMemcpyDeviceToHost, st));
     GS_HIP(hipMemcpyAsync(ids.data(), c->ids_fixed.p, bytes, hipMemcpyDeviceToHost, st));
     GS_HIP(hipStreamSynchronize(st));
     auto axis = [](float p, float o, float s, int d) {
@@ -890,7 +894,7 @@ void calib_init(gs_calib_s* c, gs_ensemble_s* e, const gs_calib_config& cfg) {
         for (uint32_t q = 0; q < P.n; ++q) ids[P.a0 + q] = tmpi[order[q]];
     }
     for (int j = 0; j < kState; ++j)
-        GS_HIP(hipMemcpyAsync(c->x0[j].p, h[j].data(), bytes, hipMemcpyHostToDevice, st));
+        GS_HIP(hipMemcpyAsync(c->x0[0].p + (size_t)j * N, h[j].data(), bytes, hipMemcpyHostToDevice, st));
     GS_HIP(hipMemcpyAsync(c->ids_fixed.p, ids.data(), bytes, hipMemcpyHostToDevice, st));
     GS_HIP(hipStreamSynchronize(st));
 }
@@ -926,15 +930,16 @@ double calib_forward(gs_calib_s* c, const gs_calib_params& th) {
     int cur = 0;
     for (uint32_t t = 0; t < T; ++t) {
         if (t % K == 0) copy_state(c, c->ckpt, (int)(t / K), c->seg, cur);
-        const int slot = (int)(t & 1u);
-        calib_search(c, cstate(c->seg, cur), slot, t > 0 ? (slot ^ 1) : -1);
+        const int slot = c->record_lists ? (int)t : (int)(t & 1u);
+        const int prev = c->record_lists ? (int)t - 1 : (slot ^ 1);
+        calib_search(c, cstate(c, c->seg, cur), slot, t > 0 ? prev : -1);
         if (nb)
-            c4_forward<<<nb, kBlock, 0, st>>>(D, cstate(c->seg, cur), cstate(c->seg, cur ^ 1), list(c, slot, 0),
+            c4_forward<<<nb, kBlock, 0, st>>>(D, cstate(c, c->seg, cur), cstate(c, c->seg, cur ^ 1), list(c, slot, 0),
                                               list(c, slot, 1), list(c, slot, 2), list(c, slot, 3), c->loss.p);
         GS_LAUNCHED("c4_forward");
         cur ^= 1;
     }
-    if (nb) c4_terminal<<<nb, kBlock, 0, st>>>(D, cstate(c->seg, cur), cadj(c->adj_own[0], e->N), c->loss.p);
+    if (nb) c4_terminal<<<nb, kBlock, 0, st>>>(D, cstate(c, c->seg, cur), cadj(c->adj_own[0], e->N), c->loss.p);
     GS_LAUNCHED("c4_terminal");
     zero_words(c->adj_acc[0].p, (size_t)e->N * kAdj, st);
     copy_state(c, c->x0, 1, c->seg, cur);
@@ -960,21 +965,25 @@ void calib_backward(gs_calib_s* c, gs_calib_params& grad) {
         const uint32_t t0 = (uint32_t)sgi * K;
         const uint32_t L = std::min(K, T - t0);
         copy_state(c, c->seg, 0, c->ckpt, sgi);
+        // lists of step t0 + j: kept by the forward, or recomputed here
+        auto slot_of = [&](int j) { return c->record_lists ? (int)t0 + j : j; };
         for (uint32_t j = 0; j < L; ++j) {
-            calib_search(c, cstate(c->seg, (int)j), (int)j, j > 0 ? (int)j - 1 : -1);
+            if (!c->record_lists) calib_search(c, cstate(c, c->seg, (int)j), (int)j, j > 0 ? (int)j - 1 : -1);
+            const int sl = slot_of((int)j);
             if (nb)
-                c4_forward<<<nb, kBlock, 0, st>>>(D, cstate(c->seg, (int)j), cstate(c->seg, (int)j + 1),
-                                                  list(c, (int)j, 0), list(c, (int)j, 1), list(c, (int)j, 2),
-                                                  list(c, (int)j, 3), c->loss.p + 1);
+                c4_forward<<<nb, kBlock, 0, st>>>(D, cstate(c, c->seg, (int)j), cstate(c, c->seg, (int)j + 1),
+                                                  list(c, sl, 0), list(c, sl, 1), list(c, sl, 2), list(c, sl, 3),
+                                                  c->loss.p + 1);
             GS_LAUNCHED("c4_forward");
         }
         for (int j = (int)L - 1; j >= 0; --j) {
             zero_words(c->adj_acc[a ^ 1].p, N * kAdj, st);
+            const int sl = slot_of(j);
             if (nb)
-                c4_backward<<<nb, kBlock, 0, st>>>(D, cstate(c->seg, j), list(c, j, 0), list(c, j, 1), list(c, j, 2),
-                                                   list(c, j, 3), cadj(c->adj_own[a], N), cadj(c->adj_acc[a], N),
-                                                   cadj(c->adj_own[a ^ 1], N), cadj(c->adj_acc[a ^ 1], N),
-                                                   c->grad.p);
+                c4_backward<<<nb, kBlock, 0, st>>>(D, cstate(c, c->seg, j), list(c, sl, 0), list(c, sl, 1),
+                                                   list(c, sl, 2), list(c, sl, 3), cadj(c->adj_own[a], N),
+                                                   cadj(c->adj_acc[a], N), cadj(c->adj_own[a ^ 1], N),
+                                                   cadj(c->adj_acc[a ^ 1], N), c->grad.p);
             GS_LAUNCHED("c4_backward");
             a ^= 1;
         }
@@ -999,8 +1008,8 @@ void calib_download(gs_calib_s* c, const EnsPop& P, gs_agents* host) {
     for (int j = 0; j < kState; ++j) {
         float* dst = hf[kStateField[j]];
         if (!dst) continue;
-        GS_HIP(hipMemcpyAsync(v.data(), c->x0[(size_t)kState + j].p + P.a0, (size_t)P.n * 4, hipMemcpyDeviceToHost,
-                              st));
+        GS_HIP(hipMemcpyAsync(v.data(), c->x0[1].p + (size_t)j * c->ens->N + P.a0, (size_t)P.n * 4,
+                              hipMemcpyDeviceToHost, st));
         GS_HIP(hipStreamSynchronize(st));
         for (uint32_t s = 0; s < P.n; ++s) dst[ids[s]] = v[s];
     }

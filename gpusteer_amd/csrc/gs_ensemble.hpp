@@ -79,7 +79,8 @@ struct gs_calib_s {
     bool forward_done = false;
     uint32_t nseg = 0;
     // fixed-order state: initial, checkpoints, the recomputed segment
-    std::vector<gs::DevBuf<float>> x0, ckpt, seg;  // 13 arrays per state
+    std::vector<gs::DevBuf<float>> x0, ckpt, seg;  // one buffer of 13 x N floats per state
+    bool record_lists = false;  // the forward keeps every step's KNN lists
     gs::DevBuf<uint32_t> ids_fixed, ids_scratch;
     gs::DevBuf<float> pos_scratch[3];
     // KNN lists of the segment's steps: per step own k, other k, walls kw, obstacles ko
