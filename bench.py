@@ -192,15 +192,15 @@ def ensemble_bench(n_scenarios: int, steps: int, warmup: int) -> dict:
         "warmup": warmup,
         "bit_identical_spot_check": bool(same),
         "autotune": {"cell_factor": factor, "table_ms_per_step": [{"cell_factor": f, "ms": m} for f, m in ftable]},
-        "workload": "half Boids flocks (ball R=50, config 0 forces, weights x U(0.7,1.3)), half choke-point "
-                    "crowds (config 1 layout, gaps 7.5..32.5); agents log-uniform 1,024-65,536 per scenario; "
+        "workload": "half Boids flocks (ball R=50, Boids plugin forces, weights x U(0.7,1.3)), half choke-point "
+                    "crowds (two groups either side of a wall with one gap, gaps 7.5..32.5); agents log-uniform 1,024-65,536 per scenario; "
                     "k=7; lattices sized to density",
     }
 
 
 def calib_bench(n_scenarios: int, n_agents: int, steps: int, every: int) -> dict:
     """Adjoint / forward cost of the differentiable calibration: crowd
-    scenarios of the config 1 layout (gap 15), the 14-parameter loss over
+    scenarios (two groups either side of a wall with a gap of 15), the 14-parameter loss over
     `steps` steps with checkpoints every `every`; HIP-event times of one
     forward pass and of one backward pass (segment recompute + adjoint)."""
     import gpusteer_amd as gs
@@ -235,7 +235,7 @@ def calib_bench(n_scenarios: int, n_agents: int, steps: int, every: int) -> dict
         "checkpoint_every": every,
         "loss": loss,
         "grad_norm": float((grad ** 2).sum() ** 0.5),
-        "workload": "crowds of the config 1 layout (gap 15); loss = mean |p_T - goal|^2 + 10 x penetration; "
+        "workload": "choke-point crowds (two groups of 8,192 either side of a wall with a gap of 15, 50 obstacles); loss = mean |p_T - goal|^2 + 10 x penetration; "
                     "gradients of 14 parameters (per group: separation, alignment, cohesion, obstacle and wall "
                     "weights, radius, max speed)",
     }
@@ -283,6 +283,8 @@ def main() -> int:
     ap.add_argument("--calib-agents", type=int, default=16384)
     ap.add_argument("--calib-steps", type=int, default=100)
     ap.add_argument("--no-calib", action="store_true")
+    ap.add_argument("--only", choices=("ensemble", "adjoint"),
+                    help="run only that leg and print its JSON object (for per-leg profiles)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 512 MiB device copy (the counter calibration for the traffic figure)")
     args = ap.parse_args()
@@ -294,10 +296,15 @@ def main() -> int:
               "multi-GPU launch; run with --gpus 1.", file=sys.stderr)
         return 2
 
-    import gpusteer_amd as gs
-    from gpusteer_amd import scenarios
+    if args.only == "ensemble":
+        print(json.dumps(ensemble_bench(args.ensemble_scenarios, args.ensemble_steps, 3)), flush=True)
+        return 0
+    if args.only == "adjoint":
+        print(json.dumps(calib_bench(args.calib_scenarios, args.calib_agents, args.calib_steps, 10)), flush=True)
+        return 0
 
-    from gpusteer_amd import autotune
+    import gpusteer_amd as gs
+    from gpusteer_amd import autotune, scenarios
 
     st, p = scenarios.boids_box(n=args.n, k=args.k, cells=args.cells if args.cells > 0 else 64)
     n = st.n

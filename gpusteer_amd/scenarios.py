@@ -250,8 +250,9 @@ def choke_cells_for(n_total: int, region: float = 40.0, extent: float = 200.0, c
 def ensemble_mix(n_scenarios: int = 256, n_min: int = 1024, n_max: int = 65536, workload: int = 3,
                  cell_factor: float = 1.0, mode: int = 0):
     """Mixed ensemble: agent counts log-uniform over [n_min, n_max]; even
-    scenarios are Boids flocks (config 0 layout) with the three weights
-    jittered by U(0.7, 1.3); odd ones are choke-point crowds (config 1 layout)
+    scenarios are Boids flocks (the ball of radius 50 with the Boids plugin's
+    forces) with the three weights jittered by U(0.7, 1.3); odd ones are
+    choke-point crowds (two groups either side of a wall line with one gap)
     with gap widths cycling through 7.5, 10, ..., 32.5.  Lattices are sized to
     each scenario's density.  Returns a list of ("boids", state, params) and
     ("choke", ChokeScenario) specs."""
@@ -327,7 +328,8 @@ def calib_params(w_sep=(10.0, 10.0), w_ali=(2.0, 2.0), w_coh=(1.0, 1.0), w_obs=(
 
 
 def calib_scenarios(n_scenarios: int = 64, n_agents: int = 16384, gap: float = 15.0, workload: int = 4):
-    """Choke-point crowds of the config 1 layout (gap 15) for calibration,
+    """Choke-point crowds (two groups either side of a wall line with one gap
+    of width 15, 50 obstacles) for calibration,
     lattices sized to density."""
     specs = []
     for i in range(n_scenarios):
