@@ -117,6 +117,9 @@ static GridDev make_grid(const float* o, const float* s, const uint32_t* c) {
     g.hx = g.sx / (float)g.dx;
     g.hy = g.sy / (float)g.dy;
     g.hz = g.sz / (float)g.dz;
+    g.rx = (float)g.dx / g.sx;
+    g.ry = (float)g.dy / g.sy;
+    g.rz = (float)g.dz / g.sz;
     g.slack = 1e-5f * (std::fabs(g.ox) + std::fabs(g.oy) + std::fabs(g.oz) + g.sx + g.sy + g.sz);
     return g;
 }
@@ -429,6 +432,8 @@ static void begin_members(gs_group_s* g, uint32_t n) {
     g->slot_of_id.reserve(n);
     launch_iota(g->ids(), n, g->ctx->stream);
     g->n = n;
+    g->inv = 0;
+    g->bits_zero = false;
     g->binned = false;
     g->snap_valid = false;
     g->membership_epoch = next_epoch();
@@ -451,6 +456,21 @@ int gs_group_upload_agents(gs_group g, uint32_t n, const gs_agents* host) {
         begin_members(g, n);
         for (int j = 0; j < kFields && n; ++j) h2d(g->F(j), hf[j], (size_t)n * 4, st);
         if (n) h2d(g->bits(), host->bits, (size_t)n * 4, st);
+        // Fields with one bit pattern in every slot (positions excluded).
+        if (n) {
+            for (int j = 3; j < kFields; ++j) {
+                const uint32_t* w = (const uint32_t*)hf[j];
+                bool same = true;
+                for (uint32_t i = 1; i < n && same; ++i) same = w[i] == w[0];
+                if (same) {
+                    g->inv |= 1u << j;
+                    g->inv_val[j] = hf[j][0];
+                }
+            }
+            bool zero = true;
+            for (uint32_t i = 0; i < n && zero; ++i) zero = host->bits[i] == 0u;
+            g->bits_zero = zero;
+        }
         GS_HIP(hipStreamSynchronize(st));
     });
 }
@@ -487,6 +507,9 @@ int gs_group_device_agents(gs_group g, gs_agents* dev) {
         check_group(g, "group", GS_KIND_AGENTS);
         if (!dev) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
         *dev = g->dev_agents();
+        // The caller may write through these pointers.
+        g->inv = 0;
+        g->bits_zero = false;
     });
 }
 

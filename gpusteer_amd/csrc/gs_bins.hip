@@ -296,19 +296,23 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
     if (n > 0) {
         FieldSet fs{};
         int nf = 0;
+        // Fields holding one value in every slot are unchanged by any
+        // permutation and stay where they are (positions always move: the
+        // snapshot is built from them).
+        const uint32_t moved = g->live & ~(g->inv & ~kFPos);
         for (int f = 0; f < kFields; ++f)
-            if (g->live & (1u << f)) {
+            if (moved & (1u << f)) {
                 fs.src[nf] = g->F(f);
                 fs.dst[nf] = g->Falt(f);
                 ++nf;
             }
         fs.nf = nf;
-        const bool has_bits = g->kind == GS_KIND_AGENTS;
+        const bool has_bits = g->kind == GS_KIND_AGENTS && !g->bits_zero;
         launch_scatter_reorder(n, g->key.p, g->rank.p, g->start.p, g->ids(), g->ids_alt(), g->perm_tmp.p,
                                g->id_tmp.p, g->new_to_old.p, g->old_to_new.p, has_bits ? g->bits() : nullptr,
                                has_bits ? g->bitsb[g->bcur ^ 1].p : nullptr, fs, g->spos.p, st);
         for (int f = 0; f < kFields; ++f)
-            if (g->live & (1u << f)) g->flip(f);
+            if (moved & (1u << f)) g->flip(f);
         if (has_bits) g->bcur ^= 1;
         g->icur ^= 1;
     }

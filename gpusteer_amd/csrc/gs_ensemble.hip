@@ -387,7 +387,10 @@ void ens_init(gs_ensemble_s* e, uint32_t ns, const gs_scenario* sc, uint32_t k,
             P.other = ng == 2 ? first + (1 - gi) : -1;
             P.scen = i;
             pops.push_back(P);
-            a += (uint32_t)((d.n[gi] + kBlock - 1) / kBlock) * kBlock;
+            const uint64_t padded = (uint64_t)((d.n[gi] + kBlock - 1) / kBlock) * kBlock;
+            if ((uint64_t)a + padded >= kFlatMaxPoints)
+                fail(GS_ERR_UNSUPPORTED, "ensemble larger than 2^27 agent slots");
+            a += (uint32_t)padded;
             c += cells + 1;
         }
     }

@@ -60,6 +60,7 @@ struct GridDev {
     float sx, sy, sz;
     int dx, dy, dz;
     float hx, hy, hz;  // cell size (s/d)
+    float rx, ry, rz;  // cells per unit length (d/s), for conservative run bounds
     float slack;       // absolute rounding allowance of the mapping
 };
 
@@ -146,6 +147,11 @@ constexpr int kFields = 23;  // gs_agents float fields
 // Live fields per kind: agents all 23; obstacles centre (0..2) + radius (19);
 // walls midpoint (0..2) + start/end/normal (3..11).
 constexpr uint32_t kFieldsAgents = (1u << kFields) - 1u;
+// Field groups of gs_agents (field order of include/gpusteer.h).
+constexpr uint32_t kFPos = 0x7u;                      // px py pz
+constexpr uint32_t kFKinematic = 0xFFFFu;             // pos fwd speed acc side up (0..15)
+constexpr uint32_t kFSteer = 0x7u << 16;              // gx gy gz
+constexpr uint32_t kFConstants = 0xFu << 19;          // radius maxSpeed maxForce mass
 constexpr uint32_t kFieldsObstacles = 0x7u | (1u << 19);
 constexpr uint32_t kFieldsWalls = 0xFFFu;
 constexpr int kWallField = 3;  // walls: start = F(3..5), end = F(6..8), normal = F(9..11)
@@ -155,6 +161,14 @@ struct gs_group_s {
     int kind = GS_KIND_AGENTS;
     uint32_t n = 0;
     uint32_t live = 0;               // live field mask
+    // Fields whose every slot holds the same bit pattern (inv_val): a
+    // permutation leaves them unchanged, so bin builds do not move them.
+    // Positions are never marked.  Kept by every launch that writes fields
+    // (mark_written / mark_integrated) and cleared when device pointers are
+    // handed out.
+    uint32_t inv = 0;
+    float inv_val[kFields] = {};
+    bool bits_zero = false;          // every slot's bits are 0
     uint64_t membership_epoch = 0;   // changes whenever the members change
     uint64_t order_epoch = 0;        // changes whenever the storage order changes
     uint64_t prev_order_epoch = 0;   // order before the last reorder (0: none)
@@ -190,6 +204,19 @@ struct gs_group_s {
     uint32_t* ids() const { return idb[icur].p; }
     uint32_t* ids_alt() const { return idb[icur ^ 1].p; }
     gs_agents dev_agents() const;
+    // Bookkeeping of inv / bits_zero after a launch that wrote `fields`
+    // (and the bits, when `bits` is set).
+    void mark_written(uint32_t fields, bool bits) {
+        inv &= ~fields;
+        if (bits) bits_zero = false;
+    }
+    // After an integrate: kinematic fields rewritten, steering and bits zero.
+    void mark_integrated() {
+        inv &= ~kFKinematic;
+        inv |= kFSteer;
+        for (int j = 16; j < 19; ++j) inv_val[j] = 0.f;
+        bits_zero = true;
+    }
 };
 
 struct gs_knn_s {

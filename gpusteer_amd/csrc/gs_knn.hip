@@ -185,7 +185,9 @@ static void run_binned_rf(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t 
     // search_radius selects the fast path's column radius: 3 x 3 columns for
     // lattices whose cell is about the k-th neighbour distance, 5 x 5 for
     // finer lattices.
-    if (R0 >= 2) run_binned<KMAX, 2>(A, out, B, R0, cap2, seed, row_map, val_map);
+    // The 3 x 3 path addresses B with 32-bit byte offsets (fewer than 2^27
+    // points); larger groups take the 5 x 5 walk, which is exact as well.
+    if (R0 >= 2 || B->n >= kFlatMaxPoints) run_binned<KMAX, 2>(A, out, B, R0, cap2, seed, row_map, val_map);
     else run_binned<KMAX, 1>(A, out, B, R0, cap2, seed, row_map, val_map);
 }
 

@@ -317,52 +317,91 @@ __device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, f
 // The column walk above runs the columns in lockstep, so a wavefront pays,
 // per column, for its lane with the longest z-run.  Here each lane instead
 // walks its own run list -- the non-empty sphere-exact z-runs of its nine
-// columns, compacted into LDS -- as one stream of candidate slots, four per
-// iteration, so the wavefront pays for its longest whole stream.  The next
-// chunk's loads are issued before the current chunk is evaluated (two chunks
-// in flight per lane), and the run after the current one is read from LDS a
-// run ahead.  Candidates within the bound are written unconditionally to the
-// lane's LDS buffer (the count advances only for passing ones); the buffer is
-// merged into the list by the whole wavefront once any lane holds more than
-// kFlatBuf - 4 entries, so an iteration can never overflow it.
-constexpr int kFlatBuf = 8;
+// columns, compacted into LDS -- as one stream of candidate slots, eight per
+// iteration (two chunks of four, each within one run).  Positions are read
+// with buffer loads: one byte offset per chunk, the four candidates at
+// immediate offsets, and the hardware range check makes a chunk that runs
+// past the end of B read zeros instead of faulting (only candidates inside
+// their run are used).  A candidate within the bound has its slot appended to
+// the lane's LDS buffer -- the store is unconditional, only the count
+// advances -- and its own slot passes like any other (d2 = 0); the merge
+// recomputes each buffered candidate's key from its position, drops the
+// query's own slot and inserts the rest.  The wavefront merges once any lane
+// holds more than kFlatBuf - 8 entries, so an iteration can never overflow
+// the buffer.
+constexpr int kFlatBuf = 16;
 constexpr int kFlatRuns = 9;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 struct FlatLds {
-    uint2* key;       // [kFlatBuf][kBlock]: (d2 bits, agent index)
-    uint32_t* slot;   // [kFlatBuf][kBlock]
-    uint32_t* run_t;  // [kFlatRuns][kBlock]: first slot of a non-empty run
-    uint16_t* run_n;  // [kFlatRuns][kBlock]: its length
+    uint32_t* slot;      // [kFlatBuf][kBlock]: deferred candidate slots
+    uint32_t* run_t;     // [kFlatRuns][kBlock]: first slot of a non-empty run
+    uint16_t* run_n;     // [kFlatRuns][kBlock]: its length
+    uint32_t slot_addr;  // LDS byte address of the lane's first buffer entry
 };
 #define GS_FLAT_LDS                                                                            \
-    __shared__ uint2 lds_flat_key[kFlatBuf * kBlock];                                          \
     __shared__ uint32_t lds_flat_slot[kFlatBuf * kBlock];                                      \
     __shared__ uint32_t lds_flat_run_t[kFlatRuns * kBlock];                                    \
     __shared__ uint16_t lds_flat_run_n[kFlatRuns * kBlock];                                    \
-    const FlatLds flds{lds_flat_key + threadIdx.x, lds_flat_slot + threadIdx.x,                \
-                      lds_flat_run_t + threadIdx.x, lds_flat_run_n + threadIdx.x};
+    const FlatLds flds{lds_flat_slot + threadIdx.x, lds_flat_run_t + threadIdx.x, lds_flat_run_n + threadIdx.x, \
+                       (uint32_t)(size_t)(lds_u32*)(lds_flat_slot + threadIdx.x)};
+
+// Buffer resource over the float4 snapshot of B (raw buffer, byte offsets,
+// range-checked to n * 16 bytes).  Flat-path searches need n < 2^27.
+constexpr uint32_t kFlatMaxPoints = 1u << 27;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pos_rsrc(const float4* p, uint32_t n) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * 16u), 0x00020000);
+}
+__device__ __forceinline__ float4 ld_pos(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                       __uint_as_float(v[3]));
+}
+
+// Cell of coordinate p along one axis for run bounds only: multiplication by
+// the precomputed cells-per-length instead of lq's division, clamped.  Off by
+// at most a few ulps of the cell coordinate from axis_cell, far inside the
+// lattice's rounding slack that the callers add to every bound.
+__device__ __forceinline__ int axis_cell_bound(float p, float o, float r, int d) {
+    const float f = fminf(fmaxf((p - o) * r, 0.f), (float)(d - 1));
+    return (int)f;
+}
 
 template <int KMAX>
-__device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLds& L) {
-    for (int e = 0; e < kFlatBuf; ++e) {
-        if (!__any(e < cnt)) break;
-        if (e < cnt) {
-            const uint2 v = L.key[e * kBlock];
-            const uint64_t kk = ((uint64_t)v.x << 32) | v.y;
-            if (h.accepts(kk)) h.push(kk, L.slot[e * kBlock]);
+__device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLds& L, __amdgpu_buffer_rsrc_t rb,
+                                           float qx, float qy, float qz, uint32_t self_slot) {
+    for (int e0 = 0; e0 < kFlatBuf; e0 += 4) {
+        if (!__any(e0 < cnt)) break;
+        uint32_t sl[4];
+        float4 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            sl[u] = L.slot[(e0 + u) * kBlock];
+            p[u] = ld_pos(rb, sl[u] * 16u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (e0 + u < cnt && sl[u] != self_slot) {
+                const uint64_t kk = make_key(sqdist4(qx, qy, qz, p[u]), id_of(p[u]));
+                if (h.accepts(kk)) h.push(kk, sl[u]);
+            }
         }
     }
     cnt = 0;
 }
 
-// Requires nB >= 1 (every load is clamped to slot nB - 1).
+// Requires 1 <= nB < kFlatMaxPoints.
 template <int KMAX>
 __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, float thr, float qx, float qy, float qz,
                                                 uint32_t self_slot, const GridDev& g, int R0, uint32_t nB,
                                                 const float4* __restrict__ bpos,
                                                 const uint32_t* __restrict__ bstart, const FlatLds& L) {
-    const int cx = axis_cell(qx, g.ox, g.sx, g.dx);
-    const int cy = axis_cell(qy, g.oy, g.sy, g.dy);
-    const int cz = axis_cell(qz, g.oz, g.sz, g.dz);
+    const __amdgpu_buffer_rsrc_t rb = pos_rsrc(bpos, nB);
+    // The query's cell only centres the 3 x 3 x 3 box; every bound below is
+    // geometric with the lattice slack, and the exactness test falls back to
+    // the shell search, so the cheap mapping is exact in effect.
+    const int cx = axis_cell_bound(qx, g.ox, g.rx, g.dx);
+    const int cy = axis_cell_bound(qy, g.oy, g.ry, g.dy);
+    const int cz = axis_cell_bound(qz, g.oz, g.rz, g.dz);
     const float grow = 1.0f + 2e-5f;
     float lim = fminf(thr, h.worst());
     const float bound = lim * grow;
@@ -379,9 +418,13 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, float thr, float 
             if (gxy <= bound) {
                 int z0 = zlo, z1 = zhi;
                 if (bound < INFINITY) {
-                    const float half = sqrtf(bound - gxy) * grow + g.slack;
-                    z0 = max(z0, axis_cell(qz - half, g.oz, g.sz, g.dz));
-                    z1 = min(z1, axis_cell(qz + half, g.oz, g.sz, g.dz));
+                    // Cells of this column that can hold a point within the
+                    // bound: |z - qz| <= sqrt(bound - gxy), widened by the
+                    // growth factor (covers the hardware square root's error)
+                    // and the lattice slack.
+                    const float half = __builtin_amdgcn_sqrtf(bound - gxy) * grow + g.slack;
+                    z0 = max(z0, axis_cell_bound(qz - half, g.oz, g.rz, g.dz));
+                    z1 = min(z1, axis_cell_bound(qz + half, g.oz, g.rz, g.dz));
                 }
                 if (z0 <= z1) {
                     const uint32_t c0 = cell_index(g, ix, iy, z0);
@@ -406,65 +449,72 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, float thr, float 
         }
     }
     if (long_run) M = 0;
-    uint32_t t = 1u, e = 1u;  // empty stream (slot 0 is a valid load address)
-    uint32_t nt = 1u, nn = 0u;  // the run after the current one
+    // A chunk is (t, e): slots t .. min(t + 4, e) - 1 of the current run,
+    // t < e; (0, 0) once the stream is exhausted.
     int m = 0;
+    auto next_chunk = [&](uint32_t t, uint32_t e, uint32_t& tn, uint32_t& en) {
+        tn = t + 4u;
+        en = e;
+        if (tn >= e) {
+            if (m < M) {
+                tn = L.run_t[m * kBlock];
+                en = tn + L.run_n[m * kBlock];
+                ++m;
+            } else {
+                tn = en = 0u;
+            }
+        }
+    };
+    uint32_t ta = 0u, ea = 0u, tb, eb;
     if (M > 0) {
-        t = L.run_t[0];
-        e = t + L.run_n[0];
+        ta = L.run_t[0];
+        ea = ta + L.run_n[0];
         m = 1;
     }
-    if (m < M) {
-        nt = L.run_t[m * kBlock];
-        nn = L.run_n[m * kBlock];
-    }
-    const uint32_t last = nB - 1u;
-    int cnt = 0;
-    // One chunk: choose the next chunk's position (rest of this run, else the
-    // next run), issue its loads, evaluate the current chunk `cur`.
-    auto step = [&](const float4 (&cur)[4], float4 (&nxt)[4]) {
-        uint32_t tn = t + 4u, en = e;
-        if (tn >= e && m < M) {
-            tn = nt;
-            en = nt + nn;
-            ++m;
-            if (m < M) {
-                nt = L.run_t[m * kBlock];
-                nn = L.run_n[m * kBlock];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) nxt[u] = bpos[min(tn + (uint32_t)u, last)];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint32_t slot = t + (uint32_t)u;
-            const float d2 = sqdist4(qx, qy, qz, cur[u]);
-            const bool pass = (slot < e) && (d2 <= lim) && (slot != self_slot);
-            L.key[cnt * kBlock] = make_uint2(__float_as_uint(d2), id_of(cur[u]));
-            L.slot[cnt * kBlock] = slot;
-            cnt += pass ? 1 : 0;
-        }
-        t = tn;
-        e = en;
-    };
-    float4 pa[4], pb[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) pa[u] = bpos[min(t + (uint32_t)u, last)];
-    // The list is only touched between scan segments, so the scan loop keeps
-    // the list's registers untouched; two chunks per trip alternate buffers.
+    next_chunk(ta, ea, tb, eb);
+    // wa: LDS byte address of the lane's next free buffer entry (entries are
+    // kBlock words apart); the scan loop leaves the list's registers
+    // untouched (merges happen between scan segments).
+    constexpr uint32_t kEntry = kBlock * 4u;
+    uint32_t wa = L.slot_addr;
+    const uint32_t wfull = L.slot_addr + (kFlatBuf - 8) * kEntry;
     for (;;) {
-        while (__any(t < e) && !__any(cnt > kFlatBuf - 4)) {
-            step(pa, pb);
-            if (!__any(t < e) || __any(cnt > kFlatBuf - 4)) {
+        while (__any(ta < ea) && !__any(wa > wfull)) {
+            // Byte offsets of the two chunks; the per-candidate offsets are
+            // immediates of the load instructions.
+            uint32_t oa = ta * 16u, ob = tb * 16u;
+            asm volatile("" : "+v"(oa), "+v"(ob));
+            float4 pa[4], pb[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) pa[u] = pb[u];
-                break;
+            for (int u = 0; u < 4; ++u) pa[u] = ld_pos(rb, oa + 16u * u);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pb[u] = ld_pos(rb, ob + 16u * u);
+            uint32_t tc, ec, td, ed;
+            next_chunk(tb, eb, tc, ec);
+            next_chunk(tc, ec, td, ed);
+            const uint32_t na = ea - ta, nb = eb - tb;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float d2 = sqdist4(qx, qy, qz, pa[u]);
+                *(lds_u32*)(size_t)wa = ta + (uint32_t)u;
+                wa += ((uint32_t)u < na && d2 <= lim) ? kEntry : 0u;
             }
-            step(pb, pa);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float d2 = sqdist4(qx, qy, qz, pb[u]);
+                *(lds_u32*)(size_t)wa = tb + (uint32_t)u;
+                wa += ((uint32_t)u < nb && d2 <= lim) ? kEntry : 0u;
+            }
+            ta = tc;
+            ea = ec;
+            tb = td;
+            eb = ed;
         }
-        flat_flush<KMAX>(h, cnt, L);
+        int cnt = (int)((wa - L.slot_addr) / kEntry);
+        flat_flush<KMAX>(h, cnt, L, rb, qx, qy, qz, self_slot);
+        wa = L.slot_addr;
         lim = fminf(thr, h.worst());
-        if (!__any(t < e)) break;
+        if (!__any(ta < ea)) break;
     }
     bool all;
     const float dmin = outside_gap(g, cx, cy, cz, 1, qx, qy, qz, all);

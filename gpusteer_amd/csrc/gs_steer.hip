@@ -176,6 +176,7 @@ void launch_flock(gs_group_s* A, gs_knn_s* knn, gs_group_s* B, const gs_flock_te
     if (knn->k <= 8) run_flock<8>(A, knn, B, sep, ali, coh, mode);
     else if (knn->k <= 16) run_flock<16>(A, knn, B, sep, ali, coh, mode);
     else run_flock<32>(A, knn, B, sep, ali, coh, mode);
+    A->mark_written(kFSteer, true);
 }
 
 void launch_seek(gs_group_s* A, const float* t, float w, uint32_t mask, int mode) {
@@ -187,6 +188,7 @@ void launch_seek(gs_group_s* A, const float* t, float w, uint32_t mask, int mode
         k_seek<GS_MODE_OPENSTEER><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, A->dev_agents(), t[0], t[1], t[2], w,
                                                                        mask);
     GS_LAUNCHED("k_seek");
+    A->mark_written(kFSteer, true);
 }
 
 void launch_avoid_obstacles(gs_group_s* A, gs_group_s* O, gs_knn_s* knn, float min_time, float w, uint32_t mask) {
@@ -195,6 +197,7 @@ void launch_avoid_obstacles(gs_group_s* A, gs_group_s* O, gs_knn_s* knn, float m
                                                                         O->F(1), O->F(2), O->F(19), knn->idx(),
                                                                         knn->k, min_time, w, mask);
     GS_LAUNCHED("k_avoid_obstacles");
+    A->mark_written(kFSteer, true);
 }
 
 void launch_avoid_walls(gs_group_s* A, gs_knn_s* knn, gs_group_s* W, float min_time, float w, uint32_t mask) {
@@ -205,6 +208,8 @@ void launch_avoid_walls(gs_group_s* A, gs_knn_s* knn, gs_group_s* W, float min_t
     k_avoid_walls<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->dev_agents(), W->n, wd, knn->idx(),
                                                                     knn->k, min_time, w, mask);
     GS_LAUNCHED("k_avoid_walls");
+    // touching agents are moved and turned (positions, forward vectors)
+    A->mark_written(kFSteer | 0x3Fu, true);
 }
 
 void launch_update(gs_group_s* A, float dt, int mode) {
@@ -215,6 +220,7 @@ void launch_update(gs_group_s* A, float dt, int mode) {
     else
         k_update<GS_MODE_OPENSTEER><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, A->dev_agents(), dt);
     GS_LAUNCHED("k_update");
+    A->mark_integrated();
 }
 
 void launch_wrap(gs_group_s* A, int shape, const float* c, const float* s) {
@@ -222,12 +228,14 @@ void launch_wrap(gs_group_s* A, int shape, const float* c, const float* s) {
     k_wrap<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->F(0), A->F(1), A->F(2), shape, c[0], c[1], c[2],
                                                             s[0], s[1], s[2]);
     GS_LAUNCHED("k_wrap");
+    A->mark_written(kFPos, false);
 }
 
 void launch_clear_bits(gs_group_s* A) {
     if (A->n == 0) return;
     k_clear_bits<<<blocks_for(A->n), kBlock, 0, A->ctx->stream>>>(A->n, A->bits());
     GS_LAUNCHED("k_clear_bits");
+    A->bits_zero = true;
 }
 
 void launch_check_finite(gs_group_s* A, uint32_t* first_bad_host) {

@@ -18,8 +18,17 @@
 
 namespace gs {
 
+// Group-wide facts the host tracks: steering and bits hold 0 in every slot
+// (then they are neither read nor rewritten), and which per-agent constants
+// hold one value in every slot (then that value is passed instead of read).
+struct StepUniforms {
+    uint32_t steer_zero;
+    uint32_t has;  // bit 0 radius, 1 maxSpeed, 2 maxForce, 3 mass
+    float v[4];
+};
+
 template <int KMAX, int MODE>
-__global__ __launch_bounds__(kBlock) void k_flock_update(uint32_t n, Agents A, float* __restrict__ npx,
+__global__ __launch_bounds__(kBlock) void k_flock_update(uint32_t n, Agents A, StepUniforms U, float* __restrict__ npx,
                                                          float* __restrict__ npy, float* __restrict__ npz,
                                                          float* __restrict__ nfx, float* __restrict__ nfy,
                                                          float* __restrict__ nfz, const float4* __restrict__ spos,
@@ -32,26 +41,32 @@ __global__ __launch_bounds__(kBlock) void k_flock_update(uint32_t n, Agents A, f
     const float4 q = spos[s];
     const f3 pos = mk3(q.x, q.y, q.z);
     const f3 fwd = ld3(A.fx, A.fy, A.fz, s);
-    uint32_t bits = A.bits[s];
+    uint32_t bits = U.steer_zero ? 0u : A.bits[s];
     f3 bp[KMAX], bf[KMAX];
     const int m = gather_neighbours<KMAX>(knn, k, n, s, n, spos, nullptr, nullptr, nullptr, A.fx, A.fy, A.fz,
                                           ali.on && !(bits & ali.mask), bp, bf);
-    f3 steering = ld3(A.gx, A.gy, A.gz, s);
-    flock_core<MODE, KMAX>(pos, fwd, A.radius[s], m, bp, bf, sep, ali, coh, steering, bits);
+    f3 steering = U.steer_zero ? mk3(0.f, 0.f, 0.f) : ld3(A.gx, A.gy, A.gz, s);
+    const float radius = (U.has & 1u) ? U.v[0] : A.radius[s];
+    const float max_speed = (U.has & 2u) ? U.v[1] : A.max_speed[s];
+    const float max_force = (U.has & 4u) ? U.v[2] : A.max_force[s];
+    const float mass = (U.has & 8u) ? U.v[3] : A.mass[s];
+    flock_core<MODE, KMAX>(pos, fwd, radius, m, bp, bf, sep, ali, coh, steering, bits);
     // Integrate (clears steering and bits as updateGroup does).
     f3 p = pos, f = fwd;
     f3 side = ld3(A.sx, A.sy, A.sz, s), up = ld3(A.ux, A.uy, A.uz, s);
     f3 sacc = ld3(A.ax, A.ay, A.az, s);
     float speed = A.speed[s];
-    integrate<MODE>(steering, p, f, speed, sacc, side, up, A.max_force[s], A.max_speed[s], A.mass[s], dt);
+    integrate<MODE>(steering, p, f, speed, sacc, side, up, max_force, max_speed, mass, dt);
     st3(npx, npy, npz, s, p);
     st3(nfx, nfy, nfz, s, f);
     st3(A.sx, A.sy, A.sz, s, side);
     st3(A.ux, A.uy, A.uz, s, up);
     if (MODE == GS_MODE_OPENSTEER) st3(A.ax, A.ay, A.az, s, sacc);
     A.speed[s] = speed;
-    st3(A.gx, A.gy, A.gz, s, mk3(0.f, 0.f, 0.f));
-    A.bits[s] = 0u;
+    if (!U.steer_zero) {
+        st3(A.gx, A.gy, A.gz, s, mk3(0.f, 0.f, 0.f));
+        A.bits[s] = 0u;
+    }
 }
 
 static FlockTerm fterm(const gs_flock_term& t) { return FlockTerm{t.min_d, t.max_d, t.cos_max, t.weight, t.mask, 1}; }
@@ -61,9 +76,17 @@ static void run_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params
     const uint32_t n = A->n;
     hipStream_t st = A->ctx->stream;
     const Agents a = A->dev_agents();
+    StepUniforms U{};
+    U.steer_zero = (A->inv & kFSteer) == kFSteer && A->bits_zero && __builtin_bit_cast(uint32_t, A->inv_val[16]) == 0u &&
+                   __builtin_bit_cast(uint32_t, A->inv_val[17]) == 0u && __builtin_bit_cast(uint32_t, A->inv_val[18]) == 0u;
+    for (int c = 0; c < 4; ++c)
+        if (A->inv & (1u << (19 + c))) {
+            U.has |= 1u << c;
+            U.v[c] = A->inv_val[19 + c];
+        }
 #define GS_FU(MODE)                                                                                                  \
     k_flock_update<KMAX, MODE><<<blocks_for(n), kBlock, 0, st>>>(                                                    \
-        n, a, A->Falt(0), A->Falt(1), A->Falt(2), A->Falt(3), A->Falt(4), A->Falt(5), A->spos.p, knn->idx(), knn->k, \
+        n, a, U, A->Falt(0), A->Falt(1), A->Falt(2), A->Falt(3), A->Falt(4), A->Falt(5), A->spos.p, knn->idx(), knn->k, \
         fterm(p->separation), fterm(p->alignment), fterm(p->cohesion), p->dt)
     if (p->mode == GS_MODE_GPUREF) GS_FU(GS_MODE_GPUREF);
     else GS_FU(GS_MODE_OPENSTEER);
@@ -76,6 +99,7 @@ void launch_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p)
     if (knn->k <= 8) run_flock_update<8>(A, knn, p);
     else if (knn->k <= 16) run_flock_update<16>(A, knn, p);
     else run_flock_update<32>(A, knn, p);
+    A->mark_integrated();
 }
 
 }  // namespace gs

@@ -68,6 +68,70 @@ def test_fused_equals_separate_calls(gs, ctx):
         h.close()
 
 
+def _per_agent_constants(st, seed):
+    g = np.random.default_rng(seed)
+    n = st.n
+    st.arrays["radius"][:] = g.uniform(0.3, 0.7, n).astype(np.float32)
+    st.arrays["max_speed"][:] = g.uniform(6.0, 9.0, n).astype(np.float32)
+    st.arrays["max_force"][:] = g.uniform(20.0, 27.0, n).astype(np.float32)
+    st.arrays["mass"][:] = g.uniform(0.8, 1.2, n).astype(np.float32)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_per_agent_constants_fused(gs, ctx, orc, mode):
+    """Per-agent radius / max speed / max force / mass (no field is uniform,
+    so bin builds move them all): 30 fused steps bit-exact."""
+    st, p = scenarios.boids_sphere(n=2048)
+    p.mode = mode
+    _per_agent_constants(st, 5)
+    ref = st.copy()
+    A = gs.AgentGroup(ctx, st)
+    bins = gs.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
+    knn = gs.KNNData(ctx, st.n, p.k)
+    for step in range(30):
+        gs.stepBoids(A, knn, bins, p, 1)
+        orc.boids_step(ref, p)
+    assert_state_equal(A.download(), ref, what="per-agent constants: ")
+    for h in (A, bins, knn):
+        h.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_steering_pending_across_bin_build(gs, ctx, orc, mode):
+    """A behaviour applied before the bin build (seek), so the build moves
+    non-zero steering and bits, then wrap -> KNN -> flocking -> integrate;
+    alternating with fused steps (steering and bits zero at the build).
+    Every step bit-exact against the same operator sequence in the oracle."""
+    st, p = scenarios.boids_sphere(n=2048)
+    p.mode = mode
+    _per_agent_constants(st, 6)
+    st.arrays["radius"][:] = 0.5  # one uniform constant field
+    ref = st.copy()
+    A = gs.AgentGroup(ctx, st)
+    bins = gs.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
+    knn = gs.KNNData(ctx, st.n, p.k)
+    g = orc.grid(p.grid_origin, p.grid_size, p.grid_cells)
+    target = (10.0, -5.0, 3.0)
+    for step in range(12):
+        if step % 3 == 2:
+            gs.stepBoids(A, knn, bins, p, 1)
+            orc.boids_step(ref, p)
+            continue
+        gs.steerForSeek(A, target, 2.0, 0, mode)
+        orc.steer_seek(ref, target, 2.0, 0, mode)
+        _separate_calls_step(gs, A, knn, bins, p)
+        orc.wrap_world(ref, p.wrap_shape, p.wrap_center, p.wrap_size)
+        pos = np.stack([ref.arrays["px"], ref.arrays["py"], ref.arrays["pz"]], axis=1).astype(np.float32)
+        idx, _ = orc.knn_binned(g, pos, pos, p.k, True)
+        for which, t in ((orc.FLOCK_SEPARATION, p.separation), (orc.FLOCK_ALIGNMENT, p.alignment),
+                         (orc.FLOCK_COHESION, p.cohesion)):
+            orc.steer_flock(which, ref, idx, ref, t.min_d, t.max_d, t.cos_max, t.weight, t.mask, mode)
+        orc.update(ref, p.dt, mode)
+        assert_state_equal(A.download(), ref, what=f"step {step}: ")
+    for h in (A, bins, knn):
+        h.close()
+
+
 @pytest.mark.parametrize("cells,sr", [(64, 1), (128, 2)])
 def test_boids_box_large_steps(gs, ctx, orc, cells, sr):
     """2^20-agent box flock: 3 steps vs 3 oracle steps, bit-exact."""
