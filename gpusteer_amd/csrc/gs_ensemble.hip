@@ -141,29 +141,37 @@ __global__ __launch_bounds__(kBlock) void e_knn(EnsDev E, uint32_t N, uint32_t k
     GS_FLAT_LDS
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const EnsPop& P = E.pops[E.block_pop[blockIdx.x]];
-    if (s >= P.a0 + P.n) return;
-    const float4 q = spos[s];
+    // Padding lanes stay through the searches (the wavefront completes its
+    // lanes' inexact queries together) and return before the small-set part.
+    const bool active = s < P.a0 + P.n;
+    const float4 q = active ? spos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     {
         TopK<8> h;
         h.init(k);
-        const float thr = out.seed_self ? seed_bound<8>(out.seed_self, k, N, new_to_old[s], N, old_to_new, q.x, q.y,
-                                                        q.z, spos)
-                                        : INFINITY;
-        knn_search_flat<8>(h, thr, q.x, q.y, q.z, s, P.g, 1, N, spos, start + P.c0, flds);
-        store_idx<8>(h, k, N, s, out.self, out.out_map);
+        const float thr = (out.seed_self && active) ? seed_bound<8>(out.seed_self, k, N, new_to_old[s], N,
+                                                                    old_to_new, q.x, q.y, q.z, spos)
+                                                    : INFINITY;
+        knn_search_flat<8>(h, active, k, thr, q.x, q.y, q.z, s, P.g, N, spos, start + P.c0, flds);
+        if (active) store_idx<8>(h, k, N, s, out.self, out.out_map);
     }
     if (P.kind != GS_SCENARIO_CHOKE) return;
     const EnsPop& O = E.pops[P.other];
-    {
+    if (O.n > 0) {
         TopK<8> h;
         h.init(k);
         const float thr = fminf(E.scens[P.scen].other_cap2,
-                                out.seed_other ? seed_bound<8>(out.seed_other, k, N, new_to_old[s], N, old_to_new,
-                                                               q.x, q.y, q.z, spos)
-                                               : INFINITY);
-        if (O.n > 0) knn_search_flat<8>(h, thr, q.x, q.y, q.z, UINT32_MAX, O.g, 1, N, spos, start + O.c0, flds);
+                                (out.seed_other && active)
+                                    ? seed_bound<8>(out.seed_other, k, N, new_to_old[s], N, old_to_new, q.x, q.y,
+                                                    q.z, spos)
+                                    : INFINITY);
+        knn_search_flat<8>(h, active, k, thr, q.x, q.y, q.z, UINT32_MAX, O.g, N, spos, start + O.c0, flds);
+        if (active) store_idx<8>(h, k, N, s, out.other, out.out_map);
+    } else if (active) {
+        TopK<8> h;
+        h.init(k);
         store_idx<8>(h, k, N, s, out.other, out.out_map);
     }
+    if (!active) return;
     const EnsScen& S = E.scens[P.scen];
     {
         TopK<8> h;

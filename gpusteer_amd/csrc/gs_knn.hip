@@ -40,23 +40,34 @@ __global__ __launch_bounds__(kBlock) void k_knn_binned(uint32_t nA, const float4
                                                        const uint32_t* __restrict__ val_map,
                                                        uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nA) return;
-    const float4 q = apos[s];
     TopK<KMAX> h;
     h.init(k);
-    if (nB > 0) {
-        const float thr = fminf(cap2, seed ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB, val_map,
-                                                              q.x, q.y, q.z, bpos)
-                                           : INFINITY);
-        if constexpr (RF == 1) {
+    if constexpr (RF == 1) {
+        // Every lane stays to the end: the wavefront completes its lanes'
+        // inexact queries together (lanes past nA only help).
+        const bool active = s < nA;
+        const float4 q = active ? apos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (nB > 0) {
+            const float thr =
+                fminf(cap2, (seed && active) ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB, val_map,
+                                                                q.x, q.y, q.z, bpos)
+                                             : INFINITY);
             GS_FLAT_LDS
-            knn_search_flat<KMAX>(h, thr, q.x, q.y, q.z, self ? s : UINT32_MAX, g, R0, nB, bpos, bstart, flds);
-        } else {
+            knn_search_flat<KMAX>(h, active, k, thr, q.x, q.y, q.z, self ? s : UINT32_MAX, g, nB, bpos, bstart, flds);
+        }
+        if (active) h.store(k, nA, s, out_idx, out_dist);
+    } else {
+        if (s >= nA) return;
+        const float4 q = apos[s];
+        if (nB > 0) {
+            const float thr = fminf(cap2, seed ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB,
+                                                                  val_map, q.x, q.y, q.z, bpos)
+                                               : INFINITY);
             GS_DEFER_LDS
             knn_search<KMAX, RF>(h, thr, q.x, q.y, q.z, self ? s : UINT32_MAX, g, R0, bpos, bstart, lbuf);
         }
+        h.store(k, nA, s, out_idx, out_dist);
     }
-    h.store(k, nA, s, out_idx, out_dist);
 }
 
 // Brute force: one lane per query slot, B staged through LDS in tiles of one

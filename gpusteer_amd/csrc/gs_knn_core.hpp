@@ -332,18 +332,24 @@ __device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, f
 constexpr int kFlatBuf = 16;
 constexpr int kFlatRuns = 9;
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
+// One pool of kFlatPool words per lane, strided by kBlock: the deferral
+// buffer, the run list (first slots, then 16-bit lengths), and after the scan
+// the cooperative shells' candidate triples.
+constexpr int kFlatPool = kFlatBuf + kFlatRuns + (kFlatRuns + 1) / 2;
+constexpr int kCoopEntries = kFlatPool / 3;
 struct FlatLds {
+    uint32_t* pool;      // this lane's column
     uint32_t* slot;      // [kFlatBuf][kBlock]: deferred candidate slots
     uint32_t* run_t;     // [kFlatRuns][kBlock]: first slot of a non-empty run
     uint16_t* run_n;     // [kFlatRuns][kBlock]: its length
     uint32_t slot_addr;  // LDS byte address of the lane's first buffer entry
 };
-#define GS_FLAT_LDS                                                                            \
-    __shared__ uint32_t lds_flat_slot[kFlatBuf * kBlock];                                      \
-    __shared__ uint32_t lds_flat_run_t[kFlatRuns * kBlock];                                    \
-    __shared__ uint16_t lds_flat_run_n[kFlatRuns * kBlock];                                    \
-    const FlatLds flds{lds_flat_slot + threadIdx.x, lds_flat_run_t + threadIdx.x, lds_flat_run_n + threadIdx.x, \
-                       (uint32_t)(size_t)(lds_u32*)(lds_flat_slot + threadIdx.x)};
+#define GS_FLAT_LDS                                                                                     \
+    __shared__ uint32_t lds_flat_pool[kFlatPool * kBlock];                                             \
+    const FlatLds flds{lds_flat_pool + threadIdx.x, lds_flat_pool + threadIdx.x,                        \
+                       lds_flat_pool + kFlatBuf * kBlock + threadIdx.x,                                 \
+                       (uint16_t*)(lds_flat_pool + (kFlatBuf + kFlatRuns) * kBlock) + threadIdx.x,      \
+                       (uint32_t)(size_t)(lds_u32*)(lds_flat_pool + threadIdx.x)};
 
 // Buffer resource over the float4 snapshot of B (raw buffer, byte offsets,
 // range-checked to n * 16 bytes).  Flat-path searches need n < 2^27.
@@ -389,11 +395,94 @@ __device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLd
     cnt = 0;
 }
 
-// Requires 1 <= nB < kFlatMaxPoints.
+// ---------------------------------------------------------------------------
+// Cooperative completion of one lane's query by its whole wavefront.
+//
+// A query whose 3 x 3 x 3 box does not prove its list exact (its k-th
+// neighbour lies beyond the box: sparse regions, a lattice finer than the
+// neighbour distance) continues with shells of Chebyshev radius R = 2, 3, ...
+// Done by its lane alone this is a serial walk of ~100 cells with a dependent
+// pair of loads per cell, which holds its whole block for as long.  Instead
+// the wavefront takes its failing queries one at a time: the cells of a shell
+// are dealt round-robin to the 64 lanes, each lane writes the candidates of
+// its cells within the query's current k-th distance to its column of the
+// (then idle) flat-path LDS as (d2 bits, agent index, slot) triples, and the
+// owner lane inserts them into its list.  Shells continue until the k-th best
+// is below the distance to every unscanned cell.  A lane with more than
+// kCoopEntries candidates in one shell sends the owner to the serial walk
+// for the remaining shells (exact either way).  Call with the wavefront
+// converged; `pool` is the calling lane's column of the flat-path LDS.
 template <int KMAX>
-__device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, float thr, float qx, float qy, float qz,
-                                                uint32_t self_slot, const GridDev& g, int R0, uint32_t nB,
-                                                const float4* __restrict__ bpos,
+__device__ void coop_shells(TopK<KMAX>& h, int owner, float qx0, float qy0, float qz0, float thr0,
+                            uint32_t self0, int cx0, int cy0, int cz0, uint32_t* pool, const GridDev& g,
+                            const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const float qx = __shfl(qx0, owner, 64), qy = __shfl(qy0, owner, 64), qz = __shfl(qz0, owner, 64);
+    const float thr = __shfl(thr0, owner, 64);
+    const uint32_t self_slot = (uint32_t)__shfl((int)self0, owner, 64);
+    const int cx = __shfl(cx0, owner, 64), cy = __shfl(cy0, owner, 64), cz = __shfl(cz0, owner, 64);
+    const float grow = 1.0f + 2e-5f;
+    for (int R = 2;; ++R) {
+        const float lim = fminf(thr, __shfl(h.worst(), owner, 64));
+        int cnt = 0;
+        bool overflow = false;
+        const int W = 2 * R + 1;
+        const int ncube = W * W * W;
+        for (int i = lane; i < ncube; i += 64) {
+            const int dx = i / (W * W) - R, dy = (i / W) % W - R, dz = i % W - R;
+            if (max(abs(dx), max(abs(dy), abs(dz))) != R) continue;
+            const int ix = cx + dx, iy = cy + dy, iz = cz + dz;
+            if (ix < 0 || ix >= g.dx || iy < 0 || iy >= g.dy || iz < 0 || iz >= g.dz) continue;
+            const float gx = axis_gap(qx, ix, ix, g.ox, g.hx, g.dx, g.slack);
+            const float gy = axis_gap(qy, iy, iy, g.oy, g.hy, g.dy, g.slack);
+            const float gz = axis_gap(qz, iz, iz, g.oz, g.hz, g.dz, g.slack);
+            if ((gx * gx + gy * gy) + gz * gz > lim * grow) continue;
+            const uint32_t c = cell_index(g, ix, iy, iz);
+            const uint32_t t1 = bstart[c + 1u];
+            for (uint32_t t = bstart[c]; t < t1; ++t) {
+                const float4 p = bpos[t];
+                const float d2 = sqdist4(qx, qy, qz, p);
+                if (d2 <= lim && t != self_slot) {
+                    if (cnt < kCoopEntries) {
+                        pool[(3 * cnt) * kBlock] = __float_as_uint(d2);
+                        pool[(3 * cnt + 1) * kBlock] = id_of(p);
+                        pool[(3 * cnt + 2) * kBlock] = t;
+                        ++cnt;
+                    } else {
+                        overflow = true;
+                    }
+                }
+            }
+        }
+        if (__any(overflow)) {
+            if (lane == owner) shell_search<KMAX>(h, thr, qx, qy, qz, self_slot, g, cx, cy, cz, R, 1, bpos, bstart);
+            return;
+        }
+        for (uint64_t have = __ballot(cnt > 0); have != 0; have &= have - 1) {
+            const int l = __ffsll((unsigned long long)have) - 1;
+            const int c = __shfl(cnt, l, 64);
+            if (lane == owner) {
+                const uint32_t* col = pool + (l - lane);  // lane l's column
+                for (int e = 0; e < c; ++e) {
+                    const uint64_t kk = ((uint64_t)col[(3 * e) * kBlock] << 32) | col[(3 * e + 1) * kBlock];
+                    if (h.accepts(kk)) h.push(kk, col[(3 * e + 2) * kBlock]);
+                }
+            }
+        }
+        bool all;
+        const float dmin = outside_gap(g, cx, cy, cz, R, qx, qy, qz, all);
+        if (all || exact_after(fminf(thr, __shfl(h.worst(), owner, 64)), dmin)) break;
+    }
+}
+
+// Requires 1 <= nB < kFlatMaxPoints and a minimum search radius of at most
+// one cell.  Called by every lane of the wavefront (lanes without a query
+// pass active = false: they scan nothing and only help complete other lanes'
+// queries).
+template <int KMAX>
+__device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint32_t k, float thr, float qx,
+                                                float qy, float qz, uint32_t self_slot, const GridDev& g,
+                                                uint32_t nB, const float4* __restrict__ bpos,
                                                 const uint32_t* __restrict__ bstart, const FlatLds& L) {
     const __amdgpu_buffer_rsrc_t rb = pos_rsrc(bpos, nB);
     // The query's cell only centres the 3 x 3 x 3 box; every bound below is
@@ -448,7 +537,7 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, float thr, float 
             ++M;
         }
     }
-    if (long_run) M = 0;
+    if (long_run || !active) M = 0;
     // A chunk is (t, e): slots t .. min(t + 4, e) - 1 of the current run,
     // t < e; (0, 0) once the stream is exhausted.
     int m = 0;
@@ -518,8 +607,16 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, float thr, float 
     }
     bool all;
     const float dmin = outside_gap(g, cx, cy, cz, 1, qx, qy, qz, all);
-    if (!long_run && R0 <= 1 && (all || exact_after(fminf(thr, h.worst()), dmin))) return;
-    shell_search<KMAX>(h, thr, qx, qy, qz, self_slot, g, cx, cy, cz, long_run ? 0 : 2, R0, bpos, bstart);
+    const bool exact = !active || long_run || all || exact_after(fminf(thr, h.worst()), dmin);
+    for (uint64_t need = __ballot(!exact); need != 0; need &= need - 1) {
+        const int owner = __ffsll((unsigned long long)need) - 1;
+        coop_shells<KMAX>(h, owner, qx, qy, qz, thr, self_slot, cx, cy, cz, L.pool, g, bpos, bstart);
+    }
+    // A cell too full for the run list (more than 65535 points in one run):
+    // the lane scans from its own cell outwards by itself, every lane of a
+    // wavefront in parallel (a cooperative walk would give one lane a whole
+    // cell per query).
+    if (active && long_run) shell_search<KMAX>(h, thr, qx, qy, qz, self_slot, g, cx, cy, cz, 0, 1, bpos, bstart);
 }
 
 }  // namespace gs
