@@ -273,7 +273,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--k", type=int, default=7)
-    ap.add_argument("--cells", type=int, default=0, help="cells per axis; 0 = chosen by the autotuner")
+    ap.add_argument("--cells", default="0",
+                    help="bin lattice: N (N^3 cells) or X,Y,Z; 0 = chosen by the autotuner")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble-scenarios", type=int, default=256)
@@ -306,20 +307,26 @@ def main() -> int:
     import gpusteer_amd as gs
     from gpusteer_amd import autotune, scenarios
 
-    st, p = scenarios.boids_box(n=args.n, k=args.k, cells=args.cells if args.cells > 0 else 64)
+    cl = [int(x) for x in str(args.cells).split(",")]
+    cells_arg = (cl * 3)[:3] if len(cl) == 1 else cl
+    st, p = scenarios.boids_box(n=args.n, k=args.k, cells=64)
+    if cells_arg[0] > 0:
+        p.grid_cells = tuple(cells_arg)
+        p.search_radius = 1 if max(cells_arg[:2]) <= 80 else 2
     n = st.n
     with gs.Context(0) as ctx:
         tuned = None
-        if args.cells <= 0:
+        if cells_arg[0] <= 0:
             (cells, rf), table = autotune.tune_flock(ctx, st, p, k=args.k)
-            p.grid_cells = (cells, cells, cells)
+            p.grid_cells = tuple(cells)
             p.search_radius = rf
-            tuned = {"chosen": {"cells": cells, "column_radius": rf},
-                     "table_ms_per_step": [{"cells": c, "column_radius": r, "ms": m} for c, r, m in table]}
-            print(f"bench: autotune {table} -> {cells}^3, radius {rf}", file=sys.stderr, flush=True)
-        cells = p.grid_cells[0]
-        C = cells ** 3
-        print(f"bench: boids_box n={n} k={args.k} cells={cells}^3 radius={p.search_radius} warmup={args.warmup} "
+            tuned = {"chosen": {"cells": list(cells), "column_radius": rf},
+                     "table_ms_per_step": [{"cells": list(c), "column_radius": r, "ms": m} for c, r, m in table]}
+            print(f"bench: autotune {table} -> {cells}, radius {rf}", file=sys.stderr, flush=True)
+        cells = list(p.grid_cells)
+        C = cells[0] * cells[1] * cells[2]
+        cells_txt = "x".join(str(c) for c in cells)
+        print(f"bench: boids_box n={n} k={args.k} cells={cells_txt} radius={p.search_radius} warmup={args.warmup} "
               f"steps={args.steps}", file=sys.stderr, flush=True)
         A = gs.AgentGroup(ctx, st)
         bins = gs.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
@@ -364,10 +371,10 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"boids_box: {n} agents, 1000^3 box with face wrap, exact KNN k={args.k}, "
-                        f"{cells}^3 bins, SteerLibrary flocking + SimpleVehicle integrate, dt=1/60",
+                        f"{cells_txt} bins, SteerLibrary flocking + SimpleVehicle integrate, dt=1/60",
             "n_agents": n,
             "k": args.k,
-            "cells": [cells] * 3,
+            "cells": cells,
             "column_radius": p.search_radius,
             "autotune": tuned,
             "parallelism": "single GPU",

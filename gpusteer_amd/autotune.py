@@ -1,4 +1,5 @@
-"""Autotuner: picks the bin lattice (cells per axis) and the KNN fast path's
+"""Autotuner: picks the bin lattice (cells along x, y, z -- z may be finer, so
+the z-runs the KNN scans are trimmed more tightly) and the KNN fast path's
 column radius (3 x 3 or 5 x 5 columns) for a flock, and the cell-size factor
 of an ensemble's per-scenario lattices, by timing short runs of the real
 step on the device.  Every candidate computes the same (exact) result; only
@@ -11,18 +12,21 @@ from typing import List, Sequence, Tuple
 
 from . import api
 
-BOIDS_CANDIDATES: Tuple[Tuple[int, int], ...] = ((48, 1), (64, 1), (80, 1), (96, 2), (128, 2))
+Cells = Tuple[int, int, int]
+BOIDS_CANDIDATES: Tuple[Tuple[Cells, int], ...] = (
+    ((48, 48, 48), 1), ((64, 64, 64), 1), ((48, 48, 96), 1), ((64, 64, 128), 1), ((64, 64, 192), 1),
+    ((80, 80, 160), 1), ((96, 96, 96), 2), ((128, 128, 128), 2))
 
 
 def tune_flock(ctx: api.Context, state, params, k: int = 7,
-               candidates: Sequence[Tuple[int, int]] = BOIDS_CANDIDATES, steps: int = 10, warmup: int = 3):
+               candidates: Sequence[Tuple[Cells, int]] = BOIDS_CANDIDATES, steps: int = 10, warmup: int = 3):
     """Times `steps` fused Boids steps (after `warmup`) of the given flock for
-    each (cells per axis, column radius) candidate; returns the fastest
+    each ((cells x, y, z), column radius) candidate; returns the fastest
     candidate and the table of (cells, radius, ms per step)."""
-    table: List[Tuple[int, int, float]] = []
+    table: List[Tuple[Cells, int, float]] = []
     for cells, rf in candidates:
         p = copy.deepcopy(params)
-        p.grid_cells = (cells, cells, cells)
+        p.grid_cells = tuple(cells)
         p.search_radius = rf
         A = api.AgentGroup(ctx, state)
         bins = api.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)

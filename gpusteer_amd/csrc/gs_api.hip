@@ -121,6 +121,9 @@ static GridDev make_grid(const float* o, const float* s, const uint32_t* c) {
     g.ry = (float)g.dy / g.sy;
     g.rz = (float)g.dz / g.sz;
     g.slack = 1e-5f * (std::fabs(g.ox) + std::fabs(g.oy) + std::fabs(g.oz) + g.sx + g.sy + g.sz);
+    // z cells per x (or, for a single x cell, y) cell width, at least 1.
+    const float hxy = g.dx > 1 ? g.hx : g.hy;
+    g.zr = g.dz > 1 ? std::max(1, std::min(8, (int)std::ceil(hxy / g.hz * 0.999f))) : 1;
     return g;
 }
 

@@ -62,6 +62,10 @@ struct GridDev {
     float hx, hy, hz;  // cell size (s/d)
     float rx, ry, rz;  // cells per unit length (d/s), for conservative run bounds
     float slack;       // absolute rounding allowance of the mapping
+    // z cells per ring of the search boxes: the box of ring R spans R cells
+    // each way in x and y and R * zr cells in z, so lattices with finer z
+    // cells (shorter z-runs to trim) keep boxes of about equal extent.
+    int zr;
 };
 
 __device__ __forceinline__ int axis_cell(float p, float o, float s, int d) {

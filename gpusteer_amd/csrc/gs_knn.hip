@@ -17,14 +17,14 @@
 // Binned search replaces KNNBinningV2Kernel (KNNBinningV2Kernel.cu:417-528),
 // which scans a fixed cell radius and is therefore approximate.  Here each
 // query (one lane, queries in sorted/cell order so a wavefront's lanes share
-// cells) scans shells of cells of growing Chebyshev radius, each shell as
-// contiguous z-runs of the sorted arrays, skips runs whose box is farther
-// than the current bound, and stops once the k-th best d2 is below the
-// squared distance to every unscanned cell.  The bound starts at +inf or, when
-// the container holds the exact result of the previous call for the same
-// groups, at the largest current d2 of those k previous neighbours (a valid
-// upper bound on the k-th distance: they are k distinct candidates).  Either
-// way the result is the exact k nearest.
+// cells) scans the box of ring 1 around its cell (3 x 3 columns, or 5 x 5 for
+// fine lattices, each column's z-run trimmed to the bound's sphere), then
+// rings of growing radius while the k-th best d2 is not below the squared
+// distance to every unscanned cell.  The bound starts at +inf or, when the
+// container holds the exact result of the previous call for the same groups,
+// at the largest current d2 of those k previous neighbours (a valid upper
+// bound on the k-th distance: they are k distinct candidates).  Either way
+// the result is the exact k nearest.
 #include "gs_knn_core.hpp"
 
 namespace gs {

@@ -123,18 +123,20 @@ __device__ __forceinline__ float seed_bound(const uint32_t* __restrict__ seed, u
     return m;
 }
 
-// Distance from the query to the nearest cell outside the box of Chebyshev
-// radius R around its cell; `all` when that box covers the whole lattice.
+// Distance from the query to the nearest cell outside the box of ring R
+// around its cell (R cells each way in x and y, R * zr in z); `all` when that
+// box covers the whole lattice.
 __device__ __forceinline__ float outside_gap(const GridDev& g, int cx, int cy, int cz, int R, float qx, float qy,
                                              float qz, bool& all) {
+    const int Z = R * g.zr;
     float dmin = INFINITY;
     all = true;
     if (cx - R > 0) { dmin = fminf(dmin, qx - (g.ox + (float)(cx - R) * g.hx)); all = false; }
     if (cx + R < g.dx - 1) { dmin = fminf(dmin, (g.ox + (float)(cx + R + 1) * g.hx) - qx); all = false; }
     if (cy - R > 0) { dmin = fminf(dmin, qy - (g.oy + (float)(cy - R) * g.hy)); all = false; }
     if (cy + R < g.dy - 1) { dmin = fminf(dmin, (g.oy + (float)(cy + R + 1) * g.hy) - qy); all = false; }
-    if (cz - R > 0) { dmin = fminf(dmin, qz - (g.oz + (float)(cz - R) * g.hz)); all = false; }
-    if (cz + R < g.dz - 1) { dmin = fminf(dmin, (g.oz + (float)(cz + R + 1) * g.hz) - qz); all = false; }
+    if (cz - Z > 0) { dmin = fminf(dmin, qz - (g.oz + (float)(cz - Z) * g.hz)); all = false; }
+    if (cz + Z < g.dz - 1) { dmin = fminf(dmin, (g.oz + (float)(cz + Z + 1) * g.hz) - qz); all = false; }
     return dmin - g.slack;
 }
 __device__ __forceinline__ bool exact_after(float lim, float dmin) {
@@ -156,10 +158,10 @@ __device__ __forceinline__ void flush_deferred(TopK<KMAX>& h, int& cnt, const De
     cnt = 0;
 }
 
-// Generic shells R >= Rstart (rare path): scan shells of cells of growing
-// Chebyshev radius, each shell as contiguous z-runs of storage slots, skip
-// runs farther than the bound, stop once the k-th best is below the distance
-// to every unscanned cell.
+// Generic shells R >= Rstart (rare path): scan the rings between the boxes of
+// ring R - 1 and R (R cells each way in x and y, R * zr in z), each as
+// contiguous z-runs of storage slots, skip runs farther than the bound, stop
+// once the k-th best is below the distance to every unscanned cell.
 template <int KMAX>
 __device__ void shell_search(TopK<KMAX>& h, float thr, float qx, float qy, float qz, uint32_t self_slot,
                              const GridDev& g, int cx, int cy, int cz, int Rstart, int R0,
@@ -175,15 +177,22 @@ __device__ void shell_search(TopK<KMAX>& h, float thr, float qx, float qy, float
                 const bool edge = xedge || (iy == cy - R) || (iy == cy + R);
                 const float gy = axis_gap(qy, iy, iy, g.oy, g.hy, g.dy, g.slack);
                 const float gxy = gx * gx + gy * gy;
+                const int Z = R * g.zr, Zin = (R - 1) * g.zr;
                 for (int run = 0; run < (edge ? 1 : 2); ++run) {
                     int z0, z1;
                     if (edge) {
-                        z0 = max(cz - R, 0);
-                        z1 = min(cz + R, g.dz - 1);
+                        z0 = cz - Z;
+                        z1 = cz + Z;
+                    } else if (run == 0) {
+                        z0 = cz - Z;
+                        z1 = cz - Zin - 1;
                     } else {
-                        z0 = z1 = (run == 0) ? cz - R : cz + R;
-                        if (z0 < 0 || z0 >= g.dz) continue;
+                        z0 = cz + Zin + 1;
+                        z1 = cz + Z;
                     }
+                    z0 = max(z0, 0);
+                    z1 = min(z1, g.dz - 1);
+                    if (z0 > z1) continue;
                     const float gz = axis_gap(qz, z0, z1, g.oz, g.hz, g.dz, g.slack);
                     float lim = fminf(thr, h.worst());
                     if (gxy + gz * gz > lim * grow) continue;
@@ -239,7 +248,7 @@ __device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, f
     const float grow = 1.0f + 2e-5f;
     float lim = fminf(thr, h.worst());
     const float bound = lim * grow;
-    const int zlo = max(cz - RF, 0), zhi = min(cz + RF, g.dz - 1);
+    const int zlo = max(cz - RF * g.zr, 0), zhi = min(cz + RF * g.zr, g.dz - 1);
     // Locate the runs against the initial bound (tighter bounds found later
     // only make the scan do more than needed, never less).
     uint32_t r0[NC], r1[NC];
@@ -398,9 +407,10 @@ __device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLd
 // ---------------------------------------------------------------------------
 // Cooperative completion of one lane's query by its whole wavefront.
 //
-// A query whose 3 x 3 x 3 box does not prove its list exact (its k-th
-// neighbour lies beyond the box: sparse regions, a lattice finer than the
-// neighbour distance) continues with shells of Chebyshev radius R = 2, 3, ...
+// A query whose ring-1 box (3 x 3 columns, zr cells each way in z) does not
+// prove its list exact (its k-th neighbour lies beyond the box: sparse
+// regions, a lattice finer than the neighbour distance) continues with the
+// rings R = 2, 3, ... (boxes of R cells each way in x and y, R * zr in z).
 // Done by its lane alone this is a serial walk of ~100 cells with a dependent
 // pair of loads per cell, which holds its whole block for as long.  Instead
 // the wavefront takes its failing queries one at a time: the cells of a shell
@@ -426,11 +436,13 @@ __device__ void coop_shells(TopK<KMAX>& h, int owner, float qx0, float qy0, floa
         const float lim = fminf(thr, __shfl(h.worst(), owner, 64));
         int cnt = 0;
         bool overflow = false;
-        const int W = 2 * R + 1;
-        const int ncube = W * W * W;
-        for (int i = lane; i < ncube; i += 64) {
-            const int dx = i / (W * W) - R, dy = (i / W) % W - R, dz = i % W - R;
-            if (max(abs(dx), max(abs(dy), abs(dz))) != R) continue;
+        // Ring R: the box of R cells each way in x and y and R * zr in z,
+        // less the box of ring R - 1.
+        const int W = 2 * R + 1, Z = R * g.zr, Zin = (R - 1) * g.zr, Wz = 2 * Z + 1;
+        const int nbox = W * W * Wz;
+        for (int i = lane; i < nbox; i += 64) {
+            const int dx = i / (W * Wz) - R, dy = (i / Wz) % W - R, dz = i % Wz - Z;
+            if (abs(dx) != R && abs(dy) != R && abs(dz) <= Zin) continue;
             const int ix = cx + dx, iy = cy + dy, iz = cz + dz;
             if (ix < 0 || ix >= g.dx || iy < 0 || iy >= g.dy || iz < 0 || iz >= g.dz) continue;
             const float gx = axis_gap(qx, ix, ix, g.ox, g.hx, g.dx, g.slack);
@@ -485,7 +497,7 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint
                                                 uint32_t nB, const float4* __restrict__ bpos,
                                                 const uint32_t* __restrict__ bstart, const FlatLds& L) {
     const __amdgpu_buffer_rsrc_t rb = pos_rsrc(bpos, nB);
-    // The query's cell only centres the 3 x 3 x 3 box; every bound below is
+    // The query's cell only centres the ring-1 box; every bound below is
     // geometric with the lattice slack, and the exactness test falls back to
     // the shell search, so the cheap mapping is exact in effect.
     const int cx = axis_cell_bound(qx, g.ox, g.rx, g.dx);
@@ -494,7 +506,7 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint
     const float grow = 1.0f + 2e-5f;
     float lim = fminf(thr, h.worst());
     const float bound = lim * grow;
-    const int zlo = max(cz - 1, 0), zhi = min(cz + 1, g.dz - 1);
+    const int zlo = max(cz - g.zr, 0), zhi = min(cz + g.zr, g.dz - 1);
     uint32_t r0[kFlatRuns], r1[kFlatRuns];
 #pragma unroll
     for (int col = 0; col < kFlatRuns; ++col) {

@@ -45,6 +45,11 @@ CASES = [
     ("c0-boids-k7-r2", 4096, 7, (16, 16, 16), 2),
     ("fine-32-k16-r2", 4096, 16, (32, 32, 32), 2),
     ("fine-64-r3", 3000, 7, (64, 64, 64), 3),
+    # finer z cells: boxes of R cells in x, y and R * zr in z
+    ("aniso-z2", 4096, 7, (16, 16, 32), 1),
+    ("aniso-z3-k16", 4096, 16, (16, 16, 48), 1),
+    ("aniso-fine-z2", 3000, 9, (48, 48, 96), 1),
+    ("aniso-z2-r2", 4096, 7, (32, 32, 64), 2),
 ]
 
 

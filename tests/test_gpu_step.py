@@ -132,10 +132,11 @@ def test_steering_pending_across_bin_build(gs, ctx, orc, mode):
         h.close()
 
 
-@pytest.mark.parametrize("cells,sr", [(64, 1), (128, 2)])
+@pytest.mark.parametrize("cells,sr", [((64, 64, 64), 1), ((64, 64, 128), 1), ((128, 128, 128), 2)])
 def test_boids_box_large_steps(gs, ctx, orc, cells, sr):
     """2^20-agent box flock: 3 steps vs 3 oracle steps, bit-exact."""
-    st, p = scenarios.boids_box(n=1 << 20, cells=cells)
+    st, p = scenarios.boids_box(n=1 << 20)
+    p.grid_cells = cells
     p.search_radius = sr
     ref = st.copy()
     A = gs.AgentGroup(ctx, st)
