@@ -886,7 +886,8 @@ static void boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const
                             hipEvent_t* ev) {
     hipStream_t st = A->ctx->stream;
     if (ev) GS_HIP(hipEventRecord(ev[0], st));
-    launch_bins_build(A, bins, p->wrap_shape, p->wrap_center, p->wrap_size);
+    const bool lazy = flock_step_lazy_ok(A);
+    launch_bins_build(A, bins, p->wrap_shape, p->wrap_center, p->wrap_size, lazy);
     after_reorder(A);
     if (ev) GS_HIP(hipEventRecord(ev[1], st));
     const Seed sd = seed_for(knn, A, A, p->knn_flags);
@@ -894,9 +895,7 @@ static void boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const
     if (A->n) knn->cur ^= 1;
     mark_result(knn, A, A);
     if (ev) GS_HIP(hipEventRecord(ev[2], st));
-    launch_flock_update(A, knn, p);
-    if (A->n)
-        for (int j = 0; j < 6; ++j) A->flip(j);
+    launch_flock_update(A, knn, p, lazy);
     if (ev) GS_HIP(hipEventRecord(ev[3], st));
 }
 

@@ -153,29 +153,18 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(uint32_t C, const uint32_
     if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-// Pass 2b: one block scans the tile sums in place (exclusive) and writes the
-// grand total to start[C].
-__global__ __launch_bounds__(kBlock) void k_scan_partials(uint32_t nparts, uint32_t* __restrict__ partial,
-                                                          uint32_t* __restrict__ start, uint32_t C) {
-    __shared__ uint32_t lds[kBlock / 64];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < nparts; base += kBlock) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < nparts ? partial[i] : 0u;
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan(v, lds, total);
-        if (i < nparts) partial[i] = carry + ex;
-        carry += total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) start[C] = carry;
-}
-
-// Pass 2c: add the tile offsets and clear the counts for the next build.
+// Pass 2b: every block sums the tile totals before its own tile (the tile
+// count is small: C / 2048), adds that offset to its tile and clears the
+// counts for the next build; the last block writes the grand total to
+// start[C].
 __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t C, const uint32_t* __restrict__ partial,
                                                      uint32_t* __restrict__ start, uint32_t* __restrict__ count) {
+    __shared__ uint32_t lds[kBlock / 64];
+    uint32_t part = 0;
+    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kBlock) part += partial[i];
+    uint32_t add;
+    (void)block_exclusive_scan(part, lds, add);
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-    const uint32_t add = partial[blockIdx.x];
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         const uint64_t c = base + (uint64_t)j * kBlock + threadIdx.x;
@@ -184,6 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t C, const uint32_t*
             count[c] = 0u;
         }
     }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[C] = add + partial[blockIdx.x];
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(uint32_t n, const uint32_t* __restrict__ key,
@@ -213,7 +203,8 @@ __global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* 
                                                     uint32_t* __restrict__ new_to_old,
                                                     uint32_t* __restrict__ old_to_new,
                                                     const uint32_t* __restrict__ bits, uint32_t* __restrict__ bits_new,
-                                                    FieldSet fs, float4* __restrict__ spos) {
+                                                    FieldSet fs, float4* __restrict__ spos,
+                                                    float4* __restrict__ sfwd) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t s = perm_tmp[t];
@@ -234,8 +225,10 @@ __global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* 
 #pragma unroll
     for (int f = 0; f < kFields; ++f)
         if (f < fs.nf) fs.dst[f][d] = fs.src[f][s];
-    // fields 0..2 (positions) are always first in the set
+    // fields 0..2 (positions) are always first in the set; a lazy build also
+    // snapshots forward and speed (fields 3..6, passed after them)
     spos[d] = make_float4(fs.src[0][s], fs.src[1][s], fs.src[2][s], __uint_as_float(id));
+    if (sfwd) sfwd[d] = make_float4(fs.src[3][s], fs.src[4][s], fs.src[5][s], fs.src[6][s]);
 }
 
 void launch_scan_counts(uint32_t* count, uint32_t* start, uint32_t C, hipStream_t st) {
@@ -243,8 +236,6 @@ void launch_scan_counts(uint32_t* count, uint32_t* start, uint32_t C, hipStream_
     uint32_t* partial = start + C + 1;
     k_scan_tiles<<<ntiles, kBlock, 0, st>>>(C, count, start, partial);
     GS_LAUNCHED("k_scan_tiles");
-    k_scan_partials<<<1, kBlock, 0, st>>>(ntiles, partial, start, C);
-    GS_LAUNCHED("k_scan_partials");
     k_scan_add<<<ntiles, kBlock, 0, st>>>(C, partial, start, count);
     GS_LAUNCHED("k_scan_add");
 }
@@ -252,16 +243,21 @@ void launch_scan_counts(uint32_t* count, uint32_t* start, uint32_t C, hipStream_
 void launch_scatter_reorder(uint32_t n, const uint32_t* key, const uint32_t* rank, const uint32_t* start,
                             const uint32_t* ids, uint32_t* ids_new, uint32_t* perm_tmp, uint32_t* id_tmp,
                             uint32_t* new_to_old, uint32_t* old_to_new, const uint32_t* bits, uint32_t* bits_new,
-                            const FieldSet& fs, float4* spos, hipStream_t st) {
+                            const FieldSet& fs, float4* spos, hipStream_t st, float4* sfwd) {
     if (n == 0) return;
     k_scatter<<<blocks_for(n), kBlock, 0, st>>>(n, key, rank, start, ids, perm_tmp, id_tmp);
     GS_LAUNCHED("k_scatter");
     k_reorder<<<blocks_for(n), kBlock, 0, st>>>(n, perm_tmp, id_tmp, key, start, ids, ids_new, new_to_old,
-                                                old_to_new, bits, bits_new, fs, spos);
+                                                old_to_new, bits, bits_new, fs, spos, sfwd);
     GS_LAUNCHED("k_reorder");
 }
 
-void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws) {
+// lazy (fused Boids step only): move nothing but the snapshots (positions,
+// and forward + speed into sfwd), the agent indices and the maps; every
+// other field stays in the previous order until the flock kernel, which reads
+// it through new_to_old and writes the step's results in the new order.
+void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws,
+                       bool lazy) {
     const uint32_t n = g->n;
     const uint32_t C = b->cells;
     hipStream_t st = g->ctx->stream;
@@ -299,7 +295,7 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
         // Fields holding one value in every slot are unchanged by any
         // permutation and stay where they are (positions always move: the
         // snapshot is built from them).
-        const uint32_t moved = g->live & ~(g->inv & ~kFPos);
+        const uint32_t moved = lazy ? 0u : g->live & ~(g->inv & ~kFPos);
         for (int f = 0; f < kFields; ++f)
             if (moved & (1u << f)) {
                 fs.src[nf] = g->F(f);
@@ -307,10 +303,15 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
                 ++nf;
             }
         fs.nf = nf;
-        const bool has_bits = g->kind == GS_KIND_AGENTS && !g->bits_zero;
+        if (lazy) {
+            for (int f = 0; f < 7; ++f) fs.src[f] = g->F(f);
+            g->sfwd.reserve(n);
+        }
+        const bool has_bits = !lazy && g->kind == GS_KIND_AGENTS && !g->bits_zero;
         launch_scatter_reorder(n, g->key.p, g->rank.p, g->start.p, g->ids(), g->ids_alt(), g->perm_tmp.p,
                                g->id_tmp.p, g->new_to_old.p, g->old_to_new.p, has_bits ? g->bits() : nullptr,
-                               has_bits ? g->bitsb[g->bcur ^ 1].p : nullptr, fs, g->spos.p, st);
+                               has_bits ? g->bitsb[g->bcur ^ 1].p : nullptr, fs, g->spos.p, st,
+                               lazy ? g->sfwd.p : nullptr);
         for (int f = 0; f < kFields; ++f)
             if (moved & (1u << f)) g->flip(f);
         if (has_bits) g->bcur ^= 1;

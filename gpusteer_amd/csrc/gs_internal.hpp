@@ -199,6 +199,9 @@ struct gs_group_s {
     // Positions at the last build in storage order, packed (x, y, z, agent
     // index bits): what KNN searches, like the reference's positionSorted.
     gs::DevBuf<float4> spos;
+    // (forward, speed) at the last lazy build, in storage order (fused step
+    // only; see launch_bins_build).
+    gs::DevBuf<float4> sfwd;
     gs::DevBuf<uint32_t> flag;  // small device scratch (finite check)
 
     float* F(int j) const { return fb[j][fcur[j]].p; }
@@ -282,14 +285,15 @@ void launch_scan_counts(uint32_t* count, uint32_t* start, uint32_t C, hipStream_
 void launch_scatter_reorder(uint32_t n, const uint32_t* key, const uint32_t* rank, const uint32_t* start,
                             const uint32_t* ids, uint32_t* ids_new, uint32_t* perm_tmp, uint32_t* id_tmp,
                             uint32_t* new_to_old, uint32_t* old_to_new, const uint32_t* bits, uint32_t* bits_new,
-                            const FieldSet& fs, float4* spos, hipStream_t st);
+                            const FieldSet& fs, float4* spos, hipStream_t st, float4* sfwd = nullptr);
 
 // Host-side launchers (one per operation) implemented in the .hip files.
 void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t st);
 // Bin build + reorder of g into cell order (optionally wrapping positions
 // first); also writes the position snapshot spos.
-void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws);
+void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws,
+                       bool lazy = false);
 // KNN into out's alternate buffer (caller flips).  row_map / val_map translate
 // a seed held at the previous storage order of A / B (NULL: same order).
 // cap2: only points with d2 <= cap2 qualify (+inf: no cap).
@@ -315,7 +319,10 @@ void launch_check_finite(gs_group_s* A, uint32_t* first_bad_slot_host);
 // Flocking + integrate of the Boids step from a current KNN result of A with
 // itself; new positions / forwards go to the alternate field buffers (caller
 // flips fields 0..5).
-void launch_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p);
+void launch_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, bool lazy);
+// Whether the fused step may build lazily (steering and bits zero, per-agent
+// constants uniform).
+bool flock_step_lazy_ok(const gs_group_s* A);
 // Inverts the id map into slot_of_id.
 void launch_invert_ids(gs_group_s* g);
 }  // namespace gs

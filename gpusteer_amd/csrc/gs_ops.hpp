@@ -90,14 +90,16 @@ __device__ __forceinline__ void flock_core(f3 pos, f3 fwd, float radius, int m, 
 // positions (and forward vectors when want_fwd) into register arrays, every
 // level of loads in flight together.  Returns m, the number of neighbours
 // before the first pad (>= nB).  Positions come from the packed snapshot when
-// bpos is given, else from the SoA arrays bx/by/bz.
+// bpos is given, else from the SoA arrays bx/by/bz; forward vectors from the
+// packed (forward, speed) snapshot when bfwd is given, else from bfx/bfy/bfz.
 template <int KMAX>
 __device__ __forceinline__ int gather_neighbours(const uint32_t* __restrict__ knn, uint32_t k, uint32_t nA,
                                                  uint32_t row, uint32_t nB, const float4* __restrict__ bpos,
                                                  const float* __restrict__ bx, const float* __restrict__ by,
                                                  const float* __restrict__ bz, const float* __restrict__ bfx,
                                                  const float* __restrict__ bfy, const float* __restrict__ bfz,
-                                                 bool want_fwd, f3* bp, f3* bf) {
+                                                 bool want_fwd, f3* bp, f3* bf,
+                                                 const float4* __restrict__ bfwd = nullptr) {
     // Loads are unconditional at in-range indices (slots past the first pad
     // reuse slot 0's neighbour) so each level is in flight together; only
     // the first m results are used.
@@ -123,7 +125,15 @@ __device__ __forceinline__ int gather_neighbours(const uint32_t* __restrict__ kn
     }
     if (want_fwd) {
 #pragma unroll
-        for (int j = 0; j < KMAX; ++j) bf[j] = ld3(bfx, bfy, bfz, (j < m) ? v[j] : v[0]);
+        for (int j = 0; j < KMAX; ++j) {
+            const uint32_t b = (j < m) ? v[j] : v[0];
+            if (bfwd) {
+                const float4 f = bfwd[b];
+                bf[j] = mk3(f.x, f.y, f.z);
+            } else {
+                bf[j] = ld3(bfx, bfy, bfz, b);
+            }
+        }
     }
     return m;
 }

@@ -2,12 +2,17 @@
 // (wrapWorld, updateKNNDatabase, findKNearestNeighbors, steerForSeparation /
 // Alignment / Cohesion, updateGroup) as
 //
-//   bin build (gs_bins.hip) with the wrap fused into the key pass; it leaves
-//   the group's state in cell order and the position snapshot in spos;
+//   bin build (gs_bins.hip) with the wrap fused into the key pass; it puts
+//   the position snapshot in spos and, lazily, forward + speed in sfwd;
 //   exact seeded KNN of the group with itself (gs_knn.hip);
 //   k_flock_update: one lane per storage slot -> the three flocking terms
 //   -> integrate -> new positions and forward vectors into the alternate
-//   buffers, the agent's other fields in place.
+//   buffers, the agent's other fields in place (or, lazily, read at the
+//   previous slot and written in the new order).
+//
+// The lazy form applies when steering and bits are zero and the per-agent
+// constants are uniform (they need no moving): the build then moves no
+// state field, and each field crosses memory once per step.
 //
 // Other agents are only ever read through the snapshot (positions) and the
 // current forward buffers, which no lane writes, so the update is Jacobi (all
@@ -27,24 +32,48 @@ struct StepUniforms {
     float v[4];
 };
 
+// Lazy step (the bin build moved only the snapshots): the agent's own
+// fields are read at its previous slot new_to_old[s], forward and speed come
+// from the (forward, speed) snapshot, and every result is written at s into
+// the alternate buffers (acc only when it changes or is not uniform).
+struct LazyIO {
+    const uint32_t* new_to_old;  // NULL: not lazy
+    const float4* sfwd;
+    float *speed, *sx, *sy, *sz, *ux, *uy, *uz, *ax, *ay, *az;  // alternate buffers
+    uint32_t write_acc;
+};
+
 template <int KMAX, int MODE>
-__global__ __launch_bounds__(kBlock) void k_flock_update(uint32_t n, Agents A, StepUniforms U, float* __restrict__ npx,
-                                                         float* __restrict__ npy, float* __restrict__ npz,
-                                                         float* __restrict__ nfx, float* __restrict__ nfy,
-                                                         float* __restrict__ nfz, const float4* __restrict__ spos,
+__global__ __launch_bounds__(kBlock) void k_flock_update(uint32_t n, Agents A, StepUniforms U, LazyIO Z,
+                                                         float* __restrict__ npx, float* __restrict__ npy,
+                                                         float* __restrict__ npz, float* __restrict__ nfx,
+                                                         float* __restrict__ nfy, float* __restrict__ nfz,
+                                                         const float4* __restrict__ spos,
                                                          const uint32_t* __restrict__ knn, uint32_t k, FlockTerm sep,
                                                          FlockTerm ali, FlockTerm coh, float dt) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    const bool lazy = Z.new_to_old != nullptr;
+    // Own fields: at s (state in this order) or at the previous slot o.
+    const uint32_t o = lazy ? Z.new_to_old[s] : s;
     // Step-t snapshot: positions in spos, forward vectors in the current
-    // buffers; new positions / forward vectors go to the alternate buffers.
+    // buffers (or the snapshot); results go to the alternate buffers.
     const float4 q = spos[s];
     const f3 pos = mk3(q.x, q.y, q.z);
-    const f3 fwd = ld3(A.fx, A.fy, A.fz, s);
+    f3 fwd;
+    float speed;
+    if (lazy) {
+        const float4 fs = Z.sfwd[s];
+        fwd = mk3(fs.x, fs.y, fs.z);
+        speed = fs.w;
+    } else {
+        fwd = ld3(A.fx, A.fy, A.fz, s);
+        speed = A.speed[s];
+    }
     uint32_t bits = U.steer_zero ? 0u : A.bits[s];
     f3 bp[KMAX], bf[KMAX];
     const int m = gather_neighbours<KMAX>(knn, k, n, s, n, spos, nullptr, nullptr, nullptr, A.fx, A.fy, A.fz,
-                                          ali.on && !(bits & ali.mask), bp, bf);
+                                          ali.on && !(bits & ali.mask), bp, bf, lazy ? Z.sfwd : nullptr);
     f3 steering = U.steer_zero ? mk3(0.f, 0.f, 0.f) : ld3(A.gx, A.gy, A.gz, s);
     const float radius = (U.has & 1u) ? U.v[0] : A.radius[s];
     const float max_speed = (U.has & 2u) ? U.v[1] : A.max_speed[s];
@@ -53,16 +82,22 @@ __global__ __launch_bounds__(kBlock) void k_flock_update(uint32_t n, Agents A, S
     flock_core<MODE, KMAX>(pos, fwd, radius, m, bp, bf, sep, ali, coh, steering, bits);
     // Integrate (clears steering and bits as updateGroup does).
     f3 p = pos, f = fwd;
-    f3 side = ld3(A.sx, A.sy, A.sz, s), up = ld3(A.ux, A.uy, A.uz, s);
-    f3 sacc = ld3(A.ax, A.ay, A.az, s);
-    float speed = A.speed[s];
+    f3 side = ld3(A.sx, A.sy, A.sz, o), up = ld3(A.ux, A.uy, A.uz, o);
+    f3 sacc = ld3(A.ax, A.ay, A.az, o);
     integrate<MODE>(steering, p, f, speed, sacc, side, up, max_force, max_speed, mass, dt);
     st3(npx, npy, npz, s, p);
     st3(nfx, nfy, nfz, s, f);
-    st3(A.sx, A.sy, A.sz, s, side);
-    st3(A.ux, A.uy, A.uz, s, up);
-    if (MODE == GS_MODE_OPENSTEER) st3(A.ax, A.ay, A.az, s, sacc);
-    A.speed[s] = speed;
+    if (lazy) {
+        st3(Z.sx, Z.sy, Z.sz, s, side);
+        st3(Z.ux, Z.uy, Z.uz, s, up);
+        Z.speed[s] = speed;
+        if (Z.write_acc) st3(Z.ax, Z.ay, Z.az, s, sacc);
+    } else {
+        st3(A.sx, A.sy, A.sz, s, side);
+        st3(A.ux, A.uy, A.uz, s, up);
+        if (MODE == GS_MODE_OPENSTEER) st3(A.ax, A.ay, A.az, s, sacc);
+        A.speed[s] = speed;
+    }
     if (!U.steer_zero) {
         st3(A.gx, A.gy, A.gz, s, mk3(0.f, 0.f, 0.f));
         A.bits[s] = 0u;
@@ -71,34 +106,72 @@ __global__ __launch_bounds__(kBlock) void k_flock_update(uint32_t n, Agents A, S
 
 static FlockTerm fterm(const gs_flock_term& t) { return FlockTerm{t.min_d, t.max_d, t.cos_max, t.weight, t.mask, 1}; }
 
-template <int KMAX>
-static void run_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p) {
-    const uint32_t n = A->n;
-    hipStream_t st = A->ctx->stream;
-    const Agents a = A->dev_agents();
+static StepUniforms step_uniforms(const gs_group_s* A) {
     StepUniforms U{};
-    U.steer_zero = (A->inv & kFSteer) == kFSteer && A->bits_zero && __builtin_bit_cast(uint32_t, A->inv_val[16]) == 0u &&
-                   __builtin_bit_cast(uint32_t, A->inv_val[17]) == 0u && __builtin_bit_cast(uint32_t, A->inv_val[18]) == 0u;
+    U.steer_zero = (A->inv & kFSteer) == kFSteer && A->bits_zero &&
+                   __builtin_bit_cast(uint32_t, A->inv_val[16]) == 0u &&
+                   __builtin_bit_cast(uint32_t, A->inv_val[17]) == 0u &&
+                   __builtin_bit_cast(uint32_t, A->inv_val[18]) == 0u;
     for (int c = 0; c < 4; ++c)
         if (A->inv & (1u << (19 + c))) {
             U.has |= 1u << c;
             U.v[c] = A->inv_val[19 + c];
         }
+    return U;
+}
+
+bool flock_step_lazy_ok(const gs_group_s* A) {
+    const StepUniforms U = step_uniforms(A);
+    return A->kind == GS_KIND_AGENTS && U.steer_zero && U.has == 0xFu;
+}
+
+template <int KMAX>
+static void run_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, bool lazy) {
+    const uint32_t n = A->n;
+    hipStream_t st = A->ctx->stream;
+    const Agents a = A->dev_agents();
+    const StepUniforms U = step_uniforms(A);
+    LazyIO Z{};
+    const bool acc_written = p->mode == GS_MODE_OPENSTEER || !(A->inv & (0x7u << 7));
+    if (lazy) {
+        Z.new_to_old = A->new_to_old.p;
+        Z.sfwd = A->sfwd.p;
+        Z.speed = A->Falt(6);
+        Z.sx = A->Falt(10);
+        Z.sy = A->Falt(11);
+        Z.sz = A->Falt(12);
+        Z.ux = A->Falt(13);
+        Z.uy = A->Falt(14);
+        Z.uz = A->Falt(15);
+        Z.ax = A->Falt(7);
+        Z.ay = A->Falt(8);
+        Z.az = A->Falt(9);
+        Z.write_acc = acc_written ? 1u : 0u;
+    }
 #define GS_FU(MODE)                                                                                                  \
     k_flock_update<KMAX, MODE><<<blocks_for(n), kBlock, 0, st>>>(                                                    \
-        n, a, U, A->Falt(0), A->Falt(1), A->Falt(2), A->Falt(3), A->Falt(4), A->Falt(5), A->spos.p, knn->idx(), knn->k, \
-        fterm(p->separation), fterm(p->alignment), fterm(p->cohesion), p->dt)
+        n, a, U, Z, A->Falt(0), A->Falt(1), A->Falt(2), A->Falt(3), A->Falt(4), A->Falt(5), A->spos.p, knn->idx(), \
+        knn->k, fterm(p->separation), fterm(p->alignment), fterm(p->cohesion), p->dt)
     if (p->mode == GS_MODE_GPUREF) GS_FU(GS_MODE_GPUREF);
     else GS_FU(GS_MODE_OPENSTEER);
 #undef GS_FU
     GS_LAUNCHED("k_flock_update");
+    // New positions and forward vectors (and, lazily, every other result)
+    // are in the alternate buffers.
+    for (int j = 0; j < 6; ++j) A->flip(j);
+    if (lazy) {
+        A->flip(6);
+        for (int j = 10; j < 16; ++j) A->flip(j);
+        if (acc_written)
+            for (int j = 7; j < 10; ++j) A->flip(j);
+    }
 }
 
-void launch_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p) {
+void launch_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, bool lazy) {
     if (A->n == 0) return;
-    if (knn->k <= 8) run_flock_update<8>(A, knn, p);
-    else if (knn->k <= 16) run_flock_update<16>(A, knn, p);
-    else run_flock_update<32>(A, knn, p);
+    if (knn->k <= 8) run_flock_update<8>(A, knn, p, lazy);
+    else if (knn->k <= 16) run_flock_update<16>(A, knn, p, lazy);
+    else run_flock_update<32>(A, knn, p, lazy);
     A->mark_integrated();
 }
 
