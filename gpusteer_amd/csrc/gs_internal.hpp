@@ -68,6 +68,18 @@ struct GridDev {
     int zr;
 };
 
+// Workgroups are dealt to the chip's 8 XCDs round-robin (block b runs on XCD
+// b % 8), and each XCD has its own L2.  Kernels whose neighbouring blocks
+// read the same data (agents in cell order reading nearby cells) use this
+// bijection of [0, nb) instead of blockIdx.x, so each XCD takes one
+// contiguous range of the work and the shared neighbourhoods stay in its L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t x = b % kXcds, i = b / kXcds;
+    const uint32_t q = nb / kXcds, r = nb % kXcds;  // XCD x runs q + (x < r) blocks
+    return x * q + min(x, r) + i;
+}
+
 __device__ __forceinline__ int axis_cell(float p, float o, float s, int d) {
     if (!(p >= o)) return 0;
     if (p >= o + s) return d - 1;

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kBlock) void k_knn_binned(uint32_t nA, const float4
                                                        const uint32_t* __restrict__ row_map,
                                                        const uint32_t* __restrict__ val_map,
                                                        uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     TopK<KMAX> h;
     h.init(k);
     if constexpr (RF == 1) {

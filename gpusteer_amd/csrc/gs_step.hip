@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kBlock) void k_flock_update(uint32_t n, Agents A, S
                                                          const float4* __restrict__ spos,
                                                          const uint32_t* __restrict__ knn, uint32_t k, FlockTerm sep,
                                                          FlockTerm ali, FlockTerm coh, float dt) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const bool lazy = Z.new_to_old != nullptr;
     // Own fields: at s (state in this order) or at the previous slot o.
