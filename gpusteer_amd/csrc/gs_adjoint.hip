@@ -424,8 +424,9 @@ __global__ __launch_bounds__(kBlock) void c4_forward(CalibDev D, CState X, CStat
                                                      const uint32_t* __restrict__ kother,
                                                      const uint32_t* __restrict__ kw, const uint32_t* __restrict__ ko,
                                                      double* __restrict__ loss) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const EnsPop& P = D.pops[D.block_pop[blockIdx.x]];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
+    const EnsPop& P = D.pops[D.block_pop[blk]];
     float lp = 0.f;
     if (i < P.a0 + P.n) {
         const EnsScen& S = D.scens[P.scen];
@@ -449,8 +450,9 @@ __global__ __launch_bounds__(kBlock) void c4_forward(CalibDev D, CState X, CStat
 // Terminal term: loss += mean over agents of |p - goal|^2 (per scenario, / S);
 // writes the adjoint seed dL/dp_T and zeroes the other adjoint components.
 __global__ __launch_bounds__(kBlock) void c4_terminal(CalibDev D, CState X, CAdj A, double* __restrict__ loss) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const EnsPop& P = D.pops[D.block_pop[blockIdx.x]];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
+    const EnsPop& P = D.pops[D.block_pop[blk]];
     float l = 0.f;
 #pragma unroll
     for (int c = 0; c < kAdj; ++c) A.a[c][i] = 0.f;
@@ -483,8 +485,9 @@ __global__ __launch_bounds__(kBlock) void c4_backward(CalibDev D, CState X, cons
                                                       const uint32_t* __restrict__ kw,
                                                       const uint32_t* __restrict__ ko, CAdj own_in, CAdj acc_in,
                                                       CAdj own_out, CAdj acc_out, double* __restrict__ theta_grad) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const EnsPop& P = D.pops[D.block_pop[blockIdx.x]];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
+    const EnsPop& P = D.pops[D.block_pop[blk]];
     float tg[kTheta];
 #pragma unroll
     for (int c = 0; c < kTheta; ++c) tg[c] = 0.f;

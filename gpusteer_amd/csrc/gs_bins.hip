@@ -153,17 +153,41 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(uint32_t C, const uint32_
     if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-// Pass 2b: every block sums the tile totals before its own tile (the tile
-// count is small: C / 2048), adds that offset to its tile and clears the
-// counts for the next build; the last block writes the grand total to
-// start[C].
-__global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t C, const uint32_t* __restrict__ partial,
-                                                     uint32_t* __restrict__ start, uint32_t* __restrict__ count) {
+// Pass 2b (many tiles): one block scans the tile sums in place (exclusive)
+// and writes the grand total to start[C].
+__global__ __launch_bounds__(kBlock) void k_scan_partials(uint32_t nparts, uint32_t* __restrict__ partial,
+                                                          uint32_t* __restrict__ start, uint32_t C) {
     __shared__ uint32_t lds[kBlock / 64];
-    uint32_t part = 0;
-    for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kBlock) part += partial[i];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nparts; base += kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nparts ? partial[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, lds, total);
+        if (i < nparts) partial[i] = carry + ex;
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[C] = carry;
+}
+
+// Pass 2c: add the tile offsets and clear the counts for the next build.
+// With few tiles (fused = 1) each block sums the tile totals before its own
+// tile itself, which saves the launch of pass 2b; the last block then writes
+// the grand total to start[C].
+constexpr uint32_t kScanFusedTiles = 1024;
+__global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t C, const uint32_t* __restrict__ partial,
+                                                     uint32_t* __restrict__ start, uint32_t* __restrict__ count,
+                                                     int fused) {
+    __shared__ uint32_t lds[kBlock / 64];
     uint32_t add;
-    (void)block_exclusive_scan(part, lds, add);
+    if (fused) {
+        uint32_t part = 0;
+        for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kBlock) part += partial[i];
+        (void)block_exclusive_scan(part, lds, add);
+    } else {
+        add = partial[blockIdx.x];
+    }
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
@@ -173,7 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t C, const uint32_t*
             count[c] = 0u;
         }
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[C] = add + partial[blockIdx.x];
+    if (fused && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[C] = add + partial[blockIdx.x];
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(uint32_t n, const uint32_t* __restrict__ key,
@@ -236,7 +260,12 @@ void launch_scan_counts(uint32_t* count, uint32_t* start, uint32_t C, hipStream_
     uint32_t* partial = start + C + 1;
     k_scan_tiles<<<ntiles, kBlock, 0, st>>>(C, count, start, partial);
     GS_LAUNCHED("k_scan_tiles");
-    k_scan_add<<<ntiles, kBlock, 0, st>>>(C, partial, start, count);
+    const int fused = ntiles <= kScanFusedTiles ? 1 : 0;
+    if (!fused) {
+        k_scan_partials<<<1, kBlock, 0, st>>>(ntiles, partial, start, C);
+        GS_LAUNCHED("k_scan_partials");
+    }
+    k_scan_add<<<ntiles, kBlock, 0, st>>>(C, partial, start, count, fused);
     GS_LAUNCHED("k_scan_add");
 }
 

@@ -44,8 +44,9 @@ struct EnsDev {
 __global__ __launch_bounds__(kBlock) void e_bin_key(EnsDev E, float* __restrict__ px, float* __restrict__ py,
                                                     float* __restrict__ pz, uint32_t* __restrict__ key,
                                                     uint32_t* __restrict__ rank, uint32_t* __restrict__ count) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    const EnsPop& P = E.pops[E.block_pop[blockIdx.x]];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t s = blk * blockDim.x + threadIdx.x;
+    const EnsPop& P = E.pops[E.block_pop[blk]];
     const bool agent = s < P.a0 + P.n;
     uint32_t c = P.c0 + P.cells;  // padding slots: the population's extra cell
     if (agent) {
@@ -139,8 +140,9 @@ __global__ __launch_bounds__(kBlock) void e_knn(EnsDev E, uint32_t N, uint32_t k
                                                 const float* __restrict__ wmz, const float* __restrict__ ox,
                                                 const float* __restrict__ oy, const float* __restrict__ oz) {
     GS_FLAT_LDS
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    const EnsPop& P = E.pops[E.block_pop[blockIdx.x]];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t s = blk * blockDim.x + threadIdx.x;
+    const EnsPop& P = E.pops[E.block_pop[blk]];
     // Padding lanes stay through the searches (the wavefront completes its
     // lanes' inexact queries together) and return before the small-set part.
     const bool active = s < P.a0 + P.n;
@@ -223,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void e_boids(EnsDev E, const uint32_t* __re
                                                   float* __restrict__ nfx, float* __restrict__ nfy,
                                                   float* __restrict__ nfz, const float4* __restrict__ spos,
                                                   const uint32_t* __restrict__ knn) {
-    const uint32_t b = blocks[blockIdx.x];
+    const uint32_t b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const uint32_t s = b * kBlock + threadIdx.x;
     const EnsPop& P = E.pops[E.block_pop[b]];
     if (s >= P.a0 + P.n) return;
@@ -239,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void e_choke_a(EnsDev E, const uint32_t* __
                                                     const uint32_t* __restrict__ kobs, gs_walls W,
                                                     const float* __restrict__ ox, const float* __restrict__ oy,
                                                     const float* __restrict__ oz, const float* __restrict__ orad) {
-    const uint32_t b = blocks[blockIdx.x];
+    const uint32_t b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const uint32_t s = b * kBlock + threadIdx.x;
     const EnsPop& P = E.pops[E.block_pop[b]];
     if (s >= P.a0 + P.n) return;
@@ -291,7 +293,7 @@ __device__ __forceinline__ void choke_b_one(const EnsScen& S, int group, uint32_
 __global__ __launch_bounds__(kBlock) void e_choke_b(EnsDev E, const uint32_t* __restrict__ blocks, uint32_t N,
                                                     uint32_t k, Agents A, const uint32_t* __restrict__ kself,
                                                     const uint32_t* __restrict__ kother) {
-    const uint32_t b = blocks[blockIdx.x];
+    const uint32_t b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const uint32_t s = b * kBlock + threadIdx.x;
     const EnsPop& P = E.pops[E.block_pop[b]];
     if (s >= P.a0 + P.n) return;
@@ -324,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void e_choke_update(EnsDev E, const uint32_
                                                          float* __restrict__ npx, float* __restrict__ npy,
                                                          float* __restrict__ npz, float* __restrict__ nfx,
                                                          float* __restrict__ nfy, float* __restrict__ nfz) {
-    const uint32_t b = blocks[blockIdx.x];
+    const uint32_t b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const uint32_t s = b * kBlock + threadIdx.x;
     const EnsPop& P = E.pops[E.block_pop[b]];
     if (s >= P.a0 + P.n) return;

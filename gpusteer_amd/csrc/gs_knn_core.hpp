@@ -422,6 +422,10 @@ __device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLd
 // kCoopEntries candidates in one shell sends the owner to the serial walk
 // for the remaining shells (exact either way).  Call with the wavefront
 // converged; `pool` is the calling lane's column of the flat-path LDS.
+// Up to this many inexact queries per wavefront are completed cooperatively,
+// one after another; more go to the lanes' own parallel walks.
+constexpr int kCoopMaxQueries = 4;
+
 template <int KMAX>
 __device__ void coop_shells(TopK<KMAX>& h, int owner, float qx0, float qy0, float qz0, float thr0,
                             uint32_t self0, int cx0, int cy0, int cz0, uint32_t* pool, const GridDev& g,
@@ -620,9 +624,17 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint
     bool all;
     const float dmin = outside_gap(g, cx, cy, cz, 1, qx, qy, qz, all);
     const bool exact = !active || long_run || all || exact_after(fminf(thr, h.worst()), dmin);
-    for (uint64_t need = __ballot(!exact); need != 0; need &= need - 1) {
-        const int owner = __ffsll((unsigned long long)need) - 1;
-        coop_shells<KMAX>(h, owner, qx, qy, qz, thr, self_slot, cx, cy, cz, L.pool, g, bpos, bstart);
+    const uint64_t need = __ballot(!exact);
+    if (__popcll(need) <= kCoopMaxQueries) {
+        for (uint64_t left = need; left != 0; left &= left - 1) {
+            const int owner = __ffsll((unsigned long long)left) - 1;
+            coop_shells<KMAX>(h, owner, qx, qy, qz, thr, self_slot, cx, cy, cz, L.pool, g, bpos, bstart);
+        }
+    } else if (!exact) {
+        // Many inexact queries in the wavefront (a lattice finer than the
+        // neighbour distance or the cap): every lane walks its own rings, in
+        // parallel.
+        shell_search<KMAX>(h, thr, qx, qy, qz, self_slot, g, cx, cy, cz, 2, 1, bpos, bstart);
     }
     // A cell too full for the run list (more than 65535 points in one run):
     // the lane scans from its own cell outwards by itself, every lane of a
