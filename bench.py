@@ -57,6 +57,23 @@ def knn_traffic_from_profile():
     return float(m.group(1)) if m else None
 
 
+SQ_SUMMARY = ROOT / "profiles" / "k_knn_binned_boids_box_1M_k7_sq_counters.md"
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies its SIMD for
+# 4 cycles; 2.4 GHz peak engine clock.
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
+
+
+def knn_valu_from_profile():
+    """VALU instructions per launch of the KNN kernel (committed rocprofv3
+    --pmc SQ_INSTS_VALU pass), or None when absent."""
+    import re
+
+    if not SQ_SUMMARY.exists():
+        return None
+    m = re.search(r"^valu_instructions_per_launch\s*=\s*([0-9.eE+]+)", SQ_SUMMARY.read_text(), re.M)
+    return float(m.group(1)) if m else None
+
+
 def _hip_runtime():
     """The HIP runtime libgpusteer is linked against (already loaded by the
     package import), driven through ctypes for the copy measurements so that
@@ -387,6 +404,11 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": knn_traffic_from_profile(),
+            # The kernel is bound by VALU issue, not bytes: the profiled VALU
+            # instruction count per launch over the chip's issue rate, against
+            # the live kernel time.
+            "valu_issue_frac": (knn_valu_from_profile() / VALU_ISSUE_PER_S / (knn_ms * 1e-3))
+            if knn_valu_from_profile() else None,
             "bytes_per_launch": n * b_knn,
             "bytes_per_agent": b_knn,
             "achievable_copy_gbs": copy_gbs,

@@ -13,19 +13,21 @@ struct FlockTerm {
 
 // Boid neighbourhood predicate: OPENSTEER = SteerLibrary::inBoidNeighborhood
 // (src/SteerLibrary.cpp:506-544, squared distances, min = 3*radius when
-// min_d < 0); GPUREF = FlockingCommon.cuh:4-30 (sqrt distances).
+// min_d < 0); GPUREF = FlockingCommon.cuh:4-30 (sqrt distances).  Both test
+// the cone with dot(fwd, offset / sqrtf(d2)); the caller passes d = sqrtf(d2)
+// and that dot product, which do not depend on the term, so a neighbour tested
+// by several terms computes them once (same values, same results).
 template <int MODE>
-__device__ __forceinline__ bool in_hood(f3 offset, float d2, f3 fwd, float radius, const FlockTerm& t) {
+__device__ __forceinline__ bool in_hood_pre(float d2, float d, float cdot, float radius, const FlockTerm& t) {
     if (MODE == GS_MODE_GPUREF) {
-        const float d = sqrtf(d2);
         if (d < t.min_d) return true;
         if (d > t.max_d) return false;
-        return dot3(fwd, offset / d) > t.cos_max;
+        return cdot > t.cos_max;
     }
     const float md = (t.min_d < 0.f) ? radius * 3.f : t.min_d;
     if (d2 < (md * md)) return true;
     if (d2 > (t.max_d * t.max_d)) return false;
-    return dot3(fwd, offset / sqrtf(d2)) > t.cos_max;
+    return cdot > t.cos_max;
 }
 
 __device__ __forceinline__ void apply(f3& steering, uint32_t& bits, f3 f, float w, uint32_t bit) {
@@ -58,15 +60,17 @@ __device__ __forceinline__ void flock_core(f3 pos, f3 fwd, float radius, int m, 
         if (j < m) {
             const f3 off = bp[j] - pos;
             const float d2 = dot3(off, off);
-            if (do_s && in_hood<MODE>(off, d2, fwd, radius, sep)) {
+            const float d = sqrtf(d2);
+            const float cdot = dot3(fwd, off / d);
+            if (do_s && in_hood_pre<MODE>(d2, d, cdot, radius, sep)) {
                 ss = ss + off / -d2;
                 ++ns;
             }
-            if (do_a && in_hood<MODE>(off, d2, fwd, radius, ali)) {
+            if (do_a && in_hood_pre<MODE>(d2, d, cdot, radius, ali)) {
                 sa = sa + bf[j];
                 ++na;
             }
-            if (do_c && in_hood<MODE>(off, d2, fwd, radius, coh)) {
+            if (do_c && in_hood_pre<MODE>(d2, d, cdot, radius, coh)) {
                 sc = sc + bp[j];
                 ++nc;
             }
