@@ -81,3 +81,35 @@ def test_forward_is_repeatable_and_backward_needs_forward(ctx):
     assert a == pytest.approx(b, rel=1e-6)
     cal.close()
     ens.close()
+
+
+def test_autograd_wrapper_matches_backward(ctx):
+    """torch.autograd over the forward / backward pair returns the library's
+    gradient, which points uphill; the wrapper drives a torch optimizer."""
+    import torch
+
+    from gpusteer_amd.autograd import calibration_loss
+
+    cfg = scenarios.CalibConfig(steps=6, checkpoint_every=3)
+    ens = Ensemble(ctx, _specs(2, 160, 60))
+    cal = Calibration(ens, cfg)
+    th0 = scenarios.calib_params()
+    theta = torch.tensor(th0, dtype=torch.float64, requires_grad=True)
+    loss = calibration_loss(cal, theta)
+    loss.backward()
+    cal.forward(th0)
+    ref = cal.backward()
+    np.testing.assert_array_equal(theta.grad.numpy(), ref)
+    assert float(loss.detach()) == cal.forward(th0)
+    # A small step against the gradient lowers the loss (first order).
+    g = ref / np.linalg.norm(ref)
+    assert cal.forward(th0 - 1e-5 * g) < cal.forward(th0)
+    # The wrapper drives a torch optimizer.
+    opt = torch.optim.Adam([theta], lr=0.01)
+    for _ in range(3):
+        opt.zero_grad()
+        calibration_loss(cal, theta).backward()
+        opt.step()
+    assert np.isfinite(float(calibration_loss(cal, theta).detach()))
+    cal.close()
+    ens.close()
