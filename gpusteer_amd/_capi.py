@@ -146,6 +146,57 @@ class GsCalibConfig(C.Structure):
     ]
 
 
+GS_PARAMS_MAX_GROUPS = 32
+GS_PARAMS_MAX_PATH = 64
+
+
+class GsGroupParams(C.Structure):
+    _fields_ = [
+        ("num_agents", C.c_uint32),
+        ("max_speed", C.c_float),
+        ("max_force", C.c_float),
+        ("start_position", C.c_float * 3),
+        ("min_start_radius", C.c_float),
+        ("max_start_radius", C.c_float),
+        ("path_points", C.c_uint32),
+        ("path", (C.c_float * 3) * GS_PARAMS_MAX_PATH),
+        ("path_radius", C.c_float),
+        ("path_cyclic", C.c_uint32),
+        ("body_color", C.c_float * 3),
+    ]
+
+
+class GsWorldParams(C.Structure):
+    _fields_ = [
+        ("dimensions", C.c_float * 3),
+        ("cells", C.c_uint32 * 3),
+        ("obstacle_count", C.c_uint32),
+        ("map_filename", C.c_char * 256),
+    ]
+
+
+SIM_FLOAT_FIELDS = ("max_pursuit_prediction_time", "min_time_to_collision", "min_time_to_obstacle",
+                    "min_time_to_wall", "path_prediction_time", "min_separation_distance",
+                    "max_separation_distance", "min_flocking_distance", "max_flocking_distance",
+                    "cos_max_flocking_angle")
+SIM_WEIGHT_FIELDS = ("weight_alignment", "weight_cohesion", "weight_separation", "weight_pursuit", "weight_seek",
+                     "weight_flee", "weight_evade", "weight_follow_path", "weight_avoid_obstacles",
+                     "weight_avoid_walls", "weight_avoid_neighbors")
+SIM_MASK_FIELDS = ("mask_alignment", "mask_anti_penetration_agents", "mask_cohesion", "mask_separation",
+                   "mask_pursuit", "mask_seek", "mask_flee", "mask_evade", "mask_follow_path",
+                   "mask_avoid_obstacles", "mask_avoid_walls", "mask_avoid_neighbors")
+
+
+class GsSimParams(C.Structure):
+    _fields_ = ([("world", GsWorldParams), ("n_groups", C.c_uint32),
+                 ("groups", GsGroupParams * GS_PARAMS_MAX_GROUPS)]
+                + [(f, C.c_uint32) for f in ("seed", "knn", "kno", "knw", "search_radius")]
+                + [(f, C.c_float) for f in SIM_FLOAT_FIELDS]
+                + [("avoid_close_neighbors", C.c_uint32)]
+                + [(f, C.c_float) for f in SIM_WEIGHT_FIELDS]
+                + [(f, C.c_uint32) for f in SIM_MASK_FIELDS])
+
+
 class GpuSteerError(RuntimeError):
     def __init__(self, status: int, where: str, detail: str):
         self.status = status
@@ -188,6 +239,17 @@ _SIGS = {
     "gs_group_upload_obstacles": (C.c_int, [_h, C.c_uint32, _fp, _fp, _fp, _fp]),
     "gs_group_upload_walls": (C.c_int, [_h, C.c_uint32, C.POINTER(GsWalls)]),
     "gs_group_download_bins": (C.c_int, [_h, _up, _up, _up, _fp, _fp, _fp]),
+    "gs_group_upload_walls_mid": (C.c_int, [_h, C.c_uint32, C.POINTER(GsWalls), _fp, _fp, _fp]),
+    "gs_group_upload_wall_set": (C.c_int, [_h, _h]),
+    "gs_wall_set_load_map": (C.c_int, [C.c_char_p, C.POINTER(_h)]),
+    "gs_wall_set_create": (C.c_int, [C.c_uint32, C.POINTER(GsWalls), C.POINTER(_h)]),
+    "gs_wall_set_split": (C.c_int, [_h, _fp, _up]),
+    "gs_wall_set_split_cells": (C.c_int, [_h, C.c_uint32, _fp, _fp]),
+    "gs_wall_set_size": (C.c_int, [_h, _up]),
+    "gs_wall_set_set_mid": (C.c_int, [_h, _fp, _fp, _fp]),
+    "gs_wall_set_get": (C.c_int, [_h, C.POINTER(GsWalls), _fp, _fp, _fp]),
+    "gs_wall_set_destroy": (C.c_int, [_h]),
+    "gs_sim_params_load": (C.c_int, [C.c_char_p, C.POINTER(GsSimParams)]),
     "gs_group_check_finite": (C.c_int, [_h, _up]),
     "gs_knn_create": (C.c_int, [_h, C.c_uint32, C.c_uint32, C.POINTER(_h)]),
     "gs_knn_destroy": (C.c_int, [_h]),
@@ -217,6 +279,7 @@ _SIGS = {
     "gs_ensemble_upload_agents": (C.c_int, [_h, C.c_uint32, C.c_uint32, C.POINTER(GsAgents)]),
     "gs_ensemble_download_agents": (C.c_int, [_h, C.c_uint32, C.c_uint32, C.POINTER(GsAgents)]),
     "gs_ensemble_upload_walls": (C.c_int, [_h, C.c_uint32, C.c_uint32, C.POINTER(GsWalls)]),
+    "gs_ensemble_upload_walls_mid": (C.c_int, [_h, C.c_uint32, C.c_uint32, C.POINTER(GsWalls), _fp, _fp, _fp]),
     "gs_ensemble_upload_obstacles": (C.c_int, [_h, C.c_uint32, C.c_uint32, _fp, _fp, _fp, _fp]),
     "gs_ensemble_download_knn": (C.c_int, [_h, C.c_uint32, C.c_uint32, _up]),
     "gs_ensemble_step": (C.c_int, [_h, C.c_uint32, C.c_uint32]),

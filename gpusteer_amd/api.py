@@ -184,10 +184,23 @@ class WallGroup(_Group):
         if walls is not None:
             self.upload(walls)
 
-    def upload(self, walls: WallState) -> None:
+    def upload(self, walls: WallState, mid: Optional[np.ndarray] = None) -> None:
+        """Walls with their KNN positions: 0.5f * (start + end) unless `mid`
+        ([n][3]) gives them, as split walls carry (WallGroupData.cu:185-190)."""
         arrs = {f: np.ascontiguousarray(walls.arrays[f], dtype=np.float32) for f in WALL_FIELDS}
         s = capi.GsWalls(**{f: fptr(arrs[f]) for f in WALL_FIELDS})
-        check(capi.lib.gs_group_upload_walls(self.h, walls.n, C.byref(s)), "gs_group_upload_walls")
+        if mid is None:
+            check(capi.lib.gs_group_upload_walls(self.h, walls.n, C.byref(s)), "gs_group_upload_walls")
+            return
+        m = np.asarray(mid, dtype=np.float32).reshape(walls.n, 3)
+        cols = [np.ascontiguousarray(m[:, a]) for a in range(3)]
+        check(capi.lib.gs_group_upload_walls_mid(self.h, walls.n, C.byref(s), *[fptr(c) for c in cols]),
+              "gs_group_upload_walls_mid")
+
+    def upload_wall_set(self, ws) -> None:
+        """The walls of a formats.WallSet (WallGroup::SyncDevice after
+        LoadFromFile / SplitWalls)."""
+        check(capi.lib.gs_group_upload_wall_set(self.h, ws.h), "gs_group_upload_wall_set")
 
 
 class KNNData:

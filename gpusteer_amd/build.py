@@ -76,7 +76,8 @@ def build(verbose: bool = False) -> Path:
 
 
 ROOT = PKG.parent
-CXX_EXAMPLES = {"boids_group_plugin": ROOT / "tests" / "cxx" / "boids_group_plugin.cpp"}
+CXX_EXAMPLES = {"boids_group_plugin": ROOT / "tests" / "cxx" / "boids_group_plugin.cpp",
+                "choke_setup": ROOT / "tests" / "cxx" / "choke_setup.cpp"}
 
 
 def build_cxx_examples() -> None:
@@ -86,7 +87,8 @@ def build_cxx_examples() -> None:
         if not src.exists():
             continue
         out = LIB_DIR / name
-        cmd = [hipcc(), "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(src), f"-L{LIB_DIR}", "-lgpusteer",
+        cmd = [hipcc(), "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-O2",
+               "-ffp-contract=off", "-std=c++17", f"-I{ROOT / 'include'}", str(src), f"-L{LIB_DIR}", "-lgpusteer",
                "-Wl,-rpath,$ORIGIN", "-o", str(out)]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -80,21 +80,6 @@ static gs_group_s* check_group(gs_group g, const char* what, int kind = -1) {
 }
 static void use_device(gs_context ctx) { GS_HIP(hipSetDevice(ctx->device)); }
 
-template <class F> static int guarded(F&& f) {
-    try {
-        f();
-        return GS_OK;
-    } catch (const Error& e) {
-        set_last_error(e.what());
-        return e.status;
-    } catch (const std::bad_alloc&) {
-        set_last_error("host allocation failed");
-        return GS_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception& e) {
-        set_last_error(e.what());
-        return GS_ERR_DEVICE;
-    }
-}
 
 static GridDev make_grid(const float* o, const float* s, const uint32_t* c) {
     for (int a = 0; a < 3; ++a) {
@@ -544,6 +529,28 @@ int gs_group_upload_obstacles(gs_group g, uint32_t n, const float* cx, const flo
     });
 }
 
+// Walls with their KNN positions (midpoints): given, or 0.5f*(start+end)
+// (CUDA/WallGroupData.cu:87-89, WallGroup.h:31).
+static void upload_walls(gs_group_s* g, uint32_t n, const float* const* hw, const float* const* given_mid) {
+    std::vector<float> mid[3];
+    const float* m[3];
+    for (int a = 0; a < 3; ++a) {
+        if (given_mid) {
+            m[a] = given_mid[a];
+        } else {
+            mid[a].resize(n);
+            for (uint32_t i = 0; i < n; ++i) mid[a][i] = 0.5f * (hw[a][i] + hw[3 + a][i]);
+            m[a] = mid[a].data();
+        }
+    }
+    use_device(g->ctx);
+    hipStream_t st = g->ctx->stream;
+    begin_members(g, n);
+    for (int j = 0; j < 9 && n; ++j) h2d(g->F(kWallField + j), hw[j], (size_t)n * 4, st);
+    for (int a = 0; a < 3 && n; ++a) h2d(g->F(a), m[a], (size_t)n * 4, st);
+    GS_HIP(hipStreamSynchronize(st));
+}
+
 int gs_group_upload_walls(gs_group g, uint32_t n, const gs_walls* host) {
     return guarded([&] {
         check_group(g, "group", GS_KIND_WALLS);
@@ -551,19 +558,38 @@ int gs_group_upload_walls(gs_group g, uint32_t n, const gs_walls* host) {
         const float* const* hw = host ? (const float* const*)host : nullptr;
         for (int j = 0; j < 9 && n; ++j)
             if (!hw[j]) fail(GS_ERR_INVALID_ARGUMENT, "wall array is NULL");
-        // The KNN position of a wall is its midpoint, mid = 0.5f*(start+end)
-        // (CUDA/WallGroupData.cu:87-89, WallGroup.h:31).
-        std::vector<float> mid[3];
-        for (int a = 0; a < 3; ++a) {
-            mid[a].resize(n);
-            for (uint32_t i = 0; i < n; ++i) mid[a][i] = 0.5f * (hw[a][i] + hw[3 + a][i]);
-        }
-        use_device(g->ctx);
-        hipStream_t st = g->ctx->stream;
-        begin_members(g, n);
-        for (int j = 0; j < 9 && n; ++j) h2d(g->F(kWallField + j), hw[j], (size_t)n * 4, st);
-        for (int a = 0; a < 3 && n; ++a) h2d(g->F(a), mid[a].data(), (size_t)n * 4, st);
-        GS_HIP(hipStreamSynchronize(st));
+        upload_walls(g, n, hw, nullptr);
+    });
+}
+
+int gs_group_upload_walls_mid(gs_group g, uint32_t n, const gs_walls* host, const float* mid_x, const float* mid_y,
+                              const float* mid_z) {
+    return guarded([&] {
+        check_group(g, "group", GS_KIND_WALLS);
+        if (n > 0 && (!host || !mid_x || !mid_y || !mid_z)) fail(GS_ERR_INVALID_ARGUMENT, "wall arrays are NULL");
+        const float* const* hw = host ? (const float* const*)host : nullptr;
+        for (int j = 0; j < 9 && n; ++j)
+            if (!hw[j]) fail(GS_ERR_INVALID_ARGUMENT, "wall array is NULL");
+        const float* mid[3] = {mid_x, mid_y, mid_z};
+        upload_walls(g, n, hw, mid);
+    });
+}
+
+int gs_group_upload_wall_set(gs_group g, gs_wall_set w) {
+    return guarded([&] {
+        check_group(g, "group", GS_KIND_WALLS);
+        uint32_t n = 0;
+        if (gs_wall_set_size(w, &n) != GS_OK) fail(GS_ERR_INVALID_HANDLE, "wall set handle is invalid");
+        std::vector<float> v[12];
+        for (auto& a : v) a.resize(n);
+        gs_walls out{v[0].data(), v[1].data(), v[2].data(), v[3].data(), v[4].data(),
+                     v[5].data(), v[6].data(), v[7].data(), v[8].data()};
+        if (gs_wall_set_get(w, &out, v[9].data(), v[10].data(), v[11].data()) != GS_OK)
+            fail(GS_ERR_INVALID_HANDLE, "wall set handle is invalid");
+        const float* hw[9];
+        for (int j = 0; j < 9; ++j) hw[j] = v[j].data();
+        const float* mid[3] = {v[9].data(), v[10].data(), v[11].data()};
+        upload_walls(g, n, hw, mid);
     });
 }
 
@@ -1082,22 +1108,40 @@ int gs_ensemble_download_agents(gs_ensemble e, uint32_t scenario, uint32_t group
     });
 }
 
+static void ens_upload_walls(gs_ensemble e, uint32_t scenario, uint32_t n, const gs_walls* host,
+                             const float* const mid[3]) {
+    check_ens(e);
+    if (scenario >= e->scen.size()) fail(GS_ERR_INVALID_ARGUMENT, "scenario index out of range");
+    if (n && !host) fail(GS_ERR_INVALID_ARGUMENT, "wall arrays are NULL");
+    const float* const* hw = (const float* const*)host;
+    for (int j = 0; j < 9 && n; ++j)
+        if (!hw[j]) fail(GS_ERR_INVALID_ARGUMENT, "wall array is NULL");
+    if (mid && n && (!mid[0] || !mid[1] || !mid[2])) fail(GS_ERR_INVALID_ARGUMENT, "midpoint arrays are NULL");
+    // 12 floats per wall: start, end, normal, KNN position.  Without given
+    // positions a wall's is its midpoint 0.5f*(start+end)
+    // (CUDA/WallGroupData.cu:87-89, WallGroup.h:31).
+    auto& v = e->walls_h[scenario];
+    v.assign((size_t)n * 12, 0.f);
+    for (uint32_t i = 0; i < n; ++i) {
+        float* r = &v[(size_t)i * 12];
+        for (int j = 0; j < 9; ++j) r[j] = hw[j][i];
+        for (int a = 0; a < 3; ++a) r[9 + a] = mid ? mid[a][i] : 0.5f * (r[a] + r[3 + a]);
+    }
+    use_device(e->ctx);
+    drop_graphs(e);
+    ens_upload_geometry(e);
+    GS_HIP(hipStreamSynchronize(e->ctx->stream));
+}
+
 int gs_ensemble_upload_walls(gs_ensemble e, uint32_t scenario, uint32_t n, const gs_walls* host) {
+    return guarded([&] { ens_upload_walls(e, scenario, n, host, nullptr); });
+}
+
+int gs_ensemble_upload_walls_mid(gs_ensemble e, uint32_t scenario, uint32_t n, const gs_walls* host,
+                                 const float* mid_x, const float* mid_y, const float* mid_z) {
     return guarded([&] {
-        check_ens(e);
-        if (scenario >= e->scen.size()) fail(GS_ERR_INVALID_ARGUMENT, "scenario index out of range");
-        if (n && !host) fail(GS_ERR_INVALID_ARGUMENT, "wall arrays are NULL");
-        const float* const* hw = (const float* const*)host;
-        for (int j = 0; j < 9 && n; ++j)
-            if (!hw[j]) fail(GS_ERR_INVALID_ARGUMENT, "wall array is NULL");
-        auto& v = e->walls_h[scenario];
-        v.assign((size_t)n * 9, 0.f);
-        for (uint32_t i = 0; i < n; ++i)
-            for (int j = 0; j < 9; ++j) v[(size_t)i * 9 + j] = hw[j][i];
-        use_device(e->ctx);
-        drop_graphs(e);
-        ens_upload_geometry(e);
-        GS_HIP(hipStreamSynchronize(e->ctx->stream));
+        const float* mid[3] = {mid_x, mid_y, mid_z};
+        ens_upload_walls(e, scenario, n, host, mid);
     });
 }
 

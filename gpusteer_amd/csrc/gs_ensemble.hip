@@ -485,13 +485,11 @@ void ens_upload_geometry(gs_ensemble_s* e) {
     for (uint32_t i = 0; i < e->scens_host.size(); ++i) {
         EnsScen& S = e->scens_host[i];
         S.w0 = (uint32_t)w[0].size();
-        S.nw = (uint32_t)e->walls_h[i].size() / 9;
+        S.nw = (uint32_t)e->walls_h[i].size() / 12;
         for (uint32_t j = 0; j < S.nw; ++j) {
-            const float* r = &e->walls_h[i][(size_t)j * 9];
-            for (int f = 0; f < 9; ++f) w[f].push_back(r[f]);
-            // KNN position of a wall: its midpoint 0.5f*(start+end)
-            // (CUDA/WallGroupData.cu:87-89, WallGroup.h:31).
-            for (int a = 0; a < 3; ++a) w[9 + a].push_back(0.5f * (r[a] + r[3 + a]));
+            // start, end, normal, KNN position
+            const float* r = &e->walls_h[i][(size_t)j * 12];
+            for (int f = 0; f < 12; ++f) w[f].push_back(r[f]);
         }
         S.o0 = (uint32_t)o[0].size();
         S.no = (uint32_t)e->obs_h[i].size() / 4;

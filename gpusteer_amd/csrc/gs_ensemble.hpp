@@ -58,7 +58,7 @@ struct gs_ensemble_s {
 
     // Walls: start, end, normal, midpoint (12 arrays); obstacles: centre, radius.
     gs::DevBuf<float> walls[12], obs[4];
-    std::vector<std::vector<float>> walls_h, obs_h;  // per scenario, 9 / 4 floats per entry
+    std::vector<std::vector<float>> walls_h, obs_h;  // per scenario, 12 / 4 floats per entry
     bool walls_mid_ready = false;
 
     // Captured pairs of steps, one per buffer parity.

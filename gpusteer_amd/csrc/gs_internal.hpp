@@ -6,6 +6,7 @@
 #include <cfloat>
 #include <cstdint>
 #include <cstring>
+#include <new>
 #include <stdexcept>
 #include <string>
 
@@ -21,6 +22,23 @@ struct Error : std::runtime_error {
     Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
 };
 void set_last_error(const std::string& m);
+// Runs f, turning exceptions into a status (and the message for
+// gs_last_error()): the shape of every C entry point.
+template <class F> int guarded(F&& f) {
+    try {
+        f();
+        return GS_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return GS_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return GS_ERR_DEVICE;
+    }
+}
 [[noreturn]] void fail(int status, const std::string& msg);
 void hip_check(hipError_t e, const char* what);
 #define GS_HIP(call) ::gs::hip_check((call), #call)

@@ -77,8 +77,15 @@ class Ensemble:
                 sc = spec[1]
                 w = {f: np.ascontiguousarray(sc.walls.arrays[f], dtype=np.float32) for f in WALL_FIELDS}
                 ws = capi.GsWalls(*[fptr(w[f]) for f in WALL_FIELDS])
-                check(capi.lib.gs_ensemble_upload_walls(self.h, i, sc.walls.n, C.byref(ws)),
-                      "gs_ensemble_upload_walls")
+                if sc.wall_mid is None:
+                    check(capi.lib.gs_ensemble_upload_walls(self.h, i, sc.walls.n, C.byref(ws)),
+                          "gs_ensemble_upload_walls")
+                else:
+                    m = np.asarray(sc.wall_mid, dtype=np.float32).reshape(sc.walls.n, 3)
+                    cols = [np.ascontiguousarray(m[:, a]) for a in range(3)]
+                    check(capi.lib.gs_ensemble_upload_walls_mid(self.h, i, sc.walls.n, C.byref(ws),
+                                                                *[fptr(c) for c in cols]),
+                          "gs_ensemble_upload_walls_mid")
                 o = [np.ascontiguousarray(a, dtype=np.float32) for a in sc.obstacles]
                 check(capi.lib.gs_ensemble_upload_obstacles(self.h, i, o[0].shape[0], *[fptr(a) for a in o]),
                       "gs_ensemble_upload_obstacles")
@@ -167,7 +174,8 @@ class Sequential:
                 sc = spec[1]
                 bins = api.KNNBinData(ctx, sc.grid_cells, sc.grid_size, origin=sc.grid_origin)
                 G = [api.AgentGroup(ctx, g) for g in sc.groups]
-                W = api.WallGroup(ctx, sc.walls)
+                W = api.WallGroup(ctx)
+                W.upload(sc.walls, sc.wall_mid)
                 O = api.ObstacleGroup(ctx, *sc.obstacles)
                 api.updateKNNDatabase(W, bins)
                 api.updateKNNDatabase(O, bins)

@@ -17,7 +17,7 @@ boids_box      2^20-agent flock in a 1000^3 periodic box (k = 7 or 16).
 from __future__ import annotations
 
 from dataclasses import dataclass
-from typing import Tuple
+from typing import Optional, Tuple
 
 import numpy as np
 
@@ -159,6 +159,10 @@ class ChokeScenario:
     # separation term's max distance (3), so the forces equal the uncapped
     # search's.
     other_radius: float = 3.5
+    # KNN positions of the walls ([n][3]); None means 0.5f * (start + end).
+    # Walls split at bin-cell borders carry their own (start + 0.5f * (end -
+    # start) of each piece, WallGroupData.cu:185-190).
+    wall_mid: Optional[np.ndarray] = None
 
 
 def choke_walls(gap: float, half_len: float = 100.0, thickness: float = 0.5) -> WallState:

@@ -308,6 +308,10 @@ int gs_ensemble_slots(gs_ensemble e, uint32_t* slots);
 int gs_ensemble_upload_agents(gs_ensemble e, uint32_t scenario, uint32_t group, const gs_agents* host);
 int gs_ensemble_download_agents(gs_ensemble e, uint32_t scenario, uint32_t group, gs_agents* host);
 int gs_ensemble_upload_walls(gs_ensemble e, uint32_t scenario, uint32_t n, const gs_walls* host);
+/* The same with each wall's KNN position given (walls split at bin-cell
+ * borders keep the midpoints SplitWalls gave them, WallGroupData.cu:185-190). */
+int gs_ensemble_upload_walls_mid(gs_ensemble e, uint32_t scenario, uint32_t n, const gs_walls* host,
+                                 const float* mid_x, const float* mid_y, const float* mid_z);
 int gs_ensemble_upload_obstacles(gs_ensemble e, uint32_t scenario, uint32_t n, const float* cx, const float* cy,
                                  const float* cz, const float* radius);
 /* The last KNN of a group with itself, [k][n] agent indices (pads UINT32_MAX). */
@@ -361,6 +365,79 @@ int gs_calib_backward(gs_calib c, gs_calib_params* grad);
 int gs_calib_download_state(gs_calib c, uint32_t scenario, uint32_t group, gs_agents* host);
 /* HIP-event milliseconds of the last forward and backward passes. */
 int gs_calib_timing(gs_calib c, float* forward_ms, float* backward_ms);
+
+/* ---- Scenario formats (host only, no device work) ----------------------
+ * Wall maps, wall splitting at bin-cell borders, simulation parameter files:
+ * what the plugins load before the per-step path runs. */
+typedef struct gs_wall_set_s* gs_wall_set;
+/* WallGroupData::LoadFromFile (include/OpenSteer/CUDA/WallGroupData.cu:73-103):
+ * records of start xyz, end xyz, normal xyz; midpoints 0.5f * (start + end). */
+int gs_wall_set_load_map(const char* path, gs_wall_set* out);
+/* A set from arrays (midpoints 0.5f * (start + end)). */
+int gs_wall_set_create(uint32_t n, const gs_walls* host, gs_wall_set* out);
+/* WallGroupData::SplitWalls (WallGroupData.cu:105-206) over the cells of
+ * KNNBinDataV2::CreateCells (CUDA/KNNBinDataV2.cu:73-136): a lattice of
+ * `cells` over a box of `world_size` centred at the origin. */
+int gs_wall_set_split(gs_wall_set w, const float world_size[3], const uint32_t cells[3]);
+/* The same over any list of cells, tried in list order: cell i spans
+ * [min_bounds[3i..3i+2], max_bounds[3i..3i+2]] (bin_cell::minBound / maxBound,
+ * CUDA/KNNBinDataGlobals.cuh:7-13). */
+int gs_wall_set_split_cells(gs_wall_set w, uint32_t n_cells, const float* min_bounds, const float* max_bounds);
+/* Replaces the midpoints (KNN positions) of the set's n walls. */
+int gs_wall_set_set_mid(gs_wall_set w, const float* mid_x, const float* mid_y, const float* mid_z);
+int gs_wall_set_size(gs_wall_set w, uint32_t* n);
+/* Copies the walls (any pointer may be NULL) into caller arrays of size n. */
+int gs_wall_set_get(gs_wall_set w, gs_walls* out, float* mid_x, float* mid_y, float* mid_z);
+int gs_wall_set_destroy(gs_wall_set w);
+/* WallGroup::SyncDevice after LoadFromFile / SplitWalls
+ * (include/OpenSteer/WallGroup.h:18-27): uploads the set with its own
+ * midpoints as the walls' KNN positions. */
+int gs_group_upload_wall_set(gs_group g, gs_wall_set w);
+/* gs_group_upload_walls with explicit KNN positions (midpoints). */
+int gs_group_upload_walls_mid(gs_group g, uint32_t n, const gs_walls* host, const float* mid_x, const float* mid_y,
+                              const float* mid_z);
+
+#define GS_PARAMS_MAX_GROUPS 32u
+#define GS_PARAMS_MAX_PATH 64u
+/* GroupParams (include/OpenSteer/Simulation.h:16-48). */
+typedef struct {
+    uint32_t num_agents;
+    float max_speed, max_force;
+    float start_position[3];
+    float min_start_radius, max_start_radius;
+    uint32_t path_points;
+    float path[GS_PARAMS_MAX_PATH][3];
+    float path_radius;
+    uint32_t path_cyclic;
+    float body_color[3];
+} gs_group_params;
+/* WorldParams (Simulation.h:50-64). */
+typedef struct {
+    float dimensions[3];
+    uint32_t cells[3];
+    uint32_t obstacle_count;
+    char map_filename[256];
+} gs_world_params;
+/* SimulationParams (Simulation.h:66-120); fields a file does not set are 0. */
+typedef struct {
+    gs_world_params world;
+    uint32_t n_groups;
+    gs_group_params groups[GS_PARAMS_MAX_GROUPS];
+    uint32_t seed, knn, kno, knw, search_radius;
+    float max_pursuit_prediction_time, min_time_to_collision, min_time_to_obstacle, min_time_to_wall,
+        path_prediction_time, min_separation_distance, max_separation_distance, min_flocking_distance,
+        max_flocking_distance, cos_max_flocking_angle;
+    uint32_t avoid_close_neighbors;
+    float weight_alignment, weight_cohesion, weight_separation, weight_pursuit, weight_seek, weight_flee,
+        weight_evade, weight_follow_path, weight_avoid_obstacles, weight_avoid_walls, weight_avoid_neighbors;
+    uint32_t mask_alignment, mask_anti_penetration_agents, mask_cohesion, mask_separation, mask_pursuit,
+        mask_seek, mask_flee, mask_evade, mask_follow_path, mask_avoid_obstacles, mask_avoid_walls,
+        mask_avoid_neighbors;
+} gs_sim_params;
+/* SimulationParams::load (src/Simulation.cpp:7-250).  Unknown keys, bad
+ * numbers, a missing file or a missing </simulation> return
+ * GS_ERR_INVALID_ARGUMENT (the reference asserts or does not return). */
+int gs_sim_params_load(const char* path, gs_sim_params* out);
 
 #ifdef __cplusplus
 }

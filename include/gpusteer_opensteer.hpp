@@ -8,9 +8,11 @@
  *   ObstacleData                  include/OpenSteer/CUDA/ObstacleGroupData.cuh:10-14
  *   AgentGroup                    include/OpenSteer/AgentGroup.h:12-56, src/AgentGroup.cpp:8-140
  *   ObstacleGroup                 include/OpenSteer/ObstacleGroup.h:10-43
- *   WallGroup                     include/OpenSteer/WallGroup.h:12-37 (segments added directly)
+ *   WallGroup                     include/OpenSteer/WallGroup.h:12-37 (LoadFromFile, SplitWalls, AddWall)
  *   KNNData                       include/OpenSteer/CUDA/KNNData.cuh:10-55
- *   KNNBinData                    include/OpenSteer/CUDA/KNNBinDataV2.cuh:43-61
+ *   KNNBinData, bin_cell          include/OpenSteer/CUDA/KNNBinDataV2.cuh:43-61, KNNBinDataGlobals.cuh:7-13
+ *   SimulationParams, WorldParams,
+ *   GroupParams                   include/OpenSteer/Simulation.h:16-120, src/Simulation.cpp:7-250
  *   wrapWorld .. updateGroup      include/OpenSteer/CUDA/GroupSteerLibrary.cuh:43-189
  *
  * Differences a plugin can observe:
@@ -105,6 +107,13 @@ struct ObstacleData {
 };
 
 // Common base of the three group kinds (include/OpenSteer/BaseGroup.h:8-33).
+struct bin_cell {
+    size_t index;      // index of the cell
+    float3 position;   // centre
+    float3 minBound;
+    float3 maxBound;
+};
+
 class BaseGroup {
 public:
     BaseGroup(int kind, uint3 const& worldCells, uint k, gpusteer::Context& ctx)
@@ -288,40 +297,92 @@ private:
 class WallGroup : public BaseGroup {
 public:
     WallGroup(uint3 const& worldCells, uint const knw, gpusteer::Context& ctx = gpusteer::Context::Default())
-        : BaseGroup(GS_KIND_WALLS, worldCells, knw, ctx) {}
-    void Clear() {
-        s_.clear();
-        e_.clear();
-        nrm_.clear();
-        n_ = 0;
-        dirty_ = true;
+        : BaseGroup(GS_KIND_WALLS, worldCells, knw, ctx) {
+        gpusteer::check(gs_wall_set_create(0, nullptr, &set_), "gs_wall_set_create");
     }
+    ~WallGroup() { gs_wall_set_destroy(set_); }
+    void Clear() { replace(nullptr, 0); }
     void AddWall(float3 const& start, float3 const& end, float3 const& normal) {
-        s_.push_back(start);
-        e_.push_back(end);
-        nrm_.push_back(normal);
-        n_ = (uint)s_.size();
-        dirty_ = true;
+        // Appends to the current walls (their midpoints are kept).
+        std::vector<float> v[12];
+        get(v);
+        const float w[9] = {start.x, start.y, start.z, end.x, end.y, end.z, normal.x, normal.y, normal.z};
+        for (int j = 0; j < 9; ++j) v[j].push_back(w[j]);
+        for (int a = 0; a < 3; ++a) v[9 + a].push_back(0.5f * (w[a] + w[3 + a]));
+        rebuild(v);
+    }
+    // WallGroup::LoadFromFile (WallGroupData.cu:73-103): replaces the walls.
+    bool LoadFromFile(char const* szFilename) {
+        gs_wall_set w = nullptr;
+        if (gs_wall_set_load_map(szFilename, &w) != GS_OK) return false;
+        gs_wall_set_destroy(set_);
+        set_ = w;
+        refresh();
+        return true;
+    }
+    // WallGroup::SplitWalls (WallGroupData.cu:105-206) over the given cells,
+    // in their order (normally KNNBinData::hvCells()).
+    void SplitWalls(std::vector<bin_cell> const& cells) {
+        std::vector<float> lo(3 * cells.size()), hi(3 * cells.size());
+        for (size_t i = 0; i < cells.size(); ++i) {
+            lo[3 * i] = cells[i].minBound.x; lo[3 * i + 1] = cells[i].minBound.y; lo[3 * i + 2] = cells[i].minBound.z;
+            hi[3 * i] = cells[i].maxBound.x; hi[3 * i + 1] = cells[i].maxBound.y; hi[3 * i + 2] = cells[i].maxBound.z;
+        }
+        gpusteer::check(gs_wall_set_split_cells(set_, (uint32_t)cells.size(), lo.data(), hi.data()),
+                        "gs_wall_set_split_cells");
+        refresh();
     }
     void SyncDevice() override {
         if (!dirty_) return;
-        const size_t n = s_.size();
-        std::vector<float> v(9 * (n + 1));
-        gs_walls w;
-        float** p[9] = {&w.sx, &w.sy, &w.sz, &w.ex, &w.ey, &w.ez, &w.nx, &w.ny, &w.nz};
-        for (int j = 0; j < 9; ++j) *p[j] = v.data() + j * (n + 1);
-        for (size_t i = 0; i < n; ++i) {
-            w.sx[i] = s_[i].x; w.sy[i] = s_[i].y; w.sz[i] = s_[i].z;
-            w.ex[i] = e_[i].x; w.ey[i] = e_[i].y; w.ez[i] = e_[i].z;
-            w.nx[i] = nrm_[i].x; w.ny[i] = nrm_[i].y; w.nz[i] = nrm_[i].z;
-        }
-        gpusteer::check(gs_group_upload_walls(h_, (uint32_t)n, &w), "gs_group_upload_walls");
+        gpusteer::check(gs_group_upload_wall_set(h_, set_), "gs_group_upload_wall_set");
         dirty_ = false;
+    }
+    // Host copies (start, end, normal, mid per wall), as WallGroupData's host vectors.
+    void GetWall(uint i, float3& start, float3& end, float3& normal, float3& mid) const {
+        std::vector<float> v[12];
+        get(v);
+        start = make_float3(v[0][i], v[1][i], v[2][i]);
+        end = make_float3(v[3][i], v[4][i], v[5][i]);
+        normal = make_float3(v[6][i], v[7][i], v[8][i]);
+        mid = make_float3(v[9][i], v[10][i], v[11][i]);
     }
 
 private:
-    std::vector<float3> s_, e_, nrm_;
+    gs_wall_set set_ = nullptr;
     bool dirty_ = false;
+
+    void get(std::vector<float>* v) const {
+        uint32_t n = 0;
+        gpusteer::check(gs_wall_set_size(set_, &n), "gs_wall_set_size");
+        for (int j = 0; j < 12; ++j) v[j].resize(n);
+        gs_walls w{v[0].data(), v[1].data(), v[2].data(), v[3].data(), v[4].data(),
+                   v[5].data(), v[6].data(), v[7].data(), v[8].data()};
+        gpusteer::check(gs_wall_set_get(set_, &w, v[9].data(), v[10].data(), v[11].data()), "gs_wall_set_get");
+    }
+    void rebuild(std::vector<float>* v) {
+        // A set from the arrays, then the midpoints restored (split walls keep theirs).
+        const uint32_t n = (uint32_t)v[0].size();
+        std::vector<float> pad[12];
+        for (int j = 0; j < 12; ++j) pad[j] = v[j], pad[j].push_back(0.f);
+        gs_walls w{pad[0].data(), pad[1].data(), pad[2].data(), pad[3].data(), pad[4].data(),
+                   pad[5].data(), pad[6].data(), pad[7].data(), pad[8].data()};
+        replace(&w, n, pad[9].data(), pad[10].data(), pad[11].data());
+    }
+    void replace(const gs_walls* w, uint32_t n, const float* mx = nullptr, const float* my = nullptr,
+                 const float* mz = nullptr) {
+        gs_wall_set s = nullptr;
+        gpusteer::check(gs_wall_set_create(n, w, &s), "gs_wall_set_create");
+        if (mx) gpusteer::check(gs_wall_set_set_mid(s, mx, my, mz), "gs_wall_set_set_mid");
+        gs_wall_set_destroy(set_);
+        set_ = s;
+        refresh();
+    }
+    void refresh() {
+        uint32_t n = 0;
+        gpusteer::check(gs_wall_set_size(set_, &n), "gs_wall_set_size");
+        n_ = n;
+        dirty_ = true;
+    }
 };
 
 class KNNData {
@@ -379,6 +440,29 @@ public:
     uint3 const& WorldCells() const { return cells_; }
     float3 const& WorldSize() const { return size_; }
     uint getNumCells() const { return cells_.x * cells_.y * cells_.z; }
+    // KNNBinDataV2::CreateCells (CUDA/KNNBinDataV2.cu:73-136): cells of the
+    // lattice over a box centred at the origin, y outer, z, x inner.
+    std::vector<bin_cell> hvCells() const {
+#if defined(__clang__)
+#pragma clang fp contract(off)  // each bound rounds as two operations, as the library's lattice
+#endif
+        const float3 step = make_float3(size_.x / cells_.x, size_.y / cells_.y, size_.z / cells_.z);
+        std::vector<bin_cell> out;
+        out.reserve((size_t)cells_.x * cells_.y * cells_.z);
+        for (uint iy = 0; iy < cells_.y; ++iy)
+            for (uint iz = 0; iz < cells_.z; ++iz)
+                for (uint ix = 0; ix < cells_.x; ++ix) {
+                    bin_cell c;
+                    c.index = ix + iz * cells_.x + iy * cells_.z * cells_.x;
+                    c.minBound = make_float3((float)ix * step.x - 0.5f * size_.x, (float)iy * step.y - 0.5f * size_.y,
+                                             (float)iz * step.z - 0.5f * size_.z);
+                    c.position = make_float3(c.minBound.x + 0.5f * step.x, c.minBound.y + 0.5f * step.y,
+                                             c.minBound.z + 0.5f * step.z);
+                    c.maxBound = make_float3(c.minBound.x + step.x, c.minBound.y + step.y, c.minBound.z + step.z);
+                    out.push_back(c);
+                }
+        return out;
+    }
     bool is3D() const { return cells_.y > 1; }
     gs_bins handle() const { return h_; }
 
@@ -459,6 +543,95 @@ inline void updateGroup(AgentGroup* pAgentGroup, const float elapsedTime) {
                     "gs_update_group");
     pAgentGroup->SetSyncHost();
 }
+
+// ---- Simulation.h parameter classes -----------------------------------------
+
+struct GroupParams {
+    uint m_nNumAgents = 0;
+    float m_fMaxSpeed = 0.f, m_fMaxForce = 0.f;
+    float3 m_f3StartPosition = make_float3(0.f, 0.f, 0.f);
+    float m_fMinStartRadius = 0.f, m_fMaxStartRadius = 0.f;
+    std::vector<float3> m_vecPathPoints;
+    float m_fPathRadius = 0.f;
+    bool m_bPathIsCyclic = false;
+    float3 m_f3BodyColor = make_float3(0.f, 0.f, 0.f);
+};
+struct WorldParams {
+    float3 m_f3Dimensions = make_float3(0.f, 0.f, 0.f);
+    uint3 m_u3Cells = make_uint3(0, 0, 0);
+    uint m_nObstacleCount = 0;
+    std::string m_strMapFilename;
+};
+class SimulationParams {
+public:
+    // SimulationParams::load (src/Simulation.cpp:7-250); throws
+    // gpusteer::Error on an unknown key, a bad number, a missing file or a
+    // missing </simulation> (the reference asserts or does not return).
+    void load(char const* szFilename) {
+        std::unique_ptr<gs_sim_params> p(new gs_sim_params());
+        gpusteer::check(gs_sim_params_load(szFilename, p.get()), "gs_sim_params_load");
+        const gs_sim_params& q = *p;
+        m_WorldParams.m_f3Dimensions = make_float3(q.world.dimensions[0], q.world.dimensions[1], q.world.dimensions[2]);
+        m_WorldParams.m_u3Cells = make_uint3(q.world.cells[0], q.world.cells[1], q.world.cells[2]);
+        m_WorldParams.m_nObstacleCount = q.world.obstacle_count;
+        m_WorldParams.m_strMapFilename = q.world.map_filename;
+        m_vecGroupParams.clear();
+        for (uint32_t i = 0; i < q.n_groups; ++i) {
+            const gs_group_params& g = q.groups[i];
+            GroupParams gp;
+            gp.m_nNumAgents = g.num_agents;
+            gp.m_fMaxSpeed = g.max_speed;
+            gp.m_fMaxForce = g.max_force;
+            gp.m_f3StartPosition = make_float3(g.start_position[0], g.start_position[1], g.start_position[2]);
+            gp.m_fMinStartRadius = g.min_start_radius;
+            gp.m_fMaxStartRadius = g.max_start_radius;
+            for (uint32_t j = 0; j < g.path_points; ++j)
+                gp.m_vecPathPoints.push_back(make_float3(g.path[j][0], g.path[j][1], g.path[j][2]));
+            gp.m_fPathRadius = g.path_radius;
+            gp.m_bPathIsCyclic = g.path_cyclic != 0;
+            gp.m_f3BodyColor = make_float3(g.body_color[0], g.body_color[1], g.body_color[2]);
+            m_vecGroupParams.push_back(gp);
+        }
+        m_nSeed = q.seed; m_nKNN = q.knn; m_nKNO = q.kno; m_nKNW = q.knw; m_nSearchRadius = q.search_radius;
+        m_fMaxPursuitPredictionTime = q.max_pursuit_prediction_time;
+        m_fMinTimeToCollision = q.min_time_to_collision;
+        m_fMinTimeToObstacle = q.min_time_to_obstacle;
+        m_fMinTimeToWall = q.min_time_to_wall;
+        m_fPathPredictionTime = q.path_prediction_time;
+        m_fMinSeparationDistance = q.min_separation_distance;
+        m_fMaxSeparationDistance = q.max_separation_distance;
+        m_fMinFlockingDistance = q.min_flocking_distance;
+        m_fMaxFlockingDistance = q.max_flocking_distance;
+        m_fCosMaxFlockingAngle = q.cos_max_flocking_angle;
+        m_bAvoidCloseNeighbors = q.avoid_close_neighbors != 0;
+        m_fWeightAlignment = q.weight_alignment; m_fWeightCohesion = q.weight_cohesion;
+        m_fWeightSeparation = q.weight_separation; m_fWeightPursuit = q.weight_pursuit;
+        m_fWeightSeek = q.weight_seek; m_fWeightFlee = q.weight_flee; m_fWeightEvade = q.weight_evade;
+        m_fWeightFollowPath = q.weight_follow_path; m_fWeightAvoidObstacles = q.weight_avoid_obstacles;
+        m_fWeightAvoidWalls = q.weight_avoid_walls; m_fWeightAvoidNeighbors = q.weight_avoid_neighbors;
+        m_nMaskAlignment = q.mask_alignment; m_nMaskAntiPenetrationAgents = q.mask_anti_penetration_agents;
+        m_nMaskCohesion = q.mask_cohesion; m_nMaskSeparation = q.mask_separation;
+        m_nMaskPursuit = q.mask_pursuit; m_nMaskSeek = q.mask_seek; m_nMaskFlee = q.mask_flee;
+        m_nMaskEvade = q.mask_evade; m_nMaskFollowPath = q.mask_follow_path;
+        m_nMaskAvoidObstacles = q.mask_avoid_obstacles; m_nMaskAvoidWalls = q.mask_avoid_walls;
+        m_nMaskAvoidNeighbors = q.mask_avoid_neighbors;
+    }
+
+    WorldParams m_WorldParams;
+    std::vector<GroupParams> m_vecGroupParams;
+    uint m_nSeed = 0, m_nKNN = 0, m_nKNO = 0, m_nKNW = 0, m_nSearchRadius = 0;
+    float m_fMaxPursuitPredictionTime = 0.f, m_fMinTimeToCollision = 0.f, m_fMinTimeToObstacle = 0.f,
+          m_fMinTimeToWall = 0.f, m_fPathPredictionTime = 0.f, m_fMinSeparationDistance = 0.f,
+          m_fMaxSeparationDistance = 0.f, m_fMinFlockingDistance = 0.f, m_fMaxFlockingDistance = 0.f,
+          m_fCosMaxFlockingAngle = 0.f;
+    bool m_bAvoidCloseNeighbors = false;
+    float m_fWeightAlignment = 0.f, m_fWeightCohesion = 0.f, m_fWeightSeparation = 0.f, m_fWeightPursuit = 0.f,
+          m_fWeightSeek = 0.f, m_fWeightFlee = 0.f, m_fWeightEvade = 0.f, m_fWeightFollowPath = 0.f,
+          m_fWeightAvoidObstacles = 0.f, m_fWeightAvoidWalls = 0.f, m_fWeightAvoidNeighbors = 0.f;
+    uint m_nMaskAlignment = 0, m_nMaskAntiPenetrationAgents = 0, m_nMaskCohesion = 0, m_nMaskSeparation = 0,
+         m_nMaskPursuit = 0, m_nMaskSeek = 0, m_nMaskFlee = 0, m_nMaskEvade = 0, m_nMaskFollowPath = 0,
+         m_nMaskAvoidObstacles = 0, m_nMaskAvoidWalls = 0, m_nMaskAvoidNeighbors = 0;
+};
 
 }  // namespace OpenSteer
 

@@ -262,7 +262,7 @@ def choke_step(groups, sc):
     avoid walls -> avoid obstacles -> separation (self, other) -> seek; then
     integrate both.  Returns the KNN results per group."""
     g = grid(sc.grid_origin, sc.grid_size, sc.grid_cells)
-    wmid = sc.walls.mid()
+    wmid = sc.walls.mid() if getattr(sc, "wall_mid", None) is None else np.asarray(sc.wall_mid, np.float32)
     opos = np.stack(sc.obstacles[:3], axis=1).astype(np.float32)
     pos = [np.stack([s.arrays["px"], s.arrays["py"], s.arrays["pz"]], axis=1).astype(np.float32) for s in groups]
     out = []

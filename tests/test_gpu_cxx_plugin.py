@@ -58,3 +58,46 @@ def test_cxx_boids_plugin_matches_oracle(orc, tmp_path, mode):
     want = np.stack([s.arrays[f] for f in ("px", "py", "pz", "fx", "fy", "fz", "speed")], axis=1)
     np.testing.assert_array_equal(idx, ridx)
     np.testing.assert_array_equal(rec, want)
+
+
+CHOKE = gbuild.LIB_DIR / "choke_setup"
+FILES = gbuild.ROOT / "tests" / "golden" / "scenario_files"
+
+
+def test_cxx_choke_setup_matches_oracle(orc, tmp_path):
+    """The set-up of a choke-point plugin through the drop-in
+    (tests/cxx/choke_setup.cpp: SimulationParams::load of
+    data/ChokePoint.params, WallGroup::LoadFromFile of its map, SplitWalls
+    over KNNBinData::hvCells(), SyncDevice, updateKNNDatabase): the split
+    walls (1254 pieces over the 512 x 1 x 512 lattice) equal the oracle's,
+    and the wall group's bin database read back from the device equals the
+    oracle's counting sort of their midpoints."""
+    import formats_ref as fref
+
+    assert CHOKE.exists(), f"{CHOKE} not built (gpusteer_amd/build.py builds it with the library)"
+    out = tmp_path / "choke.bin"
+    print("cxx choke setup: ChokePoint.params", flush=True)
+    r = subprocess.run([str(CHOKE), str(FILES / "ChokePoint.params"), str(FILES), str(out)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.uint8)
+    head = raw[:16].view(np.uint32)
+    dims = raw[16:28].view(np.float32)
+    n = int(raw[28:32].view(np.uint32)[0])
+    assert tuple(head) == (3, 512, 1, 512) and tuple(dims) == (2048.0, 10.0, 2048.0) and n == 1254
+    o = 32 + n * 48
+    rec = raw[32:o].view(np.float32).reshape(n, 12)
+    key = raw[o:o + 4 * n].view(np.uint32)
+    perm = raw[o + 4 * n:o + 8 * n].view(np.uint32)
+    sorted_mid = raw[o + 8 * n:o + 20 * n].view(np.float32).reshape(3, n).T
+
+    want = fref.split_walls(fref.load_map(FILES / "ChokePoint20.map"), fref.lattice_cells(dims, (512, 1, 512)))
+    want = np.array([[*s, *e, *nrm, *m] for s, e, nrm, m in want], np.float32)
+    np.testing.assert_array_equal(rec, want)
+
+    g = orc.grid(tuple(-0.5 * d for d in dims), tuple(dims), (512, 1, 512))
+    mid = np.ascontiguousarray(want[:, 9:12])
+    rkey, _, rperm, rsorted = orc.bins_build(g, mid)
+    np.testing.assert_array_equal(key, rkey)
+    np.testing.assert_array_equal(perm, rperm)
+    np.testing.assert_array_equal(sorted_mid, rsorted)

@@ -178,9 +178,88 @@ def test_choke_pipeline_bit_exact(gs, ctx, orc):
                 for j in range(4):
                     np.testing.assert_array_equal(knns[gi][j].download()[0], r[gi][j], err_msg=f"step {step}")
                 assert_state_equal(G[gi].download(), ref[gi], what=f"step {step} group {gi}: ")
-    bits_seen = 0
     for h in G + [W, O, bins] + [k for ks in knns for k in ks]:
         h.close()
+
+
+def _split_choke(n_total=4000):
+    """The choke point with its walls split at the borders of its own bin
+    lattice (WallGroup::SplitWalls over KNNBinData::hvCells()), the split
+    walls keeping the midpoints the split gave them."""
+    import dataclasses
+
+    from gpusteer_amd import formats
+
+    sc = scenarios.choke(n_total=n_total)
+    ws = formats.WallSet.from_walls(sc.walls).split(sc.grid_size, sc.grid_cells)
+    walls, mid = ws.get()
+    ws.close()
+    return dataclasses.replace(sc, walls=walls, wall_mid=mid), sc.walls.n
+
+
+def test_choke_pipeline_split_walls_bit_exact(gs, ctx, orc):
+    """The choke point with split walls (80 pieces of its 6 walls on its
+    5-unit lattice) for 60 steps: wall KNN on the split midpoints, compared
+    with the oracle's same sequence."""
+    sc, n_orig = _split_choke()
+    assert (n_orig, sc.walls.n) == (6, 80)
+    ref = [g.copy() for g in sc.groups]
+    bins = gs.KNNBinData(ctx, sc.grid_cells, sc.grid_size, origin=sc.grid_origin)
+    G = [gs.AgentGroup(ctx, g) for g in sc.groups]
+    W = gs.WallGroup(ctx)
+    W.upload(sc.walls, sc.wall_mid)
+    O = gs.ObstacleGroup(ctx, *sc.obstacles)
+    gs.updateKNNDatabase(W, bins)
+    gs.updateKNNDatabase(O, bins)
+    knns = [[gs.KNNData(ctx, g.n, k) for k in (sc.k, sc.k, sc.knw, sc.kno)] for g in sc.groups]
+    for step in range(60):
+        _choke_gpu_step(gs, ctx, G, knns, W, O, bins, sc)
+        r = orc.choke_step(ref, sc)
+        if step % 20 == 0 or step == 59:
+            for gi in range(2):
+                for j in range(4):
+                    np.testing.assert_array_equal(knns[gi][j].download()[0], r[gi][j], err_msg=f"step {step}")
+                assert_state_equal(G[gi].download(), ref[gi], what=f"step {step} group {gi}: ")
+    for h in G + [W, O, bins] + [k for ks in knns for k in ks]:
+        h.close()
+
+
+def test_choke_wall_mid_ensemble_and_sequential(ctx, orc):
+    """Given wall KNN positions reach both the batched ensemble
+    (gs_ensemble_upload_walls_mid) and the wall group
+    (gs_group_upload_walls_mid): two split-wall choke scenarios whose wall
+    positions are moved off the midpoints (start + 0.3 (end - start)), 20
+    steps batched, one by one and in the oracle, all equal."""
+    import dataclasses
+
+    from gpusteer_amd.ensemble import Ensemble, Sequential
+
+    specs = []
+    for n in (1500, 2500):
+        sc = _split_choke(n)[0]
+        a = sc.walls.arrays
+        f = np.float32(0.3)
+        mid = np.stack([a[s] + f * (a[e] - a[s]) for s, e in (("sx", "ex"), ("sy", "ey"), ("sz", "ez"))], axis=1)
+        assert not np.array_equal(mid, sc.wall_mid)
+        # the moved positions change the agents' wall neighbours
+        g0 = orc.grid(sc.grid_origin, sc.grid_size, sc.grid_cells)
+        pos = np.stack([sc.groups[0].arrays[c] for c in ("px", "py", "pz")], axis=1)
+        assert not np.array_equal(orc.knn_binned(g0, pos, mid, sc.knw, False)[0],
+                                  orc.knn_binned(g0, pos, sc.wall_mid, sc.knw, False)[0])
+        specs.append(("choke", dataclasses.replace(sc, wall_mid=mid)))
+    ens, seq = Ensemble(ctx, specs), Sequential(ctx, specs)
+    refs = [[g.copy() for g in sp[1].groups] for sp in specs]
+    ens.step(20)
+    seq.step(20)
+    for i, sp in enumerate(specs):
+        for _ in range(20):
+            r = orc.choke_step(refs[i], sp[1])
+        for g in range(2):
+            tag = f"scenario {i} group {g}: "
+            np.testing.assert_array_equal(ens.download_knn(i, g), seq.download_knn(i, g), err_msg=tag)
+            np.testing.assert_array_equal(ens.download_knn(i, g), r[g][0], err_msg=tag)
+            assert_state_equal(ens.download(i, g), refs[i][g], what=tag)
+            assert_state_equal(seq.download(i, g), refs[i][g], what=tag)
 
 
 def test_drift_vs_fp64(gs, ctx, orc):
