@@ -258,6 +258,153 @@ def calib_bench(n_scenarios: int, n_agents: int, steps: int, every: int) -> dict
     }
 
 
+# Algorithmic bytes per agent of the plugin-compat kernels (SoA fp32, each
+# field read or written once): follow path reads position, forward, speed,
+# steering, bits (44) and writes steering, bits (16); agent anti-penetration
+# reads position, radius, bits, the first KNN index and distance (28) and
+# writes position, bits (16) -- the other agent's fields are the same arrays
+# (a group against itself); the integrate with wall anti-penetration reads
+# position, forward, side, up, speed, steering, radius, max force, max speed,
+# mass, bits and k wall indices (84 + 4k) and writes position, forward, side,
+# up, speed, steering, bits (68).
+def bytes_follow_path() -> float:
+    return 60.0
+
+
+def bytes_anti_penetration_agents() -> float:
+    return 44.0
+
+
+def bytes_update_anti_penetration(k: int) -> float:
+    return 152.0 + 4.0 * k
+
+
+def _events_ms(L, stream, fn, reps: int) -> float:
+    """Average HIP-event time of fn() (one kernel launch) over reps calls on
+    the library's stream."""
+    import ctypes as C
+
+    ev0, ev1 = C.c_void_p(), C.c_void_p()
+    for e in (ev0, ev1):
+        if L.hipEventCreate(C.byref(e)) != 0:
+            raise RuntimeError("hipEventCreate failed")
+    fn()
+    L.hipEventRecord(ev0, C.c_void_p(stream))
+    for _ in range(reps):
+        fn()
+    L.hipEventRecord(ev1, C.c_void_p(stream))
+    L.hipEventSynchronize(ev1)
+    ms = C.c_float()
+    L.hipEventElapsedTime(C.byref(ms), ev0, ev1)
+    L.hipEventDestroy(ev0)
+    L.hipEventDestroy(ev1)
+    return ms.value / reps
+
+
+def plugin_bench(steps: int, warmup: int, n_kernel: int = 1 << 20, reps: int = 50) -> dict:
+    """The choke-point plugins on their own data: data/ChokePoint.params
+    (131,072 agents, 2048 x 10 x 2048 world, 512 x 1 x 512 bins) with
+    ChokePoint20.map split at the bin borders (1254 walls), stepped through
+    the per-step sequences of plugins/ChokePointNew.cpp (CtfGroup::update)
+    and plugins/ChokePointGroup.cpp (integrate with wall anti-penetration);
+    plus HIP-event times of the three plugin-compat kernels on 2^20-agent
+    states against their algorithmic bytes."""
+    import numpy as np
+
+    import gpusteer_amd as gs
+    from gpusteer_amd import formats, scenarios
+    from gpusteer_amd.plugins import ChokePointNew
+
+    files = ROOT / "tests" / "golden" / "scenario_files"
+    p = formats.load_params(files / "ChokePoint.params")
+    ws = formats.WallSet.load_map(files / p.map_filename).split(p.dimensions, p.cells)
+    walls, mid = ws.get()
+    ws.close()
+    t0 = time.perf_counter()
+    sc = scenarios.choke_point_new(p, walls, mid, dt=1.0 / 30.0)
+    print(f"bench: choke-point scenario ({sc.state.n} agents, {walls.n} walls) in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    L = _hip_runtime()
+    res = {"workload": f"data/ChokePoint.params: {sc.state.n} agents started around (0, 0, 512) within 450, "
+                       f"{walls.n} split walls of ChokePoint20.map, 512x1x512 bins over 2048x10x2048, knn "
+                       f"{p.knn}, knw {p.knw} (capped at {sc.wall_cap:.1f}), path (0,0,13.5)->(0,0,-1650) r 20, "
+                       "dt 1/30"}
+    with gs.Context(0) as ctx:
+        for name, variant in (("choke_point_new", "step"), ("choke_point_group", "step_with_wall_update")):
+            sim = ChokePointNew(ctx, sc)
+            run = getattr(sim, variant)
+            run(warmup)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            run(steps)
+            ctx.synchronize()
+            el = time.perf_counter() - t0
+            st, bad = sim.group.check_finite()
+            sim.close()
+            if st != 0:
+                raise RuntimeError(f"non-finite state in {name} at agent {bad}")
+            res[name] = {"ms_per_step": el * 1e3 / steps, "agent_steps_per_s": sc.state.n * steps / el,
+                         "steps": steps, "warmup": warmup}
+            print(f"bench: {name} {el * 1e3 / steps:.3f} ms/step", file=sys.stderr, flush=True)
+
+        # Kernel rooflines on 2^20-agent states.
+        g = scenarios.rng(91)
+        n = n_kernel
+        stream = ctx.stream()
+        kern = {}
+        # follow path: agents over the funnel, random headings and speeds
+        pos = np.stack([g.uniform(-450, 450, n), np.zeros(n), g.uniform(-400, 900, n)], axis=1).astype(np.float32)
+        hd = g.uniform(0, 2 * np.pi, n)
+        fwd = np.stack([np.cos(hd), np.zeros(n), np.sin(hd)], axis=1)
+        st = scenarios.make_agents(pos, fwd, g.uniform(0, 3, n).astype(np.float32), 0.5, 3.0, 3.0)
+        A = gs.AgentGroup(ctx, st)
+        P = gs.PolylinePathway(ctx, sc.path_points, sc.path_radius, sc.path_cyclic)
+        ms = _events_ms(L, stream, lambda: gs.steerToFollowPath(A, P, p.path_prediction_time, 1.0, 0), reps)
+        kern["k_follow_path"] = (ms, bytes_follow_path())
+        P.close()
+        # agent anti-penetration: a dense crowd (4 per unit area), KNN to itself
+        half = float(np.sqrt(n / 4.0) / 2.0)
+        pos = np.stack([g.uniform(-half, half, n), np.zeros(n), g.uniform(-half, half, n)], axis=1).astype(np.float32)
+        A.upload(scenarios.make_agents(pos, fwd, np.zeros(n, np.float32), 0.5, 3.0, 3.0))
+        c = int(2 * half / 2.0)
+        bins = gs.KNNBinData(ctx, (c, 1, c), (2 * half + 2, 10.0, 2 * half + 2))
+        knn = gs.KNNData(ctx, n, p.knn)
+        gs.updateKNNDatabase(A, bins)
+        gs.findKNearestNeighbors(A, knn, bins, A, 1)
+        ms = _events_ms(L, stream, lambda: gs.antiPenetrationAgents(A, knn, A, 1 << 12), reps)
+        kern["k_anti_penetration_agents"] = (ms, bytes_anti_penetration_agents())
+        bins.close()
+        knn.close()
+        # integrate with wall anti-penetration: agents on the split walls (+-1 along the normal)
+        a = walls.arrays
+        w = g.integers(0, walls.n, n)
+        t = g.uniform(0, 1, n)
+        off = g.uniform(-1, 1, n)
+        s3 = np.stack([a["sx"][w], a["sy"][w], a["sz"][w]], 1)
+        e3 = np.stack([a["ex"][w], a["ey"][w], a["ez"][w]], 1)
+        n3 = np.stack([a["nx"][w], a["ny"][w], a["nz"][w]], 1)
+        pos = (s3 + (e3 - s3) * t[:, None] + n3 * off[:, None]).astype(np.float32)
+        A.upload(scenarios.make_agents(pos, fwd, g.uniform(0, 3, n).astype(np.float32), 0.5, 3.0, 3.0))
+        bins = gs.KNNBinData(ctx, p.cells, p.dimensions)
+        W = gs.WallGroup(ctx)
+        W.upload(walls, mid)
+        gs.updateKNNDatabase(W, bins)
+        gs.updateKNNDatabase(A, bins)
+        kw = gs.KNNData(ctx, n, p.knw)
+        gs.findKNearestNeighborsWithin(A, kw, bins, W, sc.wall_cap, 1)
+        ms = _events_ms(L, stream, lambda: gs.updateGroupWithAntiPenetration(A, kw, W, 1.0 / 30.0), reps)
+        kern["k_update_anti_penetration"] = (ms, bytes_update_anti_penetration(p.knw))
+        for h in (A, W, bins, kw):
+            h.close()
+    res["kernels"] = {}
+    for name, (ms, b) in kern.items():
+        gbs = n * b / (ms * 1e-3) / 1e9
+        res["kernels"][name] = {"n_agents": n, "ms": ms, "bytes_per_agent": b, "achieved_gbs": gbs,
+                                "peak_gbs": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS}
+        print(f"bench: {name} {ms * 1e3:.1f} us at {n} agents, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    return res
+
+
 def cpu_baseline(st, p, seconds: float):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as orc
@@ -301,7 +448,9 @@ def main() -> int:
     ap.add_argument("--calib-agents", type=int, default=16384)
     ap.add_argument("--calib-steps", type=int, default=100)
     ap.add_argument("--no-calib", action="store_true")
-    ap.add_argument("--only", choices=("ensemble", "adjoint"),
+    ap.add_argument("--plugin-steps", type=int, default=100)
+    ap.add_argument("--no-plugin", action="store_true")
+    ap.add_argument("--only", choices=("ensemble", "adjoint", "plugin"),
                     help="run only that leg and print its JSON object (for per-leg profiles)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 512 MiB device copy (the counter calibration for the traffic figure)")
@@ -316,6 +465,9 @@ def main() -> int:
 
     if args.only == "ensemble":
         print(json.dumps(ensemble_bench(args.ensemble_scenarios, args.ensemble_steps, 3)), flush=True)
+        return 0
+    if args.only == "plugin":
+        print(json.dumps(plugin_bench(args.plugin_steps, 5)), flush=True)
         return 0
     if args.only == "adjoint":
         print(json.dumps(calib_bench(args.calib_scenarios, args.calib_agents, args.calib_steps, 10)), flush=True)
@@ -428,6 +580,8 @@ def main() -> int:
         out["ensemble"] = ensemble_bench(args.ensemble_scenarios, args.ensemble_steps, 3)
     if not args.no_calib:
         out["adjoint"] = calib_bench(args.calib_scenarios, args.calib_agents, args.calib_steps, 10)
+    if not args.no_plugin:
+        out["plugin_compat"] = plugin_bench(args.plugin_steps, 5)
     if not args.no_cpu:
         print("bench: cpu baseline (oracle) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(st, p, args.cpu_seconds)

@@ -13,7 +13,9 @@ from .api import (  # noqa: F401
     KNNBinData,
     KNNData,
     ObstacleGroup,
+    PolylinePathway,
     WallGroup,
+    antiPenetrationAgents,
     clearAppliedKernels,
     findKNearestNeighbors,
     findKNearestNeighborsBruteForce,
@@ -25,9 +27,11 @@ from .api import (  # noqa: F401
     steerForSeparation,
     steerToAvoidObstacles,
     steerToAvoidWalls,
+    steerToFollowPath,
     stepBoids,
     stepBoidsTimed,
     updateGroup,
+    updateGroupWithAntiPenetration,
     updateKNNDatabase,
     wrapWorld,
 )

@@ -29,6 +29,7 @@ GS_MODE_GPUREF = 1
 GS_KIND_AGENTS, GS_KIND_OBSTACLES, GS_KIND_WALLS = 0, 1, 2
 GS_WRAP_NONE, GS_WRAP_BOX, GS_WRAP_SPHERE = -1, 0, 1
 GS_KNN_DEFAULT, GS_KNN_NO_SEED = 0, 1
+GS_CLOSEST_CLAMPED, GS_CLOSEST_AS_REFERENCE = 0, 1
 
 # Behaviour bits (include/OpenSteer/CUDA/CUDAKernelGlobals.cuh:12-31).
 BIT_PURSUE = 1 << 0
@@ -269,6 +270,13 @@ _SIGS = {
     "gs_steer_avoid_obstacles": (C.c_int, [_h, _h, _h, C.c_float, C.c_float, C.c_uint32]),
     "gs_steer_avoid_walls": (C.c_int, [_h, _h, _h, C.c_float, C.c_float, C.c_uint32]),
     "gs_update_group": (C.c_int, [_h, C.c_float, C.c_int]),
+    "gs_update_group_anti_penetration": (C.c_int, [_h, _h, _h, C.c_float, C.c_int]),
+    "gs_anti_penetration_agents": (C.c_int, [_h, _h, _h, C.c_uint32]),
+    "gs_path_create": (C.c_int, [_h, C.c_uint32, _fp, C.c_float, C.c_int, C.POINTER(_h)]),
+    "gs_path_info": (C.c_int, [_h, _up, _fp]),
+    "gs_path_get": (C.c_int, [_h, _fp, _fp, _fp]),
+    "gs_path_destroy": (C.c_int, [_h]),
+    "gs_steer_follow_path": (C.c_int, [_h, _h, C.c_float, C.c_float, C.c_uint32]),
     "gs_wrap_world": (C.c_int, [_h, C.c_int, _fp, _fp]),
     "gs_clear_bits": (C.c_int, [_h]),
     "gs_step_boids": (C.c_int, [_h, _h, _h, C.POINTER(GsBoidsParams), C.c_uint32]),

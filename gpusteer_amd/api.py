@@ -298,6 +298,61 @@ def updateGroup(A, elapsedTime, mode=MODE_OPENSTEER):
     check(capi.lib.gs_update_group(A.h, elapsedTime, mode), "gs_update_group")
 
 
+def updateGroupWithAntiPenetration(A, knn, walls, elapsedTime, closest=capi.GS_CLOSEST_CLAMPED):
+    """updateGroupWithAntiPenetration (GroupSteerLibrary.cuh:198): the GPU
+    integrate with wall anti-penetration over A's KNN to the walls; bits kept.
+    closest=GS_CLOSEST_AS_REFERENCE reproduces the reference's closest point
+    (always the wall's end, UpdateCUDAKernelNew.cu:79)."""
+    check(capi.lib.gs_update_group_anti_penetration(A.h, knn.h, walls.h, elapsedTime, closest),
+          "gs_update_group_anti_penetration")
+
+
+def antiPenetrationAgents(A, knn, other, doNotApplyWith=0):
+    """antiPenetrationAgents (GroupSteerLibrary.cuh:216): push apart from the
+    nearest agent of `other` (the first entry of A's KNN to it)."""
+    check(capi.lib.gs_anti_penetration_agents(A.h, knn.h, other.h, doNotApplyWith), "gs_anti_penetration_agents")
+
+
+class PolylinePathway:
+    """PolylinePathwayCUDA (CUDA/PolylinePathwayCUDA.cuh:11-68): points,
+    tube radius, cyclic flag; host copies of the derived arrays."""
+
+    def __init__(self, ctx: Context, points, radius: float, cyclic: bool = False):
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float32).reshape(-1, 3))
+        h = C.c_void_p()
+        check(capi.lib.gs_path_create(ctx.h, pts.shape[0], fptr(pts), radius, int(bool(cyclic)), C.byref(h)),
+              "gs_path_create")
+        self.h = h
+        m, total = C.c_uint32(), C.c_float()
+        check(capi.lib.gs_path_info(h, C.byref(m), C.byref(total)), "gs_path_info")
+        self.m, self.total, self.radius, self.cyclic = m.value, total.value, np.float32(radius), bool(cyclic)
+
+    def host_arrays(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(points[m][3], unit segment directions[m][3], lengths[m]) =
+        hvPoints / hvNormals / hvLengths."""
+        p = np.zeros((self.m, 3), np.float32)
+        d = np.zeros((self.m, 3), np.float32)
+        L = np.zeros(self.m, np.float32)
+        check(capi.lib.gs_path_get(self.h, fptr(p), fptr(d), fptr(L)), "gs_path_get")
+        return p, d, L
+
+    def close(self) -> None:
+        if self.h:
+            capi.lib.gs_path_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def steerToFollowPath(A, path: PolylinePathway, predictionTime, fWeight, doNotApplyWith=0):
+    check(capi.lib.gs_steer_follow_path(A.h, path.h, predictionTime, fWeight, doNotApplyWith),
+          "gs_steer_follow_path")
+
+
 def wrapWorld(A, worldSize=None, shape=capi.GS_WRAP_BOX, center=(0.0, 0.0, 0.0), radius=None):
     """Box: wrapWorld(A, worldSize) of the reference (centred box).  Sphere:
     shape=GS_WRAP_SPHERE, radius=R."""

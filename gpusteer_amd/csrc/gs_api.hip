@@ -751,6 +751,24 @@ static float cap2_of(float max_distance) {
     return std::nextafter(r2, 0.f);
 }
 
+// Whether the exhaustive scan of B's snapshot is cheaper than the binned
+// search (the result is the same exact one either way).
+static bool prefer_exhaustive(const gs_group_s* B, const GridDev& g, uint32_t cells, float max_distance) {
+    if (B->n <= kSmallTargetSet) return true;
+    if (B->n > kExhaustiveMax || (uint64_t)B->n * kSparseCells > cells) return false;
+    uint64_t box = cells;
+    if (!std::isinf(max_distance)) {
+        const float h[3] = {g.hx, g.hy, g.hz};
+        const int d[3] = {g.dx, g.dy, g.dz};
+        box = 1;
+        for (int a = 0; a < 3; ++a) {
+            const double r = std::ceil((double)max_distance / (double)h[a]);
+            box *= (uint64_t)std::min<double>((double)d[a], 2.0 * r + 1.0);
+        }
+    }
+    return box * 4 >= B->n;
+}
+
 static void find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t search_radius, uint32_t flags,
                      float max_distance) {
     check_group(A, "group A");
@@ -765,7 +783,7 @@ static void find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t 
         fail(GS_ERR_NOT_BINNED, "groups were binned with another lattice");
     const float cap2 = cap2_of(max_distance);
     use_device(A->ctx);
-    if (B->n <= kSmallTargetSet) {
+    if (prefer_exhaustive(B, bins->g, bins->cells, max_distance)) {
         launch_knn_snap_brute(A, out, B, cap2);
     } else {
         const Seed sd = seed_for(out, A, B, flags);
@@ -872,6 +890,130 @@ int gs_update_group(gs_group A, float dt, int mode) {
         if (mode != GS_MODE_OPENSTEER && mode != GS_MODE_GPUREF) fail(GS_ERR_INVALID_ARGUMENT, "unknown mode");
         use_device(A->ctx);
         launch_update(A, dt, mode);
+    });
+}
+
+int gs_update_group_anti_penetration(gs_group A, gs_knn knn, gs_group W, float dt, int closest_point) {
+    return guarded([&] {
+        check_group(A, "group A", GS_KIND_AGENTS);
+        check_group(W, "walls", GS_KIND_WALLS);
+        check(knn, "knn");
+        if (closest_point != GS_CLOSEST_CLAMPED && closest_point != GS_CLOSEST_AS_REFERENCE)
+            fail(GS_ERR_INVALID_ARGUMENT, "unknown closest-point rule");
+        require_knn_shape(knn, A);
+        use_device(A->ctx);
+        ensure_current(knn, A, W);
+        launch_update_anti_penetration(A, knn, W, dt, closest_point);
+    });
+}
+
+int gs_anti_penetration_agents(gs_group A, gs_knn knn, gs_group B, uint32_t mask) {
+    return guarded([&] {
+        check_group(A, "group A", GS_KIND_AGENTS);
+        check_group(B, "group B", GS_KIND_AGENTS);
+        check(knn, "knn");
+        require_knn_shape(knn, A);
+        if (B->ctx != A->ctx) fail(GS_ERR_INVALID_ARGUMENT, "groups belong to different contexts");
+        use_device(A->ctx);
+        ensure_current(knn, A, B);
+        launch_anti_penetration_agents(A, knn, B, mask);
+    });
+}
+
+// PolylinePathwayCUDA::Initialize (CUDA/PolylinePathwayCUDA.cu:17-60) on the
+// host, in float, in its order: m = n (+1 closing on the first point when
+// cyclic); per segment end i >= 1 the difference to the previous point, its
+// length, its normalized direction (unchanged when the length is 0) and the
+// running total.
+int gs_path_create(gs_context ctx, uint32_t n_points, const float* points, float radius, int cyclic, gs_path* out) {
+    return guarded([&] {
+        check(ctx, "context");
+        if (!out) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
+        if (n_points < 2 || !points) fail(GS_ERR_INVALID_ARGUMENT, "a path needs at least 2 points");
+        if (n_points > (1u << 20)) fail(GS_ERR_UNSUPPORTED, "more than 2^20 path points");
+        auto* p = new gs_path_s();
+        p->ctx = ctx;
+        p->radius = radius;
+        p->cyclic = cyclic ? 1 : 0;
+        const uint32_t m = n_points + (cyclic ? 1u : 0u);
+        p->m = m;
+        p->pts.assign((size_t)m * 3, 0.f);
+        p->nrm.assign((size_t)m * 3, 0.f);
+        p->len.assign(m, 0.f);
+        float total = 0.f;
+        for (uint32_t i = 0; i < m; ++i) {
+            const uint32_t j = (cyclic && i == m - 1) ? 0 : i;
+            for (int a = 0; a < 3; ++a) p->pts[3 * i + a] = points[3 * j + a];
+            if (i == 0) continue;
+            const float dx = p->pts[3 * i] - p->pts[3 * i - 3];
+            const float dy = p->pts[3 * i + 1] - p->pts[3 * i - 2];
+            const float dz = p->pts[3 * i + 2] - p->pts[3 * i - 1];
+            const float L = std::sqrt(((dx * dx) + (dy * dy)) + (dz * dz));
+            p->len[i] = L;
+            p->nrm[3 * i] = L > 0.f ? dx / L : dx;
+            p->nrm[3 * i + 1] = L > 0.f ? dy / L : dy;
+            p->nrm[3 * i + 2] = L > 0.f ? dz / L : dz;
+            total += L;
+        }
+        p->total = total;
+        std::vector<float> packed((size_t)7 * m);
+        for (uint32_t i = 0; i < m; ++i) {
+            for (int a = 0; a < 3; ++a) {
+                packed[(size_t)a * m + i] = p->pts[3 * i + a];
+                packed[(size_t)(3 + a) * m + i] = p->nrm[3 * i + a];
+            }
+            packed[(size_t)6 * m + i] = p->len[i];
+        }
+        try {
+            use_device(ctx);
+            p->dev.reserve(packed.size());
+            h2d(p->dev.p, packed.data(), packed.size() * sizeof(float), ctx->stream);
+            GS_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            p->dev.release();
+            delete p;
+            throw;
+        }
+        reg_add(p);
+        *out = p;
+    });
+}
+
+int gs_path_info(gs_path p, uint32_t* n_points, float* total_length) {
+    return guarded([&] {
+        check(p, "path");
+        if (n_points) *n_points = p->m;
+        if (total_length) *total_length = p->total;
+    });
+}
+
+int gs_path_get(gs_path p, float* points, float* directions, float* lengths) {
+    return guarded([&] {
+        check(p, "path");
+        if (points) memcpy(points, p->pts.data(), p->pts.size() * sizeof(float));
+        if (directions) memcpy(directions, p->nrm.data(), p->nrm.size() * sizeof(float));
+        if (lengths) memcpy(lengths, p->len.data(), p->len.size() * sizeof(float));
+    });
+}
+
+int gs_path_destroy(gs_path p) {
+    return guarded([&] {
+        check(p, "path");
+        use_device(p->ctx);
+        (void)hipStreamSynchronize(p->ctx->stream);
+        p->dev.release();
+        reg_del(p);
+        delete p;
+    });
+}
+
+int gs_steer_follow_path(gs_group A, gs_path path, float prediction_time, float w, uint32_t mask) {
+    return guarded([&] {
+        check_group(A, "group A", GS_KIND_AGENTS);
+        check(path, "path");
+        if (path->ctx != A->ctx) fail(GS_ERR_INVALID_ARGUMENT, "path belongs to another context");
+        use_device(A->ctx);
+        launch_follow_path(A, path, prediction_time, w, mask);
     });
 }
 

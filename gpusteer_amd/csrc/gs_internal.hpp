@@ -9,6 +9,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/gpusteer.h"
 
@@ -278,6 +279,19 @@ struct gs_knn_s {
     float* dist_alt() const { return distb[cur ^ 1].p; }
 };
 
+// A polyline path (PolylinePathwayCUDA, CUDA/PolylinePathwayCUDA.cuh:11-68):
+// host copies and one device buffer holding 7 arrays of m floats (point x, y,
+// z, segment direction x, y, z, segment length; entry 0 of the last four is 0).
+struct gs_path_s {
+    gs_context ctx = nullptr;
+    uint32_t m = 0;
+    float radius = 0.f, total = 0.f;
+    int cyclic = 0;
+    std::vector<float> pts, nrm, len;  // [m][3], [m][3], [m]
+    gs::DevBuf<float> dev;
+    uint32_t magic = 0x67737061u;
+};
+
 namespace gs {
 // Field pointers of one reorder (current -> alternate buffers).
 struct FieldSet {
@@ -334,6 +348,13 @@ void launch_knn_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B, float cap2);
 // exhaustive scan, for small B.
 void launch_knn_snap_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B, float cap2);
 constexpr uint32_t kSmallTargetSet = 256;
+// A sparse target set (fewer than one point per kSparseCells cells, at most
+// kExhaustiveMax points) is also scanned exhaustively when a search that
+// finds nothing would cover more than nB / 4 cells (the cap's box, or the
+// whole lattice without a cap): nB distance tests beat the shell walk over
+// empty cells.
+constexpr uint32_t kExhaustiveMax = 4096;
+constexpr uint32_t kSparseCells = 16;
 // Rewrites out's contents from the previous storage order of A / B to the
 // current one (into the alternate buffer; caller flips).
 void launch_knn_remap(gs_knn_s* out, uint32_t nA, const uint32_t* row_old_to_new, const uint32_t* val_old_to_new);
@@ -343,6 +364,9 @@ void launch_seek(gs_group_s* A, const float* target, float w, uint32_t mask, int
 void launch_avoid_obstacles(gs_group_s* A, gs_group_s* O, gs_knn_s* knn, float min_time, float w, uint32_t mask);
 void launch_avoid_walls(gs_group_s* A, gs_knn_s* knn, gs_group_s* W, float min_time, float w, uint32_t mask);
 void launch_update(gs_group_s* A, float dt, int mode);
+void launch_update_anti_penetration(gs_group_s* A, gs_knn_s* knn, gs_group_s* W, float dt, int closest);
+void launch_anti_penetration_agents(gs_group_s* A, gs_knn_s* knn, gs_group_s* B, uint32_t mask);
+void launch_follow_path(gs_group_s* A, const gs_path_s* p, float prediction_time, float w, uint32_t mask);
 void launch_wrap(gs_group_s* A, int shape, const float* c, const float* s);
 void launch_clear_bits(gs_group_s* A);
 void launch_check_finite(gs_group_s* A, uint32_t* first_bad_slot_host);

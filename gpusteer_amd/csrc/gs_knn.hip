@@ -111,35 +111,77 @@ __global__ __launch_bounds__(kBlock) void k_knn_brute(uint32_t nA, const float* 
     if (active) h.store(k, nA, q, out_idx, out_dist);
 }
 
-// Binned-database semantics over a small B (a few walls or obstacles): the
-// same queries and candidates as k_knn_binned (the position snapshots of the
-// last builds, agent indices in .w) scanned exhaustively from LDS, so a small
-// target set costs nB distance tests instead of a shell expansion over empty
-// cells.  Identical results (the search is exact either way).
+// Binned-database semantics over a small or sparse B (walls, obstacles):
+// the same queries and candidates as k_knn_binned (the position snapshots of
+// the last builds, agent indices in .w) scanned exhaustively from LDS, so a
+// small target set costs nB distance tests instead of a shell expansion over
+// empty cells.  Identical results (the search is exact either way).
+//
+// The snapshot is in cell order, so each wave's 64 entries of a tile are
+// spatially coherent: the wave takes their bounding box, and a wave skips a
+// chunk when no lane's query can reach the box within its current bound
+// (its k-th key's d2, or the cap).  The box distance is formed with the
+// same per-axis subtractions and the same summation order as a candidate's
+// d2, and every step is monotone, so a member's d2 is never below it: the
+// skip drops only candidates the heap would reject.
 template <int KMAX>
 __global__ __launch_bounds__(kBlock) void k_knn_snap_brute(uint32_t nA, const float4* __restrict__ apos, uint32_t nB,
                                                            const float4* __restrict__ bpos, int self, uint32_t k,
                                                            float cap2, uint32_t* __restrict__ out_idx,
                                                            float* __restrict__ out_dist) {
     __shared__ float4 tile[kBlock];
+    __shared__ float box[kBlock / 64][6];
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = s < nA;
     const float4 q = active ? apos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     TopK<KMAX> h;
     h.init(k);
     for (uint32_t base = 0; base < nB; base += kBlock) {
         const uint32_t t = base + threadIdx.x;
-        if (t < nB) tile[threadIdx.x] = bpos[t];
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        if (t < nB) {
+            const float4 p = bpos[t];
+            tile[threadIdx.x] = p;
+            lo[0] = hi[0] = p.x;
+            lo[1] = hi[1] = p.y;
+            lo[2] = hi[2] = p.z;
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+                hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                box[wave][a] = lo[a];
+                box[wave][3 + a] = hi[a];
+            }
+        }
         __syncthreads();
         const uint32_t m = min((uint32_t)kBlock, nB - base);
-        if (active) {
-            for (uint32_t j = 0; j < m; ++j) {
-                const uint32_t b = base + j;
-                if (self && b == s) continue;
-                const float4 p = tile[j];
-                const float d2 = sqdist4(q.x, q.y, q.z, p);
-                const uint64_t kk = make_key(d2, id_of(p));
-                if (d2 <= cap2 && h.accepts(kk)) h.push(kk, b);
+        for (uint32_t c0 = 0; c0 < m; c0 += 64) {
+            const int c = c0 >> 6;
+            const float gx = fmaxf(fmaxf(box[c][0] - q.x, q.x - box[c][3]), 0.f);
+            const float gy = fmaxf(fmaxf(box[c][1] - q.y, q.y - box[c][4]), 0.f);
+            const float gz = fmaxf(fmaxf(box[c][2] - q.z, q.z - box[c][5]), 0.f);
+            const float d2box = ((gx * gx) + (gy * gy)) + (gz * gz);
+            const bool need = active && d2box <= fminf(cap2, h.worst());
+            if (__ballot(need) == 0) continue;
+            const uint32_t c1 = min(c0 + 64, m);
+            if (need) {
+                for (uint32_t j = c0; j < c1; ++j) {
+                    const uint32_t b = base + j;
+                    if (self && b == s) continue;
+                    const float4 p = tile[j];
+                    const float d2 = sqdist4(q.x, q.y, q.z, p);
+                    const uint64_t kk = make_key(d2, id_of(p));
+                    if (d2 <= cap2 && h.accepts(kk)) h.push(kk, b);
+                }
             }
         }
         __syncthreads();

@@ -343,3 +343,90 @@ def calib_scenarios(n_scenarios: int = 64, n_agents: int = 16384, gap: float = 1
         sc.mode = 1
         specs.append(("choke", sc))
     return specs
+
+
+# ---------------------------------------------------------------------------
+# The choke-point plugin of plugins/ChokePointNew.cpp driven by its data files.
+
+
+@dataclass
+class ChokeNewScenario:
+    """One group of CtfGroup (plugins/ChokePointNew.cpp:203-491) set up from a
+    parsed .params file and its split walls: agents, the group's path, the
+    world lattice and the per-step constants of CtfGroup::update
+    (:587-630)."""
+    state: AgentState
+    walls: WallState
+    wall_mid: np.ndarray
+    path_points: np.ndarray        # [n][3] as given (PolylinePathwayCUDA adds the closing point)
+    path_radius: float
+    path_cyclic: bool
+    seek_target: Tuple[float, float, float]   # m_pPath->hvPoints()[1] (:622)
+    world_size: Tuple[float, float, float]
+    world_cells: Tuple[int, int, int]
+    params: object                 # formats.SimParams
+    dt: float
+    # Wall KNN range: walls farther than this cannot touch a feeler, the
+    # agent's radius or its step (the KNN to walls is capped there; the
+    # forces equal the uncapped search's).
+    wall_cap: float
+
+
+def choke_point_new(params, walls: WallState, wall_mid: np.ndarray, n_agents: int = None, dt: float = 0.1,
+                    start=None, max_start_radius: float = None, workload: int = 1) -> ChokeNewScenario:
+    """Agents of the params' first group placed as CtfGroup::reset does
+    (:445-491, randomizeStartingPositionAndHeading2D of
+    plugins/PluginCommon.cpp:5-19): radius uniform in [min, max] start radius
+    on a random XZ direction around the start position, random heading on
+    the XZ plane, speed 0, radius 0.5, mass 1 (SimpleVehicle's defaults), no
+    two agents closer than one radius (redrawn, as the reference does).  The
+    draws are this module's own seeded stream, not the reference's rand().
+    n_agents / start / max_start_radius override the file's values (smaller
+    test crowds)."""
+    gp = params.groups[0]
+    n = gp.num_agents if n_agents is None else int(n_agents)
+    c = np.array(gp.start_position if start is None else start, np.float64)
+    r_lo = gp.min_start_radius
+    r_hi = gp.max_start_radius if max_start_radius is None else max_start_radius
+    g = rng(workload, 7)
+    radius = 0.5
+    cell = {}
+    pos = np.zeros((n, 3), np.float32)
+    filled = 0
+    while filled < n:
+        m = n - filled
+        rr = g.uniform(r_lo, r_hi, size=m)
+        th = g.uniform(0.0, 2.0 * np.pi, size=m)
+        cand = np.stack([c[0] + rr * np.cos(th), np.full(m, c[1]), c[2] + rr * np.sin(th)], axis=1).astype(np.float32)
+        for p in cand:
+            key = (int(np.floor(p[0] / radius)), int(np.floor(p[2] / radius)))
+            ok = True
+            for dx in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for q in cell.get((key[0] + dx, key[1] + dz), ()):
+                        if (p[0] - q[0]) ** 2 + (p[2] - q[2]) ** 2 < radius * radius:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if not ok:
+                continue
+            cell.setdefault(key, []).append(p)
+            pos[filled] = p
+            filled += 1
+            if filled == n:
+                break
+    hd = g.uniform(0.0, 2.0 * np.pi, size=n)
+    fwd = np.stack([np.cos(hd), np.zeros(n), np.sin(hd)], axis=1)
+    st = make_agents(pos, fwd, np.zeros(n, np.float32), radius, gp.max_speed, gp.max_force, 1.0)
+    pts = np.asarray(gp.path, np.float32).reshape(-1, 3)
+    a = walls.arrays
+    half = 0.5 * np.sqrt((a["ex"] - a["sx"]) ** 2 + (a["ey"] - a["sy"]) ** 2 + (a["ez"] - a["sz"]) ** 2)
+    reach = max(gp.max_speed * params.min_time_to_wall, 4.0 * np.sqrt(2.0) * radius, gp.max_speed * dt + radius)
+    cap = float(reach + (half.max() if walls.n else 0.0) + 1.0)
+    return ChokeNewScenario(state=st, walls=walls, wall_mid=np.asarray(wall_mid, np.float32), path_points=pts,
+                            path_radius=float(gp.path_radius), path_cyclic=bool(gp.path_cyclic),
+                            seek_target=tuple(float(v) for v in pts[1]), world_size=tuple(params.dimensions),
+                            world_cells=tuple(params.cells), params=params, dt=dt, wall_cap=cap)

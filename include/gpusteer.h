@@ -24,6 +24,12 @@
  *   gs_update_group          GSL:189  updateGroup(A, elapsedTime)
  *   gs_wrap_world            GSL:43   wrapWorld(A, float3 worldSize)
  *   gs_clear_bits            plugins/ChokePointNew.cpp:595 (cudaMemset of pdAppliedKernels)
+ *   gs_update_group_anti_penetration
+ *                            GSL:198  updateGroupWithAntiPenetration(A, KNNData*, WallGroup*, elapsedTime)
+ *   gs_anti_penetration_agents
+ *                            GSL:216  antiPenetrationAgents(A, KNNData*, B, mask)
+ *   gs_steer_follow_path     GSL:88   steerToFollowPath(A, PolylinePathwayCUDA*, predictionTime, w, mask)
+ *   gs_path_*                include/OpenSteer/CUDA/PolylinePathwayCUDA.cuh:11-68
  *   gs_group_*               include/OpenSteer/AgentGroup.h:32-54, ObstacleGroup.h:23-37,
  *                            WallGroup.h:18-33 (AddAgent/SyncDevice/GetDataForAgent/pd*)
  *   gs_knn_*                 include/OpenSteer/CUDA/KNNData.cuh:31-54
@@ -228,6 +234,43 @@ int gs_steer_avoid_obstacles(gs_group A, gs_group obstacles, gs_knn knn, float m
 int gs_steer_avoid_walls(gs_group A, gs_knn knn, gs_group walls, float min_time_to_collision, float weight,
                          uint32_t mask);
 int gs_update_group(gs_group A, float elapsed_time, int mode);
+
+/* ---- plugin-compat kernels (the choke-point plugins' per-step extras) ----- */
+/* Closest point of a wall segment for the within-radius test of
+ * gs_update_group_anti_penetration.  The reference's ClosestPointOnLine
+ * (CUDA/UpdateCUDAKernelNew.cu:73-81) swaps clamp's arguments and always
+ * returns the point start + (end - start) * 1: GS_CLOSEST_AS_REFERENCE keeps
+ * that; GS_CLOSEST_CLAMPED clamps the segment parameter into [0, 1]. */
+enum { GS_CLOSEST_CLAMPED = 0, GS_CLOSEST_AS_REFERENCE = 1 };
+/* UpdateCUDAKernelNew (CUDA/UpdateCUDAKernelNew.cu:95-268): gs_update_group's
+ * GPU-reference integrate with wall anti-penetration over the k nearest walls
+ * (knn: A to walls).  A wall within the radius moves the agent out along its
+ * normal; crossing a wall during the step removes the velocity's component
+ * along that wall's normal; either sets GS_BIT_ANTI_PENETRATION_WALL.
+ * Steering is zeroed; unlike gs_update_group the bits are kept. */
+int gs_update_group_anti_penetration(gs_group A, gs_knn knn, gs_group walls, float elapsed_time, int closest_point);
+/* AntiPenetrationAgentsCUDAKernel (CUDA/AntiPenetrationAgentsCUDAKernel.cu:57-109):
+ * each agent of A without a masked bit whose nearest agent of B (the first
+ * KNN entry, at the KNN's distance) overlaps it moves apart by the overlap
+ * (the whole of it when the other agent has a masked bit, else half) and
+ * gets GS_BIT_ANTI_PENETRATION_AGENT.  Reads the state before the call, so B
+ * may be A.  Allocates nothing per call. */
+int gs_anti_penetration_agents(gs_group A, gs_knn knn, gs_group B, uint32_t mask);
+/* Polyline path (PolylinePathwayCUDA): n_points xyz triples (>= 2), tube
+ * radius; cyclic closes it on the first point. */
+typedef struct gs_path_s* gs_path;
+int gs_path_create(gs_context ctx, uint32_t n_points, const float* points, float radius, int cyclic, gs_path* out);
+/* m = stored points (n_points + 1 when cyclic) and the total length. */
+int gs_path_info(gs_path p, uint32_t* m, float* total_length);
+/* Host copies (hvPoints / hvNormals / hvLengths): points[m][3], unit
+ * segment directions[m][3] and lengths[m] of the segment ending at each point
+ * (entry 0 is 0).  Any output may be NULL. */
+int gs_path_get(gs_path p, float* points, float* directions, float* lengths);
+int gs_path_destroy(gs_path p);
+/* FollowPathCUDAKernel (CUDA/SteerToFollowPathCUDAKernel.cu:189-287): seek
+ * the point prediction_time * speed ahead along the path unless the predicted
+ * position is inside the tube and moving forward; sets GS_BIT_FOLLOW_PATH. */
+int gs_steer_follow_path(gs_group A, gs_path path, float prediction_time, float weight, uint32_t mask);
 /* GS_WRAP_BOX: size_or_radius = box size (centred at center);
  * GS_WRAP_SPHERE: size_or_radius[0] = radius. */
 int gs_wrap_world(gs_group A, int shape, const float center[3], const float size_or_radius[3]);

@@ -68,6 +68,10 @@ public:
     void Synchronize() { check(gs_context_synchronize(h_), "gs_context_synchronize"); }
     int Mode() const { return mode_; }
     void SetMode(int mode) { mode_ = mode; }
+    // Closest-point rule of updateGroupWithAntiPenetration: GS_CLOSEST_CLAMPED
+    // (default) or GS_CLOSEST_AS_REFERENCE (the reference's wall-end point).
+    int ClosestPoint() const { return closest_; }
+    void SetClosestPoint(int rule) { closest_ = rule; }
     // The context used by constructors that are not given one.
     static Context& Default() {
         static Context ctx(0, nullptr);
@@ -77,6 +81,7 @@ public:
 private:
     gs_context h_ = nullptr;
     int mode_ = GS_MODE_GPUREF;
+    int closest_ = GS_CLOSEST_CLAMPED;
 };
 
 }  // namespace gpusteer
@@ -541,6 +546,85 @@ inline void steerForCohesion(AgentGroup* pAgentGroup, KNNData* pKNNData, AgentGr
 inline void updateGroup(AgentGroup* pAgentGroup, const float elapsedTime) {
     gpusteer::check(gs_update_group(pAgentGroup->device_handle(), elapsedTime, pAgentGroup->context().Mode()),
                     "gs_update_group");
+    pAgentGroup->SetSyncHost();
+}
+
+// PolylinePathwayCUDA (CUDA/PolylinePathwayCUDA.cuh:11-68): the path's
+// derived arrays live in the library (host copies and device buffer).
+class PolylinePathwayCUDA {
+public:
+    PolylinePathwayCUDA(std::vector<float3> const& points, float const radius, bool const cyclic,
+                        gpusteer::Context& ctx = gpusteer::Context::Default())
+        : ctx_(&ctx) {
+        Initialize(points, radius, cyclic);
+    }
+    ~PolylinePathwayCUDA() { gs_path_destroy(h_); }
+    PolylinePathwayCUDA(const PolylinePathwayCUDA&) = delete;
+    PolylinePathwayCUDA& operator=(const PolylinePathwayCUDA&) = delete;
+    void Initialize(std::vector<float3> const& points, float const radius, bool const cyclic) {
+        std::vector<float> xyz(3 * points.size());
+        for (size_t i = 0; i < points.size(); ++i) {
+            xyz[3 * i] = points[i].x;
+            xyz[3 * i + 1] = points[i].y;
+            xyz[3 * i + 2] = points[i].z;
+        }
+        gs_path p = nullptr;
+        gpusteer::check(gs_path_create(ctx_->handle(), (uint32_t)points.size(), xyz.data(), radius, cyclic ? 1 : 0, &p),
+                        "gs_path_create");
+        if (h_) gs_path_destroy(h_);
+        h_ = p;
+        radius_ = radius;
+        cyclic_ = cyclic;
+        gpusteer::check(gs_path_info(h_, &count_, &total_), "gs_path_info");
+        std::vector<float> pt(3 * count_), nm(3 * count_);
+        lengths_.assign(count_, 0.f);
+        gpusteer::check(gs_path_get(h_, pt.data(), nm.data(), lengths_.data()), "gs_path_get");
+        points_.resize(count_);
+        normals_.resize(count_);
+        for (uint i = 0; i < count_; ++i) {
+            points_[i] = make_float3(pt[3 * i], pt[3 * i + 1], pt[3 * i + 2]);
+            normals_[i] = make_float3(nm[3 * i], nm[3 * i + 1], nm[3 * i + 2]);
+        }
+    }
+    std::vector<float3> const& hvPoints() const { return points_; }
+    std::vector<float3> const& hvNormals() const { return normals_; }
+    std::vector<float> const& hvLengths() const { return lengths_; }
+    uint const& numPoints() const { return count_; }
+    float const& radius() const { return radius_; }
+    bool const& cyclic() const { return cyclic_; }
+    float const& totalPathLength() const { return total_; }
+    gs_path handle() const { return h_; }
+
+private:
+    gpusteer::Context* ctx_;
+    gs_path h_ = nullptr;
+    uint count_ = 0;
+    float total_ = 0.f, radius_ = 0.f;
+    bool cyclic_ = false;
+    std::vector<float3> points_, normals_;
+    std::vector<float> lengths_;
+};
+
+inline void steerToFollowPath(AgentGroup* pAgentGroup, PolylinePathwayCUDA* pPath, float const predictionTime,
+                              float const fWeight, uint const doNotApplyWith) {
+    gpusteer::check(gs_steer_follow_path(pAgentGroup->device_handle(), pPath->handle(), predictionTime, fWeight,
+                                         doNotApplyWith),
+                    "gs_steer_follow_path");
+    pAgentGroup->SetSyncHost();
+}
+inline void updateGroupWithAntiPenetration(AgentGroup* pAgentGroup, KNNData* pKNNData, WallGroup* pWallGroup,
+                                           const float elapsedTime) {
+    gpusteer::check(gs_update_group_anti_penetration(pAgentGroup->device_handle(), pKNNData->handle(),
+                                                     pWallGroup->device_handle(), elapsedTime,
+                                                     pAgentGroup->context().ClosestPoint()),
+                    "gs_update_group_anti_penetration");
+    pAgentGroup->SetSyncHost();
+}
+inline void antiPenetrationAgents(AgentGroup* pAgentGroup, KNNData* pKNNData, AgentGroup* pOtherGroup,
+                                  uint const doNotApplyWith) {
+    gpusteer::check(gs_anti_penetration_agents(pAgentGroup->device_handle(), pKNNData->handle(),
+                                               pOtherGroup->device_handle(), doNotApplyWith),
+                    "gs_anti_penetration_agents");
     pAgentGroup->SetSyncHost();
 }
 

@@ -334,7 +334,9 @@ enum {
     BIT_COHESION = 1u << 6,
     BIT_AVOID_OBSTACLES = 1u << 7,
     BIT_AVOID_WALLS = 1u << 8,
+    BIT_FOLLOW_PATH = 1u << 11,
     BIT_ANTI_PENETRATION_WALL = 1u << 12,
+    BIT_ANTI_PENETRATION_AGENT = 1u << 13,
 };
 
 // Applies a weighted behaviour result with the reference kernels' mask / bit /
@@ -645,6 +647,275 @@ template <class T> static void wrap_world(int shape, Agents<T>& A, int nA, const
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Plugin-compat kernels (the choke-point plugins' per-step extras).
+
+// ClosestPointOnLine (include/OpenSteer/CUDA/UpdateCUDAKernelNew.cu:73-81).
+// The reference calls clamp(0.f, 1.f, t) with its arguments in the wrong
+// order (:79 against the clamp(x, a, b) of :67-71), which returns 1 for every
+// t: its "closest point" is always start + v * 1.  CLOSEST_AS_REFERENCE keeps
+// that; CLOSEST_CLAMPED clamps t into [0, 1] (the closest point of the
+// segment, the evident intent).
+enum { CLOSEST_CLAMPED = 0, CLOSEST_AS_REFERENCE = 1 };
+template <class T> static inline V3<T> closest_point_on_line(V3<T> p, V3<T> s, V3<T> e, int closest) {
+    const V3<T> v = sub(e, s);
+    const V3<T> w = sub(p, s);
+    T t = dot(w, v) / dot(v, v);
+    if (closest == CLOSEST_AS_REFERENCE) t = T(1);
+    else t = t < T(0) ? T(0) : (t > T(1) ? T(1) : t);
+    return add(s, mul(v, t));
+}
+
+// UpdateCUDAKernelNew (include/OpenSteer/CUDA/UpdateCUDAKernelNew.cu:95-268):
+// the GPU-reference integrate with wall anti-penetration between the speed
+// limit and the Euler step.  Over the k nearest walls in KNN order (stopping
+// at a pad): the nearest crossing of the segment position -> position +
+// v dt (LinesIntersect, strict <) and the LAST wall whose closest point lies
+// within the radius (no distance comparison, :215-221).  A wall within the
+// radius moves the agent by -normal * (d - r); a crossing replaces the
+// velocity by its component perpendicular to that wall's normal; either sets
+// the ANTI_PENETRATION_WALL bit.  Steering is zeroed; the bits are NOT cleared
+// (:153 is commented out).  The KNN is read slot-major [k][n] as written by
+// the search (the reference reads it agent-major, :150, which mixes agents).
+// At speed 0 the side is left unchanged (the reference writes shared memory
+// it never loaded, :155,261).
+template <class T>
+static void update_anti_penetration(Agents<T>& A, int nA, const uint32_t* knn, int k, int nW, const T* wsx,
+                                    const T* wsy, const T* wsz, const T* wex, const T* wey, const T* wez, const T* wnx,
+                                    const T* wny, const T* wnz, T dt, int closest) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nA; ++i) {
+        V3<T> pos = P(A, i), fwd = F(A, i), side = S(A, i), up = U(A, i);
+        const T radius = A.radius[i];
+        V3<T> force = truncate_length(G(A, i), A.max_force[i]);
+        const V3<T> acc = dvd(force, A.mass[i]);
+        V3<T> new_vel = add(mul(fwd, A.speed[i]), mul(acc, dt));
+        new_vel = truncate_length(new_vel, A.max_speed[i]);
+        const V3<T> future = add(pos, mul(new_vel, dt));
+        T line_d = T(FLT_MAX);
+        uint32_t line = UINT32_MAX;
+        V3<T> line_n = mk(T(0), T(0), T(0));
+        T col_d = T(FLT_MAX);
+        uint32_t col = UINT32_MAX;
+        V3<T> col_n = line_n;
+        for (int j = 0; j < k; ++j) {
+            const uint32_t l = knn[(size_t)j * nA + i];
+            if (l >= (uint32_t)nW) break;
+            const V3<T> ls = mk(wsx[l], wsy[l], wsz[l]);
+            const V3<T> le = mk(wex[l], wey[l], wez[l]);
+            const V3<T> ln = mk(wnx[l], wny[l], wnz[l]);
+            V3<T> ip;
+            if (lines_intersect(pos, future, ls, le, ip)) {
+                const T d = distance(pos, ip);
+                if (d < col_d) {
+                    col_d = d;
+                    col = l;
+                    col_n = ln;
+                }
+            }
+            const V3<T> cp = sub(closest_point_on_line(pos, ls, le, closest), pos);
+            const T d2 = dot(cp, cp);
+            if (d2 < radius * radius) {
+                line_d = std::sqrt(d2);
+                line = l;
+                line_n = ln;
+            }
+        }
+        if (line != UINT32_MAX) {
+            pos = add(pos, mul(neg(line_n), line_d - radius));
+            A.bits[i] |= BIT_ANTI_PENETRATION_WALL;
+        }
+        if (col != UINT32_MAX) {
+            new_vel = perpendicular_component(new_vel, col_n);
+            A.bits[i] |= BIT_ANTI_PENETRATION_WALL;
+        }
+        const T speed = len(new_vel);
+        if (speed > T(0)) {
+            fwd = dvd(new_vel, speed);
+            side = normalize(cross(fwd, up));
+            up = cross(side, fwd);
+        }
+        pos = add(pos, mul(new_vel, dt));
+        A.px[i] = pos.x; A.py[i] = pos.y; A.pz[i] = pos.z;
+        A.fx[i] = fwd.x; A.fy[i] = fwd.y; A.fz[i] = fwd.z;
+        A.sx[i] = side.x; A.sy[i] = side.y; A.sz[i] = side.z;
+        A.ux[i] = up.x; A.uy[i] = up.y; A.uz[i] = up.z;
+        A.speed[i] = speed;
+        A.gx[i] = T(0); A.gy[i] = T(0); A.gz[i] = T(0);
+    }
+}
+
+// AntiPenetrationAgentsCUDAKernel (include/OpenSteer/CUDA/AntiPenetrationAgentsCUDAKernel.cu:57-109)
+// with its launcher (AntiPenetrationAgentsCUDA.cu:47-96): every agent of A
+// not masked, whose first KNN entry is an agent of B overlapping it
+// (r_a + r_b - knn distance >= 0), moves away from that agent's position
+// along the normalized offset by the whole overlap when the other agent's
+// bits match the mask, else by half; ANTI_PENETRATION_AGENT bit set.  All
+// reads (A's and B's positions and bits) are of the state before the call,
+// also when A is B (the reference reads B through textures while writing a
+// copy of A's positions).
+template <class T>
+static void anti_penetration_agents(Agents<T>& A, int nA, const uint32_t* knn, const float* knn_dist,
+                                    const Agents<T>& B, int nB, uint32_t mask) {
+    std::vector<T> ox(nA), oy(nA), oz(nA);
+    std::vector<uint32_t> ob(nA);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nA; ++i) {
+        V3<T> pos = P(A, i);
+        uint32_t bits = A.bits[i];
+        const uint32_t j = knn[i];
+        if (!(bits & mask) && j < (uint32_t)nB) {
+            const T overlap = (A.radius[i] + B.radius[j]) - T(knn_dist[i]);
+            if (!(overlap < T(0))) {
+                const V3<T> off = normalize(sub(pos, P(B, j)));
+                const T move = (B.bits[j] & mask) ? overlap : T(0.5f) * overlap;
+                pos = add(pos, mul(off, move));
+                bits |= BIT_ANTI_PENETRATION_AGENT;
+            }
+        }
+        ox[i] = pos.x; oy[i] = pos.y; oz[i] = pos.z;
+        ob[i] = bits;
+    }
+    for (int i = 0; i < nA; ++i) {
+        A.px[i] = ox[i]; A.py[i] = oy[i]; A.pz[i] = oz[i];
+        A.bits[i] = ob[i];
+    }
+}
+
+// PolylinePathwayCUDA::Initialize (include/OpenSteer/CUDA/PolylinePathwayCUDA.cu:17-60):
+// m = n (+1 when cyclic, closing on the first point); per segment end i >= 1
+// the unit direction normals[i], the length lengths[i] and the running total;
+// entry 0 of both stays zero.  Returns the total length.
+template <class T>
+static T path_init(int n, const T* pts_in, int cyclic, T* pts, T* nrm, T* lens) {
+    const int m = n + (cyclic ? 1 : 0);
+    T total = T(0);
+    for (int i = 0; i < m; ++i) {
+        const int j = (cyclic && i == m - 1) ? 0 : i;
+        for (int a = 0; a < 3; ++a) pts[3 * i + a] = pts_in[3 * j + a];
+        if (i > 0) {
+            const V3<T> d = sub(mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]),
+                                mk(pts[3 * i - 3], pts[3 * i - 2], pts[3 * i - 1]));
+            lens[i] = len(d);
+            const V3<T> u = normalize(d);
+            nrm[3 * i] = u.x; nrm[3 * i + 1] = u.y; nrm[3 * i + 2] = u.z;
+            total += lens[i];
+        } else {
+            nrm[0] = nrm[1] = nrm[2] = T(0);
+            lens[0] = T(0);
+        }
+    }
+    return total;
+}
+
+template <class T> struct Path {
+    int m;
+    const T *pts, *nrm, *lens;
+    T radius, total;
+    int cyclic;
+    V3<T> pt(int i) const { return mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]); }
+    V3<T> nm(int i) const { return mk(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]); }
+};
+
+// pointToSegmentDistance (CUDA/SteerToFollowPathCUDAKernel.cu:34-69).
+template <class T>
+static inline T point_to_segment(V3<T> p, V3<T> e0, V3<T> e1, V3<T> n, T seg_len, T& proj, V3<T>& chosen) {
+    const V3<T> local = sub(p, e0);
+    proj = dot(n, local);
+    if (proj < T(0)) {
+        chosen = e0;
+        proj = T(0);
+        return distance(p, e0);
+    }
+    if (proj > seg_len) {
+        chosen = e1;
+        proj = seg_len;
+        return distance(p, e1);
+    }
+    chosen = add(e0, mul(n, proj));
+    return distance(p, chosen);
+}
+
+// mapPointToPathDistance (:71-102).
+template <class T> static T path_distance_of(const Path<T>& P_, V3<T> p) {
+    T min_d = T(FLT_MAX), total = T(0), along = T(0);
+    for (int i = 1; i < P_.m; ++i) {
+        T proj;
+        V3<T> chosen;
+        const T d = point_to_segment(p, P_.pt(i - 1), P_.pt(i), P_.nm(i), P_.lens[i], proj, chosen);
+        if (d < min_d) {
+            min_d = d;
+            along = total + proj;
+        }
+        total += P_.lens[i];
+    }
+    return along;
+}
+
+// mapPointToPath (:104-142): the nearest point on the path and how far p is
+// outside the tube.
+template <class T> static V3<T> map_point_to_path(const Path<T>& P_, V3<T> p, T& outside) {
+    T min_d = T(FLT_MAX);
+    V3<T> on = mk(T(0), T(0), T(0));
+    for (int i = 1; i < P_.m; ++i) {
+        T proj;
+        V3<T> chosen;
+        const T d = point_to_segment(p, P_.pt(i - 1), P_.pt(i), P_.nm(i), P_.lens[i], proj, chosen);
+        if (d < min_d) {
+            min_d = d;
+            on = chosen;
+        }
+    }
+    outside = distance(on, p) - P_.radius;
+    return on;
+}
+
+// mapPathDistanceToPoint (:144-187).  A distance the segment walk does not
+// place (rounding leaves `remaining` above the last segment's length) maps to
+// the last point, where the reference returns an unset value.
+template <class T> static V3<T> path_point_at(const Path<T>& P_, T dist) {
+    T remaining = dist;
+    if (P_.cyclic) {
+        remaining = std::fmod(dist, P_.total);
+    } else {
+        if (dist < T(0)) return P_.pt(0);
+        if (dist >= P_.total) return P_.pt(P_.m - 1);
+    }
+    for (int i = 1; i < P_.m; ++i) {
+        const T seg = P_.lens[i];
+        if (seg < remaining) {
+            remaining -= seg;
+        } else {
+            const T ratio = remaining / seg;
+            return add(P_.pt(i - 1), mul(sub(P_.pt(i), P_.pt(i - 1)), ratio));  // interpolate, VectorUtils.cuh:292-295
+        }
+    }
+    return P_.pt(P_.m - 1);
+}
+
+// FollowPathCUDAKernel (CUDA/SteerToFollowPathCUDAKernel.cu:189-287).
+template <class T> static void follow_path(Agents<T>& A, int nA, const Path<T>& P_, T prediction_time, T w, uint32_t mask) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nA; ++i) {
+        if (A.bits[i] & mask) continue;
+        const V3<T> pos = P(A, i), fwd = F(A, i);
+        const T speed = A.speed[i];
+        const T offset = prediction_time * speed;
+        const V3<T> future = add(pos, mul(mul(fwd, speed), prediction_time));
+        const T now_d = path_distance_of(P_, pos);
+        const T future_d = path_distance_of(P_, future);
+        const bool rightway = (offset > T(0)) ? (now_d < future_d) : (now_d > future_d);
+        T outside;
+        map_point_to_path(P_, future, outside);
+        V3<T> steering = mk(T(0), T(0), T(0));
+        if (!((outside < T(0)) && rightway)) {
+            const V3<T> target = path_point_at(P_, now_d + offset);
+            steering = sub(sub(target, pos), fwd);
+        }
+        accumulate(A, i, normalize(steering), w, BIT_FOLLOW_PATH);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Calibration model (the differentiable crowd step) for gradient checks.
@@ -1102,6 +1373,33 @@ void or_knn_binned(const or_grid* g, int nA, const float* ax, const float* ay, c
     void or_update_##SUF(int mode, or_agents_##SUF* A, int nA, T dt) { update<T>(mode, *A, nA, dt); }               \
     void or_wrap_world_##SUF(int shape, or_agents_##SUF* A, int nA, const T* c, const T* s) {                       \
         wrap_world<T>(shape, *A, nA, c, s);                                                                         \
+    }                                                                                                               \
+    void or_update_anti_penetration_##SUF(or_agents_##SUF* A, int nA, const uint32_t* knn, int k, int nW,           \
+                                          const T* sx, const T* sy, const T* sz, const T* ex, const T* ey,          \
+                                          const T* ez, const T* nx, const T* ny, const T* nz, T dt, int closest) {  \
+        update_anti_penetration<T>(*A, nA, knn, k, nW, sx, sy, sz, ex, ey, ez, nx, ny, nz, dt, closest);            \
+    }                                                                                                               \
+    void or_anti_penetration_agents_##SUF(or_agents_##SUF* A, int nA, const uint32_t* knn, const float* dist,       \
+                                          const or_agents_##SUF* B, int nB, uint32_t mask) {                        \
+        anti_penetration_agents<T>(*A, nA, knn, dist, *B, nB, mask);                                                \
+    }                                                                                                               \
+    T or_path_init_##SUF(int n, const T* pts_in, int cyclic, T* pts, T* nrm, T* lens) {                             \
+        return path_init<T>(n, pts_in, cyclic, pts, nrm, lens);                                                     \
+    }                                                                                                               \
+    void or_follow_path_##SUF(or_agents_##SUF* A, int nA, int m, const T* pts, const T* nrm, const T* lens,         \
+                              T radius, T total, int cyclic, T prediction_time, T w, uint32_t mask) {               \
+        const Path<T> P_{m, pts, nrm, lens, radius, total, cyclic};                                                 \
+        follow_path<T>(*A, nA, P_, prediction_time, w, mask);                                                       \
+    }                                                                                                               \
+    void or_path_point_at_##SUF(int m, const T* pts, const T* nrm, const T* lens, T total, int cyclic, T dist,      \
+                                T* out) {                                                                           \
+        const Path<T> P_{m, pts, nrm, lens, T(0), total, cyclic};                                                   \
+        const V3<T> r = path_point_at<T>(P_, dist);                                                                 \
+        out[0] = r.x; out[1] = r.y; out[2] = r.z;                                                                   \
+    }                                                                                                               \
+    T or_path_distance_of_##SUF(int m, const T* pts, const T* nrm, const T* lens, const T* p) {                     \
+        const Path<T> P_{m, pts, nrm, lens, T(0), T(0), 0};                                                         \
+        return path_distance_of<T>(P_, mk(p[0], p[1], p[2]));                                                       \
     }
 
 OR_AGENT_API(float, f32)

@@ -82,6 +82,15 @@ def lib() -> C.CDLL:
             getattr(L, f"or_avoid_walls_{suf}").argtypes = [C.POINTER(A), i, u, i, i] + [t] * 9 + [T, T, C.c_uint32]
             getattr(L, f"or_update_{suf}").argtypes = [i, C.POINTER(A), i, T]
             getattr(L, f"or_wrap_world_{suf}").argtypes = [i, C.POINTER(A), i, t, t]
+            getattr(L, f"or_update_anti_penetration_{suf}").argtypes = [C.POINTER(A), i, u, i, i] + [t] * 9 + [T, i]
+            getattr(L, f"or_anti_penetration_agents_{suf}").argtypes = [C.POINTER(A), i, u, f, C.POINTER(A), i,
+                                                                         C.c_uint32]
+            getattr(L, f"or_path_init_{suf}").restype = T
+            getattr(L, f"or_path_init_{suf}").argtypes = [i, t, i, t, t, t]
+            getattr(L, f"or_follow_path_{suf}").argtypes = [C.POINTER(A), i, i, t, t, t, T, T, i, T, T, C.c_uint32]
+            getattr(L, f"or_path_point_at_{suf}").argtypes = [i, t, t, t, T, i, T, t]
+            getattr(L, f"or_path_distance_of_{suf}").restype = T
+            getattr(L, f"or_path_distance_of_{suf}").argtypes = [i, t, t, t, t]
         L.or_limit_max_deviation_angle.argtypes = [f, C.c_float, f, f]
         L.or_truncate_length.argtypes = [f, C.c_float, f]
         L.or_lines_intersect.restype = C.c_int
@@ -227,6 +236,68 @@ def update(state, dt, mode=MODE_OPENSTEER):
     getattr(lib(), f"or_update_{a.suf}")(mode, C.byref(a.s), state.n, dt)
 
 
+CLOSEST_CLAMPED, CLOSEST_AS_REFERENCE = 0, 1
+
+
+def update_anti_penetration(state, knn_idx, walls, dt, closest=CLOSEST_CLAMPED):
+    """UpdateCUDAKernelNew: GPU-reference integrate with wall anti-penetration
+    over the k nearest walls; bits are kept (ANTI_PENETRATION_WALL added)."""
+    a = _Bound(state)
+    ws = [np.ascontiguousarray(walls.arrays[f], dtype=a.T) for f in ("sx", "sy", "sz", "ex", "ey", "ez", "nx",
+                                                                      "ny", "nz")]
+    knn = np.ascontiguousarray(knn_idx, dtype=np.uint32)
+    getattr(lib(), f"or_update_anti_penetration_{a.suf}")(C.byref(a.s), state.n, _u(knn), knn.shape[0], walls.n,
+                                                           *[a.conv(x) for x in ws], dt, closest)
+
+
+def anti_penetration_agents(state, knn_idx, knn_dist, other, mask=0):
+    """AntiPenetrationAgentsCUDAKernel over the first KNN entry of each agent
+    (all reads of the state before the call)."""
+    a = _Bound(state)
+    b = a if other is state else _Bound(other)
+    knn = np.ascontiguousarray(knn_idx[0], dtype=np.uint32)
+    dist = np.ascontiguousarray(knn_dist[0], dtype=np.float32)
+    getattr(lib(), f"or_anti_penetration_agents_{a.suf}")(C.byref(a.s), state.n, _u(knn), _f(dist), C.byref(b.s),
+                                                           other.n, mask)
+
+
+class Pathway:
+    """PolylinePathwayCUDA's arrays: points[m][3], normals[m][3], lengths[m],
+    total length (m = n, + 1 when cyclic)."""
+
+    def __init__(self, points, radius, cyclic, dtype=np.float32):
+        pin = np.ascontiguousarray(np.asarray(points, dtype=dtype).reshape(-1, 3))
+        n = pin.shape[0]
+        m = n + (1 if cyclic else 0)
+        self.points = np.zeros((m, 3), dtype)
+        self.normals = np.zeros((m, 3), dtype)
+        self.lengths = np.zeros(m, dtype)
+        suf = "f64" if dtype == np.float64 else "f32"
+        conv = _d if dtype == np.float64 else _f
+        self.total = dtype(getattr(lib(), f"or_path_init_{suf}")(n, conv(pin), int(bool(cyclic)), conv(self.points),
+                                                                  conv(self.normals), conv(self.lengths)))
+        self.m, self.radius, self.cyclic, self.dtype, self.suf, self.conv = m, dtype(radius), bool(cyclic), dtype, suf, conv
+
+    def point_at(self, dist):
+        out = np.zeros(3, self.dtype)
+        getattr(lib(), f"or_path_point_at_{self.suf}")(self.m, self.conv(self.points), self.conv(self.normals),
+                                                        self.conv(self.lengths), self.total, int(self.cyclic),
+                                                        dist, self.conv(out))
+        return out
+
+    def distance_of(self, p):
+        q = np.ascontiguousarray(np.asarray(p, self.dtype))
+        return self.dtype(getattr(lib(), f"or_path_distance_of_{self.suf}")(
+            self.m, self.conv(self.points), self.conv(self.normals), self.conv(self.lengths), self.conv(q)))
+
+
+def follow_path(state, path: Pathway, prediction_time, w, mask=0):
+    a = _Bound(state)
+    getattr(lib(), f"or_follow_path_{a.suf}")(C.byref(a.s), state.n, path.m, path.conv(path.points),
+                                               path.conv(path.normals), path.conv(path.lengths), path.radius,
+                                               path.total, int(path.cyclic), prediction_time, w, mask)
+
+
 def wrap_world(state, shape, center, size_or_radius):
     a = _Bound(state)
     c = np.array(center, dtype=a.T)
@@ -284,6 +355,49 @@ def choke_step(groups, sc):
     for st in groups:
         update(st, sc.dt, sc.mode)
     return out
+
+
+def choke_new_step(state, sc, path: "Pathway", variant="new", closest=CLOSEST_CLAMPED):
+    """One step of the choke-point plugin in the GPU-reference formulas.
+    variant "new": CtfGroup::update (plugins/ChokePointNew.cpp:587-630) --
+    wrap, clear bits, KNN to itself and to the walls (capped at sc.wall_cap),
+    avoid walls, separation (mask ANTI_PENETRATION_WALL), alignment,
+    cohesion, follow path, seek, integrate, agent anti-penetration.
+    variant "group": CtfEnemyGroup::update (plugins/ChokePointGroup.cpp:683-728)
+    -- walls, path, seek, separation, alignment, cohesion, the integrate with
+    wall anti-penetration, agent anti-penetration masked by AVOID_WALLS |
+    ANTI_PENETRATION_WALL.  Returns (self KNN, wall KNN)."""
+    p = sc.params
+    size = sc.world_size
+    wrap_world(state, WRAP_BOX, (0.0, 0.0, 0.0), size)
+    g = grid(tuple(-0.5 * np.float32(v) for v in size), size, sc.world_cells)
+    state.bits[:] = 0
+    pos = np.stack([state.arrays["px"], state.arrays["py"], state.arrays["pz"]], axis=1).astype(np.float32)
+    ks, ds = knn_binned(g, pos, pos, p.knn, True)
+    kw, _ = knn_binned(g, pos, np.asarray(sc.wall_mid, np.float32), p.knw, False, cap=sc.wall_cap)
+    avoid_walls(state, kw, sc.walls, p.min_time_to_wall, p.weight_avoid_walls, p.mask_avoid_walls)
+    flock = [(FLOCK_SEPARATION, p.min_separation_distance, p.max_separation_distance, p.weight_separation,
+              1 << 12 if variant == "new" else p.mask_separation),
+             (FLOCK_ALIGNMENT, p.min_flocking_distance, p.max_flocking_distance, p.weight_alignment,
+              p.mask_alignment),
+             (FLOCK_COHESION, p.min_flocking_distance, p.max_flocking_distance, p.weight_cohesion, p.mask_cohesion)]
+    path_terms = [lambda: follow_path(state, path, p.path_prediction_time, p.weight_follow_path, p.mask_follow_path),
+                  lambda: steer_seek(state, sc.seek_target, p.weight_seek, p.mask_seek, MODE_GPUREF)]
+    if variant == "new":
+        for which, lo, hi, w, m in flock:
+            steer_flock(which, state, ks, state, lo, hi, p.cos_max_flocking_angle, w, m, MODE_GPUREF)
+        for f in path_terms:
+            f()
+        update(state, sc.dt, MODE_GPUREF)
+        anti_penetration_agents(state, ks, ds, state, p.mask_anti_penetration_agents)
+    else:
+        for f in path_terms:
+            f()
+        for which, lo, hi, w, m in flock:
+            steer_flock(which, state, ks, state, lo, hi, p.cos_max_flocking_angle, w, m, MODE_GPUREF)
+        update_anti_penetration(state, kw, sc.walls, sc.dt, closest)
+        anti_penetration_agents(state, ks, ds, state, (1 << 8) | (1 << 12))
+    return ks, kw
 
 
 # ---------------------------------------------------------------------------

@@ -101,3 +101,44 @@ def test_cxx_choke_setup_matches_oracle(orc, tmp_path):
     np.testing.assert_array_equal(key, rkey)
     np.testing.assert_array_equal(perm, rperm)
     np.testing.assert_array_equal(sorted_mid, rsorted)
+
+
+CHOKE_PLUGIN = gbuild.LIB_DIR / "choke_point_plugin"
+
+
+def test_cxx_choke_point_plugin_matches_oracle(orc, tmp_path):
+    """CtfGroup::update of plugins/ChokePointNew.cpp compiled against the
+    drop-in (tests/cxx/choke_point_plugin.cpp: ChokePoint.params, its split
+    map, 4096 agents placed around (0, 0, 60), 60 steps of dt 0.25 with
+    separation, alignment, cohesion, path following, seek, integrate and
+    agent anti-penetration) equals the oracle's sequence bit for bit.  The
+    drop-in's wall KNN is uncapped, the oracle's capped at the walls' reach:
+    equal states confirm the cap changes no force."""
+    from gpusteer_amd import formats
+
+    assert CHOKE_PLUGIN.exists(), f"{CHOKE_PLUGIN} not built"
+    p = formats.load_params(FILES / "ChokePoint.params")
+    ws = formats.WallSet.load_map(FILES / p.map_filename).split(p.dimensions, p.cells)
+    walls, mid = ws.get()
+    ws.close()
+    sc = scenarios.choke_point_new(p, walls, mid, n_agents=4096, dt=0.25, start=(0.0, 0.0, 60.0),
+                                   max_start_radius=50.0)
+    st = sc.state
+    inp, out = tmp_path / "agents.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        np.array([st.n], np.uint32).tofile(f)
+        for fld in FIELDS_IN:
+            st.arrays[fld].astype(np.float32).tofile(f)
+    steps = 60
+    print("cxx choke-point plugin", flush=True)
+    r = subprocess.run([str(CHOKE_PLUGIN), str(FILES / "ChokePoint.params"), str(FILES), str(inp), str(steps),
+                        repr(float(np.float32(sc.dt))), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rec = np.fromfile(out, dtype=np.float32).reshape(st.n, 13)
+    ref = st.copy()
+    R = orc.Pathway(sc.path_points, sc.path_radius, sc.path_cyclic)
+    for _ in range(steps):
+        orc.choke_new_step(ref, sc, R, "new")
+    want = np.stack([ref.arrays[f] for f in ("px", "py", "pz", "fx", "fy", "fz", "speed", "sx", "sy", "sz", "ux",
+                                             "uy", "uz")], axis=1)
+    np.testing.assert_array_equal(rec, want)

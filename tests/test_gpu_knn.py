@@ -217,3 +217,27 @@ def test_small_target_set(gs, ctx, orc, nb):
     idx, dist = _run_binned(gs, ctx, a, b, k, (-55,) * 3, (110,) * 3, (40, 40, 40), False)
     np.testing.assert_array_equal(idx, r_idx)
     np.testing.assert_array_equal(dist, r_dist)
+
+
+@pytest.mark.parametrize("cap", [float("inf"), 40.0, 1.5], ids=["uncapped", "cap40", "cap1.5"])
+def test_sparse_target_set(gs, ctx, orc, cap):
+    """3000 targets over 65,536 cells (sparse: the split walls' case): with
+    no cap or a wide one the search scans them exhaustively, with a narrow
+    cap it walks the bins; the result is the same exact KNN."""
+    g = np.random.default_rng(77)
+    a = np.stack([g.uniform(-128, 128, 20000), g.uniform(-4, 4, 20000), g.uniform(-128, 128, 20000)],
+                 axis=1).astype(np.float32)
+    b = np.stack([g.uniform(-128, 128, 3000), np.zeros(3000), g.uniform(-128, 128, 3000)], axis=1).astype(np.float32)
+    k = 3
+    r_idx, r_dist = orc.knn_brute(a, b, k, False, cap=cap)
+    bins = gs.KNNBinData(ctx, (256, 1, 256), (260.0, 10.0, 260.0))
+    A, B = _group(gs, ctx, a), _group(gs, ctx, b)
+    gs.updateKNNDatabase(A, bins)
+    gs.updateKNNDatabase(B, bins)
+    knn = gs.KNNData(ctx, a.shape[0], k)
+    gs.findKNearestNeighborsWithin(A, knn, bins, B, cap)
+    idx, dist = knn.download()
+    np.testing.assert_array_equal(idx, r_idx)
+    np.testing.assert_array_equal(dist, r_dist)
+    for h in (knn, A, B, bins):
+        h.close()
