@@ -279,6 +279,14 @@ def bytes_update_anti_penetration(k: int) -> float:
     return 152.0 + 4.0 * k
 
 
+# Neighbour avoidance against another group of the same size: A's position,
+# forward, side, speed, radius, steering, bits and k KNN indices and
+# distances (60 + 8k), B's position, forward, speed, radius read once per B
+# agent (32), steering and bits written (16).
+def bytes_avoid_neighbors(k: int) -> float:
+    return 108.0 + 8.0 * k
+
+
 def _events_ms(L, stream, fn, reps: int) -> float:
     """Average HIP-event time of fn() (one kernel launch) over reps calls on
     the library's stream."""
@@ -307,13 +315,15 @@ def plugin_bench(steps: int, warmup: int, n_kernel: int = 1 << 20, reps: int = 5
     ChokePoint20.map split at the bin borders (1254 walls), stepped through
     the per-step sequences of plugins/ChokePointNew.cpp (CtfGroup::update)
     and plugins/ChokePointGroup.cpp (integrate with wall anti-penetration);
-    plus HIP-event times of the three plugin-compat kernels on 2^20-agent
-    states against their algorithmic bytes."""
+    plugins/MultiGroup.cpp on data/MultiGroup.params (two groups of 5000
+    avoiding each other); plus HIP-event times of the plugin-compat and
+    neighbour-avoidance kernels on 2^20-agent states against their
+    algorithmic bytes."""
     import numpy as np
 
     import gpusteer_amd as gs
     from gpusteer_amd import formats, scenarios
-    from gpusteer_amd.plugins import ChokePointNew
+    from gpusteer_amd.plugins import ChokePointNew, MultiGroup
 
     files = ROOT / "tests" / "golden" / "scenario_files"
     p = formats.load_params(files / "ChokePoint.params")
@@ -346,6 +356,24 @@ def plugin_bench(steps: int, warmup: int, n_kernel: int = 1 << 20, reps: int = 5
             res[name] = {"ms_per_step": el * 1e3 / steps, "agent_steps_per_s": sc.state.n * steps / el,
                          "steps": steps, "warmup": warmup}
             print(f"bench: {name} {el * 1e3 / steps:.3f} ms/step", file=sys.stderr, flush=True)
+
+        # plugins/MultiGroup.cpp on data/MultiGroup.params (two groups of 5000)
+        mp = formats.load_params(files / "MultiGroup.params")
+        msc = scenarios.multi_group(mp, dt=1.0 / 30.0)
+        sim = MultiGroup(ctx, msc)
+        sim.step(warmup)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        n_mg = sum(st.n for st in msc.states)
+        sim.close()
+        res["multi_group"] = {"ms_per_step": el * 1e3 / steps, "agent_steps_per_s": n_mg * steps / el, "steps": steps,
+                              "warmup": warmup, "workload": f"data/MultiGroup.params: {len(msc.states)} groups, "
+                              f"{n_mg} agents, 100x1x100 bins over 635x10x635, knn {mp.knn} within {msc.knn_cap:.0f}, "
+                              "dt 1/30"}
+        print(f"bench: multi_group {el * 1e3 / steps:.3f} ms/step", file=sys.stderr, flush=True)
 
         # Kernel rooflines on 2^20-agent states.
         g = scenarios.rng(91)
@@ -394,7 +422,28 @@ def plugin_bench(steps: int, warmup: int, n_kernel: int = 1 << 20, reps: int = 5
         gs.findKNearestNeighborsWithin(A, kw, bins, W, sc.wall_cap, 1)
         ms = _events_ms(L, stream, lambda: gs.updateGroupWithAntiPenetration(A, kw, W, 1.0 / 30.0), reps)
         kern["k_update_anti_penetration"] = (ms, bytes_update_anti_penetration(p.knw))
-        for h in (A, W, bins, kw):
+        for h in (W, bins, kw):
+            h.close()
+        # neighbour avoidance: two crowds of 2^20 crossing head-on (2 per unit area each), k = 7
+        half = float(np.sqrt(n / 2.0) / 2.0)
+        crowds = []
+        for hz in (0.5 * np.pi, -0.5 * np.pi):
+            pos = np.stack([g.uniform(-half, half, n), np.zeros(n), g.uniform(-half, half, n)], 1).astype(np.float32)
+            hd2 = hz + g.normal(scale=0.4, size=n)
+            f2 = np.stack([np.cos(hd2), np.zeros(n), np.sin(hd2)], 1)
+            crowds.append(scenarios.make_agents(pos, f2, g.uniform(0, 3, n).astype(np.float32), 0.5, 3.0, 3.0))
+        A.upload(crowds[0])
+        B = gs.AgentGroup(ctx, crowds[1])
+        c = int(2 * half / 2.0)
+        bins = gs.KNNBinData(ctx, (c, 1, c), (2 * half + 2, 10.0, 2 * half + 2))
+        gs.updateKNNDatabase(A, bins)
+        gs.updateKNNDatabase(B, bins)
+        knn = gs.KNNData(ctx, n, 7)
+        gs.findKNearestNeighbors(A, knn, bins, B, 1)
+        ms = _events_ms(L, stream, lambda: gs.steerToAvoidNeighbors(A, knn, B, 10.0, 0.5, True, 1.0, 0,
+                                                                     gs.MODE_GPUREF), reps)
+        kern["k_avoid_neighbors<1>"] = (ms, bytes_avoid_neighbors(7))
+        for h in (A, B, bins, knn):
             h.close()
     res["kernels"] = {}
     for name, (ms, b) in kern.items():

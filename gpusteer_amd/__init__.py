@@ -25,6 +25,7 @@ from .api import (  # noqa: F401
     steerForFlocking,
     steerForSeek,
     steerForSeparation,
+    steerToAvoidNeighbors,
     steerToAvoidObstacles,
     steerToAvoidWalls,
     steerToFollowPath,

@@ -277,6 +277,8 @@ _SIGS = {
     "gs_path_get": (C.c_int, [_h, _fp, _fp, _fp]),
     "gs_path_destroy": (C.c_int, [_h]),
     "gs_steer_follow_path": (C.c_int, [_h, _h, C.c_float, C.c_float, C.c_uint32]),
+    "gs_steer_avoid_neighbors": (C.c_int, [_h, _h, _h, C.c_float, C.c_float, C.c_int, C.c_float, C.c_uint32,
+                                           C.c_int]),
     "gs_wrap_world": (C.c_int, [_h, C.c_int, _fp, _fp]),
     "gs_clear_bits": (C.c_int, [_h]),
     "gs_step_boids": (C.c_int, [_h, _h, _h, C.POINTER(GsBoidsParams), C.c_uint32]),

@@ -353,6 +353,15 @@ def steerToFollowPath(A, path: PolylinePathway, predictionTime, fWeight, doNotAp
           "gs_steer_follow_path")
 
 
+def steerToAvoidNeighbors(A, knn, other, fMinTimeToCollision, fMinSeparationDistance, bAvoidCloseNeighbors,
+                          fWeight, doNotApplyWith=0, mode=MODE_OPENSTEER):
+    """steerToAvoidNeighbors (GroupSteerLibrary.cuh:111) over A's KNN in
+    `other`."""
+    check(capi.lib.gs_steer_avoid_neighbors(A.h, knn.h, other.h, fMinTimeToCollision, fMinSeparationDistance,
+                                            int(bool(bAvoidCloseNeighbors)), fWeight, doNotApplyWith, mode),
+          "gs_steer_avoid_neighbors")
+
+
 def wrapWorld(A, worldSize=None, shape=capi.GS_WRAP_BOX, center=(0.0, 0.0, 0.0), radius=None):
     """Box: wrapWorld(A, worldSize) of the reference (centred box).  Sphere:
     shape=GS_WRAP_SPHERE, radius=R."""

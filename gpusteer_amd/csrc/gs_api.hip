@@ -920,6 +920,23 @@ int gs_anti_penetration_agents(gs_group A, gs_knn knn, gs_group B, uint32_t mask
     });
 }
 
+int gs_steer_avoid_neighbors(gs_group A, gs_knn knn, gs_group B, float min_time_to_collision,
+                             float min_separation_distance, int avoid_close_neighbors, float w, uint32_t mask,
+                             int mode) {
+    return guarded([&] {
+        check_group(A, "group A", GS_KIND_AGENTS);
+        check_group(B, "group B", GS_KIND_AGENTS);
+        check(knn, "knn");
+        if (mode != GS_MODE_OPENSTEER && mode != GS_MODE_GPUREF) fail(GS_ERR_INVALID_ARGUMENT, "unknown mode");
+        require_knn_shape(knn, A);
+        if (B->ctx != A->ctx) fail(GS_ERR_INVALID_ARGUMENT, "groups belong to different contexts");
+        use_device(A->ctx);
+        ensure_current(knn, A, B);
+        launch_avoid_neighbors(A, knn, B, min_time_to_collision, min_separation_distance, avoid_close_neighbors ? 1 : 0,
+                               w, mask, mode);
+    });
+}
+
 // PolylinePathwayCUDA::Initialize (CUDA/PolylinePathwayCUDA.cu:17-60) on the
 // host, in float, in its order: m = n (+1 closing on the first point when
 // cyclic); per segment end i >= 1 the difference to the previous point, its

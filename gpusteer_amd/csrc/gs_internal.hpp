@@ -367,6 +367,8 @@ void launch_update(gs_group_s* A, float dt, int mode);
 void launch_update_anti_penetration(gs_group_s* A, gs_knn_s* knn, gs_group_s* W, float dt, int closest);
 void launch_anti_penetration_agents(gs_group_s* A, gs_knn_s* knn, gs_group_s* B, uint32_t mask);
 void launch_follow_path(gs_group_s* A, const gs_path_s* p, float prediction_time, float w, uint32_t mask);
+void launch_avoid_neighbors(gs_group_s* A, gs_knn_s* knn, gs_group_s* B, float min_time, float min_sep,
+                            int avoid_close, float w, uint32_t mask, int mode);
 void launch_wrap(gs_group_s* A, int shape, const float* c, const float* s);
 void launch_clear_bits(gs_group_s* A);
 void launch_check_finite(gs_group_s* A, uint32_t* first_bad_slot_host);

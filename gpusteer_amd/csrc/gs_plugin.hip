@@ -6,6 +6,9 @@
 //   k_anti_penetration_agents  AntiPenetrationAgentsCUDAKernel (CUDA/AntiPenetrationAgentsCUDAKernel.cu:57-109)
 //   k_follow_path              FollowPathCUDAKernel (CUDA/SteerToFollowPathCUDAKernel.cu:189-287)
 //                              over a PolylinePathwayCUDA (CUDA/PolylinePathwayCUDA.cu:17-60)
+//   k_avoid_neighbors          SteerToAvoidCloseNeighborsCUDAKernel + SteerToAvoidNeighborsCUDAKernel
+//                              (CUDA/SteerToAvoidNeighborsCUDAKernel.cu:93-402), the multi-group
+//                              plugin's avoidance of the other groups (plugins/MultiGroup.cpp:571-579)
 //
 // One lane per agent, SoA fields read and written coalesced, like the other
 // behaviours.  Walls are gathered through the agent's KNN column; the path
@@ -259,6 +262,139 @@ __global__ __launch_bounds__(kBlock) void k_follow_path(uint32_t n, Agents A, Pa
     apply(acc, bits, normalize3(steering), w, GS_BIT_FOLLOW_PATH);
     st3(A.gx, A.gy, A.gz, i, acc);
     A.bits[i] = bits;
+}
+
+// Neighbour avoidance over A's KNN in B (B is only read, so it may be A).
+// GS_MODE_GPUREF: with avoid_close, SteerToAvoidCloseNeighborsCUDAKernel
+// (CUDA/SteerToAvoidNeighborsCUDAKernel.cu:93-187) first, then
+// SteerToAvoidNeighborsCUDAKernel (:235-402) for agents without the
+// AVOID_CLOSE_NEIGHBORS bit (SteerToAvoidNeighborsCUDA.cu:143), fused into
+// one pass per agent.  The head-on rule reads the approach position the
+// kernel's helper wrote last (the last neighbour passing the time test,
+// chosen or not, :334-338); the parallel rule uses the threat's current
+// position (the kernel reads an unset variable, :372).
+// GS_MODE_OPENSTEER: SteerLibrary::steerToAvoidNeighbors
+// (src/SteerLibrary.cpp:290-389) with steerToAvoidCloseNeighbors(v, 0, .)
+// (:470-495): current distances, threshold 2 r, the chosen threat's
+// approach position, the perpendicular rule when the threat is not faster,
+// forces added unnormalized.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_avoid_neighbors(uint32_t nA, Agents A, uint32_t nB,
+                                                            const float* __restrict__ bpx, const float* __restrict__ bpy,
+                                                            const float* __restrict__ bpz, const float* __restrict__ bfx,
+                                                            const float* __restrict__ bfy, const float* __restrict__ bfz,
+                                                            const float* __restrict__ bspeed,
+                                                            const float* __restrict__ brad,
+                                                            const uint32_t* __restrict__ knn,
+                                                            const float* __restrict__ knn_dist, uint32_t k,
+                                                            float min_time, float min_sep, int avoid_close, float w,
+                                                            uint32_t mask) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nA) return;
+    uint32_t bits = A.bits[i];
+    if (bits & mask) return;
+    const f3 pos = ld3(A.px, A.py, A.pz, i), dir = ld3(A.fx, A.fy, A.fz, i), side = ld3(A.sx, A.sy, A.sz, i);
+    const float speed = A.speed[i], radius = A.radius[i];
+    f3 acc = ld3(A.gx, A.gy, A.gz, i);
+    bool done = false;
+    if (MODE == GS_MODE_GPUREF) {
+        if (avoid_close) {
+            f3 steering = mk3(0.f, 0.f, 0.f);
+            for (uint32_t j = 0; j < k; ++j) {
+                const uint32_t b = knn[(size_t)j * nA + i];
+                if (b >= nB) break;
+                const float min_c = min_sep + (radius + brad[b]);
+                if (knn_dist[(size_t)j * nA + i] < min_c) {
+                    steering = perp3(-(ld3(bpx, bpy, bpz, b) - pos), dir);
+                    break;
+                }
+            }
+            apply(acc, bits, normalize3(steering), w, GS_BIT_AVOID_CLOSE_NEIGHBORS);
+        }
+        done = (bits & GS_BIT_AVOID_CLOSE_NEIGHBORS) != 0;
+    } else {
+        f3 sep = mk3(0.f, 0.f, 0.f);
+        for (uint32_t j = 0; j < k; ++j) {
+            const uint32_t b = knn[(size_t)j * nA + i];
+            if (b >= nB) break;
+            const float min_c = 0.f + (radius + brad[b]);
+            const f3 off = ld3(bpx, bpy, bpz, b) - pos;
+            if (len3(off) < min_c) {
+                sep = perp3(-off, dir);
+                break;
+            }
+        }
+        if (!zero3(sep)) {
+            apply(acc, bits, sep, w, GS_BIT_AVOID_CLOSE_NEIGHBORS);
+            done = true;
+        }
+    }
+    if (!done) {
+        float min_t = min_time;
+        bool found = false;
+        f3 last_near = mk3(0.f, 0.f, 0.f), threat_near = last_near, t_dir = last_near, t_pos = last_near;
+        float t_speed = 0.f;
+        for (uint32_t j = 0; j < k; ++j) {
+            const uint32_t b = knn[(size_t)j * nA + i];
+            if (b >= nB) break;
+            const f3 bp = ld3(bpx, bpy, bpz, b), bd = ld3(bfx, bfy, bfz, b);
+            const float bs = bspeed[b];
+            const float threshold = MODE == GS_MODE_GPUREF ? radius + brad[b] : radius * 2.f;
+            // predictNearestApproachTime (:205-233)
+            float time = 0.f;
+            const f3 rel = bd * bs - dir * speed;
+            const float rel_speed = len3(rel);
+            if (rel_speed != 0.f) time = dot3(rel / rel_speed, pos - bp) / rel_speed;
+            if (time >= 0.f && time < min_t) {
+                const f3 my_near = pos + dir * (speed * time);
+                const f3 his_near = bp + bd * (bs * time);
+                last_near = his_near;
+                if (len3(my_near - his_near) < threshold) {
+                    min_t = time;
+                    found = true;
+                    t_dir = bd;
+                    t_speed = bs;
+                    t_pos = bp;
+                    threat_near = his_near;
+                }
+            }
+        }
+        float steer = 0.f;
+        if (found) {
+            const float par = dot3(dir, t_dir);
+            if (par < -0.707f) {
+                const f3 off = (MODE == GS_MODE_GPUREF ? last_near : threat_near) - pos;
+                steer = dot3(off, side) > 0.f ? -1.f : 1.f;
+            } else if (par > 0.707f) {
+                steer = dot3(t_pos - pos, side) > 0.f ? -1.f : 1.f;
+            } else if (MODE == GS_MODE_GPUREF ? (speed <= t_speed) : (t_speed <= speed)) {
+                steer = dot3(side, t_dir * t_speed) > 0.f ? -1.f : 1.f;
+            }
+        }
+        if (MODE == GS_MODE_GPUREF) {
+            if (found) apply(acc, bits, normalize3(side * steer), w, GS_BIT_AVOID_NEIGHBORS);
+        } else {
+            apply(acc, bits, side * steer, w, GS_BIT_AVOID_NEIGHBORS);
+        }
+    }
+    st3(A.gx, A.gy, A.gz, i, acc);
+    A.bits[i] = bits;
+}
+
+void launch_avoid_neighbors(gs_group_s* A, gs_knn_s* knn, gs_group_s* B, float min_time, float min_sep,
+                            int avoid_close, float w, uint32_t mask, int mode) {
+    if (A->n == 0) return;
+    hipStream_t st = A->ctx->stream;
+#define GS_AN(M)                                                                                                  \
+    k_avoid_neighbors<M><<<blocks_for(A->n), kBlock, 0, st>>>(A->n, A->dev_agents(), B->n, B->F(0), B->F(1),     \
+                                                              B->F(2), B->F(3), B->F(4), B->F(5), B->F(6),        \
+                                                              B->F(19), knn->idx(), knn->dist(), knn->k, min_time, \
+                                                              min_sep, avoid_close, w, mask)
+    if (mode == GS_MODE_GPUREF) GS_AN(GS_MODE_GPUREF);
+    else GS_AN(GS_MODE_OPENSTEER);
+#undef GS_AN
+    GS_LAUNCHED("k_avoid_neighbors");
+    A->mark_written(kFSteer, true);
 }
 
 void launch_update_anti_penetration(gs_group_s* A, gs_knn_s* knn, gs_group_s* W, float dt, int closest) {

@@ -6,7 +6,7 @@ from __future__ import annotations
 
 from . import api
 from .api import MODE_GPUREF
-from .scenarios import ChokeNewScenario
+from .scenarios import ChokeNewScenario, MultiGroupScenario
 
 BIT_ANTI_PENETRATION_WALL = 1 << 12
 
@@ -80,3 +80,66 @@ class ChokePointNew:
     def close(self) -> None:
         for h in (self.group, self.walls, self.bins, self.knn_self, self.knn_walls, self.path):
             h.close()
+
+
+class MultiGroup:
+    """plugins/MultiGroup.cpp: groups sharing one bin lattice, each with a KNN
+    to every group and an optional path (Group :180-230).  A frame runs
+    Group::preUpdate for every group (wrap, bins, clear bits, :546-556), then
+    Group::update for each group in turn (:558-611): KNN to every group,
+    avoid the other groups, separation from every group, alignment and
+    cohesion with its own, path and seek, integrate, anti-penetration against
+    every group.  A later group sees the earlier groups' new state.  The
+    group searches are capped at the scenario's knn_cap (the reach of every
+    behaviour that reads them)."""
+
+    def __init__(self, ctx: api.Context, sc: MultiGroupScenario, mode=MODE_GPUREF):
+        self.sc = sc
+        self.mode = mode
+        p = sc.params
+        self.bins = api.KNNBinData(ctx, sc.world_cells, sc.world_size)
+        self.groups = [api.AgentGroup(ctx, st) for st in sc.states]
+        self.knn = [[api.KNNData(ctx, st.n, p.knn) for _ in sc.states] for st in sc.states]
+        self.paths = [api.PolylinePathway(ctx, pts, r, c) if len(pts) >= 2 else None for pts, r, c in sc.paths]
+
+    def step(self, steps: int = 1) -> None:
+        sc, p, G, m = self.sc, self.sc.params, self.groups, self.mode
+        for _ in range(steps):
+            for A in G:
+                api.wrapWorld(A, sc.world_size)
+                api.updateKNNDatabase(A, self.bins)
+                api.clearAppliedKernels(A)
+            for gi, A in enumerate(G):
+                K = self.knn[gi]
+                for j, B in enumerate(G):
+                    api.findKNearestNeighborsWithin(A, K[j], self.bins, B, sc.knn_cap, p.search_radius)
+                for j, B in enumerate(G):
+                    if j != gi:
+                        api.steerToAvoidNeighbors(A, K[j], B, p.min_time_to_collision, p.min_separation_distance,
+                                                  p.avoid_close_neighbors, p.weight_avoid_neighbors,
+                                                  p.mask_avoid_neighbors, m)
+                for j, B in enumerate(G):
+                    api.steerForSeparation(A, K[j], B, p.min_separation_distance, p.max_separation_distance,
+                                           p.cos_max_flocking_angle, p.weight_separation, p.mask_separation, m)
+                api.steerForAlignment(A, K[gi], A, p.min_flocking_distance, p.max_flocking_distance,
+                                      p.cos_max_flocking_angle, p.weight_alignment, p.mask_alignment, m)
+                api.steerForCohesion(A, K[gi], A, p.min_flocking_distance, p.max_flocking_distance,
+                                     p.cos_max_flocking_angle, p.weight_cohesion, p.mask_cohesion, m)
+                P = self.paths[gi]
+                if P is not None:
+                    api.steerToFollowPath(A, P, p.path_prediction_time, p.weight_follow_path, p.mask_follow_path)
+                    api.steerForSeek(A, tuple(float(v) for v in sc.paths[gi][0][1]), p.weight_seek, p.mask_seek, m)
+                api.updateGroup(A, sc.dt, m)
+                for j, B in enumerate(G):
+                    api.antiPenetrationAgents(A, K[j], B, p.mask_anti_penetration_agents)
+
+    def close(self) -> None:
+        for A in self.groups:
+            A.close()
+        for row in self.knn:
+            for k in row:
+                k.close()
+        for P in self.paths:
+            if P is not None:
+                P.close()
+        self.bins.close()

@@ -430,3 +430,48 @@ def choke_point_new(params, walls: WallState, wall_mid: np.ndarray, n_agents: in
                             path_radius=float(gp.path_radius), path_cyclic=bool(gp.path_cyclic),
                             seek_target=tuple(float(v) for v in pts[1]), world_size=tuple(params.dimensions),
                             world_cells=tuple(params.cells), params=params, dt=dt, wall_cap=cap)
+
+
+@dataclass
+class MultiGroupScenario:
+    """The groups of plugins/MultiGroup.cpp (Group, :180-230) set up from a
+    parsed .params file: one agent state and path per group, the world
+    lattice and the per-step constants of Group::update (:558-611)."""
+    states: Tuple[AgentState, ...]
+    paths: Tuple[Tuple[np.ndarray, float, bool], ...]   # (points, radius, cyclic) or points empty
+    world_size: Tuple[float, float, float]
+    world_cells: Tuple[int, int, int]
+    params: object
+    dt: float
+    # KNN range of every group search: no agent farther than this can be a
+    # collision threat within minTimeToCollision (closing speed at most twice
+    # the largest max speed), nor a separation, flocking or overlap neighbour.
+    knn_cap: float
+
+
+def multi_group(params, n_per_group: int = None, dt: float = 0.1, max_start_radius: float = None,
+                start_scale: float = 1.0, workload: int = 1) -> MultiGroupScenario:
+    """Agents of every group placed as Group::reset does (:430-451): radius
+    uniform in [min, max] start radius on a random XZ direction around the
+    group's start, random XZ heading, speed = max speed, radius 0.5, mass 1, no
+    overlap test.  The draws are this module's own seeded stream.
+    n_per_group / max_start_radius override the file; start_scale moves the
+    groups' starts toward the origin (smaller test crowds that meet sooner)."""
+    g = rng(workload, 11)
+    states, paths = [], []
+    for gp in params.groups:
+        n = gp.num_agents if n_per_group is None else int(n_per_group)
+        hi = gp.max_start_radius if max_start_radius is None else max_start_radius
+        rr = g.uniform(gp.min_start_radius, hi, size=n)
+        th = g.uniform(0.0, 2.0 * np.pi, size=n)
+        c = np.asarray(gp.start_position, np.float64) * start_scale
+        pos = np.stack([c[0] + rr * np.cos(th), np.full(n, c[1]), c[2] + rr * np.sin(th)], axis=1).astype(np.float32)
+        hd = g.uniform(0.0, 2.0 * np.pi, size=n)
+        fwd = np.stack([np.cos(hd), np.zeros(n), np.sin(hd)], axis=1)
+        states.append(make_agents(pos, fwd, np.full(n, gp.max_speed, np.float32), 0.5, gp.max_speed, gp.max_force))
+        paths.append((np.asarray(gp.path, np.float32).reshape(-1, 3), float(gp.path_radius), bool(gp.path_cyclic)))
+    vmax = max(gp.max_speed for gp in params.groups)
+    reach = max(2.0 * vmax * params.min_time_to_collision + 2.0 * 0.5, params.max_flocking_distance,
+                params.max_separation_distance, params.min_separation_distance + 2.0 * 0.5)
+    return MultiGroupScenario(states=tuple(states), paths=tuple(paths), world_size=tuple(params.dimensions),
+                              world_cells=tuple(params.cells), params=params, dt=dt, knn_cap=float(reach + 1.0))

@@ -29,6 +29,8 @@
  *   gs_anti_penetration_agents
  *                            GSL:216  antiPenetrationAgents(A, KNNData*, B, mask)
  *   gs_steer_follow_path     GSL:88   steerToFollowPath(A, PolylinePathwayCUDA*, predictionTime, w, mask)
+ *   gs_steer_avoid_neighbors GSL:111  steerToAvoidNeighbors(A, KNNData*, B, minTimeToCollision,
+ *                                     minSeparationDistance, bAvoidCloseNeighbors, w, mask)
  *   gs_path_*                include/OpenSteer/CUDA/PolylinePathwayCUDA.cuh:11-68
  *   gs_group_*               include/OpenSteer/AgentGroup.h:32-54, ObstacleGroup.h:23-37,
  *                            WallGroup.h:18-33 (AddAgent/SyncDevice/GetDataForAgent/pd*)
@@ -271,6 +273,19 @@ int gs_path_destroy(gs_path p);
  * the point prediction_time * speed ahead along the path unless the predicted
  * position is inside the tube and moving forward; sets GS_BIT_FOLLOW_PATH. */
 int gs_steer_follow_path(gs_group A, gs_path path, float prediction_time, float weight, uint32_t mask);
+/* Unaligned collision avoidance against B over A's KNN in B (B is only read;
+ * it may be A).  GS_MODE_GPUREF = SteerToAvoidNeighborsCUDA
+ * (CUDA/SteerToAvoidNeighborsCUDA.cu:85-150): with avoid_close_neighbors,
+ * agents whose KNN distance to a neighbour is below min_separation_distance +
+ * both radii steer away from it (GS_BIT_AVOID_CLOSE_NEIGHBORS); the others
+ * steer sideways from the neighbour of earliest nearest approach within
+ * min_time_to_collision that comes closer than both radii
+ * (GS_BIT_AVOID_NEIGHBORS).  GS_MODE_OPENSTEER = SteerLibrary's
+ * steerToAvoidNeighbors (src/SteerLibrary.cpp:290-389): close neighbours
+ * always checked with separation 0, threshold 2 r, forces unnormalized. */
+int gs_steer_avoid_neighbors(gs_group A, gs_knn knn, gs_group B, float min_time_to_collision,
+                             float min_separation_distance, int avoid_close_neighbors, float weight, uint32_t mask,
+                             int mode);
 /* GS_WRAP_BOX: size_or_radius = box size (centred at center);
  * GS_WRAP_SPHERE: size_or_radius[0] = radius. */
 int gs_wrap_world(gs_group A, int shape, const float center[3], const float size_or_radius[3]);

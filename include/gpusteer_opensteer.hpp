@@ -612,6 +612,16 @@ inline void steerToFollowPath(AgentGroup* pAgentGroup, PolylinePathwayCUDA* pPat
                     "gs_steer_follow_path");
     pAgentGroup->SetSyncHost();
 }
+inline void steerToAvoidNeighbors(AgentGroup* pAgentGroup, KNNData* pKNNData, AgentGroup* pOtherGroup,
+                                  const float fMinTimeToCollision, float const fMinSeparationDistance,
+                                  bool const bAvoidCloseNeighbors, float const fWeight, uint const doNotApplyWith) {
+    gpusteer::check(gs_steer_avoid_neighbors(pAgentGroup->device_handle(), pKNNData->handle(),
+                                             pOtherGroup->device_handle(), fMinTimeToCollision, fMinSeparationDistance,
+                                             bAvoidCloseNeighbors ? 1 : 0, fWeight, doNotApplyWith,
+                                             pAgentGroup->context().Mode()),
+                    "gs_steer_avoid_neighbors");
+    pAgentGroup->SetSyncHost();
+}
 inline void updateGroupWithAntiPenetration(AgentGroup* pAgentGroup, KNNData* pKNNData, WallGroup* pWallGroup,
                                            const float elapsedTime) {
     gpusteer::check(gs_update_group_anti_penetration(pAgentGroup->device_handle(), pKNNData->handle(),

@@ -334,6 +334,8 @@ enum {
     BIT_COHESION = 1u << 6,
     BIT_AVOID_OBSTACLES = 1u << 7,
     BIT_AVOID_WALLS = 1u << 8,
+    BIT_AVOID_NEIGHBORS = 1u << 9,
+    BIT_AVOID_CLOSE_NEIGHBORS = 1u << 10,
     BIT_FOLLOW_PATH = 1u << 11,
     BIT_ANTI_PENETRATION_WALL = 1u << 12,
     BIT_ANTI_PENETRATION_AGENT = 1u << 13,
@@ -918,6 +920,138 @@ template <class T> static void follow_path(Agents<T>& A, int nA, const Path<T>& 
 }
 
 // ---------------------------------------------------------------------------
+// Neighbour avoidance (unaligned collision avoidance).
+
+// predictNearestApproachTime (CUDA/SteerToAvoidNeighborsCUDAKernel.cu:205-233,
+// src/SteerLibrary.cpp:404-438).
+template <class T>
+static inline T nearest_approach_time(V3<T> pos, V3<T> dir, T speed, V3<T> opos, V3<T> odir, T ospeed) {
+    const V3<T> my_vel = mul(dir, speed);
+    const V3<T> other_vel = mul(odir, ospeed);
+    const V3<T> rel = sub(other_vel, my_vel);
+    const T rel_speed = len(rel);
+    if (rel_speed == T(0)) return T(0);
+    const V3<T> tangent = dvd(rel, rel_speed);
+    const V3<T> rel_pos = sub(pos, opos);
+    const T projection = dot(tangent, rel_pos);
+    return projection / rel_speed;
+}
+
+// steerToAvoidNeighbors over the KNN of A in B (B's fields are only read, so
+// B may be A).
+//  GPUREF: SteerToAvoidNeighborsCUDA::run (CUDA/SteerToAvoidNeighborsCUDA.cu:85-150):
+//    with avoid_close, SteerToAvoidCloseNeighborsCUDAKernel
+//    (CUDA/SteerToAvoidNeighborsCUDAKernel.cu:93-187: the first neighbour in
+//    KNN order whose KNN distance is below min_sep + r_a + r_b gives
+//    perpendicularComponent(-offset, forward); normalized, weighted,
+//    AVOID_CLOSE_NEIGHBORS bit), then SteerToAvoidNeighborsCUDAKernel (:235-402)
+//    masked by mask | AVOID_CLOSE_NEIGHBORS: the neighbour with the earliest
+//    nearest approach in [0, min_time) closer than r_a + r_b; head-on
+//    (parallelness < -0.707) steers away from the threat position at nearest
+//    approach -- as in the kernel, the value its helper wrote last, for the
+//    last neighbour that passed the time test (:334-338), chosen or not;
+//    parallel (> 0.707) away from the threat's position (the kernel reads an
+//    unset variable there, :372; its current position is the CPU rule);
+//    otherwise, when not faster than the threat, behind it.  side * steer is
+//    normalized and weighted; nothing is added without a threat.
+//  OPENSTEER: SteerLibrary::steerToAvoidNeighbors (src/SteerLibrary.cpp:290-389):
+//    steerToAvoidCloseNeighbors(v, 0, others) first (:470-495: current
+//    distances, min separation 0, the perpendicular component unnormalized,
+//    AVOID_CLOSE_NEIGHBORS bit); otherwise the threat within min_time closer
+//    than 2 r_a, the head-on rule with the chosen threat's approach position,
+//    the perpendicular rule when the threat is not faster; side * steer added
+//    unnormalized (zero without a threat); min_sep and avoid_close unused.
+template <class T>
+static void avoid_neighbors(int mode, Agents<T>& A, int nA, const uint32_t* knn, const float* knn_dist, int k,
+                            const Agents<T>& B, int nB, T min_time, T min_sep, int avoid_close, T w, uint32_t mask) {
+    const T angle = T(0.707f);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nA; ++i) {
+        if (A.bits[i] & mask) continue;
+        const V3<T> pos = P(A, i), dir = F(A, i), side = S(A, i);
+        const T speed = A.speed[i], radius = A.radius[i];
+        if (mode == MODE_GPUREF && avoid_close) {
+            V3<T> steering = mk(T(0), T(0), T(0));
+            for (int j = 0; j < k; ++j) {
+                const uint32_t b = knn[(size_t)j * nA + i];
+                if (b >= (uint32_t)nB) break;
+                const T d = T(knn_dist[(size_t)j * nA + i]);
+                const T min_c = min_sep + (radius + B.radius[b]);
+                if (d < min_c) {
+                    steering = perpendicular_component(neg(sub(P(B, b), pos)), dir);
+                    break;
+                }
+            }
+            accumulate(A, i, normalize(steering), w, BIT_AVOID_CLOSE_NEIGHBORS);
+            if (A.bits[i] & BIT_AVOID_CLOSE_NEIGHBORS) continue;
+        }
+        if (mode == MODE_GPUREF && (A.bits[i] & BIT_AVOID_CLOSE_NEIGHBORS)) continue;
+        if (mode == MODE_OPENSTEER) {
+            V3<T> sep = mk(T(0), T(0), T(0));
+            for (int j = 0; j < k; ++j) {
+                const uint32_t b = knn[(size_t)j * nA + i];
+                if (b >= (uint32_t)nB) break;
+                const T min_c = T(0) + (radius + B.radius[b]);
+                const V3<T> off = sub(P(B, b), pos);
+                if (len(off) < min_c) {
+                    sep = perpendicular_component(neg(off), dir);
+                    break;
+                }
+            }
+            if (!is_zero(sep)) {
+                accumulate(A, i, sep, w, BIT_AVOID_CLOSE_NEIGHBORS);
+                continue;
+            }
+        }
+        T min_t = min_time;
+        bool found = false;
+        V3<T> last_near = mk(T(0), T(0), T(0)), threat_near = last_near;
+        V3<T> t_dir = last_near, t_pos = last_near;
+        T t_speed = T(0);
+        for (int j = 0; j < k; ++j) {
+            const uint32_t b = knn[(size_t)j * nA + i];
+            if (b >= (uint32_t)nB) break;
+            const V3<T> bp = P(B, b), bd = F(B, b);
+            const T bs = B.speed[b];
+            const T threshold = mode == MODE_GPUREF ? radius + B.radius[b] : radius * T(2);
+            const T time = nearest_approach_time(pos, dir, speed, bp, bd, bs);
+            if (time >= T(0) && time < min_t) {
+                const V3<T> my_near = add(pos, mul(dir, speed * time));
+                const V3<T> his_near = add(bp, mul(bd, bs * time));
+                last_near = his_near;
+                if (distance(my_near, his_near) < threshold) {
+                    min_t = time;
+                    found = true;
+                    t_dir = bd;
+                    t_speed = bs;
+                    t_pos = bp;
+                    threat_near = his_near;
+                }
+            }
+        }
+        T steer = T(0);
+        if (found) {
+            const T par = dot(dir, t_dir);
+            if (par < -angle) {
+                const V3<T> off = sub(mode == MODE_GPUREF ? last_near : threat_near, pos);
+                steer = dot(off, side) > T(0) ? T(-1) : T(1);
+            } else if (par > angle) {
+                const V3<T> off = sub(t_pos, pos);
+                steer = dot(off, side) > T(0) ? T(-1) : T(1);
+            } else if (mode == MODE_GPUREF ? (speed <= t_speed) : (t_speed <= speed)) {
+                const T sd = dot(side, mul(t_dir, t_speed));
+                steer = sd > T(0) ? T(-1) : T(1);
+            }
+        }
+        if (mode == MODE_GPUREF) {
+            if (found) accumulate(A, i, normalize(mul(side, steer)), w, BIT_AVOID_NEIGHBORS);
+        } else {
+            accumulate(A, i, mul(side, steer), w, BIT_AVOID_NEIGHBORS);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Calibration model (the differentiable crowd step) for gradient checks.
 //
 // One step per agent of group g, every neighbour read from the step's start
@@ -1390,6 +1524,11 @@ void or_knn_binned(const or_grid* g, int nA, const float* ax, const float* ay, c
                               T radius, T total, int cyclic, T prediction_time, T w, uint32_t mask) {               \
         const Path<T> P_{m, pts, nrm, lens, radius, total, cyclic};                                                 \
         follow_path<T>(*A, nA, P_, prediction_time, w, mask);                                                       \
+    }                                                                                                               \
+    void or_avoid_neighbors_##SUF(int mode, or_agents_##SUF* A, int nA, const uint32_t* knn, const float* dist,    \
+                                  int k, const or_agents_##SUF* B, int nB, T min_time, T min_sep, int avoid_close,  \
+                                  T w, uint32_t mask) {                                                             \
+        avoid_neighbors<T>(mode, *A, nA, knn, dist, k, *B, nB, min_time, min_sep, avoid_close, w, mask);            \
     }                                                                                                               \
     void or_path_point_at_##SUF(int m, const T* pts, const T* nrm, const T* lens, T total, int cyclic, T dist,      \
                                 T* out) {                                                                           \

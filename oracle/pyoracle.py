@@ -89,6 +89,8 @@ def lib() -> C.CDLL:
             getattr(L, f"or_path_init_{suf}").argtypes = [i, t, i, t, t, t]
             getattr(L, f"or_follow_path_{suf}").argtypes = [C.POINTER(A), i, i, t, t, t, T, T, i, T, T, C.c_uint32]
             getattr(L, f"or_path_point_at_{suf}").argtypes = [i, t, t, t, T, i, T, t]
+            getattr(L, f"or_avoid_neighbors_{suf}").argtypes = [i, C.POINTER(A), i, u, f, i, C.POINTER(A), i, T, T, i,
+                                                                 T, C.c_uint32]
             getattr(L, f"or_path_distance_of_{suf}").restype = T
             getattr(L, f"or_path_distance_of_{suf}").argtypes = [i, t, t, t, t]
         L.or_limit_max_deviation_angle.argtypes = [f, C.c_float, f, f]
@@ -261,6 +263,19 @@ def anti_penetration_agents(state, knn_idx, knn_dist, other, mask=0):
                                                            other.n, mask)
 
 
+def avoid_neighbors(state, knn_idx, knn_dist, other, min_time, min_sep, avoid_close, w, mask=0, mode=MODE_GPUREF):
+    """steerToAvoidNeighbors over the KNN of `state` in `other` (close
+    neighbours first when avoid_close, GPU-reference rules; or the CPU
+    SteerLibrary rules in MODE_OPENSTEER)."""
+    a = _Bound(state)
+    b = a if other is state else _Bound(other)
+    knn = np.ascontiguousarray(knn_idx, dtype=np.uint32)
+    dist = np.ascontiguousarray(knn_dist, dtype=np.float32)
+    getattr(lib(), f"or_avoid_neighbors_{a.suf}")(mode, C.byref(a.s), state.n, _u(knn), _f(dist), knn.shape[0],
+                                                   C.byref(b.s), other.n, min_time, min_sep, int(bool(avoid_close)), w,
+                                                   mask)
+
+
 class Pathway:
     """PolylinePathwayCUDA's arrays: points[m][3], normals[m][3], lengths[m],
     total length (m = n, + 1 when cyclic)."""
@@ -398,6 +413,45 @@ def choke_new_step(state, sc, path: "Pathway", variant="new", closest=CLOSEST_CL
         update_anti_penetration(state, kw, sc.walls, sc.dt, closest)
         anti_penetration_agents(state, ks, ds, state, (1 << 8) | (1 << 12))
     return ks, kw
+
+
+def multi_group_step(states, sc, paths, mode=MODE_GPUREF):
+    """One frame of plugins/MultiGroup.cpp: Group::preUpdate of every group
+    (:546-556: wrap, clear bits; the bins are the positions after the wrap),
+    then Group::update of each group in turn (:558-611).  KNN to group j uses
+    group j's positions at its preUpdate (its bin database); the forces read
+    the groups' current state.  Returns the KNN results of the last frame
+    [group][other] = (idx, dist)."""
+    p = sc.params
+    size = sc.world_size
+    g = grid(tuple(-0.5 * np.float32(v) for v in size), size, sc.world_cells)
+    snap = []
+    for st in states:
+        wrap_world(st, WRAP_BOX, (0.0, 0.0, 0.0), size)
+        st.bits[:] = 0
+        snap.append(np.stack([st.arrays["px"], st.arrays["py"], st.arrays["pz"]], axis=1).astype(np.float32))
+    out = []
+    for gi, A in enumerate(states):
+        knn = [knn_binned(g, snap[gi], snap[j], p.knn, j == gi, cap=sc.knn_cap) for j in range(len(states))]
+        for j, B in enumerate(states):
+            if j != gi:
+                avoid_neighbors(A, knn[j][0], knn[j][1], B, p.min_time_to_collision, p.min_separation_distance,
+                                p.avoid_close_neighbors, p.weight_avoid_neighbors, p.mask_avoid_neighbors, mode)
+        for j, B in enumerate(states):
+            steer_flock(FLOCK_SEPARATION, A, knn[j][0], B, p.min_separation_distance, p.max_separation_distance,
+                        p.cos_max_flocking_angle, p.weight_separation, p.mask_separation, mode)
+        for which, w, m in ((FLOCK_ALIGNMENT, p.weight_alignment, p.mask_alignment),
+                            (FLOCK_COHESION, p.weight_cohesion, p.mask_cohesion)):
+            steer_flock(which, A, knn[gi][0], A, p.min_flocking_distance, p.max_flocking_distance,
+                        p.cos_max_flocking_angle, w, m, mode)
+        if paths[gi] is not None:
+            follow_path(A, paths[gi], p.path_prediction_time, p.weight_follow_path, p.mask_follow_path)
+            steer_seek(A, tuple(float(v) for v in sc.paths[gi][0][1]), p.weight_seek, p.mask_seek, mode)
+        update(A, sc.dt, mode)
+        for j, B in enumerate(states):
+            anti_penetration_agents(A, knn[j][0], knn[j][1], B, p.mask_anti_penetration_agents)
+        out.append(knn)
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -220,3 +220,88 @@ def test_choke_point_plugin_steps_bit_exact(gs, ctx, orc, choke_files, variant, 
     z = ref.arrays["pz"]
     print(f"agents past z = 13.5: {np.count_nonzero(z < 13.5)}", flush=True)
     sim.close()
+
+
+def _crossing(n, seed, heading_z):
+    g = scenarios.rng(seed)
+    pos = np.stack([g.uniform(-40, 40, n), np.zeros(n), g.uniform(-40, 40, n)], axis=1).astype(np.float32)
+    hd = np.where(g.uniform(size=n) < 0.7, np.arctan2(heading_z, 0.0), g.uniform(0, 2 * np.pi, n))
+    hd = hd + g.normal(scale=0.3, size=n)
+    fwd = np.stack([np.cos(hd), np.zeros(n), np.sin(hd)], axis=1)
+    st = scenarios.make_agents(pos, fwd, g.uniform(0, 3, n).astype(np.float32), 0.5, 3.0, 3.0)
+    st.arrays["radius"][:] = g.uniform(0.3, 0.7, n)
+    st.arrays["gx"][:] = g.normal(size=n)
+    st.bits[:] = np.where(g.uniform(size=n) < 0.1, 16, 0).astype(np.uint32)
+    return st
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["openSteer", "gpuRef"])
+@pytest.mark.parametrize("close", [0, 1])
+@pytest.mark.parametrize("self_group", [False, True])
+def test_avoid_neighbors_equals_oracle(gs, ctx, orc, mode, close, self_group):
+    """Two crowds crossing head-on (70 % heading +z / -z, the rest random),
+    k = 7: steering and bits of steerToAvoidNeighbors equal the oracle's; the
+    close-neighbour rule fires where it applies (the crowd is dense, so it
+    takes most agents) and the collision rule on the rest."""
+    a = _crossing(12000, 51, 1.0)
+    b = a if self_group else _crossing(10000, 52, -1.0)
+    ra = a.copy()
+    rb = ra if self_group else b.copy()
+    cells, size = (32, 1, 32), (84.0, 10.0, 84.0)
+    bins = gs.KNNBinData(ctx, cells, size)
+    A = gs.AgentGroup(ctx, a)
+    B = A if self_group else gs.AgentGroup(ctx, b)
+    gs.updateKNNDatabase(A, bins)
+    if not self_group:
+        gs.updateKNNDatabase(B, bins)
+    knn = gs.KNNData(ctx, a.n, 7)
+    gs.findKNearestNeighbors(A, knn, bins, B, 1)
+    print(f"avoid neighbours mode={mode} close={close} self={self_group}", flush=True)
+    gs.steerToAvoidNeighbors(A, knn, B, 10.0, 0.5, close, 1.5, 16, mode)
+    pa = np.stack([ra.arrays[f] for f in ("px", "py", "pz")], axis=1)
+    pb = np.stack([rb.arrays[f] for f in ("px", "py", "pz")], axis=1)
+    g = orc.grid((-42.0, -5.0, -42.0), size, cells)
+    ki, kd = orc.knn_binned(g, pa, pb, 7, self_group)
+    orc.avoid_neighbors(ra, ki, kd, rb, 10.0, 0.5, close, 1.5, 16, mode)
+    np.testing.assert_array_equal(knn.download()[0], ki)
+    out = A.download()
+    assert_state_equal(out, ra)
+    n_an = np.count_nonzero(out.bits & (1 << 9))
+    n_close = np.count_nonzero(out.bits & (1 << 10))
+    print(f"avoid {n_an} close {n_close}", flush=True)
+    if close or mode == 0:
+        assert n_close > 100 and n_an > 0
+    else:
+        assert n_close == 0 and n_an > 100
+    for h in (A, bins, knn) + (() if self_group else (B,)):
+        h.close()
+
+
+@pytest.mark.parametrize("name,close", [("MultiGroup.params", 0), ("MultiGroup4.params", 1)])
+def test_multi_group_steps_bit_exact(gs, ctx, orc, name, close):
+    """plugins/MultiGroup.cpp's frame (preUpdate of every group, then each
+    group's update: KNN to every group, avoid the others, separation from
+    all, alignment and cohesion with its own, path and seek, integrate,
+    anti-penetration against all) on the reference's MultiGroup files, 1500
+    agents per group started 0.15 x as far out within 30, dt 0.5, 30 frames,
+    group searches capped at the behaviours' reach: every group's KNN lists
+    and state equal the oracle's."""
+    from gpusteer_amd.plugins import MultiGroup
+
+    p = formats.load_params(FILES / name)
+    p.values = dict(p.values, avoid_close_neighbors=close)
+    sc = scenarios.multi_group(p, n_per_group=1500, dt=0.5, max_start_radius=30.0, start_scale=0.15)
+    refs = [s.copy() for s in sc.states]
+    paths = [orc.Pathway(pts, r, c) if len(pts) >= 2 else None for pts, r, c in sc.paths]
+    sim = MultiGroup(ctx, sc)
+    print(f"multi-group {name} groups={len(refs)}", flush=True)
+    for step in range(30):
+        sim.step()
+        knn = orc.multi_group_step(refs, sc, paths)
+        if step % 10 == 0 or step == 29:
+            for gi in range(len(refs)):
+                for j in range(len(refs)):
+                    np.testing.assert_array_equal(sim.knn[gi][j].download()[0], knn[gi][j][0],
+                                                  err_msg=f"step {step} knn {gi}->{j}")
+                assert_state_equal(sim.groups[gi].download(), refs[gi], what=f"step {step} group {gi}: ")
+    sim.close()

@@ -244,3 +244,122 @@ def test_follow_path_masked_and_accumulating(orc):
     s.arrays["gx"][0] = 1.0
     orc.follow_path(s, P, 3.0, 1.0)
     assert s.arrays["gx"][0] == pytest.approx(1.0 - 5.0 / math.sqrt(50.0), abs=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# Neighbour avoidance.
+
+
+def _pair(a_pos, a_fwd, a_speed, others):
+    """Agent 0 (side (-1, 0, 0) for forward +z) and neighbours given as
+    (position, forward, speed); KNN rows list the neighbours in order."""
+    s = one_agent(pos=a_pos, fwd=a_fwd, side=(-1.0, 0.0, 0.0), speed=a_speed, extra=[o[0] for o in others])
+    for i, (_, f, v) in enumerate(others, start=1):
+        s.arrays["fx"][i], s.arrays["fy"][i], s.arrays["fz"][i] = f
+        s.arrays["speed"][i] = v
+    idx = knn_rows_f(len(others))
+    pos = np.stack([s.arrays["px"], s.arrays["py"], s.arrays["pz"]], axis=1)
+    dist = np.full(idx.shape, np.finfo(np.float32).max, np.float32)
+    for j in range(len(others)):
+        dist[j, 0] = np.linalg.norm(pos[j + 1] - pos[0])
+    return s, idx, dist
+
+
+def knn_rows_f(n_nb):
+    out = np.full((n_nb, 1 + n_nb), PAD, np.uint32)
+    out[:, 0] = np.arange(1, n_nb + 1)
+    return out
+
+
+def _steer(s):
+    return [s.arrays[f][0] for f in ("gx", "gy", "gz")]
+
+
+BIT_AN, BIT_CLOSE = 1 << 9, 1 << 10
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_avoid_head_on(orc, mode):
+    """Head-on: the other agent 0.3 to the +x side, 10 ahead, both at speed
+    1: nearest approach at t = 5, 0.3 apart (< 2 r): steer along +side, away
+    from +x: (-1, 0, 0) x w."""
+    s, idx, dist = _pair((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0, [((0.3, 0.0, 10.0), (0.0, 0.0, -1.0), 1.0)])
+    orc.avoid_neighbors(s, idx, dist, s, 10.0, 0.5, False, 2.0, mode=mode)
+    assert _steer(s) == pytest.approx([-2.0, 0.0, 0.0], abs=1e-15)
+    assert s.bits[0] == BIT_AN
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_avoid_parallel_overtaking(orc, mode):
+    """Parallel: a faster agent behind, 0.3 to +x, catching up at t = 5:
+    steer away from its current position (+x): (-1, 0, 0)."""
+    s, idx, dist = _pair((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0, [((0.3, 0.0, -5.0), (0.0, 0.0, 1.0), 2.0)])
+    orc.avoid_neighbors(s, idx, dist, s, 10.0, 0.5, False, 1.0, mode=mode)
+    assert _steer(s) == pytest.approx([-1.0, 0.0, 0.0], abs=1e-15)
+
+
+def test_avoid_perpendicular_slower_steers_behind(orc):
+    """Crossing paths meeting at (0, 0, 5) at t = 5 (the other moves -x at
+    speed 1): the GPU rule lets the agent not faster than the threat steer
+    behind it, sign of side . threat velocity = +1 -> steer -1 -> (1, 0, 0);
+    the CPU rule lets the agent steer when the threat is not faster (equal
+    speeds: both rules steer).  With the agent at speed 2 (meeting arranged at
+    t = 2.5) only the CPU rule steers; the GPU rule adds zero with the bit
+    unset."""
+    s, idx, dist = _pair((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0, [((5.0, 0.0, 5.0), (-1.0, 0.0, 0.0), 1.0)])
+    orc.avoid_neighbors(s, idx, dist, s, 10.0, 0.5, False, 1.0, mode=1)
+    assert _steer(s) == pytest.approx([1.0, 0.0, 0.0], abs=1e-15)
+    for mode, want, bit in ((1, [0.0, 0.0, 0.0], 0), (0, [1.0, 0.0, 0.0], BIT_AN)):
+        s, idx, dist = _pair((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 2.0, [((2.5, 0.0, 5.0), (-1.0, 0.0, 0.0), 1.0)])
+        orc.avoid_neighbors(s, idx, dist, s, 10.0, 0.5, False, 1.0, mode=mode)
+        assert _steer(s) == pytest.approx(want, abs=1e-15), mode
+        assert s.bits[0] == bit
+
+
+def test_avoid_no_threat_adds_nothing(orc):
+    """Moving apart (nearest approach in the past) or meeting after
+    min_time: no threat, steering unchanged, no bit."""
+    for other in (((0.3, 0.0, -10.0), (0.0, 0.0, -1.0), 1.0), ((0.3, 0.0, 30.0), (0.0, 0.0, -1.0), 1.0)):
+        s, idx, dist = _pair((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0, [other])
+        s.arrays["gx"][0] = 0.25
+        orc.avoid_neighbors(s, idx, dist, s, 10.0, 0.5, False, 1.0, mode=1)
+        assert _steer(s) == [0.25, 0.0, 0.0] and s.bits[0] == 0
+
+
+def test_avoid_head_on_uses_the_last_approach_position(orc):
+    """GPU rule: the head-on offset is the approach position the kernel's
+    helper wrote last.  Threat T1 (0.2 to +x, head-on, t = 3) is chosen; T2
+    (5 to -x, head-on, t = 2 < 3, never closer than 5) is evaluated after it
+    and overwrites the approach position with (-5, 0, 2): the agent steers
+    toward +x.  The CPU rule keeps T1's approach point and steers to -x."""
+    others = [((0.2, 0.0, 6.0), (0.0, 0.0, -1.0), 1.0), ((-5.0, 0.0, 4.0), (0.0, 0.0, -1.0), 1.0)]
+    s, idx, dist = _pair((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0, others)
+    orc.avoid_neighbors(s, idx, dist, s, 10.0, 0.5, False, 1.0, mode=1)
+    assert _steer(s) == pytest.approx([1.0, 0.0, 0.0], abs=1e-15)
+    s, idx, dist = _pair((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0, others)
+    orc.avoid_neighbors(s, idx, dist, s, 10.0, 0.5, False, 1.0, mode=0)
+    assert _steer(s) == pytest.approx([-1.0, 0.0, 0.0], abs=1e-15)
+
+
+def test_avoid_close_neighbours(orc):
+    """A neighbour at (0.8, 0, 0.3) (KNN distance 0.854 < 0.5 + 1): steer
+    along the component of -offset perpendicular to forward, (-0.8, 0, 0),
+    normalized and weighted (GPU), with the close bit; the collision pass is
+    skipped.  Without avoid_close the GPU rule only looks for collisions.
+    The CPU rule always checks close neighbours (min separation 0: 0.854 <
+    1) and adds the unnormalized (-0.8, 0, 0)."""
+    other = [((0.8, 0.0, 0.3), (0.0, 0.0, -1.0), 1.0)]
+    s, idx, dist = _pair((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0, other)
+    orc.avoid_neighbors(s, idx, dist, s, 10.0, 0.5, True, 3.0, mode=1)
+    assert _steer(s) == pytest.approx([-3.0, 0.0, 0.0], abs=1e-15) and s.bits[0] == BIT_CLOSE
+    s, idx, dist = _pair((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0, other)
+    orc.avoid_neighbors(s, idx, dist, s, 10.0, 0.5, False, 3.0, mode=1)
+    assert s.bits[0] == BIT_AN  # head-on collision pass instead
+    s, idx, dist = _pair((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0, other)
+    orc.avoid_neighbors(s, idx, dist, s, 10.0, 0.5, False, 3.0, mode=0)
+    assert _steer(s) == pytest.approx([-2.4, 0.0, 0.0], abs=1e-12) and s.bits[0] == BIT_CLOSE
+    # masked: nothing
+    s, idx, dist = _pair((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1.0, other)
+    s.bits[0] = 1 << 4
+    orc.avoid_neighbors(s, idx, dist, s, 10.0, 0.5, True, 3.0, mask=1 << 4, mode=1)
+    assert _steer(s) == [0.0, 0.0, 0.0] and s.bits[0] == 1 << 4
