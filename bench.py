@@ -445,6 +445,27 @@ def plugin_bench(steps: int, warmup: int, n_kernel: int = 1 << 20, reps: int = 5
         kern["k_avoid_neighbors<1>"] = (ms, bytes_avoid_neighbors(7))
         for h in (A, B, bins, knn):
             h.close()
+        # per-cell extraction of a 2^20-agent flock over 64 x 64 x 128 bins
+        # (ComputeCellDensity / ComputeAvgCellVelocity: kernel, device-to-host
+        # copy and the temporary buffer, as the reference's calls do)
+        st, bp = scenarios.boids_box(n=n, k=7, cells=64)
+        cells = (64, 64, 128)
+        A = gs.AgentGroup(ctx, st)
+        bins = gs.KNNBinData(ctx, cells, bp.grid_size, origin=bp.grid_origin)
+        gs.updateKNNDatabase(A, bins)
+        C = cells[0] * cells[1] * cells[2]
+        calls = {}
+        for name, fn in (("cell_density", lambda: A.cell_density(C)), ("cell_velocity", lambda: A.cell_velocity(C))):
+            fn()
+            t0 = time.perf_counter()
+            for _ in range(20):
+                fn()
+            calls[name + "_call_ms"] = (time.perf_counter() - t0) * 1e3 / 20
+        res["extraction"] = dict(calls, n_agents=n, cells=C,
+                                 workload="boids_box flock, 2^20 agents, 64x64x128 bins; host outputs")
+        print(f"bench: extraction {calls}", file=sys.stderr, flush=True)
+        A.close()
+        bins.close()
     res["kernels"] = {}
     for name, (ms, b) in kern.items():
         gbs = n * b / (ms * 1e-3) / 1e9

@@ -277,6 +277,10 @@ _SIGS = {
     "gs_path_get": (C.c_int, [_h, _fp, _fp, _fp]),
     "gs_path_destroy": (C.c_int, [_h]),
     "gs_steer_follow_path": (C.c_int, [_h, _h, C.c_float, C.c_float, C.c_uint32]),
+    "gs_group_cell_density": (C.c_int, [_h, _up]),
+    "gs_group_cell_velocity": (C.c_int, [_h, _fp]),
+    "gs_write_cell_density": (C.c_int, [_h, C.c_char_p, _up, C.c_uint32, C.c_char_p, C.c_int]),
+    "gs_write_cell_velocity": (C.c_int, [_h, C.c_char_p, _up, C.c_uint32, C.c_char_p, C.c_int]),
     "gs_steer_avoid_neighbors": (C.c_int, [_h, _h, _h, C.c_float, C.c_float, C.c_int, C.c_float, C.c_uint32,
                                            C.c_int]),
     "gs_wrap_world": (C.c_int, [_h, C.c_int, _fp, _fp]),

@@ -151,6 +151,34 @@ class AgentGroup(_Group):
         d["bits"] = C.cast(s.bits, C.c_void_p).value
         return d
 
+    def cell_density(self, num_cells: int) -> np.ndarray:
+        """Agents per cell of the last bin build (ComputeCellDensity,
+        CUDA/ProcessData.cu:52-80), KNNBinDataV2 cell numbering."""
+        out = np.zeros(num_cells, np.uint32)
+        check(capi.lib.gs_group_cell_density(self.h, uptr(out)), "gs_group_cell_density")
+        return out
+
+    def cell_velocity(self, num_cells: int) -> np.ndarray:
+        """Mean velocity per cell ([C][3], ComputeAvgCellVelocity,
+        CUDA/ProcessData.cu:83-129)."""
+        out = np.zeros((num_cells, 3), np.float32)
+        check(capi.lib.gs_group_cell_velocity(self.h, fptr(out)), "gs_group_cell_velocity")
+        return out
+
+    def write_cell_density(self, prefix, selected=(), selected_prefix="Frames/cell_density",
+                           all_rows: bool = False) -> None:
+        """WriteCellDensity (src/ExtractData.cpp:19-88)."""
+        sel = np.ascontiguousarray(selected, dtype=np.uint32)
+        check(capi.lib.gs_write_cell_density(self.h, str(prefix).encode(), uptr(sel), sel.shape[0],
+                                             str(selected_prefix).encode(), int(all_rows)), "gs_write_cell_density")
+
+    def write_cell_velocity(self, prefix, selected=(), selected_prefix="Frames/cell_avg_velocity",
+                            all_rows: bool = False) -> None:
+        """WriteAvgCellVelocity (src/ExtractData.cpp:90-140)."""
+        sel = np.ascontiguousarray(selected, dtype=np.uint32)
+        check(capi.lib.gs_write_cell_velocity(self.h, str(prefix).encode(), uptr(sel), sel.shape[0],
+                                              str(selected_prefix).encode(), int(all_rows)), "gs_write_cell_velocity")
+
     def check_finite(self) -> Tuple[int, int]:
         bad = C.c_uint32()
         st = capi.lib.gs_group_check_finite(self.h, C.byref(bad))

@@ -593,6 +593,70 @@ int gs_group_upload_wall_set(gs_group g, gs_wall_set w) {
     });
 }
 
+// ComputeCellDensity / ComputeAvgCellVelocity (CUDA/ProcessData.cu:52-129)
+// into host memory through a temporary device buffer.
+static void extract_to_host(gs_group_s* g, size_t count, void* host, bool velocity) {
+    if (!g->binned || !g->snap_valid) fail(GS_ERR_NOT_BINNED, "group has no bin database (gs_update_knn_database)");
+    use_device(g->ctx);
+    hipStream_t st = g->ctx->stream;
+    if (count == 0) return;
+    DevBuf<uint32_t> dev;  // uint32 or float elements, 4 bytes each
+    dev.reserve(count);
+    try {
+        if (velocity) launch_cell_velocity(g, (float*)dev.p);
+        else launch_cell_density(g, dev.p);
+        GS_HIP(hipMemcpyAsync(host, dev.p, count * 4, hipMemcpyDeviceToHost, st));
+        GS_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        dev.release();
+        throw;
+    }
+    dev.release();
+}
+
+int gs_group_cell_density(gs_group g, uint32_t* agents_per_cell) {
+    return guarded([&] {
+        check_group(g, "group");
+        if (!agents_per_cell) fail(GS_ERR_INVALID_ARGUMENT, "output is NULL");
+        extract_to_host(g, g->bin_cells, agents_per_cell, false);
+    });
+}
+
+int gs_group_cell_velocity(gs_group g, float* avg_velocity) {
+    return guarded([&] {
+        check_group(g, "group", GS_KIND_AGENTS);
+        if (!avg_velocity) fail(GS_ERR_INVALID_ARGUMENT, "output is NULL");
+        extract_to_host(g, (size_t)g->bin_cells * 3, avg_velocity, true);
+    });
+}
+
+int gs_write_cell_density(gs_group g, const char* prefix, const uint32_t* selected_cells, uint32_t n_selected,
+                          const char* selected_prefix, int all_rows) {
+    return guarded([&] {
+        check_group(g, "group");
+        if (!prefix) fail(GS_ERR_INVALID_ARGUMENT, "prefix is NULL");
+        if (n_selected && (!selected_cells || !selected_prefix))
+            fail(GS_ERR_INVALID_ARGUMENT, "selected cells without a prefix");
+        std::vector<uint32_t> d(g->bin_cells);
+        extract_to_host(g, d.size(), d.data(), false);
+        write_cell_density(g->bin_grid, d, prefix, selected_cells, n_selected, selected_prefix, all_rows);
+    });
+}
+
+int gs_write_cell_velocity(gs_group g, const char* prefix, const uint32_t* selected_cells, uint32_t n_selected,
+                           const char* selected_prefix, int all_rows) {
+    return guarded([&] {
+        check_group(g, "group", GS_KIND_AGENTS);
+        if (!prefix) fail(GS_ERR_INVALID_ARGUMENT, "prefix is NULL");
+        if (n_selected && (!selected_cells || !selected_prefix))
+            fail(GS_ERR_INVALID_ARGUMENT, "selected cells without a prefix");
+        std::vector<float> v((size_t)g->bin_cells * 3);
+        extract_to_host(g, v.size(), v.data(), true);
+        write_cell_velocity(g->bin_grid, v, prefix, selected_cells, n_selected, selected_prefix, all_rows);
+    });
+}
+
 int gs_group_download_bins(gs_group g, uint32_t* key, uint32_t* start, uint32_t* perm, float* spx, float* spy,
                            float* spz) {
     return guarded([&] {

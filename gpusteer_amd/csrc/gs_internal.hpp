@@ -369,6 +369,14 @@ void launch_anti_penetration_agents(gs_group_s* A, gs_knn_s* knn, gs_group_s* B,
 void launch_follow_path(gs_group_s* A, const gs_path_s* p, float prediction_time, float w, uint32_t mask);
 void launch_avoid_neighbors(gs_group_s* A, gs_knn_s* knn, gs_group_s* B, float min_time, float min_sep,
                             int avoid_close, float w, uint32_t mask, int mode);
+// Per-cell extraction of a group's last bin build, in KNNBinDataV2's cell
+// numbering: agents per cell (uint32 [C]) and mean velocity (float [C][3]).
+void launch_cell_density(gs_group_s* g, uint32_t* out);
+void launch_cell_velocity(gs_group_s* g, float* out);
+void write_cell_density(const GridDev& G, const std::vector<uint32_t>& d, const char* prefix,
+                        const uint32_t* selected, uint32_t n_selected, const char* selected_prefix, int all_rows);
+void write_cell_velocity(const GridDev& G, const std::vector<float>& v, const char* prefix,
+                         const uint32_t* selected, uint32_t n_selected, const char* selected_prefix, int all_rows);
 void launch_wrap(gs_group_s* A, int shape, const float* c, const float* s);
 void launch_clear_bits(gs_group_s* A);
 void launch_check_finite(gs_group_s* A, uint32_t* first_bad_slot_host);

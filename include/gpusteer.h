@@ -32,6 +32,10 @@
  *   gs_steer_avoid_neighbors GSL:111  steerToAvoidNeighbors(A, KNNData*, B, minTimeToCollision,
  *                                     minSeparationDistance, bAvoidCloseNeighbors, w, mask)
  *   gs_path_*                include/OpenSteer/CUDA/PolylinePathwayCUDA.cuh:11-68
+ *   gs_group_cell_density    ComputeCellDensity(AgentGroup*, uint*) (CUDA/ProcessData.cu:52-80)
+ *   gs_group_cell_velocity   ComputeAvgCellVelocity(AgentGroup*, float3*) (CUDA/ProcessData.cu:83-129)
+ *   gs_write_cell_density    WriteCellDensity(prefix, AgentGroup*, selected) (src/ExtractData.cpp:19-88)
+ *   gs_write_cell_velocity   WriteAvgCellVelocity(prefix, AgentGroup*, selected) (src/ExtractData.cpp:90-140)
  *   gs_group_*               include/OpenSteer/AgentGroup.h:32-54, ObstacleGroup.h:23-37,
  *                            WallGroup.h:18-33 (AddAgent/SyncDevice/GetDataForAgent/pd*)
  *   gs_knn_*                 include/OpenSteer/CUDA/KNNData.cuh:31-54
@@ -236,6 +240,26 @@ int gs_steer_avoid_obstacles(gs_group A, gs_group obstacles, gs_knn knn, float m
 int gs_steer_avoid_walls(gs_group A, gs_knn knn, gs_group walls, float min_time_to_collision, float weight,
                          uint32_t mask);
 int gs_update_group(gs_group A, float elapsed_time, int mode);
+
+/* ---- per-cell extraction (density and mean velocity) ---------------------- */
+/* Of the group's last bin build (gs_update_knn_database), C = cells of its
+ * lattice, in KNNBinDataV2's cell numbering ix + iz*X + iy*X*Z
+ * (CUDA/KNNBinDataV2.cu:103-135).  Density: agents per cell.  Velocity:
+ * the mean of forward * speed over each cell's agents (summed in agent
+ * index order, divided by the count), (0, 0, 0) for an empty cell; the
+ * current forward and speed of the agents binned there. */
+int gs_group_cell_density(gs_group g, uint32_t* agents_per_cell /* host [C] */);
+int gs_group_cell_velocity(gs_group g, float* avg_velocity /* host [C][3] */);
+/* The reference's text files: "<prefix>.dat" (overwritten) with its header
+ * and rows i = 0 .. Z-2 of cells i*X + j (all Z rows when all_rows != 0; the
+ * reference stops one row short); each selected cell's value appended to
+ * "<selected_prefix>_<cell>.dat" (the reference uses "Frames/cell_density"
+ * and "Frames/cell_avg_velocity").  Floats print as std::ostream's default
+ * (%g). */
+int gs_write_cell_density(gs_group g, const char* prefix, const uint32_t* selected_cells, uint32_t n_selected,
+                          const char* selected_prefix, int all_rows);
+int gs_write_cell_velocity(gs_group g, const char* prefix, const uint32_t* selected_cells, uint32_t n_selected,
+                           const char* selected_prefix, int all_rows);
 
 /* ---- plugin-compat kernels (the choke-point plugins' per-step extras) ----- */
 /* Closest point of a wall segment for the within-radius test of

@@ -638,6 +638,29 @@ inline void antiPenetrationAgents(AgentGroup* pAgentGroup, KNNData* pKNNData, Ag
     pAgentGroup->SetSyncHost();
 }
 
+// ---- ProcessData.cu / ExtractData.h -----------------------------------------
+// Of the group's last updateKNNDatabase, KNNBinDataV2's cell numbering.
+inline void ComputeCellDensity(AgentGroup* pAgentGroup, uint* pAgentsPerCell) {
+    gpusteer::check(gs_group_cell_density(pAgentGroup->device_handle(), pAgentsPerCell), "gs_group_cell_density");
+}
+inline void ComputeAvgCellVelocity(AgentGroup* pAgentGroup, float3* pAvgCellVelocity) {
+    static_assert(sizeof(float3) == 3 * sizeof(float), "float3 must be packed");
+    gpusteer::check(gs_group_cell_velocity(pAgentGroup->device_handle(), (float*)pAvgCellVelocity),
+                    "gs_group_cell_velocity");
+}
+inline void WriteCellDensity(char const* szFilenamePrefix, AgentGroup* pAgentGroup,
+                             std::vector<uint> vecSelectedCells = std::vector<uint>()) {
+    gpusteer::check(gs_write_cell_density(pAgentGroup->device_handle(), szFilenamePrefix, vecSelectedCells.data(),
+                                          (uint32_t)vecSelectedCells.size(), "Frames/cell_density", 0),
+                    "gs_write_cell_density");
+}
+inline void WriteAvgCellVelocity(char const* szFilenamePrefix, AgentGroup* pAgentGroup,
+                                 std::vector<uint> vecSelectedCells = std::vector<uint>()) {
+    gpusteer::check(gs_write_cell_velocity(pAgentGroup->device_handle(), szFilenamePrefix, vecSelectedCells.data(),
+                                           (uint32_t)vecSelectedCells.size(), "Frames/cell_avg_velocity", 0),
+                    "gs_write_cell_velocity");
+}
+
 // ---- Simulation.h parameter classes -----------------------------------------
 
 struct GroupParams {

@@ -381,7 +381,8 @@ def choke_new_step(state, sc, path: "Pathway", variant="new", closest=CLOSEST_CL
     variant "group": CtfEnemyGroup::update (plugins/ChokePointGroup.cpp:683-728)
     -- walls, path, seek, separation, alignment, cohesion, the integrate with
     wall anti-penetration, agent anti-penetration masked by AVOID_WALLS |
-    ANTI_PENETRATION_WALL.  Returns (self KNN, wall KNN)."""
+    ANTI_PENETRATION_WALL.  Returns (self KNN, wall KNN, the positions the
+    step binned)."""
     p = sc.params
     size = sc.world_size
     wrap_world(state, WRAP_BOX, (0.0, 0.0, 0.0), size)
@@ -412,7 +413,7 @@ def choke_new_step(state, sc, path: "Pathway", variant="new", closest=CLOSEST_CL
             steer_flock(which, state, ks, state, lo, hi, p.cos_max_flocking_angle, w, m, MODE_GPUREF)
         update_anti_penetration(state, kw, sc.walls, sc.dt, closest)
         anti_penetration_agents(state, ks, ds, state, (1 << 8) | (1 << 12))
-    return ks, kw
+    return ks, kw, pos
 
 
 def multi_group_step(states, sc, paths, mode=MODE_GPUREF):

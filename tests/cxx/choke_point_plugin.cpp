@@ -12,6 +12,9 @@
 //     max_force, mass.
 // OUT: n x (px,py,pz, fx,fy,fz, speed, sx,sy,sz, ux,uy,uz) floats in
 //     agent-id order.
+// Every 30th frame the plugin's extractData (:413-435) writes
+// Frames/cell_density_frame_<f>.dat and Frames/cell_avg_velocity_frame_<f>.dat
+// and appends the selected cells' values (relative to the working directory).
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -85,6 +88,9 @@ int main(int argc, char** argv) {
         KNNData knnWalls(n, sp.m_nKNW);
         PolylinePathwayCUDA path(gp.m_vecPathPoints, gp.m_fPathRadius, gp.m_bPathIsCyclic);
 
+        // CtfSimulation's selected cells (plugins/ChokePointNew.cpp:342-350).
+        const std::vector<uint> cells = {16512, 31872, 32896, 33920, 34944, 35968, 36992, 38016};
+        uint frame = 1;
         for (int s = 0; s < steps; ++s) {
             wrapWorld(&group, wp.m_f3Dimensions);
             updateKNNDatabase(&group, &bins);
@@ -104,6 +110,15 @@ int main(int argc, char** argv) {
             steerForSeek(&group, path.hvPoints()[1], sp.m_fWeightSeek, sp.m_nMaskSeek);
             updateGroup(&group, elapsedTime);
             antiPenetrationAgents(&group, &knnSelf, &group, sp.m_nMaskAntiPenetrationAgents);
+            // extractData
+            if (frame % 30 == 0) {
+                char name[100];
+                snprintf(name, sizeof name, "Frames/cell_density_frame_%u", frame);
+                WriteCellDensity(name, &group, cells);
+                snprintf(name, sizeof name, "Frames/cell_avg_velocity_frame_%u", frame);
+                WriteAvgCellVelocity(name, &group, cells);
+            }
+            ++frame;
         }
 
         FILE* out = fopen(argv[6], "wb");

@@ -113,7 +113,11 @@ def test_cxx_choke_point_plugin_matches_oracle(orc, tmp_path):
     separation, alignment, cohesion, path following, seek, integrate and
     agent anti-penetration) equals the oracle's sequence bit for bit.  The
     drop-in's wall KNN is uncapped, the oracle's capped at the walls' reach:
-    equal states confirm the cap changes no force."""
+    equal states confirm the cap changes no force.  The plugin's data
+    extraction at frames 30 and 60 (WriteCellDensity / WriteAvgCellVelocity,
+    the selected cells' appended lines) equals the oracle writers' text."""
+    import extract_ref as ex
+
     from gpusteer_amd import formats
 
     assert CHOKE_PLUGIN.exists(), f"{CHOKE_PLUGIN} not built"
@@ -131,14 +135,38 @@ def test_cxx_choke_point_plugin_matches_oracle(orc, tmp_path):
             st.arrays[fld].astype(np.float32).tofile(f)
     steps = 60
     print("cxx choke-point plugin", flush=True)
+    (tmp_path / "Frames").mkdir()
     r = subprocess.run([str(CHOKE_PLUGIN), str(FILES / "ChokePoint.params"), str(FILES), str(inp), str(steps),
-                        repr(float(np.float32(sc.dt))), str(out)], capture_output=True, text=True, timeout=120)
+                        repr(float(np.float32(sc.dt))), str(out)], capture_output=True, text=True, timeout=120,
+                       cwd=tmp_path)
     assert r.returncode == 0, r.stderr
     rec = np.fromfile(out, dtype=np.float32).reshape(st.n, 13)
     ref = st.copy()
     R = orc.Pathway(sc.path_points, sc.path_radius, sc.path_cyclic)
-    for _ in range(steps):
-        orc.choke_new_step(ref, sc, R, "new")
+    cells = tuple(p.cells)
+    g = orc.grid(tuple(-0.5 * np.float32(v) for v in p.dimensions), p.dimensions, cells)
+    selected = [16512, 31872, 32896, 33920, 34944, 35968, 36992, 38016]
+    lines_d = {c: "" for c in selected}
+    lines_v = {c: "" for c in selected}
+    for s in range(1, steps + 1):
+        _, _, pos = orc.choke_new_step(ref, sc, R, "new")
+        if s % 30 == 0:
+            key = orc.bins_build(g, pos)[0]
+            dens = ex.cell_density(key, cells)
+            fwd = np.stack([ref.arrays[f] for f in ("fx", "fy", "fz")], axis=1)
+            vel = ex.cell_velocity(key, cells, fwd, ref.arrays["speed"])
+            ex.write_density(tmp_path / f"ref_density_{s}", dens, cells)
+            ex.write_velocity(tmp_path / f"ref_velocity_{s}", vel, cells)
+            assert (tmp_path / f"Frames/cell_density_frame_{s}.dat").read_text() == \
+                (tmp_path / f"ref_density_{s}.dat").read_text()
+            assert (tmp_path / f"Frames/cell_avg_velocity_frame_{s}.dat").read_text() == \
+                (tmp_path / f"ref_velocity_{s}.dat").read_text()
+            for c in selected:
+                lines_d[c] += f"{dens[c]}\n"
+                lines_v[c] += f"{float(vel[c][0]):g}\t{float(vel[c][2]):g}\n"
+    for c in selected:
+        assert (tmp_path / f"Frames/cell_density_{c}.dat").read_text() == lines_d[c]
+        assert (tmp_path / f"Frames/cell_avg_velocity_{c}.dat").read_text() == lines_v[c]
     want = np.stack([ref.arrays[f] for f in ("px", "py", "pz", "fx", "fy", "fz", "speed", "sx", "sy", "sz", "ux",
                                              "uy", "uz")], axis=1)
     np.testing.assert_array_equal(rec, want)

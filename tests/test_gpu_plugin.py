@@ -212,7 +212,7 @@ def test_choke_point_plugin_steps_bit_exact(gs, ctx, orc, choke_files, variant, 
             sim.step()
         else:
             sim.step_with_wall_update()
-        ks, kw = orc.choke_new_step(ref, sc, R, variant, closest)
+        ks, kw, _ = orc.choke_new_step(ref, sc, R, variant, closest)
         if step % 20 == 0 or step == 79:
             np.testing.assert_array_equal(sim.knn_self.download()[0], ks, err_msg=f"step {step}")
             np.testing.assert_array_equal(sim.knn_walls.download()[0], kw, err_msg=f"step {step}")
