@@ -490,10 +490,24 @@ def cpu_baseline(st, p, seconds: float):
         el = time.perf_counter() - t0
         if el >= seconds or steps >= 50:
             break
+    # the same restatement on one core, over fewer steps of the same state
+    orc.set_threads(1)
+    s1 = st.copy()
+    t0 = time.perf_counter()
+    steps1 = 0
+    while True:
+        orc.boids_step(s1, p)
+        steps1 += 1
+        el1 = time.perf_counter() - t0
+        if el1 >= seconds / 2 or steps1 >= 3:
+            break
+    orc.set_threads(threads)
     return {
         "value": st.n * steps / el,
         "unit": "agent-steps/s",
         "cores": threads,
+        "single_core_value": st.n * steps1 / el1,
+        "single_core_steps": steps1,
         "kind": "port",
         "sample": f"{steps} full steps of the same {st.n}-agent state (oracle restatement: stable counting-sort "
                   f"bins, exact binned KNN, SteerLibrary forces, SimpleVehicle integrate; OpenMP over agents)",
@@ -518,6 +532,7 @@ def main() -> int:
     ap.add_argument("--calib-agents", type=int, default=16384)
     ap.add_argument("--calib-steps", type=int, default=100)
     ap.add_argument("--no-calib", action="store_true")
+    ap.add_argument("--no-k16", action="store_true")
     ap.add_argument("--plugin-steps", type=int, default=100)
     ap.add_argument("--no-plugin", action="store_true")
     ap.add_argument("--only", choices=("ensemble", "adjoint", "plugin"),
@@ -581,9 +596,25 @@ def main() -> int:
         tsteps = min(args.steps, 50)
         bin_ms, knn_ms, si_ms, step_ms = gs.stepBoidsTimed(A, knn, bins, p, tsteps)
         status, bad = A.check_finite()
+        knn.close()
+        # The same flock with k = 16 (the other neighbour count of the
+        # workload), continuing from the current state on the same lattice.
+        k16 = None
+        if args.k != 16 and not args.no_k16:
+            knn16 = gs.KNNData(ctx, n, 16)
+            gs.stepBoids(A, knn16, bins, p, 5)
+            ctx.synchronize()
+            s16 = max(args.steps // 2, 1)
+            t0 = time.perf_counter()
+            gs.stepBoids(A, knn16, bins, p, s16)
+            ctx.synchronize()
+            el16 = time.perf_counter() - t0
+            knn16.close()
+            k16 = {"agent_steps_per_s": n * s16 / el16, "ms_per_step": el16 * 1e3 / s16, "steps": s16,
+                   "cells": list(p.grid_cells)}
+            print(f"bench: k=16 {el16 * 1e3 / s16:.3f} ms/step", file=sys.stderr, flush=True)
         A.close()
         bins.close()
-        knn.close()
     if status != 0:
         print(f"bench: non-finite state at agent {bad}", file=sys.stderr)
         return 1
@@ -637,6 +668,7 @@ def main() -> int:
             "kernel_ms": knn_ms,
         },
         "stage_ms": {"bin_build": bin_ms, "knn": knn_ms, "steer_integrate": si_ms},
+        "k16": k16,
         "roofline_step": {
             "what": "whole step, algorithmic bytes 224 + 8k + 12 C/N per agent-step over the timed wall time",
             "bytes_per_agent_step": bytes_per_agent_step(args.k, C, n),
