@@ -32,7 +32,7 @@ namespace gs {
 // One lane per query slot s of A (storage = cell order, so a wavefront's lanes
 // share cells and their candidate runs overlap in cache).
 template <int KMAX, int RF>
-__global__ __launch_bounds__(kBlock) void k_knn_binned(uint32_t nA, const float4* __restrict__ apos, uint32_t nB,
+__global__ __launch_bounds__(kBlock, (KMAX <= 8 && RF == 1) ? 5 : 1) void k_knn_binned(uint32_t nA, const float4* __restrict__ apos, uint32_t nB,
                                                        const float4* __restrict__ bpos,
                                                        const uint32_t* __restrict__ bstart, GridDev g, int self,
                                                        uint32_t k, int R0, float cap2, const uint32_t* __restrict__ seed,
@@ -240,7 +240,10 @@ static void run_binned_rf(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t 
     // finer lattices.
     // The 3 x 3 path addresses B with 32-bit byte offsets (fewer than 2^27
     // points); larger groups take the 5 x 5 walk, which is exact as well.
-    if (R0 >= 2 || B->n >= kFlatMaxPoints) run_binned<KMAX, 2>(A, out, B, R0, cap2, seed, row_map, val_map);
+    // Lists longer than 8 keep the 3 x 3 path on every lattice: the 5 x 5
+    // walk holding 16 keys exceeds the register file and spills.
+    if ((R0 >= 2 && KMAX <= 8) || B->n >= kFlatMaxPoints)
+        run_binned<KMAX, 2>(A, out, B, R0, cap2, seed, row_map, val_map);
     else run_binned<KMAX, 1>(A, out, B, R0, cap2, seed, row_map, val_map);
 }
 
