@@ -81,6 +81,12 @@ __device__ __forceinline__ uint32_t id_of(const float4& p) { return __float_as_u
 __device__ __forceinline__ float sqdist4(float qx, float qy, float qz, const float4& p) {
     return sqdist(qx, qy, qz, p.x, p.y, p.z);
 }
+// d2 with fused multiply-adds: within a few ulps of sqdist4, for filters only
+// (a key is always formed from sqdist4).
+__device__ __forceinline__ float sqdist_filter(float qx, float qy, float qz, const float4& p) {
+    const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
 
 // Distance from coordinate q to the slab of cells [i0, i1] along one axis;
 // boundary cells extend to infinity outward (points outside the lattice are
@@ -383,7 +389,7 @@ __device__ __forceinline__ int axis_cell_bound(float p, float o, float r, int d)
 
 template <int KMAX>
 __device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLds& L, __amdgpu_buffer_rsrc_t rb,
-                                           float qx, float qy, float qz, uint32_t self_slot) {
+                                           float qx, float qy, float qz, uint32_t self_slot, float thr) {
     for (int e0 = 0; e0 < kFlatBuf; e0 += 4) {
         if (!__any(e0 < cnt)) break;
         uint32_t sl[4];
@@ -396,8 +402,9 @@ __device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLd
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (e0 + u < cnt && sl[u] != self_slot) {
-                const uint64_t kk = make_key(sqdist4(qx, qy, qz, p[u]), id_of(p[u]));
-                if (h.accepts(kk)) h.push(kk, sl[u]);
+                const float d2 = sqdist4(qx, qy, qz, p[u]);
+                const uint64_t kk = make_key(d2, id_of(p[u]));
+                if (d2 <= thr && h.accepts(kk)) h.push(kk, sl[u]);
             }
         }
     }
@@ -583,6 +590,11 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint
     constexpr uint32_t kEntry = kBlock * 4u;
     uint32_t wa = L.slot_addr;
     const uint32_t wfull = L.slot_addr + (kFlatBuf - 8) * kEntry;
+    // The scan only filters: a candidate enters the buffer when its fused
+    // d2 is within the bound widened by `grow` (more than the few ulps by
+    // which the fused form can differ from the exact one), and the merge
+    // recomputes the exact key and bound.
+    float limf = lim * grow;
     for (;;) {
         while (__any(ta < ea) && !__any(wa > wfull)) {
             // Byte offsets of the two chunks; the per-candidate offsets are
@@ -600,15 +612,15 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint
             const uint32_t na = ea - ta, nb = eb - tb;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const float d2 = sqdist4(qx, qy, qz, pa[u]);
+                const float d2 = sqdist_filter(qx, qy, qz, pa[u]);
                 *(lds_u32*)(size_t)wa = ta + (uint32_t)u;
-                wa += ((uint32_t)u < na && d2 <= lim) ? kEntry : 0u;
+                wa += ((uint32_t)u < na && d2 <= limf) ? kEntry : 0u;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const float d2 = sqdist4(qx, qy, qz, pb[u]);
+                const float d2 = sqdist_filter(qx, qy, qz, pb[u]);
                 *(lds_u32*)(size_t)wa = tb + (uint32_t)u;
-                wa += ((uint32_t)u < nb && d2 <= lim) ? kEntry : 0u;
+                wa += ((uint32_t)u < nb && d2 <= limf) ? kEntry : 0u;
             }
             ta = tc;
             ea = ec;
@@ -616,9 +628,10 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint
             eb = ed;
         }
         int cnt = (int)((wa - L.slot_addr) / kEntry);
-        flat_flush<KMAX>(h, cnt, L, rb, qx, qy, qz, self_slot);
+        flat_flush<KMAX>(h, cnt, L, rb, qx, qy, qz, self_slot, thr);
         wa = L.slot_addr;
         lim = fminf(thr, h.worst());
+        limf = lim * grow;
         if (!__any(ta < ea)) break;
     }
     bool all;
