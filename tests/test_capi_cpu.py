@@ -45,6 +45,13 @@ def test_null_handles_rejected():
     assert L.gs_steer_separation(None, None, None, 1, 2, 0, 1, 0, 0) == c.GS_ERR_INVALID_HANDLE
     assert L.gs_update_group(None, 0.1, 0) == c.GS_ERR_INVALID_HANDLE
     assert L.gs_knn_download(None, None, None) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_update_group_anti_penetration(None, None, None, 0.1, 0) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_anti_penetration_agents(None, None, None, 0) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_steer_follow_path(None, None, 1.0, 1.0, 0) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_steer_avoid_neighbors(None, None, None, 1.0, 0.5, 0, 1.0, 0, 1) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_path_destroy(None) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_group_cell_density(None, None) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_write_cell_velocity(None, b"x", None, 0, None, 0) == c.GS_ERR_INVALID_HANDLE
     assert b"invalid" in L.gs_last_error()
     # A pointer that was never a handle is rejected, not dereferenced.
     bogus = C.c_void_p(0x1234)
