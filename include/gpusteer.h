@@ -175,7 +175,9 @@ int gs_group_download_agents(gs_group g, gs_agents* host);
 /* Device pointers of the agent arrays, in storage order: each bin build
  * (gs_update_knn_database, gs_step_boids) reorders a group's state into
  * cell order, so storage slot s holds agent id_of_slot[s] (see
- * gs_group_device_order).  Valid until the next build or upload. */
+ * gs_group_device_order).  Valid until the next build, upload,
+ * gs_step_boids or gs_anti_penetration_agents on the group (these write
+ * the group's alternate buffers and switch to them). */
 int gs_group_device_agents(gs_group g, gs_agents* dev);
 /* The group's storage order: agent index of each slot and slot of each agent
  * index (device arrays of n entries; valid until the next build or upload). */
