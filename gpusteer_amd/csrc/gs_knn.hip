@@ -32,7 +32,7 @@ namespace gs {
 // One lane per query slot s of A (storage = cell order, so a wavefront's lanes
 // share cells and their candidate runs overlap in cache).
 template <int KMAX, int RF>
-__global__ __launch_bounds__(kBlock, (KMAX <= 8 && RF == 1) ? 5 : 1) void k_knn_binned(uint32_t nA, const float4* __restrict__ apos, uint32_t nB,
+__global__ __launch_bounds__(kBlock, RF == 1 ? (KMAX <= 8 ? 5 : (KMAX <= 16 ? 4 : 1)) : 1) void k_knn_binned(uint32_t nA, const float4* __restrict__ apos, uint32_t nB,
                                                        const float4* __restrict__ bpos,
                                                        const uint32_t* __restrict__ bstart, GridDev g, int self,
                                                        uint32_t k, int R0, float cap2, const uint32_t* __restrict__ seed,
