@@ -78,7 +78,8 @@ def build(verbose: bool = False) -> Path:
 ROOT = PKG.parent
 CXX_EXAMPLES = {"boids_group_plugin": ROOT / "tests" / "cxx" / "boids_group_plugin.cpp",
                 "choke_setup": ROOT / "tests" / "cxx" / "choke_setup.cpp",
-                "choke_point_plugin": ROOT / "tests" / "cxx" / "choke_point_plugin.cpp"}
+                "choke_point_plugin": ROOT / "tests" / "cxx" / "choke_point_plugin.cpp",
+                "multi_group_plugin": ROOT / "tests" / "cxx" / "multi_group_plugin.cpp"}
 
 
 def build_cxx_examples() -> None:

@@ -1,8 +1,11 @@
-"""GPU parity of the C++ drop-in layer (include/gpusteer_opensteer.hpp): a
-Boids plugin written against the reference's group API
-(tests/cxx/boids_group_plugin.cpp, the per-frame sequence of
-plugins/BoidsGroup.cpp:606-637) run for 20 steps, compared bit for bit with
-the oracle running the same sequence."""
+"""GPU parity of the C++ drop-in layer (include/gpusteer_opensteer.hpp):
+plugins written against the reference's group API -- Boids
+(tests/cxx/boids_group_plugin.cpp, plugins/BoidsGroup.cpp:606-637), the
+choke-point world set-up and frame (tests/cxx/choke_setup.cpp,
+tests/cxx/choke_point_plugin.cpp, plugins/ChokePointNew.cpp) and the
+multi-group frame (tests/cxx/multi_group_plugin.cpp, plugins/MultiGroup.cpp)
+-- run as programs and compared bit for bit with the oracle running the same
+sequence."""
 import subprocess
 
 import numpy as np
@@ -170,3 +173,56 @@ def test_cxx_choke_point_plugin_matches_oracle(orc, tmp_path):
     want = np.stack([ref.arrays[f] for f in ("px", "py", "pz", "fx", "fy", "fz", "speed", "sx", "sy", "sz", "ux",
                                              "uy", "uz")], axis=1)
     np.testing.assert_array_equal(rec, want)
+
+
+MULTI_PLUGIN = gbuild.LIB_DIR / "multi_group_plugin"
+
+
+@pytest.mark.parametrize("name,close", [("MultiGroup.params", 0), ("MultiGroup4.params", 1)])
+def test_cxx_multi_group_plugin_matches_oracle(orc, tmp_path, name, close):
+    """plugins/MultiGroup.cpp's frame compiled against the drop-in
+    (tests/cxx/multi_group_plugin.cpp: every group's preUpdate, then each
+    group's update -- KNN to every group, avoid the others, separation from
+    all, alignment and cohesion with its own, path and seek, integrate,
+    anti-penetration against all) on the reference's MultiGroup files
+    (avoidCloseNeighbors set to `close`), 1500 agents per group started
+    0.15 x as far out within 30, 30 frames of dt 0.5: every group's state
+    equals the oracle's uncapped sequence bit for bit."""
+    import dataclasses
+    import re
+
+    from gpusteer_amd import formats
+
+    assert MULTI_PLUGIN.exists(), f"{MULTI_PLUGIN} not built"
+    text = re.sub(r"(avoidCloseNeighbors\s*=\s*)\d+", rf"\g<1>{close}", (FILES / name).read_text())
+    params = tmp_path / name
+    params.write_text(text)
+    p = formats.load_params(params)
+    assert p.avoid_close_neighbors == close
+    sc = scenarios.multi_group(p, n_per_group=1500, dt=0.5, max_start_radius=30.0, start_scale=0.15)
+    inp, out = tmp_path / "agents.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        np.array([len(sc.states)], np.uint32).tofile(f)
+        for st in sc.states:
+            np.array([st.n], np.uint32).tofile(f)
+            for fld in FIELDS_IN:
+                st.arrays[fld].astype(np.float32).tofile(f)
+    steps = 30
+    print(f"cxx multi-group plugin {name} close={close} groups={len(sc.states)}", flush=True)
+    r = subprocess.run([str(MULTI_PLUGIN), str(params), str(inp), str(steps), repr(float(np.float32(sc.dt))),
+                        str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rec = np.fromfile(out, dtype=np.float32)
+    refs = [s.copy() for s in sc.states]
+    paths = [orc.Pathway(pts, rr, c) if len(pts) >= 2 else None for pts, rr, c in sc.paths]
+    uncapped = dataclasses.replace(sc, knn_cap=float("inf"))
+    for _ in range(steps):
+        orc.multi_group_step(refs, uncapped, paths)
+    off = 0
+    for gi, ref in enumerate(refs):
+        got = rec[off:off + 13 * ref.n].reshape(ref.n, 13)
+        off += 13 * ref.n
+        want = np.stack([ref.arrays[f] for f in ("px", "py", "pz", "fx", "fy", "fz", "speed", "sx", "sy", "sz",
+                                                 "ux", "uy", "uz")], axis=1)
+        np.testing.assert_array_equal(got, want, err_msg=f"group {gi}")
+    assert off == rec.size
