@@ -53,6 +53,20 @@ __device__ __forceinline__ void flock_core(f3 pos, f3 fwd, float radius, int m, 
     const bool do_a = ali.on && !(bits & ali.mask);
     const bool do_c = coh.on && !(bits & coh.mask);
     if (!do_s && !do_a && !do_c) return;
+    // The cone input dot3(fwd, off / d) only enters the comparisons
+    // `> cos_max` of the three terms.  Its estimate fma-dot(fwd, off) times
+    // the hardware reciprocal of d (1 ulp) lies within 12 u |fwd| of it
+    // (u = 2^-24: three roundings of the estimate's dot, the reciprocal and
+    // its product, against four on each term of the exact form, with
+    // |off| / d <= 1 + 3u).  An estimate farther than
+    // E = 2^-18 (|fx| + |fy| + |fz|) + 2^-100 from every threshold therefore
+    // decides all three comparisons as the exact value does; otherwise, or
+    // for d outside [2^-40, 2^40] or a forward sum above 2^40 (where
+    // underflow or overflow could void the bound; NaN fails the tests too),
+    // the exact value is computed.
+    const float fsum = (fabsf(fwd.x) + fabsf(fwd.y)) + fabsf(fwd.z);
+    const bool fok = fsum <= 0x1p40f;
+    const float cone_e = __builtin_fmaf(fsum, 0x1p-18f, 0x1p-100f);
     f3 ss = mk3(0.f, 0.f, 0.f), sa = ss, sc = ss;
     int ns = 0, na = 0, nc = 0;
 #pragma unroll
@@ -61,7 +75,11 @@ __device__ __forceinline__ void flock_core(f3 pos, f3 fwd, float radius, int m, 
             const f3 off = bp[j] - pos;
             const float d2 = dot3(off, off);
             const float d = sqrtf(d2);
-            const float cdot = dot3(fwd, off / d);
+            float cdot = __builtin_fmaf(fwd.z, off.z, __builtin_fmaf(fwd.y, off.y, fwd.x * off.x)) *
+                         __builtin_amdgcn_rcpf(d);
+            const bool sure = fok && d >= 0x1p-40f && d <= 0x1p40f && fabsf(cdot - sep.cos_max) > cone_e &&
+                              fabsf(cdot - ali.cos_max) > cone_e && fabsf(cdot - coh.cos_max) > cone_e;
+            if (!sure) cdot = dot3(fwd, off / d);
             if (do_s && in_hood_pre<MODE>(d2, d, cdot, radius, sep)) {
                 ss = ss + off / -d2;
                 ++ns;
