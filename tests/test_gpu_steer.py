@@ -185,3 +185,80 @@ def test_check_finite(gs, ctx):
     status, bad = A.check_finite()
     assert status == 8 and bad == 417
     A.close()
+
+
+def _cone_lattice(n_side=16, spacing=0.7, origin=(100.3, -57.1, 33.9), jitter=2e-5):
+    """Agents on a lattice far from the origin, moved by a few ulps of their
+    coordinates, all with one forward vector: the cone inputs
+    dot(fwd, off / d) of each lattice direction form clusters about 1e-4
+    wide, many times the estimate's margin."""
+    ax = np.arange(n_side, dtype=np.float32) * np.float32(spacing)
+    g = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), axis=-1).reshape(-1, 3)
+    r = np.random.default_rng(5)
+    pos = (g + np.asarray(origin, np.float32) + r.uniform(-jitter, jitter, size=g.shape)).astype(np.float32)
+    f = np.array([1.0, 2.0, 3.0], np.float32)
+    f = (f / np.float32(np.sqrt(np.float32(14.0)))).astype(np.float32)
+    fwd = np.tile(f, (len(pos), 1))
+    return scenarios.make_agents(pos, fwd, np.full(len(pos), 1.0, np.float32), 0.5, 9.0, 27.0)
+
+
+def _cone_inputs(st, idx):
+    """dot3(fwd, off / d) in the kernels' fp32 order for every KNN pair."""
+    pos = st.pos()
+    f32 = np.float32
+    out = np.empty(idx.shape, np.float32)
+    for j in range(idx.shape[0]):
+        off = (pos[idx[j]] - pos).astype(f32)
+        d2 = ((off[:, 0] * off[:, 0]) + (off[:, 1] * off[:, 1])) + (off[:, 2] * off[:, 2])
+        d = np.sqrt(d2).astype(f32)
+        q = (off / d[:, None]).astype(f32)
+        fw = np.stack([st.arrays["fx"], st.arrays["fy"], st.arrays["fz"]], axis=1)
+        out[j] = ((fw[:, 0] * q[:, 0]) + (fw[:, 1] * q[:, 1])) + (fw[:, 2] * q[:, 2])
+    return out
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_cone_thresholds_at_the_cone_inputs(gs, ctx, orc, mode):
+    """The kernels decide the cone test `dot(fwd, off / d) > cos_max` from an
+    estimate unless it is near the threshold.  Thresholds placed exactly on
+    attained cone inputs (the middle of a cluster about 1e-4 wide, so the
+    test flips inside it, with pairs both within and beyond the estimate's
+    margin of it) give the oracle's results bit for bit for each term alone
+    and for the fused flocking."""
+    st = _cone_lattice()
+    idx, _ = _knn(orc, st)
+    c = _cone_inputs(st, idx)
+    # Three clusters: the inputs of the first, fourth and last KNN slots'
+    # most common rounded values.
+    cos = []
+    for j in (0, 3, 6):
+        vals, counts = np.unique(np.round(c[j].astype(np.float64), 3), return_counts=True)
+        centre = vals[np.argmax(counts)]
+        cl = np.sort(c[j][np.abs(c[j] - centre) < 1e-3])
+        cos.append(float(cl[len(cl) // 2]))
+        below, at, above = (np.count_nonzero(cl < cl[len(cl) // 2]), np.count_nonzero(cl == cl[len(cl) // 2]),
+                            np.count_nonzero(cl > cl[len(cl) // 2]))
+        print(f"slot {j}: threshold {cos[-1]!r} cluster below {below} at {at} above {above}", flush=True)
+        assert at > 0 and above > 0
+    terms = [scenarios.FlockTerm(0.1, 5.0, cos[0], 1.5, 0), scenarios.FlockTerm(0.1, 5.0, cos[1], 0.75, 0),
+             scenarios.FlockTerm(0.1, 5.0, cos[2], 2.0, 0)]
+    for w, (name, t) in enumerate(zip(("separation", "alignment", "cohesion"), terms)):
+        ref = st.copy()
+        A = gs.AgentGroup(ctx, st)
+        knn = _upload_knn_via_brute(gs, ctx, A, A, 7)
+        fn = {"separation": gs.steerForSeparation, "alignment": gs.steerForAlignment,
+              "cohesion": gs.steerForCohesion}[name]
+        fn(A, knn, A, t.min_d, t.max_d, t.cos_max, t.weight, 0, mode)
+        orc.steer_flock(w, ref, idx, ref, t.min_d, t.max_d, t.cos_max, t.weight, 0, mode)
+        assert_state_equal(A.download(), ref, what=f"{name}: ")
+        A.close()
+        knn.close()
+    ref = st.copy()
+    A = gs.AgentGroup(ctx, st)
+    knn = _upload_knn_via_brute(gs, ctx, A, A, 7)
+    gs.steerForFlocking(A, knn, A, *terms, mode)
+    for w, t in enumerate(terms):
+        orc.steer_flock(w, ref, idx, ref, t.min_d, t.max_d, t.cos_max, t.weight, 0, mode)
+    assert_state_equal(A.download(), ref, what="fused: ")
+    A.close()
+    knn.close()
