@@ -328,7 +328,10 @@ __device__ __forceinline__ void integrate(f3 force, f3& pos, f3& fwd, float& spe
     f3 new_vel;
     if (MODE == GS_MODE_GPUREF) {
         force = truncate3(force, max_force);
-        const f3 acc = force / mass;
+        // x / 1 is x exactly (signed zeros included), so a unit mass skips
+        // the three divisions.
+        f3 acc = force;
+        if (mass != 1.f) acc = force / mass;
         new_vel = truncate3(fwd * speed + acc * dt, max_speed);
         speed = len3(new_vel);
         if (speed > 0.f) {
@@ -345,7 +348,8 @@ __device__ __forceinline__ void integrate(f3 force, f3& pos, f3& fwd, float& spe
             const float cosine = 1.f + (-1.f - 1.f) * pow20(range);
             adjusted = limit_max_deviation(force, cosine, fwd);
         }
-        const f3 new_acc = truncate3(adjusted, max_force) / mass;
+        f3 new_acc = truncate3(adjusted, max_force);
+        if (mass != 1.f) new_acc = new_acc / mass;
         if (dt > 0.f) {
             const float rate = clipf(clipf(9.f * dt, 0.15f, 0.4f), 0.f, 1.f);
             sacc = sacc + (new_acc - sacc) * rate;

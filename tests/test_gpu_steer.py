@@ -152,6 +152,8 @@ def test_update(gs, ctx, orc, mode):
     st.arrays["gx"][64:80] = 0
     st.arrays["gy"][64:80] = 0
     st.arrays["gz"][64:80] = 0
+    # Unit and non-unit masses interleaved (a unit mass skips the divisions).
+    st.arrays["mass"][1::2] = np.random.default_rng(8).uniform(0.5, 2.0, st.n // 2).astype(np.float32)
     ref = st.copy()
     A = gs.AgentGroup(ctx, st)
     gs.updateGroup(A, p.dt, mode)
