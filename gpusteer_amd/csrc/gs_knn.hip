@@ -250,7 +250,10 @@ static void run_binned_rf(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t 
 void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, float cap2,
                        const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map) {
     if (A->n == 0) return;
-    if (out->k <= 8) run_binned_rf<8>(A, out, B, search_radius, cap2, seed, row_map, val_map);
+    // k = 7 (the parameter files' default) gets lists of exactly 7 entries:
+    // one compare-exchange fewer per insertion than the 8-entry lists.
+    if (out->k == 7) run_binned_rf<7>(A, out, B, search_radius, cap2, seed, row_map, val_map);
+    else if (out->k <= 8) run_binned_rf<8>(A, out, B, search_radius, cap2, seed, row_map, val_map);
     else if (out->k <= 16) run_binned_rf<16>(A, out, B, search_radius, cap2, seed, row_map, val_map);
     else run_binned_rf<32>(A, out, B, search_radius, cap2, seed, row_map, val_map);
 }

@@ -169,7 +169,8 @@ static void run_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params
 
 void launch_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, bool lazy) {
     if (A->n == 0) return;
-    if (knn->k <= 8) run_flock_update<8>(A, knn, p, lazy);
+    if (knn->k == 7) run_flock_update<7>(A, knn, p, lazy);
+    else if (knn->k <= 8) run_flock_update<8>(A, knn, p, lazy);
     else if (knn->k <= 16) run_flock_update<16>(A, knn, p, lazy);
     else run_flock_update<32>(A, knn, p, lazy);
     A->mark_integrated();

@@ -38,6 +38,9 @@ CASES = [
     # name, n, k, cells, search radius (fast-path column radius)
     ("c0-boids-k7", 4096, 7, (16, 16, 16), 1),
     ("c0-boids-k1", 4096, 1, (16, 16, 16), 1),
+    # k = 7 has lists of exactly 7 entries; other k <= 8 use 8-entry lists
+    ("c0-boids-k8", 4096, 8, (16, 16, 16), 1),
+    ("c0-boids-k5-r2", 4096, 5, (16, 16, 16), 2),
     ("c0-boids-k16", 4096, 16, (16, 16, 16), 1),
     ("c0-boids-k32", 4096, 32, (16, 16, 16), 1),
     ("coarse-1cell", 500, 7, (1, 1, 1), 1),

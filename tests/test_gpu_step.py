@@ -49,6 +49,33 @@ def test_boids_fused_steps_bit_exact(gs, ctx, orc, mode):
         h.close()
 
 
+@pytest.mark.parametrize("k", [5, 8, 16])
+def test_boids_fused_steps_other_k(gs, ctx, orc, k):
+    """30 fused steps of the sphere flock with k other than 7 (8- and
+    16-entry lists; k = 7 has its own 7-entry kernels), GPU rule: KNN sets
+    and the whole state equal the oracle's after every 10th step."""
+    st, p = scenarios.boids_sphere()
+    p.mode = 1
+    p.k = k
+    for t in (p.separation, p.alignment, p.cohesion):
+        t.min_d = 1.5
+    ref = st.copy()
+    A = gs.AgentGroup(ctx, st)
+    bins = gs.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
+    knn = gs.KNNData(ctx, st.n, p.k)
+    print(f"fused steps k={k}", flush=True)
+    for step in range(30):
+        gs.stepBoids(A, knn, bins, p, 1)
+        r_idx, r_dist = orc.boids_step(ref, p)
+        if step % 10 == 9:
+            idx, dist = knn.download()
+            np.testing.assert_array_equal(idx, r_idx, err_msg=f"step {step}")
+            np.testing.assert_array_equal(dist, r_dist, err_msg=f"step {step}")
+            assert_state_equal(A.download(), ref, what=f"step {step}: ")
+    for h in (A, bins, knn):
+        h.close()
+
+
 def test_fused_equals_separate_calls(gs, ctx):
     st, p = scenarios.boids_sphere()
     A1, A2 = gs.AgentGroup(ctx, st), gs.AgentGroup(ctx, st)
