@@ -28,7 +28,7 @@ GS_MODE_OPENSTEER = 0
 GS_MODE_GPUREF = 1
 GS_KIND_AGENTS, GS_KIND_OBSTACLES, GS_KIND_WALLS = 0, 1, 2
 GS_WRAP_NONE, GS_WRAP_BOX, GS_WRAP_SPHERE = -1, 0, 1
-GS_KNN_DEFAULT, GS_KNN_NO_SEED = 0, 1
+GS_KNN_DEFAULT, GS_KNN_NO_SEED, GS_KNN_BINNED = 0, 1, 2
 GS_CLOSEST_CLAMPED, GS_CLOSEST_AS_REFERENCE = 0, 1
 
 # Behaviour bits (include/OpenSteer/CUDA/CUDAKernelGlobals.cuh:12-31).

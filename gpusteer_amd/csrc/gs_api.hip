@@ -742,6 +742,8 @@ int gs_knn_destroy(gs_knn knn) {
             knn->idxb[b].release();
             knn->distb[b].release();
         }
+        knn->part_key.release();
+        knn->part_slot.release();
         reg_del(knn);
         delete knn;
     });
@@ -815,22 +817,38 @@ static float cap2_of(float max_distance) {
     return std::nextafter(r2, 0.f);
 }
 
+// Cells of the box a capped search can reach (the whole lattice without a
+// cap).
+static uint64_t reach_cells(const GridDev& g, uint32_t cells, float max_distance) {
+    if (std::isinf(max_distance)) return cells;
+    const float h[3] = {g.hx, g.hy, g.hz};
+    const int d[3] = {g.dx, g.dy, g.dz};
+    uint64_t box = 1;
+    for (int a = 0; a < 3; ++a) {
+        const double r = std::ceil((double)max_distance / (double)h[a]);
+        box *= (uint64_t)std::min<double>((double)d[a], 2.0 * r + 1.0);
+    }
+    return box;
+}
+
 // Whether the exhaustive scan of B's snapshot is cheaper than the binned
 // search (the result is the same exact one either way).
 static bool prefer_exhaustive(const gs_group_s* B, const GridDev& g, uint32_t cells, float max_distance) {
     if (B->n <= kSmallTargetSet) return true;
     if (B->n > kExhaustiveMax || (uint64_t)B->n * kSparseCells > cells) return false;
-    uint64_t box = cells;
-    if (!std::isinf(max_distance)) {
-        const float h[3] = {g.hx, g.hy, g.hz};
-        const int d[3] = {g.dx, g.dy, g.dz};
-        box = 1;
-        for (int a = 0; a < 3; ++a) {
-            const double r = std::ceil((double)max_distance / (double)h[a]);
-            box *= (uint64_t)std::min<double>((double)d[a], 2.0 * r + 1.0);
-        }
-    }
-    return box * 4 >= B->n;
+    return reach_cells(g, cells, max_distance) * 4 >= B->n;
+}
+
+// A small query group searching another small group over a wide reach: the
+// split exhaustive scan fills the chip where a one-lane-per-query search
+// would run as a few dozen wavefronts, each walking the cells around a query
+// that the other group may be far from.  A group's search of itself (its
+// points are where its queries are) and a narrowly capped search stay in the
+// bins.
+static bool prefer_split(const gs_group_s* A, const gs_group_s* B, const GridDev& g, uint32_t cells,
+                         float max_distance) {
+    return A != B && A->n <= kSplitMaxA && B->n <= kSplitMaxB && (uint64_t)A->n * B->n <= kSplitMaxPairs &&
+           reach_cells(g, cells, max_distance) > kSplitMinCells && knn_split_ranges(A->n, B->n) >= 2u;
 }
 
 static void find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t search_radius, uint32_t flags,
@@ -847,7 +865,10 @@ static void find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t 
         fail(GS_ERR_NOT_BINNED, "groups were binned with another lattice");
     const float cap2 = cap2_of(max_distance);
     use_device(A->ctx);
-    if (prefer_exhaustive(B, bins->g, bins->cells, max_distance)) {
+    if (!(flags & GS_KNN_BINNED) && prefer_split(A, B, bins->g, bins->cells, max_distance)) {
+        const Seed sd = seed_for(out, A, B, flags);
+        launch_knn_split(A, out, B, cap2, sd.seed, sd.row_map, sd.val_map);
+    } else if (!(flags & GS_KNN_BINNED) && prefer_exhaustive(B, bins->g, bins->cells, max_distance)) {
         launch_knn_snap_brute(A, out, B, cap2);
     } else {
         const Seed sd = seed_for(out, A, B, flags);

@@ -271,6 +271,9 @@ struct gs_knn_s {
     const gs_group_s* a = nullptr;
     const gs_group_s* b = nullptr;
     uint64_t a_members = 0, b_members = 0, a_order = 0, b_order = 0;
+    // Partial lists of the split exhaustive search, [n][ranges][k].
+    gs::DevBuf<uint64_t> part_key;
+    gs::DevBuf<uint32_t> part_slot;
     uint32_t magic = 0x67736b6eu;
 
     uint32_t* idx() const { return idxb[cur].p; }
@@ -355,6 +358,25 @@ constexpr uint32_t kSmallTargetSet = 256;
 // empty cells.
 constexpr uint32_t kExhaustiveMax = 4096;
 constexpr uint32_t kSparseCells = 16;
+// A small query group's search of another small group (at most kSplitMaxA
+// queries, at most kSplitMaxPairs query-candidate pairs, B at most
+// kSplitMaxB points) whose reach covers more than kSplitMinCells cells is
+// done by the split exhaustive scan: B cut into ranges of at least
+// kSplitMinRange points, about kSplitWaves wavefronts in all, at most 64
+// ranges.
+constexpr uint32_t kSplitMaxA = 16384;
+constexpr uint32_t kSplitMaxB = 16384;
+constexpr uint64_t kSplitMaxPairs = 1ull << 26;
+constexpr uint32_t kSplitWaves = 4096;
+constexpr uint32_t kSplitMinRange = 64;
+constexpr uint64_t kSplitMinCells = 64;
+constexpr int kSplitBuf = 16;  // deferred entries per lane
+// Number of ranges for nA queries over nB points (< 2: no split).
+uint32_t knn_split_ranges(uint32_t nA, uint32_t nB);
+// The split exhaustive scan (falls back to the unsplit one below 2 ranges);
+// seed / row_map / val_map as for launch_knn_binned.
+void launch_knn_split(gs_group_s* A, gs_knn_s* out, gs_group_s* B, float cap2, const uint32_t* seed,
+                      const uint32_t* row_map, const uint32_t* val_map);
 // Rewrites out's contents from the previous storage order of A / B to the
 // current one (into the alternate buffer; caller flips).
 void launch_knn_remap(gs_knn_s* out, uint32_t nA, const uint32_t* row_old_to_new, const uint32_t* val_old_to_new);

@@ -87,6 +87,11 @@ __device__ __forceinline__ float sqdist_filter(float qx, float qy, float qz, con
     const float dx = qx - p.x, dy = qy - p.y, dz = qz - p.z;
     return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
 }
+// The filter's limit for an exact limit lim: widened by 2e-5 relative (far
+// more than the few ulps between the two d2 forms) and 2^-100 absolute (their
+// rounding in the subnormal range), so no candidate with d2 <= lim fails
+// sqdist_filter <= filter_limit(lim).  -inf and +inf stay as they are.
+__device__ __forceinline__ float filter_limit(float lim) { return __builtin_fmaf(lim, 1.0f + 2e-5f, 0x1p-100f); }
 
 // Distance from coordinate q to the slab of cells [i0, i1] along one axis;
 // boundary cells extend to infinity outward (points outside the lattice are
@@ -594,7 +599,7 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint
     // d2 is within the bound widened by `grow` (more than the few ulps by
     // which the fused form can differ from the exact one), and the merge
     // recomputes the exact key and bound.
-    float limf = lim * grow;
+    float limf = filter_limit(lim);
     for (;;) {
         while (__any(ta < ea) && !__any(wa > wfull)) {
             // Byte offsets of the two chunks; the per-candidate offsets are
@@ -631,7 +636,7 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint
         flat_flush<KMAX>(h, cnt, L, rb, qx, qy, qz, self_slot, thr);
         wa = L.slot_addr;
         lim = fminf(thr, h.worst());
-        limf = lim * grow;
+        limf = filter_limit(lim);
         if (!__any(ta < ea)) break;
     }
     bool all;

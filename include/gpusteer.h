@@ -88,7 +88,10 @@ enum { GS_WRAP_BOX = 0, GS_WRAP_SPHERE = 1 };
 enum {
     GS_KNN_DEFAULT = 0,
     /* Do not use the container's previous (exact) result as a search bound. */
-    GS_KNN_NO_SEED = 1u << 0
+    GS_KNN_NO_SEED = 1u << 0,
+    /* Search through the bins even where an exhaustive scan of B is chosen
+     * (small query groups, small or sparse B); the result is the same. */
+    GS_KNN_BINNED = 1u << 1
 };
 
 /* Behaviour bits, include/OpenSteer/CUDA/CUDAKernelGlobals.cuh:12-31. */
@@ -215,7 +218,11 @@ int gs_update_knn_database(gs_group g, gs_bins bins);
  * both have a current bin database built with `bins`.  search_radius is the
  * cell radius scanned before the first exactness test (the result does not
  * depend on it).  Unless GS_KNN_NO_SEED, a previous exact result held by `out`
- * for the same A, B and memberships bounds the search (result unchanged). */
+ * for the same A, B and memberships bounds the search (result unchanged).
+ * Unless GS_KNN_BINNED, a small A searching another small group B (at most
+ * 16384 agents each, at most 2^26 pairs) over more than 64 cells of reach is
+ * served by an exhaustive scan of B's snapshot split over many wavefronts,
+ * and a small or sparse B by an exhaustive scan: the same exact result. */
 int gs_find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t search_radius, uint32_t flags);
 /* gs_find_knn restricted to points closer than max_distance (strict, the
  * locality test of src/lq.c:327: d^2 < r^2 in fp32); rows with fewer than k

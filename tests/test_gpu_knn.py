@@ -15,7 +15,12 @@ def _group(gs, ctx, pos):
     return gs.AgentGroup(ctx, st)
 
 
-def _run_binned(gs, ctx, a_pos, b_pos, k, origin, size, cells, self_group, flags=0, repeat=1, sr=1):
+# The binned search unless a test asks for the default dispatch (which takes
+# an exhaustive scan for small query groups and small or sparse targets).
+BINNED = 2  # GS_KNN_BINNED
+
+
+def _run_binned(gs, ctx, a_pos, b_pos, k, origin, size, cells, self_group, flags=BINNED, repeat=1, sr=1):
     bins = gs.KNNBinData(ctx, cells, size, origin=origin)
     A = _group(gs, ctx, a_pos)
     B = A if self_group else _group(gs, ctx, b_pos)
@@ -211,13 +216,14 @@ def test_capped_knn_equals_oracle(gs, ctx, orc, cap, self_group):
 @pytest.mark.parametrize("nb", [1, 6, 50, 256])
 def test_small_target_set(gs, ctx, orc, nb):
     """Target sets of at most 256 points are scanned exhaustively (walls,
-    obstacles); the result is the same exact KNN."""
+    obstacles; 20,000 queries, above the split scan's query limit); the
+    result is the same exact KNN."""
     g = np.random.default_rng(nb)
-    a = g.uniform(-50, 50, size=(2000, 3)).astype(np.float32)
+    a = g.uniform(-50, 50, size=(20000, 3)).astype(np.float32)
     b = g.uniform(-50, 50, size=(nb, 3)).astype(np.float32)
     k = 3
     r_idx, r_dist = orc.knn_brute(a, b, k, False)
-    idx, dist = _run_binned(gs, ctx, a, b, k, (-55,) * 3, (110,) * 3, (40, 40, 40), False)
+    idx, dist = _run_binned(gs, ctx, a, b, k, (-55,) * 3, (110,) * 3, (40, 40, 40), False, flags=0)
     np.testing.assert_array_equal(idx, r_idx)
     np.testing.assert_array_equal(dist, r_dist)
 
@@ -243,4 +249,53 @@ def test_sparse_target_set(gs, ctx, orc, cap):
     np.testing.assert_array_equal(idx, r_idx)
     np.testing.assert_array_equal(dist, r_dist)
     for h in (knn, A, B, bins):
+        h.close()
+
+
+SPLIT_CASES = [
+    # name, nA, nB, k, cap (the 12^3 lattice's cells are 10.8 wide: every
+    # case reaches more than 64 cells, so the default dispatch splits)
+    ("5000x5000-k7", 5000, 5000, 7, float("inf")),
+    ("5000x3000-k7-cap12", 5000, 3000, 7, 12.0),
+    ("1000x129-k16", 1000, 129, 16, float("inf")),
+    ("4096x16384-k8", 4096, 16384, 8, float("inf")),
+    ("777x900-k32", 777, 900, 32, float("inf")),
+    ("300x200-k7-cap20-padded", 300, 200, 7, 20.0),
+    ("2048x2048-k1", 2048, 2048, 1, float("inf")),
+]
+
+
+@pytest.mark.parametrize("name,na,nb,k,cap", SPLIT_CASES)
+def test_split_exhaustive_equals_oracle(gs, ctx, orc, name, na, nb, k, cap):
+    """A small query group searching another small group over a wide reach
+    takes the split exhaustive scan (B cut into up to 64 ranges, partial
+    lists merged by a wavefront per query): KNN sets and distances equal the
+    oracle's brute force, including exact duplicates among B's positions and
+    queries sitting on B's points (ties broken by agent index), a cap that
+    leaves rows padded, ranges of 65 points and ragged query counts,
+    unseeded and seeded by the previous call; the binned search gives the
+    same result."""
+    print(f"case {name}", flush=True)
+    g = np.random.default_rng(na + nb + k)
+    a = g.uniform(-60, 60, size=(na, 3)).astype(np.float32)
+    b = g.uniform(-60, 60, size=(nb, 3)).astype(np.float32)
+    b[1::17] = b[::17][: len(b[1::17])]  # duplicated positions
+    m = min(na, nb) // 5
+    a[:m] = b[:m]  # queries on B's points
+    r_idx, r_dist = orc.knn_brute(a, b, k, False, cap=cap)
+    bins = gs.KNNBinData(ctx, (12, 12, 12), (130.0,) * 3, origin=(-65.0,) * 3)
+    A, B = _group(gs, ctx, a), _group(gs, ctx, b)
+    gs.updateKNNDatabase(A, bins)
+    gs.updateKNNDatabase(B, bins)
+    for flags in (0, BINNED):
+        knn = gs.KNNData(ctx, na, k)
+        for call in range(2):  # the second call is seeded by the first
+            gs.findKNearestNeighborsWithin(A, knn, bins, B, cap, flags=flags)
+            idx, dist = knn.download()
+            np.testing.assert_array_equal(idx, r_idx, err_msg=f"flags {flags} call {call}")
+            np.testing.assert_array_equal(dist, r_dist, err_msg=f"flags {flags} call {call}")
+        knn.close()
+    if "padded" in name:
+        assert np.any(r_idx == 0xFFFFFFFF)
+    for h in (A, B, bins):
         h.close()
