@@ -6,6 +6,9 @@ Importing the package loads lib/libgpusteer.so; there is no CPU fallback.
 from .build import LIB  # noqa: F401
 from . import _capi  # noqa: F401  (raises ImportError when the library is not built)
 from .api import (  # noqa: F401
+    KNN_BINNED,
+    KNN_DEFAULT,
+    KNN_NO_SEED,
     MODE_GPUREF,
     MODE_OPENSTEER,
     AgentGroup,

@@ -20,6 +20,11 @@ from .state import AGENT_FIELDS, WALL_FIELDS, AgentState, WallState
 
 MODE_OPENSTEER = capi.GS_MODE_OPENSTEER
 MODE_GPUREF = capi.GS_MODE_GPUREF
+# findKNearestNeighbors flags: no seeding from the container's previous
+# result; always search through the bins (never an exhaustive scan).
+KNN_DEFAULT = capi.GS_KNN_DEFAULT
+KNN_NO_SEED = capi.GS_KNN_NO_SEED
+KNN_BINNED = capi.GS_KNN_BINNED
 
 
 class Context:

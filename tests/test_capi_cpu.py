@@ -27,6 +27,23 @@ def test_library_exports_every_declared_symbol():
     assert set(names) <= set(_capi.EXPORTED)
 
 
+def test_enum_constants_match_header():
+    """Every GS_* enum constant the header defines has the same value in
+    the ctypes table."""
+    from gpusteer_amd import _capi
+
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / "gpusteer.h").read_text(), flags=re.S)
+    found = 0
+    for body in re.findall(r"enum\s*\{(.*?)\}", text, flags=re.S):
+        for name, expr in re.findall(r"(GS_\w+)\s*=\s*([^,]+)", body):
+            m = re.fullmatch(r"(\d+)u?(?:\s*<<\s*(\d+))?", expr.strip())
+            assert m, f"{name}: unexpected initializer {expr!r}"
+            want = int(m.group(1)) << int(m.group(2) or 0)
+            assert getattr(_capi, name) == want, name
+            found += 1
+    assert found >= 10
+
+
 def test_status_strings():
     from gpusteer_amd import _capi
 

@@ -17,7 +17,7 @@ def _group(gs, ctx, pos):
 
 # The binned search unless a test asks for the default dispatch (which takes
 # an exhaustive scan for small query groups and small or sparse targets).
-BINNED = 2  # GS_KNN_BINNED
+BINNED = 2  # GS_KNN_BINNED (gpusteer_amd.KNN_BINNED)
 
 
 def _run_binned(gs, ctx, a_pos, b_pos, k, origin, size, cells, self_group, flags=BINNED, repeat=1, sr=1):
