@@ -165,6 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_knn_split_merge(uint32_t nA, uint32_
 }
 
 uint32_t knn_split_ranges(uint32_t nA, uint32_t nB) {
+    if (nA == 0) return 0;
     const uint32_t waves = (nA + 63u) / 64u;
     uint32_t S = (kSplitWaves + waves - 1u) / waves;
     S = std::min<uint32_t>(S, std::min<uint32_t>(64u, nB / kSplitMinRange));
@@ -192,8 +193,8 @@ static void run_split(gs_group_s* A, gs_knn_s* out, gs_group_s* B, float cap2, u
 
 void launch_knn_split(gs_group_s* A, gs_knn_s* out, gs_group_s* B, float cap2, const uint32_t* seed,
                       const uint32_t* row_map, const uint32_t* val_map) {
-    const uint32_t S = knn_split_ranges(A->n, B->n);
     if (A->n == 0) return;
+    const uint32_t S = knn_split_ranges(A->n, B->n);
     if (S < 2u) {
         launch_knn_snap_brute(A, out, B, cap2);
         return;

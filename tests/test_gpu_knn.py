@@ -299,3 +299,37 @@ def test_split_exhaustive_equals_oracle(gs, ctx, orc, name, na, nb, k, cap):
         assert np.any(r_idx == 0xFFFFFFFF)
     for h in (A, B, bins):
         h.close()
+
+
+@pytest.mark.parametrize("flags", [0, BINNED], ids=["default", "binned"])
+def test_empty_groups(gs, ctx, orc, flags):
+    """An empty query group gets an empty result and an empty target group
+    all pads, through every search the dispatch can take (uncapped and over
+    a wide cap: exhaustive and split scans, bins); behaviours and the
+    integration on an empty group do nothing."""
+    g = np.random.default_rng(3)
+    b = g.uniform(-40, 40, size=(300, 3)).astype(np.float32)
+    empty = np.zeros((0, 3), np.float32)
+    bins = gs.KNNBinData(ctx, (8, 8, 8), (100.0,) * 3, origin=(-50.0,) * 3)
+    E, B = _group(gs, ctx, empty), _group(gs, ctx, b)
+    gs.updateKNNDatabase(E, bins)
+    gs.updateKNNDatabase(B, bins)
+    for cap in (float("inf"), 60.0):
+        print(f"empty A, cap {cap}, flags {flags}", flush=True)
+        knn = gs.KNNData(ctx, 0, 7)
+        gs.findKNearestNeighborsWithin(E, knn, bins, B, cap, flags=flags)
+        idx, dist = knn.download()
+        assert idx.shape == (7, 0) and dist.shape == (7, 0)
+        gs.steerForSeparation(E, knn, B, 0.0, 5.0, -0.7, 1.0)
+        knn.close()
+        print(f"empty B, cap {cap}, flags {flags}", flush=True)
+        knn = gs.KNNData(ctx, 300, 7)
+        gs.findKNearestNeighborsWithin(B, knn, bins, E, cap, flags=flags)
+        idx, dist = knn.download()
+        assert np.all(idx == 0xFFFFFFFF) and np.all(dist == np.finfo(np.float32).max)
+        knn.close()
+    gs.updateGroup(E, 0.1)
+    gs.wrapWorld(E, (100.0, 100.0, 100.0))
+    assert E.download().n == 0
+    for h in (E, B, bins):
+        h.close()
