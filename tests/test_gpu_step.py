@@ -308,3 +308,24 @@ def test_drift_vs_fp64(gs, ctx, orc):
     d = np.sqrt(sum((out.arrays[f].astype(np.float64) - ref.arrays[f]) ** 2 for f in ("px", "py", "pz")))
     print(f"drift after 200 steps: median {np.median(d):.3e} p99 {np.quantile(d, 0.99):.3e} max {d.max():.3e}")
     assert np.median(d) < 0.1
+
+
+def test_empty_group_steps(gs, ctx):
+    """The fused step, the per-cell extraction and the separate calls on a
+    group with no agents do nothing and report no error."""
+    st, p = scenarios.boids_sphere(n=16)
+    empty = scenarios.make_agents(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32),
+                                  np.zeros(0, np.float32), 0.5, 9.0, 27.0)
+    E = gs.AgentGroup(ctx, empty)
+    bins = gs.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
+    knn = gs.KNNData(ctx, 0, p.k)
+    print("empty fused step", flush=True)
+    gs.stepBoids(E, knn, bins, p, 3)
+    cells = int(np.prod(p.grid_cells))
+    assert not np.any(E.cell_density(cells))
+    assert not np.any(E.cell_velocity(cells))
+    print("empty separate calls", flush=True)
+    _separate_calls_step(gs, E, knn, bins, p)
+    assert E.download().n == 0
+    for h in (E, bins, knn):
+        h.close()
