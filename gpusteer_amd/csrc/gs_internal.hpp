@@ -271,7 +271,7 @@ struct gs_knn_s {
     const gs_group_s* a = nullptr;
     const gs_group_s* b = nullptr;
     uint64_t a_members = 0, b_members = 0, a_order = 0, b_order = 0;
-    // Partial lists of the split exhaustive search, [n][ranges][k].
+    // Partial lists of the split exhaustive search, [ranges][n][k].
     gs::DevBuf<uint64_t> part_key;
     gs::DevBuf<uint32_t> part_slot;
     uint32_t magic = 0x67736b6eu;

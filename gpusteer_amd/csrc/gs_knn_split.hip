@@ -8,7 +8,7 @@
 //
 //   k_knn_split_part: every lane scans one range for its query (the range's
 //   candidates are the same for the whole block, staged through LDS) and keeps the range's k best keys (d2 bits, agent
-//   index) in a register list; the S partial lists go to scratch, [nA][S][k].
+//   index) in a register list; the S partial lists go to scratch, [S][nA][k].
 //   A previous exact result held by the container bounds the scan as in the
 //   binned search, so a range far from the query's neighbours inserts
 //   nothing.
@@ -78,13 +78,54 @@ __global__ __launch_bounds__(kBlock) void k_knn_split_part(uint32_t nA, const fl
     // Tile entries past the range hold NaN coordinates, which fail every
     // filter, so groups of eight need no bounds test; the eight filters are
     // branch-free and independent.
+    //
+    // Each 64-entry chunk of a tile carries its bounding box; a wavefront
+    // skips a chunk when no lane's query is within its limit of the box.
+    // The box distance uses the same per-axis subtractions and summation
+    // order as a candidate's exact d2 and every step is monotone, so it is
+    // never above a member's d2: the skip drops only candidates with
+    // d2 > lim, which the exact test would reject.
+    __shared__ float box[kBlock / 64][6];
     const float nan = __builtin_nanf("");
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (uint32_t base = b0; base < b1; base += kBlock) {
         const uint32_t m = min((uint32_t)kBlock, b1 - base);
         __syncthreads();
-        tile[threadIdx.x] = threadIdx.x < m ? bpos[base + threadIdx.x] : make_float4(nan, nan, nan, 0.f);
+        const float4 mine = threadIdx.x < m ? bpos[base + threadIdx.x] : make_float4(nan, nan, nan, 0.f);
+        tile[threadIdx.x] = mine;
+        // fminf / fmaxf ignore the NaN of entries past the range; a chunk
+        // with no entry keeps the empty box (+inf, -inf), which no query
+        // reaches.
+        float lo[3] = {fminf(mine.x, INFINITY), fminf(mine.y, INFINITY), fminf(mine.z, INFINITY)};
+        float hi[3] = {fmaxf(mine.x, -INFINITY), fmaxf(mine.y, -INFINITY), fmaxf(mine.z, -INFINITY)};
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+                hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                box[wave][a] = lo[a];
+                box[wave][3 + a] = hi[a];
+            }
+        }
         __syncthreads();
         for (uint32_t j0 = 0; j0 < m; j0 += 8u) {
+            if ((j0 & 63u) == 0u) {
+                const uint32_t c = j0 >> 6;
+                const float gx = fmaxf(fmaxf(box[c][0] - q.x, q.x - box[c][3]), 0.f);
+                const float gy = fmaxf(fmaxf(box[c][1] - q.y, q.y - box[c][4]), 0.f);
+                const float gz = fmaxf(fmaxf(box[c][2] - q.z, q.z - box[c][5]), 0.f);
+                const float d2box = ((gx * gx) + (gy * gy)) + (gz * gz);
+                if (!__any(d2box <= lim)) {
+                    j0 += 56u;  // the rest of the chunk (the loop adds 8)
+                    continue;
+                }
+            }
             float4 p[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) p[u] = tile[j0 + (uint32_t)u];
@@ -102,9 +143,9 @@ __global__ __launch_bounds__(kBlock) void k_knn_split_part(uint32_t nA, const fl
         flush(base);
     }
     if (!active) return;
-    // [nA][range][k]: the merge's wavefront reads one query's lists as one
-    // contiguous block (whole cache lines, none shared with another query).
-    const size_t o0 = ((size_t)s * gridDim.y + blockIdx.y) * k;
+    // [range][nA][k]: a wavefront's lists are one contiguous block, and the
+    // merge's lane reads its range's list as k consecutive entries.
+    const size_t o0 = ((size_t)blockIdx.y * nA + s) * k;
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) {
         if (j >= KMAX - (int)k) {
@@ -131,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_knn_split_merge(uint32_t nA, uint32_
         key[j] = kKeyEmpty;
         t[j] = UINT32_MAX;
         if (lane < S && (uint32_t)j < k) {
-            const size_t o = ((size_t)s * S + lane) * k + (uint32_t)j;
+            const size_t o = ((size_t)lane * nA + s) * k + (uint32_t)j;
             key[j] = part_key[o];
             t[j] = part_slot[o];
         }
