@@ -1,7 +1,8 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+bench.py's cpu_baseline leg, to check and time the HIP path.  The product
+never imports it.
 
 The composite steps below issue the oracle's per-behaviour passes in the same
 order as the corresponding sequence of C-ABI calls, so a state advanced here

@@ -2,8 +2,8 @@
 
 The HIP kernels and the oracle perform the same sequence of correctly rounded
 fp32 operations (both built without FMA contraction), so the results are
-compared for exact equality; the north-star tolerance (1e-5 relative) is
-also asserted separately on the steering vectors as a second, weaker bar."""
+compared for exact equality; a 1e-5 relative tolerance on the steering
+vectors is also asserted separately as a second, weaker bar."""
 import numpy as np
 import pytest
 
