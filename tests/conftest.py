@@ -1,5 +1,5 @@
 """Shared fixtures.  GPU tests carry @pytest.mark.gpu; everything else runs on
-the CPU.  The oracle (oracle/pyoracle.py) is imported here as the checker."""
+the CPU.  The oracle (oracle/pyoracle.py) is imported here to check the HIP results."""
 import sys
 from pathlib import Path
 
