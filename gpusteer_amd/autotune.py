@@ -8,21 +8,40 @@ from __future__ import annotations
 
 import copy
 import time
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 from . import api
 
 Cells = Tuple[int, int, int]
+# The largest lattice gs_bins_create accepts.
+MAX_CELLS = 1 << 24
 BOIDS_CANDIDATES: Tuple[Tuple[Cells, int], ...] = (
     ((48, 48, 48), 1), ((64, 64, 64), 1), ((48, 48, 96), 1), ((64, 64, 128), 1), ((64, 64, 192), 1),
     ((80, 80, 160), 1), ((96, 96, 96), 2), ((128, 128, 128), 2))
 
 
+def scaled_candidates(n: int, base: Sequence[Tuple[Cells, int]] = BOIDS_CANDIDATES) -> List[Tuple[Cells, int]]:
+    """The candidate lattices (set for 2^20 agents) scaled to n agents in the
+    same volume: every axis by (n / 2^20)^(1/3), rounded to a multiple of 8,
+    so the agents per cell stay what the base set was chosen for; lattices
+    above the library's 2^24 cells are left out."""
+    f = (n / float(1 << 20)) ** (1.0 / 3.0)
+    out: List[Tuple[Cells, int]] = []
+    for cells, rf in base:
+        c = tuple(max(8, int(round(v * f / 8.0)) * 8) for v in cells)
+        if c[0] * c[1] * c[2] <= MAX_CELLS and (c, rf) not in out:
+            out.append((c, rf))
+    return out
+
+
 def tune_flock(ctx: api.Context, state, params, k: int = 7,
-               candidates: Sequence[Tuple[Cells, int]] = BOIDS_CANDIDATES, steps: int = 10, warmup: int = 3):
+               candidates: Optional[Sequence[Tuple[Cells, int]]] = None, steps: int = 10, warmup: int = 3):
     """Times `steps` fused Boids steps (after `warmup`) of the given flock for
-    each ((cells x, y, z), column radius) candidate; returns the fastest
-    candidate and the table of (cells, radius, ms per step)."""
+    each ((cells x, y, z), column radius) candidate (by default the base set
+    scaled to the flock's size); returns the fastest candidate and the table
+    of (cells, radius, ms per step)."""
+    if candidates is None:
+        candidates = scaled_candidates(state.n)
     table: List[Tuple[Cells, int, float]] = []
     for cells, rf in candidates:
         p = copy.deepcopy(params)

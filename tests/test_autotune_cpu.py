@@ -1,0 +1,25 @@
+"""The autotuner's candidate lattices (no GPU): the base set is the one for
+2^20 agents, and other sizes scale every axis by the cube root of the agent
+ratio, so the agents per cell stay the same."""
+from gpusteer_amd import autotune
+
+
+def test_base_set_at_one_million():
+    assert autotune.scaled_candidates(1 << 20) == list(autotune.BOIDS_CANDIDATES)
+
+
+def test_eight_times_the_agents_doubles_every_axis():
+    for (c, rf), (b, rb) in zip(autotune.scaled_candidates(1 << 23), autotune.BOIDS_CANDIDATES):
+        assert rf == rb and c == tuple(2 * v for v in b)
+
+
+def test_small_flocks_keep_whole_distinct_lattices():
+    cands = autotune.scaled_candidates(4096)
+    assert len(cands) == len(set(cands))
+    assert all(v >= 8 and v % 8 == 0 for c, _ in cands for v in c)
+
+
+def test_lattices_above_the_limit_are_left_out():
+    cands = autotune.scaled_candidates(1 << 24)
+    assert cands and all(c[0] * c[1] * c[2] <= autotune.MAX_CELLS for c, _ in cands)
+    assert len(cands) < len(autotune.BOIDS_CANDIDATES)
