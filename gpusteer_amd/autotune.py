@@ -14,7 +14,7 @@ from . import api
 
 Cells = Tuple[int, int, int]
 # The largest lattice gs_bins_create accepts.
-MAX_CELLS = 1 << 24
+MAX_CELLS = 1 << 28
 BOIDS_CANDIDATES: Tuple[Tuple[Cells, int], ...] = (
     ((48, 48, 48), 1), ((64, 64, 64), 1), ((48, 48, 96), 1), ((64, 64, 128), 1), ((64, 64, 192), 1),
     ((80, 80, 160), 1), ((96, 96, 96), 2), ((128, 128, 128), 2))
@@ -24,7 +24,7 @@ def scaled_candidates(n: int, base: Sequence[Tuple[Cells, int]] = BOIDS_CANDIDAT
     """The candidate lattices (set for 2^20 agents) scaled to n agents in the
     same volume: every axis by (n / 2^20)^(1/3), rounded to a multiple of 8,
     so the agents per cell stay what the base set was chosen for; lattices
-    above the library's 2^24 cells are left out."""
+    above the library's 2^28 cells are left out."""
     f = (n / float(1 << 20)) ** (1.0 / 3.0)
     out: List[Tuple[Cells, int]] = []
     for cells, rf in base:

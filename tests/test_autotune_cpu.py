@@ -20,6 +20,6 @@ def test_small_flocks_keep_whole_distinct_lattices():
 
 
 def test_lattices_above_the_limit_are_left_out():
-    cands = autotune.scaled_candidates(1 << 24)
+    cands = autotune.scaled_candidates(1 << 30)
     assert cands and all(c[0] * c[1] * c[2] <= autotune.MAX_CELLS for c, _ in cands)
     assert len(cands) < len(autotune.BOIDS_CANDIDATES)

@@ -333,3 +333,35 @@ def test_empty_groups(gs, ctx, orc, flags):
     assert E.download().n == 0
     for h in (E, B, bins):
         h.close()
+
+
+def test_lattice_above_2_pow_24_cells(gs, ctx, orc):
+    """A 512 x 256 x 256 lattice (2^25 cells of one unit) with 8,000 agents
+    packed into a 20-unit cube near its far x end, so their cell indices lie
+    above 2^24: the bin database (keys, starts, order) and the binned KNN
+    equal the oracle's."""
+    g = np.random.default_rng(25)
+    pos = np.stack([g.uniform(200, 220, 8000), g.uniform(-10, 10, 8000), g.uniform(-10, 10, 8000)],
+                   axis=1).astype(np.float32)
+    origin, size, cells = (-256.0, -128.0, -128.0), (512.0, 256.0, 256.0), (512, 256, 256)
+    og = orc.grid(origin, size, cells)
+    r_key, r_start, r_perm, r_sp = orc.bins_build(og, pos)
+    assert r_key.min() > (1 << 24)
+    bins = gs.KNNBinData(ctx, cells, size, origin=origin)
+    A = _group(gs, ctx, pos)
+    print("lattice 2^25: bins", flush=True)
+    gs.updateKNNDatabase(A, bins)
+    key, start, perm, sp = A.download_bins(bins.num_cells)
+    np.testing.assert_array_equal(key, r_key)
+    np.testing.assert_array_equal(start, r_start)
+    np.testing.assert_array_equal(perm, r_perm)
+    np.testing.assert_array_equal(sp, r_sp)
+    print("lattice 2^25: knn", flush=True)
+    r_idx, r_dist = orc.knn_brute(pos, pos, 7, True)
+    knn = gs.KNNData(ctx, pos.shape[0], 7)
+    gs.findKNearestNeighbors(A, knn, bins, A, 1, BINNED)
+    idx, dist = knn.download()
+    np.testing.assert_array_equal(idx, r_idx)
+    np.testing.assert_array_equal(dist, r_dist)
+    for h in (knn, A, bins):
+        h.close()
