@@ -942,9 +942,9 @@ double calib_forward(gs_calib_s* c, const gs_calib_params& th) {
         GS_LAUNCHED("c4_forward");
         cur ^= 1;
     }
+    // Terminal loss (its adjoint seed is rewritten by each backward).
     if (nb) c4_terminal<<<nb, kBlock, 0, st>>>(D, cstate(c, c->seg, cur), cadj(c->adj_own[0], e->N), c->loss.p);
     GS_LAUNCHED("c4_terminal");
-    zero_words(c->adj_acc[0].p, (size_t)e->N * kAdj, st);
     copy_state(c, c->x0, 1, c->seg, cur);
     GS_HIP(hipEventRecord(c->ev[1], st));
     double loss = 0.0;
@@ -963,6 +963,13 @@ void calib_backward(gs_calib_s* c, gs_calib_params& grad) {
     const size_t N = e->N;
     GS_HIP(hipEventRecord(c->ev[2], st));
     zero_words(c->grad.p, 2 * kTheta, st);
+    // Seed the adjoint of the final state (kept by the forward in x0[1]) here
+    // rather than relying on the forward's seed: the sweep below reuses both
+    // adjoint buffer pairs, so a second backward after one forward must start
+    // again from the terminal term.  Its loss goes to the scratch slot.
+    if (nb) c4_terminal<<<nb, kBlock, 0, st>>>(D, cstate(c, c->x0, 1), cadj(c->adj_own[0], N), c->loss.p + 1);
+    GS_LAUNCHED("c4_terminal");
+    zero_words(c->adj_acc[0].p, N * kAdj, st);
     int a = 0;  // adj[a] holds the adjoint of the state after the current step
     for (int sgi = (int)c->nseg - 1; sgi >= 0; --sgi) {
         const uint32_t t0 = (uint32_t)sgi * K;

@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -47,7 +48,12 @@ template <class T> static inline V3<T> mul(V3<T> a, T s) { return mk(a.x * s, a.
 template <class T> static inline V3<T> dvd(V3<T> a, T s) { return mk(a.x / s, a.y / s, a.z / s); }
 // VectorUtils.cuh:86-89: (x*x' + y*y') + z*z', left to right.
 template <class T> static inline T dot(V3<T> a, V3<T> b) { return ((a.x * b.x) + (a.y * b.y)) + (a.z * b.z); }
-template <class T> static inline T len(V3<T> a) { return std::sqrt(dot(a, a)); }
+// Square root and absolute value through overloadable names, so that the
+// calibration model also runs on the tangent type below (float and double
+// call std::sqrt / std::fabs unchanged).
+template <class T> static inline T osqrt(T x) { return std::sqrt(x); }
+template <class T> static inline T ofabs(T x) { return std::fabs(x); }
+template <class T> static inline T len(V3<T> a) { return osqrt(dot(a, a)); }
 // VectorUtils.cuh:115-120: normalize skips the divide when the length is 0.
 template <class T> static inline V3<T> normalize(V3<T> a) {
     const T l = len(a);
@@ -1068,6 +1074,90 @@ static void avoid_neighbors(int mode, Agents<T>& A, int nA, const uint32_t* knn,
 // Every branch goes through a decision tape: recorded on a base run, replayed
 // under perturbed parameters, so central differences measure the derivative
 // of the branch the base run took (KNN lists are taped the same way).
+// Forward-mode tangent of the calibration model (the FP64 analytic
+// gradient).  A value carries its primal P -- float: the fp32 model's own
+// arithmetic, so the primal trajectory is bit for bit the fp32 run's; double:
+// the FP64 model -- and its derivative with respect to each of the 14
+// parameters in double.  Every operation applies the textbook rule at its
+// primal operands (sum, product, quotient, square root, absolute value), so
+// the loss's tangents are the exact derivative of the branch the decision
+// tape fixes, evaluated along the primal trajectory.  This is the forward
+// (tangent-linear) form of the derivative the GPU's reverse-mode adjoint
+// computes, derived independently of it.
+constexpr int kC4Params = 14;
+template <class P> struct Tan {
+    P v;
+    double d[kC4Params];
+    Tan() : v(P(0)) { for (double& x : d) x = 0.0; }
+    Tan(double x) : v(P(x)) { for (double& y : d) y = 0.0; }
+    explicit operator double() const { return double(v); }
+};
+template <class P> static inline Tan<P> operator+(const Tan<P>& a, const Tan<P>& b) {
+    Tan<P> r(0.0);
+    r.v = a.v + b.v;
+    for (int j = 0; j < kC4Params; ++j) r.d[j] = a.d[j] + b.d[j];
+    return r;
+}
+template <class P> static inline Tan<P> operator-(const Tan<P>& a, const Tan<P>& b) {
+    Tan<P> r(0.0);
+    r.v = a.v - b.v;
+    for (int j = 0; j < kC4Params; ++j) r.d[j] = a.d[j] - b.d[j];
+    return r;
+}
+template <class P> static inline Tan<P> operator-(const Tan<P>& a) {
+    Tan<P> r(0.0);
+    r.v = -a.v;
+    for (int j = 0; j < kC4Params; ++j) r.d[j] = -a.d[j];
+    return r;
+}
+template <class P> static inline Tan<P> operator*(const Tan<P>& a, const Tan<P>& b) {
+    Tan<P> r(0.0);
+    r.v = a.v * b.v;
+    const double av = double(a.v), bv = double(b.v);
+    for (int j = 0; j < kC4Params; ++j) r.d[j] = a.d[j] * bv + av * b.d[j];
+    return r;
+}
+template <class P> static inline Tan<P> operator/(const Tan<P>& a, const Tan<P>& b) {
+    Tan<P> r(0.0);
+    r.v = a.v / b.v;
+    const double bv = double(b.v), q = double(a.v) / bv;
+    for (int j = 0; j < kC4Params; ++j) r.d[j] = (a.d[j] - q * b.d[j]) / bv;
+    return r;
+}
+template <class P> static inline Tan<P>& operator+=(Tan<P>& a, const Tan<P>& b) { return a = a + b; }
+template <class P> static inline bool operator<(const Tan<P>& a, const Tan<P>& b) { return a.v < b.v; }
+template <class P> static inline bool operator>(const Tan<P>& a, const Tan<P>& b) { return a.v > b.v; }
+template <class P> static inline bool operator<=(const Tan<P>& a, const Tan<P>& b) { return a.v <= b.v; }
+template <class P> static inline bool operator>=(const Tan<P>& a, const Tan<P>& b) { return a.v >= b.v; }
+template <class P> static inline bool operator==(const Tan<P>& a, const Tan<P>& b) { return a.v == b.v; }
+template <class P> static inline bool operator!=(const Tan<P>& a, const Tan<P>& b) { return a.v != b.v; }
+// d sqrt(x) = dx / (2 sqrt(x)); at x = 0 (a zero length, which the model
+// only reaches on branches that do not use it) the tangent is taken as 0.
+template <class P> static inline Tan<P> osqrt(const Tan<P>& a) {
+    Tan<P> r(0.0);
+    r.v = std::sqrt(a.v);
+    const double s = std::sqrt(double(a.v));
+    if (s > 0.0)
+        for (int j = 0; j < kC4Params; ++j) r.d[j] = a.d[j] / (2.0 * s);
+    return r;
+}
+template <class P> static inline Tan<P> ofabs(const Tan<P>& a) { return a.v < P(0) ? -a : a; }
+
+// A parameter's value as the model's scalar: T(v), and for the tangent type
+// the unit derivative d(theta_j)/d(theta_j) = 1.
+template <class T> struct C4Seed {
+    static T of(double v, int) { return T(v); }
+};
+template <class P> struct C4Seed<Tan<P>> {
+    static Tan<P> of(double v, int j) {
+        Tan<P> t(v);
+        t.d[j] = 1.0;
+        return t;
+    }
+};
+
+#pragma omp declare reduction(+ : Tan<float>, Tan<double> : omp_out = omp_out + omp_in)
+
 struct C4Params {
     double w_sep[2], w_ali[2], w_coh[2], w_obs[2], w_wall[2], radius[2], vmax[2];
 };
@@ -1110,15 +1200,15 @@ template <class T> static inline V3<T> normalize_d(V3<T> a, Dec& D) {
 template <class T> static inline V3<T> truncate_d(V3<T> v, T m, Dec& D) {
     const T l2 = dot(v, v);
     if (D(l2 <= m * m)) return v;
-    return mul(v, m / std::sqrt(l2));
+    return mul(v, m / osqrt(l2));
 }
 template <class T> static bool li_d(V3<T> sa, V3<T> ea, V3<T> sb, V3<T> eb, V3<T>& ip, Dec& D) {
     const T eps = T(0.001f);
     const T denom = ((eb.z - sb.z) * (ea.x - sa.x)) - ((eb.x - sb.x) * (ea.z - sa.z));
     const T numera = ((eb.x - sb.x) * (sa.z - sb.z)) - ((eb.z - sb.z) * (sa.x - sb.x));
     const T numerb = ((ea.x - sa.x) * (sa.z - sb.z)) - ((ea.z - sa.z) * (sa.x - sb.x));
-    if (D(std::fabs(denom) < eps)) {
-        if (D(std::fabs(numera) < eps && std::fabs(numerb) < eps)) {
+    if (D(ofabs(denom) < eps)) {
+        if (D(ofabs(numera) < eps && ofabs(numerb) < eps)) {
             ip = sa;
             return true;
         }
@@ -1133,7 +1223,7 @@ template <class T> static bool li_d(V3<T> sa, V3<T> ea, V3<T> sb, V3<T> eb, V3<T
     return false;
 }
 template <class T> static bool hood_d(V3<T> off, T d2, V3<T> fwd, T mn, T mx, T cs, Dec& D) {
-    const T d = std::sqrt(d2);
+    const T d = osqrt(d2);
     if (D(d < mn)) return true;
     if (D(d > mx)) return false;
     return D(dot(fwd, dvd(off, d)) > cs);
@@ -1161,7 +1251,7 @@ static T c4_step(const C4Scenario& sc, const C4Params& th, const C4Config& cf, c
         const uint32_t* oth = lists[4 * g + 1].data();
         const uint32_t* kw = lists[4 * g + 2].data();
         const uint32_t* ko = lists[4 * g + 3].data();
-        const T r = T(th.radius[g]), vmax = T(th.vmax[g]);
+        const T r = C4Seed<T>::of(th.radius[g], 10 + g), vmax = C4Seed<T>::of(th.vmax[g], 12 + g);
         T pen = T(0);
 #pragma omp parallel for schedule(static) reduction(+ : pen)
         for (int i = 0; i < n; ++i) {
@@ -1217,7 +1307,7 @@ static T c4_step(const C4Scenario& sc, const C4Params& th, const C4Config& cf, c
                     steer = mul(nearest_normal, len(sub(fe[feeler], nearest_ip)));
                     bits |= BIT_AVOID_WALLS;
                 }
-                G = add(mul(steer, T(th.w_wall[g])), G);
+                G = add(mul(steer, C4Seed<T>::of(th.w_wall[g], 8 + g)), G);
             }
             // obstacles
             {
@@ -1238,7 +1328,7 @@ static T c4_step(const C4Scenario& sc, const C4Params& th, const C4Config& cf, c
                     const T c = ((lc.x * lc.x + lc.y * lc.y) + lc.z * lc.z) - (orr + r) * (orr + r);
                     const T d = (b * b) - (T(4) * c);
                     if (D(!(d < T(0)))) {
-                        const T sq = std::sqrt(d);
+                        const T sq = osqrt(d);
                         const T pp = (-b + sq) / T(2), qq = (-b - sq) / T(2);
                         if (D(!((pp < T(0)) && (qq < T(0))))) {
                             x_hit = true;
@@ -1253,7 +1343,7 @@ static T c4_step(const C4Scenario& sc, const C4Params& th, const C4Config& cf, c
                 }
                 V3<T> fo = mk(T(0), T(0), T(0));
                 if (D(n_hit && (n_dist < min_dist))) fo = perpendicular_component(sub(p, n_pos), f);
-                apply_d(G, bits, fo, T(th.w_obs[g]), BIT_AVOID_OBSTACLES, D);
+                apply_d(G, bits, fo, C4Seed<T>::of(th.w_obs[g], 6 + g), BIT_AVOID_OBSTACLES, D);
             }
             // separation: own group, other group
             for (int w = 0; w < 2; ++w) {
@@ -1274,7 +1364,7 @@ static T c4_step(const C4Scenario& sc, const C4Params& th, const C4Config& cf, c
                         }
                     }
                     if (D(cnt > 0)) acc = normalize_d(dvd(acc, (T)cnt), D);
-                    apply_d(G, bits, acc, T(th.w_sep[g]), BIT_SEPARATION, D);
+                    apply_d(G, bits, acc, C4Seed<T>::of(th.w_sep[g], 0 + g), BIT_SEPARATION, D);
                 }
             }
             // alignment
@@ -1291,7 +1381,7 @@ static T c4_step(const C4Scenario& sc, const C4Params& th, const C4Config& cf, c
                     }
                 }
                 if (D(cnt > 0)) acc = normalize_d(sub(dvd(acc, (T)cnt), f), D);
-                apply_d(G, bits, acc, T(th.w_ali[g]), BIT_ALIGNMENT, D);
+                apply_d(G, bits, acc, C4Seed<T>::of(th.w_ali[g], 2 + g), BIT_ALIGNMENT, D);
             }
             // cohesion
             if (D(!(bits & cf.coh_mask))) {
@@ -1308,7 +1398,7 @@ static T c4_step(const C4Scenario& sc, const C4Params& th, const C4Config& cf, c
                     }
                 }
                 if (D(cnt > 0)) acc = normalize_d(sub(dvd(acc, (T)cnt), p), D);
-                apply_d(G, bits, acc, T(th.w_coh[g]), BIT_COHESION, D);
+                apply_d(G, bits, acc, C4Seed<T>::of(th.w_coh[g], 4 + g), BIT_COHESION, D);
             }
             // seek
             if (D(!(bits & cf.seek_mask))) {
@@ -1341,7 +1431,7 @@ static T c4_step(const C4Scenario& sc, const C4Params& th, const C4Config& cf, c
                 const uint32_t* lst = w == 0 ? own : oth;
                 const std::vector<T>* B = w == 0 ? Xg : Xo;
                 const int nb = w == 0 ? n : no;
-                const T R = r + T(th.radius[w == 0 ? g : 1 - g]);
+                const T R = r + C4Seed<T>::of(th.radius[w == 0 ? g : 1 - g], 10 + (w == 0 ? g : 1 - g));
                 for (int j = 0; j < k; ++j) {
                     const uint32_t b = lst[(size_t)j * n + i];
                     if (D(b >= (uint32_t)nb)) break;
@@ -1363,7 +1453,7 @@ static void c4_lists(const C4Scenario& sc, const C4Config& cf, const std::vector
     for (int g = 0; g < 2; ++g)
         for (int a = 0; a < 3; ++a) {
             pos[g][a].resize(sc.n[g]);
-            for (int i = 0; i < sc.n[g]; ++i) pos[g][a][i] = (float)X[13 * g + a][i];
+            for (int i = 0; i < sc.n[g]; ++i) pos[g][a][i] = (float)(double)X[13 * g + a][i];
         }
     const int nw = (int)sc.w[0].size(), nob = (int)sc.o[0].size();
     std::vector<float> wm[3], oc[3];
@@ -1393,7 +1483,8 @@ static void c4_lists(const C4Scenario& sc, const C4Config& cf, const std::vector
 
 // Whole trajectory: returns the loss.  mode 1 records tape + lists, 2 replays.
 template <class T>
-static double c4_run(C4Scenario& sc, const C4Params& th, const C4Config& cf, int mode, double* final_state) {
+static T c4_run_t(C4Scenario& sc, const C4Params& th, const C4Config& cf, int mode, double* final_state,
+                  double* pos_tangent_rms) {
     std::vector<T> X[26], Y[26];
     for (int g = 0; g < 2; ++g)
         for (int c = 0; c < 13; ++c) {
@@ -1410,6 +1501,19 @@ static double c4_run(C4Scenario& sc, const C4Params& th, const C4Config& cf, int
         if (mode == 1) c4_lists(sc, cf, X, lists);
         loss += c4_step(sc, th, cf, X, Y, lists, sc.tape[t], mode);
         for (int c = 0; c < 26; ++c) std::swap(X[c], Y[c]);
+        if constexpr (!std::is_floating_point<T>::value) {
+            // RMS over agents of the position's tangent |d p / d theta_j| after
+            // step t: how a parameter change's effect on the state grows.
+            if (pos_tangent_rms) {
+                double acc[kC4Params] = {};
+                for (int g = 0; g < 2; ++g)
+                    for (int a = 0; a < 3; ++a)
+                        for (int i = 0; i < sc.n[g]; ++i)
+                            for (int j = 0; j < kC4Params; ++j) acc[j] += X[13 * g + a][i].d[j] * X[13 * g + a][i].d[j];
+                for (int j = 0; j < kC4Params; ++j)
+                    pos_tangent_rms[(size_t)t * kC4Params + j] = std::sqrt(acc[j] / double(sc.n[0] + sc.n[1]));
+            }
+        }
     }
     T term = T(0);
     for (int g = 0; g < 2; ++g)
@@ -1425,7 +1529,11 @@ static double c4_run(C4Scenario& sc, const C4Params& th, const C4Config& cf, int
             for (int c = 0; c < 13; ++c)
                 for (int i = 0; i < sc.n[g]; ++i) final_state[at++] = (double)X[13 * g + c][i];
     }
-    return (double)loss;
+    return loss;
+}
+template <class T>
+static double c4_run(C4Scenario& sc, const C4Params& th, const C4Config& cf, int mode, double* final_state) {
+    return (double)c4_run_t<T>(sc, th, cf, mode, final_state, nullptr);
 }
 
 }  // namespace orc
@@ -1545,6 +1653,16 @@ OR_AGENT_API(float, f32)
 OR_AGENT_API(double, f64)
 
 
+static void c4_config_of(const double* cfgd, const int* cfgi, C4Config& cf) {
+    cf.sep_min = cfgd[0]; cf.sep_max = cfgd[1]; cf.sep_cos = cfgd[2];
+    cf.ali_min = cfgd[3]; cf.ali_max = cfgd[4]; cf.ali_cos = cfgd[5];
+    cf.coh_min = cfgd[6]; cf.coh_max = cfgd[7]; cf.coh_cos = cfgd[8];
+    cf.w_seek = cfgd[9]; cf.mt_walls = cfgd[10]; cf.mt_obs = cfgd[11]; cf.max_force = cfgd[12];
+    cf.mass = cfgd[13]; cf.dt = cfgd[14]; cf.pen_w = cfgd[15]; cf.other_cap = cfgd[16];
+    cf.sep_mask = (uint32_t)cfgi[0]; cf.ali_mask = (uint32_t)cfgi[1]; cf.coh_mask = (uint32_t)cfgi[2];
+    cf.seek_mask = (uint32_t)cfgi[3]; cf.k = cfgi[4]; cf.kw = cfgi[5]; cf.ko = cfgi[6]; cf.steps = cfgi[7];
+}
+
 // Calibration model (gradient checks).  or_c4_create copies the scenario:
 // state[2][13][n_g] (p f speed side up), goals[2][3], walls[9][nw], obstacles[4][no].
 void* or_c4_create(int n0, int n1, const double* state0, const double* state1, const double* goals, int nw,
@@ -1570,15 +1688,30 @@ double or_c4_run(void* h, const double* params, const double* cfgd, const int* c
     C4Params th;
     memcpy(&th, params, sizeof th);
     C4Config cf;
-    cf.sep_min = cfgd[0]; cf.sep_max = cfgd[1]; cf.sep_cos = cfgd[2];
-    cf.ali_min = cfgd[3]; cf.ali_max = cfgd[4]; cf.ali_cos = cfgd[5];
-    cf.coh_min = cfgd[6]; cf.coh_max = cfgd[7]; cf.coh_cos = cfgd[8];
-    cf.w_seek = cfgd[9]; cf.mt_walls = cfgd[10]; cf.mt_obs = cfgd[11]; cf.max_force = cfgd[12];
-    cf.mass = cfgd[13]; cf.dt = cfgd[14]; cf.pen_w = cfgd[15]; cf.other_cap = cfgd[16];
-    cf.sep_mask = (uint32_t)cfgi[0]; cf.ali_mask = (uint32_t)cfgi[1]; cf.coh_mask = (uint32_t)cfgi[2];
-    cf.seek_mask = (uint32_t)cfgi[3]; cf.k = cfgi[4]; cf.kw = cfgi[5]; cf.ko = cfgi[6]; cf.steps = cfgi[7];
+    c4_config_of(cfgd, cfgi, cf);
     if (precision == 32) return c4_run<float>(*(C4Scenario*)h, th, cf, mode, final_state);
     return c4_run<double>(*(C4Scenario*)h, th, cf, mode, final_state);
+}
+// The analytic gradient of the taped model (forward-mode tangents, replaying
+// the tape and KNN lists of the last recording run): grad[14] = d loss /
+// d params.  primal_precision 32: along the fp32 trajectory (the GPU's);
+// 64: along the FP64 trajectory (the finite differences' function).
+// pos_tangent_rms (optional, [steps][14]): RMS over agents of the position's
+// tangent after each step.
+double or_c4_tangent(void* h, const double* params, const double* cfgd, const int* cfgi, int primal_precision,
+                     double* grad, double* pos_tangent_rms) {
+    C4Params th;
+    memcpy(&th, params, sizeof th);
+    C4Config cf;
+    c4_config_of(cfgd, cfgi, cf);
+    if (primal_precision == 32) {
+        const Tan<float> l = c4_run_t<Tan<float>>(*(C4Scenario*)h, th, cf, 2, nullptr, pos_tangent_rms);
+        for (int j = 0; j < kC4Params; ++j) grad[j] = l.d[j];
+        return double(l.v);
+    }
+    const Tan<double> l = c4_run_t<Tan<double>>(*(C4Scenario*)h, th, cf, 2, nullptr, pos_tangent_rms);
+    for (int j = 0; j < kC4Params; ++j) grad[j] = l.d[j];
+    return l.v;
 }
 
 float or_pow20(float r) { return pow20(r); }

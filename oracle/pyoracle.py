@@ -515,6 +515,24 @@ class C4:
             g[j] = (self.run(theta + e, 2) - self.run(theta - e, 2)) / (2 * h)
         return base, g
 
+    def tangent_grad(self, theta, primal_precision: int = 64, want_rms: bool = False):
+        """The analytic gradient of the taped model by forward-mode tangents
+        (replays the tape and KNN lists of the last recording run): returns
+        (loss, grad[14]) and, with want_rms, the RMS over agents of the
+        position tangent after every step ([steps][14]).  primal_precision
+        32 evaluates it along the fp32 trajectory (the GPU's), 64 along the
+        FP64 one (the finite differences' function)."""
+        L = lib()
+        d = C.POINTER(C.c_double)
+        L.or_c4_tangent.restype = C.c_double
+        L.or_c4_tangent.argtypes = [C.c_void_p, d, d, C.POINTER(C.c_int), C.c_int, d, d]
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        g = np.zeros(14, np.float64)
+        rms = np.zeros((self.cfgi[7], 14), np.float64) if want_rms else None
+        loss = L.or_c4_tangent(self.h, _d(th), _d(self.cfgd), self.cfgi, primal_precision, _d(g),
+                               None if rms is None else _d(rms))
+        return (loss, g, rms) if want_rms else (loss, g)
+
     def close(self):
         if self.h:
             lib().or_c4_destroy(self.h)

@@ -113,3 +113,98 @@ def test_autograd_wrapper_matches_backward(ctx):
     assert np.isfinite(float(calibration_loss(cal, theta).detach()))
     cal.close()
     ens.close()
+
+
+# ---------------------------------------------------------------------------
+# The calibration workload itself: 4 of the 64 crowds of 16,384 agents (gap
+# 15, 50 obstacles), checkpoints every 10 steps.
+#
+# The crowd is chaotic at this density: the oracle's tangents (RMS position
+# derivative) grow about 10x every 4-5 steps from step 20 on (1e0 at step 20,
+# 1e4 at 40, 1e7 at 60, 1e13 at 100), so central differences of +-1e-3
+# resolve the derivative only over the first ~10 steps (measured: the FP64
+# differences equal the FP64 tangent within 7e-5 at 10 steps, 0.2 off at 20,
+# 2e3 off at 40), and the FP64 trajectory itself separates from the fp32 one
+# the GPU follows.  So the gradient is checked two ways: against FP64
+# central differences at h = 1e-3 over 10 steps, and over the full 100 steps
+# against the oracle's forward-mode tangent evaluated along the GPU's own
+# fp32 trajectory (the final fp32 state is bit-exact with the oracle's).
+
+def _c4_specs():
+    return scenarios.calib_scenarios(n_scenarios=4, n_agents=16384)
+
+
+def _rel_ok(got, want, rel):
+    scale = np.abs(want)
+    return np.abs(got - want) <= rel * scale + 1e-6 * np.max(scale)
+
+
+@pytest.mark.timeout(600)
+def test_calibration_workload_10_steps_vs_fp64_differences(ctx, orc):
+    specs = _c4_specs()
+    cfg = scenarios.CalibConfig(steps=10, checkpoint_every=10)
+    th = scenarios.calib_params()
+    ens = Ensemble(ctx, specs)
+    cal = Calibration(ens, cfg)
+    loss = cal.forward(th)
+    grad = cal.backward()
+    cal.close()
+    ens.close()
+    base, fd, t64 = 0.0, np.zeros(14), np.zeros(14)
+    for i, (_, sc) in enumerate(specs):
+        print(f"scenario {i}: oracle fp32 record, FP64 differences h = 1e-3, FP64 tangent", flush=True)
+        c = orc.C4(sc, cfg)
+        b, g = c.fd_grad(th, 1e-3, record_precision=32)
+        _, t = c.tangent_grad(th, 64)
+        c.close()
+        base += b / len(specs)
+        fd += g / len(specs)
+        t64 += t / len(specs)
+    print("grad gpu   ", np.array2string(grad, precision=6))
+    print("fd (h=1e-3)", np.array2string(fd, precision=6))
+    print("tangent f64", np.array2string(t64, precision=6))
+    assert abs(loss - base) <= 1e-5 * abs(base)
+    # The tangent code is the FP64 differences' derivative (measured 3e-5).
+    assert _rel_ok(t64, fd, 1e-4).all()
+    ok = _rel_ok(grad, fd, 1e-3)
+    assert ok.all(), f"components {np.nonzero(~ok)[0]}: gpu {grad[~ok]} fd {fd[~ok]}"
+
+
+@pytest.mark.timeout(900)
+def test_calibration_workload_100_steps_vs_tangent(ctx, orc):
+    specs = _c4_specs()
+    cfg = scenarios.CalibConfig(steps=100, checkpoint_every=10)
+    th = scenarios.calib_params()
+    ens = Ensemble(ctx, specs)
+    cal = Calibration(ens, cfg)
+    loss = cal.forward(th)
+    grad = cal.backward()
+    # A second backward of the same forward: the same gradient up to the
+    # order of the float atomics that gather neighbour adjoints (not bitwise
+    # reproducible; measured spread below 1e-5 relative).
+    grad2 = cal.backward()
+    finals = [[cal.final_state(i, g) for g in range(2)] for i in range(len(specs))]
+    cal.close()
+    ens.close()
+    base, t32 = 0.0, np.zeros(14)
+    for i, (_, sc) in enumerate(specs):
+        print(f"scenario {i}: oracle fp32 record (100 steps), tangent along the fp32 trajectory", flush=True)
+        c = orc.C4(sc, cfg)
+        _, fs32 = c.run(th, 1, want_state=True, precision=32)
+        n0, n1 = sc.groups[0].n, sc.groups[1].n
+        for g, (off, n) in enumerate(((0, n0), (13 * n0, n1))):
+            for j, f in enumerate(orc.C4_STATE):
+                np.testing.assert_array_equal(finals[i][g].arrays[f],
+                                              fs32[off + j * n: off + (j + 1) * n].astype(np.float32),
+                                              err_msg=f"scenario {i} group {g} field {f}")
+        b, t = c.tangent_grad(th, 32)
+        c.close()
+        base += b / len(specs)
+        t32 += t / len(specs)
+    print("grad gpu    ", np.array2string(grad, precision=6))
+    print("grad gpu (2)", np.array2string(grad2, precision=6))
+    print("tangent f32 ", np.array2string(t32, precision=6))
+    assert abs(loss - base) <= 1e-5 * abs(base)
+    ok = _rel_ok(grad, t32, 1e-3)
+    assert ok.all(), f"components {np.nonzero(~ok)[0]}: gpu {grad[~ok]} tangent {t32[~ok]}"
+    assert _rel_ok(grad2, grad, 1e-4).all()
