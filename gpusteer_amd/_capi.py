@@ -23,6 +23,10 @@ GS_ERR_OUT_OF_MEMORY = 6
 GS_ERR_DEVICE = 7
 GS_ERR_NONFINITE = 8
 GS_MAX_K = 32
+# Limits of gs_find_knn_noalloc (its fixed device array).
+GS_NOALLOC_MAX_POINTS = 65536
+GS_NOALLOC_MAX_CELLS = 131072
+GS_NOALLOC_MAX_K = 16
 
 GS_MODE_OPENSTEER = 0
 GS_MODE_GPUREF = 1
@@ -261,6 +265,9 @@ _SIGS = {
     "gs_find_knn": (C.c_int, [_h, _h, _h, _h, C.c_uint32, C.c_uint32]),
     "gs_find_knn_within": (C.c_int, [_h, _h, _h, _h, C.c_uint32, C.c_uint32, C.c_float]),
     "gs_find_knn_brute": (C.c_int, [_h, _h, _h]),
+    # device arrays: px, py, pz (n floats), idx, dist (k * n)
+    "gs_find_knn_noalloc": (C.c_int, [_h, _h, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                      C.c_void_p, C.c_void_p]),
     "gs_steer_separation": (C.c_int, [_h, _h, _h, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_int]),
     "gs_steer_alignment": (C.c_int, [_h, _h, _h, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_int]),
     "gs_steer_cohesion": (C.c_int, [_h, _h, _h, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_int]),

@@ -257,6 +257,8 @@ int gs_context_create(int device, void* hip_stream, gs_context* out) {
         }
         if (device < 0 || device >= count) fail(GS_ERR_INVALID_ARGUMENT, "device index out of range");
         GS_HIP(hipSetDevice(device));
+        // Per-device set-up of the allocation-free entry point's array.
+        noalloc_init_device(device);
         auto* c = new gs_context_s;
         c->device = device;
         if (hip_stream) {
@@ -885,6 +887,17 @@ int gs_find_knn_within(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_
 
 int gs_find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t search_radius, uint32_t flags) {
     return guarded([&] { find_knn(A, out, bins, B, search_radius, flags, INFINITY); });
+}
+
+int gs_find_knn_noalloc(gs_context ctx, gs_bins bins, uint32_t n, const float* px, const float* py, const float* pz,
+                        uint32_t k, uint32_t* idx, float* dist) {
+    return guarded([&] {
+        check(ctx, "context");
+        check(bins, "bins");
+        if (bins->ctx != ctx) fail(GS_ERR_INVALID_ARGUMENT, "bins belong to another context");
+        use_device(ctx);
+        noalloc_find_knn(ctx, bins, n, px, py, pz, k, idx, dist);
+    });
 }
 
 int gs_find_knn_brute(gs_group A, gs_knn out, gs_group B) {

@@ -411,4 +411,10 @@ void launch_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p,
 bool flock_step_lazy_ok(const gs_group_s* A);
 // Inverts the id map into slot_of_id.
 void launch_invert_ids(gs_group_s* g);
+// Allocation-free KNN (gs_noalloc.hip): locates the fixed scratch array on
+// `device` (current), and runs the search of n device points into it.
+void noalloc_init_device(int device);
+void noalloc_find_knn(gs_context ctx, const gs_bins_s* bins, uint32_t n, const float* px, const float* py,
+                      const float* pz, uint32_t k, uint32_t* idx, float* dist);
+size_t noalloc_scratch_bytes();
 }  // namespace gs

@@ -457,6 +457,30 @@ int gs_calib_download_state(gs_calib c, uint32_t scenario, uint32_t group, gs_ag
 /* HIP-event milliseconds of the last forward and backward passes. */
 int gs_calib_timing(gs_calib c, float* forward_ms, float* backward_ms);
 
+/* ---- allocation-free KNN ------------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (work under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The result of gs_update_knn_database on a group holding
+ * these points followed by gs_find_knn of that group with itself over `bins`:
+ * for each point the k nearest other points by (d^2, index), idx[j*n + i] the
+ * point index of the j-th nearest of point i, dist = sqrtf(d^2), pads
+ * UINT32_MAX / FLT_MAX.  px, py, pz (n floats each) and idx, dist (k*n each)
+ * are device arrays.  The bin database and the intermediate lists live in
+ * one fixed device array of the library (about 13 MB on each device the
+ * library runs on, held until the process ends, located once per device by
+ * gs_context_create); the call allocates and frees nothing.  Limits:
+ * n <= GS_NOALLOC_MAX_POINTS, lattice cells <= GS_NOALLOC_MAX_CELLS,
+ * 1 <= k <= GS_NOALLOC_MAX_K (GS_ERR_UNSUPPORTED beyond).  Every call on a
+ * device shares that device's array, which keeps the last call's contents:
+ * calls that overlap on one device (other threads, other streams, other
+ * contexts) race and must be serialized by the caller.  Queued on the
+ * context's stream; returns before the work completes. */
+#define GS_NOALLOC_MAX_POINTS 65536u
+#define GS_NOALLOC_MAX_CELLS 131072u
+#define GS_NOALLOC_MAX_K 16u
+int gs_find_knn_noalloc(gs_context ctx, gs_bins bins, uint32_t n, const float* px, const float* py, const float* pz,
+                        uint32_t k, uint32_t* idx, float* dist);
+
 /* ---- Scenario formats (host only, no device work) ----------------------
  * Wall maps, wall splitting at bin-cell borders, simulation parameter files:
  * what the plugins load before the per-step path runs. */
