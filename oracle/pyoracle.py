@@ -358,8 +358,11 @@ def choke_step(groups, sc):
         st.bits[:] = 0
         ks = knn_binned(g, pos[gi], pos[gi], sc.k, True)[0]
         ko = knn_binned(g, pos[gi], pos[1 - gi], sc.k, False, cap=sc.other_radius)[0]
-        kw = knn_binned(g, pos[gi], wmid, sc.knw, False)[0]
-        kob = knn_binned(g, pos[gi], opos, sc.kno, False)[0]
+        # Walls and obstacles are a few dozen points: brute force gives the
+        # same exact lists (binned == brute is pinned) without expanding
+        # rings across the lattice's empty cells.
+        kw = knn_brute(pos[gi], wmid, sc.knw, False)[0]
+        kob = knn_brute(pos[gi], opos, sc.kno, False)[0]
         avoid_walls(st, kw, sc.walls, sc.min_time_walls, sc.w_walls, 0)
         avoid_obstacles(st, kob, sc.obstacles, sc.min_time_obstacles, sc.w_obstacles, 0)
         steer_flock(FLOCK_SEPARATION, st, ks, st, sc.sep.min_d, sc.sep.max_d, sc.sep.cos_max, sc.sep.weight,

@@ -65,3 +65,52 @@ def test_ensemble_reupload_restarts(ctx):
     _compare(ens, seq, specs, "after re-upload")
     ens.close()
     seq.close()
+
+
+def test_ensemble_workload_subset_equals_oracle(ctx, orc):
+    """The batched-ensemble workload itself: all 256 scenarios of the mix
+    (agent counts log-uniform over 1,024-65,536, about 3.9 M agents; even
+    scenarios Boids flocks with jittered weights, odd ones choke-point crowds
+    with gaps 7.5 ... 32.5), 21 steps (one eager step, then ten replays of
+    the captured two-step graph), lattices at the bench's cell factor.  A
+    stratified subset is compared with the oracle directly, bit for bit
+    (KNN lists and state): the smallest and largest flock, the smallest and
+    largest crowd, and the first crowd of every gap width."""
+    specs = scenarios.ensemble_mix(cell_factor=1.6)
+    boids = [i for i, s in enumerate(specs) if s[0] == "boids"]
+    crowds = [i for i, s in enumerate(specs) if s[0] == "choke"]
+    size = {i: (s[1].n if s[0] == "boids" else 2 * s[1].groups[0].n) for i, s in enumerate(specs)}
+    pick = {min(boids, key=size.get), max(boids, key=size.get), min(crowds, key=size.get),
+            max(crowds, key=size.get)}
+    gaps = {}
+    for i in crowds:
+        w = float(specs[i][1].walls.arrays["ex"][2])  # the left piece's end cap sits at x = -gap / 2
+        gaps.setdefault(w, i)
+    assert len(gaps) == 13
+    pick |= set(gaps.values())
+    print(f"ensemble workload: {len(specs)} scenarios, {sum(size.values())} agents; oracle subset "
+          f"{sorted((i, specs[i][0], size[i]) for i in pick)}", flush=True)
+    refs = {}
+    for i in pick:
+        s = specs[i]
+        refs[i] = s[1].copy() if s[0] == "boids" else [g.copy() for g in s[1].groups]
+    ens = Ensemble(ctx, specs)
+    steps = 21
+    ens.step(steps)
+    for i in sorted(pick):
+        s = specs[i]
+        print(f"oracle: scenario {i} ({s[0]}, {size[i]} agents), {steps} steps", flush=True)
+        if s[0] == "boids":
+            for _ in range(steps):
+                r_idx, _ = orc.boids_step(refs[i], s[2])
+            tag = f"scenario {i} (boids, {size[i]}): "
+            np.testing.assert_array_equal(ens.download_knn(i, 0), r_idx, err_msg=tag)
+            assert_state_equal(ens.download(i, 0), refs[i], what=tag)
+        else:
+            for _ in range(steps):
+                r = orc.choke_step(refs[i], s[1])
+            for g in range(2):
+                tag = f"scenario {i} (crowd, {size[i]}) group {g}: "
+                np.testing.assert_array_equal(ens.download_knn(i, g), r[g][0], err_msg=tag)
+                assert_state_equal(ens.download(i, g), refs[i][g], what=tag)
+    ens.close()

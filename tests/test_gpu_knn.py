@@ -169,15 +169,18 @@ def test_bin_database_matches_oracle(gs, ctx, orc):
         bins.close()
 
 
-def test_large_flock_subset_against_brute(gs, ctx, orc):
-    """2^20 agents in a 1000^3 box, 64^3 lattice: binned KNN for every agent,
-    checked against the oracle's brute force on 1,024 random queries (each
-    against all 2^20 agents)."""
-    st, p = scenarios.boids_box(n=1 << 20)
+@pytest.mark.parametrize("k,cells", [(7, (64, 64, 64)), (7, (64, 64, 192)), (16, (48, 48, 96))])
+def test_large_flock_subset_against_brute(gs, ctx, orc, k, cells):
+    """2^20 agents in a 1000^3 box at k = 7 (64^3, and 64 x 64 x 192, the
+    autotuner's lattice) and k = 16 (48 x 48 x 96, its lattice for k = 16):
+    binned KNN for every agent, checked against the oracle's brute force on
+    16,384 random queries (each against all 2^20 agents)."""
+    print(f"flock 2^20 k={k} cells={cells}", flush=True)
+    st, p = scenarios.boids_box(n=1 << 20, k=k)
     pos = st.pos()
-    idx, dist = _run_binned(gs, ctx, pos, pos, 7, p.grid_origin, p.grid_size, p.grid_cells, True, repeat=2)
-    q = np.random.default_rng(11).choice(pos.shape[0], size=1024, replace=False).astype(np.uint32)
-    r_idx, r_dist = orc.knn_brute(pos, pos, 7, True, queries=q)
+    idx, dist = _run_binned(gs, ctx, pos, pos, k, p.grid_origin, p.grid_size, cells, True, repeat=2)
+    q = np.random.default_rng(11).choice(pos.shape[0], size=16384, replace=False).astype(np.uint32)
+    r_idx, r_dist = orc.knn_brute(pos, pos, k, True, queries=q)
     np.testing.assert_array_equal(idx[:, q], r_idx[:, q])
     np.testing.assert_array_equal(dist[:, q], r_dist[:, q])
     # Size-independent properties for all agents: no self, ascending keys.

@@ -159,10 +159,13 @@ def test_steering_pending_across_bin_build(gs, ctx, orc, mode):
         h.close()
 
 
-@pytest.mark.parametrize("cells,sr", [((64, 64, 64), 1), ((64, 64, 128), 1), ((128, 128, 128), 2)])
-def test_boids_box_large_steps(gs, ctx, orc, cells, sr):
-    """2^20-agent box flock: 3 steps vs 3 oracle steps, bit-exact."""
-    st, p = scenarios.boids_box(n=1 << 20)
+@pytest.mark.parametrize("cells,sr,k", [((64, 64, 64), 1, 7), ((64, 64, 128), 1, 7), ((128, 128, 128), 2, 7),
+                                        ((64, 64, 192), 1, 7), ((48, 48, 96), 1, 16)])
+def test_boids_box_large_steps(gs, ctx, orc, cells, sr, k):
+    """2^20-agent box flock: 3 fused steps vs 3 oracle steps, bit-exact, at
+    k = 7 and k = 16 on the lattices the autotuner picks for them."""
+    print(f"flock 2^20 k={k} cells={cells} sr={sr}", flush=True)
+    st, p = scenarios.boids_box(n=1 << 20, k=k)
     p.grid_cells = cells
     p.search_radius = sr
     ref = st.copy()
@@ -185,7 +188,7 @@ def _choke_gpu_step(gs, ctx, G, knns, W, O, bins, sc):
 
 
 def test_choke_pipeline_bit_exact(gs, ctx, orc):
-    """The 10,000-agent choke point for 150 steps through the operator API
+    """The 10,000-agent choke point for its 1,000 steps through the operator API
     (walls, obstacles, separation within and across groups, seek,
     integrate), compared with the oracle's same sequence."""
     sc = scenarios.choke()
@@ -197,10 +200,11 @@ def test_choke_pipeline_bit_exact(gs, ctx, orc):
     gs.updateKNNDatabase(W, bins)
     gs.updateKNNDatabase(O, bins)
     knns = [[gs.KNNData(ctx, g.n, k) for k in (sc.k, sc.k, sc.knw, sc.kno)] for g in sc.groups]
-    for step in range(150):
+    for step in range(1000):
         _choke_gpu_step(gs, ctx, G, knns, W, O, bins, sc)
         r = orc.choke_step(ref, sc)
-        if step % 25 == 0 or step == 149:
+        if step % 100 == 0 or step == 999:
+            print(f"choke point step {step}: compare", flush=True)
             for gi in range(2):
                 for j in range(4):
                     np.testing.assert_array_equal(knns[gi][j].download()[0], r[gi][j], err_msg=f"step {step}")
@@ -293,9 +297,13 @@ def test_drift_vs_fp64(gs, ctx, orc):
     """200 fp32 steps on the GPU vs an FP64 re-run of the oracle from the same
     initial state (KNN computed from the fp64 positions rounded to fp32).
     The dynamics are chaotic, so fp32 and fp64 trajectories separate
-    exponentially; the stated bound is on the median agent: its position
-    deviation after 200 steps (3.3 s of simulated time, world radius 50)
-    stays below 0.1 world units (measured on MI355X: 0.041)."""
+    exponentially.  Bounds on the position deviation after 200 steps (3.3 s
+    of simulated time, world radius 50), measured on MI355X: median 0.041,
+    p99 2.53, max 99.9.  The median agent stays below 0.1 world units and
+    the 99th percentile below 5; the largest deviations are agents that
+    crossed the sphere and took the antipode wrap in one run but not in the
+    other, a jump of at most the diameter, so no agent is off by more than
+    2 R + 1."""
     st, p = scenarios.boids_sphere()
     ref = st.copy(np.float64)
     A = gs.AgentGroup(ctx, st)
@@ -308,6 +316,8 @@ def test_drift_vs_fp64(gs, ctx, orc):
     d = np.sqrt(sum((out.arrays[f].astype(np.float64) - ref.arrays[f]) ** 2 for f in ("px", "py", "pz")))
     print(f"drift after 200 steps: median {np.median(d):.3e} p99 {np.quantile(d, 0.99):.3e} max {d.max():.3e}")
     assert np.median(d) < 0.1
+    assert np.quantile(d, 0.99) < 5.0
+    assert d.max() <= 2 * 50.0 + 1.0
 
 
 def test_empty_group_steps(gs, ctx):
