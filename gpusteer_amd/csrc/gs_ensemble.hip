@@ -153,7 +153,8 @@ __global__ __launch_bounds__(kBlock) void e_knn(EnsDev E, uint32_t N, uint32_t k
         const float thr = (out.seed_self && active) ? seed_bound<8>(out.seed_self, k, N, new_to_old[s], N,
                                                                     old_to_new, q.x, q.y, q.z, spos)
                                                     : INFINITY;
-        knn_search_flat<8>(h, active, k, thr, q.x, q.y, q.z, s, P.g, N, spos, start + P.c0, flds);
+        FlatQuery Q{q.x, q.y, q.z, thr, s, active};
+        knn_search_flat<8>(h, Q, P.g, N, spos, start + P.c0, flds);
         if (active) store_idx<8>(h, k, N, s, out.self, out.out_map);
     }
     if (P.kind != GS_SCENARIO_CHOKE) return;
@@ -166,7 +167,8 @@ __global__ __launch_bounds__(kBlock) void e_knn(EnsDev E, uint32_t N, uint32_t k
                                     ? seed_bound<8>(out.seed_other, k, N, new_to_old[s], N, old_to_new, q.x, q.y,
                                                     q.z, spos)
                                     : INFINITY);
-        knn_search_flat<8>(h, active, k, thr, q.x, q.y, q.z, UINT32_MAX, O.g, N, spos, start + O.c0, flds);
+        FlatQuery Q{q.x, q.y, q.z, thr, UINT32_MAX, active};
+        knn_search_flat<8>(h, Q, O.g, N, spos, start + O.c0, flds);
         if (active) store_idx<8>(h, k, N, s, out.other, out.out_map);
     } else if (active) {
         TopK<8> h;

@@ -47,13 +47,13 @@ __global__ __launch_bounds__(kBlock, RF == 1 ? (KMAX <= 8 ? 5 : (KMAX <= 16 ? 4 
         // inexact queries together (lanes past nA only help).
         const bool active = s < nA;
         const float4 q = active ? apos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+        FlatQuery Q{q.x, q.y, q.z, INFINITY, self ? s : UINT32_MAX, active};
         if (nB > 0) {
-            const float thr =
-                fminf(cap2, (seed && active) ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB, val_map,
-                                                                q.x, q.y, q.z, bpos)
-                                             : INFINITY);
+            Q.thr = fminf(cap2, (seed && active) ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB,
+                                                                    val_map, q.x, q.y, q.z, bpos)
+                                                 : INFINITY);
             GS_FLAT_LDS
-            knn_search_flat<KMAX>(h, active, k, thr, q.x, q.y, q.z, self ? s : UINT32_MAX, g, nB, bpos, bstart, flds);
+            knn_search_flat<KMAX>(h, Q, g, nB, bpos, bstart, flds);
         }
         if (active) h.store(k, nA, s, out_idx, out_dist);
     } else {

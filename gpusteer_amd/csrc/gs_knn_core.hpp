@@ -350,7 +350,7 @@ __device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, f
 // holds more than kFlatBuf - 8 entries, so an iteration can never overflow
 // the buffer.
 constexpr int kFlatBuf = 16;
-constexpr int kFlatRuns = 9;
+constexpr int kFlatRuns = 10;  // nine columns, one of them split at the query
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 // One pool of kFlatPool words per lane, strided by kBlock: the deferral
 // buffer, the run list (first slots, then 16-bit lengths), and after the scan
@@ -414,6 +414,54 @@ __device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLd
         }
     }
     cnt = 0;
+}
+
+// The scan's only merge when the list is still empty and no lane of the
+// wavefront holds more than eight entries -- the common case of a seeded
+// bound with seven-entry lists: the seven previous neighbours, now and then
+// one more.  The batch's exact keys are sorted at once by the 19-comparator
+// network for eight inputs (Knuth, TAOCP vol. 3, sec. 5.3.4) and the seven
+// smallest become the list: the same list as inserting them one by one
+// (keys are distinct), for 19 compare-exchanges instead of up to 8 x 6.
+// Missing entries, the query's own slot and keys beyond the bound sort last
+// as empty entries.
+__device__ __forceinline__ void sort_cx(uint64_t& ka, uint32_t& ta, uint64_t& kb, uint32_t& tb) {
+    const bool sw = kb < ka;
+    const uint64_t lo = sw ? kb : ka, hi = sw ? ka : kb;
+    const uint32_t tl = sw ? tb : ta, th = sw ? ta : tb;
+    ka = lo;
+    kb = hi;
+    ta = tl;
+    tb = th;
+}
+__device__ __forceinline__ void flat_sort_fresh(TopK<7>& h, int cnt, const FlatLds& L, __amdgpu_buffer_rsrc_t rb,
+                                                float qx, float qy, float qz, uint32_t self_slot, float thr) {
+    uint64_t key[8];
+    uint32_t sl[8];
+    float4 p[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        // Entries past cnt hold stale slots: their loads are range-checked
+        // (zeros past the end of B) and their keys are discarded below.
+        sl[u] = L.slot[u * kBlock];
+        p[u] = ld_pos(rb, sl[u] * 16u);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const float d2 = sqdist4(qx, qy, qz, p[u]);
+        const bool use = u < cnt && sl[u] != self_slot && d2 <= thr;
+        key[u] = use ? make_key(d2, id_of(p[u])) : kKeyEmpty;
+        sl[u] = use ? sl[u] : UINT32_MAX;
+    }
+    constexpr int kNet[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 1}, {2, 3},
+                                 {4, 5}, {6, 7}, {2, 4}, {3, 5}, {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+    for (int c = 0; c < 19; ++c) sort_cx(key[kNet[c][0]], sl[kNet[c][0]], key[kNet[c][1]], sl[kNet[c][1]]);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        h.key[j] = key[j];
+        h.t[j] = sl[j];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -503,69 +551,124 @@ __device__ void coop_shells(TopK<KMAX>& h, int owner, float qx0, float qy0, floa
     }
 }
 
+// The slot run [x, y) of one column of the flat path's 3 x 3: the cells of
+// the column (base = cell index of z = 0) whose z-range can hold a point
+// within the bound, |z - qz| <= sqrt(bound - gxy) (widened by the growth
+// factor, which covers the hardware square root's error, and the lattice
+// slack), capped to [zlo, zhi]; (0, 0) for a column outside the lattice or
+// beyond the bound.
+__device__ __forceinline__ uint2 flat_column_run(bool valid, float gxy, float bound, float grow, float qz, int zlo,
+                                                 int zhi, uint32_t base, const GridDev& g,
+                                                 const uint32_t* __restrict__ bstart) {
+    uint2 r = make_uint2(0u, 0u);
+    if (valid && gxy <= bound) {
+        int z0 = zlo, z1 = zhi;
+        if (bound < INFINITY) {
+            const float half = __builtin_amdgcn_sqrtf(bound - gxy) * grow + g.slack;
+            z0 = max(z0, axis_cell_bound(qz - half, g.oz, g.rz, g.dz));
+            z1 = min(z1, axis_cell_bound(qz + half, g.oz, g.rz, g.dz));
+        }
+        if (z0 <= z1) {
+            const uint32_t c0 = base + (uint32_t)z0;
+            r.x = bstart[c0];
+            r.y = bstart[c0 + (uint32_t)(z1 - z0) + 1u];
+        }
+    }
+    return r;
+}
+
+// One query of the flat path: position, bound (an upper bound on the k-th
+// distance, or the cap), the query's own slot (UINT32_MAX: none), and whether
+// the lane has a query (lanes without one scan nothing and only help complete
+// other lanes' queries).
+struct FlatQuery {
+    float x, y, z, thr;
+    uint32_t self;
+    bool active;
+};
+
 // Requires 1 <= nB < kFlatMaxPoints and a minimum search radius of at most
-// one cell.  Called by every lane of the wavefront (lanes without a query
-// pass active = false: they scan nothing and only help complete other lanes'
-// queries).
+// one cell.  Called by every lane of the wavefront.
 template <int KMAX>
-__device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint32_t k, float thr, float qx,
-                                                float qy, float qz, uint32_t self_slot, const GridDev& g,
-                                                uint32_t nB, const float4* __restrict__ bpos,
-                                                const uint32_t* __restrict__ bstart, const FlatLds& L) {
+__device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, FlatQuery& Q, const GridDev& g, uint32_t nB,
+                                                const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart,
+                                                const FlatLds& L) {
     const __amdgpu_buffer_rsrc_t rb = pos_rsrc(bpos, nB);
-    // The query's cell only centres the ring-1 box; every bound below is
-    // geometric with the lattice slack, and the exactness test falls back to
-    // the shell search, so the cheap mapping is exact in effect.
+    const float grow = 1.0f + 2e-5f;
+    int M = 0;
+    bool long_run = false;
+    {
+        // The query's cell only centres the ring-1 box; every bound below is
+        // geometric with the lattice slack, and the exactness test falls
+        // back to the shell search, so the cheap mapping is exact in effect.
+        const float qx = Q.x, qy = Q.y, qz = Q.z;
+        const int cx = axis_cell_bound(qx, g.ox, g.rx, g.dx);
+        const int cy = axis_cell_bound(qy, g.oy, g.ry, g.dy);
+        const int cz = axis_cell_bound(qz, g.oz, g.rz, g.dz);
+        const float bound = fminf(Q.thr, h.worst()) * grow;
+        const int zlo = max(cz - g.zr, 0), zhi = min(cz + g.zr, g.dz - 1);
+        // Per-axis squared gaps and cell-index bases of the columns at
+        // offsets -1, 0, +1 (formed once per axis, not once per column).
+        float gx2[3], gy2[3];
+        bool vx[3], vy[3];
+        uint32_t xbase[3], ybase[3];
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            const int ix = cx + o - 1, iy = cy + o - 1;
+            const float gx = axis_gap(qx, ix, ix, g.ox, g.hx, g.dx, g.slack);
+            const float gy = axis_gap(qy, iy, iy, g.oy, g.hy, g.dy, g.slack);
+            gx2[o] = gx * gx;
+            gy2[o] = gy * gy;
+            vx[o] = ix >= 0 && ix < g.dx;
+            vy[o] = iy >= 0 && iy < g.dy;
+            xbase[o] = (uint32_t)(ix * g.dy * g.dz);
+            ybase[o] = (uint32_t)(iy * g.dz);
+        }
+        // Runs of the nine columns, plus a tenth: the query's own slot splits
+        // the run that holds it (the centre column's, whenever the query lies
+        // in the cell its coordinates map to), so the scan never buffers the
+        // query itself; the merge still drops it wherever it appears.  Every
+        // element is written unconditionally (selects, not branches), which
+        // keeps the arrays in scalar registers.
+        uint32_t r0[kFlatRuns], r1[kFlatRuns];
+#pragma unroll
+        for (int col = 0; col < 9; ++col) {
+            const int ox = col / 3, oy = col % 3;
+            const uint2 r = flat_column_run(vx[ox] && vy[oy], gx2[ox] + gy2[oy], bound, grow, qz, zlo, zhi,
+                                            xbase[ox] + ybase[oy], g, bstart);
+            r0[col] = r.x;
+            r1[col] = r.y;
+        }
+        {
+            const bool split = Q.self >= r0[4] && Q.self < r1[4];
+            r0[9] = split ? Q.self + 1u : 0u;
+            r1[9] = split ? r1[4] : 0u;
+            r1[4] = split ? Q.self : r1[4];
+        }
+        // Compact the non-empty runs into the lane's LDS list.  A run longer
+        // than the 16-bit length field sends the lane to the shell search
+        // instead.
+#pragma unroll
+        for (int col = 0; col < kFlatRuns; ++col) {
+            const uint32_t len = r1[col] - r0[col];
+            long_run = long_run || len > 0xFFFFu;
+            if (len != 0u) {
+                L.run_t[M * kBlock] = r0[col];
+                L.run_n[M * kBlock] = (uint16_t)len;
+                ++M;
+            }
+        }
+        if (long_run || !Q.active) M = 0;
+    }
+    const uint32_t* rt = L.run_t;
+    const uint16_t* rn = L.run_n;
+    const float qx = Q.x, qy = Q.y, qz = Q.z, thr = Q.thr;
+    const uint32_t self_slot = Q.self;
+    const bool active = Q.active;
     const int cx = axis_cell_bound(qx, g.ox, g.rx, g.dx);
     const int cy = axis_cell_bound(qy, g.oy, g.ry, g.dy);
     const int cz = axis_cell_bound(qz, g.oz, g.rz, g.dz);
-    const float grow = 1.0f + 2e-5f;
     float lim = fminf(thr, h.worst());
-    const float bound = lim * grow;
-    const int zlo = max(cz - g.zr, 0), zhi = min(cz + g.zr, g.dz - 1);
-    uint32_t r0[kFlatRuns], r1[kFlatRuns];
-#pragma unroll
-    for (int col = 0; col < kFlatRuns; ++col) {
-        const int ix = cx + col / 3 - 1, iy = cy + col % 3 - 1;
-        r0[col] = r1[col] = 0u;
-        if (ix >= 0 && ix < g.dx && iy >= 0 && iy < g.dy) {
-            const float gx = axis_gap(qx, ix, ix, g.ox, g.hx, g.dx, g.slack);
-            const float gy = axis_gap(qy, iy, iy, g.oy, g.hy, g.dy, g.slack);
-            const float gxy = gx * gx + gy * gy;
-            if (gxy <= bound) {
-                int z0 = zlo, z1 = zhi;
-                if (bound < INFINITY) {
-                    // Cells of this column that can hold a point within the
-                    // bound: |z - qz| <= sqrt(bound - gxy), widened by the
-                    // growth factor (covers the hardware square root's error)
-                    // and the lattice slack.
-                    const float half = __builtin_amdgcn_sqrtf(bound - gxy) * grow + g.slack;
-                    z0 = max(z0, axis_cell_bound(qz - half, g.oz, g.rz, g.dz));
-                    z1 = min(z1, axis_cell_bound(qz + half, g.oz, g.rz, g.dz));
-                }
-                if (z0 <= z1) {
-                    const uint32_t c0 = cell_index(g, ix, iy, z0);
-                    r0[col] = bstart[c0];
-                    r1[col] = bstart[c0 + (uint32_t)(z1 - z0) + 1u];
-                }
-            }
-        }
-    }
-    // Compact the non-empty runs into the lane's LDS list.  A run longer than
-    // the 16-bit length field sends the lane to the shell search instead.
-    int M = 0;
-    bool long_run = false;
-#pragma unroll
-    for (int col = 0; col < kFlatRuns; ++col) {
-        const uint32_t len = r1[col] - r0[col];
-        long_run = long_run || len > 0xFFFFu;
-        if (len != 0u) {
-            L.run_t[M * kBlock] = r0[col];
-            L.run_n[M * kBlock] = (uint16_t)len;
-            ++M;
-        }
-    }
-    if (long_run || !active) M = 0;
     // A chunk is (t, e): slots t .. min(t + 4, e) - 1 of the current run,
     // t < e; (0, 0) once the stream is exhausted.
     int m = 0;
@@ -574,8 +677,8 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint
         en = e;
         if (tn >= e) {
             if (m < M) {
-                tn = L.run_t[m * kBlock];
-                en = tn + L.run_n[m * kBlock];
+                tn = rt[m * kBlock];
+                en = tn + rn[m * kBlock];
                 ++m;
             } else {
                 tn = en = 0u;
@@ -584,8 +687,8 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint
     };
     uint32_t ta = 0u, ea = 0u, tb, eb;
     if (M > 0) {
-        ta = L.run_t[0];
-        ea = ta + L.run_n[0];
+        ta = rt[0];
+        ea = ta + rn[0];
         m = 1;
     }
     next_chunk(ta, ea, tb, eb);
@@ -600,6 +703,7 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint
     // which the fused form can differ from the exact one), and the merge
     // recomputes the exact key and bound.
     float limf = filter_limit(lim);
+    bool fresh = true;  // no merge yet: the list is empty
     for (;;) {
         while (__any(ta < ea) && !__any(wa > wfull)) {
             // Byte offsets of the two chunks; the per-candidate offsets are
@@ -633,11 +737,19 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, bool active, uint
             eb = ed;
         }
         int cnt = (int)((wa - L.slot_addr) / kEntry);
+        const bool done = !__any(ta < ea);
+        if constexpr (KMAX == 7) {
+            if (fresh && done && !__any(cnt > 8)) {
+                flat_sort_fresh(h, cnt, L, rb, qx, qy, qz, self_slot, thr);
+                break;
+            }
+        }
         flat_flush<KMAX>(h, cnt, L, rb, qx, qy, qz, self_slot, thr);
+        fresh = false;
         wa = L.slot_addr;
         lim = fminf(thr, h.worst());
         limf = filter_limit(lim);
-        if (!__any(ta < ea)) break;
+        if (done) break;
     }
     bool all;
     const float dmin = outside_gap(g, cx, cy, cz, 1, qx, qy, qz, all);
