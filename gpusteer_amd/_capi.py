@@ -265,6 +265,7 @@ _SIGS = {
     "gs_find_knn": (C.c_int, [_h, _h, _h, _h, C.c_uint32, C.c_uint32]),
     "gs_find_knn_within": (C.c_int, [_h, _h, _h, _h, C.c_uint32, C.c_uint32, C.c_float]),
     "gs_find_knn_brute": (C.c_int, [_h, _h, _h]),
+    "gs_group_bin_geometry": (C.c_int, [_h, _fp, _fp, _up]),
     # device arrays: px, py, pz (n floats), idx, dist (k * n)
     "gs_find_knn_noalloc": (C.c_int, [_h, _h, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                       C.c_void_p, C.c_void_p]),

@@ -79,7 +79,8 @@ ROOT = PKG.parent
 CXX_EXAMPLES = {"boids_group_plugin": ROOT / "tests" / "cxx" / "boids_group_plugin.cpp",
                 "choke_setup": ROOT / "tests" / "cxx" / "choke_setup.cpp",
                 "choke_point_plugin": ROOT / "tests" / "cxx" / "choke_point_plugin.cpp",
-                "multi_group_plugin": ROOT / "tests" / "cxx" / "multi_group_plugin.cpp"}
+                "multi_group_plugin": ROOT / "tests" / "cxx" / "multi_group_plugin.cpp",
+                "ctf_seeker_enemies": ROOT / "tests" / "cxx" / "ctf_seeker_enemies.cpp"}
 
 
 def build_cxx_examples() -> None:

@@ -494,7 +494,7 @@ int gs_group_download_agents(gs_group g, gs_agents* host) {
 
 int gs_group_device_agents(gs_group g, gs_agents* dev) {
     return guarded([&] {
-        check_group(g, "group", GS_KIND_AGENTS);
+        check_group(g, "group");
         if (!dev) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
         *dev = g->dev_agents();
         // The caller may write through these pointers.
@@ -685,6 +685,17 @@ int gs_group_download_bins(gs_group g, uint32_t* key, uint32_t* start, uint32_t*
     });
 }
 
+int gs_group_bin_geometry(gs_group g, float origin[3], float size[3], uint32_t cells[3]) {
+    return guarded([&] {
+        check_group(g, "group");
+        if (!g->binned) fail(GS_ERR_NOT_BINNED, "group has no bin database");
+        const GridDev& b = g->bin_grid;
+        if (origin) origin[0] = b.ox, origin[1] = b.oy, origin[2] = b.oz;
+        if (size) size[0] = b.sx, size[1] = b.sy, size[2] = b.sz;
+        if (cells) cells[0] = (uint32_t)b.dx, cells[1] = (uint32_t)b.dy, cells[2] = (uint32_t)b.dz;
+    });
+}
+
 int gs_group_check_finite(gs_group g, uint32_t* first_bad) {
     uint32_t bad = UINT32_MAX;
     int s = guarded([&] {
@@ -693,7 +704,8 @@ int gs_group_check_finite(gs_group g, uint32_t* first_bad) {
         uint32_t slot = UINT32_MAX;
         launch_check_finite(g, &slot);
         if (slot != UINT32_MAX) {
-            GS_HIP(hipMemcpy(&bad, g->ids() + slot, 4, hipMemcpyDeviceToHost));
+            GS_HIP(hipMemcpyAsync(&bad, g->ids() + slot, 4, hipMemcpyDeviceToHost, g->ctx->stream));
+            GS_HIP(hipStreamSynchronize(g->ctx->stream));
         }
     });
     if (first_bad) *first_bad = bad;

@@ -180,7 +180,11 @@ int gs_group_download_agents(gs_group g, gs_agents* host);
  * cell order, so storage slot s holds agent id_of_slot[s] (see
  * gs_group_device_order).  Valid until the next build, upload,
  * gs_step_boids or gs_anti_penetration_agents on the group (these write
- * the group's alternate buffers and switch to them). */
+ * the group's alternate buffers and switch to them).  For a wall group the
+ * arrays px/py/pz hold the KNN positions (midpoints), fx/fy/fz the start,
+ * speed/ax/ay the end and az/sx/sy the normal points (fields 3..11 of
+ * gs_agents); for an obstacle group px/py/pz the centres and radius the
+ * radii; the other pointers are NULL for those kinds. */
 int gs_group_device_agents(gs_group g, gs_agents* dev);
 /* The group's storage order: agent index of each slot and slot of each agent
  * index (device arrays of n entries; valid until the next build or upload). */
@@ -196,6 +200,10 @@ int gs_group_upload_walls(gs_group g, uint32_t n, const gs_walls* host);
  * sorted positions spx/spy/spz[n]. */
 int gs_group_download_bins(gs_group g, uint32_t* key, uint32_t* start, uint32_t* perm, float* spx, float* spy,
                            float* spz);
+/* The lattice of the group's last bin build (KNNDatabase::worldCells,
+ * include/OpenSteer/CUDA/KNNDatabase.cuh:80); GS_ERR_NOT_BINNED before the
+ * first build.  Any output may be NULL. */
+int gs_group_bin_geometry(gs_group g, float origin[3], float size[3], uint32_t cells[3]);
 /* Scan the agent state for NaN/Inf: GS_NONFINITE if any, *first_bad = agent index. */
 int gs_group_check_finite(gs_group g, uint32_t* first_bad);
 

@@ -6,9 +6,14 @@
  * Re-created names (reference file:line):
  *   AgentData                     include/OpenSteer/AgentData.h:13-42
  *   ObstacleData                  include/OpenSteer/CUDA/ObstacleGroupData.cuh:10-14
+ *   BaseGroup                     include/OpenSteer/BaseGroup.h:8-33 (GetKNNDatabase, Size, pd*)
+ *   KNNDatabase                   include/OpenSteer/CUDA/KNNDatabase.cuh:30-123 (host view)
  *   AgentGroup                    include/OpenSteer/AgentGroup.h:12-56, src/AgentGroup.cpp:8-140
+ *   AgentGroupData                include/OpenSteer/AgentGroupData.cuh:19-297 (getAgentData, size, pd*)
  *   ObstacleGroup                 include/OpenSteer/ObstacleGroup.h:10-43
- *   WallGroup                     include/OpenSteer/WallGroup.h:12-37 (LoadFromFile, SplitWalls, AddWall)
+ *   WallGroup                     include/OpenSteer/WallGroup.h:12-37 (LoadFromFile, SplitWalls, AddWall,
+ *                                 GetWallGroupData)
+ *   WallGroupData                 include/OpenSteer/CUDA/WallGroupData.cuh:12-58 (hvLine*, size, pdLine*)
  *   KNNData                       include/OpenSteer/CUDA/KNNData.cuh:10-55
  *   KNNBinData, bin_cell          include/OpenSteer/CUDA/KNNBinDataV2.cuh:43-61, KNNBinDataGlobals.cuh:7-13
  *   SimulationParams, WorldParams,
@@ -24,7 +29,15 @@
  *     kernels (GS_MODE_GPUREF) unless gpusteer::Context::SetMode selects the
  *     CPU OpenSteer formulas (GS_MODE_OPENSTEER);
  *   - the binned KNN is exact (the reference's V2 kernel scans a fixed cell
- *     radius and can miss neighbours).
+ *     radius and can miss neighbours);
+ *   - device arrays are struct-of-arrays in the library's storage order (a
+ *     bin build puts each group's agents in cell order): pdSpeed(),
+ *     pdRadius() and the other scalar fields are float* as in the reference,
+ *     but slot s holds agent pdID()[s], not agent s; pdPosition(),
+ *     pdDirection() and the other vector fields return gpusteer::DeviceVec3
+ *     (three float arrays) where the reference returns float4* / float3*.
+ *     Host views (getAgentData, hvLine*, KNNDatabase's hv*) are in agent
+ *     index order as in the reference.
  */
 #ifndef GPUSTEER_OPENSTEER_HPP
 #define GPUSTEER_OPENSTEER_HPP
@@ -84,6 +97,15 @@ private:
     int closest_ = GS_CLOSEST_CLAMPED;
 };
 
+// A device vector field as three float arrays (the library's struct of
+// arrays), in storage order: what pdPosition() and the other vector-field
+// accessors return where the reference returns float4* / float3*.
+struct DeviceVec3 {
+    float* x;
+    float* y;
+    float* z;
+};
+
 }  // namespace gpusteer
 
 namespace OpenSteer {
@@ -119,10 +141,45 @@ struct bin_cell {
     float3 maxBound;
 };
 
+class BaseGroup;
+
+// The host view of a group's bin database after updateKNNDatabase
+// (KNNDatabase, include/OpenSteer/CUDA/KNNDatabase.cuh:30-123): in the
+// reference's cell numbering ix + iz X + iy X Z (CUDA/KNNBinDataV2.cu:103-135)
+// and sort order (agents stably sorted by that cell index, the result of its
+// thrust::sort_by_key), downloaded on first access after each build.  The
+// device arrays of the database are the library's own (storage order, its
+// cell numbering) and are not exposed.
+class KNNDatabase {
+public:
+    KNNDatabase(BaseGroup& g, uint k, uint3 const& worldCells)
+        : g_(&g), k_(k), cells_(worldCells.x * worldCells.y * worldCells.z), worldCells_(worldCells) {}
+    uint const& k() const { return k_; }
+    uint const& size() const { return n_; }
+    uint const& cells() const { return cells_; }
+    uint3 const& worldCells() const { return worldCells_; }
+    // Cell of each agent (agent index order), the cells and agent indices in
+    // sorted order, and the positions in sorted order (w = 0).
+    std::vector<uint> const& hvCellIndices() { syncHost(); return cell_; }
+    std::vector<uint> const& hvCellIndicesSorted() { syncHost(); return cell_sorted_; }
+    std::vector<uint> const& hvAgentIndicesSorted() { syncHost(); return agent_sorted_; }
+    std::vector<float4> const& hvPositionSorted() { syncHost(); return pos_sorted_; }
+    void setSyncHost() { host_valid_ = false; }
+    inline void syncHost();
+
+private:
+    BaseGroup* g_;
+    uint k_, n_ = 0, cells_;
+    uint3 worldCells_;
+    std::vector<uint> cell_, cell_sorted_, agent_sorted_;
+    std::vector<float4> pos_sorted_;
+    bool host_valid_ = false;
+};
+
 class BaseGroup {
 public:
     BaseGroup(int kind, uint3 const& worldCells, uint k, gpusteer::Context& ctx)
-        : ctx_(&ctx), worldCells_(worldCells), k_(k) {
+        : ctx_(&ctx), kind_(kind), worldCells_(worldCells), k_(k), db_(*this, k, worldCells) {
         gpusteer::check(gs_group_create(ctx.handle(), kind, &h_), "gs_group_create");
     }
     virtual ~BaseGroup() { gs_group_destroy(h_); }
@@ -133,25 +190,143 @@ public:
     uint3 const& WorldCells() const { return worldCells_; }
     uint K() const { return k_; }
     gpusteer::Context& context() const { return *ctx_; }
+    // BaseGroup::GetKNNDatabase (include/OpenSteer/BaseGroup.h:24).
+    KNNDatabase& GetKNNDatabase() { return db_; }
     // Uploads pending host edits; called by every operation before it runs.
     virtual void SyncDevice() {}
     gs_group device_handle() {
         SyncDevice();
         return h_;
     }
+    // The handle without uploading pending host edits (host views).
+    gs_group raw_handle() const { return h_; }
+    // Called by every operation that may change the device state.
+    virtual void SetSyncHost() { db_.setSyncHost(); }
+
+    // BaseGroup::pdPosition / pdDirection / pdSpeed / pdRadius
+    // (include/OpenSteer/BaseGroup.h:27-30): the group's device arrays in
+    // storage order (slot s holds agent pdID()[s]), valid until the next
+    // operation on the group.  A wall group's positions are its midpoints
+    // and it has no direction, speed or radius (NULL, as WallGroup.h:29-31);
+    // an obstacle group has positions and radii.
+    gpusteer::DeviceVec3 pdPosition() { return vec3(0); }
+    gpusteer::DeviceVec3 pdDirection() {
+        return kind_ == GS_KIND_AGENTS ? vec3(3) : gpusteer::DeviceVec3{nullptr, nullptr, nullptr};
+    }
+    float* pdSpeed() { return kind_ == GS_KIND_AGENTS ? field(6) : nullptr; }
+    float* pdRadius() { return kind_ == GS_KIND_WALLS ? nullptr : field(19); }
+    // Agent index of each storage slot.
+    uint* pdID() {
+        uint32_t* ids = nullptr;
+        gpusteer::check(gs_group_device_order(device_handle(), &ids, nullptr), "gs_group_device_order");
+        return ids;
+    }
 
 protected:
+    gs_agents device_arrays() {
+        gs_agents a;
+        gpusteer::check(gs_group_device_agents(device_handle(), &a), "gs_group_device_agents");
+        return a;
+    }
+    // Field j of gs_agents (the order of include/gpusteer.h).
+    float* field(int j) {
+        const gs_agents a = device_arrays();
+        return ((float* const*)&a)[j];
+    }
+    gpusteer::DeviceVec3 vec3(int j) {
+        const gs_agents a = device_arrays();
+        float* const* f = (float* const*)&a;
+        return gpusteer::DeviceVec3{f[j], f[j + 1], f[j + 2]};
+    }
+
     gpusteer::Context* ctx_;
+    int kind_;
     gs_group h_ = nullptr;
     uint3 worldCells_;
     uint k_;
     uint n_ = 0;
+    KNNDatabase db_;
+};
+
+inline void KNNDatabase::syncHost() {
+    if (host_valid_) return;
+    gs_group h = g_->raw_handle();
+    uint32_t n = 0;
+    gpusteer::check(gs_group_size(h, &n), "gs_group_size");
+    uint32_t c[3];
+    gpusteer::check(gs_group_bin_geometry(h, nullptr, nullptr, c), "gs_group_bin_geometry");
+    std::vector<uint32_t> key(n), perm(n);
+    std::vector<float> sx(n), sy(n), sz(n);
+    gpusteer::check(gs_group_download_bins(h, key.data(), nullptr, perm.data(), sx.data(), sy.data(), sz.data()),
+                    "gs_group_download_bins");
+    // The library numbers cells ix*Y*Z + iy*Z + iz (src/lq.c's order); the
+    // reference's KNNBinDataV2 numbers them ix + iz*X + iy*X*Z.
+    const uint32_t X = c[0], Y = c[1], Z = c[2];
+    n_ = n;
+    worldCells_ = make_uint3(X, Y, Z);
+    cells_ = X * Y * Z;
+    cell_.resize(n);
+    for (uint32_t a = 0; a < n; ++a) {
+        const uint32_t ix = key[a] / (Y * Z), iy = (key[a] / Z) % Y, iz = key[a] % Z;
+        cell_[a] = ix + iz * X + iy * X * Z;
+    }
+    std::vector<float4> pos(n);
+    for (uint32_t t = 0; t < n; ++t) pos[perm[t]] = make_float4(sx[t], sy[t], sz[t], 0.f);
+    // Stable sort of the agent indices by cell (ties in agent index order).
+    std::vector<uint32_t> count((size_t)cells_ + 1, 0u);
+    for (uint32_t a = 0; a < n; ++a) ++count[cell_[a] + 1];
+    for (size_t i = 0; i < cells_; ++i) count[i + 1] += count[i];
+    agent_sorted_.assign(n, 0u);
+    for (uint32_t a = 0; a < n; ++a) agent_sorted_[count[cell_[a]]++] = a;
+    cell_sorted_.resize(n);
+    pos_sorted_.resize(n);
+    for (uint32_t s = 0; s < n; ++s) {
+        cell_sorted_[s] = cell_[agent_sorted_[s]];
+        pos_sorted_[s] = pos[agent_sorted_[s]];
+    }
+    host_valid_ = true;
+}
+
+class AgentGroup;
+
+// AgentGroupData (include/OpenSteer/AgentGroupData.cuh:19-297): a group's
+// agents as plugins read them.  getAgentData(i, ad) is agent i of the group
+// (insertion order, as the reference's arrays), synchronised from the device
+// when an operation has run since the last read.  The pd* accessors return
+// the device arrays in storage order (see BaseGroup::pdPosition).
+class AgentGroupData {
+public:
+    explicit AgentGroupData(AgentGroup& g) : g_(&g) {}
+    inline size_t size() const;
+    inline void getAgentData(size_t const index, AgentData& ad);
+    inline void syncHost();
+    inline void syncDevice();
+    inline void setSyncDevice();
+    inline uint* pdAppliedKernels();
+    inline uint* pdID();
+    inline gpusteer::DeviceVec3 pdPosition();
+    inline gpusteer::DeviceVec3 pdDirection();
+    inline gpusteer::DeviceVec3 pdSide();
+    inline gpusteer::DeviceVec3 pdUp();
+    inline gpusteer::DeviceVec3 pdSteering();
+    inline float* pdSpeed();
+    inline float* pdMaxSpeed();
+    inline float* pdMaxForce();
+    inline float* pdRadius();
+    inline float* pdMass();
+
+private:
+    AgentGroup* g_;
 };
 
 class AgentGroup : public BaseGroup {
+    friend class AgentGroupData;
+
 public:
     AgentGroup(uint3 const& worldCells, uint const knn, gpusteer::Context& ctx = gpusteer::Context::Default())
-        : BaseGroup(GS_KIND_AGENTS, worldCells, knn, ctx) {}
+        : BaseGroup(GS_KIND_AGENTS, worldCells, knn, ctx), data_(*this) {}
+    // AgentGroup::GetAgentGroupData (include/OpenSteer/AgentGroup.h:42).
+    AgentGroupData& GetAgentGroupData() { return data_; }
 
     // Adds an agent unless its id is already present (src/AgentGroup.cpp:34-55).
     bool AddAgent(AgentData const& ad) {
@@ -189,8 +364,11 @@ public:
         ad = agents_[it->second];
         return true;
     }
-    // Marks the host copy stale; the next host access downloads.
-    void SetSyncHost() { host_valid_ = false; }
+    // Marks the host copies stale; the next host access downloads.
+    void SetSyncHost() override {
+        host_valid_ = false;
+        BaseGroup::SetSyncHost();
+    }
 
     void SyncDevice() override {
         if (!dirty_) return;
@@ -251,7 +429,32 @@ private:
     std::map<id_type, size_t> index_;
     bool dirty_ = false;
     bool host_valid_ = true;
+    AgentGroupData data_;
 };
+
+inline size_t AgentGroupData::size() const { return g_->agents_.size(); }
+inline void AgentGroupData::getAgentData(size_t const index, AgentData& ad) {
+    g_->SyncHost();
+    if (index < g_->agents_.size()) ad = g_->agents_[index];
+}
+inline void AgentGroupData::syncHost() { g_->SyncHost(); }
+inline void AgentGroupData::syncDevice() { g_->SyncDevice(); }
+// Host edits made through getAgentData's copies are not seen; AddAgent /
+// RemoveAgent mark the group for upload themselves.  Kept for plugins that
+// call it after editing their own copies.
+inline void AgentGroupData::setSyncDevice() { g_->dirty_ = true; }
+inline uint* AgentGroupData::pdAppliedKernels() { return g_->device_arrays().bits; }
+inline uint* AgentGroupData::pdID() { return g_->pdID(); }
+inline gpusteer::DeviceVec3 AgentGroupData::pdPosition() { return g_->vec3(0); }
+inline gpusteer::DeviceVec3 AgentGroupData::pdDirection() { return g_->vec3(3); }
+inline gpusteer::DeviceVec3 AgentGroupData::pdSide() { return g_->vec3(10); }
+inline gpusteer::DeviceVec3 AgentGroupData::pdUp() { return g_->vec3(13); }
+inline gpusteer::DeviceVec3 AgentGroupData::pdSteering() { return g_->vec3(16); }
+inline float* AgentGroupData::pdSpeed() { return g_->field(6); }
+inline float* AgentGroupData::pdMaxSpeed() { return g_->field(20); }
+inline float* AgentGroupData::pdMaxForce() { return g_->field(21); }
+inline float* AgentGroupData::pdRadius() { return g_->field(19); }
+inline float* AgentGroupData::pdMass() { return g_->field(22); }
 
 class ObstacleGroup : public BaseGroup {
 public:
@@ -298,13 +501,42 @@ private:
     bool dirty_ = false;
 };
 
+class WallGroup;
+
+// WallGroupData (include/OpenSteer/CUDA/WallGroupData.cuh:12-58): the host
+// vectors of the walls (start, midpoint, end, normal; w = 0) in wall order,
+// and their device arrays (storage order, see BaseGroup::pdPosition).
+class WallGroupData {
+public:
+    explicit WallGroupData(WallGroup& w) : w_(&w) {}
+    std::vector<float4> const& hvLineStart() const { return start_; }
+    std::vector<float4> const& hvLineMid() const { return mid_; }
+    std::vector<float4> const& hvLineEnd() const { return end_; }
+    std::vector<float4> const& hvLineNormal() const { return normal_; }
+    uint const& size() const { return n_; }
+    inline gpusteer::DeviceVec3 pdLineStart();
+    inline gpusteer::DeviceVec3 pdLineMid();
+    inline gpusteer::DeviceVec3 pdLineEnd();
+    inline gpusteer::DeviceVec3 pdLineNormal();
+
+private:
+    friend class WallGroup;
+    WallGroup* w_;
+    std::vector<float4> start_, mid_, end_, normal_;
+    uint n_ = 0;
+};
+
 // Wall segments (line start, end and normal; include/OpenSteer/CUDA/WallGroupData.cuh).
 class WallGroup : public BaseGroup {
+    friend class WallGroupData;
+
 public:
     WallGroup(uint3 const& worldCells, uint const knw, gpusteer::Context& ctx = gpusteer::Context::Default())
-        : BaseGroup(GS_KIND_WALLS, worldCells, knw, ctx) {
+        : BaseGroup(GS_KIND_WALLS, worldCells, knw, ctx), data_(*this) {
         gpusteer::check(gs_wall_set_create(0, nullptr, &set_), "gs_wall_set_create");
     }
+    // WallGroup::GetWallGroupData (include/OpenSteer/WallGroup.h:25).
+    WallGroupData& GetWallGroupData() { return data_; }
     ~WallGroup() { gs_wall_set_destroy(set_); }
     void Clear() { replace(nullptr, 0); }
     void AddWall(float3 const& start, float3 const& end, float3 const& normal) {
@@ -383,12 +615,27 @@ private:
         refresh();
     }
     void refresh() {
-        uint32_t n = 0;
-        gpusteer::check(gs_wall_set_size(set_, &n), "gs_wall_set_size");
+        std::vector<float> v[12];
+        get(v);
+        const uint32_t n = (uint32_t)v[0].size();
         n_ = n;
         dirty_ = true;
+        data_.n_ = n;
+        std::vector<float4>* h[4] = {&data_.start_, &data_.end_, &data_.normal_, &data_.mid_};
+        for (int q = 0; q < 4; ++q) {
+            h[q]->resize(n);
+            for (uint32_t i = 0; i < n; ++i) (*h[q])[i] = make_float4(v[3 * q][i], v[3 * q + 1][i], v[3 * q + 2][i], 0.f);
+        }
     }
+    WallGroupData data_;
 };
+
+// Wall fields in the group's device arrays: midpoint 0..2, start 3..5,
+// end 6..8, normal 9..11 (gs_group_device_agents).
+inline gpusteer::DeviceVec3 WallGroupData::pdLineStart() { return w_->vec3(3); }
+inline gpusteer::DeviceVec3 WallGroupData::pdLineMid() { return w_->vec3(0); }
+inline gpusteer::DeviceVec3 WallGroupData::pdLineEnd() { return w_->vec3(6); }
+inline gpusteer::DeviceVec3 WallGroupData::pdLineNormal() { return w_->vec3(9); }
 
 class KNNData {
 public:
@@ -508,6 +755,7 @@ inline void steerToAvoidWalls(AgentGroup* pAgentGroup, KNNData* pKNNData, WallGr
 }
 inline void updateKNNDatabase(BaseGroup* pGroup, KNNBinData* pKNNBinData) {
     gpusteer::check(gs_update_knn_database(pGroup->device_handle(), pKNNBinData->handle()), "gs_update_knn_database");
+    pGroup->SetSyncHost();
 }
 inline void findKNearestNeighbors(AgentGroup* pAgentGroup, KNNData* pKNNData, KNNBinData* pKNNBinData,
                                   BaseGroup* pOtherGroup, uint const searchRadius) {

@@ -226,3 +226,95 @@ def test_cxx_multi_group_plugin_matches_oracle(orc, tmp_path, name, close):
                                                  "ux", "uy", "uz")], axis=1)
         np.testing.assert_array_equal(got, want, err_msg=f"group {gi}")
     assert off == rec.size
+
+
+SEEKER = gbuild.LIB_DIR / "ctf_seeker_enemies"
+
+
+def test_cxx_seeker_reads_enemies_through_host_views(orc, tmp_path):
+    """The choke-point seeker's enemy loop (plugins/ChokePointGroup.cpp:886-894:
+    AgentGroupData & agd = gEnemies->GetAgentGroupData(); agd.getAgentData(i,
+    aData)) on the C++ drop-in after 30 frames of the enemies on the device
+    (KNN, separation, seek of the seeker, integrate) and one more frame's
+    steering: every enemy's state and the seeker's distances equal the
+    oracle's, bit for bit; the applied-behaviour bits are the oracle's and a
+    hipMemset through pdAppliedKernels() clears them on the device; the KNN
+    database's host vectors (GetKNNDatabase) are the oracle's bins in the
+    reference's cell numbering and stable sort order; the wall group's host
+    vectors (GetWallGroupData().hvLine*) are the split walls."""
+    from gpusteer_amd import formats
+
+    assert SEEKER.exists(), f"{SEEKER} not built"
+    p = formats.load_params(FILES / "ChokePoint.params")
+    ws = formats.WallSet.load_map(FILES / p.map_filename).split(p.dimensions, p.cells)
+    walls, mid = ws.get()
+    ws.close()
+    sc = scenarios.choke_point_new(p, walls, mid, n_agents=3000, dt=0.25, start=(0.0, 0.0, 60.0),
+                                   max_start_radius=50.0)
+    st = sc.state
+    inp, out = tmp_path / "agents.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        np.array([st.n], np.uint32).tofile(f)
+        for fld in FIELDS_IN:
+            st.arrays[fld].astype(np.float32).tofile(f)
+    steps, dt, seeker = 30, np.float32(0.25), (np.float32(3.5), np.float32(-20.0))
+    print("cxx seeker over the enemy group", flush=True)
+    r = subprocess.run([str(SEEKER), str(FILES / "ChokePoint.params"), str(FILES), str(inp), str(steps),
+                        repr(float(dt)), repr(float(seeker[0])), repr(float(seeker[1])), str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.uint8)
+    n = st.n
+    rec = raw[: n * 32].view(np.float32).reshape(n, 8)
+    bits = rec[:, 6:8].view(np.uint32)
+    at = n * 32
+    db_cell = raw[at: at + 4 * n].view(np.uint32); at += 4 * n
+    db_agent = raw[at: at + 4 * n].view(np.uint32); at += 4 * n
+    db_cell_sorted = raw[at: at + 4 * n].view(np.uint32); at += 4 * n
+    db_pos = raw[at: at + 16 * n].view(np.float32).reshape(n, 4); at += 16 * n
+    nw = int(raw[at: at + 4].view(np.uint32)[0]); at += 4
+    wall_rec = raw[at: at + 64 * nw].view(np.float32).reshape(nw, 4, 4)
+
+    # Oracle: the same frames in the reference kernels' formulas.
+    ref = st.copy()
+    cells = tuple(p.cells)
+    g = orc.grid(tuple(-0.5 * np.float32(v) for v in p.dimensions), p.dimensions, cells)
+    target = (float(seeker[0]), 0.0, float(seeker[1]))
+
+    def steer(state):
+        pos = state.pos().astype(np.float32)
+        idx, _ = orc.knn_binned(g, pos, pos, p.knn, True)
+        orc.steer_flock(orc.FLOCK_SEPARATION, state, idx, state, p.min_separation_distance,
+                        p.max_separation_distance, p.cos_max_flocking_angle, p.weight_separation,
+                        p.mask_separation, orc.MODE_GPUREF)
+        orc.steer_seek(state, target, p.weight_seek, p.mask_seek, orc.MODE_GPUREF)
+
+    for _ in range(steps):
+        steer(ref)
+        orc.update(ref, float(dt), orc.MODE_GPUREF)
+    steer(ref)
+    pos = np.stack([ref.arrays[f] for f in ("px", "py", "pz")], axis=1).astype(np.float32)
+    np.testing.assert_array_equal(rec[:, 0:3], pos)
+    np.testing.assert_array_equal(rec[:, 3], ref.arrays["speed"])
+    off = [pos[:, 0] - seeker[0], pos[:, 1] - np.float32(0.0), pos[:, 2] - seeker[1]]
+    d2 = (off[0] * off[0] + off[1] * off[1]) + off[2] * off[2]
+    np.testing.assert_array_equal(rec[:, 4], np.sqrt(d2))
+    fwd = (np.float32(0.0) * off[0] + np.float32(0.0) * off[1]) + np.float32(1.0) * off[2]
+    np.testing.assert_array_equal(rec[:, 5], fwd)
+    np.testing.assert_array_equal(bits[:, 0], ref.bits)
+    assert bits[:, 0].any() and not bits[:, 1].any()
+    # KNN database: lq-numbered oracle bins -> ix + iz X + iy X Z, stable sort.
+    key = orc.bins_build(g, pos)[0]
+    X, Y, Z = cells
+    ix, iy, iz = key // (Y * Z), (key // Z) % Y, key % Z
+    ref_cell = (ix + iz * X + iy * X * Z).astype(np.uint32)
+    order = np.argsort(ref_cell, kind="stable").astype(np.uint32)
+    np.testing.assert_array_equal(db_cell, ref_cell)
+    np.testing.assert_array_equal(db_agent, order)
+    np.testing.assert_array_equal(db_cell_sorted, ref_cell[order])
+    np.testing.assert_array_equal(db_pos[:, :3], pos[order])
+    # Walls: start, mid, end, normal of the split walls.
+    assert nw == walls.n
+    for q, cols in ((0, ("sx", "sy", "sz")), (2, ("ex", "ey", "ez")), (3, ("nx", "ny", "nz"))):
+        np.testing.assert_array_equal(wall_rec[:, q, :3], np.stack([walls.arrays[c] for c in cols], axis=1))
+    np.testing.assert_array_equal(wall_rec[:, 1, :3], np.asarray(mid, np.float32).reshape(nw, 3))
