@@ -23,6 +23,8 @@ GS_ERR_OUT_OF_MEMORY = 6
 GS_ERR_DEVICE = 7
 GS_ERR_NONFINITE = 8
 GS_MAX_K = 32
+GS_MAX_CELLS = 1 << 28
+GS_MAX_SPLIT_CELLS = 1 << 24
 # Limits of gs_find_knn_noalloc (its fixed device array).
 GS_NOALLOC_MAX_POINTS = 65536
 GS_NOALLOC_MAX_CELLS = 131072

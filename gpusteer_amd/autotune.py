@@ -11,10 +11,11 @@ import time
 from typing import List, Optional, Sequence, Tuple
 
 from . import api
+from ._capi import GS_MAX_CELLS
 
 Cells = Tuple[int, int, int]
 # The largest lattice gs_bins_create accepts.
-MAX_CELLS = 1 << 28
+MAX_CELLS = GS_MAX_CELLS
 BOIDS_CANDIDATES: Tuple[Tuple[Cells, int], ...] = (
     ((48, 48, 48), 1), ((64, 64, 64), 1), ((48, 48, 96), 1), ((64, 64, 128), 1), ((64, 64, 192), 1),
     ((80, 80, 160), 1), ((96, 96, 96), 2), ((128, 128, 128), 2))

@@ -88,7 +88,7 @@ static GridDev make_grid(const float* o, const float* s, const uint32_t* c) {
         if (c[a] == 0) fail(GS_ERR_INVALID_ARGUMENT, "bin lattice needs at least one cell per axis");
     }
     const uint64_t C = (uint64_t)c[0] * c[1] * c[2];
-    if (C > (1ull << 28)) fail(GS_ERR_UNSUPPORTED, "bin lattice larger than 2^28 cells");
+    if (C > GS_MAX_CELLS) fail(GS_ERR_UNSUPPORTED, "bin lattice larger than GS_MAX_CELLS (2^28) cells");
     GridDev g;
     g.ox = o[0];
     g.oy = o[1];

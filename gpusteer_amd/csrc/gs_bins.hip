@@ -21,9 +21,13 @@
 
 namespace gs {
 
-// Cells with more members than this keep the atomic arrival order in pass 4
-// (KNN results never depend on the order inside a cell).
-constexpr uint32_t kCanonCap = 4096;
+// Members of a cell are put in ascending agent index (the stable sort's
+// order) by counting, for each member, the members with a smaller index: m
+// reads per member of an m-member cell, issued by all members at once.
+// Cells with more members than this keep the atomic arrival order instead
+// (KNN results never depend on the order inside a cell; the per-cell
+// velocity sum of such a cell does).
+constexpr uint32_t kCanonCap = 65536;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;  // 2048 cells per scan block
 

@@ -444,8 +444,8 @@ int gs_wall_set_split(gs_wall_set w, const float world_size[3], const uint32_t c
             if (!(world_size[a] > 0.f) || !std::isfinite(world_size[a]) || cells[a] == 0)
                 fail(GS_ERR_INVALID_ARGUMENT, "lattice size and cells must be positive");
         }
-        if ((uint64_t)cells[0] * cells[1] * cells[2] > (1ull << 24))
-            fail(GS_ERR_UNSUPPORTED, "bin lattice larger than 2^24 cells");
+        if ((uint64_t)cells[0] * cells[1] * cells[2] > GS_MAX_SPLIT_CELLS)
+            fail(GS_ERR_UNSUPPORTED, "lattice larger than GS_MAX_SPLIT_CELLS (2^24) cells for a wall split");
         split_walls(w, lattice_cells(world_size, cells));
     });
 }

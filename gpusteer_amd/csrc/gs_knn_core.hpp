@@ -169,6 +169,42 @@ __device__ __forceinline__ void flush_deferred(TopK<KMAX>& h, int& cnt, const De
     cnt = 0;
 }
 
+// Every point of the lattice, slots bstart[0] .. bstart[C] (C = the
+// lattice's cells): the exact completion of a query whose next search box
+// would hold more cells than the lattice holds points (sparse or tiny
+// target sets on large lattices), at a cost bounded by the point count.
+// Points already in the list (from the boxes scanned before) are skipped by
+// slot; sentinel and empty entries hold no slot.
+template <int KMAX>
+__device__ void full_scan(TopK<KMAX>& h, float thr, float qx, float qy, float qz, uint32_t self_slot,
+                          uint32_t t0, uint32_t t1, const float4* __restrict__ bpos) {
+    float lim = fminf(thr, h.worst());
+    for (uint32_t t = t0; t < t1; ++t) {
+        const float4 p = bpos[t];
+        const float d2 = sqdist4(qx, qy, qz, p);
+        if (d2 <= lim && t != self_slot) {
+            const uint64_t kk = make_key(d2, id_of(p));
+            if (h.accepts(kk)) {
+                bool dup = false;
+#pragma unroll
+                for (int j = 0; j < KMAX; ++j) dup = dup || h.t[j] == t;
+                if (!dup) {
+                    h.push(kk, t);
+                    lim = fminf(thr, h.worst());
+                }
+            }
+        }
+    }
+}
+// Cells of the ring-R box (R cells each way in x and y, R * zr in z),
+// clipped to the lattice, and whether that box holds more cells than twice
+// the lattice's points (then a full scan is cheaper than the ring walk).
+__device__ __forceinline__ bool box_beyond_points(const GridDev& g, int R, uint32_t points) {
+    const uint64_t bx = (uint64_t)min(2 * R + 1, g.dx), by = (uint64_t)min(2 * R + 1, g.dy);
+    const uint64_t bz = (uint64_t)min(2 * R * g.zr + 1, g.dz);
+    return bx * by * bz > 2ull * points + 64ull;
+}
+
 // Generic shells R >= Rstart (rare path): scan the rings between the boxes of
 // ring R - 1 and R (R cells each way in x and y, R * zr in z), each as
 // contiguous z-runs of storage slots, skip runs farther than the bound, stop
@@ -178,7 +214,12 @@ __device__ void shell_search(TopK<KMAX>& h, float thr, float qx, float qy, float
                              const GridDev& g, int cx, int cy, int cz, int Rstart, int R0,
                              const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart) {
     const float grow = 1.0f + 2e-5f;
+    const uint32_t tlo = bstart[0], thi = bstart[(uint32_t)g.dx * (uint32_t)g.dy * (uint32_t)g.dz];
     for (int R = Rstart;; ++R) {
+        if (R > 1 && box_beyond_points(g, R, thi - tlo)) {
+            full_scan<KMAX>(h, thr, qx, qy, qz, self_slot, tlo, thi, bpos);
+            return;
+        }
         const int x0 = max(cx - R, 0), x1 = min(cx + R, g.dx - 1);
         const int y0 = max(cy - R, 0), y1 = min(cy + R, g.dy - 1);
         for (int ix = x0; ix <= x1; ++ix) {
@@ -496,12 +537,19 @@ __device__ void coop_shells(TopK<KMAX>& h, int owner, float qx0, float qy0, floa
     const uint32_t self_slot = (uint32_t)__shfl((int)self0, owner, 64);
     const int cx = __shfl(cx0, owner, 64), cy = __shfl(cy0, owner, 64), cz = __shfl(cz0, owner, 64);
     const float grow = 1.0f + 2e-5f;
+    const uint32_t tlo = bstart[0], thi = bstart[(uint32_t)g.dx * (uint32_t)g.dy * (uint32_t)g.dz];
     for (int R = 2;; ++R) {
+        // A box with more cells than the lattice has points: the owner scans
+        // every point instead (the enumeration below visits the whole box).
+        if ((uint64_t)(2 * R + 1) * (uint64_t)(2 * R + 1) * (uint64_t)(2 * R * g.zr + 1) > 2ull * (thi - tlo) + 64ull) {
+            if (lane == owner) full_scan<KMAX>(h, thr, qx, qy, qz, self_slot, tlo, thi, bpos);
+            return;
+        }
         const float lim = fminf(thr, __shfl(h.worst(), owner, 64));
         int cnt = 0;
         bool overflow = false;
         // Ring R: the box of R cells each way in x and y and R * zr in z,
-        // less the box of ring R - 1.
+        // less the box of ring R - 1 (at most 2 * points + 64 cells here).
         const int W = 2 * R + 1, Z = R * g.zr, Zin = (R - 1) * g.zr, Wz = 2 * Z + 1;
         const int nbox = W * W * Wz;
         for (int i = lane; i < nbox; i += 64) {

@@ -70,6 +70,10 @@ enum {
 };
 
 #define GS_MAX_K 32u
+/* Largest bin lattice (cells), and the largest lattice gs_wall_set_split
+ * splits over (it lists every cell on the host, 40 bytes each). */
+#define GS_MAX_CELLS (1u << 28)
+#define GS_MAX_SPLIT_CELLS (1u << 24)
 
 /* Force / integrate semantics.
  * GS_MODE_OPENSTEER: the CPU OpenSteer formulas (src/SteerLibrary.cpp,
@@ -196,7 +200,8 @@ int gs_group_upload_obstacles(gs_group g, uint32_t n, const float* cx, const flo
 int gs_group_upload_walls(gs_group g, uint32_t n, const gs_walls* host);
 /* Copy the bin database of a group (after gs_update_knn_database): any output
  * may be NULL.  key[n] cell per agent; start[C+1] first sorted slot per cell;
- * perm[n] agent index per sorted slot (ascending within a cell);
+ * perm[n] agent index per sorted slot (ascending within a cell; a cell of
+ * more than 65,536 agents keeps the order in which its agents were counted);
  * sorted positions spx/spy/spz[n]. */
 int gs_group_download_bins(gs_group g, uint32_t* key, uint32_t* start, uint32_t* perm, float* spx, float* spy,
                            float* spz);
@@ -263,8 +268,9 @@ int gs_update_group(gs_group A, float elapsed_time, int mode);
  * lattice, in KNNBinDataV2's cell numbering ix + iz*X + iy*X*Z
  * (CUDA/KNNBinDataV2.cu:103-135).  Density: agents per cell.  Velocity:
  * the mean of forward * speed over each cell's agents (summed in agent
- * index order, divided by the count), (0, 0, 0) for an empty cell; the
- * current forward and speed of the agents binned there. */
+ * index order -- for a cell of more than 65,536 agents in the order its
+ * agents were counted --, divided by the count), (0, 0, 0) for an empty
+ * cell; the current forward and speed of the agents binned there. */
 int gs_group_cell_density(gs_group g, uint32_t* agents_per_cell /* host [C] */);
 int gs_group_cell_velocity(gs_group g, float* avg_velocity /* host [C][3] */);
 /* The reference's text files: "<prefix>.dat" (overwritten) with its header
@@ -500,7 +506,8 @@ int gs_wall_set_load_map(const char* path, gs_wall_set* out);
 int gs_wall_set_create(uint32_t n, const gs_walls* host, gs_wall_set* out);
 /* WallGroupData::SplitWalls (WallGroupData.cu:105-206) over the cells of
  * KNNBinDataV2::CreateCells (CUDA/KNNBinDataV2.cu:73-136): a lattice of
- * `cells` over a box of `world_size` centred at the origin. */
+ * `cells` over a box of `world_size` centred at the origin, at most
+ * GS_MAX_SPLIT_CELLS cells (GS_ERR_UNSUPPORTED beyond). */
 int gs_wall_set_split(gs_wall_set w, const float world_size[3], const uint32_t cells[3]);
 /* The same over any list of cells, tried in list order: cell i spans
  * [min_bounds[3i..3i+2], max_bounds[3i..3i+2]] (bin_cell::minBound / maxBound,

@@ -118,3 +118,16 @@ def test_ensemble_specs_and_struct():
     # Lattices follow density: more agents, more cells.
     assert scenarios.boids_cells_for(65536) > scenarios.boids_cells_for(1024)
     assert scenarios.choke_cells_for(65536) > scenarios.choke_cells_for(1024)
+
+
+def test_lattice_limits_match_header():
+    """The lattice limits the header defines are the ones the ctypes table
+    and the autotuner use."""
+    from gpusteer_amd import _capi, autotune
+
+    text = (ROOT / "include" / "gpusteer.h").read_text()
+    for name in ("GS_MAX_CELLS", "GS_MAX_SPLIT_CELLS"):
+        m = re.search(rf"#define {name} \(1u << (\d+)\)", text)
+        assert m, name
+        assert getattr(_capi, name) == 1 << int(m.group(1))
+    assert autotune.MAX_CELLS == _capi.GS_MAX_CELLS

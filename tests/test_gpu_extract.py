@@ -24,7 +24,8 @@ def _flock(n, seed, dims):
     return st
 
 
-@pytest.mark.parametrize("cells,dims", [((40, 1, 40), 2), ((12, 7, 9), 3)], ids=["2d", "3d"])
+@pytest.mark.parametrize("cells,dims", [((40, 1, 40), 2), ((12, 7, 9), 3), ((2, 1, 2), 2)],
+                         ids=["2d", "3d", "10000-per-cell"])
 def test_cell_density_and_velocity(gs, ctx, orc, cells, dims):
     """40,000 agents binned, then integrated once without a rebuild (the
     velocities are the current ones, the cells those of the build): density
