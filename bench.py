@@ -476,11 +476,40 @@ def plugin_bench(steps: int, warmup: int, n_kernel: int = 1 << 20, reps: int = 5
     return res
 
 
+def host_cpu() -> dict:
+    """The host's processor as the CPU baseline ran on it."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(st, p, seconds: float):
+    """The oracle's fp32 restatement of the path (stable counting-sort bins,
+    exact binned KNN, the same forces and integrate; OpenMP over agents) on
+    the host: a bounded sample of the 2^20-agent flock on every core the
+    process may use and on one core, and configs 0 (4,096-agent flock, 200
+    steps) and 1 (10,000-agent choke-point crowd, 1,000 steps) in full."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as orc
 
-    threads = min(os.cpu_count() or 1, 16)
+    from gpusteer_amd import scenarios
+
+    cpu = host_cpu()
+    threads = cpu["affinity_cpus"] or os.cpu_count() or 1
+    if cpu["omp_num_threads_env"]:
+        threads = min(threads, int(cpu["omp_num_threads_env"]))
     orc.set_threads(threads)
     s = st.copy()
     steps = 0
@@ -503,15 +532,33 @@ def cpu_baseline(st, p, seconds: float):
         if el1 >= seconds / 2 or steps1 >= 3:
             break
     orc.set_threads(threads)
+    # config 0: the 4,096-agent sphere flock, 200 steps
+    s0, p0 = scenarios.boids_sphere(n=4096)
+    t0 = time.perf_counter()
+    for _ in range(200):
+        orc.boids_step(s0, p0)
+    el0 = time.perf_counter() - t0
+    # config 1: the 10,000-agent choke point, 1,000 steps
+    sc = scenarios.choke()
+    groups = [g.copy() for g in sc.groups]
+    t0 = time.perf_counter()
+    for _ in range(1000):
+        orc.choke_step(groups, sc)
+    el1c = time.perf_counter() - t0
+    n1 = sum(g.n for g in sc.groups)
     return {
         "value": st.n * steps / el,
         "unit": "agent-steps/s",
         "cores": threads,
+        "host": cpu,
         "single_core_value": st.n * steps1 / el1,
         "single_core_steps": steps1,
         "kind": "port",
         "sample": f"{steps} full steps of the same {st.n}-agent state (oracle restatement: stable counting-sort "
                   f"bins, exact binned KNN, SteerLibrary forces, SimpleVehicle integrate; OpenMP over agents)",
+        "config0_boids_sphere_4096x200": {"agent_steps_per_s": s0.n * 200 / el0, "seconds": el0,
+                                          "cores": threads},
+        "config1_choke_10000x1000": {"agent_steps_per_s": n1 * 1000 / el1c, "seconds": el1c, "cores": threads},
     }
 
 

@@ -296,6 +296,8 @@ _SIGS = {
     "gs_wrap_world": (C.c_int, [_h, C.c_int, _fp, _fp]),
     "gs_clear_bits": (C.c_int, [_h]),
     "gs_step_boids": (C.c_int, [_h, _h, _h, C.POINTER(GsBoidsParams), C.c_uint32]),
+    # A0, A1, walls, obstacles, bins, knn[8], scenario (crowd fields), steps
+    "gs_step_choke": (C.c_int, [_h, _h, _h, _h, _h, C.POINTER(_h), C.POINTER(GsScenario), C.c_uint32]),
     "gs_step_boids_timed": (C.c_int, [_h, _h, _h, C.POINTER(GsBoidsParams), C.c_uint32, C.POINTER(GsBoidsTiming)]),
     "gs_ensemble_create": (C.c_int, [_h, C.c_uint32, C.POINTER(GsScenario), C.c_uint32, C.POINTER(_h)]),
     "gs_ensemble_destroy": (C.c_int, [_h]),

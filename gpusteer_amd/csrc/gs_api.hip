@@ -1194,6 +1194,48 @@ static void boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const
     if (ev) GS_HIP(hipEventRecord(ev[3], st));
 }
 
+int gs_step_choke(gs_group A0, gs_group A1, gs_group walls, gs_group obstacles, gs_bins bins, gs_knn const* knn,
+                  const gs_scenario* s, uint32_t steps) {
+    return guarded([&] {
+        if (!knn || !s) fail(GS_ERR_INVALID_ARGUMENT, "knn or scenario is NULL");
+        check_group(A0, "group 0", GS_KIND_AGENTS);
+        check_group(A1, "group 1", GS_KIND_AGENTS);
+        check_group(walls, "walls", GS_KIND_WALLS);
+        check_group(obstacles, "obstacles", GS_KIND_OBSTACLES);
+        check(bins, "bins");
+        for (int j = 0; j < 8; ++j) check(knn[j], "knn");
+        if (s->mode != GS_MODE_OPENSTEER && s->mode != GS_MODE_GPUREF) fail(GS_ERR_INVALID_ARGUMENT, "unknown mode");
+        // Each call below is the C entry point itself, so the sequence is
+        // the one a plugin issues; a failing call ends the step with its status.
+        auto ok = [](int st) {
+            if (st != GS_OK) throw Error(st, gs_last_error());
+        };
+        gs_group A[2] = {A0, A1};
+        const gs_flock_term& t = s->choke_separation;
+        const float cap = s->other_max_distance > 0.f ? s->other_max_distance : INFINITY;
+        for (uint32_t step = 0; step < steps; ++step) {
+            ok(gs_update_knn_database(A[0], bins));
+            ok(gs_update_knn_database(A[1], bins));
+            for (int g = 0; g < 2; ++g) {
+                gs_knn const* k = knn + 4 * g;
+                ok(gs_clear_bits(A[g]));
+                ok(gs_find_knn(A[g], k[0], bins, A[g], 1, GS_KNN_DEFAULT));
+                ok(gs_find_knn_within(A[g], k[1], bins, A[1 - g], 1, GS_KNN_DEFAULT, cap));
+                ok(gs_find_knn(A[g], k[2], bins, walls, 1, GS_KNN_DEFAULT));
+                ok(gs_find_knn(A[g], k[3], bins, obstacles, 1, GS_KNN_DEFAULT));
+                ok(gs_steer_avoid_walls(A[g], k[2], walls, s->walls_min_time, s->walls_weight, 0));
+                ok(gs_steer_avoid_obstacles(A[g], obstacles, k[3], s->obstacles_min_time, s->obstacles_weight, 0));
+                ok(gs_steer_separation(A[g], k[0], A[g], t.min_d, t.max_d, t.cos_max, t.weight, t.mask, s->mode));
+                ok(gs_steer_separation(A[g], k[1], A[1 - g], t.min_d, t.max_d, t.cos_max, t.weight, t.mask,
+                                       s->mode));
+                ok(gs_steer_seek(A[g], s->goal[g], s->seek_weight, s->seek_mask, s->mode));
+            }
+            ok(gs_update_group(A[0], s->dt, s->mode));
+            ok(gs_update_group(A[1], s->dt, s->mode));
+        }
+    });
+}
+
 int gs_step_boids(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* p, uint32_t steps) {
     return guarded([&] {
         check_boids_args(A, knn, bins, p);

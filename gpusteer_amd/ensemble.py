@@ -155,10 +155,19 @@ def choke_step_calls(G, knns, W, O, bins, sc) -> None:
         api.updateGroup(grp, sc.dt, sc.mode)
 
 
+def step_choke(G, knns, W, O, bins, sc, steps: int = 1) -> None:
+    """choke_step_calls `steps` times, issued by the library
+    (gs_step_choke): one call instead of about thirty per step."""
+    hs = (C.c_void_p * 8)(*[k.h.value if isinstance(k.h, C.c_void_p) else k.h for ks in knns for k in ks])
+    s = scenario_struct(("choke", sc))
+    check(capi.lib.gs_step_choke(G[0].h, G[1].h, W.h, O.h, bins.h, hs, C.byref(s), steps), "gs_step_choke")
+
+
 class Sequential:
     """The same scenarios, each stepped alone through the single-instance
-    calls (gs_step_boids for a flock, the choke call sequence for a crowd),
-    one scenario after another."""
+    calls (gs_step_boids for a flock, gs_step_choke -- the choke call
+    sequence issued by the library -- for a crowd), one scenario after
+    another."""
 
     def __init__(self, ctx: api.Context, specs, k: int = 7):
         self.specs = specs
@@ -189,8 +198,7 @@ class Sequential:
                 api.stepBoids(A, knn, bins, p, steps)
             else:
                 _, G, knns, W, O, bins, sc = it
-                for _ in range(steps):
-                    choke_step_calls(G, knns, W, O, bins, sc)
+                step_choke(G, knns, W, O, bins, sc, steps)
 
     def download(self, scenario: int, group: int = 0) -> AgentState:
         it = self.items[scenario]

@@ -366,6 +366,23 @@ typedef struct {
 int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* params, uint32_t steps,
                         gs_boids_timing* timing);
 
+/* ---- fused choke-point crowd step -------------------------------------------
+ * `steps` times the per-frame sequence of plugins/ChokePointGroup.cpp:683-728
+ * on two agent groups A[0], A[1] with a wall group and an obstacle group,
+ * all binned over `bins` (walls and obstacles once, beforehand):
+ * gs_update_knn_database on both groups; then per group g = 0, 1:
+ * gs_clear_bits, gs_find_knn of A[g] with itself into knn[4g], of A[g] in
+ * A[1-g] within s->other_max_distance into knn[4g+1] (no cap when <= 0),
+ * of A[g] in the walls into knn[4g+2] and in the obstacles into knn[4g+3],
+ * gs_steer_avoid_walls, gs_steer_avoid_obstacles, gs_steer_separation
+ * (s->choke_separation) from A[g] and from A[1-g], gs_steer_seek of
+ * s->goal[g]; then gs_update_group on both (s->dt, s->mode).  Same results
+ * as those calls, issued from the library instead of the caller.  Reads
+ * s's crowd fields only.  No single reference counterpart (a plugin's frame). */
+typedef struct gs_scenario_s gs_scenario;
+int gs_step_choke(gs_group A0, gs_group A1, gs_group walls, gs_group obstacles, gs_bins bins, gs_knn const* knn,
+                  const gs_scenario* s, uint32_t steps);
+
 /* ---- batched ensemble -------------------------------------------------------------
  * Many independent scenarios of uneven size stepped together: every phase of a
  * step is one launch over all scenarios and the steps are replayed from a
@@ -384,7 +401,7 @@ int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_par
  * No counterpart in the reference, which runs one scenario per process. */
 typedef struct gs_ensemble_s* gs_ensemble;
 enum { GS_SCENARIO_BOIDS = 0, GS_SCENARIO_CHOKE = 1 };
-typedef struct {
+struct gs_scenario_s {
     int kind;                          /* GS_SCENARIO_BOIDS / GS_SCENARIO_CHOKE */
     uint32_t n[2];                     /* agents of group 0 (and group 1 for a crowd) */
     float origin[3], size[3];          /* this scenario's bin lattice */
@@ -403,7 +420,7 @@ typedef struct {
     float walls_min_time, walls_weight, obstacles_min_time, obstacles_weight;
     uint32_t k_walls, k_obstacles;     /* 1..8 */
     float other_max_distance;          /* KNN to the other group: gs_find_knn_within cap (<= 0: none) */
-} gs_scenario;
+};
 /* k: neighbours per agent (1..8), the same for every scenario. */
 int gs_ensemble_create(gs_context ctx, uint32_t n_scenarios, const gs_scenario* scenarios, uint32_t k,
                        gs_ensemble* out);

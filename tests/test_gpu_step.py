@@ -213,6 +213,39 @@ def test_choke_pipeline_bit_exact(gs, ctx, orc):
         h.close()
 
 
+def test_fused_choke_step_equals_oracle(gs, ctx, orc):
+    """gs_step_choke (the choke-point call sequence issued by the library)
+    on the 10,000-agent crowd: 60 steps in one call, then 40 in steps of 1,
+    equal to the oracle's same sequence bit for bit (KNN lists and state)."""
+    from gpusteer_amd.ensemble import step_choke
+
+    sc = scenarios.choke()
+    ref = [g.copy() for g in sc.groups]
+    bins = gs.KNNBinData(ctx, sc.grid_cells, sc.grid_size, origin=sc.grid_origin)
+    G = [gs.AgentGroup(ctx, g) for g in sc.groups]
+    W = gs.WallGroup(ctx, sc.walls)
+    O = gs.ObstacleGroup(ctx, *sc.obstacles)
+    gs.updateKNNDatabase(W, bins)
+    gs.updateKNNDatabase(O, bins)
+    knns = [[gs.KNNData(ctx, g.n, k) for k in (sc.k, sc.k, sc.knw, sc.kno)] for g in sc.groups]
+    print("fused choke step: 60 steps in one call", flush=True)
+    step_choke(G, knns, W, O, bins, sc, 60)
+    for _ in range(60):
+        orc.choke_step(ref, sc)
+    for gi in range(2):
+        assert_state_equal(G[gi].download(), ref[gi], what=f"after 60 steps group {gi}: ")
+    print("fused choke step: 40 single steps", flush=True)
+    for _ in range(40):
+        step_choke(G, knns, W, O, bins, sc, 1)
+        r = orc.choke_step(ref, sc)
+    for gi in range(2):
+        for j in range(4):
+            np.testing.assert_array_equal(knns[gi][j].download()[0], r[gi][j], err_msg=f"group {gi} list {j}")
+        assert_state_equal(G[gi].download(), ref[gi], what=f"after 100 steps group {gi}: ")
+    for h in G + [W, O, bins] + [k for ks in knns for k in ks]:
+        h.close()
+
+
 def _split_choke(n_total=4000):
     """The choke point with its walls split at the borders of its own bin
     lattice (WallGroup::SplitWalls over KNNBinData::hvCells()), the split
