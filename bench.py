@@ -147,7 +147,7 @@ def achievable_copy_gbs() -> float:
     return device_copy_gbs(1024, 10)
 
 
-def ensemble_bench(n_scenarios: int, steps: int, warmup: int) -> dict:
+def ensemble_bench(n_scenarios: int, steps: int, warmup: int, cell_factor: float = 0.0) -> dict:
     """Batched-ensemble speedup: total agent-steps/s of the mixed ensemble
     (Boids flocks and choke-point crowds, agent counts log-uniform over
     1,024-65,536) stepped together -- one launch per phase over all
@@ -161,10 +161,14 @@ def ensemble_bench(n_scenarios: int, steps: int, warmup: int) -> dict:
     from gpusteer_amd import autotune, scenarios
 
     with gs.Context(0) as ctx:
-        # Cell-size factor of the per-scenario lattices, tuned on a 32-scenario sample.
-        factor, ftable = autotune.tune_ensemble_cells(
-            ctx, lambda f: scenarios.ensemble_mix(n_scenarios=min(32, n_scenarios), cell_factor=f))
-        print(f"bench: ensemble autotune {ftable} -> cell factor {factor}", file=sys.stderr, flush=True)
+        # Cell-size factor of the per-scenario lattices, tuned on the ensemble
+        # itself unless given.
+        if cell_factor > 0:
+            factor, ftable = cell_factor, []
+        else:
+            factor, ftable = autotune.tune_ensemble_cells(
+                ctx, lambda f: scenarios.ensemble_mix(n_scenarios=n_scenarios, cell_factor=f))
+            print(f"bench: ensemble autotune {ftable} -> cell factor {factor}", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         specs = scenarios.ensemble_mix(n_scenarios=n_scenarios, cell_factor=factor)
         print(f"bench: ensemble of {n_scenarios} scenarios generated in {time.perf_counter() - t0:.1f} s",
@@ -576,6 +580,8 @@ def main() -> int:
     ap.add_argument("--ensemble-scenarios", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-cell-factor", type=float, default=0.0,
+                    help="cell-size factor of the ensemble lattices; 0 = chosen by the autotuner")
     ap.add_argument("--calib-scenarios", type=int, default=64)
     ap.add_argument("--calib-agents", type=int, default=16384)
     ap.add_argument("--calib-steps", type=int, default=100)
@@ -597,7 +603,8 @@ def main() -> int:
         return 2
 
     if args.only == "ensemble":
-        print(json.dumps(ensemble_bench(args.ensemble_scenarios, args.ensemble_steps, 3)), flush=True)
+        print(json.dumps(ensemble_bench(args.ensemble_scenarios, args.ensemble_steps, 3, args.ensemble_cell_factor)),
+              flush=True)
         return 0
     if args.only == "plugin":
         print(json.dumps(plugin_bench(args.plugin_steps, 5)), flush=True)
@@ -734,7 +741,7 @@ def main() -> int:
         },
     }
     if not args.no_ensemble:
-        out["ensemble"] = ensemble_bench(args.ensemble_scenarios, args.ensemble_steps, 3)
+        out["ensemble"] = ensemble_bench(args.ensemble_scenarios, args.ensemble_steps, 3, args.ensemble_cell_factor)
     if not args.no_calib:
         out["adjoint"] = calib_bench(args.calib_scenarios, args.calib_agents, args.calib_steps, 10)
     if not args.no_plugin:

@@ -13,6 +13,7 @@ from .api import (  # noqa: F401
     MODE_OPENSTEER,
     AgentGroup,
     Context,
+    Frame,
     KNNBinData,
     KNNData,
     ObstacleGroup,

@@ -44,9 +44,96 @@ class Context:
         check(capi.lib.gs_context_stream(self.h, C.byref(s)), "gs_context_stream")
         return s.value or 0
 
+    def record(self, calls) -> "Frame":
+        """Records the library calls that calls() issues on this context into
+        a frame (gs_frame_begin / gs_frame_end): nothing runs, and every
+        handle they touch is left as it was.  Frame.launch(times) replays
+        them."""
+        check(capi.lib.gs_frame_begin(self.h), "gs_frame_begin")
+        try:
+            calls()
+        except BaseException:
+            f = C.c_void_p()
+            if capi.lib.gs_frame_end(self.h, C.byref(f)) == 0:
+                capi.lib.gs_frame_destroy(f)
+            raise
+        f = C.c_void_p()
+        check(capi.lib.gs_frame_end(self.h, C.byref(f)), "gs_frame_end")
+        return Frame(f)
+
     def close(self) -> None:
         if self.h:
             check(capi.lib.gs_context_destroy(self.h), "gs_context_destroy")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class FrameCache:
+    """Runs a step function (library calls only) from recorded frames.  The
+    first per_frame steps run unrecorded, so every device buffer has its
+    size; then per_frame steps are recorded as one frame and replayed.  A
+    step leaves double buffers swapped, so after an odd number of unrecorded
+    steps the handles are in the other state: a frame is recorded for each
+    state met (GS_ERR_FRAME from a replay means "not this state")."""
+
+    def __init__(self, ctx: "Context", step, per_frame: int = 2):
+        self.ctx, self.step, self.per_frame = ctx, step, int(per_frame)
+        self.frames: list = []
+        self.warm = 0
+
+    def _launch(self, times: int) -> bool:
+        for f in self.frames:
+            try:
+                f.launch(1)
+            except capi.GpuSteerError as e:
+                if e.status != capi.GS_ERR_FRAME:
+                    raise
+                continue
+            if times > 1:
+                f.launch(times - 1)
+            return True
+        return False
+
+    def run(self, steps: int) -> None:
+        n = int(steps)
+        while n > 0 and self.warm < self.per_frame:
+            self.step()
+            self.warm += 1
+            n -= 1
+        if n >= self.per_frame:
+            k = n // self.per_frame
+            if not self._launch(k):
+                self.frames.append(self.ctx.record(lambda: [self.step() for _ in range(self.per_frame)]))
+                if not self._launch(k):
+                    raise RuntimeError("a frame recorded in the current state does not replay in it")
+            n -= k * self.per_frame
+        for _ in range(n):
+            self.step()
+
+    def close(self) -> None:
+        for f in self.frames:
+            f.close()
+        self.frames = []
+
+
+class Frame:
+    """A recorded call sequence (gs_frame_*): launch(times) replays it on the
+    context's stream, with the same results as issuing the calls."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def launch(self, times: int = 1) -> None:
+        check(capi.lib.gs_frame_launch(self.h, int(times)), "gs_frame_launch")
+
+    def close(self) -> None:
+        if self.h:
+            check(capi.lib.gs_frame_destroy(self.h), "gs_frame_destroy")
             self.h = None
 
     def __enter__(self):

@@ -61,7 +61,7 @@ def tune_flock(ctx: api.Context, state, params, k: int = 7,
     return (best[0], best[1]), table
 
 
-def tune_ensemble_cells(ctx: api.Context, make_specs, factors: Sequence[float] = (0.8, 1.0, 1.25, 1.6),
+def tune_ensemble_cells(ctx: api.Context, make_specs, factors: Sequence[float] = (0.8, 1.0, 1.25, 1.6, 2.0, 2.5, 3.2),
                         steps: int = 6):
     """Times an ensemble built by make_specs(cell_factor) for each factor
     (the first step after the upload and the graph capture excluded);

@@ -7,6 +7,9 @@
 #include <atomic>
 #include <mutex>
 #include <string>
+#include <memory>
+#include <type_traits>
+#include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
@@ -16,7 +19,13 @@
 namespace gs {
 
 static thread_local std::string g_last_error;
-void set_last_error(const std::string& m) { g_last_error = m; }
+static void frame_note_failure(const std::string& msg);
+// Every failing entry point records its message here (and, while a frame is
+// recorded, marks the recording as failed).
+void set_last_error(const std::string& m) {
+    g_last_error = m;
+    frame_note_failure(m);
+}
 void fail(int status, const std::string& msg) { throw Error(status, msg); }
 void hip_check(hipError_t e, const char* what) {
     if (e == hipSuccess) return;
@@ -24,10 +33,23 @@ void hip_check(hipError_t e, const char* what) {
     fail(GS_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Frame recording on this thread (gs_frame_begin .. gs_frame_end), or NULL.
+struct Recording;
+static thread_local Recording* t_rec = nullptr;
+// Fails a call that cannot be recorded into a frame while one is recorded.
+static void no_frame(const char* what) {
+    if (t_rec)
+        fail(GS_ERR_FRAME, std::string(what) + " cannot be recorded into a frame (it copies, allocates, frees or "
+                                               "synchronizes); issue it before gs_frame_begin or after gs_frame_end");
+}
+
 // Allocation-failure injection for tests: after `g_fail_after` more
 // successful allocations, every allocation fails (negative = never).
 static std::atomic<int64_t> g_fail_after{-1};
 void* dev_alloc(size_t bytes) {
+    if (t_rec)
+        fail(GS_ERR_FRAME, "a device buffer would grow while recording a frame; issue the calls once unrecorded "
+                           "first so every buffer has its size");
     int64_t left = g_fail_after.load();
     if (left >= 0) {
         if (left == 0) fail(GS_ERR_OUT_OF_MEMORY, "device allocation failed (injected)");
@@ -43,6 +65,7 @@ void* dev_alloc(size_t bytes) {
     return p;
 }
 void dev_free(void* p) {
+    if (p && t_rec) fail(GS_ERR_FRAME, "device memory would be freed while recording a frame");
     if (p) (void)hipFree(p);
 }
 
@@ -69,8 +92,17 @@ static bool reg_has(const void* h) {
 static std::atomic<uint64_t> g_epoch{1};
 static uint64_t next_epoch() { return g_epoch.fetch_add(1); }
 
+// While a frame is recorded, the first check of a group or KNN container
+// snapshots its host-side state (see the frame section below).
+static void frame_touch(gs_group_s* g);
+static void frame_touch(gs_knn_s* k);
+static void frame_touch(gs_bins_s* b);
+static void frame_touch(gs_path_s* p);
+static void frame_touch(const void*) {}
+
 template <class H> static H* check(H* h, const char* what) {
     if (!reg_has(h)) fail(GS_ERR_INVALID_HANDLE, std::string("invalid ") + what + " handle");
+    if (t_rec) frame_touch(h);
     return h;
 }
 static gs_group_s* check_group(gs_group g, const char* what, int kind = -1) {
@@ -205,7 +237,183 @@ template <class T> static std::vector<T> d2h_vec(const T* d, size_t n, hipStream
     return v;
 }
 
+// ---- frames: the host-side state of recorded handles ------------------------
+// A recorded call sequence depends on, and changes, the host-side bookkeeping
+// of the groups and KNN containers it touches: sizes, current double buffers,
+// bin database flags, which result a KNN container holds.  The recording
+// snapshots each touched handle at its first check and again at the end;
+// gs_frame_end puts the begin state back (nothing has run), and every replay
+// requires the begin state and then installs the end state.  Epochs are
+// unique names, so they are compared and installed up to renaming: a replay
+// requires the current epochs to relate to each other as the recorded ones
+// did (the same ones equal, different ones different, none where there was
+// none), maps each recorded epoch to its current counterpart, and gives the
+// epochs the recorded calls created fresh values.
+struct Snap {
+    std::vector<uint64_t> exact;   // compared bit for bit
+    std::vector<uint64_t> epochs;  // compared up to renaming
+};
+
+// Visits the bookkeeping of a handle in a fixed order: reading appends the
+// values to a Snap, writing installs them from one.  Device buffers (pointer
+// and capacity) are compared, never written.
+struct SnapIO {
+    Snap* s;
+    bool write;
+    size_t ix = 0, ie = 0;
+    template <class T> void val(T& v) {
+        static_assert(sizeof(T) <= 8 && std::is_trivially_copyable<T>::value, "one word per value");
+        if (write) {
+            const uint64_t w = s->exact[ix++];
+            memcpy(&v, &w, sizeof(T));
+        } else {
+            uint64_t w = 0;
+            memcpy(&w, &v, sizeof(T));
+            s->exact.push_back(w);
+        }
+    }
+    void ep(uint64_t& v) {
+        if (write)
+            v = s->epochs[ie++];
+        else
+            s->epochs.push_back(v);
+    }
+    template <class T> void buf(gs::DevBuf<T>& b) {
+        if (write) {
+            ix += 2;
+        } else {
+            s->exact.push_back((uint64_t)(uintptr_t)b.p);
+            s->exact.push_back((uint64_t)b.cap);
+        }
+    }
+    void grid(GridDev& g) {
+        static_assert(sizeof(GridDev) % 4 == 0, "GridDev is 4-byte fields");
+        uint32_t w[sizeof(GridDev) / 4];
+        memcpy(w, &g, sizeof w);
+        for (auto& x : w) val(x);
+        if (write) memcpy(&g, w, sizeof w);
+    }
+};
+
+static void visit(gs_group_s* g, SnapIO& io) {
+    io.val(g->kind);
+    io.val(g->n);
+    io.val(g->live);
+    io.val(g->inv);
+    for (auto& v : g->inv_val) io.val(v);
+    io.val(g->bits_zero);
+    io.ep(g->membership_epoch);
+    io.ep(g->order_epoch);
+    io.ep(g->prev_order_epoch);
+    for (auto& c : g->fcur) io.val(c);
+    io.val(g->bcur);
+    io.val(g->icur);
+    io.val(g->binned);
+    io.val(g->snap_valid);
+    io.grid(g->bin_grid);
+    io.val(g->bin_cells);
+    for (auto& f : g->fb)
+        for (auto& b : f) io.buf(b);
+    for (auto& b : g->bitsb) io.buf(b);
+    for (auto& b : g->idb) io.buf(b);
+    gs::DevBuf<uint32_t>* u[] = {&g->slot_of_id, &g->new_to_old, &g->old_to_new, &g->key,   &g->rank,
+                                 &g->perm_tmp,   &g->id_tmp,     &g->start,      &g->count, &g->flag};
+    for (auto* b : u) io.buf(*b);
+    io.buf(g->spos);
+    io.buf(g->sfwd);
+}
+
+static void visit(gs_knn_s* k, SnapIO& io) {
+    io.val(k->n);
+    io.val(k->k);
+    io.val(k->cur);
+    io.val(k->valid);
+    io.val(k->a);
+    io.val(k->b);
+    io.ep(k->a_members);
+    io.ep(k->b_members);
+    io.ep(k->a_order);
+    io.ep(k->b_order);
+    for (auto& b : k->idxb) io.buf(b);
+    for (auto& b : k->distb) io.buf(b);
+    io.buf(k->part_key);
+    io.buf(k->part_slot);
+}
+
+// A handle the recorded calls used: a group or KNN container with its
+// bookkeeping, or a bin lattice or path, which only has to stay alive.
+struct Touched {
+    gs_group_s* g = nullptr;
+    gs_knn_s* k = nullptr;
+    const void* keep = nullptr;
+    Snap begin, end;
+    const void* handle() const { return g ? (const void*)g : k ? (const void*)k : keep; }
+};
+
+template <class F> static void visit_touched(Touched& t, F&& f) {
+    if (t.g)
+        f(t.g);
+    else if (t.k)
+        f(t.k);
+}
+
+static Snap snap_of(Touched& t) {
+    Snap s;
+    SnapIO io{&s, false};
+    visit_touched(t, [&](auto* h) { visit(h, io); });
+    return s;
+}
+
+static void install(Touched& t, Snap& s) {
+    SnapIO io{&s, true};
+    visit_touched(t, [&](auto* h) { visit(h, io); });
+}
+
+struct Recording {
+    gs_context_s* ctx = nullptr;
+    std::vector<Touched> touched;
+    bool failed = false;
+    std::string first_error;
+};
+
+static void frame_note_failure(const std::string& msg) {
+    if (t_rec && !t_rec->failed) {
+        t_rec->failed = true;
+        t_rec->first_error = msg;
+    }
+}
+
+template <class H> static void frame_touch_h(H* h, H* Touched::*member) {
+    if (h->ctx != t_rec->ctx) fail(GS_ERR_FRAME, "a handle of another context was used while recording a frame");
+    for (auto& t : t_rec->touched)
+        if (t.*member == h) return;
+    Touched t;
+    t.*member = h;
+    t.begin = snap_of(t);
+    t_rec->touched.push_back(std::move(t));
+}
+static void frame_touch(gs_group_s* g) { frame_touch_h(g, &Touched::g); }
+static void frame_touch(gs_knn_s* k) { frame_touch_h(k, &Touched::k); }
+template <class H> static void frame_keep(H* h) {
+    if (h->ctx != t_rec->ctx) fail(GS_ERR_FRAME, "a handle of another context was used while recording a frame");
+    for (auto& t : t_rec->touched)
+        if (t.keep == h) return;
+    Touched t;
+    t.keep = h;
+    t_rec->touched.push_back(std::move(t));
+}
+static void frame_touch(gs_bins_s* b) { frame_keep(b); }
+static void frame_touch(gs_path_s* p) { frame_keep(p); }
+
 }  // namespace gs
+
+struct gs_frame_s {
+    gs_context ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<gs::Touched> touched;
+    uint32_t magic = 0x67736672u;
+};
 
 using namespace gs;
 
@@ -230,6 +438,7 @@ const char* gs_status_string(int status) {
         case GS_ERR_OUT_OF_MEMORY: return "out of device memory";
         case GS_ERR_DEVICE: return "device error";
         case GS_ERR_NONFINITE: return "non-finite state";
+        case GS_ERR_FRAME: return "not recordable, or frame replayed in another state";
         default: return "unknown status";
     }
 }
@@ -247,6 +456,7 @@ int gs_debug_fail_alloc_after(int64_t n) {
 // ---- context -----------------------------------------------------------------
 int gs_context_create(int device, void* hip_stream, gs_context* out) {
     return guarded([&] {
+        no_frame("gs_context_create");
         if (!out) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
         *out = nullptr;
         int count = 0;
@@ -278,6 +488,7 @@ int gs_context_create(int device, void* hip_stream, gs_context* out) {
 
 int gs_context_destroy(gs_context ctx) {
     return guarded([&] {
+        no_frame("gs_context_destroy");
         check(ctx, "context");
         use_device(ctx);
         (void)hipStreamSynchronize(ctx->stream);
@@ -289,6 +500,7 @@ int gs_context_destroy(gs_context ctx) {
 
 int gs_context_synchronize(gs_context ctx) {
     return guarded([&] {
+        no_frame("gs_context_synchronize");
         check(ctx, "context");
         use_device(ctx);
         GS_HIP(hipStreamSynchronize(ctx->stream));
@@ -306,6 +518,7 @@ int gs_context_stream(gs_context ctx, void** s) {
 // ---- bins --------------------------------------------------------------------
 int gs_bins_create(gs_context ctx, const float origin[3], const float size[3], const uint32_t cells[3], gs_bins* out) {
     return guarded([&] {
+        no_frame("gs_bins_create");
         check(ctx, "context");
         if (!origin || !size || !cells || !out) fail(GS_ERR_INVALID_ARGUMENT, "NULL argument");
         auto* b = new gs_bins_s;
@@ -330,6 +543,7 @@ int gs_bins_create_centered(gs_context ctx, const uint32_t world_cells[3], const
 
 int gs_bins_destroy(gs_bins b) {
     return guarded([&] {
+        no_frame("gs_bins_destroy");
         check(b, "bins");
         reg_del(b);
         delete b;
@@ -360,6 +574,7 @@ int gs_bins_geometry(gs_bins b, float origin[3], float size[3], uint32_t cells[3
 // ---- groups ------------------------------------------------------------------
 int gs_group_create(gs_context ctx, int kind, gs_group* out) {
     return guarded([&] {
+        no_frame("gs_group_create");
         check(ctx, "context");
         if (!out) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
         if (kind != GS_KIND_AGENTS && kind != GS_KIND_OBSTACLES && kind != GS_KIND_WALLS)
@@ -388,6 +603,7 @@ static void release_group(gs_group_s* g) {
 
 int gs_group_destroy(gs_group g) {
     return guarded([&] {
+        no_frame("gs_group_destroy");
         check_group(g, "group");
         use_device(g->ctx);
         (void)hipStreamSynchronize(g->ctx->stream);
@@ -433,6 +649,7 @@ static void begin_members(gs_group_s* g, uint32_t n) {
 
 int gs_group_upload_agents(gs_group g, uint32_t n, const gs_agents* host) {
     return guarded([&] {
+        no_frame("gs_group_upload_agents");
         check_group(g, "group", GS_KIND_AGENTS);
         if (n > 0 && !host) fail(GS_ERR_INVALID_ARGUMENT, "host arrays are NULL");
         const float* const* hf = host ? (const float* const*)host : nullptr;
@@ -467,6 +684,7 @@ int gs_group_upload_agents(gs_group g, uint32_t n, const gs_agents* host) {
 
 int gs_group_download_agents(gs_group g, gs_agents* host) {
     return guarded([&] {
+        no_frame("gs_group_download_agents");
         check_group(g, "group", GS_KIND_AGENTS);
         if (!host) fail(GS_ERR_INVALID_ARGUMENT, "host arrays are NULL");
         use_device(g->ctx);
@@ -519,6 +737,7 @@ int gs_group_device_order(gs_group g, uint32_t** id_of_slot, uint32_t** slot_of_
 int gs_group_upload_obstacles(gs_group g, uint32_t n, const float* cx, const float* cy, const float* cz,
                               const float* radius) {
     return guarded([&] {
+        no_frame("gs_group_upload_obstacles");
         check_group(g, "group", GS_KIND_OBSTACLES);
         if (n > 0 && (!cx || !cy || !cz || !radius)) fail(GS_ERR_INVALID_ARGUMENT, "obstacle arrays are NULL");
         use_device(g->ctx);
@@ -555,6 +774,7 @@ static void upload_walls(gs_group_s* g, uint32_t n, const float* const* hw, cons
 
 int gs_group_upload_walls(gs_group g, uint32_t n, const gs_walls* host) {
     return guarded([&] {
+        no_frame("gs_group_upload_walls");
         check_group(g, "group", GS_KIND_WALLS);
         if (n > 0 && !host) fail(GS_ERR_INVALID_ARGUMENT, "wall arrays are NULL");
         const float* const* hw = host ? (const float* const*)host : nullptr;
@@ -567,6 +787,7 @@ int gs_group_upload_walls(gs_group g, uint32_t n, const gs_walls* host) {
 int gs_group_upload_walls_mid(gs_group g, uint32_t n, const gs_walls* host, const float* mid_x, const float* mid_y,
                               const float* mid_z) {
     return guarded([&] {
+        no_frame("gs_group_upload_walls_mid");
         check_group(g, "group", GS_KIND_WALLS);
         if (n > 0 && (!host || !mid_x || !mid_y || !mid_z)) fail(GS_ERR_INVALID_ARGUMENT, "wall arrays are NULL");
         const float* const* hw = host ? (const float* const*)host : nullptr;
@@ -579,6 +800,7 @@ int gs_group_upload_walls_mid(gs_group g, uint32_t n, const gs_walls* host, cons
 
 int gs_group_upload_wall_set(gs_group g, gs_wall_set w) {
     return guarded([&] {
+        no_frame("gs_group_upload_wall_set");
         check_group(g, "group", GS_KIND_WALLS);
         uint32_t n = 0;
         if (gs_wall_set_size(w, &n) != GS_OK) fail(GS_ERR_INVALID_HANDLE, "wall set handle is invalid");
@@ -619,6 +841,7 @@ static void extract_to_host(gs_group_s* g, size_t count, void* host, bool veloci
 
 int gs_group_cell_density(gs_group g, uint32_t* agents_per_cell) {
     return guarded([&] {
+        no_frame("gs_group_cell_density");
         check_group(g, "group");
         if (!agents_per_cell) fail(GS_ERR_INVALID_ARGUMENT, "output is NULL");
         extract_to_host(g, g->bin_cells, agents_per_cell, false);
@@ -627,6 +850,7 @@ int gs_group_cell_density(gs_group g, uint32_t* agents_per_cell) {
 
 int gs_group_cell_velocity(gs_group g, float* avg_velocity) {
     return guarded([&] {
+        no_frame("gs_group_cell_velocity");
         check_group(g, "group", GS_KIND_AGENTS);
         if (!avg_velocity) fail(GS_ERR_INVALID_ARGUMENT, "output is NULL");
         extract_to_host(g, (size_t)g->bin_cells * 3, avg_velocity, true);
@@ -636,6 +860,7 @@ int gs_group_cell_velocity(gs_group g, float* avg_velocity) {
 int gs_write_cell_density(gs_group g, const char* prefix, const uint32_t* selected_cells, uint32_t n_selected,
                           const char* selected_prefix, int all_rows) {
     return guarded([&] {
+        no_frame("gs_write_cell_density");
         check_group(g, "group");
         if (!prefix) fail(GS_ERR_INVALID_ARGUMENT, "prefix is NULL");
         if (n_selected && (!selected_cells || !selected_prefix))
@@ -649,6 +874,7 @@ int gs_write_cell_density(gs_group g, const char* prefix, const uint32_t* select
 int gs_write_cell_velocity(gs_group g, const char* prefix, const uint32_t* selected_cells, uint32_t n_selected,
                            const char* selected_prefix, int all_rows) {
     return guarded([&] {
+        no_frame("gs_write_cell_velocity");
         check_group(g, "group", GS_KIND_AGENTS);
         if (!prefix) fail(GS_ERR_INVALID_ARGUMENT, "prefix is NULL");
         if (n_selected && (!selected_cells || !selected_prefix))
@@ -662,6 +888,7 @@ int gs_write_cell_velocity(gs_group g, const char* prefix, const uint32_t* selec
 int gs_group_download_bins(gs_group g, uint32_t* key, uint32_t* start, uint32_t* perm, float* spx, float* spy,
                            float* spz) {
     return guarded([&] {
+        no_frame("gs_group_download_bins");
         check_group(g, "group");
         if (!g->binned || !g->snap_valid) fail(GS_ERR_NOT_BINNED, "group has no bin database");
         use_device(g->ctx);
@@ -699,6 +926,7 @@ int gs_group_bin_geometry(gs_group g, float origin[3], float size[3], uint32_t c
 int gs_group_check_finite(gs_group g, uint32_t* first_bad) {
     uint32_t bad = UINT32_MAX;
     int s = guarded([&] {
+        no_frame("gs_group_check_finite");
         check_group(g, "group", GS_KIND_AGENTS);
         use_device(g->ctx);
         uint32_t slot = UINT32_MAX;
@@ -720,6 +948,7 @@ int gs_group_check_finite(gs_group g, uint32_t* first_bad) {
 // ---- KNN container --------------------------------------------------------------
 int gs_knn_create(gs_context ctx, uint32_t n, uint32_t k, gs_knn* out) {
     return guarded([&] {
+        no_frame("gs_knn_create");
         check(ctx, "context");
         if (!out) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
         if (k == 0) fail(GS_ERR_INVALID_ARGUMENT, "k must be at least 1");
@@ -749,6 +978,7 @@ int gs_knn_create(gs_context ctx, uint32_t n, uint32_t k, gs_knn* out) {
 
 int gs_knn_destroy(gs_knn knn) {
     return guarded([&] {
+        no_frame("gs_knn_destroy");
         check(knn, "knn");
         use_device(knn->ctx);
         (void)hipStreamSynchronize(knn->ctx->stream);
@@ -773,6 +1003,7 @@ int gs_knn_shape(gs_knn knn, uint32_t* n, uint32_t* k) {
 
 int gs_knn_download(gs_knn knn, uint32_t* idx, float* dist) {
     return guarded([&] {
+        no_frame("gs_knn_download");
         check(knn, "knn");
         if (!knn->valid) fail(GS_ERR_INVALID_ARGUMENT, "KNN container holds no result");
         if (!reg_has(knn->a) || !reg_has(knn->b)) fail(GS_ERR_INVALID_HANDLE, "the KNN result's groups are gone");
@@ -1054,6 +1285,7 @@ int gs_steer_avoid_neighbors(gs_group A, gs_knn knn, gs_group B, float min_time_
 // running total.
 int gs_path_create(gs_context ctx, uint32_t n_points, const float* points, float radius, int cyclic, gs_path* out) {
     return guarded([&] {
+        no_frame("gs_path_create");
         check(ctx, "context");
         if (!out) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
         if (n_points < 2 || !points) fail(GS_ERR_INVALID_ARGUMENT, "a path needs at least 2 points");
@@ -1125,6 +1357,7 @@ int gs_path_get(gs_path p, float* points, float* directions, float* lengths) {
 
 int gs_path_destroy(gs_path p) {
     return guarded([&] {
+        no_frame("gs_path_destroy");
         check(p, "path");
         use_device(p->ctx);
         (void)hipStreamSynchronize(p->ctx->stream);
@@ -1248,6 +1481,7 @@ int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_par
                         gs_boids_timing* timing) {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     int status = guarded([&] {
+        no_frame("gs_step_boids_timed");
         check_boids_args(A, knn, bins, p);
         if (!timing) fail(GS_ERR_INVALID_ARGUMENT, "timing is NULL");
         if (steps == 0) fail(GS_ERR_INVALID_ARGUMENT, "steps must be at least 1");
@@ -1277,6 +1511,149 @@ int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_par
 }
 
 
+// ---- recorded frames -----------------------------------------------------------
+
+int gs_frame_begin(gs_context ctx) {
+    return guarded([&] {
+        check(ctx, "context");
+        if (t_rec) fail(GS_ERR_FRAME, "a frame is already being recorded on this thread");
+        use_device(ctx);
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        GS_HIP(hipStreamIsCapturing(ctx->stream, &cs));
+        if (cs != hipStreamCaptureStatusNone) fail(GS_ERR_FRAME, "the context's stream is already being captured");
+        // Thread-local mode: HIP itself also refuses unsafe calls of this
+        // thread (allocation, synchronization) while recording.
+        GS_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        auto* r = new Recording;
+        r->ctx = ctx;
+        t_rec = r;
+    });
+}
+
+int gs_frame_end(gs_context ctx, gs_frame* out) {
+    return guarded([&] {
+        check(ctx, "context");
+        if (!t_rec || t_rec->ctx != ctx) fail(GS_ERR_FRAME, "no frame is being recorded on this context by this thread");
+        if (out) *out = nullptr;
+        std::unique_ptr<Recording> r(t_rec);
+        t_rec = nullptr;
+        use_device(ctx);
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        // Nothing has run: every touched handle goes back to its begin state.
+        for (auto& t : r->touched) {
+            t.end = snap_of(t);
+            install(t, t.begin);
+        }
+        if (ce != hipSuccess) {
+            (void)hipGetLastError();
+            if (graph) (void)hipGraphDestroy(graph);
+            fail(GS_ERR_FRAME, std::string("the recording was invalidated: ") + hipGetErrorString(ce) +
+                                   (r->failed ? " (first failing call: " + r->first_error + ")" : ""));
+        }
+        if (r->failed) {
+            if (graph) (void)hipGraphDestroy(graph);
+            fail(GS_ERR_FRAME, "a call failed while recording the frame: " + r->first_error);
+        }
+        if (!out) {
+            (void)hipGraphDestroy(graph);
+            fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
+        }
+        auto* f = new gs_frame_s;
+        f->ctx = ctx;
+        f->graph = graph;
+        size_t nodes = 0;
+        hipError_t e = hipGraphGetNodes(graph, nullptr, &nodes);
+        if (e == hipSuccess && nodes > 0) e = hipGraphInstantiate(&f->exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            delete f;
+            hip_check(e, "hipGraphInstantiate");
+        }
+        f->touched = std::move(r->touched);
+        reg_add(f);
+        *out = f;
+    });
+}
+
+// A replay's precondition and renaming: the current state of every touched
+// handle against its begin snapshot.
+static bool frame_matches(gs_frame_s* f, std::vector<Snap>& cur, std::unordered_map<uint64_t, uint64_t>& ren,
+                          std::string& why) {
+    std::unordered_map<uint64_t, uint64_t> back;
+    cur.clear();
+    for (auto& t : f->touched) {
+        if (!reg_has(t.handle())) {
+            why = "a handle the frame touches was destroyed";
+            return false;
+        }
+        cur.push_back(snap_of(t));
+        const Snap& b = t.begin;
+        const Snap& c = cur.back();
+        if (b.exact != c.exact) {
+            why = std::string("a ") + (t.g ? "group" : t.k ? "KNN container" : "handle") +
+                  " the frame touches has other sizes, buffers or current double buffers than when it was recorded";
+            return false;
+        }
+        for (size_t i = 0; i < b.epochs.size(); ++i) {
+            const uint64_t x = b.epochs[i], y = c.epochs[i];
+            if ((x == 0) != (y == 0)) {
+                why = "a bin database or KNN result the frame relies on is missing (or one it lacked is present)";
+                return false;
+            }
+            if (x == 0) continue;
+            auto fx = ren.emplace(x, y);
+            auto by = back.emplace(y, x);
+            if (fx.first->second != y || by.first->second != x) {
+                why = "the KNN results the frame relies on are for other orders of their groups than when recorded";
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
+int gs_frame_launch(gs_frame f, uint32_t times) {
+    return guarded([&] {
+        check(f, "frame");
+        no_frame("gs_frame_launch");
+        check(f->ctx, "frame's context");
+        use_device(f->ctx);
+        std::vector<Snap> cur;
+        for (uint32_t rep = 0; rep < times; ++rep) {
+            std::unordered_map<uint64_t, uint64_t> ren;
+            std::string why;
+            if (!frame_matches(f, cur, ren, why))
+                fail(GS_ERR_FRAME, "replay " + std::to_string(rep + 1) + " of " + std::to_string(times) + ": " + why);
+            if (f->exec) GS_HIP(hipGraphLaunch(f->exec, f->ctx->stream));
+            // Install the end state, recorded epochs renamed, new ones fresh.
+            for (auto& t : f->touched) {
+                Snap e = t.end;
+                for (auto& x : e.epochs) {
+                    if (x == 0) continue;
+                    auto it = ren.find(x);
+                    if (it == ren.end()) it = ren.emplace(x, next_epoch()).first;
+                    x = it->second;
+                }
+                install(t, e);
+            }
+        }
+    });
+}
+
+int gs_frame_destroy(gs_frame f) {
+    return guarded([&] {
+        if (!f) return;
+        check(f, "frame");
+        no_frame("gs_frame_destroy");
+        if (reg_has(f->ctx)) use_device(f->ctx);
+        if (f->exec) (void)hipGraphExecDestroy(f->exec);
+        if (f->graph) (void)hipGraphDestroy(f->graph);
+        reg_del(f);
+        delete f;
+    });
+}
+
 // ---- batched ensemble -----------------------------------------------------------
 
 static gs_ensemble_s* check_ens(gs_ensemble e) { return check(e, "ensemble"); }
@@ -1294,6 +1671,7 @@ static void drop_graphs(gs_ensemble_s* e) { release_ensemble(e); }
 
 int gs_ensemble_create(gs_context ctx, uint32_t ns, const gs_scenario* sc, uint32_t k, gs_ensemble* out) {
     return guarded([&] {
+        no_frame("gs_ensemble_create");
         check(ctx, "context");
         if (!out) fail(GS_ERR_INVALID_ARGUMENT, "out is NULL");
         *out = nullptr;
@@ -1341,6 +1719,7 @@ int gs_ensemble_create(gs_context ctx, uint32_t ns, const gs_scenario* sc, uint3
 
 int gs_ensemble_destroy(gs_ensemble e) {
     return guarded([&] {
+        no_frame("gs_ensemble_destroy");
         check_ens(e);
         reg_del(e);
         use_device(e->ctx);
@@ -1369,6 +1748,7 @@ static const EnsPop& ens_pop(gs_ensemble_s* e, uint32_t scenario, uint32_t group
 
 int gs_ensemble_upload_agents(gs_ensemble e, uint32_t scenario, uint32_t group, const gs_agents* host) {
     return guarded([&] {
+        no_frame("gs_ensemble_upload_agents");
         check_ens(e);
         const EnsPop& P = ens_pop(e, scenario, group);
         if (P.n && !host) fail(GS_ERR_INVALID_ARGUMENT, "host arrays are NULL");
@@ -1394,6 +1774,7 @@ int gs_ensemble_upload_agents(gs_ensemble e, uint32_t scenario, uint32_t group, 
 
 int gs_ensemble_download_agents(gs_ensemble e, uint32_t scenario, uint32_t group, gs_agents* host) {
     return guarded([&] {
+        no_frame("gs_ensemble_download_agents");
         check_ens(e);
         const EnsPop& P = ens_pop(e, scenario, group);
         if (!host) fail(GS_ERR_INVALID_ARGUMENT, "host arrays are NULL");
@@ -1445,12 +1826,16 @@ static void ens_upload_walls(gs_ensemble e, uint32_t scenario, uint32_t n, const
 }
 
 int gs_ensemble_upload_walls(gs_ensemble e, uint32_t scenario, uint32_t n, const gs_walls* host) {
-    return guarded([&] { ens_upload_walls(e, scenario, n, host, nullptr); });
+    return guarded([&] {
+        no_frame("gs_ensemble_upload_walls");
+        ens_upload_walls(e, scenario, n, host, nullptr);
+    });
 }
 
 int gs_ensemble_upload_walls_mid(gs_ensemble e, uint32_t scenario, uint32_t n, const gs_walls* host,
                                  const float* mid_x, const float* mid_y, const float* mid_z) {
     return guarded([&] {
+        no_frame("gs_ensemble_upload_walls_mid");
         const float* mid[3] = {mid_x, mid_y, mid_z};
         ens_upload_walls(e, scenario, n, host, mid);
     });
@@ -1459,6 +1844,7 @@ int gs_ensemble_upload_walls_mid(gs_ensemble e, uint32_t scenario, uint32_t n, c
 int gs_ensemble_upload_obstacles(gs_ensemble e, uint32_t scenario, uint32_t n, const float* cx, const float* cy,
                                  const float* cz, const float* radius) {
     return guarded([&] {
+        no_frame("gs_ensemble_upload_obstacles");
         check_ens(e);
         if (scenario >= e->scen.size()) fail(GS_ERR_INVALID_ARGUMENT, "scenario index out of range");
         if (n && (!cx || !cy || !cz || !radius)) fail(GS_ERR_INVALID_ARGUMENT, "obstacle arrays are NULL");
@@ -1479,6 +1865,7 @@ int gs_ensemble_upload_obstacles(gs_ensemble e, uint32_t scenario, uint32_t n, c
 
 int gs_ensemble_download_knn(gs_ensemble e, uint32_t scenario, uint32_t group, uint32_t* idx) {
     return guarded([&] {
+        no_frame("gs_ensemble_download_knn");
         check_ens(e);
         const EnsPop& P = ens_pop(e, scenario, group);
         if (!idx) fail(GS_ERR_INVALID_ARGUMENT, "idx is NULL");
@@ -1501,6 +1888,7 @@ int gs_ensemble_download_knn(gs_ensemble e, uint32_t scenario, uint32_t group, u
 
 int gs_ensemble_step(gs_ensemble e, uint32_t steps, uint32_t flags) {
     return guarded([&] {
+        no_frame("gs_ensemble_step");
         check_ens(e);
         if (flags & ~(uint32_t)GS_ENSEMBLE_EAGER) fail(GS_ERR_INVALID_ARGUMENT, "unknown flags");
         use_device(e->ctx);
@@ -1542,6 +1930,7 @@ int gs_ensemble_step(gs_ensemble e, uint32_t steps, uint32_t flags) {
 
 int gs_calib_create(gs_ensemble e, const gs_calib_config* cfg, gs_calib* out) {
     return guarded([&] {
+        no_frame("gs_calib_create");
         check_ens(e);
         if (!cfg || !out) fail(GS_ERR_INVALID_ARGUMENT, "config / out is NULL");
         *out = nullptr;
@@ -1567,6 +1956,7 @@ int gs_calib_create(gs_ensemble e, const gs_calib_config* cfg, gs_calib* out) {
 
 int gs_calib_destroy(gs_calib c) {
     return guarded([&] {
+        no_frame("gs_calib_destroy");
         check(c, "calibration");
         reg_del(c);
         use_device(c->ens->ctx);
@@ -1578,6 +1968,7 @@ int gs_calib_destroy(gs_calib c) {
 
 int gs_calib_forward(gs_calib c, const gs_calib_params* params, double* loss) {
     return guarded([&] {
+        no_frame("gs_calib_forward");
         check(c, "calibration");
         check_ens(c->ens);
         if (!params || !loss) fail(GS_ERR_INVALID_ARGUMENT, "params / loss is NULL");
@@ -1594,6 +1985,7 @@ int gs_calib_forward(gs_calib c, const gs_calib_params* params, double* loss) {
 
 int gs_calib_backward(gs_calib c, gs_calib_params* grad) {
     return guarded([&] {
+        no_frame("gs_calib_backward");
         check(c, "calibration");
         check_ens(c->ens);
         if (!grad) fail(GS_ERR_INVALID_ARGUMENT, "grad is NULL");
@@ -1605,6 +1997,7 @@ int gs_calib_backward(gs_calib c, gs_calib_params* grad) {
 
 int gs_calib_download_state(gs_calib c, uint32_t scenario, uint32_t group, gs_agents* host) {
     return guarded([&] {
+        no_frame("gs_calib_download_state");
         check(c, "calibration");
         if (!host) fail(GS_ERR_INVALID_ARGUMENT, "host is NULL");
         if (!c->forward_done) fail(GS_ERR_INVALID_ARGUMENT, "call gs_calib_forward first");
@@ -1616,6 +2009,7 @@ int gs_calib_download_state(gs_calib c, uint32_t scenario, uint32_t group, gs_ag
 
 int gs_calib_timing(gs_calib c, float* forward_ms, float* backward_ms) {
     return guarded([&] {
+        no_frame("gs_calib_timing");
         check(c, "calibration");
         if (forward_ms) *forward_ms = c->fwd_ms;
         if (backward_ms) *backward_ms = c->bwd_ms;

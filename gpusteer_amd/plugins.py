@@ -27,6 +27,15 @@ class ChokePointNew:
         self.knn_self = api.KNNData(ctx, sc.state.n, p.knn)
         self.knn_walls = api.KNNData(ctx, sc.state.n, p.knw)
         self.path = api.PolylinePathway(ctx, sc.path_points, sc.path_radius, sc.path_cyclic)
+        self.ctx = ctx
+        self._frames = {}
+
+    def recorded(self, variant: str, steps: int) -> None:
+        """`steps` steps of step / step_with_wall_update replayed from
+        recorded frames (the same calls; see api.FrameCache)."""
+        if variant not in self._frames:
+            self._frames[variant] = api.FrameCache(self.ctx, lambda: getattr(self, variant)(1))
+        self._frames[variant].run(steps)
 
     def step(self, steps: int = 1) -> None:
         sc, p, A = self.sc, self.sc.params, self.group
@@ -78,6 +87,8 @@ class ChokePointNew:
             api.antiPenetrationAgents(A, self.knn_self, A, (1 << 8) | BIT_ANTI_PENETRATION_WALL)
 
     def close(self) -> None:
+        for f in self._frames.values():
+            f.close()
         for h in (self.group, self.walls, self.bins, self.knn_self, self.knn_walls, self.path):
             h.close()
 
@@ -101,6 +112,11 @@ class MultiGroup:
         self.groups = [api.AgentGroup(ctx, st) for st in sc.states]
         self.knn = [[api.KNNData(ctx, st.n, p.knn) for _ in sc.states] for st in sc.states]
         self.paths = [api.PolylinePathway(ctx, pts, r, c) if len(pts) >= 2 else None for pts, r, c in sc.paths]
+        self.frames = api.FrameCache(ctx, lambda: self.step(1))
+
+    def recorded(self, steps: int) -> None:
+        """`steps` steps replayed from recorded frames (api.FrameCache)."""
+        self.frames.run(steps)
 
     def step(self, steps: int = 1) -> None:
         sc, p, G, m = self.sc, self.sc.params, self.groups, self.mode
@@ -134,6 +150,7 @@ class MultiGroup:
                     api.antiPenetrationAgents(A, K[j], B, p.mask_anti_penetration_agents)
 
     def close(self) -> None:
+        self.frames.close()
         for A in self.groups:
             A.close()
         for row in self.knn:

@@ -66,7 +66,8 @@ enum {
     GS_ERR_NOT_BINNED = 5,       /* KNN on a group without a current bin database */
     GS_ERR_OUT_OF_MEMORY = 6,    /* device allocation failed */
     GS_ERR_DEVICE = 7,           /* HIP runtime error (message in gs_last_error) */
-    GS_ERR_NONFINITE = 8         /* gs_group_check_finite found NaN/Inf */
+    GS_ERR_NONFINITE = 8,        /* gs_group_check_finite found NaN/Inf */
+    GS_ERR_FRAME = 9             /* call not recordable into a frame, or a frame replayed on handles in another state */
 };
 
 #define GS_MAX_K 32u
@@ -382,6 +383,45 @@ int gs_step_boids_timed(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_par
 typedef struct gs_scenario_s gs_scenario;
 int gs_step_choke(gs_group A0, gs_group A1, gs_group walls, gs_group obstacles, gs_bins bins, gs_knn const* knn,
                   const gs_scenario* s, uint32_t steps);
+
+/* ---- recorded frames --------------------------------------------------------------
+ * A frame is a call sequence on one context recorded once into a HIP graph
+ * and replayed: a plugin's per-frame calls (or a fused step) without the host
+ * cost of issuing every launch again.  No counterpart in the reference, whose
+ * plugins launch every kernel eagerly each frame (plugins/ChokePointNew.cpp,
+ * plugins/ChokePointGroup.cpp:683-728).
+ *
+ * gs_frame_begin(ctx): the calls that follow on ctx, from this thread, are
+ * recorded instead of run.  Recordable are the calls that only launch
+ * kernels on existing handles: gs_update_knn_database, the gs_find_knn*
+ * searches (gs_find_knn_noalloc included), the gs_steer_* calls,
+ * gs_update_group*, gs_anti_penetration_agents, gs_wrap_world,
+ * gs_clear_bits, gs_step_boids and gs_step_choke.  A call that would copy
+ * between host and device, allocate or free device memory, or synchronize
+ * (creating, destroying, uploading, downloading, extraction to the host,
+ * gs_group_check_finite, gs_step_boids_timed, the ensemble and calibration
+ * calls, a search whose buffers must first grow) fails with GS_ERR_FRAME
+ * while recording; run the sequence once unrecorded first, so every buffer
+ * has its size.  Work the caller queues on the context's stream meanwhile is
+ * recorded too.
+ * gs_frame_end(ctx, &f): ends the recording.  The recorded calls have not
+ * run: every handle they touched is back in its state at gs_frame_begin.
+ * Fails with GS_ERR_FRAME (and records nothing) if a call failed while
+ * recording.
+ * gs_frame_launch(f, times): runs the recorded calls `times` times on the
+ * context's stream, with the same results as issuing them.  Each replay
+ * requires every handle the calls touched to be in the state it had at
+ * gs_frame_begin (the same sizes, device buffers and current double buffers,
+ * and KNN containers holding results for the same groups and orders, up to
+ * later reorders); otherwise it fails with GS_ERR_FRAME before launching.  A
+ * Boids or crowd step swaps double buffers, so a frame of an even number of
+ * steps replays any number of times.
+ * gs_frame_destroy(f): releases the graph (handles are not touched). */
+typedef struct gs_frame_s* gs_frame;
+int gs_frame_begin(gs_context ctx);
+int gs_frame_end(gs_context ctx, gs_frame* out);
+int gs_frame_launch(gs_frame f, uint32_t times);
+int gs_frame_destroy(gs_frame f);
 
 /* ---- batched ensemble -------------------------------------------------------------
  * Many independent scenarios of uneven size stepped together: every phase of a

@@ -47,8 +47,8 @@ def test_enum_constants_match_header():
 def test_status_strings():
     from gpusteer_amd import _capi
 
-    for s in range(9):
-        assert _capi.lib.gs_status_string(s)
+    for s in range(10):
+        assert _capi.lib.gs_status_string(s) != b"unknown status"
     assert _capi.lib.gs_version() == 100
 
 
@@ -74,6 +74,13 @@ def test_null_handles_rejected():
     bogus = C.c_void_p(0x1234)
     assert L.gs_context_synchronize(bogus) == c.GS_ERR_INVALID_HANDLE
     assert L.gs_step_boids(bogus, bogus, bogus, None, 1) == c.GS_ERR_INVALID_HANDLE
+    # Frames: bad handles, and no recording to end.
+    f = C.c_void_p()
+    assert L.gs_frame_begin(None) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_frame_end(None, C.byref(f)) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_frame_launch(bogus, 1) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_frame_destroy(bogus) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_frame_destroy(None) == c.GS_OK
 
 
 def test_context_without_gpu_reports_device_error():
