@@ -330,7 +330,7 @@ def plugin_bench(steps: int, warmup: int, n_kernel: int = 1 << 20, reps: int = 5
     from gpusteer_amd import formats, scenarios
     from gpusteer_amd.plugins import ChokePointNew, MultiGroup
 
-    files = ROOT / "tests" / "golden" / "scenario_files"
+    files = formats.DATA_DIR
     p = formats.load_params(files / "ChokePoint.params")
     ws = formats.WallSet.load_map(files / p.map_filename).split(p.dimensions, p.cells)
     walls, mid = ws.get()

@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass, field
+from pathlib import Path
 from typing import List, Optional, Tuple
 
 import numpy as np
@@ -14,6 +15,10 @@ import numpy as np
 from . import _capi as capi
 from ._capi import check
 from .state import WALL_FIELDS, WallState
+
+# The reference's scenario files (data/*.params, data/*.map), as its plugins
+# load them.
+DATA_DIR = Path(__file__).resolve().parent / "data"
 
 
 def _f(a: np.ndarray):

@@ -1,9 +1,8 @@
 """Scenario formats (host-side C-ABI, no GPU): wall maps, wall splitting at
 bin-cell borders and simulation parameter files, on the reference's own data
-files (tests/golden/scenario_files, copied from data/), against the oracle's
+files (gpusteer_amd/data, copied from data/), against the oracle's
 restatement (oracle/formats_ref.py), hand-computed cases and properties the
 splitting must have."""
-from pathlib import Path
 
 import numpy as np
 import pytest
@@ -13,7 +12,7 @@ from gpusteer_amd._capi import GpuSteerError
 
 import formats_ref as ref
 
-FILES = Path(__file__).resolve().parent / "golden" / "scenario_files"
+FILES = formats.DATA_DIR
 MAPS = sorted(FILES.glob("*.map"))
 PARAMS = sorted(FILES.glob("*.params"))
 

@@ -64,7 +64,7 @@ def test_cxx_boids_plugin_matches_oracle(orc, tmp_path, mode):
 
 
 CHOKE = gbuild.LIB_DIR / "choke_setup"
-FILES = gbuild.ROOT / "tests" / "golden" / "scenario_files"
+FILES = gbuild.ROOT / "gpusteer_amd" / "data"
 
 
 def test_cxx_choke_setup_matches_oracle(orc, tmp_path):

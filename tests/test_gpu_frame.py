@@ -5,7 +5,6 @@ multi-group plugin's frames and the fused crowd step against eager runs of
 the same calls); recording runs nothing and leaves every handle as it was;
 calls that copy, allocate or synchronize are refused while recording; a
 replay on handles in another state is refused before it launches."""
-from pathlib import Path
 
 import numpy as np
 import pytest
@@ -17,7 +16,7 @@ from helpers import assert_state_equal
 
 pytestmark = pytest.mark.gpu
 
-FILES = Path(__file__).resolve().parent / "golden" / "scenario_files"
+FILES = formats.DATA_DIR
 
 
 def _knn_equal(a, b, what):

@@ -14,7 +14,7 @@ from helpers import assert_state_equal
 
 pytestmark = pytest.mark.gpu
 
-FILES = Path(__file__).resolve().parent / "golden" / "scenario_files"
+FILES = formats.DATA_DIR
 
 
 @pytest.fixture(scope="module")
