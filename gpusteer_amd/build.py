@@ -98,6 +98,70 @@ def build_cxx_examples() -> None:
             raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr[-6000:]}")
 
 
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer build of the
+# sources without kernels (the C entry points and the scenario-file readers),
+# linked with the regular objects of the kernel sources; -Xarch_host puts
+# every sanitizer flag on the host pass only.
+HOST_SANITIZE_FLAGS = [
+    "--offload-host-only",
+    "-O1",
+    "-g",
+    "-std=c++17",
+    "-fPIC",
+    "-fno-omit-frame-pointer",
+    "-ffp-contract=off",
+    "-Xarch_host", "-fsanitize=address",
+    "-Xarch_host", "-fsanitize=undefined",
+    "-Xarch_host", "-fno-sanitize-recover=undefined",
+    "-Wno-unused-command-line-argument",
+]
+HOST_SANITIZED_SOURCES = ("gs_api.hip", "gs_formats.hip")
+
+
+def build_host_sanitized(out_dir: Path) -> Path:
+    """Builds out_dir/libgpusteer_hostsan.so -- the kernel-free sources
+    (HOST_SANITIZED_SOURCES) under the sanitizers, the kernel sources from
+    their regular objects -- and the exercise program
+    tests/cxx/host_sanitize.cpp against it; returns the program's path."""
+    out_dir = Path(out_dir)
+    out_dir.mkdir(parents=True, exist_ok=True)
+    OBJ_DIR.mkdir(exist_ok=True)
+    objs = []
+    for src in sorted(CSRC.glob("*.hip")):
+        if src.name in HOST_SANITIZED_SOURCES:
+            if "__global__" in src.read_text():
+                raise RuntimeError(f"{src.name} defines kernels; it cannot be built host-only")
+            obj = out_dir / (src.stem + ".o")
+            cmd = [hipcc(), *HOST_SANITIZE_FLAGS, f"-I{ROOT / 'include'}", "-x", "hip", "-c", str(src), "-o",
+                   str(obj)]
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"hipcc (host sanitizers) failed on {src.name}:\n{r.stderr[-6000:]}")
+        else:
+            obj = OBJ_DIR / (src.stem + ".o")
+            if not obj.exists() or obj.stat().st_mtime < src.stat().st_mtime:
+                obj = _compile(src)
+        objs.append(obj)
+    lib = out_dir / "libgpusteer_hostsan.so"
+    link = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC",
+            "-Xarch_host", "-fsanitize=address",
+            "-Xarch_host", "-fsanitize=undefined",
+            "-o", str(lib), *map(str, objs)]
+    r = subprocess.run(link, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link (host sanitizers) failed:\n{r.stderr[-6000:]}")
+    exe = out_dir / "host_sanitize"
+    cmd = [hipcc(), "--offload-host-only", "-O1", "-g", "-std=c++17", "-fno-omit-frame-pointer",
+           "-Xarch_host", "-fsanitize=address",
+           "-Xarch_host", "-fsanitize=undefined",
+           f"-I{ROOT / 'include'}", "-x", "c++", str(ROOT / "tests" / "cxx" / "host_sanitize.cpp"),
+           "-x", "none", str(lib), f"-Wl,-rpath,{out_dir}", "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on host_sanitize.cpp:\n{r.stderr[-6000:]}")
+    return exe
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)

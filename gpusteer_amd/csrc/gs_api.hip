@@ -76,15 +76,15 @@ static std::unordered_set<const void*>& registry() {
     static std::unordered_set<const void*> r;
     return r;
 }
-static void reg_add(const void* h) {
+void reg_add(const void* h) {
     std::lock_guard<std::mutex> l(g_reg_mu);
     registry().insert(h);
 }
-static void reg_del(const void* h) {
+void reg_del(const void* h) {
     std::lock_guard<std::mutex> l(g_reg_mu);
     registry().erase(h);
 }
-static bool reg_has(const void* h) {
+bool reg_has(const void* h) {
     if (!h) return false;
     std::lock_guard<std::mutex> l(g_reg_mu);
     return registry().count(h) != 0;

@@ -42,7 +42,6 @@
 
 struct gs_wall_set_s {
     std::vector<float> v[12];  // start xyz, end xyz, normal xyz, mid xyz
-    uint32_t magic = 0x67737773u;
     size_t size() const { return v[0].size(); }
 };
 
@@ -54,7 +53,7 @@ struct W {
 };
 
 gs_wall_set_s* check_set(gs_wall_set w) {
-    if (!w || w->magic != 0x67737773u) fail(GS_ERR_INVALID_HANDLE, "wall set handle is invalid");
+    if (!reg_has(w)) fail(GS_ERR_INVALID_HANDLE, "wall set handle is invalid");
     return w;
 }
 
@@ -416,6 +415,7 @@ int gs_wall_set_load_map(const char* path, gs_wall_set* out) {
             delete w;
             throw;
         }
+        reg_add(w);
         *out = w;
     });
 }
@@ -432,6 +432,7 @@ int gs_wall_set_create(uint32_t n, const gs_walls* host, gs_wall_set* out) {
             w->v[9 + a].resize(n);
             for (uint32_t i = 0; i < n; ++i) w->v[9 + a][i] = 0.5f * (hw[a][i] + hw[3 + a][i]);
         }
+        reg_add(w);
         *out = w;
     });
 }
@@ -502,7 +503,7 @@ int gs_wall_set_destroy(gs_wall_set w) {
     return guarded([&] {
         if (!w) return;
         check_set(w);
-        w->magic = 0;
+        reg_del(w);
         delete w;
     });
 }

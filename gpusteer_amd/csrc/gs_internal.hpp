@@ -41,6 +41,12 @@ template <class F> int guarded(F&& f) {
     }
 }
 [[noreturn]] void fail(int status, const std::string& msg);
+// Live-handle registry: every handle the library hands out is registered
+// until destroyed, so a destroyed or foreign pointer is rejected without
+// being dereferenced.
+void reg_add(const void* h);
+void reg_del(const void* h);
+bool reg_has(const void* h);
 void hip_check(hipError_t e, const char* what);
 #define GS_HIP(call) ::gs::hip_check((call), #call)
 // Checks the launch of the preceding kernel.

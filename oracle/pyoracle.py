@@ -11,13 +11,16 @@ and on the GPU must agree (bit for bit for KNN and forces).
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "liboracle.so"
+# GS_ORACLE_LIB selects another build of the same source (the sanitizer
+# build, oracle/_san/liboracle_san.so).
+LIB_PATH = Path(os.environ["GS_ORACLE_LIB"]) if os.environ.get("GS_ORACLE_LIB") else HERE / "liboracle.so"
 REF_LQ = HERE / "_ref" / "liblq.so"
 
 AGENT_FIELDS = (
