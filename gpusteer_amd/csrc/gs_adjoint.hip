@@ -26,8 +26,12 @@
 // Checkpoints every `checkpoint_every` steps; the backward pass recomputes
 // each segment forward (KNN included) from its checkpoint, keeping the
 // segment's states and lists, then runs the adjoint kernel backwards through
-// it.  Neighbour contributions to the adjoint state and the parameter
-// gradient are accumulated with hardware float atomics (sum order varies).
+// it.  Every sum is taken in a fixed order, so the loss and the gradient are
+// identical, bit for bit, from run to run: a neighbour's contribution to an
+// agent's adjoint goes to its own slot of the step's reverse lists and the
+// backward of the step before adds an agent's slots in (query, slot) order;
+// loss and parameter gradient are summed per block (waves in order), and the
+// per-block sums are added in block order at the end.
 #include <algorithm>
 #include <vector>
 
@@ -419,11 +423,103 @@ __device__ __forceinline__ float pen_loss(const CalibDev& D, const EnsPop& P, ui
     return L * D.pen_w;
 }
 
+// Adds the block's sum of v (waves in order) to *part, which the block owns.
+__device__ inline void block_add(float v, double* __restrict__ part, double* lds) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = (double)v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) t += lds[w];
+        *part += t;
+    }
+    __syncthreads();
+}
+
+// out[c] = sum over rows r (in order, then a fixed tree) of part[r * cols + c];
+// one block per column.
+__global__ __launch_bounds__(kBlock) void c4_reduce_rows(const double* __restrict__ part, uint32_t rows,
+                                                         uint32_t cols, double* __restrict__ out) {
+    __shared__ double lds[kBlock];
+    const uint32_t c = blockIdx.x;
+    double t = 0.0;
+    for (uint32_t r = threadIdx.x; r < rows; r += kBlock) t += part[(size_t)r * cols + c];
+    lds[threadIdx.x] = t;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) lds[threadIdx.x] += lds[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = lds[0];
+}
+
+// Reverse lists of one step: for every agent b, the entries (query i, list
+// w, slot j) of the own-group (w = 0) and other-group (w = 1) lists that name
+// b, as the segment [rstart[b], rstart[b+1]) ordered by entry code.  The
+// backward writes entry e's contribution to slot slot_of[code(e)]; the
+// backward of the step before adds agent b's segment in order.
+__device__ inline uint32_t rev_code(uint32_t i, uint32_t w, uint32_t j, uint32_t k) {
+    return (i * 2u + w) * k + j;
+}
+
+__global__ __launch_bounds__(kBlock) void c4_rev_count(CalibDev D, const uint32_t* __restrict__ kself,
+                                                       const uint32_t* __restrict__ kother,
+                                                       uint32_t* __restrict__ cnt) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
+    const EnsPop& P = D.pops[D.block_pop[blk]];
+    if (i >= P.a0 + P.n) return;
+    for (int w = 0; w < 2; ++w) {
+        const uint32_t* lst = w == 0 ? kself : kother;
+        for (uint32_t j = 0; j < D.k; ++j) {
+            const uint32_t b = lst[(size_t)j * D.N + i];
+            if (b >= D.N) break;
+            atomicAdd(cnt + b, 1u);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void c4_rev_fill(CalibDev D, const uint32_t* __restrict__ kself,
+                                                      const uint32_t* __restrict__ kother,
+                                                      const uint32_t* __restrict__ rstart, uint32_t* __restrict__ cursor,
+                                                      uint32_t* __restrict__ rev) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
+    const EnsPop& P = D.pops[D.block_pop[blk]];
+    if (i >= P.a0 + P.n) return;
+    for (int w = 0; w < 2; ++w) {
+        const uint32_t* lst = w == 0 ? kself : kother;
+        for (uint32_t j = 0; j < D.k; ++j) {
+            const uint32_t b = lst[(size_t)j * D.N + i];
+            if (b >= D.N) break;
+            rev[rstart[b] + atomicAdd(cursor + b, 1u)] = rev_code(i, (uint32_t)w, j, D.k);
+        }
+    }
+}
+
+// slot_of[code] = the code's rank in its agent's segment (the segments hold
+// each code once); clears the agent's cursor for the next build.
+__global__ __launch_bounds__(kBlock) void c4_rev_rank(uint32_t N, const uint32_t* __restrict__ rstart,
+                                                      const uint32_t* __restrict__ rev, uint32_t* __restrict__ cursor,
+                                                      uint32_t* __restrict__ slot_of) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= N) return;
+    const uint32_t s0 = rstart[b], s1 = rstart[b + 1];
+    for (uint32_t s = s0; s < s1; ++s) {
+        const uint32_t c = rev[s];
+        uint32_t r = 0;
+        for (uint32_t t = s0; t < s1; ++t) r += rev[t] < c ? 1u : 0u;
+        slot_of[c] = s0 + r;
+    }
+    cursor[b] = 0u;
+}
+
 __global__ __launch_bounds__(kBlock) void c4_forward(CalibDev D, CState X, CState Y,
                                                      const uint32_t* __restrict__ kself,
                                                      const uint32_t* __restrict__ kother,
                                                      const uint32_t* __restrict__ kw, const uint32_t* __restrict__ ko,
-                                                     double* __restrict__ loss) {
+                                                     double* __restrict__ loss_part) {
+    __shared__ double lds[kBlock / 64];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = blk * blockDim.x + threadIdx.x;
     const EnsPop& P = D.pops[D.block_pop[blk]];
@@ -442,14 +538,13 @@ __global__ __launch_bounds__(kBlock) void c4_forward(CalibDev D, CState X, CStat
         Y.a[10][i] = upn.x; Y.a[11][i] = upn.y; Y.a[12][i] = upn.z;
         lp = pen_loss(D, P, i, X, kself, kother) * D.inv_S;
     }
-    // wave sum, one atomic per wave
-    for (int o = 32; o > 0; o >>= 1) lp += __shfl_xor(lp, o, 64);
-    if ((threadIdx.x & 63) == 0 && lp != 0.f) atomicAdd(loss, (double)lp);
+    if (loss_part) block_add(lp, loss_part + blockIdx.x, lds);
 }
 
 // Terminal term: loss += mean over agents of |p - goal|^2 (per scenario, / S);
 // writes the adjoint seed dL/dp_T and zeroes the other adjoint components.
-__global__ __launch_bounds__(kBlock) void c4_terminal(CalibDev D, CState X, CAdj A, double* __restrict__ loss) {
+__global__ __launch_bounds__(kBlock) void c4_terminal(CalibDev D, CState X, CAdj A, double* __restrict__ loss_part) {
+    __shared__ double lds[kBlock / 64];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = blk * blockDim.x + threadIdx.x;
     const EnsPop& P = D.pops[D.block_pop[blk]];
@@ -468,23 +563,29 @@ __global__ __launch_bounds__(kBlock) void c4_terminal(CalibDev D, CState X, CAdj
         A.a[1][i] = gp.y;
         A.a[2][i] = gp.z;
     }
-    for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
-    if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, (double)l);
+    if (loss_part) block_add(l, loss_part + blockIdx.x, lds);
 }
 
 // Parameter-gradient slots (gs_calib_params order).
 enum { TW_SEP = 0, TW_ALI = 2, TW_COH = 4, TW_OBS = 6, TW_WALL = 8, TR = 10, TV = 12, kTheta = 14 };
 
 // Adjoint of one step.  In: the step-t state X, the lists of step t, the
-// adjoint of the step-(t+1) state as own + acc.  Out: the adjoint of the
-// step-t state as own_out (each agent's own terms, plain stores) + acc_out
-// (neighbour terms, one atomic add per neighbour slot and component), and
-// the parameter gradient.
+// adjoint of the step-(t+1) state as own_in plus the neighbour terms
+// contrib_in summed over each agent's segment of rstart_in (the reverse lists
+// of step t+1; none when rstart_in is NULL).  Out: the adjoint of the step-t
+// state as own_out (each agent's own terms) and contrib_out (one slot per
+// neighbour entry, slot_out[code]: p and f terms as two float4), and the
+// block's parameter-gradient sums added to its row of grad_part.
 __global__ __launch_bounds__(kBlock) void c4_backward(CalibDev D, CState X, const uint32_t* __restrict__ kself,
                                                       const uint32_t* __restrict__ kother,
                                                       const uint32_t* __restrict__ kw,
-                                                      const uint32_t* __restrict__ ko, CAdj own_in, CAdj acc_in,
-                                                      CAdj own_out, CAdj acc_out, double* __restrict__ theta_grad) {
+                                                      const uint32_t* __restrict__ ko, CAdj own_in,
+                                                      const uint32_t* __restrict__ rstart_in,
+                                                      const float4* __restrict__ contrib_in, CAdj own_out,
+                                                      const uint32_t* __restrict__ slot_out,
+                                                      float4* __restrict__ contrib_out,
+                                                      double* __restrict__ grad_part) {
+    __shared__ double lds[kBlock / 64];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = blk * blockDim.x + threadIdx.x;
     const EnsPop& P = D.pops[D.block_pop[blk]];
@@ -499,11 +600,21 @@ __global__ __launch_bounds__(kBlock) void c4_backward(CalibDev D, CState X, cons
         f3 p1, f1, G, pn, fn, sdn, upn;
         float sn;
         c4_fwd_agent(D, P, S, i, X, kself, kother, kw, ko, in, p1, f1, G, pn, fn, sn, sdn, upn, rec);
-        auto lam = [&](int c) { return own_in.a[c][i] + acc_in.a[c][i]; };
-        f3 pnb = mk3(lam(0), lam(1), lam(2));
-        f3 fnb = mk3(lam(3), lam(4), lam(5));
-        const float snb = lam(6);
-        const f3 sdnb = mk3(lam(7), lam(8), lam(9));
+        // neighbour terms of the step after, in segment order
+        f3 accp = mk3(0.f, 0.f, 0.f), accf = accp;
+        if (rstart_in) {
+            const uint32_t s1 = rstart_in[i + 1];
+            for (uint32_t s = rstart_in[i]; s < s1; ++s) {
+                const float4 u = contrib_in[2 * (size_t)s], v = contrib_in[2 * (size_t)s + 1];
+                accp = accp + mk3(u.x, u.y, u.z);
+                accf = accf + mk3(u.w, v.x, v.y);
+            }
+        }
+        auto own = [&](int c) { return own_in.a[c][i]; };
+        f3 pnb = mk3(own(0), own(1), own(2)) + accp;
+        f3 fnb = mk3(own(3), own(4), own(5)) + accf;
+        const float snb = own(6);
+        const f3 sdnb = mk3(own(7), own(8), own(9));
         f3 pb = mk3(0.f, 0.f, 0.f), fb = pb, sdb = pb;
         float sb = 0.f;
         f3 p1b = mk3(0.f, 0.f, 0.f), f1b = p1b;
@@ -628,16 +739,9 @@ __global__ __launch_bounds__(kBlock) void c4_backward(CalibDev D, CState X, cons
                     cp = cp + gpj;
                     pb = pb - gpj;
                 }
-                if (!zero3(cp)) {
-                    unsafeAtomicAdd(acc_out.a[0] + b, cp.x);
-                    unsafeAtomicAdd(acc_out.a[1] + b, cp.y);
-                    unsafeAtomicAdd(acc_out.a[2] + b, cp.z);
-                }
-                if (!zero3(cf)) {
-                    unsafeAtomicAdd(acc_out.a[3] + b, cf.x);
-                    unsafeAtomicAdd(acc_out.a[4] + b, cf.y);
-                    unsafeAtomicAdd(acc_out.a[5] + b, cf.z);
-                }
+                const size_t so = slot_out[rev_code(i, (uint32_t)w, j, D.k)];
+                contrib_out[2 * so] = make_float4(cp.x, cp.y, cp.z, cf.x);
+                contrib_out[2 * so + 1] = make_float4(cf.y, cf.z, 0.f, 0.f);
             }
         }
         // ---- obstacles
@@ -735,13 +839,9 @@ __global__ __launch_bounds__(kBlock) void c4_backward(CalibDev D, CState X, cons
         own_out.a[8][i] = sdb.y;
         own_out.a[9][i] = sdb.z;
     }
-    // parameter gradient: wave sums, one atomic per slot per wave
+    // parameter gradient: the block's sums, added to its own row
 #pragma unroll
-    for (int c = 0; c < kTheta; ++c) {
-        float v = tg[c];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        if ((threadIdx.x & 63) == 0 && v != 0.f) atomicAdd(theta_grad + c, (double)v);
-    }
+    for (int c = 0; c < kTheta; ++c) block_add(tg[c], grad_part + (size_t)blockIdx.x * kTheta + c, lds);
 }
 
 }  // namespace gs
@@ -846,10 +946,19 @@ void calib_init(gs_calib_s* c, gs_ensemble_s* e, const gs_calib_config& cfg) {
         const int w = (int)(j % 4);
         c->lists[j].reserve(N * (w < 2 ? e->k : (w == 2 ? e->kw_max : e->ko_max)));
     }
+    const size_t entries = N * 2 * (size_t)e->k;  // own-group and other-group list slots
+    const size_t scan_words = N + 1 + (N + 2047) / 2048;  // launch_scan_counts' start + tile sums
     for (int b = 0; b < 2; ++b) {
         c->adj_own[b].reserve(N * kAdj);
-        c->adj_acc[b].reserve(N * kAdj);
+        c->rstart[b].reserve(scan_words);
+        c->contrib[b].reserve(2 * entries);
     }
+    c->rev.reserve(entries);
+    c->slot_of.reserve(entries);
+    c->rcnt.reserve(N);
+    launch_fill_u32(c->rcnt.p, (uint32_t)N, 0u, st);
+    c->loss_part.reserve(std::max<size_t>(N / kBlock, 1));
+    c->grad_part.reserve(std::max<size_t>(N / kBlock, 1) * kTheta);
     c->ids_fixed.reserve(N);
     c->ids_scratch.reserve(N);
     for (int a = 0; a < 3; ++a) c->pos_scratch[a].reserve(N);
@@ -910,8 +1019,14 @@ void calib_release(gs_calib_s* c) {
     for (auto& b : c->lists) b.release();
     for (int b = 0; b < 2; ++b) {
         c->adj_own[b].release();
-        c->adj_acc[b].release();
+        c->rstart[b].release();
+        c->contrib[b].release();
     }
+    c->rev.release();
+    c->slot_of.release();
+    c->rcnt.release();
+    c->loss_part.release();
+    c->grad_part.release();
     c->ids_fixed.release();
     c->ids_scratch.release();
     for (int a = 0; a < 3; ++a) c->pos_scratch[a].release();
@@ -927,7 +1042,7 @@ double calib_forward(gs_calib_s* c, const gs_calib_params& th) {
     c->theta = th;
     const CalibDev D = calib_dev(c);
     const uint32_t T = c->cfg.steps, K = c->cfg.checkpoint_every, nb = e->N / kBlock;
-    zero_words(c->loss.p, 4, st);
+    zero_words(c->loss_part.p, 2 * std::max<uint32_t>(nb, 1), st);
     GS_HIP(hipEventRecord(c->ev[0], st));
     copy_state(c, c->seg, 0, c->x0, 0);
     int cur = 0;
@@ -938,13 +1053,16 @@ double calib_forward(gs_calib_s* c, const gs_calib_params& th) {
         calib_search(c, cstate(c, c->seg, cur), slot, t > 0 ? prev : -1);
         if (nb)
             c4_forward<<<nb, kBlock, 0, st>>>(D, cstate(c, c->seg, cur), cstate(c, c->seg, cur ^ 1), list(c, slot, 0),
-                                              list(c, slot, 1), list(c, slot, 2), list(c, slot, 3), c->loss.p);
+                                              list(c, slot, 1), list(c, slot, 2), list(c, slot, 3),
+                                              c->loss_part.p);
         GS_LAUNCHED("c4_forward");
         cur ^= 1;
     }
     // Terminal loss (its adjoint seed is rewritten by each backward).
-    if (nb) c4_terminal<<<nb, kBlock, 0, st>>>(D, cstate(c, c->seg, cur), cadj(c->adj_own[0], e->N), c->loss.p);
+    if (nb) c4_terminal<<<nb, kBlock, 0, st>>>(D, cstate(c, c->seg, cur), cadj(c->adj_own[0], e->N), c->loss_part.p);
     GS_LAUNCHED("c4_terminal");
+    c4_reduce_rows<<<1, kBlock, 0, st>>>(c->loss_part.p, nb, 1, c->loss.p);
+    GS_LAUNCHED("c4_reduce_rows");
     copy_state(c, c->x0, 1, c->seg, cur);
     GS_HIP(hipEventRecord(c->ev[1], st));
     double loss = 0.0;
@@ -962,15 +1080,15 @@ void calib_backward(gs_calib_s* c, gs_calib_params& grad) {
     const uint32_t T = c->cfg.steps, K = c->cfg.checkpoint_every, nb = e->N / kBlock;
     const size_t N = e->N;
     GS_HIP(hipEventRecord(c->ev[2], st));
-    zero_words(c->grad.p, 2 * kTheta, st);
+    zero_words(c->grad_part.p, 2 * (size_t)std::max<uint32_t>(nb, 1) * kTheta, st);
     // Seed the adjoint of the final state (kept by the forward in x0[1]) here
     // rather than relying on the forward's seed: the sweep below reuses both
     // adjoint buffer pairs, so a second backward after one forward must start
-    // again from the terminal term.  Its loss goes to the scratch slot.
-    if (nb) c4_terminal<<<nb, kBlock, 0, st>>>(D, cstate(c, c->x0, 1), cadj(c->adj_own[0], N), c->loss.p + 1);
+    // again from the terminal term.  Its loss is not summed again.
+    if (nb) c4_terminal<<<nb, kBlock, 0, st>>>(D, cstate(c, c->x0, 1), cadj(c->adj_own[0], N), nullptr);
     GS_LAUNCHED("c4_terminal");
-    zero_words(c->adj_acc[0].p, N * kAdj, st);
-    int a = 0;  // adj[a] holds the adjoint of the state after the current step
+    int a = 0;  // adj_own[a] (+ the reverse lists rstart[a] / contrib[a]) hold the adjoint of the state after the step
+    bool have_in = false;  // the final state has no neighbour terms
     for (int sgi = (int)c->nseg - 1; sgi >= 0; --sgi) {
         const uint32_t t0 = (uint32_t)sgi * K;
         const uint32_t L = std::min(K, T - t0);
@@ -983,21 +1101,35 @@ void calib_backward(gs_calib_s* c, gs_calib_params& grad) {
             if (nb)
                 c4_forward<<<nb, kBlock, 0, st>>>(D, cstate(c, c->seg, (int)j), cstate(c, c->seg, (int)j + 1),
                                                   list(c, sl, 0), list(c, sl, 1), list(c, sl, 2), list(c, sl, 3),
-                                                  c->loss.p + 1);
+                                                  nullptr);
             GS_LAUNCHED("c4_forward");
         }
         for (int j = (int)L - 1; j >= 0; --j) {
-            zero_words(c->adj_acc[a ^ 1].p, N * kAdj, st);
             const int sl = slot_of(j);
-            if (nb)
+            if (nb) {
+                // reverse lists of step j into parity a ^ 1
+                c4_rev_count<<<nb, kBlock, 0, st>>>(D, list(c, sl, 0), list(c, sl, 1), c->rcnt.p);
+                GS_LAUNCHED("c4_rev_count");
+                launch_scan_counts(c->rcnt.p, c->rstart[a ^ 1].p, (uint32_t)N, st);  // leaves rcnt zero
+                c4_rev_fill<<<nb, kBlock, 0, st>>>(D, list(c, sl, 0), list(c, sl, 1), c->rstart[a ^ 1].p, c->rcnt.p,
+                                                   c->rev.p);
+                GS_LAUNCHED("c4_rev_fill");
+                c4_rev_rank<<<blocks_for(N), kBlock, 0, st>>>((uint32_t)N, c->rstart[a ^ 1].p, c->rev.p, c->rcnt.p,
+                                                              c->slot_of.p);
+                GS_LAUNCHED("c4_rev_rank");
                 c4_backward<<<nb, kBlock, 0, st>>>(D, cstate(c, c->seg, j), list(c, sl, 0), list(c, sl, 1),
                                                    list(c, sl, 2), list(c, sl, 3), cadj(c->adj_own[a], N),
-                                                   cadj(c->adj_acc[a], N), cadj(c->adj_own[a ^ 1], N),
-                                                   cadj(c->adj_acc[a ^ 1], N), c->grad.p);
-            GS_LAUNCHED("c4_backward");
+                                                   have_in ? c->rstart[a].p : nullptr, c->contrib[a].p,
+                                                   cadj(c->adj_own[a ^ 1], N), c->slot_of.p, c->contrib[a ^ 1].p,
+                                                   c->grad_part.p);
+                GS_LAUNCHED("c4_backward");
+            }
+            have_in = true;
             a ^= 1;
         }
     }
+    c4_reduce_rows<<<kTheta, kBlock, 0, st>>>(c->grad_part.p, nb, kTheta, c->grad.p);
+    GS_LAUNCHED("c4_reduce_rows");
     GS_HIP(hipEventRecord(c->ev[3], st));
     double g[kTheta];
     GS_HIP(hipMemcpyAsync(g, c->grad.p, sizeof g, hipMemcpyDeviceToHost, st));

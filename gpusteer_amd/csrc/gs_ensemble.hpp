@@ -85,10 +85,15 @@ struct gs_calib_s {
     gs::DevBuf<float> pos_scratch[3];
     // KNN lists of the segment's steps: per step own k, other k, walls kw, obstacles ko
     std::vector<gs::DevBuf<uint32_t>> lists;
-    // adjoint of a state, as each agent's own terms + accumulated neighbour
-    // terms (10 components x N each), double-buffered
-    gs::DevBuf<float> adj_own[2], adj_acc[2];
-    gs::DevBuf<double> loss, grad;
+    // adjoint of a state: each agent's own terms (10 components x N),
+    // double-buffered, plus the neighbour terms of the step's reverse lists
+    // (rstart, one 8-float slot per list entry in contrib), double-buffered;
+    // rev / slot_of / rcnt are the scratch of a reverse-list build
+    gs::DevBuf<float> adj_own[2];
+    gs::DevBuf<uint32_t> rstart[2], rev, slot_of, rcnt;
+    gs::DevBuf<float4> contrib[2];
+    // loss and gradient, and their per-block sums (added in block order)
+    gs::DevBuf<double> loss, grad, loss_part, grad_part;
     int final_buf = 0;  // which of seg[0] / seg[1] holds the forward's final state
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     float fwd_ms = 0.f, bwd_ms = 0.f;
