@@ -220,6 +220,11 @@ def ensemble_bench(n_scenarios: int, steps: int, warmup: int, cell_factor: float
     }
 
 
+# gs_calib_params order (include/gpusteer.h).
+CALIB_PARAM_NAMES = ("w_sep_0", "w_sep_1", "w_ali_0", "w_ali_1", "w_coh_0", "w_coh_1", "w_obs_0", "w_obs_1",
+                     "w_wall_0", "w_wall_1", "radius_0", "radius_1", "max_speed_0", "max_speed_1")
+
+
 def calib_bench(n_scenarios: int, n_agents: int, steps: int, every: int) -> dict:
     """Adjoint / forward cost of the differentiable calibration: crowd
     scenarios (two groups either side of a wall with a gap of 15), the 14-parameter loss over
@@ -239,6 +244,8 @@ def calib_bench(n_scenarios: int, n_agents: int, steps: int, every: int) -> dict
         loss = cal.forward(th)
         grad = cal.backward()
         fwd_ms, bwd_ms = cal.timing()
+        # The gradient is bitwise reproducible: a second backward is equal.
+        reproducible = bool((cal.backward() == grad).all())
         agents = ens.agents()
         cal.close()
         ens.close()
@@ -257,6 +264,11 @@ def calib_bench(n_scenarios: int, n_agents: int, steps: int, every: int) -> dict
         "checkpoint_every": every,
         "loss": loss,
         "grad_norm": float((grad ** 2).sum() ** 0.5),
+        "grad": {name: float(v) for name, v in zip(CALIB_PARAM_NAMES, grad)},
+        "grad_bitwise_reproducible": reproducible,
+        "grad_note": "the crowd is chaotic at this density (the oracle's position tangents grow about 10x every "
+                     "4-5 steps from step 20 on), so 100-step gradients are large; they equal the oracle's FP64 "
+                     "forward-mode tangent along the same fp32 trajectory (tests/test_gpu_calib.py)",
         "workload": "choke-point crowds (two groups of 8,192 either side of a wall with a gap of 15, 50 obstacles); loss = mean |p_T - goal|^2 + 10 x penetration; "
                     "gradients of 14 parameters (per group: separation, alignment, cohesion, obstacle and wall "
                     "weights, radius, max speed)",
@@ -632,6 +644,13 @@ def main() -> int:
             tuned = {"chosen": {"cells": list(cells), "column_radius": rf},
                      "table_ms_per_step": [{"cells": list(c), "column_radius": r, "ms": m} for c, r, m in table]}
             print(f"bench: autotune {table} -> {cells}, radius {rf}", file=sys.stderr, flush=True)
+        # Launch shapes of the k = 7 search and flocking kernels on that lattice.
+        (sh_knn, sh_flock), shape_table = autotune.tune_shapes(ctx, st, p, k=args.k)
+        shapes = {"knn": {"block_threads": sh_knn[0], "agents_per_lane": sh_knn[1]},
+                  "flock": {"block_threads": sh_flock[0], "agents_per_lane": sh_flock[1]},
+                  "table_ms": [{"block_threads": sh[0], "agents_per_lane": sh[1], "knn_ms": km, "flock_ms": fm,
+                                "step_ms": sm} for sh, km, fm, sm in shape_table]}
+        print(f"bench: launch shapes {shape_table} -> knn {sh_knn}, flock {sh_flock}", file=sys.stderr, flush=True)
         cells = list(p.grid_cells)
         C = cells[0] * cells[1] * cells[2]
         cells_txt = "x".join(str(c) for c in cells)
@@ -709,10 +728,13 @@ def main() -> int:
             "cells": cells,
             "column_radius": p.search_radius,
             "autotune": tuned,
+            "launch_shapes": shapes,
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",
+            # What actually limits the kernel (SQ counters): VALU issue, see valu_issue_frac.
+            "limiter": "valu",
             "kernel": "k_knn_binned (exact seeded KNN)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

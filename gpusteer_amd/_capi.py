@@ -23,6 +23,8 @@ GS_ERR_OUT_OF_MEMORY = 6
 GS_ERR_DEVICE = 7
 GS_ERR_NONFINITE = 8
 GS_ERR_FRAME = 9
+GS_SHAPE_KNN = 0
+GS_SHAPE_FLOCK = 1
 GS_MAX_K = 32
 GS_MAX_CELLS = 1 << 28
 GS_MAX_SPLIT_CELLS = 1 << 24
@@ -299,6 +301,8 @@ _SIGS = {
     "gs_step_boids": (C.c_int, [_h, _h, _h, C.POINTER(GsBoidsParams), C.c_uint32]),
     # A0, A1, walls, obstacles, bins, knn[8], scenario (crowd fields), steps
     "gs_step_choke": (C.c_int, [_h, _h, _h, _h, _h, C.POINTER(_h), C.POINTER(GsScenario), C.c_uint32]),
+    "gs_context_set_shape": (C.c_int, [_h, C.c_int, C.c_uint32, C.c_uint32]),
+    "gs_context_get_shape": (C.c_int, [_h, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "gs_frame_begin": (C.c_int, [_h]),
     "gs_frame_end": (C.c_int, [_h, C.POINTER(_h)]),
     "gs_frame_launch": (C.c_int, [_h, C.c_uint32]),

@@ -44,6 +44,17 @@ class Context:
         check(capi.lib.gs_context_stream(self.h, C.byref(s)), "gs_context_stream")
         return s.value or 0
 
+    def set_shape(self, kernel: int, block_threads: int, agents_per_lane: int) -> None:
+        """Launch shape of the k = 7 search (capi.GS_SHAPE_KNN) or flocking
+        kernel (capi.GS_SHAPE_FLOCK): 128 / 256 / 512 threads per block,
+        1 or 2 agents per lane."""
+        check(capi.lib.gs_context_set_shape(self.h, kernel, block_threads, agents_per_lane), "gs_context_set_shape")
+
+    def shape(self, kernel: int) -> Tuple[int, int]:
+        b, a = C.c_uint32(), C.c_uint32()
+        check(capi.lib.gs_context_get_shape(self.h, kernel, C.byref(b), C.byref(a)), "gs_context_get_shape")
+        return b.value, a.value
+
     def record(self, calls) -> "Frame":
         """Records the library calls that calls() issues on this context into
         a frame (gs_frame_begin / gs_frame_end): nothing runs, and every

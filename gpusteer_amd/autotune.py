@@ -1,9 +1,10 @@
 """Autotuner: picks the bin lattice (cells along x, y, z -- z may be finer, so
 the z-runs the KNN scans are trimmed more tightly) and the KNN fast path's
-column radius (3 x 3 or 5 x 5 columns) for a flock, and the cell-size factor
-of an ensemble's per-scenario lattices, by timing short runs of the real
-step on the device.  Every candidate computes the same (exact) result; only
-the time differs."""
+column radius (3 x 3 or 5 x 5 columns) for a flock, the launch shapes (block
+size, agents per lane) of the k = 7 search and flocking kernels, and the
+cell-size factor of an ensemble's per-scenario lattices, by timing short
+runs of the real step on the device.  Every candidate computes the same
+(exact) result; only the time differs."""
 from __future__ import annotations
 
 import copy
@@ -59,6 +60,40 @@ def tune_flock(ctx: api.Context, state, params, k: int = 7,
         A.close()
     best = min(table, key=lambda r: r[2])
     return (best[0], best[1]), table
+
+
+# (threads per block, agents per lane) of the k = 7 search and flocking kernels.
+SHAPES: Tuple[Tuple[int, int], ...] = ((128, 1), (256, 1), (512, 1), (128, 2), (256, 2), (512, 2))
+
+
+def tune_shapes(ctx: api.Context, state, params, k: int = 7, steps: int = 10, warmup: int = 3):
+    """Times `steps` fused Boids steps of the flock (lattice and radius as in
+    params) for each launch shape, set for both tuned kernels at once, and
+    keeps for each kernel the shape of its lowest HIP-event time; sets them
+    on ctx and returns ((knn shape, flock shape), table of (shape, knn ms,
+    flocking ms, step ms)).  Only k = 7 has tunable shapes."""
+    from ._capi import GS_SHAPE_FLOCK, GS_SHAPE_KNN
+
+    if k != 7:
+        return (ctx.shape(GS_SHAPE_KNN), ctx.shape(GS_SHAPE_FLOCK)), []
+    table = []
+    for shape in SHAPES:
+        ctx.set_shape(GS_SHAPE_KNN, *shape)
+        ctx.set_shape(GS_SHAPE_FLOCK, *shape)
+        A = api.AgentGroup(ctx, state)
+        bins = api.KNNBinData(ctx, params.grid_cells, params.grid_size, origin=params.grid_origin)
+        knn = api.KNNData(ctx, state.n, k)
+        api.stepBoids(A, knn, bins, params, warmup)
+        _, knn_ms, flock_ms, step_ms = api.stepBoidsTimed(A, knn, bins, params, steps)
+        table.append((shape, knn_ms, flock_ms, step_ms))
+        knn.close()
+        bins.close()
+        A.close()
+    best_knn = min(table, key=lambda r: r[1])[0]
+    best_flock = min(table, key=lambda r: r[2])[0]
+    ctx.set_shape(GS_SHAPE_KNN, *best_knn)
+    ctx.set_shape(GS_SHAPE_FLOCK, *best_flock)
+    return (best_knn, best_flock), table
 
 
 def tune_ensemble_cells(ctx: api.Context, make_specs, factors: Sequence[float] = (0.8, 1.0, 1.25, 1.6, 2.0, 2.5, 3.2),

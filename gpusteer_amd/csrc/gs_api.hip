@@ -515,6 +515,26 @@ int gs_context_stream(gs_context ctx, void** s) {
     });
 }
 
+int gs_context_set_shape(gs_context ctx, int kernel, uint32_t block_threads, uint32_t agents_per_lane) {
+    return guarded([&] {
+        check(ctx, "context");
+        if (kernel != GS_SHAPE_KNN && kernel != GS_SHAPE_FLOCK) fail(GS_ERR_INVALID_ARGUMENT, "unknown kernel");
+        if ((block_threads != 128 && block_threads != 256 && block_threads != 512) ||
+            (agents_per_lane != 1 && agents_per_lane != 2))
+            fail(GS_ERR_UNSUPPORTED, "launch shape must be 128, 256 or 512 threads and 1 or 2 agents per lane");
+        ctx->shape[kernel] = LaunchShape{block_threads, agents_per_lane};
+    });
+}
+
+int gs_context_get_shape(gs_context ctx, int kernel, uint32_t* block_threads, uint32_t* agents_per_lane) {
+    return guarded([&] {
+        check(ctx, "context");
+        if (kernel != GS_SHAPE_KNN && kernel != GS_SHAPE_FLOCK) fail(GS_ERR_INVALID_ARGUMENT, "unknown kernel");
+        if (block_threads) *block_threads = ctx->shape[kernel].block;
+        if (agents_per_lane) *agents_per_lane = ctx->shape[kernel].apl;
+    });
+}
+
 // ---- bins --------------------------------------------------------------------
 int gs_bins_create(gs_context ctx, const float origin[3], const float size[3], const uint32_t cells[3], gs_bins* out) {
     return guarded([&] {

@@ -165,10 +165,19 @@ __device__ __forceinline__ void st3(float* x, float* y, float* z, uint32_t i, f3
 
 // ---------------------------------------------------------------------------
 // Handles.
+namespace gs {
+// Launch shape of a tuned kernel: threads per block and agent slots per lane.
+struct LaunchShape {
+    uint32_t block = 256;
+    uint32_t apl = 1;
+};
+}  // namespace gs
+
 struct gs_context_s {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    gs::LaunchShape shape[2];  // GS_SHAPE_KNN, GS_SHAPE_FLOCK
     uint32_t magic = 0x67736378u;
 };
 

@@ -30,33 +30,47 @@
 namespace gs {
 
 // One lane per query slot s of A (storage = cell order, so a wavefront's lanes
-// share cells and their candidate runs overlap in cache).
-template <int KMAX, int RF>
-__global__ __launch_bounds__(kBlock, RF == 1 ? (KMAX <= 8 ? 5 : (KMAX <= 16 ? 4 : 1)) : 1) void k_knn_binned(uint32_t nA, const float4* __restrict__ apos, uint32_t nB,
+// share cells and their candidate runs overlap in cache).  The flat path
+// (RF == 1) takes its launch shape as template arguments: BS threads per
+// block and APL queries per lane (a block covers APL consecutive runs of BS
+// slots), both picked by the autotuner (gs_context_set_shape).
+template <int KMAX, int RF, int BS = kBlock, int APL = 1>
+__global__ __launch_bounds__(BS, RF == 1 ? (KMAX <= 8 ? 5 : (KMAX <= 16 ? 4 : 1)) : 1) void k_knn_binned(uint32_t nA, const float4* __restrict__ apos, uint32_t nB,
                                                        const float4* __restrict__ bpos,
                                                        const uint32_t* __restrict__ bstart, GridDev g, int self,
                                                        uint32_t k, int R0, float cap2, const uint32_t* __restrict__ seed,
                                                        const uint32_t* __restrict__ row_map,
                                                        const uint32_t* __restrict__ val_map,
                                                        uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    TopK<KMAX> h;
-    h.init(k);
     if constexpr (RF == 1) {
-        // Every lane stays to the end: the wavefront completes its lanes'
-        // inexact queries together (lanes past nA only help).
-        const bool active = s < nA;
-        const float4 q = active ? apos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-        FlatQuery Q{q.x, q.y, q.z, INFINITY, self ? s : UINT32_MAX, active};
-        if (nB > 0) {
-            Q.thr = fminf(cap2, (seed && active) ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB,
-                                                                    val_map, q.x, q.y, q.z, bpos)
-                                                 : INFINITY);
-            GS_FLAT_LDS
-            knn_search_flat<KMAX>(h, Q, g, nB, bpos, bstart, flds);
+        // Slot a of the lane's APL slots (a block covers APL runs of BS slots).
+        auto one = [&](uint32_t s) {
+            TopK<KMAX> h;
+            h.init(k);
+            // Every lane stays to the end: the wavefront completes its lanes'
+            // inexact queries together (lanes past nA only help).
+            const bool active = s < nA;
+            const float4 q = active ? apos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+            FlatQuery Q{q.x, q.y, q.z, INFINITY, self ? s : UINT32_MAX, active};
+            if (nB > 0) {
+                Q.thr = fminf(cap2, (seed && active) ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB,
+                                                                        val_map, q.x, q.y, q.z, bpos)
+                                                     : INFINITY);
+                GS_FLAT_LDS_N(BS)
+                knn_search_flat<KMAX, BS>(h, Q, g, nB, bpos, bstart, flds);
+            }
+            if (active) h.store(k, nA, s, out_idx, out_dist);
+        };
+        if constexpr (APL == 1) {
+            one(xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+        } else {
+            for (int a = 0; a < APL; ++a) one((xcd_block(blockIdx.x, gridDim.x) * APL + a) * BS + threadIdx.x);
         }
-        if (active) h.store(k, nA, s, out_idx, out_dist);
     } else {
+        static_assert(BS == kBlock && APL == 1, "the box walk has the default shape");
+        const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+        TopK<KMAX> h;
+        h.init(k);
         if (s >= nA) return;
         const float4 q = apos[s];
         if (nB > 0) {
@@ -222,14 +236,29 @@ __global__ __launch_bounds__(kBlock) void k_knn_remap(uint32_t nA, uint32_t k, c
     }
 }
 
-template <int KMAX, int RF>
-static void run_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, float cap2, const uint32_t* seed,
-                       const uint32_t* row_map, const uint32_t* val_map) {
+template <int KMAX, int RF, int BS = kBlock, int APL = 1>
+static void run_binned_shape(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, float cap2,
+                             const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map) {
     const uint32_t nA = A->n;
-    k_knn_binned<KMAX, RF><<<blocks_for(nA), kBlock, 0, A->ctx->stream>>>(
+    k_knn_binned<KMAX, RF, BS, APL><<<blocks_for(nA, BS * APL), BS, 0, A->ctx->stream>>>(
         nA, A->spos.p, B->n, B->spos.p, B->start.p, B->bin_grid, A == B ? 1 : 0, out->k, (int)R0, cap2, seed,
         row_map, val_map, out->idx_alt(), out->dist_alt());
     GS_LAUNCHED("k_knn_binned");
+}
+
+template <int KMAX, int RF>
+static void run_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, float cap2, const uint32_t* seed,
+                       const uint32_t* row_map, const uint32_t* val_map) {
+    // The k = 7 flat search runs in the context's launch shape.
+    if constexpr (KMAX == 7 && RF == 1) {
+        const LaunchShape sh = A->ctx->shape[GS_SHAPE_KNN];
+        if (sh.block == 128 && sh.apl == 1) return run_binned_shape<7, 1, 128, 1>(A, out, B, R0, cap2, seed, row_map, val_map);
+        if (sh.block == 128 && sh.apl == 2) return run_binned_shape<7, 1, 128, 2>(A, out, B, R0, cap2, seed, row_map, val_map);
+        if (sh.block == 256 && sh.apl == 2) return run_binned_shape<7, 1, 256, 2>(A, out, B, R0, cap2, seed, row_map, val_map);
+        if (sh.block == 512 && sh.apl == 1) return run_binned_shape<7, 1, 512, 1>(A, out, B, R0, cap2, seed, row_map, val_map);
+        if (sh.block == 512 && sh.apl == 2) return run_binned_shape<7, 1, 512, 2>(A, out, B, R0, cap2, seed, row_map, val_map);
+    }
+    run_binned_shape<KMAX, RF>(A, out, B, R0, cap2, seed, row_map, val_map);
 }
 
 template <int KMAX>

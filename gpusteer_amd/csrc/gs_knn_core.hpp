@@ -393,24 +393,27 @@ __device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, f
 constexpr int kFlatBuf = 16;
 constexpr int kFlatRuns = 10;  // nine columns, one of them split at the query
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
-// One pool of kFlatPool words per lane, strided by kBlock: the deferral
+// One pool of kFlatPool words per lane, strided by the block size: the deferral
 // buffer, the run list (first slots, then 16-bit lengths), and after the scan
 // the cooperative shells' candidate triples.
 constexpr int kFlatPool = kFlatBuf + kFlatRuns + (kFlatRuns + 1) / 2;
 constexpr int kCoopEntries = kFlatPool / 3;
 struct FlatLds {
     uint32_t* pool;      // this lane's column
-    uint32_t* slot;      // [kFlatBuf][kBlock]: deferred candidate slots
-    uint32_t* run_t;     // [kFlatRuns][kBlock]: first slot of a non-empty run
-    uint16_t* run_n;     // [kFlatRuns][kBlock]: its length
+    uint32_t* slot;      // [kFlatBuf][block size]: deferred candidate slots
+    uint32_t* run_t;     // [kFlatRuns][block size]: first slot of a non-empty run
+    uint16_t* run_n;     // [kFlatRuns][block size]: its length
     uint32_t slot_addr;  // LDS byte address of the lane's first buffer entry
 };
-#define GS_FLAT_LDS                                                                                     \
-    __shared__ uint32_t lds_flat_pool[kFlatPool * kBlock];                                             \
+// For a block of BS threads: lane columns strided by BS (the search
+// functions' STRIDE template argument).
+#define GS_FLAT_LDS_N(BS)                                                                               \
+    __shared__ uint32_t lds_flat_pool[kFlatPool * (BS)];                                               \
     const FlatLds flds{lds_flat_pool + threadIdx.x, lds_flat_pool + threadIdx.x,                        \
-                       lds_flat_pool + kFlatBuf * kBlock + threadIdx.x,                                 \
-                       (uint16_t*)(lds_flat_pool + (kFlatBuf + kFlatRuns) * kBlock) + threadIdx.x,      \
+                       lds_flat_pool + kFlatBuf * (BS) + threadIdx.x,                                   \
+                       (uint16_t*)(lds_flat_pool + (kFlatBuf + kFlatRuns) * (BS)) + threadIdx.x,        \
                        (uint32_t)(size_t)(lds_u32*)(lds_flat_pool + threadIdx.x)};
+#define GS_FLAT_LDS GS_FLAT_LDS_N(kBlock)
 
 // Buffer resource over the float4 snapshot of B (raw buffer, byte offsets,
 // range-checked to n * 16 bytes).  Flat-path searches need n < 2^27.
@@ -433,7 +436,7 @@ __device__ __forceinline__ int axis_cell_bound(float p, float o, float r, int d)
     return (int)f;
 }
 
-template <int KMAX>
+template <int KMAX, int STRIDE = kBlock>
 __device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLds& L, __amdgpu_buffer_rsrc_t rb,
                                            float qx, float qy, float qz, uint32_t self_slot, float thr) {
     for (int e0 = 0; e0 < kFlatBuf; e0 += 4) {
@@ -442,7 +445,7 @@ __device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLd
         float4 p[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            sl[u] = L.slot[(e0 + u) * kBlock];
+            sl[u] = L.slot[(e0 + u) * STRIDE];
             p[u] = ld_pos(rb, sl[u] * 16u);
         }
 #pragma unroll
@@ -475,6 +478,7 @@ __device__ __forceinline__ void sort_cx(uint64_t& ka, uint32_t& ta, uint64_t& kb
     ta = tl;
     tb = th;
 }
+template <int STRIDE = kBlock>
 __device__ __forceinline__ void flat_sort_fresh(TopK<7>& h, int cnt, const FlatLds& L, __amdgpu_buffer_rsrc_t rb,
                                                 float qx, float qy, float qz, uint32_t self_slot, float thr) {
     uint64_t key[8];
@@ -484,7 +488,7 @@ __device__ __forceinline__ void flat_sort_fresh(TopK<7>& h, int cnt, const FlatL
     for (int u = 0; u < 8; ++u) {
         // Entries past cnt hold stale slots: their loads are range-checked
         // (zeros past the end of B) and their keys are discarded below.
-        sl[u] = L.slot[u * kBlock];
+        sl[u] = L.slot[u * STRIDE];
         p[u] = ld_pos(rb, sl[u] * 16u);
     }
 #pragma unroll
@@ -527,7 +531,7 @@ __device__ __forceinline__ void flat_sort_fresh(TopK<7>& h, int cnt, const FlatL
 // one after another; more go to the lanes' own parallel walks.
 constexpr int kCoopMaxQueries = 4;
 
-template <int KMAX>
+template <int KMAX, int STRIDE = kBlock>
 __device__ void coop_shells(TopK<KMAX>& h, int owner, float qx0, float qy0, float qz0, float thr0,
                             uint32_t self0, int cx0, int cy0, int cz0, uint32_t* pool, const GridDev& g,
                             const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart) {
@@ -568,9 +572,9 @@ __device__ void coop_shells(TopK<KMAX>& h, int owner, float qx0, float qy0, floa
                 const float d2 = sqdist4(qx, qy, qz, p);
                 if (d2 <= lim && t != self_slot) {
                     if (cnt < kCoopEntries) {
-                        pool[(3 * cnt) * kBlock] = __float_as_uint(d2);
-                        pool[(3 * cnt + 1) * kBlock] = id_of(p);
-                        pool[(3 * cnt + 2) * kBlock] = t;
+                        pool[(3 * cnt) * STRIDE] = __float_as_uint(d2);
+                        pool[(3 * cnt + 1) * STRIDE] = id_of(p);
+                        pool[(3 * cnt + 2) * STRIDE] = t;
                         ++cnt;
                     } else {
                         overflow = true;
@@ -588,8 +592,8 @@ __device__ void coop_shells(TopK<KMAX>& h, int owner, float qx0, float qy0, floa
             if (lane == owner) {
                 const uint32_t* col = pool + (l - lane);  // lane l's column
                 for (int e = 0; e < c; ++e) {
-                    const uint64_t kk = ((uint64_t)col[(3 * e) * kBlock] << 32) | col[(3 * e + 1) * kBlock];
-                    if (h.accepts(kk)) h.push(kk, col[(3 * e + 2) * kBlock]);
+                    const uint64_t kk = ((uint64_t)col[(3 * e) * STRIDE] << 32) | col[(3 * e + 1) * STRIDE];
+                    if (h.accepts(kk)) h.push(kk, col[(3 * e + 2) * STRIDE]);
                 }
             }
         }
@@ -637,7 +641,7 @@ struct FlatQuery {
 
 // Requires 1 <= nB < kFlatMaxPoints and a minimum search radius of at most
 // one cell.  Called by every lane of the wavefront.
-template <int KMAX>
+template <int KMAX, int STRIDE = kBlock>
 __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, FlatQuery& Q, const GridDev& g, uint32_t nB,
                                                 const float4* __restrict__ bpos, const uint32_t* __restrict__ bstart,
                                                 const FlatLds& L) {
@@ -701,8 +705,8 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, FlatQuery& Q, con
             const uint32_t len = r1[col] - r0[col];
             long_run = long_run || len > 0xFFFFu;
             if (len != 0u) {
-                L.run_t[M * kBlock] = r0[col];
-                L.run_n[M * kBlock] = (uint16_t)len;
+                L.run_t[M * STRIDE] = r0[col];
+                L.run_n[M * STRIDE] = (uint16_t)len;
                 ++M;
             }
         }
@@ -725,8 +729,8 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, FlatQuery& Q, con
         en = e;
         if (tn >= e) {
             if (m < M) {
-                tn = rt[m * kBlock];
-                en = tn + rn[m * kBlock];
+                tn = rt[m * STRIDE];
+                en = tn + rn[m * STRIDE];
                 ++m;
             } else {
                 tn = en = 0u;
@@ -741,9 +745,9 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, FlatQuery& Q, con
     }
     next_chunk(ta, ea, tb, eb);
     // wa: LDS byte address of the lane's next free buffer entry (entries are
-    // kBlock words apart); the scan loop leaves the list's registers
+    // STRIDE words apart); the scan loop leaves the list's registers
     // untouched (merges happen between scan segments).
-    constexpr uint32_t kEntry = kBlock * 4u;
+    constexpr uint32_t kEntry = STRIDE * 4u;
     uint32_t wa = L.slot_addr;
     const uint32_t wfull = L.slot_addr + (kFlatBuf - 8) * kEntry;
     // The scan only filters: a candidate enters the buffer when its fused
@@ -788,11 +792,11 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, FlatQuery& Q, con
         const bool done = !__any(ta < ea);
         if constexpr (KMAX == 7) {
             if (fresh && done && !__any(cnt > 8)) {
-                flat_sort_fresh(h, cnt, L, rb, qx, qy, qz, self_slot, thr);
+                flat_sort_fresh<STRIDE>(h, cnt, L, rb, qx, qy, qz, self_slot, thr);
                 break;
             }
         }
-        flat_flush<KMAX>(h, cnt, L, rb, qx, qy, qz, self_slot, thr);
+        flat_flush<KMAX, STRIDE>(h, cnt, L, rb, qx, qy, qz, self_slot, thr);
         fresh = false;
         wa = L.slot_addr;
         lim = fminf(thr, h.worst());
@@ -806,7 +810,7 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, FlatQuery& Q, con
     if (__popcll(need) <= kCoopMaxQueries) {
         for (uint64_t left = need; left != 0; left &= left - 1) {
             const int owner = __ffsll((unsigned long long)left) - 1;
-            coop_shells<KMAX>(h, owner, qx, qy, qz, thr, self_slot, cx, cy, cz, L.pool, g, bpos, bstart);
+            coop_shells<KMAX, STRIDE>(h, owner, qx, qy, qz, thr, self_slot, cx, cy, cz, L.pool, g, bpos, bstart);
         }
     } else if (!exact) {
         // Many inexact queries in the wavefront (a lattice finer than the

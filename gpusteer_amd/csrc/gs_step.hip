@@ -44,14 +44,12 @@ struct LazyIO {
 };
 
 template <int KMAX, int MODE>
-__global__ __launch_bounds__(kBlock) void k_flock_update(uint32_t n, Agents A, StepUniforms U, LazyIO Z,
-                                                         float* __restrict__ npx, float* __restrict__ npy,
-                                                         float* __restrict__ npz, float* __restrict__ nfx,
-                                                         float* __restrict__ nfy, float* __restrict__ nfz,
-                                                         const float4* __restrict__ spos,
-                                                         const uint32_t* __restrict__ knn, uint32_t k, FlockTerm sep,
-                                                         FlockTerm ali, FlockTerm coh, float dt) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void flock_update_slot(uint32_t s, uint32_t n, Agents A, StepUniforms U, LazyIO Z,
+                                                  float* __restrict__ npx, float* __restrict__ npy,
+                                                  float* __restrict__ npz, float* __restrict__ nfx,
+                                                  float* __restrict__ nfy, float* __restrict__ nfz,
+                                                  const float4* __restrict__ spos, const uint32_t* __restrict__ knn,
+                                                  uint32_t k, FlockTerm sep, FlockTerm ali, FlockTerm coh, float dt) {
     if (s >= n) return;
     const bool lazy = Z.new_to_old != nullptr;
     // Own fields: at s (state in this order) or at the previous slot o.
@@ -104,6 +102,29 @@ __global__ __launch_bounds__(kBlock) void k_flock_update(uint32_t n, Agents A, S
     }
 }
 
+// One lane per agent slot; the launch shape (BS threads per block, APL slots
+// per lane: a block covers APL consecutive runs of BS slots) is picked by the
+// autotuner for k = 7 (gs_context_set_shape).
+template <int KMAX, int MODE, int BS = kBlock, int APL = 1>
+__global__ __launch_bounds__(BS) void k_flock_update(uint32_t n, Agents A, StepUniforms U, LazyIO Z,
+                                                     float* __restrict__ npx, float* __restrict__ npy,
+                                                     float* __restrict__ npz, float* __restrict__ nfx,
+                                                     float* __restrict__ nfy, float* __restrict__ nfz,
+                                                     const float4* __restrict__ spos,
+                                                     const uint32_t* __restrict__ knn, uint32_t k, FlockTerm sep,
+                                                     FlockTerm ali, FlockTerm coh, float dt) {
+    if constexpr (APL == 1) {
+        const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+        flock_update_slot<KMAX, MODE>(s, n, A, U, Z, npx, npy, npz, nfx, nfy, nfz, spos, knn, k, sep, ali, coh, dt);
+    } else {
+        for (int a = 0; a < APL; ++a) {
+            const uint32_t s = (xcd_block(blockIdx.x, gridDim.x) * APL + a) * BS + threadIdx.x;
+            flock_update_slot<KMAX, MODE>(s, n, A, U, Z, npx, npy, npz, nfx, nfy, nfz, spos, knn, k, sep, ali, coh,
+                                          dt);
+        }
+    }
+}
+
 static FlockTerm fterm(const gs_flock_term& t) { return FlockTerm{t.min_d, t.max_d, t.cos_max, t.weight, t.mask, 1}; }
 
 static StepUniforms step_uniforms(const gs_group_s* A) {
@@ -148,12 +169,28 @@ static void run_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params
         Z.az = A->Falt(9);
         Z.write_acc = acc_written ? 1u : 0u;
     }
-#define GS_FU(MODE)                                                                                                  \
-    k_flock_update<KMAX, MODE><<<blocks_for(n), kBlock, 0, st>>>(                                                    \
+    const LaunchShape sh = A->ctx->shape[GS_SHAPE_FLOCK];
+#define GS_FU(MODE, BS, APL)                                                                                         \
+    k_flock_update<KMAX, MODE, BS, APL><<<blocks_for(n, BS * APL), BS, 0, st>>>(                                     \
         n, a, U, Z, A->Falt(0), A->Falt(1), A->Falt(2), A->Falt(3), A->Falt(4), A->Falt(5), A->spos.p, knn->idx(), \
         knn->k, fterm(p->separation), fterm(p->alignment), fterm(p->cohesion), p->dt)
-    if (p->mode == GS_MODE_GPUREF) GS_FU(GS_MODE_GPUREF);
-    else GS_FU(GS_MODE_OPENSTEER);
+#define GS_FU_SHAPES(MODE)                                                                                           \
+    if constexpr (KMAX == 7) {                                                                                       \
+        if (sh.block == 128 && sh.apl == 1) GS_FU(MODE, 128, 1);                                                     \
+        else if (sh.block == 128 && sh.apl == 2) GS_FU(MODE, 128, 2);                                                \
+        else if (sh.block == 256 && sh.apl == 2) GS_FU(MODE, 256, 2);                                                \
+        else if (sh.block == 512 && sh.apl == 1) GS_FU(MODE, 512, 1);                                                \
+        else if (sh.block == 512 && sh.apl == 2) GS_FU(MODE, 512, 2);                                                \
+        else GS_FU(MODE, kBlock, 1);                                                                                 \
+    } else {                                                                                                         \
+        GS_FU(MODE, kBlock, 1);                                                                                      \
+    }
+    if (p->mode == GS_MODE_GPUREF) {
+        GS_FU_SHAPES(GS_MODE_GPUREF)
+    } else {
+        GS_FU_SHAPES(GS_MODE_OPENSTEER)
+    }
+#undef GS_FU_SHAPES
 #undef GS_FU
     GS_LAUNCHED("k_flock_update");
     // New positions and forward vectors (and, lazily, every other result)

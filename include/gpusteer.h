@@ -339,6 +339,20 @@ int gs_steer_avoid_neighbors(gs_group A, gs_knn knn, gs_group B, float min_time_
 int gs_wrap_world(gs_group A, int shape, const float center[3], const float size_or_radius[3]);
 int gs_clear_bits(gs_group A);
 
+/* ---- launch shapes ----------------------------------------------------------------
+ * The k = 7 exact search (the 3 x 3 flat path of gs_find_knn* and the fused
+ * step) and the k = 7 flocking + integrate kernel of gs_step_boids run in a
+ * launch shape the caller may set per context: block_threads in {128, 256,
+ * 512} threads per block and agents_per_lane in {1, 2} (a lane handles that
+ * many agent slots, so a wavefront covers 64 x agents_per_lane agents).
+ * Default 256 x 1.  Results do not depend on the shape.  gpusteer_amd's
+ * autotuner picks the shapes by timing them on the workload.  Other values
+ * return GS_ERR_UNSUPPORTED.  No counterpart in the reference (fixed
+ * block size THREADSPERBLOCK = 256, include/OpenSteer/CUDA/CUDAKernelGlobals.cuh:9). */
+enum { GS_SHAPE_KNN = 0, GS_SHAPE_FLOCK = 1 };
+int gs_context_set_shape(gs_context ctx, int kernel, uint32_t block_threads, uint32_t agents_per_lane);
+int gs_context_get_shape(gs_context ctx, int kernel, uint32_t* block_threads, uint32_t* agents_per_lane);
+
 /* ---- fused Boids step ------------------------------------------------------------ */
 typedef struct {
     int wrap_shape;               /* GS_WRAP_BOX / GS_WRAP_SPHERE, or -1 for none */

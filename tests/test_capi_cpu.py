@@ -81,6 +81,8 @@ def test_null_handles_rejected():
     assert L.gs_frame_launch(bogus, 1) == c.GS_ERR_INVALID_HANDLE
     assert L.gs_frame_destroy(bogus) == c.GS_ERR_INVALID_HANDLE
     assert L.gs_frame_destroy(None) == c.GS_OK
+    assert L.gs_context_set_shape(None, c.GS_SHAPE_KNN, 256, 1) == c.GS_ERR_INVALID_HANDLE
+    assert L.gs_context_get_shape(bogus, c.GS_SHAPE_FLOCK, None, None) == c.GS_ERR_INVALID_HANDLE
 
 
 def test_context_without_gpu_reports_device_error():
