@@ -43,35 +43,42 @@ def bytes_knn_kernel(k: int) -> float:
     return 12.0 + 4.0 * k
 
 
-TRAFFIC_SUMMARY = ROOT / "profiles" / "k_knn_binned_boids_box_1M_k7_hbm_traffic.md"
+def traffic_summary(cells_txt: str) -> Path:
+    return ROOT / "profiles" / f"k_knn_binned_boids_box_1M_k7_{cells_txt}_hbm_traffic.md"
 
 
-def knn_traffic_from_profile():
-    """HBM bytes per launch of the KNN kernel as measured by the committed
-    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes (each scaled by the same
-    counter's reading for a copy of known size), or None when absent."""
+def sq_summary(cells_txt: str) -> Path:
+    return ROOT / "profiles" / f"k_knn_binned_boids_box_1M_k7_{cells_txt}_sq_counters.md"
+
+
+def knn_traffic_from_profile(cells_txt: str):
+    """HBM bytes per launch of the KNN kernel on this lattice as measured by
+    the committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes (each scaled
+    by the same counter's reading for a copy of known size), or None when no
+    summary covers the lattice."""
     import re
 
-    if not TRAFFIC_SUMMARY.exists():
+    f = traffic_summary(cells_txt)
+    if not f.exists():
         return None
-    m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", TRAFFIC_SUMMARY.read_text(), re.M)
+    m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
     return float(m.group(1)) if m else None
 
 
-SQ_SUMMARY = ROOT / "profiles" / "k_knn_binned_boids_box_1M_k7_sq_counters.md"
 # MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies its SIMD for
 # 4 cycles; 2.4 GHz peak engine clock.
 VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 
 
-def knn_valu_from_profile():
-    """VALU instructions per launch of the KNN kernel (committed rocprofv3
-    --pmc SQ_INSTS_VALU pass), or None when absent."""
+def knn_valu_from_profile(cells_txt: str):
+    """VALU instructions per launch of the KNN kernel on this lattice
+    (committed rocprofv3 --pmc SQ_INSTS_VALU pass), or None when absent."""
     import re
 
-    if not SQ_SUMMARY.exists():
+    f = sq_summary(cells_txt)
+    if not f.exists():
         return None
-    m = re.search(r"^valu_instructions_per_launch\s*=\s*([0-9.eE+]+)", SQ_SUMMARY.read_text(), re.M)
+    m = re.search(r"^valu_instructions_per_launch\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
     return float(m.group(1)) if m else None
 
 
@@ -740,12 +747,13 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": knn_traffic_from_profile(),
+            "traffic": knn_traffic_from_profile(cells_txt),
+            "traffic_profile": str(traffic_summary(cells_txt).relative_to(ROOT)),
             # The kernel is bound by VALU issue, not bytes: the profiled VALU
             # instruction count per launch over the chip's issue rate, against
             # the live kernel time.
-            "valu_issue_frac": (knn_valu_from_profile() / VALU_ISSUE_PER_S / (knn_ms * 1e-3))
-            if knn_valu_from_profile() else None,
+            "valu_issue_frac": (knn_valu_from_profile(cells_txt) / VALU_ISSUE_PER_S / (knn_ms * 1e-3))
+            if knn_valu_from_profile(cells_txt) else None,
             "bytes_per_launch": n * b_knn,
             "bytes_per_agent": b_knn,
             "achievable_copy_gbs": copy_gbs,
