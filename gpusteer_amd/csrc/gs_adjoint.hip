@@ -462,9 +462,11 @@ __device__ inline uint32_t rev_code(uint32_t i, uint32_t w, uint32_t j, uint32_t
     return (i * 2u + w) * k + j;
 }
 
+// Counts the entries naming each agent; an entry's arrival index among them
+// goes to arrival[code] (the order of arrival varies, the count does not).
 __global__ __launch_bounds__(kBlock) void c4_rev_count(CalibDev D, const uint32_t* __restrict__ kself,
                                                        const uint32_t* __restrict__ kother,
-                                                       uint32_t* __restrict__ cnt) {
+                                                       uint32_t* __restrict__ cnt, uint32_t* __restrict__ arrival) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = blk * blockDim.x + threadIdx.x;
     const EnsPop& P = D.pops[D.block_pop[blk]];
@@ -474,15 +476,18 @@ __global__ __launch_bounds__(kBlock) void c4_rev_count(CalibDev D, const uint32_
         for (uint32_t j = 0; j < D.k; ++j) {
             const uint32_t b = lst[(size_t)j * D.N + i];
             if (b >= D.N) break;
-            atomicAdd(cnt + b, 1u);
+            arrival[rev_code(i, (uint32_t)w, j, D.k)] = atomicAdd(cnt + b, 1u);
         }
     }
 }
 
+// Places each entry's code in its agent's segment at its arrival index, and
+// the agent beside it (rev_agent).
 __global__ __launch_bounds__(kBlock) void c4_rev_fill(CalibDev D, const uint32_t* __restrict__ kself,
                                                       const uint32_t* __restrict__ kother,
-                                                      const uint32_t* __restrict__ rstart, uint32_t* __restrict__ cursor,
-                                                      uint32_t* __restrict__ rev) {
+                                                      const uint32_t* __restrict__ rstart,
+                                                      const uint32_t* __restrict__ arrival, uint32_t* __restrict__ rev,
+                                                      uint32_t* __restrict__ rev_agent) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = blk * blockDim.x + threadIdx.x;
     const EnsPop& P = D.pops[D.block_pop[blk]];
@@ -492,26 +497,29 @@ __global__ __launch_bounds__(kBlock) void c4_rev_fill(CalibDev D, const uint32_t
         for (uint32_t j = 0; j < D.k; ++j) {
             const uint32_t b = lst[(size_t)j * D.N + i];
             if (b >= D.N) break;
-            rev[rstart[b] + atomicAdd(cursor + b, 1u)] = rev_code(i, (uint32_t)w, j, D.k);
+            const uint32_t c = rev_code(i, (uint32_t)w, j, D.k);
+            const uint32_t pos = rstart[b] + arrival[c];
+            rev[pos] = c;
+            rev_agent[pos] = b;
         }
     }
 }
 
-// slot_of[code] = the code's rank in its agent's segment (the segments hold
-// each code once); clears the agent's cursor for the next build.
+// One thread per filled slot s: slot_of[code] = the code's rank in its
+// agent's segment (the segments hold each code once), so the slot order is
+// the code order, whatever the arrival.
 __global__ __launch_bounds__(kBlock) void c4_rev_rank(uint32_t N, const uint32_t* __restrict__ rstart,
-                                                      const uint32_t* __restrict__ rev, uint32_t* __restrict__ cursor,
+                                                      const uint32_t* __restrict__ rev,
+                                                      const uint32_t* __restrict__ rev_agent,
                                                       uint32_t* __restrict__ slot_of) {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= N) return;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= rstart[N]) return;
+    const uint32_t b = rev_agent[s];
     const uint32_t s0 = rstart[b], s1 = rstart[b + 1];
-    for (uint32_t s = s0; s < s1; ++s) {
-        const uint32_t c = rev[s];
-        uint32_t r = 0;
-        for (uint32_t t = s0; t < s1; ++t) r += rev[t] < c ? 1u : 0u;
-        slot_of[c] = s0 + r;
-    }
-    cursor[b] = 0u;
+    const uint32_t c = rev[s];
+    uint32_t r = 0;
+    for (uint32_t t = s0; t < s1; ++t) r += rev[t] < c ? 1u : 0u;
+    slot_of[c] = s0 + r;
 }
 
 __global__ __launch_bounds__(kBlock) void c4_forward(CalibDev D, CState X, CState Y,
@@ -954,6 +962,7 @@ void calib_init(gs_calib_s* c, gs_ensemble_s* e, const gs_calib_config& cfg) {
         c->contrib[b].reserve(2 * entries);
     }
     c->rev.reserve(entries);
+    c->rev_agent.reserve(entries);
     c->slot_of.reserve(entries);
     c->rcnt.reserve(N);
     launch_fill_u32(c->rcnt.p, (uint32_t)N, 0u, st);
@@ -1023,6 +1032,7 @@ void calib_release(gs_calib_s* c) {
         c->contrib[b].release();
     }
     c->rev.release();
+    c->rev_agent.release();
     c->slot_of.release();
     c->rcnt.release();
     c->loss_part.release();
@@ -1108,14 +1118,15 @@ void calib_backward(gs_calib_s* c, gs_calib_params& grad) {
             const int sl = slot_of(j);
             if (nb) {
                 // reverse lists of step j into parity a ^ 1
-                c4_rev_count<<<nb, kBlock, 0, st>>>(D, list(c, sl, 0), list(c, sl, 1), c->rcnt.p);
+                // (slot_of holds the arrival indices until the rank overwrites it)
+                c4_rev_count<<<nb, kBlock, 0, st>>>(D, list(c, sl, 0), list(c, sl, 1), c->rcnt.p, c->slot_of.p);
                 GS_LAUNCHED("c4_rev_count");
                 launch_scan_counts(c->rcnt.p, c->rstart[a ^ 1].p, (uint32_t)N, st);  // leaves rcnt zero
-                c4_rev_fill<<<nb, kBlock, 0, st>>>(D, list(c, sl, 0), list(c, sl, 1), c->rstart[a ^ 1].p, c->rcnt.p,
-                                                   c->rev.p);
+                c4_rev_fill<<<nb, kBlock, 0, st>>>(D, list(c, sl, 0), list(c, sl, 1), c->rstart[a ^ 1].p,
+                                                   c->slot_of.p, c->rev.p, c->rev_agent.p);
                 GS_LAUNCHED("c4_rev_fill");
-                c4_rev_rank<<<blocks_for(N), kBlock, 0, st>>>((uint32_t)N, c->rstart[a ^ 1].p, c->rev.p, c->rcnt.p,
-                                                              c->slot_of.p);
+                c4_rev_rank<<<blocks_for(N * 2 * e->k), kBlock, 0, st>>>((uint32_t)N, c->rstart[a ^ 1].p, c->rev.p,
+                                                                         c->rev_agent.p, c->slot_of.p);
                 GS_LAUNCHED("c4_rev_rank");
                 c4_backward<<<nb, kBlock, 0, st>>>(D, cstate(c, c->seg, j), list(c, sl, 0), list(c, sl, 1),
                                                    list(c, sl, 2), list(c, sl, 3), cadj(c->adj_own[a], N),

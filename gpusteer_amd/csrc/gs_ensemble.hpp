@@ -88,9 +88,9 @@ struct gs_calib_s {
     // adjoint of a state: each agent's own terms (10 components x N),
     // double-buffered, plus the neighbour terms of the step's reverse lists
     // (rstart, one 8-float slot per list entry in contrib), double-buffered;
-    // rev / slot_of / rcnt are the scratch of a reverse-list build
+    // rev / rev_agent / slot_of / rcnt are the scratch of a reverse-list build
     gs::DevBuf<float> adj_own[2];
-    gs::DevBuf<uint32_t> rstart[2], rev, slot_of, rcnt;
+    gs::DevBuf<uint32_t> rstart[2], rev, rev_agent, slot_of, rcnt;
     gs::DevBuf<float4> contrib[2];
     // loss and gradient, and their per-block sums (added in block order)
     gs::DevBuf<double> loss, grad, loss_part, grad_part;
