@@ -132,6 +132,9 @@ struct EnsKnnOut {
     const uint32_t* out_map;
 };
 
+// KMAX: 7 for k = 7 (the lists of exactly 7 entries of the single-instance
+// search), else 8.
+template <int KMAX>
 __global__ __launch_bounds__(kBlock) void e_knn(EnsDev E, uint32_t N, uint32_t k, const float4* __restrict__ spos,
                                                 const uint32_t* __restrict__ start,
                                                 const uint32_t* __restrict__ new_to_old,
@@ -148,32 +151,32 @@ __global__ __launch_bounds__(kBlock) void e_knn(EnsDev E, uint32_t N, uint32_t k
     const bool active = s < P.a0 + P.n;
     const float4 q = active ? spos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
     {
-        TopK<8> h;
+        TopK<KMAX> h;
         h.init(k);
-        const float thr = (out.seed_self && active) ? seed_bound<8>(out.seed_self, k, N, new_to_old[s], N,
+        const float thr = (out.seed_self && active) ? seed_bound<KMAX>(out.seed_self, k, N, new_to_old[s], N,
                                                                     old_to_new, q.x, q.y, q.z, spos)
                                                     : INFINITY;
         FlatQuery Q{q.x, q.y, q.z, thr, s, active};
-        knn_search_flat<8>(h, Q, P.g, N, spos, start + P.c0, flds);
-        if (active) store_idx<8>(h, k, N, s, out.self, out.out_map);
+        knn_search_flat<KMAX>(h, Q, P.g, N, spos, start + P.c0, flds);
+        if (active) store_idx<KMAX>(h, k, N, s, out.self, out.out_map);
     }
     if (P.kind != GS_SCENARIO_CHOKE) return;
     const EnsPop& O = E.pops[P.other];
     if (O.n > 0) {
-        TopK<8> h;
+        TopK<KMAX> h;
         h.init(k);
         const float thr = fminf(E.scens[P.scen].other_cap2,
                                 (out.seed_other && active)
-                                    ? seed_bound<8>(out.seed_other, k, N, new_to_old[s], N, old_to_new, q.x, q.y,
+                                    ? seed_bound<KMAX>(out.seed_other, k, N, new_to_old[s], N, old_to_new, q.x, q.y,
                                                     q.z, spos)
                                     : INFINITY);
         FlatQuery Q{q.x, q.y, q.z, thr, UINT32_MAX, active};
-        knn_search_flat<8>(h, Q, O.g, N, spos, start + O.c0, flds);
-        if (active) store_idx<8>(h, k, N, s, out.other, out.out_map);
+        knn_search_flat<KMAX>(h, Q, O.g, N, spos, start + O.c0, flds);
+        if (active) store_idx<KMAX>(h, k, N, s, out.other, out.out_map);
     } else if (active) {
-        TopK<8> h;
+        TopK<KMAX> h;
         h.init(k);
-        store_idx<8>(h, k, N, s, out.other, out.out_map);
+        store_idx<KMAX>(h, k, N, s, out.other, out.out_map);
     }
     if (!active) return;
     const EnsScen& S = E.scens[P.scen];
@@ -521,6 +524,15 @@ static EnsDev ens_dev(gs_ensemble_s* e) {
 // binned into a sorted snapshot (the ensemble's bin buffers as scratch; its
 // own state and KNN containers are not touched) and the lists are written in
 // the fixed order (rows and neighbour values are fixed indices).  Seeds are
+// The ensemble search kernel for the ensemble's k (7-entry lists for k = 7).
+#define GS_E_KNN(...)                                                                            \
+    do {                                                                                         \
+        if (e->k == 7)                                                                           \
+            e_knn<7><<<nblocks, kBlock, 0, st>>>(__VA_ARGS__);                                   \
+        else                                                                                     \
+            e_knn<8><<<nblocks, kBlock, 0, st>>>(__VA_ARGS__);                                   \
+    } while (0)
+
 // previous lists in the same fixed order (or NULL).
 void ens_search_fixed(gs_ensemble_s* e, float* px, float* py, float* pz, const uint32_t* ids_fixed,
                       uint32_t* ids_scratch, float* const scratch[3], const uint32_t* seed_self,
@@ -549,7 +561,7 @@ void ens_search_fixed(gs_ensemble_s* e, float* px, float* py, float* pz, const u
     ko.seed_self = seed_self;
     ko.seed_other = seed_other;
     ko.out_map = e->new_to_old.p;
-    e_knn<<<nblocks, kBlock, 0, st>>>(E, N, e->k, e->spos.p, e->start.p, e->new_to_old.p, e->old_to_new.p, ko,
+    GS_E_KNN(E, N, e->k, e->spos.p, e->start.p, e->new_to_old.p, e->old_to_new.p, ko,
                                       e->walls[9].p, e->walls[10].p, e->walls[11].p, e->obs[0].p, e->obs[1].p,
                                       e->obs[2].p);
     GS_LAUNCHED("e_knn");
@@ -584,7 +596,7 @@ void ens_step(gs_ensemble_s* e) {
     ko.seed_self = e->seeded ? e->knn_self[e->kcur].p : nullptr;
     ko.seed_other = e->seeded ? e->knn_other[e->kcur].p : nullptr;
     ko.out_map = nullptr;
-    e_knn<<<nblocks, kBlock, 0, st>>>(E, N, e->k, e->spos.p, e->start.p, e->new_to_old.p, e->old_to_new.p, ko,
+    GS_E_KNN(E, N, e->k, e->spos.p, e->start.p, e->new_to_old.p, e->old_to_new.p, ko,
                                       e->walls[9].p, e->walls[10].p, e->walls[11].p, e->obs[0].p, e->obs[1].p,
                                       e->obs[2].p);
     GS_LAUNCHED("e_knn");
