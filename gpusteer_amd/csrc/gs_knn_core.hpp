@@ -44,39 +44,41 @@ template <int KMAX> struct TopK {
     }
     __device__ __forceinline__ float worst() const { return __uint_as_float((uint32_t)(key[KMAX - 1] >> 32)); }
     __device__ __forceinline__ bool accepts(uint64_t kk) const { return kk < key[KMAX - 1]; }
-    // Caller guarantees accepts(kk).  Branch-free.  Lists of up to 8: kk goes
-    // to the first position whose key exceeds it (all compares independent)
-    // and every entry from there on moves up one.  Longer lists: a
-    // compare-exchange ripple from the back (fewer live registers; the
-    // 16-entry kernel is at its register limit).
+    // Caller guarantees accepts(kk).  Branch-free compare-exchange ripple
+    // from the back (few live registers).
     __device__ __forceinline__ void push(uint64_t kk, uint32_t tt) {
-        if constexpr (KMAX <= 8) {
-            bool c[KMAX];
+        key[KMAX - 1] = kk;
+        t[KMAX - 1] = tt;
 #pragma unroll
-            for (int j = 0; j < KMAX; ++j) c[j] = kk < key[j];
-#pragma unroll
-            for (int j = KMAX - 1; j >= 1; --j) {
-                key[j] = c[j - 1] ? key[j - 1] : (c[j] ? kk : key[j]);
-                t[j] = c[j - 1] ? t[j - 1] : (c[j] ? tt : t[j]);
-            }
-            key[0] = c[0] ? kk : key[0];
-            t[0] = c[0] ? tt : t[0];
-        } else {
-            key[KMAX - 1] = kk;
-            t[KMAX - 1] = tt;
-#pragma unroll
-            for (int j = KMAX - 2; j >= 0; --j) {
-                const bool sw = key[j + 1] < key[j];
-                const uint64_t lo_k = sw ? key[j + 1] : key[j];
-                const uint64_t hi_k = sw ? key[j] : key[j + 1];
-                const uint32_t lo_t = sw ? t[j + 1] : t[j];
-                const uint32_t hi_t = sw ? t[j] : t[j + 1];
-                key[j] = lo_k;
-                key[j + 1] = hi_k;
-                t[j] = lo_t;
-                t[j + 1] = hi_t;
-            }
+        for (int j = KMAX - 2; j >= 0; --j) {
+            const bool sw = key[j + 1] < key[j];
+            const uint64_t lo_k = sw ? key[j + 1] : key[j];
+            const uint64_t hi_k = sw ? key[j] : key[j + 1];
+            const uint32_t lo_t = sw ? t[j + 1] : t[j];
+            const uint32_t hi_t = sw ? t[j] : t[j + 1];
+            key[j] = lo_k;
+            key[j + 1] = hi_k;
+            t[j] = lo_t;
+            t[j + 1] = hi_t;
         }
+    }
+    // The same insertion with every compare independent: kk goes to the
+    // first position whose key exceeds it and every entry from there on
+    // moves up one (8 64-bit compares instead of 13 for 7 entries).  Used by
+    // the flat search's merge of 7-entry lists, where the extra live
+    // compares fit the register budget (with 8 entries, or inlined at the
+    // box walk's many sites, they spill).
+    __device__ __forceinline__ void push_wide(uint64_t kk, uint32_t tt) {
+        bool c[KMAX];
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) c[j] = kk < key[j];
+#pragma unroll
+        for (int j = KMAX - 1; j >= 1; --j) {
+            key[j] = c[j - 1] ? key[j - 1] : (c[j] ? kk : key[j]);
+            t[j] = c[j - 1] ? t[j - 1] : (c[j] ? tt : t[j]);
+        }
+        key[0] = c[0] ? kk : key[0];
+        t[0] = c[0] ? tt : t[0];
     }
     // Writes column `row` of the slot-major [k][nA] output: neighbour slots
     // and distances (pads UINT32_MAX / FLT_MAX).
@@ -470,7 +472,12 @@ __device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLd
             if (e0 + u < cnt && sl[u] != self_slot) {
                 const float d2 = sqdist4(qx, qy, qz, p[u]);
                 const uint64_t kk = make_key(d2, id_of(p[u]));
-                if (d2 <= thr && h.accepts(kk)) h.push(kk, sl[u]);
+                if (d2 <= thr && h.accepts(kk)) {
+                    if constexpr (KMAX == 7)
+                        h.push_wide(kk, sl[u]);
+                    else
+                        h.push(kk, sl[u]);
+                }
             }
         }
     }
