@@ -1789,6 +1789,30 @@ int gs_ensemble_upload_agents(gs_ensemble e, uint32_t scenario, uint32_t group, 
         }
         GS_HIP(hipStreamSynchronize(st));
         e->seeded = false;
+        // Which of the never-written fields hold one value in this population
+        // (bit patterns), and so need not move in the counting sort.
+        const uint32_t consts = (0x7u << 7) | (0xFu << 19);
+        uint32_t uni = 0;
+        for (int j = 0; j < kFields; ++j) {
+            if (!(consts & (1u << j))) continue;
+            const uint32_t* w = P.n ? (const uint32_t*)hf[j] : nullptr;
+            bool same = true;
+            for (uint32_t i = 1; i < P.n && same; ++i) same = w[i] == w[0];
+            if (same) uni |= 1u << j;
+        }
+        if (e->pop_uniform.size() != e->pops.size()) e->pop_uniform.assign(e->pops.size(), 0u);
+        e->pop_uniform[(size_t)(&P - e->pops.data())] = uni;
+        uint32_t frozen = consts;
+        for (uint32_t u : e->pop_uniform) frozen &= u;
+        bool opensteer = false;
+        for (const gs_scenario& d : e->scen) opensteer = opensteer || d.mode == GS_MODE_OPENSTEER;
+        if (opensteer) frozen &= ~(0x7u << 7);  // the smoothed acceleration is written
+        if (frozen != e->frozen) {
+            drop_graphs(e);  // captured steps move the fields of the old set
+            e->frozen = frozen;
+        }
+        // The uploaded steering and bits may be non-zero: the next step moves them.
+        e->steer_zero = false;
     });
 }
 
@@ -1922,7 +1946,9 @@ int gs_ensemble_step(gs_ensemble e, uint32_t steps, uint32_t flags) {
             for (; left; --left) ens_step(e);
         } else {
             while (left >= 2) {
-                const int par = e->bcur;
+                // Captured pairs of steps start at either parity of the ids
+                // (every moved field flips with them; frozen ones stay put).
+                const int par = e->icur;
                 if (!e->exec[par]) {
                     GS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
                     try {

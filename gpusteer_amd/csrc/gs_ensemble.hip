@@ -576,17 +576,26 @@ void ens_step(gs_ensemble_s* e) {
     e_bin_key<<<nblocks, kBlock, 0, st>>>(E, e->F(0), e->F(1), e->F(2), e->key.p, e->rank.p, e->count.p);
     GS_LAUNCHED("e_bin_key");
     launch_scan_counts(e->count.p, e->start.p, e->C, st);
+    // Moved: every field but the frozen ones (positions first, as the
+    // reorder's snapshot reads them there).
+    const uint32_t steer = 0x7u << 16;
+    const uint32_t moved = ((1u << kFields) - 1u) & ~e->frozen & ~(e->steer_zero ? steer : 0u);
     FieldSet fs{};
-    for (int f = 0; f < kFields; ++f) {
-        fs.src[f] = e->F(f);
-        fs.dst[f] = e->Falt(f);
-    }
-    fs.nf = kFields;
+    int nf = 0;
+    for (int f = 0; f < kFields; ++f)
+        if (moved & (1u << f)) {
+            fs.src[nf] = e->F(f);
+            fs.dst[nf] = e->Falt(f);
+            ++nf;
+        }
+    fs.nf = nf;
+    const bool move_bits = !e->steer_zero;
     launch_scatter_reorder(N, e->key.p, e->rank.p, e->start.p, e->ids(), e->idb[e->icur ^ 1].p, e->perm_tmp.p,
-                           e->id_tmp.p, e->new_to_old.p, e->old_to_new.p, e->bits(), e->bitsb[e->bcur ^ 1].p, fs,
-                           e->spos.p, st);
-    for (int f = 0; f < kFields; ++f) e->fcur[f] ^= 1;
-    e->bcur ^= 1;
+                           e->id_tmp.p, e->new_to_old.p, e->old_to_new.p, move_bits ? e->bits() : nullptr,
+                           move_bits ? e->bitsb[e->bcur ^ 1].p : nullptr, fs, e->spos.p, st);
+    for (int f = 0; f < kFields; ++f)
+        if (moved & (1u << f)) e->fcur[f] ^= 1;
+    if (move_bits) e->bcur ^= 1;
     e->icur ^= 1;
     EnsKnnOut ko;
     ko.self = e->knn_self[e->kcur ^ 1].p;
@@ -629,6 +638,8 @@ void ens_step(gs_ensemble_s* e) {
         GS_LAUNCHED("e_choke_update");
     }
     for (int f = 0; f < 6; ++f) e->fcur[f] ^= 1;
+    // Every agent's steering and bits are now zero.
+    e->steer_zero = true;
 }
 
 }  // namespace gs

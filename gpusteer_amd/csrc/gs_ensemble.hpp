@@ -51,6 +51,16 @@ struct gs_ensemble_s {
     gs::DevBuf<uint32_t> knn_self[2], knn_other[2];
     uint8_t kcur = 0;
     bool seeded = false;
+    // Fields a step's counting sort need not move: the per-agent constants
+    // (and the smoothed acceleration when no scenario integrates in the
+    // OpenSteer mode) when every population holds one value for them, since
+    // the sort keeps each population in its slot range; and, from the second
+    // step after an upload on, the steering vector and the bits, which every
+    // step leaves zero.  pop_uniform: per population, the fields uniform at
+    // its last upload.
+    std::vector<uint32_t> pop_uniform;
+    uint32_t frozen = 0;   // frozen constants (fields 7..9, 19..22)
+    bool steer_zero = false;
     uint32_t kw_max = 1, ko_max = 1;
     gs::DevBuf<uint32_t> knn_walls, knn_obs;
     gs::DevBuf<uint32_t> key, rank, perm_tmp, id_tmp, new_to_old, old_to_new, count, start;

@@ -200,6 +200,10 @@ class Sequential:
                 _, G, knns, W, O, bins, sc = it
                 step_choke(G, knns, W, O, bins, sc, steps)
 
+    def upload(self, scenario: int, group: int, st: AgentState) -> None:
+        it = self.items[scenario]
+        (it[1] if it[0] == "boids" else it[1][group]).upload(st)
+
     def download(self, scenario: int, group: int = 0) -> AgentState:
         it = self.items[scenario]
         return it[1].download() if it[0] == "boids" else it[1][group].download()

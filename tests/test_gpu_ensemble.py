@@ -67,6 +67,42 @@ def test_ensemble_reupload_restarts(ctx):
     seq.close()
 
 
+def test_ensemble_per_agent_constants_and_steering(ctx):
+    """The counting sort leaves in place the fields every population holds
+    one value for (radius, speed and force limits, mass) and, after the
+    first step, the steering and bits (zero after every step).  Agents
+    uploaded with varying radii and non-zero steering and bits must still
+    equal the sequential calls; a re-upload that makes the radii uniform
+    and then varied again changes the set the captured steps move."""
+    specs = scenarios.ensemble_mix(n_scenarios=4, n_min=600, n_max=1500, workload=14)
+    g = np.random.default_rng(5)
+
+    def varied(st):
+        st = st.copy()
+        a = st.arrays
+        a["radius"][:] = g.uniform(0.3, 0.7, st.n).astype(np.float32)
+        for f in ("gx", "gy", "gz"):
+            a[f][:] = g.uniform(-1, 1, st.n).astype(np.float32)
+        st.bits[:] = g.integers(0, 1 << 12, st.n).astype(np.uint32)
+        return st
+
+    def states(spec, fn):
+        return [fn(spec[1])] if spec[0] == "boids" else [fn(x) for x in spec[1].groups]
+
+    ens = Ensemble(ctx, specs)
+    seq = Sequential(ctx, specs)
+    for round_, fn in enumerate((varied, lambda st: st.copy(), varied)):
+        for i, spec in enumerate(specs):
+            for gi, st in enumerate(states(spec, fn)):
+                ens.upload(i, gi, st)
+                seq.upload(i, gi, st)
+        ens.step(7)
+        seq.step(7)
+        _compare(ens, seq, specs, f"upload round {round_}")
+    ens.close()
+    seq.close()
+
+
 def test_ensemble_workload_subset_equals_oracle(ctx, orc):
     """The batched-ensemble workload itself: all 256 scenarios of the mix
     (agent counts log-uniform over 1,024-65,536, about 3.9 M agents; even
