@@ -1429,9 +1429,11 @@ static void check_boids_args(gs_group A, gs_knn knn, gs_bins bins, const gs_boid
 
 // One Boids step: build + reorder (wrap fused), seeded exact KNN of A with
 // itself, then flocking + integrate (new positions / forwards into the
-// alternate buffers, flipped afterwards).
+// alternate buffers, flipped afterwards).  dist false: the search writes
+// neighbour slots only -- the flocking kernel and the next step's seed read
+// no distances, so only the last step of a call stores them.
 static void boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const gs_boids_params* p,
-                            hipEvent_t* ev) {
+                            hipEvent_t* ev, bool dist = true) {
     hipStream_t st = A->ctx->stream;
     if (ev) GS_HIP(hipEventRecord(ev[0], st));
     const bool lazy = flock_step_lazy_ok(A);
@@ -1439,7 +1441,7 @@ static void boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const
     after_reorder(A);
     if (ev) GS_HIP(hipEventRecord(ev[1], st));
     const Seed sd = seed_for(knn, A, A, p->knn_flags);
-    launch_knn_binned(A, knn, A, p->search_radius, INFINITY, sd.seed, sd.row_map, sd.val_map);
+    launch_knn_binned(A, knn, A, p->search_radius, INFINITY, sd.seed, sd.row_map, sd.val_map, dist);
     if (A->n) knn->cur ^= 1;
     mark_result(knn, A, A);
     if (ev) GS_HIP(hipEventRecord(ev[2], st));
@@ -1493,7 +1495,7 @@ int gs_step_boids(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* p
     return guarded([&] {
         check_boids_args(A, knn, bins, p);
         use_device(A->ctx);
-        for (uint32_t s = 0; s < steps; ++s) boids_step_once(A, knn, bins, p, nullptr);
+        for (uint32_t s = 0; s < steps; ++s) boids_step_once(A, knn, bins, p, nullptr, s + 1 == steps);
     });
 }
 

@@ -359,8 +359,10 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
 // KNN into out's alternate buffer (caller flips).  row_map / val_map translate
 // a seed held at the previous storage order of A / B (NULL: same order).
 // cap2: only points with d2 <= cap2 qualify (+inf: no cap).
+// dist false: the alternate distance buffer is left unwritten (a caller that
+// reads only the indices before the next search).
 void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, float cap2,
-                       const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map);
+                       const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map, bool dist = true);
 void launch_knn_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B, float cap2);
 // The binned search's semantics (position snapshots of both builds) by an
 // exhaustive scan, for small B.

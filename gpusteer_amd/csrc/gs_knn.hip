@@ -238,32 +238,32 @@ __global__ __launch_bounds__(kBlock) void k_knn_remap(uint32_t nA, uint32_t k, c
 
 template <int KMAX, int RF, int BS = kBlock, int APL = 1>
 static void run_binned_shape(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, float cap2,
-                             const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map) {
+                             const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map, bool dist) {
     const uint32_t nA = A->n;
     k_knn_binned<KMAX, RF, BS, APL><<<blocks_for(nA, BS * APL), BS, 0, A->ctx->stream>>>(
         nA, A->spos.p, B->n, B->spos.p, B->start.p, B->bin_grid, A == B ? 1 : 0, out->k, (int)R0, cap2, seed,
-        row_map, val_map, out->idx_alt(), out->dist_alt());
+        row_map, val_map, out->idx_alt(), dist ? out->dist_alt() : nullptr);
     GS_LAUNCHED("k_knn_binned");
 }
 
 template <int KMAX, int RF>
 static void run_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, float cap2, const uint32_t* seed,
-                       const uint32_t* row_map, const uint32_t* val_map) {
+                       const uint32_t* row_map, const uint32_t* val_map, bool d) {
     // The k = 7 flat search runs in the context's launch shape.
     if constexpr (KMAX == 7 && RF == 1) {
         const LaunchShape sh = A->ctx->shape[GS_SHAPE_KNN];
-        if (sh.block == 128 && sh.apl == 1) return run_binned_shape<7, 1, 128, 1>(A, out, B, R0, cap2, seed, row_map, val_map);
-        if (sh.block == 128 && sh.apl == 2) return run_binned_shape<7, 1, 128, 2>(A, out, B, R0, cap2, seed, row_map, val_map);
-        if (sh.block == 256 && sh.apl == 2) return run_binned_shape<7, 1, 256, 2>(A, out, B, R0, cap2, seed, row_map, val_map);
-        if (sh.block == 512 && sh.apl == 1) return run_binned_shape<7, 1, 512, 1>(A, out, B, R0, cap2, seed, row_map, val_map);
-        if (sh.block == 512 && sh.apl == 2) return run_binned_shape<7, 1, 512, 2>(A, out, B, R0, cap2, seed, row_map, val_map);
+        if (sh.block == 128 && sh.apl == 1) return run_binned_shape<7, 1, 128, 1>(A, out, B, R0, cap2, seed, row_map, val_map, d);
+        if (sh.block == 128 && sh.apl == 2) return run_binned_shape<7, 1, 128, 2>(A, out, B, R0, cap2, seed, row_map, val_map, d);
+        if (sh.block == 256 && sh.apl == 2) return run_binned_shape<7, 1, 256, 2>(A, out, B, R0, cap2, seed, row_map, val_map, d);
+        if (sh.block == 512 && sh.apl == 1) return run_binned_shape<7, 1, 512, 1>(A, out, B, R0, cap2, seed, row_map, val_map, d);
+        if (sh.block == 512 && sh.apl == 2) return run_binned_shape<7, 1, 512, 2>(A, out, B, R0, cap2, seed, row_map, val_map, d);
     }
-    run_binned_shape<KMAX, RF>(A, out, B, R0, cap2, seed, row_map, val_map);
+    run_binned_shape<KMAX, RF>(A, out, B, R0, cap2, seed, row_map, val_map, d);
 }
 
 template <int KMAX>
 static void run_binned_rf(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t R0, float cap2,
-                          const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map) {
+                          const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map, bool d) {
     // search_radius selects the fast path's column radius: 3 x 3 columns for
     // lattices whose cell is about the k-th neighbour distance, 5 x 5 for
     // finer lattices.
@@ -272,19 +272,19 @@ static void run_binned_rf(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t 
     // Lists longer than 8 keep the 3 x 3 path on every lattice: the 5 x 5
     // walk holding 16 keys exceeds the register file and spills.
     if ((R0 >= 2 && KMAX <= 8) || B->n >= kFlatMaxPoints)
-        run_binned<KMAX, 2>(A, out, B, R0, cap2, seed, row_map, val_map);
-    else run_binned<KMAX, 1>(A, out, B, R0, cap2, seed, row_map, val_map);
+        run_binned<KMAX, 2>(A, out, B, R0, cap2, seed, row_map, val_map, d);
+    else run_binned<KMAX, 1>(A, out, B, R0, cap2, seed, row_map, val_map, d);
 }
 
 void launch_knn_binned(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32_t search_radius, float cap2,
-                       const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map) {
+                       const uint32_t* seed, const uint32_t* row_map, const uint32_t* val_map, bool dist) {
     if (A->n == 0) return;
     // k = 7 (the parameter files' default) gets lists of exactly 7 entries:
     // one compare-exchange fewer per insertion than the 8-entry lists.
-    if (out->k == 7) run_binned_rf<7>(A, out, B, search_radius, cap2, seed, row_map, val_map);
-    else if (out->k <= 8) run_binned_rf<8>(A, out, B, search_radius, cap2, seed, row_map, val_map);
-    else if (out->k <= 16) run_binned_rf<16>(A, out, B, search_radius, cap2, seed, row_map, val_map);
-    else run_binned_rf<32>(A, out, B, search_radius, cap2, seed, row_map, val_map);
+    if (out->k == 7) run_binned_rf<7>(A, out, B, search_radius, cap2, seed, row_map, val_map, dist);
+    else if (out->k <= 8) run_binned_rf<8>(A, out, B, search_radius, cap2, seed, row_map, val_map, dist);
+    else if (out->k <= 16) run_binned_rf<16>(A, out, B, search_radius, cap2, seed, row_map, val_map, dist);
+    else run_binned_rf<32>(A, out, B, search_radius, cap2, seed, row_map, val_map, dist);
 }
 
 void launch_knn_brute(gs_group_s* A, gs_knn_s* out, gs_group_s* B, float cap2) {

@@ -81,7 +81,8 @@ template <int KMAX> struct TopK {
         t[0] = c[0] ? tt : t[0];
     }
     // Writes column `row` of the slot-major [k][nA] output: neighbour slots
-    // and distances (pads UINT32_MAX / FLT_MAX).
+    // and distances (pads UINT32_MAX / FLT_MAX); no distances when out_dist
+    // is NULL.
     __device__ __forceinline__ void store(uint32_t k, uint32_t nA, uint32_t row, uint32_t* __restrict__ out_idx,
                                           float* __restrict__ out_dist) const {
 #pragma unroll
@@ -90,7 +91,7 @@ template <int KMAX> struct TopK {
                 const size_t o = (size_t)(j - (KMAX - (int)k)) * nA + row;
                 const bool pad = t[j] == UINT32_MAX;
                 out_idx[o] = t[j];
-                out_dist[o] = pad ? FLT_MAX : sqrtf(__uint_as_float((uint32_t)(key[j] >> 32)));
+                if (out_dist) out_dist[o] = pad ? FLT_MAX : sqrtf(__uint_as_float((uint32_t)(key[j] >> 32)));
             }
         }
     }

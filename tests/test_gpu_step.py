@@ -76,6 +76,27 @@ def test_boids_fused_steps_other_k(gs, ctx, orc, k):
         h.close()
 
 
+def test_boids_multi_step_calls_bit_exact(gs, ctx, orc):
+    """Fused calls of 7 steps each (the search stores distances only in a
+    call's last step): after every call the KNN sets, their distances and the
+    whole state equal the oracle's after the same number of steps."""
+    st, p = scenarios.boids_sphere()
+    ref = st.copy()
+    A = gs.AgentGroup(ctx, st)
+    bins = gs.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
+    knn = gs.KNNData(ctx, st.n, p.k)
+    for call in range(4):
+        gs.stepBoids(A, knn, bins, p, 7)
+        for _ in range(7):
+            r_idx, r_dist = orc.boids_step(ref, p)
+        idx, dist = knn.download()
+        np.testing.assert_array_equal(idx, r_idx, err_msg=f"call {call}")
+        np.testing.assert_array_equal(dist, r_dist, err_msg=f"call {call}")
+        assert_state_equal(A.download(), ref, what=f"call {call}: ")
+    for h in (A, bins, knn):
+        h.close()
+
+
 def test_fused_equals_separate_calls(gs, ctx):
     st, p = scenarios.boids_sphere()
     A1, A2 = gs.AgentGroup(ctx, st), gs.AgentGroup(ctx, st)
