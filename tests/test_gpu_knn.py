@@ -173,6 +173,28 @@ def test_bin_database_matches_oracle(gs, ctx, orc):
         bins.close()
 
 
+def test_bin_database_across_lattices(gs, ctx, orc):
+    """One group rebuilt on lattices of 2 tiles of 2,048 cells, 2,560,000
+    cells (more than 1,024 tiles: the two-pass scan) and 256 tiles, back and
+    forth, two builds each: the single-pass scan's tile sums (kept between
+    builds) never carry counts over from another lattice or build."""
+    pos = scenarios.boids_sphere(n=4096)[0].pos()
+    origin, size = (-55,) * 3, (110,) * 3
+    A = _group(gs, ctx, pos)
+    for cells in [(16, 16, 16), (160, 160, 100), (16, 16, 16), (64, 64, 128), (160, 160, 100), (64, 64, 128)]:
+        print(f"lattice {cells}", flush=True)
+        g = orc.grid(origin, size, cells)
+        r_key, r_start, r_perm, _ = orc.bins_build(g, pos)
+        bins = gs.KNNBinData(ctx, cells, size, origin=origin)
+        for rep in range(2):
+            gs.updateKNNDatabase(A, bins)
+            key, start, perm, _ = A.download_bins(bins.num_cells)
+            np.testing.assert_array_equal(start, r_start, err_msg=f"{cells} build {rep}")
+            np.testing.assert_array_equal(perm, r_perm, err_msg=f"{cells} build {rep}")
+        bins.close()
+    A.close()
+
+
 @pytest.mark.parametrize("k,cells", [(7, (64, 64, 64)), (7, (64, 64, 192)), (16, (48, 48, 96))])
 def test_large_flock_subset_against_brute(gs, ctx, orc, k, cells):
     """2^20 agents in a 1000^3 box at k = 7 (64^3, and 64 x 64 x 192, the
