@@ -133,24 +133,26 @@ struct EnsKnnOut {
 };
 
 // KMAX: 7 for k = 7 (the lists of exactly 7 entries of the single-instance
-// search), else 8.
-template <int KMAX>
-__global__ __launch_bounds__(kBlock) void e_knn(EnsDev E, uint32_t N, uint32_t k, const float4* __restrict__ spos,
-                                                const uint32_t* __restrict__ start,
-                                                const uint32_t* __restrict__ new_to_old,
-                                                const uint32_t* __restrict__ old_to_new, EnsKnnOut out,
-                                                const float* __restrict__ wmx, const float* __restrict__ wmy,
-                                                const float* __restrict__ wmz, const float* __restrict__ ox,
-                                                const float* __restrict__ oy, const float* __restrict__ oz) {
+// search), else 8.  Two passes: PART 0 searches every agent's own group (all
+// blocks; the search alone, at the single-instance search's register budget
+// of five wavefronts per SIMD), PART 1 the crowd agents' other group, walls
+// and obstacles (crowd blocks only, from the `blocks` list).
+template <int KMAX, int PART>
+__global__ __launch_bounds__(kBlock, PART == 0 ? 5 : 1) void e_knn(
+    EnsDev E, const uint32_t* __restrict__ blocks, uint32_t N, uint32_t k, const float4* __restrict__ spos,
+    const uint32_t* __restrict__ start, const uint32_t* __restrict__ new_to_old,
+    const uint32_t* __restrict__ old_to_new, EnsKnnOut out, const float* __restrict__ wmx,
+    const float* __restrict__ wmy, const float* __restrict__ wmz, const float* __restrict__ ox,
+    const float* __restrict__ oy, const float* __restrict__ oz) {
     GS_FLAT_LDS
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t blk = PART == 0 ? xcd_block(blockIdx.x, gridDim.x) : blocks[xcd_block(blockIdx.x, gridDim.x)];
     const uint32_t s = blk * blockDim.x + threadIdx.x;
     const EnsPop& P = E.pops[E.block_pop[blk]];
     // Padding lanes stay through the searches (the wavefront completes its
     // lanes' inexact queries together) and return before the small-set part.
     const bool active = s < P.a0 + P.n;
     const float4 q = active ? spos[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-    {
+    if constexpr (PART == 0) {
         TopK<KMAX> h;
         h.init(k);
         const float thr = (out.seed_self && active) ? seed_bound<KMAX>(out.seed_self, k, N, new_to_old[s], N,
@@ -159,6 +161,7 @@ __global__ __launch_bounds__(kBlock) void e_knn(EnsDev E, uint32_t N, uint32_t k
         FlatQuery Q{q.x, q.y, q.z, thr, s, active};
         knn_search_flat<KMAX>(h, Q, P.g, N, spos, start + P.c0, flds);
         if (active) store_idx<KMAX>(h, k, N, s, out.self, out.out_map);
+        return;
     }
     if (P.kind != GS_SCENARIO_CHOKE) return;
     const EnsPop& O = E.pops[P.other];
@@ -524,13 +527,20 @@ static EnsDev ens_dev(gs_ensemble_s* e) {
 // binned into a sorted snapshot (the ensemble's bin buffers as scratch; its
 // own state and KNN containers are not touched) and the lists are written in
 // the fixed order (rows and neighbour values are fixed indices).  Seeds are
-// The ensemble search kernel for the ensemble's k (7-entry lists for k = 7).
+// The ensemble search kernels for the ensemble's k (7-entry lists for
+// k = 7): own groups over all blocks, then the crowds' other lists.
 #define GS_E_KNN(...)                                                                            \
     do {                                                                                         \
         if (e->k == 7)                                                                           \
-            e_knn<7><<<nblocks, kBlock, 0, st>>>(__VA_ARGS__);                                   \
+            e_knn<7, 0><<<nblocks, kBlock, 0, st>>>(E, nullptr, __VA_ARGS__);                    \
         else                                                                                     \
-            e_knn<8><<<nblocks, kBlock, 0, st>>>(__VA_ARGS__);                                   \
+            e_knn<8, 0><<<nblocks, kBlock, 0, st>>>(E, nullptr, __VA_ARGS__);                    \
+        GS_LAUNCHED("e_knn");                                                                    \
+        const unsigned ncb = (unsigned)e->blocks_choke.size();                                   \
+        if (ncb && e->k == 7)                                                                    \
+            e_knn<7, 1><<<ncb, kBlock, 0, st>>>(E, e->d_blocks_choke.p, __VA_ARGS__);            \
+        else if (ncb)                                                                            \
+            e_knn<8, 1><<<ncb, kBlock, 0, st>>>(E, e->d_blocks_choke.p, __VA_ARGS__);            \
     } while (0)
 
 // previous lists in the same fixed order (or NULL).
@@ -561,7 +571,7 @@ void ens_search_fixed(gs_ensemble_s* e, float* px, float* py, float* pz, const u
     ko.seed_self = seed_self;
     ko.seed_other = seed_other;
     ko.out_map = e->new_to_old.p;
-    GS_E_KNN(E, N, e->k, e->spos.p, e->start.p, e->new_to_old.p, e->old_to_new.p, ko,
+    GS_E_KNN(N, e->k, e->spos.p, e->start.p, e->new_to_old.p, e->old_to_new.p, ko,
                                       e->walls[9].p, e->walls[10].p, e->walls[11].p, e->obs[0].p, e->obs[1].p,
                                       e->obs[2].p);
     GS_LAUNCHED("e_knn");
@@ -605,7 +615,7 @@ void ens_step(gs_ensemble_s* e) {
     ko.seed_self = e->seeded ? e->knn_self[e->kcur].p : nullptr;
     ko.seed_other = e->seeded ? e->knn_other[e->kcur].p : nullptr;
     ko.out_map = nullptr;
-    GS_E_KNN(E, N, e->k, e->spos.p, e->start.p, e->new_to_old.p, e->old_to_new.p, ko,
+    GS_E_KNN(N, e->k, e->spos.p, e->start.p, e->new_to_old.p, e->old_to_new.p, ko,
                                       e->walls[9].p, e->walls[10].p, e->walls[11].p, e->obs[0].p, e->obs[1].p,
                                       e->obs[2].p);
     GS_LAUNCHED("e_knn");

@@ -68,8 +68,10 @@ BUDGETS = [
     (r"k_knn_binnedILi8ELi2E", 136, 0),
     (r"k_knn_binnedILi16ELi1E", 128, 300),          # 4 wavefronts per SIMD (fewer spills were slower)
     (r"k_flock_updateILi7E", 96, 0),
-    (r"e_knnILi7E", 152, 0),
-    (r"e_knnILi8E", 152, 0),
+    (r"e_knnILi7ELi0E", 96, 128),                    # ensemble own-group search: 5 wavefronts per SIMD
+    (r"e_knnILi8ELi0E", 96, 144),
+    (r"e_knnILi7ELi1E", 152, 0),                     # ensemble crowd lists (other group, walls, obstacles)
+    (r"e_knnILi8ELi1E", 152, 0),
     (r"k_knn_split_part", 400, 0),
     (r"k_knn_split_merge", 128, 0),
     (r"c4_backward", 128, 0),
