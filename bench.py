@@ -585,6 +585,109 @@ def cpu_baseline(st, p, seconds: float):
     }
 
 
+def candidate_stats(cells, k: int = 7, warmup: int = 20, sample: int = 1 << 16) -> dict:
+    """What the k = 7 flat search scans at the bench state: the 2^20-agent box
+    after `warmup` fused steps on the GPU, then on the CPU (numpy) for
+    `sample` consecutive queries in cell order (sample / 64 wavefronts), the
+    search's own bound -- the largest current d2 of the previous step's k
+    neighbours, widened by 2e-5 -- and its runs: the 3 x 3 columns around the
+    query's cell, each column's z-run trimmed to the bound's sphere and capped
+    to zr cells each way.  Reports candidates per query (mean and the mean of
+    each wavefront's maximum, which sets the wavefront's scan length),
+    iterations of the scan loop (two chunks of four slots of one run each)
+    per wavefront, candidates within the bound (the deferred entries) and the
+    share of wavefronts with a lane holding more than eight of them (those
+    merge by insertion instead of the sorting network)."""
+    import numpy as np
+
+    import gpusteer_amd as gs
+    from gpusteer_amd import scenarios
+
+    st, p = scenarios.boids_box(n=1 << 20, k=k, cells=64)
+    p.grid_cells = tuple(cells)
+    p.search_radius = 1
+    with gs.Context(0) as ctx:
+        A = gs.AgentGroup(ctx, st)
+        bins = gs.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
+        knn = gs.KNNData(ctx, st.n, k)
+        gs.stepBoids(A, knn, bins, p, warmup)
+        ctx.synchronize()
+        cur = A.download()
+        seed, _ = knn.download()
+        for h in (knn, A, bins):
+            h.close()
+    out = {"cells": list(cells), "warmup_steps": warmup}
+    out.update(scan_stats(cur.pos(), seed, cells, p.grid_size, sample))
+    return out
+
+
+def scan_stats(pos, seed, cells, grid_size, sample: int = 1 << 16) -> dict:
+    """The candidate statistics of candidate_stats for positions `pos` [n][3]
+    (agent order, a lattice with origin 0) and previous lists `seed` [k][n]
+    (agent indices)."""
+    import numpy as np
+
+    size = np.array(grid_size, np.float64)
+    D = np.array(cells, np.int64)
+    hcell = size / D
+    P = np.asarray(pos, np.float64)
+    P = np.where(P < 0.0, size, np.where(P > size, 0.0, P))  # the step's face wrap
+    c = np.clip(np.floor(P / hcell).astype(np.int64), 0, D - 1)
+    key = (c[:, 0] * D[1] + c[:, 1]) * D[2] + c[:, 2]
+    order = np.argsort(key, kind="stable")
+    start = np.concatenate([[0], np.cumsum(np.bincount(key, minlength=int(D.prod())))])
+    s0 = (P.shape[0] // 3) // 64 * 64
+    q = order[s0:s0 + sample]
+    qp, qc = P[q], c[q]
+    sp = P[seed[:, q].astype(np.int64)]  # [k][sample][3]
+    bound = np.max(np.sum((sp - qp[None]) ** 2, axis=2), axis=0) * (1.0 + 2e-5)
+    zr = max(1, min(8, int(np.ceil(hcell[0] / hcell[2] * 0.999))))
+    zlo, zhi = np.maximum(qc[:, 2] - zr, 0), np.minimum(qc[:, 2] + zr, D[2] - 1)
+    n_cand = np.zeros(sample, np.int64)
+    chunks = np.zeros(sample, np.int64)
+    within = np.zeros(sample, np.int64)
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            ix, iy = qc[:, 0] + dx, qc[:, 1] + dy
+            ok = (ix >= 0) & (ix < D[0]) & (iy >= 0) & (iy < D[1])
+            gx = np.where(dx < 0, qp[:, 0] - (ix + 1) * hcell[0], np.where(dx > 0, ix * hcell[0] - qp[:, 0], 0.0))
+            gy = np.where(dy < 0, qp[:, 1] - (iy + 1) * hcell[1], np.where(dy > 0, iy * hcell[1] - qp[:, 1], 0.0))
+            gxy = np.maximum(gx, 0.0) ** 2 + np.maximum(gy, 0.0) ** 2
+            ok &= gxy <= bound
+            half = np.sqrt(np.maximum(bound - gxy, 0.0)) * (1.0 + 2e-5)
+            z0 = np.maximum(np.clip(np.floor((qp[:, 2] - half) / hcell[2]), 0, D[2] - 1).astype(np.int64), zlo)
+            z1 = np.minimum(np.clip(np.floor((qp[:, 2] + half) / hcell[2]), 0, D[2] - 1).astype(np.int64), zhi)
+            ok &= z0 <= z1
+            base = (np.clip(ix, 0, D[0] - 1) * D[1] + np.clip(iy, 0, D[1] - 1)) * D[2]
+            a = np.where(ok, start[base + z0], 0)
+            b = np.where(ok, start[np.minimum(base + z1 + 1, start.size - 1)], 0)
+            m = b - a
+            n_cand += m
+            chunks += (m + 3) // 4
+            rows = np.repeat(np.arange(sample), m)
+            slots = np.repeat(a, m) + (np.arange(m.sum()) - np.repeat(np.cumsum(m) - m, m))
+            ids = order[slots]
+            d2 = np.sum((P[ids] - qp[rows]) ** 2, axis=1)
+            within += np.bincount(rows, weights=(d2 <= bound[rows]) & (ids != q[rows]), minlength=sample).astype(
+                np.int64)
+    n_cand -= 1  # the query itself (its own run is split around it; counted here as one run)
+    waves = sample // 64
+    wmax = n_cand.reshape(waves, 64).max(axis=1)
+    iters = ((chunks + 1) // 2).reshape(waves, 64).max(axis=1)
+    wb = within.reshape(waves, 64).max(axis=1)
+    return {
+        "sample_queries": sample,
+        "candidates_per_query": float(n_cand.mean()),
+        "wave_max_candidates": float(wmax.mean()),
+        "scan_iterations_per_wave": float(iters.mean()),
+        "within_bound_per_query": float(within.mean()),
+        "waves_with_lane_over_8_within": float((wb > 8).mean()),
+        # Median: a query whose previous neighbour was wrapped to the far face
+        # has a bound as large as the box.
+        "sphere_points_median": float(np.median(4.0 / 3.0 * np.pi * bound ** 1.5 * P.shape[0] / size.prod())),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -608,7 +711,7 @@ def main() -> int:
     ap.add_argument("--no-k16", action="store_true")
     ap.add_argument("--plugin-steps", type=int, default=100)
     ap.add_argument("--no-plugin", action="store_true")
-    ap.add_argument("--only", choices=("ensemble", "adjoint", "plugin"),
+    ap.add_argument("--only", choices=("ensemble", "adjoint", "plugin", "candidates"),
                     help="run only that leg and print its JSON object (for per-leg profiles)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 512 MiB device copy (the counter calibration for the traffic figure)")
@@ -627,6 +730,13 @@ def main() -> int:
         return 0
     if args.only == "plugin":
         print(json.dumps(plugin_bench(args.plugin_steps, 5)), flush=True)
+        return 0
+    if args.only == "candidates":
+        cl = [int(x) for x in str(args.cells).split(",")]
+        cells = (cl * 3)[:3] if len(cl) == 1 else cl
+        if cells[0] <= 0:
+            cells = [64, 64, 128]
+        print(json.dumps(candidate_stats(cells, args.k, max(args.warmup, 2))), flush=True)
         return 0
     if args.only == "adjoint":
         print(json.dumps(calib_bench(args.calib_scenarios, args.calib_agents, args.calib_steps, 10)), flush=True)
