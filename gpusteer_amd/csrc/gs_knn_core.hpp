@@ -534,6 +534,47 @@ __device__ __forceinline__ void flat_sort_fresh(TopK<7>& h, int cnt, const FlatL
     }
 }
 
+// The first merge of a 16-entry list with k = 16 (no sentinel entries): the
+// buffer's up to sixteen entries become the list, sorted by Batcher's
+// odd-even merge network for sixteen inputs (63 compare-exchanges, applied to
+// the list's own registers) instead of up to 16 x 15 of insertion.  Missing
+// entries, the query's own slot and keys beyond the bound sort last as empty
+// entries.
+constexpr int kNet16[63][2] = {
+    {0, 1},   {2, 3},   {0, 2},   {1, 3},   {1, 2},   {4, 5},   {6, 7},   {4, 6},   {5, 7},   {5, 6},
+    {0, 4},   {2, 6},   {2, 4},   {1, 5},   {3, 7},   {3, 5},   {1, 2},   {3, 4},   {5, 6},   {8, 9},
+    {10, 11}, {8, 10},  {9, 11},  {9, 10},  {12, 13}, {14, 15}, {12, 14}, {13, 15}, {13, 14}, {8, 12},
+    {10, 14}, {10, 12}, {9, 13},  {11, 15}, {11, 13}, {9, 10},  {11, 12}, {13, 14}, {0, 8},   {4, 12},
+    {4, 8},   {2, 10},  {6, 14},  {6, 10},  {2, 4},   {6, 8},   {10, 12}, {1, 9},   {5, 13},  {5, 9},
+    {3, 11},  {7, 15},  {7, 11},  {3, 5},   {7, 9},   {11, 13}, {1, 2},   {3, 4},   {5, 6},   {7, 8},
+    {9, 10},  {11, 12}, {13, 14}};
+template <int STRIDE = kBlock>
+__device__ __forceinline__ void flat_sort_fresh16(TopK<16>& h, int cnt, const FlatLds& L, __amdgpu_buffer_rsrc_t rb,
+                                                  float qx, float qy, float qz, uint32_t self_slot, float thr) {
+    static_assert(kFlatBuf == 16, "the buffer fills the list exactly");
+#pragma unroll
+    for (int h4 = 0; h4 < 16; h4 += 4) {
+        uint32_t sl[4];
+        float4 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            // Entries past cnt hold stale slots: their loads are range-checked
+            // (zeros past the end of B) and their keys are discarded below.
+            sl[u] = L.slot[(h4 + u) * STRIDE];
+            p[u] = ld_pos(rb, sl[u] * 16u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float d2 = sqdist4(qx, qy, qz, p[u]);
+            const bool use = h4 + u < cnt && sl[u] != self_slot && d2 <= thr;
+            h.key[h4 + u] = use ? make_key(d2, id_of(p[u])) : kKeyEmpty;
+            h.t[h4 + u] = use ? sl[u] : UINT32_MAX;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 63; ++c) sort_cx(h.key[kNet16[c][0]], h.t[kNet16[c][0]], h.key[kNet16[c][1]], h.t[kNet16[c][1]]);
+}
+
 // ---------------------------------------------------------------------------
 // Cooperative completion of one lane's query by its whole wavefront.
 //
@@ -821,7 +862,17 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, FlatQuery& Q, con
                 break;
             }
         }
-        flat_flush<KMAX, STRIDE>(h, cnt, L, rb, qx, qy, qz, self_slot, thr);
+        // k = 16 (key[0] is a sentinel 0 for k < 16): the empty list takes
+        // the whole first buffer, sorted at once.
+        bool merged = false;
+        if constexpr (KMAX == 16) {
+            if (fresh && h.key[0] != 0ull) {
+                flat_sort_fresh16<STRIDE>(h, cnt, L, rb, qx, qy, qz, self_slot, thr);
+                cnt = 0;
+                merged = true;
+            }
+        }
+        if (!merged) flat_flush<KMAX, STRIDE>(h, cnt, L, rb, qx, qy, qz, self_slot, thr);
         fresh = false;
         wa = L.slot_addr;
         lim = fminf(thr, h.worst());
