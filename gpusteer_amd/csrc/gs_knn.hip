@@ -56,7 +56,7 @@ __global__ __launch_bounds__(BS, RF == 1 ? (KMAX <= 8 ? 5 : (KMAX <= 16 ? 4 : 1)
                 Q.thr = fminf(cap2, (seed && active) ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB,
                                                                         val_map, q.x, q.y, q.z, bpos)
                                                      : INFINITY);
-                GS_FLAT_LDS_N(BS)
+                GS_FLAT_LDS_NK(BS, KMAX)
                 knn_search_flat<KMAX, BS>(h, Q, g, nB, bpos, bstart, flds);
             }
             if (active) h.store(k, nA, s, out_idx, out_dist);

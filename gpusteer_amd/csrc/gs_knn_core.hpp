@@ -408,31 +408,37 @@ __device__ __forceinline__ void knn_search(TopK<KMAX>& h, float thr, float qx, f
 // advances -- and its own slot passes like any other (d2 = 0); the merge
 // recomputes each buffered candidate's key from its position, drops the
 // query's own slot and inserts the rest.  The wavefront merges once any lane
-// holds more than kFlatBuf - 8 entries, so an iteration can never overflow
+// holds more than flat_buf - 8 entries, so an iteration can never overflow
 // the buffer.
 constexpr int kFlatBuf = 16;
+// 16-entry lists defer 24: the first merge then mostly comes after the scan
+// and takes the whole batch at once (their kernel holds 4 wavefronts per
+// SIMD by its registers, which the larger pool still fits).
+template <int KMAX> constexpr int flat_buf() { return KMAX == 16 ? 24 : kFlatBuf; }
 constexpr int kFlatRuns = 10;  // nine columns, one of them split at the query
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 // One pool of kFlatPool words per lane, strided by the block size: the deferral
 // buffer, the run list (first slots, then 16-bit lengths), and after the scan
 // the cooperative shells' candidate triples.
 constexpr int kFlatPool = kFlatBuf + kFlatRuns + (kFlatRuns + 1) / 2;
+template <int KMAX> constexpr int flat_pool() { return flat_buf<KMAX>() + kFlatRuns + (kFlatRuns + 1) / 2; }
 constexpr int kCoopEntries = kFlatPool / 3;
 struct FlatLds {
     uint32_t* pool;      // this lane's column
-    uint32_t* slot;      // [kFlatBuf][block size]: deferred candidate slots
+    uint32_t* slot;      // [flat_buf][block size]: deferred candidate slots
     uint32_t* run_t;     // [kFlatRuns][block size]: first slot of a non-empty run
     uint16_t* run_n;     // [kFlatRuns][block size]: its length
     uint32_t slot_addr;  // LDS byte address of the lane's first buffer entry
 };
 // For a block of BS threads: lane columns strided by BS (the search
 // functions' STRIDE template argument).
-#define GS_FLAT_LDS_N(BS)                                                                               \
-    __shared__ uint32_t lds_flat_pool[kFlatPool * (BS)];                                               \
+#define GS_FLAT_LDS_NK(BS, KMAX)                                                                        \
+    __shared__ uint32_t lds_flat_pool[flat_pool<KMAX>() * (BS)];                                       \
     const FlatLds flds{lds_flat_pool + threadIdx.x, lds_flat_pool + threadIdx.x,                        \
-                       lds_flat_pool + kFlatBuf * (BS) + threadIdx.x,                                   \
-                       (uint16_t*)(lds_flat_pool + (kFlatBuf + kFlatRuns) * (BS)) + threadIdx.x,        \
+                       lds_flat_pool + flat_buf<KMAX>() * (BS) + threadIdx.x,                           \
+                       (uint16_t*)(lds_flat_pool + (flat_buf<KMAX>() + kFlatRuns) * (BS)) + threadIdx.x, \
                        (uint32_t)(size_t)(lds_u32*)(lds_flat_pool + threadIdx.x)};
+#define GS_FLAT_LDS_N(BS) GS_FLAT_LDS_NK(BS, 8)
 #define GS_FLAT_LDS GS_FLAT_LDS_N(kBlock)
 
 // Buffer resource over the float4 snapshot of B (raw buffer, byte offsets,
@@ -458,8 +464,10 @@ __device__ __forceinline__ int axis_cell_bound(float p, float o, float r, int d)
 
 template <int KMAX, int STRIDE = kBlock>
 __device__ __forceinline__ void flat_flush(TopK<KMAX>& h, int& cnt, const FlatLds& L, __amdgpu_buffer_rsrc_t rb,
-                                           float qx, float qy, float qz, uint32_t self_slot, float thr) {
-    for (int e0 = 0; e0 < kFlatBuf; e0 += 4) {
+                                           float qx, float qy, float qz, uint32_t self_slot, float thr,
+                                           int first = 0) {
+    // Entries from `first` on (the ones before were merged otherwise).
+    for (int e0 = first; e0 < flat_buf<KMAX>(); e0 += 4) {
         if (!__any(e0 < cnt)) break;
         uint32_t sl[4];
         float4 p[4];
@@ -535,11 +543,11 @@ __device__ __forceinline__ void flat_sort_fresh(TopK<7>& h, int cnt, const FlatL
 }
 
 // The first merge of a 16-entry list with k = 16 (no sentinel entries): the
-// buffer's up to sixteen entries become the list, sorted by Batcher's
+// buffer's first sixteen entries become the list, sorted by Batcher's
 // odd-even merge network for sixteen inputs (63 compare-exchanges, applied to
-// the list's own registers) instead of up to 16 x 15 of insertion.  Missing
-// entries, the query's own slot and keys beyond the bound sort last as empty
-// entries.
+// the list's own registers) instead of up to 16 x 15 of insertion; the caller
+// inserts the entries past the sixteenth.  Missing entries, the query's own
+// slot and keys beyond the bound sort last as empty entries.
 constexpr int kNet16[63][2] = {
     {0, 1},   {2, 3},   {0, 2},   {1, 3},   {1, 2},   {4, 5},   {6, 7},   {4, 6},   {5, 7},   {5, 6},
     {0, 4},   {2, 6},   {2, 4},   {1, 5},   {3, 7},   {3, 5},   {1, 2},   {3, 4},   {5, 6},   {8, 9},
@@ -551,7 +559,7 @@ constexpr int kNet16[63][2] = {
 template <int STRIDE = kBlock>
 __device__ __forceinline__ void flat_sort_fresh16(TopK<16>& h, int cnt, const FlatLds& L, __amdgpu_buffer_rsrc_t rb,
                                                   float qx, float qy, float qz, uint32_t self_slot, float thr) {
-    static_assert(kFlatBuf == 16, "the buffer fills the list exactly");
+    static_assert(flat_buf<16>() >= 16, "the buffer fills the list");
 #pragma unroll
     for (int h4 = 0; h4 < 16; h4 += 4) {
         uint32_t sl[4];
@@ -815,7 +823,7 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, FlatQuery& Q, con
     // untouched (merges happen between scan segments).
     constexpr uint32_t kEntry = STRIDE * 4u;
     uint32_t wa = L.slot_addr;
-    const uint32_t wfull = L.slot_addr + (kFlatBuf - 8) * kEntry;
+    const uint32_t wfull = L.slot_addr + (flat_buf<KMAX>() - 8) * kEntry;
     // The scan only filters: a candidate enters the buffer when its fused
     // d2 is within the bound widened by `grow` (more than the few ulps by
     // which the fused form can differ from the exact one), and the merge
@@ -863,12 +871,12 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, FlatQuery& Q, con
             }
         }
         // k = 16 (key[0] is a sentinel 0 for k < 16): the empty list takes
-        // the whole first buffer, sorted at once.
+        // the first sixteen entries, sorted at once, and the rest by insertion.
         bool merged = false;
         if constexpr (KMAX == 16) {
             if (fresh && h.key[0] != 0ull) {
                 flat_sort_fresh16<STRIDE>(h, cnt, L, rb, qx, qy, qz, self_slot, thr);
-                cnt = 0;
+                flat_flush<KMAX, STRIDE>(h, cnt, L, rb, qx, qy, qz, self_slot, thr, 16);
                 merged = true;
             }
         }
