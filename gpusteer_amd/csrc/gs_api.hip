@@ -321,6 +321,10 @@ static void visit(gs_group_s* g, SnapIO& io) {
     for (auto* b : u) io.buf(*b);
     io.buf(g->spos);
     io.buf(g->sfwd);
+    io.val(g->bbox_cur);
+    io.val(g->bbox_wanted);
+    io.val(g->bbox_fresh);
+    for (auto& b : g->bbox) io.buf(b);
 }
 
 static void visit(gs_knn_s* k, SnapIO& io) {
@@ -605,6 +609,14 @@ int gs_group_create(gs_context ctx, int kind, gs_group* out) {
         g->live = kind == GS_KIND_AGENTS ? kFieldsAgents : kind == GS_KIND_OBSTACLES ? kFieldsObstacles : kFieldsWalls;
         g->membership_epoch = next_epoch();
         g->order_epoch = next_epoch();
+        // The bounding-box pair of capped searches (12 words, allocated here
+        // so that no later call allocates for it), both empty.
+        use_device(ctx);
+        for (auto& b : g->bbox) {
+            b.reserve(6);
+            launch_fill_u32(b.p, 3, kBoxEmptyMin, ctx->stream);
+            launch_fill_u32(b.p + 3, 3, kBoxEmptyMax, ctx->stream);
+        }
         reg_add(g);
         *out = g;
     });
@@ -619,6 +631,8 @@ static void release_group(gs_group_s* g) {
                                  &g->perm_tmp,   &g->id_tmp, &g->start,      &g->count,      &g->flag};
     for (auto* b : u) b->release();
     g->spos.release();
+    g->sfwd.release();
+    for (auto& b : g->bbox) b.release();
 }
 
 int gs_group_destroy(gs_group g) {
@@ -1130,6 +1144,9 @@ static void find_knn(gs_group A, gs_knn out, gs_bins bins, gs_group B, uint32_t 
         fail(GS_ERR_NOT_BINNED, "groups were binned with another lattice");
     const float cap2 = cap2_of(max_distance);
     use_device(A->ctx);
+    // B's builds from now on keep its bounding box, which lets the binned
+    // search skip queries the cap cannot reach it from.
+    if (std::isfinite(cap2)) B->bbox_wanted = true;
     if (!(flags & GS_KNN_BINNED) && prefer_split(A, B, bins->g, bins->cells, max_distance)) {
         const Seed sd = seed_for(out, A, B, flags);
         launch_knn_split(A, out, B, cap2, sd.seed, sd.row_map, sd.val_map);
@@ -1731,6 +1748,7 @@ int gs_ensemble_create(gs_context ctx, uint32_t ns, const gs_scenario* sc, uint3
             ens_upload_geometry(e);
             GS_HIP(hipStreamSynchronize(ctx->stream));
         } catch (...) {
+            ens_release(e);
             delete e;
             throw;
         }
@@ -1747,6 +1765,7 @@ int gs_ensemble_destroy(gs_ensemble e) {
         use_device(e->ctx);
         (void)hipStreamSynchronize(e->ctx->stream);
         release_ensemble(e);
+        ens_release(e);
         delete e;
     });
 }

@@ -69,14 +69,16 @@ __global__ __launch_bounds__(kBlock) void k_bin_key(uint32_t n, float* __restric
                                                     float* __restrict__ pz, GridDev g, uint32_t* __restrict__ key,
                                                     uint32_t* __restrict__ rank, uint32_t* __restrict__ count,
                                                     int wrap_shape, float wcx, float wcy, float wcz, float wsx,
-                                                    float wsy, float wsz) {
+                                                    float wsy, float wsz, uint32_t* __restrict__ box_acc,
+                                                    uint32_t* __restrict__ box_clear) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = i < n;
     const int lane = threadIdx.x & 63;
     // Inactive lanes get a key no cell has, so they form segments of their own.
     uint32_t c = 0xFFFFFFC0u | (uint32_t)lane;
+    float x = 0.f, y = 0.f, z = 0.f;
     if (active) {
-        float x = px[i], y = py[i], z = pz[i];
+        x = px[i], y = py[i], z = pz[i];
         if (wrap_shape == GS_WRAP_BOX) {
             // WrapWorldKernel (CUDA/WrapWorldCUDAKernel.cu:25-35), all three axes.
             const float hx = wsx * 0.5f, hy = wsy * 0.5f, hz = wsz * 0.5f;
@@ -109,6 +111,12 @@ __global__ __launch_bounds__(kBlock) void k_bin_key(uint32_t n, float* __restric
         key[i] = c;
     }
     count_and_rank(active, c, i, count, rank);
+    if (box_acc) {
+        // The bounding box of the positions this build snapshots (after the
+        // wrap); the other buffer is cleared for the next build.
+        if (blockIdx.x == 0 && threadIdx.x < 6) box_clear[threadIdx.x] = threadIdx.x < 3 ? kBoxEmptyMin : kBoxEmptyMax;
+        box_accumulate(active, x, y, z, box_acc);
+    }
 }
 
 // Wavefront-then-block exclusive scan of one value per thread.
@@ -315,10 +323,19 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
     float wcv[3] = {0, 0, 0}, wsv[3] = {0, 0, 0};
     if (wc) memcpy(wcv, wc, sizeof wcv);
     if (ws) memcpy(wsv, ws, sizeof wsv);
+    // Bounding box of this build's positions, once a capped search targets
+    // the group.
+    uint32_t *box_acc = nullptr, *box_clear = nullptr;
+    if (g->bbox_wanted && g->bbox[0].p && g->bbox[1].p && n > 0) {
+        g->bbox_cur ^= 1;
+        box_acc = g->bbox[g->bbox_cur].p;
+        box_clear = g->bbox[g->bbox_cur ^ 1].p;
+    }
+    g->bbox_fresh = box_acc != nullptr;
     if (n > 0) {
         k_bin_key<<<blocks_for(n), kBlock, 0, st>>>(n, g->F(0), g->F(1), g->F(2), b->g, g->key.p, g->rank.p,
                                                     g->count.p, wrap_shape, wcv[0], wcv[1], wcv[2], wsv[0], wsv[1],
-                                                    wsv[2]);
+                                                    wsv[2], box_acc, box_clear);
         GS_LAUNCHED("k_bin_key");
     }
     launch_scan_counts(g->count.p, g->start.p, C, st);

@@ -43,14 +43,17 @@ struct EnsDev {
 
 __global__ __launch_bounds__(kBlock) void e_bin_key(EnsDev E, float* __restrict__ px, float* __restrict__ py,
                                                     float* __restrict__ pz, uint32_t* __restrict__ key,
-                                                    uint32_t* __restrict__ rank, uint32_t* __restrict__ count) {
+                                                    uint32_t* __restrict__ rank, uint32_t* __restrict__ count,
+                                                    uint32_t* __restrict__ box_acc, uint32_t* __restrict__ box_clear,
+                                                    uint32_t nbox) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = blk * blockDim.x + threadIdx.x;
     const EnsPop& P = E.pops[E.block_pop[blk]];
     const bool agent = s < P.a0 + P.n;
     uint32_t c = P.c0 + P.cells;  // padding slots: the population's extra cell
+    float x = 0.f, y = 0.f, z = 0.f;
     if (agent) {
-        float x = px[s], y = py[s], z = pz[s];
+        x = px[s], y = py[s], z = pz[s];
         const EnsScen& S = E.scens[P.scen];
         if (S.wrap_shape == GS_WRAP_BOX) {
             const float hx = S.ws[0] * 0.5f, hy = S.ws[1] * 0.5f, hz = S.ws[2] * 0.5f;
@@ -83,6 +86,12 @@ __global__ __launch_bounds__(kBlock) void e_bin_key(EnsDev E, float* __restrict_
     }
     key[s] = c;
     count_and_rank(true, c, s, count, rank);
+    // Crowd populations' bounding boxes (block-uniform branches: a block
+    // belongs to one population); the other buffer is cleared for the next
+    // build.
+    if (blockIdx.x == 0)
+        for (uint32_t j = threadIdx.x; j < nbox; j += kBlock) box_clear[j] = j % 6 < 3 ? kBoxEmptyMin : kBoxEmptyMax;
+    if (P.kind == GS_SCENARIO_CHOKE) box_accumulate(agent, x, y, z, box_acc + 6 * (size_t)E.block_pop[blk]);
 }
 
 // Brute-force top-k over a scenario's few walls (midpoints) or obstacles
@@ -130,6 +139,8 @@ struct EnsKnnOut {
     // both are translated through it (sorted slot -> fixed index), for state
     // kept in a fixed order with a sorted snapshot for the search.
     const uint32_t* out_map;
+    // The populations' bounding boxes of this build (6 words each).
+    const uint32_t* box;
 };
 
 // KMAX: 7 for k = 7 (the lists of exactly 7 entries of the single-instance
@@ -168,11 +179,13 @@ __global__ __launch_bounds__(kBlock, PART == 0 ? 5 : 1) void e_knn(
     if (O.n > 0) {
         TopK<KMAX> h;
         h.init(k);
-        const float thr = fminf(E.scens[P.scen].other_cap2,
-                                (out.seed_other && active)
-                                    ? seed_bound<KMAX>(out.seed_other, k, N, new_to_old[s], N, old_to_new, q.x, q.y,
-                                                    q.z, spos)
-                                    : INFINITY);
+        const float cap2 = E.scens[P.scen].other_cap2;
+        float thr = fminf(cap2, (out.seed_other && active) ? seed_bound<KMAX>(out.seed_other, k, N, new_to_old[s], N,
+                                                                           old_to_new, q.x, q.y, q.z, spos)
+                                                        : INFINITY);
+        // A query the cap cannot reach the other group's box from has no
+        // candidate.
+        if (active && box_beyond(out.box + 6 * (size_t)P.other, q.x, q.y, q.z, cap2)) thr = -1.f;
         FlatQuery Q{q.x, q.y, q.z, thr, UINT32_MAX, active};
         knn_search_flat<KMAX>(h, Q, O.g, N, spos, start + O.c0, flds);
         if (active) store_idx<KMAX>(h, k, N, s, out.other, out.out_map);
@@ -433,6 +446,27 @@ void ens_init(gs_ensemble_s* e, uint32_t ns, const gs_scenario* sc, uint32_t k,
     e->block_pop_h = block_pop;
 }
 
+void ens_release(gs_ensemble_s* e) {
+    e->d_pops.release();
+    e->d_scens.release();
+    for (auto* b : {&e->d_block_pop, &e->d_blocks_boids, &e->d_blocks_g0, &e->d_blocks_g1, &e->d_blocks_choke,
+                    &e->knn_walls, &e->knn_obs, &e->key, &e->rank, &e->perm_tmp, &e->id_tmp, &e->new_to_old,
+                    &e->old_to_new, &e->count, &e->start})
+        b->release();
+    for (auto& f : e->fb)
+        for (auto& b : f) b.release();
+    for (int b = 0; b < 2; ++b) {
+        e->bitsb[b].release();
+        e->idb[b].release();
+        e->knn_self[b].release();
+        e->knn_other[b].release();
+        e->box[b].release();
+    }
+    e->spos.release();
+    for (auto& b : e->walls) b.release();
+    for (auto& b : e->obs) b.release();
+}
+
 void ens_alloc(gs_ensemble_s* e) {
     hipStream_t st = e->ctx->stream;
     const size_t N = e->N ? e->N : 1;
@@ -464,6 +498,14 @@ void ens_alloc(gs_ensemble_s* e) {
     e->count.reserve(e->C ? e->C : 1);
     e->start.reserve((size_t)e->C + 1 + ntiles);
     launch_fill_u32(e->count.p, e->C, 0u, st);
+    const uint32_t nbox = 6 * (uint32_t)std::max<size_t>(e->pops.size(), 1);
+    for (auto& b : e->box) {
+        b.reserve(nbox);
+        for (uint32_t j = 0; j < nbox; j += 6) {
+            launch_fill_u32(b.p + j, 3, kBoxEmptyMin, st);
+            launch_fill_u32(b.p + j + 3, 3, kBoxEmptyMax, st);
+        }
+    }
     h2d_vec(e->d_pops, e->pops, st);
     h2d_vec(e->d_scens, e->scens_host, st);
     h2d_vec(e->d_block_pop, e->block_pop_h, st);
@@ -552,7 +594,10 @@ void ens_search_fixed(gs_ensemble_s* e, float* px, float* py, float* pz, const u
     const uint32_t N = e->N, nblocks = N / kBlock;
     if (N == 0) return;
     const EnsDev E = ens_dev(e);
-    e_bin_key<<<nblocks, kBlock, 0, st>>>(E, px, py, pz, e->key.p, e->rank.p, e->count.p);
+    e->box_cur ^= 1;
+    const uint32_t nbox = 6 * (uint32_t)e->pops.size();
+    e_bin_key<<<nblocks, kBlock, 0, st>>>(E, px, py, pz, e->key.p, e->rank.p, e->count.p, e->box[e->box_cur].p,
+                                          e->box[e->box_cur ^ 1].p, nbox);
     GS_LAUNCHED("e_bin_key");
     launch_scan_counts(e->count.p, e->start.p, e->C, st);
     FieldSet fs{};
@@ -571,6 +616,7 @@ void ens_search_fixed(gs_ensemble_s* e, float* px, float* py, float* pz, const u
     ko.seed_self = seed_self;
     ko.seed_other = seed_other;
     ko.out_map = e->new_to_old.p;
+    ko.box = e->box[e->box_cur].p;
     GS_E_KNN(N, e->k, e->spos.p, e->start.p, e->new_to_old.p, e->old_to_new.p, ko,
                                       e->walls[9].p, e->walls[10].p, e->walls[11].p, e->obs[0].p, e->obs[1].p,
                                       e->obs[2].p);
@@ -583,7 +629,10 @@ void ens_step(gs_ensemble_s* e) {
     const uint32_t N = e->N, nblocks = N / kBlock;
     if (N == 0) return;
     const EnsDev E = ens_dev(e);
-    e_bin_key<<<nblocks, kBlock, 0, st>>>(E, e->F(0), e->F(1), e->F(2), e->key.p, e->rank.p, e->count.p);
+    e->box_cur ^= 1;
+    const uint32_t nbox = 6 * (uint32_t)e->pops.size();
+    e_bin_key<<<nblocks, kBlock, 0, st>>>(E, e->F(0), e->F(1), e->F(2), e->key.p, e->rank.p, e->count.p,
+                                          e->box[e->box_cur].p, e->box[e->box_cur ^ 1].p, nbox);
     GS_LAUNCHED("e_bin_key");
     launch_scan_counts(e->count.p, e->start.p, e->C, st);
     // Moved: every field but the frozen ones (positions first, as the
@@ -615,6 +664,7 @@ void ens_step(gs_ensemble_s* e) {
     ko.seed_self = e->seeded ? e->knn_self[e->kcur].p : nullptr;
     ko.seed_other = e->seeded ? e->knn_other[e->kcur].p : nullptr;
     ko.out_map = nullptr;
+    ko.box = e->box[e->box_cur].p;
     GS_E_KNN(N, e->k, e->spos.p, e->start.p, e->new_to_old.p, e->old_to_new.p, ko,
                                       e->walls[9].p, e->walls[10].p, e->walls[11].p, e->obs[0].p, e->obs[1].p,
                                       e->obs[2].p);

@@ -63,6 +63,12 @@ struct gs_ensemble_s {
     bool steer_zero = false;
     uint32_t kw_max = 1, ko_max = 1;
     gs::DevBuf<uint32_t> knn_walls, knn_obs;
+    // Bounding boxes of the crowd populations' snapshot positions, 6 words
+    // per population (as a group's bbox pair: the key pass accumulates into
+    // box[box_cur] and clears the other buffer): the other-group search skips
+    // queries its cap cannot reach the other group's box from.
+    gs::DevBuf<uint32_t> box[2];
+    uint8_t box_cur = 0;
     gs::DevBuf<uint32_t> key, rank, perm_tmp, id_tmp, new_to_old, old_to_new, count, start;
     gs::DevBuf<float4> spos;
 
@@ -119,6 +125,9 @@ void calib_download(gs_calib_s* c, const EnsPop& P, gs_agents* host);
 void ens_init(gs_ensemble_s* e, uint32_t ns, const gs_scenario* sc, uint32_t k,
               GridDev (*make_grid)(const float*, const float*, const uint32_t*));
 void ens_alloc(gs_ensemble_s* e);
+// Frees every device buffer of the ensemble (its graphs are destroyed by the
+// caller).
+void ens_release(gs_ensemble_s* e);
 void ens_upload_geometry(gs_ensemble_s* e);
 void ens_step(gs_ensemble_s* e);
 void ens_search_fixed(gs_ensemble_s* e, float* px, float* py, float* pz, const uint32_t* ids_fixed,

@@ -105,6 +105,69 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
     return x * q + min(x, r) + i;
 }
 
+// Order-preserving map of floats to unsigned integers (min / max of the
+// integers are min / max of the floats) and back.
+__device__ __forceinline__ uint32_t ord_of(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float float_of_ord(uint32_t o) {
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+// An empty box: min above max.
+constexpr uint32_t kBoxEmptyMin = 0xFFFFFFFFu, kBoxEmptyMax = 0u;
+// Adds the active lanes' points to the box bb[6]: wavefront min / max, then
+// across the block's wavefronts in LDS, then one atomic per bound from the
+// block.  Called by every thread of a kBlock-thread block (it synchronises
+// the block).
+__device__ __forceinline__ void box_accumulate(bool active, float x, float y, float z, uint32_t* __restrict__ bb) {
+    __shared__ uint32_t part[kBlock / 64][6];
+    uint32_t lo[3] = {active ? ord_of(x) : kBoxEmptyMin, active ? ord_of(y) : kBoxEmptyMin,
+                      active ? ord_of(z) : kBoxEmptyMin};
+    uint32_t hi[3] = {active ? lo[0] : kBoxEmptyMax, active ? lo[1] : kBoxEmptyMax, active ? lo[2] : kBoxEmptyMax};
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = min(lo[a], (uint32_t)__shfl_xor((int)lo[a], d, 64));
+            hi[a] = max(hi[a], (uint32_t)__shfl_xor((int)hi[a], d, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            part[threadIdx.x >> 6][a] = lo[a];
+            part[threadIdx.x >> 6][3 + a] = hi[a];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const uint32_t j = threadIdx.x;
+        uint32_t v = part[0][j];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; ++w) v = j < 3 ? min(v, part[w][j]) : max(v, part[w][j]);
+        // A block without points holds the empty bounds, which change nothing.
+        if (j < 3 && v != kBoxEmptyMin) atomicMin(&bb[j], v);
+        if (j >= 3 && v != kBoxEmptyMax) atomicMax(&bb[j], v);
+    }
+}
+// Whether no point of the box bb can lie within d2 <= cap2 of the query:
+// the squared gap from the query to the box exceeds cap2 by more than the
+// rounding of either d2 form (2e-5 relative).  False for an empty or
+// non-finite box (nothing is skipped on doubt).
+__device__ __forceinline__ bool box_beyond(const uint32_t* __restrict__ bb, float qx, float qy, float qz,
+                                           float cap2) {
+    if (bb[0] > bb[3]) return false;
+    float g2 = 0.f;
+    const float q[3] = {qx, qy, qz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float gap = fmaxf(fmaxf(float_of_ord(bb[a]) - q[a], q[a] - float_of_ord(bb[3 + a])), 0.f);
+        g2 += gap * gap;
+    }
+    return g2 > cap2 * (1.0f + 2e-5f);
+}
+
 __device__ __forceinline__ int axis_cell(float p, float o, float s, int d) {
     if (!(p >= o)) return 0;
     if (p >= o + s) return d - 1;
@@ -249,6 +312,18 @@ struct gs_group_s {
     // only; see launch_bins_build).
     gs::DevBuf<float4> sfwd;
     gs::DevBuf<uint32_t> flag;  // small device scratch (finite check)
+    // Bounding box of the snapshot positions (a capped search skips a query
+    // whose cap cannot reach it): order-preserving integer encodings of min
+    // x, y, z then max x, y, z.  Kept only once a capped search has targeted
+    // the group (bbox_wanted): each build's key pass then accumulates into
+    // bbox[bbox_cur] (cleared by the build before) and clears the other
+    // buffer for the next build.  bbox_fresh: bbox[bbox_cur] covers the
+    // current snapshot.  An accumulation that was not cleared only widens
+    // the box, so a stale box is still a valid bound.
+    gs::DevBuf<uint32_t> bbox[2];
+    uint8_t bbox_cur = 0;
+    bool bbox_wanted = false;
+    bool bbox_fresh = false;
 
     float* F(int j) const { return fb[j][fcur[j]].p; }
     float* Falt(int j) const { return fb[j][fcur[j] ^ 1].p; }

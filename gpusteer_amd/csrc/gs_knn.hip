@@ -41,7 +41,8 @@ __global__ __launch_bounds__(BS, RF == 1 ? (KMAX <= 8 ? 5 : (KMAX <= 16 ? 4 : 1)
                                                        uint32_t k, int R0, float cap2, const uint32_t* __restrict__ seed,
                                                        const uint32_t* __restrict__ row_map,
                                                        const uint32_t* __restrict__ val_map,
-                                                       uint32_t* __restrict__ out_idx, float* __restrict__ out_dist) {
+                                                       uint32_t* __restrict__ out_idx, float* __restrict__ out_dist,
+                                                       const uint32_t* __restrict__ box) {
     if constexpr (RF == 1) {
         // Slot a of the lane's APL slots (a block covers APL runs of BS slots).
         auto one = [&](uint32_t s) {
@@ -56,6 +57,9 @@ __global__ __launch_bounds__(BS, RF == 1 ? (KMAX <= 8 ? 5 : (KMAX <= 16 ? 4 : 1)
                 Q.thr = fminf(cap2, (seed && active) ? seed_bound<KMAX>(seed, k, nA, row_map ? row_map[s] : s, nB,
                                                                         val_map, q.x, q.y, q.z, bpos)
                                                      : INFINITY);
+                // A capped query that cannot reach B's bounding box has no
+                // candidate (box: NULL for an uncapped search).
+                if (box && active && box_beyond(box, q.x, q.y, q.z, cap2)) Q.thr = -1.f;
                 GS_FLAT_LDS_NK(BS, KMAX)
                 knn_search_flat<KMAX, BS>(h, Q, g, nB, bpos, bstart, flds);
             }
@@ -242,7 +246,8 @@ static void run_binned_shape(gs_group_s* A, gs_knn_s* out, gs_group_s* B, uint32
     const uint32_t nA = A->n;
     k_knn_binned<KMAX, RF, BS, APL><<<blocks_for(nA, BS * APL), BS, 0, A->ctx->stream>>>(
         nA, A->spos.p, B->n, B->spos.p, B->start.p, B->bin_grid, A == B ? 1 : 0, out->k, (int)R0, cap2, seed,
-        row_map, val_map, out->idx_alt(), dist ? out->dist_alt() : nullptr);
+        row_map, val_map, out->idx_alt(), dist ? out->dist_alt() : nullptr,
+        (std::isfinite(cap2) && B->bbox_fresh) ? B->bbox[B->bbox_cur].p : nullptr);
     GS_LAUNCHED("k_knn_binned");
 }
 

@@ -784,7 +784,8 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, FlatQuery& Q, con
                 ++M;
             }
         }
-        if (long_run || !Q.active) M = 0;
+        // thr < 0: the query was found to have no candidate at all.
+        if (long_run || !Q.active || Q.thr < 0.f) M = 0;
     }
     const uint32_t* rt = L.run_t;
     const uint16_t* rn = L.run_n;
@@ -889,7 +890,7 @@ __device__ __forceinline__ void knn_search_flat(TopK<KMAX>& h, FlatQuery& Q, con
     }
     bool all;
     const float dmin = outside_gap(g, cx, cy, cz, 1, qx, qy, qz, all);
-    const bool exact = !active || long_run || all || exact_after(fminf(thr, h.worst()), dmin);
+    const bool exact = !active || long_run || all || thr < 0.f || exact_after(fminf(thr, h.worst()), dmin);
     const uint64_t need = __ballot(!exact);
     if (__popcll(need) <= kCoopMaxQueries) {
         for (uint64_t left = need; left != 0; left &= left - 1) {

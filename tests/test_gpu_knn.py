@@ -242,6 +242,42 @@ def test_capped_knn_equals_oracle(gs, ctx, orc, cap, self_group):
         h.close()
 
 
+@pytest.mark.parametrize("flags", [BINNED, 0])
+def test_capped_knn_across_moving_groups(gs, ctx, orc, flags):
+    """A capped search of group A in group B while B moves from beyond the cap
+    (every list empty; from the second build on, the search skips queries by
+    B's bounding box) through partial overlap to full overlap and back, one
+    rebuild of B per stage, each search done twice (the second seeded): equal
+    to the oracle at every stage, with and without the forced binned search."""
+    g = np.random.default_rng(31)
+    a = g.uniform(-20, 20, size=(4000, 3)).astype(np.float32)
+    b0 = g.uniform(-10, 10, size=(3000, 3)).astype(np.float32)
+    cap, k = 3.0, 7
+    bins = gs.KNNBinData(ctx, (24, 24, 24), (120.0,) * 3, origin=(-60.0,) * 3)
+    A = _group(gs, ctx, a)
+    gs.updateKNNDatabase(A, bins)
+    B = _group(gs, ctx, b0)
+    knn = gs.KNNData(ctx, a.shape[0], k)
+    found = []
+    for shift in (45.0, 40.0, 33.0, 28.0, 20.0, 8.0, 0.0, 30.0, 45.0):
+        print(f"B shifted by {shift}", flush=True)
+        b = b0 + np.float32(shift)
+        B.upload(scenarios.make_agents(b, np.tile([[0.0, 0.0, 1.0]], (b.shape[0], 1)), np.zeros(b.shape[0]), 0.5,
+                                       1.0, 1.0))
+        gs.updateKNNDatabase(B, bins)
+        r_idx, r_dist = orc.knn_brute(a, b, k, False, cap=cap)
+        for rep in range(2):
+            gs.findKNearestNeighborsWithin(A, knn, bins, B, cap, 1, flags)
+            idx, dist = knn.download()
+            np.testing.assert_array_equal(idx, r_idx, err_msg=f"shift {shift} call {rep}")
+            np.testing.assert_array_equal(dist, r_dist, err_msg=f"shift {shift} call {rep}")
+        found.append(int(np.sum(idx != 0xFFFFFFFF)))
+    # The stages cover empty, partial and full overlap.
+    assert found[0] == 0 and found[-1] == 0 and 0 < found[4] < found[6]
+    for h in (knn, A, B, bins):
+        h.close()
+
+
 @pytest.mark.parametrize("nb", [1, 6, 50, 256])
 def test_small_target_set(gs, ctx, orc, nb):
     """Target sets of at most 256 points are scanned exhaustively (walls,
