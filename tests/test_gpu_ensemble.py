@@ -150,3 +150,37 @@ def test_ensemble_workload_subset_equals_oracle(ctx, orc):
                 np.testing.assert_array_equal(ens.download_knn(i, g), r[g][0], err_msg=tag)
                 assert_state_equal(ens.download(i, g), refs[i][g], what=tag)
     ens.close()
+
+
+def test_ensemble_crowds_meeting_equal_oracle(ctx, orc):
+    """Crowds whose groups overlap: group 1 starts shifted 40 units toward
+    group 0, so part of each group has the other within the cap and part lies
+    beyond the other group's bounding box (those queries skip the search).
+    Three crowds of uneven size and a flock, 8 steps (graph pairs), each
+    crowd compared with the oracle bit for bit (own lists and state; the
+    other-group lists enter the state through separation)."""
+    specs = []
+    for i, (n, gap) in enumerate(((1200, 10.0), (3000, 15.0), (5000, 25.0))):
+        sc = scenarios.choke(n_total=n, gap=gap, workload=40 + i)
+        sc.groups[1].arrays["pz"] += np.float32(40.0)
+        c = scenarios.choke_cells_for(n, cell_factor=1.6)
+        sc.grid_cells = (c, 1, c)
+        specs.append(("choke", sc))
+    st, p = scenarios.boids_sphere(n=1500)
+    specs.insert(1, ("boids", st, p))
+    refs = {i: [g.copy() for g in s[1].groups] for i, s in enumerate(specs) if s[0] == "choke"}
+    ens = Ensemble(ctx, specs)
+    steps = 8
+    ens.step(steps)
+    met = 0
+    for i, r_groups in refs.items():
+        sc = specs[i][1]
+        for _ in range(steps):
+            r = orc.choke_step(r_groups, sc)
+        met += int(np.sum(r[0][1] != 0xFFFFFFFF))
+        for g in range(2):
+            tag = f"crowd {i} group {g}: "
+            np.testing.assert_array_equal(ens.download_knn(i, g), r[g][0], err_msg=tag)
+            assert_state_equal(ens.download(i, g), r_groups[g], what=tag)
+    assert met > 0  # the groups do meet within the cap
+    ens.close()
