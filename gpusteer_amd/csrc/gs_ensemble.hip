@@ -498,14 +498,11 @@ void ens_alloc(gs_ensemble_s* e) {
     e->count.reserve(e->C ? e->C : 1);
     e->start.reserve((size_t)e->C + 1 + ntiles);
     launch_fill_u32(e->count.p, e->C, 0u, st);
-    const uint32_t nbox = 6 * (uint32_t)std::max<size_t>(e->pops.size(), 1);
-    for (auto& b : e->box) {
-        b.reserve(nbox);
-        for (uint32_t j = 0; j < nbox; j += 6) {
-            launch_fill_u32(b.p + j, 3, kBoxEmptyMin, st);
-            launch_fill_u32(b.p + j + 3, 3, kBoxEmptyMax, st);
-        }
-    }
+    // Both box buffers start empty (min above max for every population).
+    std::vector<uint32_t> box0(6 * std::max<size_t>(e->pops.size(), 1));
+    for (size_t j = 0; j < box0.size(); ++j) box0[j] = j % 6 < 3 ? kBoxEmptyMin : kBoxEmptyMax;
+    for (auto& b : e->box) h2d_vec(b, box0, st);
+    GS_HIP(hipStreamSynchronize(st));  // box0 is read before it goes out of scope
     h2d_vec(e->d_pops, e->pops, st);
     h2d_vec(e->d_scens, e->scens_host, st);
     h2d_vec(e->d_block_pop, e->block_pop_h, st);
