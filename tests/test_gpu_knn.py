@@ -195,10 +195,11 @@ def test_bin_database_across_lattices(gs, ctx, orc):
     A.close()
 
 
-@pytest.mark.parametrize("k,cells", [(7, (64, 64, 64)), (7, (64, 64, 192)), (16, (48, 48, 96))])
+@pytest.mark.parametrize("k,cells", [(7, (64, 64, 64)), (7, (64, 64, 192)), (16, (48, 48, 96)), (16, (48, 48, 144))])
 def test_large_flock_subset_against_brute(gs, ctx, orc, k, cells):
     """2^20 agents in a 1000^3 box at k = 7 (64^3, and 64 x 64 x 192, the
-    autotuner's lattice) and k = 16 (48 x 48 x 96, its lattice for k = 16):
+    autotuner's lattice) and k = 16 (48 x 48 x 96 and 48 x 48 x 144, its
+    lattices for k = 16):
     binned KNN for every agent, checked against the oracle's brute force on
     16,384 random queries (each against all 2^20 agents)."""
     print(f"flock 2^20 k={k} cells={cells}", flush=True)
