@@ -210,12 +210,24 @@ __global__ __launch_bounds__(kBlock, PART == 0 ? 5 : 1) void e_knn(
     }
 }
 
+// Side, up and (unless frozen) the smoothed acceleration are read only by the
+// agent's own step, so the counting sort leaves them in the previous storage
+// order: the step kernels read them at the agent's previous slot old[s] from
+// the current buffers and the integrate writes them in the new order into
+// the alternate buffers (the host flips them after the step).
+struct OwnFields {
+    const uint32_t* old;           // new slot -> previous slot
+    float *sx, *sy, *sz, *ux, *uy, *uz;  // alternate buffers of side and up
+    float *ax, *ay, *az;           // of the smoothed acceleration; NULL when frozen
+};
+
 // Boids: flocking + integrate from the step-t snapshot (as k_flock_update).
 template <int MODE>
 __device__ __forceinline__ void boids_one(const EnsScen& S, uint32_t s, uint32_t N, uint32_t k, Agents A,
                                           float* __restrict__ npx, float* __restrict__ npy, float* __restrict__ npz,
                                           float* __restrict__ nfx, float* __restrict__ nfy, float* __restrict__ nfz,
-                                          const float4* __restrict__ spos, const uint32_t* __restrict__ knn) {
+                                          const float4* __restrict__ spos, const uint32_t* __restrict__ knn,
+                                          const OwnFields& O) {
     const float4 q = spos[s];
     const f3 pos = mk3(q.x, q.y, q.z);
     const f3 fwd = ld3(A.fx, A.fy, A.fz, s);
@@ -226,15 +238,16 @@ __device__ __forceinline__ void boids_one(const EnsScen& S, uint32_t s, uint32_t
     f3 steering = ld3(A.gx, A.gy, A.gz, s);
     flock_core<MODE, 8>(pos, fwd, A.radius[s], m, bp, bf, S.sep, S.ali, S.coh, steering, bits);
     f3 p = pos, f = fwd;
-    f3 side = ld3(A.sx, A.sy, A.sz, s), up = ld3(A.ux, A.uy, A.uz, s);
-    f3 sacc = ld3(A.ax, A.ay, A.az, s);
+    const uint32_t o = O.old[s];
+    f3 side = ld3(A.sx, A.sy, A.sz, o), up = ld3(A.ux, A.uy, A.uz, o);
+    f3 sacc = ld3(A.ax, A.ay, A.az, O.ax ? o : s);
     float speed = A.speed[s];
     integrate<MODE>(steering, p, f, speed, sacc, side, up, A.max_force[s], A.max_speed[s], A.mass[s], S.dt);
     st3(npx, npy, npz, s, p);
     st3(nfx, nfy, nfz, s, f);
-    st3(A.sx, A.sy, A.sz, s, side);
-    st3(A.ux, A.uy, A.uz, s, up);
-    if (MODE == GS_MODE_OPENSTEER) st3(A.ax, A.ay, A.az, s, sacc);
+    st3(O.sx, O.sy, O.sz, s, side);
+    st3(O.ux, O.uy, O.uz, s, up);
+    if (O.ax) st3(O.ax, O.ay, O.az, s, sacc);  // every mode: the buffer flips
     A.speed[s] = speed;
     st3(A.gx, A.gy, A.gz, s, mk3(0.f, 0.f, 0.f));
     A.bits[s] = 0u;
@@ -245,14 +258,16 @@ __global__ __launch_bounds__(kBlock) void e_boids(EnsDev E, const uint32_t* __re
                                                   float* __restrict__ npy, float* __restrict__ npz,
                                                   float* __restrict__ nfx, float* __restrict__ nfy,
                                                   float* __restrict__ nfz, const float4* __restrict__ spos,
-                                                  const uint32_t* __restrict__ knn) {
+                                                  const uint32_t* __restrict__ knn, OwnFields O) {
     const uint32_t b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const uint32_t s = b * kBlock + threadIdx.x;
     const EnsPop& P = E.pops[E.block_pop[b]];
     if (s >= P.a0 + P.n) return;
     const EnsScen& S = E.scens[P.scen];
-    if (S.mode == GS_MODE_GPUREF) boids_one<GS_MODE_GPUREF>(S, s, N, k, A, npx, npy, npz, nfx, nfy, nfz, spos, knn);
-    else boids_one<GS_MODE_OPENSTEER>(S, s, N, k, A, npx, npy, npz, nfx, nfy, nfz, spos, knn);
+    if (S.mode == GS_MODE_GPUREF)
+        boids_one<GS_MODE_GPUREF>(S, s, N, k, A, npx, npy, npz, nfx, nfy, nfz, spos, knn, O);
+    else
+        boids_one<GS_MODE_OPENSTEER>(S, s, N, k, A, npx, npy, npz, nfx, nfy, nfz, spos, knn, O);
 }
 
 // Crowd phase a: clear the bits, avoid walls (moves touching agents), avoid
@@ -261,7 +276,8 @@ __global__ __launch_bounds__(kBlock) void e_choke_a(EnsDev E, const uint32_t* __
                                                     Agents A, const uint32_t* __restrict__ kwall,
                                                     const uint32_t* __restrict__ kobs, gs_walls W,
                                                     const float* __restrict__ ox, const float* __restrict__ oy,
-                                                    const float* __restrict__ oz, const float* __restrict__ orad) {
+                                                    const float* __restrict__ oz, const float* __restrict__ orad,
+                                                    const uint32_t* __restrict__ old) {
     const uint32_t b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const uint32_t s = b * kBlock + threadIdx.x;
     const EnsPop& P = E.pops[E.block_pop[b]];
@@ -270,13 +286,14 @@ __global__ __launch_bounds__(kBlock) void e_choke_a(EnsDev E, const uint32_t* __
     uint32_t bits = 0u;
     f3 steering = ld3(A.gx, A.gy, A.gz, s);
     f3 pos = ld3(A.px, A.py, A.pz, s), dir = ld3(A.fx, A.fy, A.fz, s);
-    const f3 side = ld3(A.sx, A.sy, A.sz, s);
+    const uint32_t o = old[s];  // side and up are in the previous order
+    const f3 side = ld3(A.sx, A.sy, A.sz, o);
     const float radius = A.radius[s], speed = A.speed[s];
     const gs_walls Ws{W.sx + S.w0, W.sy + S.w0, W.sz + S.w0, W.ex + S.w0, W.ey + S.w0,
                       W.ez + S.w0, W.nx + S.w0, W.ny + S.w0, W.nz + S.w0};
     avoid_walls_one(pos, dir, side, radius, speed, kwall + s, N, S.k_walls, S.nw, Ws, S.mt_walls, S.w_walls,
                     steering, bits);
-    avoid_obstacles_one(pos, dir, side, ld3(A.ux, A.uy, A.uz, s), radius, speed, kobs + s, N, S.k_obs, S.no,
+    avoid_obstacles_one(pos, dir, side, ld3(A.ux, A.uy, A.uz, o), radius, speed, kobs + s, N, S.k_obs, S.no,
                         ox + S.o0, oy + S.o0, oz + S.o0, orad + S.o0, S.mt_obs, S.w_obs, steering, bits);
     st3(A.px, A.py, A.pz, s, pos);
     st3(A.fx, A.fy, A.fz, s, dir);
@@ -326,18 +343,19 @@ __global__ __launch_bounds__(kBlock) void e_choke_b(EnsDev E, const uint32_t* __
 template <int MODE>
 __device__ __forceinline__ void update_one(float dt, uint32_t s, Agents A, float* __restrict__ npx,
                                            float* __restrict__ npy, float* __restrict__ npz, float* __restrict__ nfx,
-                                           float* __restrict__ nfy, float* __restrict__ nfz) {
+                                           float* __restrict__ nfy, float* __restrict__ nfz, const OwnFields& O) {
     f3 pos = ld3(A.px, A.py, A.pz, s), fwd = ld3(A.fx, A.fy, A.fz, s);
-    f3 side = ld3(A.sx, A.sy, A.sz, s), up = ld3(A.ux, A.uy, A.uz, s);
-    f3 sacc = ld3(A.ax, A.ay, A.az, s);
+    const uint32_t o = O.old[s];
+    f3 side = ld3(A.sx, A.sy, A.sz, o), up = ld3(A.ux, A.uy, A.uz, o);
+    f3 sacc = ld3(A.ax, A.ay, A.az, O.ax ? o : s);
     float speed = A.speed[s];
     integrate<MODE>(ld3(A.gx, A.gy, A.gz, s), pos, fwd, speed, sacc, side, up, A.max_force[s], A.max_speed[s],
                     A.mass[s], dt);
     st3(npx, npy, npz, s, pos);
     st3(nfx, nfy, nfz, s, fwd);
-    st3(A.sx, A.sy, A.sz, s, side);
-    st3(A.ux, A.uy, A.uz, s, up);
-    if (MODE == GS_MODE_OPENSTEER) st3(A.ax, A.ay, A.az, s, sacc);
+    st3(O.sx, O.sy, O.sz, s, side);
+    st3(O.ux, O.uy, O.uz, s, up);
+    if (O.ax) st3(O.ax, O.ay, O.az, s, sacc);  // every mode: the buffer flips
     A.speed[s] = speed;
     st3(A.gx, A.gy, A.gz, s, mk3(0.f, 0.f, 0.f));
     A.bits[s] = 0u;
@@ -346,14 +364,15 @@ __device__ __forceinline__ void update_one(float dt, uint32_t s, Agents A, float
 __global__ __launch_bounds__(kBlock) void e_choke_update(EnsDev E, const uint32_t* __restrict__ blocks, Agents A,
                                                          float* __restrict__ npx, float* __restrict__ npy,
                                                          float* __restrict__ npz, float* __restrict__ nfx,
-                                                         float* __restrict__ nfy, float* __restrict__ nfz) {
+                                                         float* __restrict__ nfy, float* __restrict__ nfz,
+                                                         OwnFields O) {
     const uint32_t b = blocks[xcd_block(blockIdx.x, gridDim.x)];
     const uint32_t s = b * kBlock + threadIdx.x;
     const EnsPop& P = E.pops[E.block_pop[b]];
     if (s >= P.a0 + P.n) return;
     const EnsScen& S = E.scens[P.scen];
-    if (S.mode == GS_MODE_GPUREF) update_one<GS_MODE_GPUREF>(S.dt, s, A, npx, npy, npz, nfx, nfy, nfz);
-    else update_one<GS_MODE_OPENSTEER>(S.dt, s, A, npx, npy, npz, nfx, nfy, nfz);
+    if (S.mode == GS_MODE_GPUREF) update_one<GS_MODE_GPUREF>(S.dt, s, A, npx, npy, npz, nfx, nfy, nfz, O);
+    else update_one<GS_MODE_OPENSTEER>(S.dt, s, A, npx, npy, npz, nfx, nfy, nfz, O);
 }
 
 // ---------------------------------------------------------------------------
@@ -632,10 +651,12 @@ void ens_step(gs_ensemble_s* e) {
                                           e->box[e->box_cur].p, e->box[e->box_cur ^ 1].p, nbox);
     GS_LAUNCHED("e_bin_key");
     launch_scan_counts(e->count.p, e->start.p, e->C, st);
-    // Moved: every field but the frozen ones (positions first, as the
-    // reorder's snapshot reads them there).
+    // Moved: every field but the frozen ones and the agent's own fields
+    // (side, up, smoothed acceleration: see OwnFields), positions first, as
+    // the reorder's snapshot reads them there.
     const uint32_t steer = 0x7u << 16;
-    const uint32_t moved = ((1u << kFields) - 1u) & ~e->frozen & ~(e->steer_zero ? steer : 0u);
+    const uint32_t own = (0x3Fu << 10) | ((0x7u << 7) & ~e->frozen);
+    const uint32_t moved = ((1u << kFields) - 1u) & ~e->frozen & ~(e->steer_zero ? steer : 0u) & ~own;
     FieldSet fs{};
     int nf = 0;
     for (int f = 0; f < kFields; ++f)
@@ -669,12 +690,15 @@ void ens_step(gs_ensemble_s* e) {
     e->kcur ^= 1;
     e->seeded = true;
     const Agents A = ens_agents(e);
+    const bool acc_own = (own >> 7) & 1u;
+    const OwnFields O{e->new_to_old.p, e->Falt(10), e->Falt(11), e->Falt(12), e->Falt(13), e->Falt(14), e->Falt(15),
+                      acc_own ? e->Falt(7) : nullptr, acc_own ? e->Falt(8) : nullptr, acc_own ? e->Falt(9) : nullptr};
     const uint32_t* kself = e->knn_self[e->kcur].p;
     const uint32_t* kother = e->knn_other[e->kcur].p;
     if (!e->blocks_boids.empty()) {
         e_boids<<<(unsigned)e->blocks_boids.size(), kBlock, 0, st>>>(E, e->d_blocks_boids.p, N, e->k, A, e->Falt(0),
                                                                       e->Falt(1), e->Falt(2), e->Falt(3), e->Falt(4),
-                                                                      e->Falt(5), e->spos.p, kself);
+                                                                      e->Falt(5), e->spos.p, kself, O);
         GS_LAUNCHED("e_boids");
     }
     if (!e->blocks_choke.empty()) {
@@ -685,16 +709,19 @@ void ens_step(gs_ensemble_s* e) {
             const uint32_t* dbl = g == 0 ? e->d_blocks_g0.p : e->d_blocks_g1.p;
             if (bl.empty()) continue;
             e_choke_a<<<(unsigned)bl.size(), kBlock, 0, st>>>(E, dbl, N, A, e->knn_walls.p, e->knn_obs.p, W,
-                                                              e->obs[0].p, e->obs[1].p, e->obs[2].p, e->obs[3].p);
+                                                              e->obs[0].p, e->obs[1].p, e->obs[2].p, e->obs[3].p,
+                                                              e->new_to_old.p);
             GS_LAUNCHED("e_choke_a");
             e_choke_b<<<(unsigned)bl.size(), kBlock, 0, st>>>(E, dbl, N, e->k, A, kself, kother);
             GS_LAUNCHED("e_choke_b");
         }
         e_choke_update<<<(unsigned)e->blocks_choke.size(), kBlock, 0, st>>>(
-            E, e->d_blocks_choke.p, A, e->Falt(0), e->Falt(1), e->Falt(2), e->Falt(3), e->Falt(4), e->Falt(5));
+            E, e->d_blocks_choke.p, A, e->Falt(0), e->Falt(1), e->Falt(2), e->Falt(3), e->Falt(4), e->Falt(5), O);
         GS_LAUNCHED("e_choke_update");
     }
     for (int f = 0; f < 6; ++f) e->fcur[f] ^= 1;
+    for (int f = 7; f < 16; ++f)
+        if (own & (1u << f)) e->fcur[f] ^= 1;
     // Every agent's steering and bits are now zero.
     e->steer_zero = true;
 }
