@@ -1449,12 +1449,17 @@ static void check_boids_args(gs_group A, gs_knn knn, gs_bins bins, const gs_boid
 // alternate buffers, flipped afterwards).  dist false: the search writes
 // neighbour slots only -- the flocking kernel and the next step's seed read
 // no distances, so only the last step of a call stores them.
-static void boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const gs_boids_params* p,
-                            hipEvent_t* ev, bool dist = true) {
+//
+// keys_ready: the previous step of the same call did this step's key pass;
+// next: another step of the call follows (this step then does its key pass
+// in the flocking kernel, unless the group keeps a bounding box).  Returns
+// whether it did.
+static bool boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const gs_boids_params* p,
+                            hipEvent_t* ev, bool dist = true, bool keys_ready = false, bool next = false) {
     hipStream_t st = A->ctx->stream;
     if (ev) GS_HIP(hipEventRecord(ev[0], st));
     const bool lazy = flock_step_lazy_ok(A);
-    launch_bins_build(A, bins, p->wrap_shape, p->wrap_center, p->wrap_size, lazy);
+    launch_bins_build(A, bins, p->wrap_shape, p->wrap_center, p->wrap_size, lazy, keys_ready);
     after_reorder(A);
     if (ev) GS_HIP(hipEventRecord(ev[1], st));
     const Seed sd = seed_for(knn, A, A, p->knn_flags);
@@ -1462,8 +1467,10 @@ static void boids_step_once(gs_group_s* A, gs_knn_s* knn, gs_bins_s* bins, const
     if (A->n) knn->cur ^= 1;
     mark_result(knn, A, A);
     if (ev) GS_HIP(hipEventRecord(ev[2], st));
-    launch_flock_update(A, knn, p, lazy);
+    const bool fuse = next && A->n > 0 && !A->bbox_wanted;
+    launch_flock_update(A, knn, p, lazy, fuse ? bins : nullptr);
     if (ev) GS_HIP(hipEventRecord(ev[3], st));
+    return fuse;
 }
 
 int gs_step_choke(gs_group A0, gs_group A1, gs_group walls, gs_group obstacles, gs_bins bins, gs_knn const* knn,
@@ -1512,7 +1519,9 @@ int gs_step_boids(gs_group A, gs_knn knn, gs_bins bins, const gs_boids_params* p
     return guarded([&] {
         check_boids_args(A, knn, bins, p);
         use_device(A->ctx);
-        for (uint32_t s = 0; s < steps; ++s) boids_step_once(A, knn, bins, p, nullptr, s + 1 == steps);
+        bool keys = false;
+        for (uint32_t s = 0; s < steps; ++s) keys = boids_step_once(A, knn, bins, p, nullptr, s + 1 == steps, keys,
+                                                                    s + 1 < steps);
     });
 }
 

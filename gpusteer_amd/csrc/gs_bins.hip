@@ -79,33 +79,12 @@ __global__ __launch_bounds__(kBlock) void k_bin_key(uint32_t n, float* __restric
     float x = 0.f, y = 0.f, z = 0.f;
     if (active) {
         x = px[i], y = py[i], z = pz[i];
-        if (wrap_shape == GS_WRAP_BOX) {
-            // WrapWorldKernel (CUDA/WrapWorldCUDAKernel.cu:25-35), all three axes.
-            const float hx = wsx * 0.5f, hy = wsy * 0.5f, hz = wsz * 0.5f;
-            const float x0 = x, y0 = y, z0 = z;
-            if (x < wcx - hx) x = wcx + hx;
-            else if (x > wcx + hx) x = wcx - hx;
-            if (y < wcy - hy) y = wcy + hy;
-            else if (y > wcy + hy) y = wcy - hy;
-            if (z < wcz - hz) z = wcz + hz;
-            else if (z > wcz + hz) z = wcz - hz;
-            if (x != x0) px[i] = x;
-            if (y != y0) py[i] = y;
-            if (z != z0) pz[i] = z;
-        } else if (wrap_shape == GS_WRAP_SPHERE) {
-            // sphericalWrapAround (plugins/Boids.cpp:200-206).
-            const f3 off = mk3(x - wcx, y - wcy, z - wcz);
-            const float r = len3(off);
-            if (r > wsx) {
-                const f3 p = mk3(x, y, z) + ((off / r) * wsx) * -2.f;
-                x = p.x;
-                y = p.y;
-                z = p.z;
-                px[i] = x;
-                py[i] = y;
-                pz[i] = z;
-            }
-        }
+        const float x0 = x, y0 = y, z0 = z;
+        wrap_point(wrap_shape, wcx, wcy, wcz, wsx, wsy, wsz, x, y, z);
+        // Only wrapped coordinates are written back.
+        if (__float_as_uint(x) != __float_as_uint(x0)) px[i] = x;
+        if (__float_as_uint(y) != __float_as_uint(y0)) py[i] = y;
+        if (__float_as_uint(z) != __float_as_uint(z0)) pz[i] = z;
         c = cell_index(g, axis_cell(x, g.ox, g.sx, g.dx), axis_cell(y, g.oy, g.sy, g.dy),
                        axis_cell(z, g.oz, g.sz, g.dz));
         key[i] = c;
@@ -298,11 +277,14 @@ void launch_scatter_reorder(uint32_t n, const uint32_t* key, const uint32_t* ran
 // other field stays in the previous order until the flock kernel, which reads
 // it through new_to_old and writes the step's results in the new order.
 void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws,
-                       bool lazy) {
+                       bool lazy, bool keys_ready) {
     const uint32_t n = g->n;
     const uint32_t C = b->cells;
     hipStream_t st = g->ctx->stream;
     const bool new_cells = !g->binned || g->bin_cells != C || g->count.cap < C;
+    // Keys from the previous step's flock kernel (counts already taken):
+    // only for the lattice that step was built on, never with a box to keep.
+    if (keys_ready && (new_cells || g->bbox_wanted)) fail(GS_ERR_INVALID_ARGUMENT, "internal: stale fused keys");
     g->key.reserve(n);
     g->rank.reserve(n);
     g->perm_tmp.reserve(n);
@@ -332,7 +314,7 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
         box_clear = g->bbox[g->bbox_cur ^ 1].p;
     }
     g->bbox_fresh = box_acc != nullptr;
-    if (n > 0) {
+    if (n > 0 && !keys_ready) {
         k_bin_key<<<blocks_for(n), kBlock, 0, st>>>(n, g->F(0), g->F(1), g->F(2), b->g, g->key.p, g->rank.p,
                                                     g->count.p, wrap_shape, wcv[0], wcv[1], wcv[2], wsv[0], wsv[1],
                                                     wsv[2], box_acc, box_clear);

@@ -397,6 +397,32 @@ struct FieldSet {
 // consecutive lanes (storage is in cell order from the previous build) take
 // one atomic per run; rank[i] = the lane's arrival position in its cell.
 // Inactive lanes must hold a key no cell has, distinct per lane.
+// The step's world wrap of one point: GS_WRAP_BOX is WrapWorldKernel's face
+// teleport on all three axes (CUDA/WrapWorldCUDAKernel.cu:25-35),
+// GS_WRAP_SPHERE sphericalWrapAround (plugins/Boids.cpp:200-206); any other
+// shape leaves the point.
+__device__ __forceinline__ void wrap_point(int shape, float wcx, float wcy, float wcz, float wsx, float wsy, float wsz,
+                                           float& x, float& y, float& z) {
+    if (shape == GS_WRAP_BOX) {
+        const float hx = wsx * 0.5f, hy = wsy * 0.5f, hz = wsz * 0.5f;
+        if (x < wcx - hx) x = wcx + hx;
+        else if (x > wcx + hx) x = wcx - hx;
+        if (y < wcy - hy) y = wcy + hy;
+        else if (y > wcy + hy) y = wcy - hy;
+        if (z < wcz - hz) z = wcz + hz;
+        else if (z > wcz + hz) z = wcz - hz;
+    } else if (shape == GS_WRAP_SPHERE) {
+        const f3 off = mk3(x - wcx, y - wcy, z - wcz);
+        const float r = len3(off);
+        if (r > wsx) {
+            const f3 p = mk3(x, y, z) + ((off / r) * wsx) * -2.f;
+            x = p.x;
+            y = p.y;
+            z = p.z;
+        }
+    }
+}
+
 __device__ __forceinline__ void count_and_rank(bool active, uint32_t c, uint32_t i, uint32_t* __restrict__ count,
                                                uint32_t* __restrict__ rank) {
     const int lane = threadIdx.x & 63;
@@ -429,8 +455,10 @@ void launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t st);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t st);
 // Bin build + reorder of g into cell order (optionally wrapping positions
 // first); also writes the position snapshot spos.
+// keys_ready: the previous step's flock kernel already did the key pass
+// (launch_flock_update's next_bins) for this lattice and wrap.
 void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const float* wc, const float* ws,
-                       bool lazy = false);
+                       bool lazy = false, bool keys_ready = false);
 // KNN into out's alternate buffer (caller flips).  row_map / val_map translate
 // a seed held at the previous storage order of A / B (NULL: same order).
 // cap2: only points with d2 <= cap2 qualify (+inf: no cap).
@@ -497,7 +525,11 @@ void launch_check_finite(gs_group_s* A, uint32_t* first_bad_slot_host);
 // Flocking + integrate of the Boids step from a current KNN result of A with
 // itself; new positions / forwards go to the alternate field buffers (caller
 // flips fields 0..5).
-void launch_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, bool lazy);
+// next_bins: the same call steps again on this lattice, so this step also
+// does the next step's key pass (wrap, cell keys, counts) for the positions
+// it writes, and the next build is called with keys_ready.
+void launch_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, bool lazy,
+                         const gs_bins_s* next_bins = nullptr);
 // Whether the fused step may build lazily (steering and bits zero, per-agent
 // constants uniform).
 bool flock_step_lazy_ok(const gs_group_s* A);

@@ -43,14 +43,26 @@ struct LazyIO {
     uint32_t write_acc;
 };
 
+// The next step's key pass, done by this step for the positions it writes
+// (when the same call steps again on the same lattice): the world wrap of the
+// next step's start, the cell key and the histogram count with arrival rank,
+// exactly as k_bin_key would.  key == NULL: not done.
+struct NextKeys {
+    GridDev g;
+    uint32_t *key, *rank, *count;
+    int wrap_shape;
+    float wc[3], ws[3];
+};
+
+// Returns the cell key of the new position when N.key is set.
 template <int KMAX, int MODE>
-__device__ __forceinline__ void flock_update_slot(uint32_t s, uint32_t n, Agents A, StepUniforms U, LazyIO Z,
+__device__ __forceinline__ uint32_t flock_update_slot(uint32_t s, uint32_t n, Agents A, StepUniforms U, LazyIO Z,
+                                                      const NextKeys& N,
                                                   float* __restrict__ npx, float* __restrict__ npy,
                                                   float* __restrict__ npz, float* __restrict__ nfx,
                                                   float* __restrict__ nfy, float* __restrict__ nfz,
                                                   const float4* __restrict__ spos, const uint32_t* __restrict__ knn,
                                                   uint32_t k, FlockTerm sep, FlockTerm ali, FlockTerm coh, float dt) {
-    if (s >= n) return;
     const bool lazy = Z.new_to_old != nullptr;
     // Own fields: at s (state in this order) or at the previous slot o.
     const uint32_t o = lazy ? Z.new_to_old[s] : s;
@@ -83,6 +95,14 @@ __device__ __forceinline__ void flock_update_slot(uint32_t s, uint32_t n, Agents
     f3 side = ld3(A.sx, A.sy, A.sz, o), up = ld3(A.ux, A.uy, A.uz, o);
     f3 sacc = ld3(A.ax, A.ay, A.az, o);
     integrate<MODE>(steering, p, f, speed, sacc, side, up, max_force, max_speed, mass, dt);
+    uint32_t c = 0u;
+    if (N.key) {
+        wrap_point(N.wrap_shape, N.wc[0], N.wc[1], N.wc[2], N.ws[0], N.ws[1], N.ws[2], p.x, p.y, p.z);
+        const GridDev& g = N.g;
+        c = cell_index(g, axis_cell(p.x, g.ox, g.sx, g.dx), axis_cell(p.y, g.oy, g.sy, g.dy),
+                       axis_cell(p.z, g.oz, g.sz, g.dz));
+        N.key[s] = c;
+    }
     st3(npx, npy, npz, s, p);
     st3(nfx, nfy, nfz, s, f);
     if (lazy) {
@@ -100,28 +120,34 @@ __device__ __forceinline__ void flock_update_slot(uint32_t s, uint32_t n, Agents
         st3(A.gx, A.gy, A.gz, s, mk3(0.f, 0.f, 0.f));
         A.bits[s] = 0u;
     }
+    return c;
 }
 
 // One lane per agent slot; the launch shape (BS threads per block, APL slots
 // per lane: a block covers APL consecutive runs of BS slots) is picked by the
 // autotuner for k = 7 (gs_context_set_shape).
 template <int KMAX, int MODE, int BS = kBlock, int APL = 1>
-__global__ __launch_bounds__(BS) void k_flock_update(uint32_t n, Agents A, StepUniforms U, LazyIO Z,
+__global__ __launch_bounds__(BS) void k_flock_update(uint32_t n, Agents A, StepUniforms U, LazyIO Z, NextKeys N,
                                                      float* __restrict__ npx, float* __restrict__ npy,
                                                      float* __restrict__ npz, float* __restrict__ nfx,
                                                      float* __restrict__ nfy, float* __restrict__ nfz,
                                                      const float4* __restrict__ spos,
                                                      const uint32_t* __restrict__ knn, uint32_t k, FlockTerm sep,
                                                      FlockTerm ali, FlockTerm coh, float dt) {
+    // Every lane of the wavefront takes part in the histogram count (lanes
+    // past n with a key no cell has, as in k_bin_key).
+    auto one = [&](uint32_t s) {
+        const bool active = s < n;
+        uint32_t c = 0xFFFFFFC0u | (threadIdx.x & 63u);
+        if (active)
+            c = flock_update_slot<KMAX, MODE>(s, n, A, U, Z, N, npx, npy, npz, nfx, nfy, nfz, spos, knn, k, sep,
+                                              ali, coh, dt);
+        if (N.key) count_and_rank(active, c, s, N.count, N.rank);
+    };
     if constexpr (APL == 1) {
-        const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-        flock_update_slot<KMAX, MODE>(s, n, A, U, Z, npx, npy, npz, nfx, nfy, nfz, spos, knn, k, sep, ali, coh, dt);
+        one(xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x);
     } else {
-        for (int a = 0; a < APL; ++a) {
-            const uint32_t s = (xcd_block(blockIdx.x, gridDim.x) * APL + a) * BS + threadIdx.x;
-            flock_update_slot<KMAX, MODE>(s, n, A, U, Z, npx, npy, npz, nfx, nfy, nfz, spos, knn, k, sep, ali, coh,
-                                          dt);
-        }
+        for (int a = 0; a < APL; ++a) one((xcd_block(blockIdx.x, gridDim.x) * APL + a) * BS + threadIdx.x);
     }
 }
 
@@ -147,7 +173,8 @@ bool flock_step_lazy_ok(const gs_group_s* A) {
 }
 
 template <int KMAX>
-static void run_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, bool lazy) {
+static void run_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, bool lazy,
+                             const gs_bins_s* next_bins) {
     const uint32_t n = A->n;
     hipStream_t st = A->ctx->stream;
     const Agents a = A->dev_agents();
@@ -169,10 +196,22 @@ static void run_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params
         Z.az = A->Falt(9);
         Z.write_acc = acc_written ? 1u : 0u;
     }
+    NextKeys N{};
+    if (next_bins) {
+        N.g = next_bins->g;
+        N.key = A->key.p;
+        N.rank = A->rank.p;
+        N.count = A->count.p;
+        N.wrap_shape = p->wrap_shape;
+        for (int j = 0; j < 3; ++j) {
+            N.wc[j] = p->wrap_center[j];
+            N.ws[j] = p->wrap_size[j];
+        }
+    }
     const LaunchShape sh = A->ctx->shape[GS_SHAPE_FLOCK];
 #define GS_FU(MODE, BS, APL)                                                                                         \
     k_flock_update<KMAX, MODE, BS, APL><<<blocks_for(n, BS * APL), BS, 0, st>>>(                                     \
-        n, a, U, Z, A->Falt(0), A->Falt(1), A->Falt(2), A->Falt(3), A->Falt(4), A->Falt(5), A->spos.p, knn->idx(), \
+        n, a, U, Z, N, A->Falt(0), A->Falt(1), A->Falt(2), A->Falt(3), A->Falt(4), A->Falt(5), A->spos.p, knn->idx(), \
         knn->k, fterm(p->separation), fterm(p->alignment), fterm(p->cohesion), p->dt)
 #define GS_FU_SHAPES(MODE)                                                                                           \
     if constexpr (KMAX == 7) {                                                                                       \
@@ -204,12 +243,13 @@ static void run_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params
     }
 }
 
-void launch_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, bool lazy) {
+void launch_flock_update(gs_group_s* A, gs_knn_s* knn, const gs_boids_params* p, bool lazy,
+                         const gs_bins_s* next_bins) {
     if (A->n == 0) return;
-    if (knn->k == 7) run_flock_update<7>(A, knn, p, lazy);
-    else if (knn->k <= 8) run_flock_update<8>(A, knn, p, lazy);
-    else if (knn->k <= 16) run_flock_update<16>(A, knn, p, lazy);
-    else run_flock_update<32>(A, knn, p, lazy);
+    if (knn->k == 7) run_flock_update<7>(A, knn, p, lazy, next_bins);
+    else if (knn->k <= 8) run_flock_update<8>(A, knn, p, lazy, next_bins);
+    else if (knn->k <= 16) run_flock_update<16>(A, knn, p, lazy, next_bins);
+    else run_flock_update<32>(A, knn, p, lazy, next_bins);
     A->mark_integrated();
 }
 

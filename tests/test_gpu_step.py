@@ -76,11 +76,25 @@ def test_boids_fused_steps_other_k(gs, ctx, orc, k):
         h.close()
 
 
-def test_boids_multi_step_calls_bit_exact(gs, ctx, orc):
-    """Fused calls of 7 steps each (the search stores distances only in a
-    call's last step): after every call the KNN sets, their distances and the
-    whole state equal the oracle's after the same number of steps."""
-    st, p = scenarios.boids_sphere()
+@pytest.mark.parametrize("world,mode", [("sphere", 0), ("sphere", 1), ("box", 0), ("constants", 0)])
+def test_boids_multi_step_calls_bit_exact(gs, ctx, orc, world, mode):
+    """Fused calls of 7 steps each (within a call each step's flocking kernel
+    also does the next step's wrap and key pass, and the search stores
+    distances only in the call's last step): after every call the KNN sets,
+    their distances and the whole state equal the oracle's after the same
+    number of steps.  Sphere wrap (4,096 agents, both modes), face wrap
+    (4,096 agents in a 100^3 box, agents crossing faces) and per-agent
+    constants (the builds move every field)."""
+    if world == "box":
+        st, p = scenarios.boids_box(n=4096, size=100.0, k=7, cells=8)
+    else:
+        st, p = scenarios.boids_sphere()
+        if world == "constants":
+            _per_agent_constants(st, 7)
+    p.mode = mode
+    if mode == 1:
+        for t in (p.separation, p.alignment, p.cohesion):
+            t.min_d = 1.5
     ref = st.copy()
     A = gs.AgentGroup(ctx, st)
     bins = gs.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
