@@ -417,9 +417,10 @@ constexpr int kFlatBuf = 16;
 template <int KMAX> constexpr int flat_buf() { return KMAX == 16 ? 24 : kFlatBuf; }
 constexpr int kFlatRuns = 10;  // nine columns, one of them split at the query
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
-// One pool of kFlatPool words per lane, strided by the block size: the deferral
-// buffer, the run list (first slots, then 16-bit lengths), and after the scan
-// the cooperative shells' candidate triples.
+// One pool of flat_pool<KMAX>() words per lane, strided by the block size:
+// the deferral buffer, the run list (first slots, then 16-bit lengths), and
+// after the scan the cooperative shells' candidate triples (as many as the
+// smallest pool, kFlatPool words, holds).
 constexpr int kFlatPool = kFlatBuf + kFlatRuns + (kFlatRuns + 1) / 2;
 template <int KMAX> constexpr int flat_pool() { return flat_buf<KMAX>() + kFlatRuns + (kFlatRuns + 1) / 2; }
 constexpr int kCoopEntries = kFlatPool / 3;
@@ -430,8 +431,8 @@ struct FlatLds {
     uint16_t* run_n;     // [kFlatRuns][block size]: its length
     uint32_t slot_addr;  // LDS byte address of the lane's first buffer entry
 };
-// For a block of BS threads: lane columns strided by BS (the search
-// functions' STRIDE template argument).
+// For a block of BS threads searching KMAX-entry lists: lane columns strided
+// by BS (the search functions' STRIDE template argument).
 #define GS_FLAT_LDS_NK(BS, KMAX)                                                                        \
     __shared__ uint32_t lds_flat_pool[flat_pool<KMAX>() * (BS)];                                       \
     const FlatLds flds{lds_flat_pool + threadIdx.x, lds_flat_pool + threadIdx.x,                        \
