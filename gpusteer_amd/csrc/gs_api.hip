@@ -316,8 +316,8 @@ static void visit(gs_group_s* g, SnapIO& io) {
         for (auto& b : f) io.buf(b);
     for (auto& b : g->bitsb) io.buf(b);
     for (auto& b : g->idb) io.buf(b);
-    gs::DevBuf<uint32_t>* u[] = {&g->slot_of_id, &g->new_to_old, &g->old_to_new, &g->key,   &g->rank,
-                                 &g->perm_tmp,   &g->id_tmp,     &g->start,      &g->count, &g->flag};
+    gs::DevBuf<uint32_t>* u[] = {&g->slot_of_id, &g->new_to_old, &g->old_to_new, &g->key,
+                                 &g->rank,       &g->id_tmp,     &g->start,      &g->count, &g->flag};
     for (auto* b : u) io.buf(*b);
     io.buf(g->spos);
     io.buf(g->sfwd);
@@ -628,7 +628,7 @@ static void release_group(gs_group_s* g) {
     for (auto& b : g->bitsb) b.release();
     for (auto& b : g->idb) b.release();
     gs::DevBuf<uint32_t>* u[] = {&g->slot_of_id, &g->new_to_old, &g->old_to_new, &g->key, &g->rank,
-                                 &g->perm_tmp,   &g->id_tmp, &g->start,      &g->count,      &g->flag};
+                                 &g->id_tmp, &g->start,      &g->count,      &g->flag};
     for (auto* b : u) b->release();
     g->spos.release();
     g->sfwd.release();

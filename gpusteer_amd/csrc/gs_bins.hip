@@ -8,14 +8,13 @@
 // Passes (all over storage slots, coalesced):
 //   1. key      cell per point + atomic arrival rank within the cell (+ optional wrap)
 //   2. scan     exclusive scan of the per-cell counts -> start[C+1]; clears counts
-//   3. scatter  perm_tmp[start[cell] + rank] = old slot
-//   4. reorder  fixes the order inside each cell to ascending agent index (a
-//               stable sort by cell, as thrust's), then gathers every live
-//               field, the bits and the agent index into the alternate
-//               buffers in the new order and records old <-> new slot maps.
+//   3. scatter  id_tmp[start[cell] + rank] = agent index (the arrival order)
+//   4. reorder  per old slot: its final slot puts the members of each cell in
+//               ascending agent index (a stable sort by cell, as thrust's);
+//               every live field, the bits and the agent index are written
+//               there in the alternate buffers, with the old <-> new maps.
 // Because the previous build left the state in cell order and agents move
-// little between builds, the reorder's gathers read nearly sequential
-// addresses.  The counts array is all-zero between builds (pass 2 clears it),
+// little between builds, the reorder's writes land nearly sequentially.  The counts array is all-zero between builds (pass 2 clears it),
 // so no memset is recorded in the stream.
 #include "gs_internal.hpp"
 
@@ -191,26 +190,25 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t C, const uint32_t*
     if (fused && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[C] = add + partial[blockIdx.x];
 }
 
+// Pass 3: the agent index of every slot of the arrival order (slot t =
+// start[cell] + arrival rank), which the reorder's canonical ranks read.
 __global__ __launch_bounds__(kBlock) void k_scatter(uint32_t n, const uint32_t* __restrict__ key,
                                                     const uint32_t* __restrict__ rank,
                                                     const uint32_t* __restrict__ start,
-                                                    const uint32_t* __restrict__ ids,
-                                                    uint32_t* __restrict__ perm_tmp, uint32_t* __restrict__ id_tmp) {
+                                                    const uint32_t* __restrict__ ids, uint32_t* __restrict__ id_tmp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     // start[key]+rank < n whenever the counts were zero before the key pass;
     // the guard keeps a corrupted count array from writing out of bounds.
-    const uint32_t d = start[key[i]] + rank[i];
-    if (d < n) {
-        perm_tmp[d] = i;
-        id_tmp[d] = ids[i];
-    }
+    const uint32_t t = start[key[i]] + rank[i];
+    if (t < n) id_tmp[t] = ids[i];
 }
 
-// Pass 4: new position t initially holds old slot s = perm_tmp[t] (arrival
-// order).  Its final position is the cell start plus the number of members of
-// its cell with a smaller agent index.
-__global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* __restrict__ perm_tmp,
+// Pass 4, one lane per old slot s (every read coalesced, the writes land
+// near s since agents rarely change cells): the agent's arrival slot is
+// t = start[cell] + rank, its final slot the cell start plus the number of
+// members of its cell with a smaller agent index.
+__global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* __restrict__ rank,
                                                     const uint32_t* __restrict__ id_tmp,
                                                     const uint32_t* __restrict__ key,
                                                     const uint32_t* __restrict__ start,
@@ -220,13 +218,13 @@ __global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t* 
                                                     const uint32_t* __restrict__ bits, uint32_t* __restrict__ bits_new,
                                                     FieldSet fs, float4* __restrict__ spos,
                                                     float4* __restrict__ sfwd) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const uint32_t s = perm_tmp[t];
-    if (s >= n) return;  // only after a corrupted build (see k_scatter)
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
     const uint32_t c = key[s];
     const uint32_t b = start[c], e = min(start[c + 1], n);
-    const uint32_t id = id_tmp[t];
+    const uint32_t t = b + rank[s];
+    if (t >= n) return;  // only after a corrupted build (see k_scatter)
+    const uint32_t id = ids[s];
     uint32_t d = t;
     if (e - b <= kCanonCap) {
         uint32_t r = 0;
@@ -261,13 +259,13 @@ void launch_scan_counts(uint32_t* count, uint32_t* start, uint32_t C, hipStream_
 }
 
 void launch_scatter_reorder(uint32_t n, const uint32_t* key, const uint32_t* rank, const uint32_t* start,
-                            const uint32_t* ids, uint32_t* ids_new, uint32_t* perm_tmp, uint32_t* id_tmp,
+                            const uint32_t* ids, uint32_t* ids_new, uint32_t* id_tmp,
                             uint32_t* new_to_old, uint32_t* old_to_new, const uint32_t* bits, uint32_t* bits_new,
                             const FieldSet& fs, float4* spos, hipStream_t st, float4* sfwd) {
     if (n == 0) return;
-    k_scatter<<<blocks_for(n), kBlock, 0, st>>>(n, key, rank, start, ids, perm_tmp, id_tmp);
+    k_scatter<<<blocks_for(n), kBlock, 0, st>>>(n, key, rank, start, ids, id_tmp);
     GS_LAUNCHED("k_scatter");
-    k_reorder<<<blocks_for(n), kBlock, 0, st>>>(n, perm_tmp, id_tmp, key, start, ids, ids_new, new_to_old,
+    k_reorder<<<blocks_for(n), kBlock, 0, st>>>(n, rank, id_tmp, key, start, ids, ids_new, new_to_old,
                                                 old_to_new, bits, bits_new, fs, spos, sfwd);
     GS_LAUNCHED("k_reorder");
 }
@@ -287,7 +285,6 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
     if (keys_ready && (new_cells || g->bbox_wanted)) fail(GS_ERR_INVALID_ARGUMENT, "internal: stale fused keys");
     g->key.reserve(n);
     g->rank.reserve(n);
-    g->perm_tmp.reserve(n);
     g->id_tmp.reserve(n);
     g->new_to_old.reserve(n);
     g->old_to_new.reserve(n);
@@ -340,8 +337,8 @@ void launch_bins_build(gs_group_s* g, const gs_bins_s* b, int wrap_shape, const 
             g->sfwd.reserve(n);
         }
         const bool has_bits = !lazy && g->kind == GS_KIND_AGENTS && !g->bits_zero;
-        launch_scatter_reorder(n, g->key.p, g->rank.p, g->start.p, g->ids(), g->ids_alt(), g->perm_tmp.p,
-                               g->id_tmp.p, g->new_to_old.p, g->old_to_new.p, has_bits ? g->bits() : nullptr,
+        launch_scatter_reorder(n, g->key.p, g->rank.p, g->start.p, g->ids(), g->ids_alt(), g->id_tmp.p,
+                               g->new_to_old.p, g->old_to_new.p, has_bits ? g->bits() : nullptr,
                                has_bits ? g->bitsb[g->bcur ^ 1].p : nullptr, fs, g->spos.p, st,
                                lazy ? g->sfwd.p : nullptr);
         for (int f = 0; f < kFields; ++f)

@@ -469,7 +469,7 @@ void ens_release(gs_ensemble_s* e) {
     e->d_pops.release();
     e->d_scens.release();
     for (auto* b : {&e->d_block_pop, &e->d_blocks_boids, &e->d_blocks_g0, &e->d_blocks_g1, &e->d_blocks_choke,
-                    &e->knn_walls, &e->knn_obs, &e->key, &e->rank, &e->perm_tmp, &e->id_tmp, &e->new_to_old,
+                    &e->knn_walls, &e->knn_obs, &e->key, &e->rank, &e->id_tmp, &e->new_to_old,
                     &e->old_to_new, &e->count, &e->start})
         b->release();
     for (auto& f : e->fb)
@@ -508,7 +508,6 @@ void ens_alloc(gs_ensemble_s* e) {
     e->knn_obs.reserve(N * ko);
     e->key.reserve(N);
     e->rank.reserve(N);
-    e->perm_tmp.reserve(N);
     e->id_tmp.reserve(N);
     e->new_to_old.reserve(N);
     e->old_to_new.reserve(N);
@@ -622,7 +621,7 @@ void ens_search_fixed(gs_ensemble_s* e, float* px, float* py, float* pz, const u
     fs.src[2] = pz;
     for (int a = 0; a < 3; ++a) fs.dst[a] = scratch[a];
     fs.nf = 3;
-    launch_scatter_reorder(N, e->key.p, e->rank.p, e->start.p, ids_fixed, ids_scratch, e->perm_tmp.p, e->id_tmp.p,
+    launch_scatter_reorder(N, e->key.p, e->rank.p, e->start.p, ids_fixed, ids_scratch, e->id_tmp.p,
                            e->new_to_old.p, e->old_to_new.p, nullptr, nullptr, fs, e->spos.p, st);
     EnsKnnOut ko;
     ko.self = out_self;
@@ -667,8 +666,8 @@ void ens_step(gs_ensemble_s* e) {
         }
     fs.nf = nf;
     const bool move_bits = !e->steer_zero;
-    launch_scatter_reorder(N, e->key.p, e->rank.p, e->start.p, e->ids(), e->idb[e->icur ^ 1].p, e->perm_tmp.p,
-                           e->id_tmp.p, e->new_to_old.p, e->old_to_new.p, move_bits ? e->bits() : nullptr,
+    launch_scatter_reorder(N, e->key.p, e->rank.p, e->start.p, e->ids(), e->idb[e->icur ^ 1].p, e->id_tmp.p,
+                           e->new_to_old.p, e->old_to_new.p, move_bits ? e->bits() : nullptr,
                            move_bits ? e->bitsb[e->bcur ^ 1].p : nullptr, fs, e->spos.p, st);
     for (int f = 0; f < kFields; ++f)
         if (moved & (1u << f)) e->fcur[f] ^= 1;

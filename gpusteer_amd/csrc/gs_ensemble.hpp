@@ -69,7 +69,7 @@ struct gs_ensemble_s {
     // queries its cap cannot reach the other group's box from.
     gs::DevBuf<uint32_t> box[2];
     uint8_t box_cur = 0;
-    gs::DevBuf<uint32_t> key, rank, perm_tmp, id_tmp, new_to_old, old_to_new, count, start;
+    gs::DevBuf<uint32_t> key, rank, id_tmp, new_to_old, old_to_new, count, start;
     gs::DevBuf<float4> spos;
 
     // Walls: start, end, normal, midpoint (12 arrays); obstacles: centre, radius.

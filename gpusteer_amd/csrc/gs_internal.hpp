@@ -304,7 +304,7 @@ struct gs_group_s {
     bool snap_valid = false;  // spos holds the positions of the last build
     gs::GridDev bin_grid{};
     uint32_t bin_cells = 0;
-    gs::DevBuf<uint32_t> key, rank, perm_tmp, id_tmp, start, count;
+    gs::DevBuf<uint32_t> key, rank, id_tmp, start, count;
     // Positions at the last build in storage order, packed (x, y, z, agent
     // index bits): what KNN searches, like the reference's positionSorted.
     gs::DevBuf<float4> spos;
@@ -446,7 +446,7 @@ void launch_scan_counts(uint32_t* count, uint32_t* start, uint32_t C, hipStream_
 // field in fs, bits and ids into the new order; writes the order maps and
 // the packed position snapshot.
 void launch_scatter_reorder(uint32_t n, const uint32_t* key, const uint32_t* rank, const uint32_t* start,
-                            const uint32_t* ids, uint32_t* ids_new, uint32_t* perm_tmp, uint32_t* id_tmp,
+                            const uint32_t* ids, uint32_t* ids_new, uint32_t* id_tmp,
                             uint32_t* new_to_old, uint32_t* old_to_new, const uint32_t* bits, uint32_t* bits_new,
                             const FieldSet& fs, float4* spos, hipStream_t st, float4* sfwd = nullptr);
 

@@ -25,8 +25,8 @@ constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 // Byte offsets of the regions of the array.
 constexpr size_t kOffSpos = 0;                                     // float4 [N]
-constexpr size_t kOffU32 = kOffSpos + align256(16 * kN);           // 8 x uint32 [N]
-constexpr size_t kU32Arrays = 8;  // key rank perm_tmp id_tmp new_to_old old_to_new ids ids_alt
+constexpr size_t kOffU32 = kOffSpos + align256(16 * kN);           // 7 x uint32 [N]
+constexpr size_t kU32Arrays = 7;  // key rank id_tmp new_to_old old_to_new ids ids_alt
 constexpr size_t kOffPos = kOffU32 + kU32Arrays * align256(4 * kN); // 3 x float [N] (sorted positions)
 constexpr size_t kOffCount = kOffPos + 3 * align256(4 * kN);        // uint32 [C]
 constexpr size_t kOffStart = kOffCount + align256(4 * kC);          // uint32 [kStartWords]
@@ -113,7 +113,7 @@ void noalloc_find_knn(gs_context ctx, const gs_bins_s* bins, uint32_t n, const f
         g.fb[a][0].cap = n;
         point_into(g.fb[a][1], base, kOffPos + (size_t)a * align256(4 * kN), kN);
     }
-    DevBuf<uint32_t>* u32[kU32Arrays] = {&g.key, &g.rank, &g.perm_tmp, &g.id_tmp, &g.new_to_old, &g.old_to_new,
+    DevBuf<uint32_t>* u32[kU32Arrays] = {&g.key, &g.rank, &g.id_tmp, &g.new_to_old, &g.old_to_new,
                                          &g.idb[0], &g.idb[1]};
     for (size_t j = 0; j < kU32Arrays; ++j) point_into(*u32[j], base, kOffU32 + j * align256(4 * kN), kN);
     point_into(g.spos, base, kOffSpos, kN);
