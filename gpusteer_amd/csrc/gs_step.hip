@@ -8,7 +8,9 @@
 //   k_flock_update: one lane per storage slot -> the three flocking terms
 //   -> integrate -> new positions and forward vectors into the alternate
 //   buffers, the agent's other fields in place (or, lazily, read at the
-//   previous slot and written in the new order).
+//   previous slot and written in the new order); when the same call steps
+//   again, also the next step's wrap and key pass for the new positions, so
+//   that step's build starts at the scan.
 //
 // The lazy form applies when steering and bits are zero and the per-agent
 // constants are uniform (they need no moving): the build then moves no
