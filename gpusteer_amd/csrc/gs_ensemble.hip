@@ -196,18 +196,23 @@ __global__ __launch_bounds__(kBlock, PART == 0 ? 5 : 1) void e_knn(
     }
     if (!active) return;
     const EnsScen& S = E.scens[P.scen];
-    {
-        TopK<8> h;
-        h.init(S.k_walls);
-        brute_small<8>(h, q.x, q.y, q.z, S.nw, wmx + S.w0, wmy + S.w0, wmz + S.w0);
-        store_local<8>(h, S.k_walls, N, s, out.walls, out.out_map);
-    }
-    {
-        TopK<8> h;
-        h.init(S.k_obs);
-        brute_small<8>(h, q.x, q.y, q.z, S.no, ox + S.o0, oy + S.o0, oz + S.o0);
-        store_local<8>(h, S.k_obs, N, s, out.obs, out.out_map);
-    }
+    // Lists of up to four entries (the crowds' 3 walls and 2 obstacles) take
+    // 4-entry registers: each insertion ripples through 4 entries, not 8.
+    auto small = [&](uint32_t kk, uint32_t n, const float* x, const float* y, const float* z, uint32_t* o) {
+        if (kk <= 4) {
+            TopK<4> h;
+            h.init(kk);
+            brute_small<4>(h, q.x, q.y, q.z, n, x, y, z);
+            store_local<4>(h, kk, N, s, o, out.out_map);
+        } else {
+            TopK<8> h;
+            h.init(kk);
+            brute_small<8>(h, q.x, q.y, q.z, n, x, y, z);
+            store_local<8>(h, kk, N, s, o, out.out_map);
+        }
+    };
+    small(S.k_walls, S.nw, wmx + S.w0, wmy + S.w0, wmz + S.w0, out.walls);
+    small(S.k_obs, S.no, ox + S.o0, oy + S.o0, oz + S.o0, out.obs);
 }
 
 // Side, up and (unless frozen) the smoothed acceleration are read only by the
