@@ -42,70 +42,113 @@ __device__ __forceinline__ void apply(f3& steering, uint32_t& bits, f3 f, float 
 // of each other, so they are gathered in one pass over the neighbours and then
 // applied in the order separation, alignment, cohesion with the masks tested
 // against the bits as each previous term left them, exactly as three
-// consecutive launches would.  bp / bf hold the first m neighbours' positions
+// consecutive launches would.  The pass is split into flock_begin,
+// flock_accumulate (any number of consecutive chunks of neighbours, in slot
+// order) and flock_finish, so a long list can be gathered a chunk at a time.
+struct FlockAcc {
+    f3 ss, sa, sc;
+    int ns, na, nc;
+    bool do_s, do_a, do_c;
+    bool fok;
+    float cone_e;
+};
+
+__device__ __forceinline__ FlockAcc flock_begin(f3 fwd, uint32_t bits, const FlockTerm& sep, const FlockTerm& ali,
+                                                const FlockTerm& coh) {
+    FlockAcc a;
+    // Bits only accumulate, so a term masked now stays masked.
+    a.do_s = sep.on && !(bits & sep.mask);
+    a.do_a = ali.on && !(bits & ali.mask);
+    a.do_c = coh.on && !(bits & coh.mask);
+    // The cone input dot3(fwd, off / d) only enters the comparisons
+    // `> cos_max` of the three terms.  Its estimate fma-dot(fwd, off) times
+    // the hardware reciprocal of d (GPU rule) or reciprocal square root of
+    // d2 (OpenSteer rule, whose predicate compares squared distances, so d
+    // itself is needed only for the exact value), each within 1 ulp, lies
+    // within 12 u |fwd| of it (u = 2^-24: three roundings of the estimate's
+    // dot, the reciprocal (with d's rounding) and its product, against four
+    // on each term of the exact form, with |off| / d <= 1 + 3u).  An
+    // estimate farther than E = 2^-18 (|fx| + |fy| + |fz|) + 2^-100 from
+    // every threshold therefore decides all three comparisons as the exact
+    // value does; otherwise, or for d2 outside [2^-80, 2^80] or a forward sum
+    // above 2^40 (where underflow or overflow could void the bound; NaN fails
+    // the tests too), the exact value is computed.
+    const float fsum = (fabsf(fwd.x) + fabsf(fwd.y)) + fabsf(fwd.z);
+    a.fok = fsum <= 0x1p40f;
+    a.cone_e = __builtin_fmaf(fsum, 0x1p-18f, 0x1p-100f);
+    a.ss = a.sa = a.sc = mk3(0.f, 0.f, 0.f);
+    a.ns = a.na = a.nc = 0;
+    return a;
+}
+
+__device__ __forceinline__ bool flock_any(const FlockAcc& a) { return a.do_s || a.do_a || a.do_c; }
+
+// The next cnt neighbours (cnt <= CH) from the register arrays bp / bf.
+template <int MODE, int CH>
+__device__ __forceinline__ void flock_accumulate(FlockAcc& a, f3 pos, f3 fwd, float radius, int cnt, const f3* bp,
+                                                 const f3* bf, const FlockTerm& sep, const FlockTerm& ali,
+                                                 const FlockTerm& coh) {
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+        if (j < cnt) {
+            const f3 off = bp[j] - pos;
+            const float d2 = dot3(off, off);
+            const bool gpuref = MODE == GS_MODE_GPUREF;
+            float d = gpuref ? sqrtf(d2) : 0.f;
+            float cdot = __builtin_fmaf(fwd.z, off.z, __builtin_fmaf(fwd.y, off.y, fwd.x * off.x)) *
+                         (gpuref ? __builtin_amdgcn_rcpf(d) : __builtin_amdgcn_rsqf(d2));
+            const bool sure = a.fok && d2 >= 0x1p-80f && d2 <= 0x1p80f && fabsf(cdot - sep.cos_max) > a.cone_e &&
+                              fabsf(cdot - ali.cos_max) > a.cone_e && fabsf(cdot - coh.cos_max) > a.cone_e;
+            if (!sure) {
+                if (!gpuref) d = sqrtf(d2);
+                cdot = dot3(fwd, off / d);
+            }
+            if (a.do_s && in_hood_pre<MODE>(d2, d, cdot, radius, sep)) {
+                a.ss = a.ss + off / -d2;
+                ++a.ns;
+            }
+            if (a.do_a && in_hood_pre<MODE>(d2, d, cdot, radius, ali)) {
+                a.sa = a.sa + bf[j];
+                ++a.na;
+            }
+            if (a.do_c && in_hood_pre<MODE>(d2, d, cdot, radius, coh)) {
+                a.sc = a.sc + bp[j];
+                ++a.nc;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void flock_finish(const FlockAcc& a, f3 pos, f3 fwd, const FlockTerm& sep,
+                                             const FlockTerm& ali, const FlockTerm& coh, f3& steering,
+                                             uint32_t& bits) {
+    if (a.do_s) {
+        f3 ss = a.ss;
+        if (a.ns > 0) ss = normalize3(ss / (float)a.ns);
+        apply(steering, bits, ss, sep.w, GS_BIT_SEPARATION);
+    }
+    if (a.do_a && !(bits & ali.mask)) {
+        f3 sa = a.sa;
+        if (a.na > 0) sa = normalize3((sa / (float)a.na) - fwd);
+        apply(steering, bits, sa, ali.w, GS_BIT_ALIGNMENT);
+    }
+    if (a.do_c && !(bits & coh.mask)) {
+        f3 sc = a.sc;
+        if (a.nc > 0) sc = normalize3((sc / (float)a.nc) - pos);
+        apply(steering, bits, sc, coh.w, GS_BIT_COHESION);
+    }
+}
+
+// The whole pass over the first m neighbours: bp / bf hold their positions
 // and forward vectors (register arrays of KMAX, walked fully unrolled).
 template <int MODE, int KMAX>
 __device__ __forceinline__ void flock_core(f3 pos, f3 fwd, float radius, int m, const f3* bp, const f3* bf,
                                            const FlockTerm& sep, const FlockTerm& ali, const FlockTerm& coh,
                                            f3& steering, uint32_t& bits) {
-    // Bits only accumulate, so a term masked now stays masked.
-    const bool do_s = sep.on && !(bits & sep.mask);
-    const bool do_a = ali.on && !(bits & ali.mask);
-    const bool do_c = coh.on && !(bits & coh.mask);
-    if (!do_s && !do_a && !do_c) return;
-    // The cone input dot3(fwd, off / d) only enters the comparisons
-    // `> cos_max` of the three terms.  Its estimate fma-dot(fwd, off) times
-    // the hardware reciprocal of d (1 ulp) lies within 12 u |fwd| of it
-    // (u = 2^-24: three roundings of the estimate's dot, the reciprocal and
-    // its product, against four on each term of the exact form, with
-    // |off| / d <= 1 + 3u).  An estimate farther than
-    // E = 2^-18 (|fx| + |fy| + |fz|) + 2^-100 from every threshold therefore
-    // decides all three comparisons as the exact value does; otherwise, or
-    // for d outside [2^-40, 2^40] or a forward sum above 2^40 (where
-    // underflow or overflow could void the bound; NaN fails the tests too),
-    // the exact value is computed.
-    const float fsum = (fabsf(fwd.x) + fabsf(fwd.y)) + fabsf(fwd.z);
-    const bool fok = fsum <= 0x1p40f;
-    const float cone_e = __builtin_fmaf(fsum, 0x1p-18f, 0x1p-100f);
-    f3 ss = mk3(0.f, 0.f, 0.f), sa = ss, sc = ss;
-    int ns = 0, na = 0, nc = 0;
-#pragma unroll
-    for (int j = 0; j < KMAX; ++j) {
-        if (j < m) {
-            const f3 off = bp[j] - pos;
-            const float d2 = dot3(off, off);
-            const float d = sqrtf(d2);
-            float cdot = __builtin_fmaf(fwd.z, off.z, __builtin_fmaf(fwd.y, off.y, fwd.x * off.x)) *
-                         __builtin_amdgcn_rcpf(d);
-            const bool sure = fok && d >= 0x1p-40f && d <= 0x1p40f && fabsf(cdot - sep.cos_max) > cone_e &&
-                              fabsf(cdot - ali.cos_max) > cone_e && fabsf(cdot - coh.cos_max) > cone_e;
-            if (!sure) cdot = dot3(fwd, off / d);
-            if (do_s && in_hood_pre<MODE>(d2, d, cdot, radius, sep)) {
-                ss = ss + off / -d2;
-                ++ns;
-            }
-            if (do_a && in_hood_pre<MODE>(d2, d, cdot, radius, ali)) {
-                sa = sa + bf[j];
-                ++na;
-            }
-            if (do_c && in_hood_pre<MODE>(d2, d, cdot, radius, coh)) {
-                sc = sc + bp[j];
-                ++nc;
-            }
-        }
-    }
-    if (do_s) {
-        if (ns > 0) ss = normalize3(ss / (float)ns);
-        apply(steering, bits, ss, sep.w, GS_BIT_SEPARATION);
-    }
-    if (do_a && !(bits & ali.mask)) {
-        if (na > 0) sa = normalize3((sa / (float)na) - fwd);
-        apply(steering, bits, sa, ali.w, GS_BIT_ALIGNMENT);
-    }
-    if (do_c && !(bits & coh.mask)) {
-        if (nc > 0) sc = normalize3((sc / (float)nc) - pos);
-        apply(steering, bits, sc, coh.w, GS_BIT_COHESION);
-    }
+    FlockAcc a = flock_begin(fwd, bits, sep, ali, coh);
+    if (!flock_any(a)) return;
+    flock_accumulate<MODE, KMAX>(a, pos, fwd, radius, m, bp, bf, sep, ali, coh);
+    flock_finish(a, pos, fwd, sep, ali, coh, steering, bits);
 }
 
 // Loads the KNN slots of row `row` ([k][nA] slot-major) and the neighbours'

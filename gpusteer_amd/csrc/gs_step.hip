@@ -56,6 +56,59 @@ struct NextKeys {
     float wc[3], ws[3];
 };
 
+// Lists longer than kFlockChunk entries are gathered and summed a chunk at a
+// time, in the same slot order (so the same sums): register arrays for a
+// whole 16-entry list would hold the kernel to three wavefronts per SIMD.
+constexpr int kFlockChunk = 8;
+
+// flock_core over the KNN row s, gathering kFlockChunk neighbours at a time
+// (positions from spos; forward vectors from sfwd when given, else from
+// fx / fy / fz, only when alignment is live).  Slots past the first pad
+// reuse slot 0's neighbour so every load of a chunk is in flight together.
+template <int KMAX, int MODE>
+__device__ __forceinline__ void flock_chunked(const uint32_t* __restrict__ knn, uint32_t k, uint32_t n, uint32_t s,
+                                              const float4* __restrict__ spos, const float4* __restrict__ sfwd,
+                                              const float* __restrict__ fx, const float* __restrict__ fy,
+                                              const float* __restrict__ fz, f3 pos, f3 fwd, float radius,
+                                              const FlockTerm& sep, const FlockTerm& ali, const FlockTerm& coh,
+                                              f3& steering, uint32_t& bits) {
+    FlockAcc a = flock_begin(fwd, bits, sep, ali, coh);
+    if (!flock_any(a)) return;
+    uint32_t v[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) v[j] = knn[(size_t)min(j, (int)k - 1) * n + s];
+    int m = (int)k;
+#pragma unroll
+    for (int j = KMAX - 1; j >= 0; --j)
+        if (j < (int)k && v[j] >= n) m = j;
+#pragma unroll
+    for (int c = 0; c < KMAX; c += kFlockChunk) {
+        if (c >= m) break;
+        f3 bp[kFlockChunk], bf[kFlockChunk];
+#pragma unroll
+        for (int j = 0; j < kFlockChunk; ++j) {
+            const uint32_t b = (c + j < m) ? v[c + j] : v[0];
+            const float4 q = spos[b];
+            bp[j] = mk3(q.x, q.y, q.z);
+            bf[j] = mk3(0.f, 0.f, 0.f);
+        }
+        if (a.do_a) {
+#pragma unroll
+            for (int j = 0; j < kFlockChunk; ++j) {
+                const uint32_t b = (c + j < m) ? v[c + j] : v[0];
+                if (sfwd) {
+                    const float4 f = sfwd[b];
+                    bf[j] = mk3(f.x, f.y, f.z);
+                } else {
+                    bf[j] = ld3(fx, fy, fz, b);
+                }
+            }
+        }
+        flock_accumulate<MODE, kFlockChunk>(a, pos, fwd, radius, min(m - c, kFlockChunk), bp, bf, sep, ali, coh);
+    }
+    flock_finish(a, pos, fwd, sep, ali, coh, steering, bits);
+}
+
 // Returns the cell key of the new position when N.key is set.
 template <int KMAX, int MODE>
 __device__ __forceinline__ uint32_t flock_update_slot(uint32_t s, uint32_t n, Agents A, StepUniforms U, LazyIO Z,
@@ -83,15 +136,22 @@ __device__ __forceinline__ uint32_t flock_update_slot(uint32_t s, uint32_t n, Ag
         speed = A.speed[s];
     }
     uint32_t bits = U.steer_zero ? 0u : A.bits[s];
-    f3 bp[KMAX], bf[KMAX];
-    const int m = gather_neighbours<KMAX>(knn, k, n, s, n, spos, nullptr, nullptr, nullptr, A.fx, A.fy, A.fz,
-                                          ali.on && !(bits & ali.mask), bp, bf, lazy ? Z.sfwd : nullptr);
-    f3 steering = U.steer_zero ? mk3(0.f, 0.f, 0.f) : ld3(A.gx, A.gy, A.gz, s);
+    f3 steering;
     const float radius = (U.has & 1u) ? U.v[0] : A.radius[s];
     const float max_speed = (U.has & 2u) ? U.v[1] : A.max_speed[s];
     const float max_force = (U.has & 4u) ? U.v[2] : A.max_force[s];
     const float mass = (U.has & 8u) ? U.v[3] : A.mass[s];
-    flock_core<MODE, KMAX>(pos, fwd, radius, m, bp, bf, sep, ali, coh, steering, bits);
+    if constexpr (KMAX > kFlockChunk) {
+        steering = U.steer_zero ? mk3(0.f, 0.f, 0.f) : ld3(A.gx, A.gy, A.gz, s);
+        flock_chunked<KMAX, MODE>(knn, k, n, s, spos, lazy ? Z.sfwd : nullptr, A.fx, A.fy, A.fz, pos, fwd, radius,
+                                  sep, ali, coh, steering, bits);
+    } else {
+        f3 bp[KMAX], bf[KMAX];
+        const int m = gather_neighbours<KMAX>(knn, k, n, s, n, spos, nullptr, nullptr, nullptr, A.fx, A.fy, A.fz,
+                                              ali.on && !(bits & ali.mask), bp, bf, lazy ? Z.sfwd : nullptr);
+        steering = U.steer_zero ? mk3(0.f, 0.f, 0.f) : ld3(A.gx, A.gy, A.gz, s);
+        flock_core<MODE, KMAX>(pos, fwd, radius, m, bp, bf, sep, ali, coh, steering, bits);
+    }
     // Integrate (clears steering and bits as updateGroup does).
     f3 p = pos, f = fwd;
     f3 side = ld3(A.sx, A.sy, A.sz, o), up = ld3(A.ux, A.uy, A.uz, o);

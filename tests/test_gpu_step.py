@@ -49,21 +49,25 @@ def test_boids_fused_steps_bit_exact(gs, ctx, orc, mode):
         h.close()
 
 
-@pytest.mark.parametrize("k", [5, 8, 16])
-def test_boids_fused_steps_other_k(gs, ctx, orc, k):
+@pytest.mark.parametrize("k,mode,n", [(5, 1, 4096), (8, 1, 4096), (12, 0, 4096), (16, 1, 4096), (16, 0, 4096),
+                                      (16, 0, 10), (16, 1, 13)])
+def test_boids_fused_steps_other_k(gs, ctx, orc, k, mode, n):
     """30 fused steps of the sphere flock with k other than 7 (8- and
-    16-entry lists; k = 7 has its own 7-entry kernels), GPU rule: KNN sets
-    and the whole state equal the oracle's after every 10th step."""
-    st, p = scenarios.boids_sphere()
-    p.mode = 1
+    16-entry lists; k = 7 has its own 7-entry kernels; lists longer than 8
+    are summed 8 neighbours at a time): KNN sets and the whole state equal
+    the oracle's after every 10th step.  The flocks of 10 and 13 agents have
+    9 and 12 neighbours per list, so the pad falls inside the second chunk."""
+    st, p = scenarios.boids_sphere(n=n, radius=50.0 if n > 100 else 5.0)
+    p.mode = mode
     p.k = k
-    for t in (p.separation, p.alignment, p.cohesion):
-        t.min_d = 1.5
+    if mode == 1:
+        for t in (p.separation, p.alignment, p.cohesion):
+            t.min_d = 1.5
     ref = st.copy()
     A = gs.AgentGroup(ctx, st)
     bins = gs.KNNBinData(ctx, p.grid_cells, p.grid_size, origin=p.grid_origin)
     knn = gs.KNNData(ctx, st.n, p.k)
-    print(f"fused steps k={k}", flush=True)
+    print(f"fused steps k={k} mode={mode} n={n}", flush=True)
     for step in range(30):
         gs.stepBoids(A, knn, bins, p, 1)
         r_idx, r_dist = orc.boids_step(ref, p)
@@ -76,21 +80,24 @@ def test_boids_fused_steps_other_k(gs, ctx, orc, k):
         h.close()
 
 
-@pytest.mark.parametrize("world,mode", [("sphere", 0), ("sphere", 1), ("box", 0), ("constants", 0)])
-def test_boids_multi_step_calls_bit_exact(gs, ctx, orc, world, mode):
+@pytest.mark.parametrize("world,mode,k", [("sphere", 0, 7), ("sphere", 1, 7), ("box", 0, 7), ("constants", 0, 7),
+                                          ("sphere", 0, 16), ("constants", 1, 16)])
+def test_boids_multi_step_calls_bit_exact(gs, ctx, orc, world, mode, k):
     """Fused calls of 7 steps each (within a call each step's flocking kernel
     also does the next step's wrap and key pass, and the search stores
     distances only in the call's last step): after every call the KNN sets,
     their distances and the whole state equal the oracle's after the same
     number of steps.  Sphere wrap (4,096 agents, both modes), face wrap
     (4,096 agents in a 100^3 box, agents crossing faces) and per-agent
-    constants (the builds move every field)."""
+    constants (the builds move every field), at k = 7 and k = 16."""
+    print(f"multi-step calls {world} mode={mode} k={k}", flush=True)
     if world == "box":
-        st, p = scenarios.boids_box(n=4096, size=100.0, k=7, cells=8)
+        st, p = scenarios.boids_box(n=4096, size=100.0, k=k, cells=8)
     else:
         st, p = scenarios.boids_sphere()
         if world == "constants":
             _per_agent_constants(st, 7)
+    p.k = k
     p.mode = mode
     if mode == 1:
         for t in (p.separation, p.alignment, p.cohesion):

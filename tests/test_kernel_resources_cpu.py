@@ -69,6 +69,7 @@ BUDGETS = [
     (r"k_knn_binnedILi16ELi1E", 128, 300),          # 4 wavefronts per SIMD (fewer spills were slower)
     (r"k_flock_updateILi7ELi\dELi\d+ELi1E", 96, 0),   # one agent per lane: 5 wavefronts per SIMD
     (r"k_flock_updateILi7ELi\dELi\d+ELi2E", 128, 0),
+    (r"k_flock_updateILi16E", 112, 0),               # 16-entry lists summed 8 at a time: 4 wavefronts per SIMD
     (r"e_knnILi7ELi0E", 96, 128),                    # ensemble own-group search: 5 wavefronts per SIMD
     (r"e_knnILi8ELi0E", 96, 144),
     (r"e_knnILi7ELi1E", 152, 0),                     # ensemble crowd lists (other group, walls, obstacles)
