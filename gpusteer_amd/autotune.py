@@ -17,9 +17,11 @@ from ._capi import GS_MAX_CELLS
 Cells = Tuple[int, int, int]
 # The largest lattice gs_bins_create accepts.
 MAX_CELLS = GS_MAX_CELLS
+# Ties go to the earlier candidate (pick), so 48 x 48 x 144, the k = 16
+# lattice, comes after the two that k = 7 picks between.
 BOIDS_CANDIDATES: Tuple[Tuple[Cells, int], ...] = (
-    ((48, 48, 48), 1), ((64, 64, 64), 1), ((48, 48, 96), 1), ((48, 48, 144), 1), ((64, 64, 128), 1),
-    ((64, 64, 192), 1), ((80, 80, 160), 1), ((96, 96, 96), 2), ((128, 128, 128), 2))
+    ((48, 48, 48), 1), ((64, 64, 64), 1), ((48, 48, 96), 1), ((64, 64, 128), 1), ((64, 64, 192), 1),
+    ((48, 48, 144), 1), ((80, 80, 160), 1), ((96, 96, 96), 2), ((128, 128, 128), 2))
 
 
 # A candidate later in the list replaces an earlier one only when it is
