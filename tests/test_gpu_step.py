@@ -202,7 +202,8 @@ def test_steering_pending_across_bin_build(gs, ctx, orc, mode):
 
 
 @pytest.mark.parametrize("cells,sr,k", [((64, 64, 64), 1, 7), ((64, 64, 128), 1, 7), ((128, 128, 128), 2, 7),
-                                        ((64, 64, 192), 1, 7), ((48, 48, 96), 1, 16)])
+                                        ((64, 64, 192), 1, 7), ((48, 48, 144), 1, 7), ((48, 48, 96), 1, 16),
+                                        ((48, 48, 144), 1, 16)])
 def test_boids_box_large_steps(gs, ctx, orc, cells, sr, k):
     """2^20-agent box flock: 3 fused steps vs 3 oracle steps, bit-exact, at
     k = 7 and k = 16 on the lattices the autotuner picks for them."""
