@@ -807,12 +807,15 @@ def main() -> int:
             gs.stepBoids(A, knn16, bins, p16, args.steps)
             ctx.synchronize()
             el16 = time.perf_counter() - t0
+            b16, kn16, si16, _ = gs.stepBoidsTimed(A, knn16, bins, p16, tsteps)
             for h in (knn16, A, bins):
                 h.close()
             k16 = {"agent_steps_per_s": n * args.steps / el16, "ms_per_step": el16 * 1e3 / args.steps,
                    "steps": args.steps, "cells": list(c16), "column_radius": rf16,
+                   "stage_ms": {"bin_build": b16, "knn": kn16, "steer_integrate": si16},
                    "autotune_ms_per_step": [{"cells": list(c), "column_radius": r, "ms": m} for c, r, m in table16]}
-            print(f"bench: k=16 {el16 * 1e3 / args.steps:.3f} ms/step on {c16}", file=sys.stderr, flush=True)
+            print(f"bench: k=16 {el16 * 1e3 / args.steps:.3f} ms/step on {c16} (event-timed stages: bin build "
+                  f"{b16:.3f} ms, knn {kn16:.3f} ms, steer+integrate {si16:.3f} ms)", file=sys.stderr, flush=True)
     if status != 0:
         print(f"bench: non-finite state at agent {bad}", file=sys.stderr)
         return 1
