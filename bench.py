@@ -43,22 +43,22 @@ def bytes_knn_kernel(k: int) -> float:
     return 12.0 + 4.0 * k
 
 
-def traffic_summary(cells_txt: str) -> Path:
-    return ROOT / "profiles" / f"k_knn_binned_boids_box_1M_k7_{cells_txt}_hbm_traffic.md"
+def traffic_summary(cells_txt: str, k: int = 7) -> Path:
+    return ROOT / "profiles" / f"k_knn_binned_boids_box_1M_k{k}_{cells_txt}_hbm_traffic.md"
 
 
-def sq_summary(cells_txt: str) -> Path:
-    return ROOT / "profiles" / f"k_knn_binned_boids_box_1M_k7_{cells_txt}_sq_counters.md"
+def sq_summary(cells_txt: str, k: int = 7) -> Path:
+    return ROOT / "profiles" / f"k_knn_binned_boids_box_1M_k{k}_{cells_txt}_sq_counters.md"
 
 
-def knn_traffic_from_profile(cells_txt: str):
+def knn_traffic_from_profile(cells_txt: str, k: int = 7):
     """HBM bytes per launch of the KNN kernel on this lattice as measured by
     the committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes (each scaled
     by the same counter's reading for a copy of known size), or None when no
     summary covers the lattice."""
     import re
 
-    f = traffic_summary(cells_txt)
+    f = traffic_summary(cells_txt, k)
     if not f.exists():
         return None
     m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
@@ -70,12 +70,12 @@ def knn_traffic_from_profile(cells_txt: str):
 VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 
 
-def knn_valu_from_profile(cells_txt: str):
+def knn_valu_from_profile(cells_txt: str, k: int = 7):
     """VALU instructions per launch of the KNN kernel on this lattice
     (committed rocprofv3 --pmc SQ_INSTS_VALU pass), or None when absent."""
     import re
 
-    f = sq_summary(cells_txt)
+    f = sq_summary(cells_txt, k)
     if not f.exists():
         return None
     m = re.search(r"^valu_instructions_per_launch\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
@@ -808,11 +808,15 @@ def main() -> int:
             ctx.synchronize()
             el16 = time.perf_counter() - t0
             b16, kn16, si16, _ = gs.stepBoidsTimed(A, knn16, bins, p16, tsteps)
+            c16txt = "x".join(str(c) for c in c16)
             for h in (knn16, A, bins):
                 h.close()
             k16 = {"agent_steps_per_s": n * args.steps / el16, "ms_per_step": el16 * 1e3 / args.steps,
                    "steps": args.steps, "cells": list(c16), "column_radius": rf16,
                    "stage_ms": {"bin_build": b16, "knn": kn16, "steer_integrate": si16},
+                   "knn_traffic": knn_traffic_from_profile(c16txt, 16),
+                   "knn_valu_issue_frac": (knn_valu_from_profile(c16txt, 16) / VALU_ISSUE_PER_S / (kn16 * 1e-3))
+                   if knn_valu_from_profile(c16txt, 16) else None,
                    "autotune_ms_per_step": [{"cells": list(c), "column_radius": r, "ms": m} for c, r, m in table16]}
             print(f"bench: k=16 {el16 * 1e3 / args.steps:.3f} ms/step on {c16} (event-timed stages: bin build "
                   f"{b16:.3f} ms, knn {kn16:.3f} ms, steer+integrate {si16:.3f} ms)", file=sys.stderr, flush=True)
@@ -860,13 +864,13 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": knn_traffic_from_profile(cells_txt),
-            "traffic_profile": str(traffic_summary(cells_txt).relative_to(ROOT)),
+            "traffic": knn_traffic_from_profile(cells_txt, args.k),
+            "traffic_profile": str(traffic_summary(cells_txt, args.k).relative_to(ROOT)),
             # The kernel is bound by VALU issue, not bytes: the profiled VALU
             # instruction count per launch over the chip's issue rate, against
             # the live kernel time.
-            "valu_issue_frac": (knn_valu_from_profile(cells_txt) / VALU_ISSUE_PER_S / (knn_ms * 1e-3))
-            if knn_valu_from_profile(cells_txt) else None,
+            "valu_issue_frac": (knn_valu_from_profile(cells_txt, args.k) / VALU_ISSUE_PER_S / (knn_ms * 1e-3))
+            if knn_valu_from_profile(cells_txt, args.k) else None,
             "bytes_per_launch": n * b_knn,
             "bytes_per_agent": b_knn,
             "achievable_copy_gbs": copy_gbs,
